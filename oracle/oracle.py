@@ -1,0 +1,255 @@
+"""ctypes bindings of the CPU checkers (TEST INFRASTRUCTURE ONLY).
+
+`Oracle` wraps liboracle.so, the project's C restatement of the HAMS-MREL hot
+path (oracle/hm_oracle.c). `Reference` wraps oracle/_ref/libhamsref.so, the
+reference's own Fortran kernels compiled from its sources with a bind(C) shim.
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py use
+this module; the product library never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_d = ctypes.c_double
+_i = ctypes.c_int
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def build():
+    """Compile liboracle.so (and oracle/_ref when the reference tree is present)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "-f", os.path.join(HERE, "Makefile"), "all"], check=True)
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("n", _i), ("ni", _i), ("nbody", _i), ("body_start", _ip),
+                ("xyz", _dp), ("ncn", _ip), ("ncon", _ip),
+                ("xyz_p", _dp), ("ds", _dp), ("pnsz", _dp), ("dxyz_p", _dp),
+                ("ixyz_p", _dp), ("h", _d), ("rho", _d), ("g", _d), ("g_disp", _d),
+                ("xw", _d * 2), ("amp", _d), ("nbeta", _i), ("beta", _dp)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("amas", _dp), ("bdmp", _dp), ("exfc", _dp), ("amat", _dp),
+                ("smat", _dp), ("rhs", _dp), ("mxpot", _dp)]
+
+
+class Oracle:
+    def __init__(self, path=os.path.join(HERE, "liboracle.so")):
+        self.lib = L = ctypes.CDLL(path)
+        L.hmo_error.restype = _d
+        L.hmo_e1xa.restype = _d
+        L.hmo_enxa.restype = _d
+        L.hmo_eix.restype = _d
+        L.hmo_limes.restype = _d
+        L.hmo_flops_get.restype = _d
+        for f in ("hmo_error", "hmo_e1xa", "hmo_eix"):
+            getattr(L, f).argtypes = [_d]
+        L.hmo_enxa.argtypes = [_i, _d]
+
+    # ---- scalar special functions -----------------------------------------
+    def error(self, x):
+        return self.lib.hmo_error(_d(x))
+
+    def e1xa(self, x):
+        return self.lib.hmo_e1xa(_d(x))
+
+    def enxa(self, n, x):
+        return self.lib.hmo_enxa(_i(n), _d(x))
+
+    def eix(self, x):
+        return self.lib.hmo_eix(_d(x))
+
+    def ik01a(self, x):
+        a, b = _d(), _d()
+        self.lib.hmo_ik01a(_d(x), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def jy01b(self, x):
+        o = [_d() for _ in range(4)]
+        self.lib.hmo_jy01b(_d(x), *[ctypes.byref(v) for v in o])
+        return tuple(v.value for v in o)
+
+    def limes(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64).copy()
+        return self.lib.hmo_limes(_i(len(s) - 1), _ptr(s))
+
+    def dispersion(self, w, h, nrt=200, g_disp=9.807):
+        wvn = np.zeros(nrt)
+        self.lib.hmo_dispersion(_ptr(wvn), _i(nrt), _d(w), _d(h), _d(g_disp))
+        return wvn
+
+    # ---- Green functions -----------------------------------------------------
+    def fingreen3d(self, src, fld, v, wvn, h, flag):
+        g = np.zeros(4, dtype=np.complex128)
+        wvn = np.ascontiguousarray(wvn, dtype=np.float64)
+        self.lib.hmo_fingreen3d(_d(src[0]), _d(fld[0]), _d(src[1]), _d(fld[1]), _d(src[2]),
+                                _d(fld[2]), _d(v), _ptr(wvn), _i(len(wvn)), _d(h), _ptr(g), _i(flag))
+        return g
+
+    def infgreen3d(self, src, fld, v, flag):
+        g = np.zeros(4, dtype=np.complex128)
+        self.lib.hmo_infgreen3d(_d(src[0]), _d(fld[0]), _d(src[1]), _d(fld[1]), _d(src[2]),
+                                _d(fld[2]), _d(v), _ptr(g), _i(flag))
+        return g
+
+    def sglintbd(self, verts, xf):
+        """verts: (ncn, 3) panel vertices; xf: field point."""
+        verts = np.asarray(verts, dtype=np.float64)
+        v = np.zeros(12)
+        v[:verts.size] = verts.ravel()
+        xf = np.ascontiguousarray(xf, dtype=np.float64)
+        s = _d()
+        d = np.zeros(3)
+        self.lib.hmo_sglintbd(_i(verts.shape[0]), _ptr(v), _ptr(xf), ctypes.byref(s), _ptr(d))
+        return s.value, d
+
+    def flops_reset(self):
+        self.lib.hmo_flops_reset()
+
+    def flops(self):
+        return self.lib.hmo_flops_get()
+
+    # ---- geometry and one panel-frequency system -----------------------------
+    def panel_geometry(self, xyz, ncn, ncon, rg=(0.0, 0.0, 0.0)):
+        ne = len(ncn)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        ncn = np.ascontiguousarray(ncn, dtype=np.int32)
+        ncon = np.ascontiguousarray(ncon, dtype=np.int32)
+        rg = np.ascontiguousarray(rg, dtype=np.float64)
+        xp = np.zeros((ne, 3))
+        ds = np.zeros(ne)
+        pn = np.zeros(ne)
+        dx = np.zeros((ne, 6))
+        self.lib.hmo_panel_geometry(_i(ne), _ptr(ncn, _ip), _ptr(ncon, _ip), _ptr(xyz), _ptr(rg),
+                                    _ptr(xp), _ptr(ds), _ptr(pn), _ptr(dx))
+        return xp, ds, pn, dx
+
+    def solve_frequency(self, prob, w1, want_matrices=False):
+        """prob: mesh.Problem-like object (see tests/meshes.py). Returns dict."""
+        n, ni, nb = prob.n, prob.ni, prob.nbody
+        m, nbeta = 6 * nb, len(prob.beta)
+        keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                np.ascontiguousarray(prob.ncn, dtype=np.int32),
+                np.ascontiguousarray(prob.ncon, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
+                np.ascontiguousarray(prob.ds, dtype=np.float64),
+                np.ascontiguousarray(prob.pnsz, dtype=np.float64),
+                np.ascontiguousarray(prob.dxyz_p, dtype=np.float64),
+                np.ascontiguousarray(prob.ixyz_p if ni else np.zeros((1, 3)), dtype=np.float64),
+                np.ascontiguousarray(prob.beta, dtype=np.float64)]
+        P = _Problem(n, ni, nb, _ptr(keep[0], _ip), _ptr(keep[1]), _ptr(keep[2], _ip),
+                     _ptr(keep[3], _ip), _ptr(keep[4]), _ptr(keep[5]), _ptr(keep[6]),
+                     _ptr(keep[7]), _ptr(keep[8]), _d(prob.h), _d(prob.rho), _d(prob.g),
+                     _d(prob.g_disp), (_d * 2)(*prob.xw), _d(prob.amp), nbeta, _ptr(keep[9]))
+        amas = np.zeros((m, m), order="F")
+        bdmp = np.zeros((m, m), order="F")
+        exfc = np.zeros((m, nbeta), dtype=np.complex128, order="F")
+        R = _Result(_ptr(amas), _ptr(bdmp), _ptr(exfc), None, None, None, None)
+        out = {}
+        if want_matrices:
+            nt = n + ni
+            out["amat"] = np.zeros((nt, n), dtype=np.complex128, order="F")
+            out["smat"] = np.zeros((nt, n), dtype=np.complex128, order="F")
+            out["rhs"] = np.zeros((nt, m + nbeta), dtype=np.complex128, order="F")
+            out["mxpot"] = np.zeros((n, m + nbeta), dtype=np.complex128, order="F")
+            R.amat, R.smat = _ptr(out["amat"]), _ptr(out["smat"])
+            R.rhs, R.mxpot = _ptr(out["rhs"]), _ptr(out["mxpot"])
+        info = self.lib.hmo_solve_frequency(ctypes.byref(P), _d(w1), ctypes.byref(R))
+        out.update(amas=amas, bdmp=bdmp, exfc=exfc, info=info)
+        return out
+
+    def zgetrf_solve(self, a, b):
+        a = np.asfortranarray(a, dtype=np.complex128).copy()
+        b = np.asfortranarray(b, dtype=np.complex128).copy()
+        n = a.shape[0]
+        ipiv = np.zeros(n, dtype=np.int32)
+        info = self.lib.hmo_zgetrf(_i(n), _ptr(a), _i(n), _ptr(ipiv, _ip))
+        self.lib.hmo_zgetrs(_i(n), _i(b.shape[1]), _ptr(a), _i(n), _ptr(ipiv, _ip), _ptr(b), _i(n))
+        return b, info
+
+
+class Reference:
+    """The reference's Fortran kernels (oracle/_ref/libhamsref.so)."""
+
+    PATH = os.path.join(HERE, "_ref", "libhamsref.so")
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(self.PATH)
+
+    def fingreen3d(self, src, fld, v, wvn, h, flag):
+        g = np.zeros(4, dtype=np.complex128)
+        wvn = np.ascontiguousarray(wvn, dtype=np.float64)
+        self.lib.ref_fingreen3d(_d(src[0]), _d(fld[0]), _d(src[1]), _d(fld[1]), _d(src[2]),
+                                _d(fld[2]), _d(v), _ptr(wvn), _i(len(wvn)), _d(h), _ptr(g), _i(flag))
+        return g
+
+    def infgreen3d(self, src, fld, v, flag):
+        g = np.zeros(4, dtype=np.complex128)
+        self.lib.ref_infgreen3d(_d(src[0]), _d(fld[0]), _d(src[1]), _d(fld[1]), _d(src[2]),
+                                _d(fld[2]), _d(v), _ptr(g), _i(flag))
+        return g
+
+    def dispersion(self, w, h, nrt=200):
+        wvn = np.zeros(nrt)
+        self.lib.ref_dispersion(_ptr(wvn), _i(nrt), _d(w), _d(h))
+        return wvn
+
+    def sglintbd(self, verts, xf):
+        verts = np.asarray(verts, dtype=np.float64)
+        v = np.zeros((4, 3))
+        v[:verts.shape[0]] = verts
+        v = np.ascontiguousarray(v)
+        xf = np.ascontiguousarray(xf, dtype=np.float64)
+        s = _d()
+        d = np.zeros(3)
+        self.lib.ref_sglintbd(_i(verts.shape[0]), _ptr(v), _ptr(xf), ctypes.byref(s), _ptr(d))
+        return s.value, d
+
+    def ik01a(self, x):
+        a, b = _d(), _d()
+        self.lib.ref_ik01a(_d(x), ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def jy01b(self, x):
+        o = [_d() for _ in range(4)]
+        self.lib.ref_jy01b(_d(x), *[ctypes.byref(v) for v in o])
+        return tuple(v.value for v in o)
+
+    def error(self, x):
+        e = _d()
+        self.lib.ref_error(_d(x), ctypes.byref(e))
+        return e.value
+
+    def e1xa(self, x):
+        e = _d()
+        self.lib.ref_e1xa(_d(x), ctypes.byref(e))
+        return e.value
+
+    def enxa(self, n, x):
+        e = _d()
+        self.lib.ref_enxa(_i(n), _d(x), ctypes.byref(e))
+        return e.value
+
+    def eix(self, x):
+        e = _d()
+        self.lib.ref_eix(_d(x), ctypes.byref(e))
+        return e.value
+
+    def limes(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64).copy()
+        r = _d()
+        self.lib.ref_limes(_i(len(s) - 1), _ptr(s), ctypes.byref(r))
+        return r.value
