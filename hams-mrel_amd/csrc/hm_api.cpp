@@ -1,0 +1,738 @@
+// hm_api.cpp -- C-ABI of the boundary-integral hot path (include/hm_bem.h).
+//
+// A context owns one HIP stream and the HBM-resident state of one problem:
+// geometry, the classified pair lists, the per-frequency wave state, the
+// assembled matrices and the factors. The per-frequency chain runs entirely on
+// the device; only the 6nb x 6nb added mass / damping and the excitation forces
+// come back to the host.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "hm_assembly.hpp"
+#include "hm_bem.h"
+#include "hm_device.hpp"
+#include "hm_linalg.hpp"
+
+namespace {
+
+using hm::kNk;
+
+// grow-only device buffer
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b) {
+    if (b <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, b ? b : 16);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+constexpr int kStages = 7;
+enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6 };
+
+}  // namespace
+
+struct hm_ctx {
+  hipStream_t stream = nullptr;
+  std::string err;
+  // constants
+  double rho = 1025.0, g = 9.81, g_disp = 9.81, amp = 1.0, xw = 0.0, yw = 0.0;
+  // geometry
+  int n = 0, ni = 0, nbody = 0;
+  std::vector<int> body_start;
+  DevBuf d_body, d_fp, d_verts, d_ncn, d_ds, d_pnsz, d_dxyz;
+  bool have_geom = false;
+  // depth + classification
+  double h = 0.0;
+  bool have_depth = false;
+  DevBuf d_pairs, d_hist, d_cursor;
+  long long seg_off[4] = {0, 0, 0, 0}, seg_cnt[4] = {0, 0, 0, 0};
+  // headings
+  std::vector<double> beta;
+  DevBuf d_beta;
+  // frequency
+  double w1 = 0.0, v = 0.0, wk = 0.0;
+  double wvn[kNk];
+  DevBuf d_wvn;
+  bool have_freq = false;
+  // system
+  DevBuf d_A, d_S, d_W, d_R, d_C, d_D, d_X, d_A32, d_X32, d_Rw, d_ipiv, d_info, d_counter, d_norm;
+  DevBuf d_amas, d_bdmp, d_exfc;
+  bool assembled = false, factored = false, solved = false;
+  int precision = HM_FP64;
+  // timing
+  bool timing = false;
+  double stage_ms[kStages] = {0};
+  int stage_cnt[kStages] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  int nt() const { return n + ni; }
+  int m() const { return 6 * nbody; }
+  int ncol() const { return 6 * nbody + (int)beta.size(); }
+  hm::GeomDev geom() const {
+    hm::GeomDev gd;
+    gd.n = n; gd.ni = ni; gd.nt = n + ni; gd.nbody = nbody;
+    gd.body_start = d_body.as<int>();
+    gd.fp = d_fp.as<double>();
+    gd.xyz_p = d_fp.as<double>();
+    gd.verts = d_verts.as<double>();
+    gd.ncn = d_ncn.as<int>();
+    gd.ds = d_ds.as<double>();
+    gd.pnsz = d_pnsz.as<double>();
+    gd.dxyz = d_dxyz.as<double>();
+    return gd;
+  }
+  hm::FreqDev freq() const {
+    hm::FreqDev f;
+    f.v = v; f.w1 = w1; f.wk = wk; f.h = h; f.rho = rho; f.g = g; f.amp = amp;
+    f.wvn = d_wvn.as<double>();
+    return f;
+  }
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return HM_OK;
+    (void)hipGetLastError();   // do not let this failure resurface at a later launch check
+    return fail(HM_ERR_HIP - (int)e, "%s: %s", what, hipGetErrorString(e));
+  }
+  void stage_begin() {
+    if (timing) (void)hipEventRecord(ev0, stream);
+  }
+  int stage_end(int s) {
+    if (!timing) return HM_OK;
+    (void)hipEventRecord(ev1, stream);
+    hipError_t e = hipEventSynchronize(ev1);
+    if (e != hipSuccess) return hip(e, "stage timing");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    stage_ms[s] += ms;
+    stage_cnt[s] += 1;
+    return HM_OK;
+  }
+};
+
+#define HIPCHK(ctx, call, what)                              \
+  do {                                                       \
+    int _rc = (ctx)->hip((call), (what));                    \
+    if (_rc != HM_OK) return _rc;                            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// DISPERSION, WavDynSubs.f90:40-127 (Newman 1990 third-order iteration)
+// ---------------------------------------------------------------------------
+extern "C" int hm_dispersion(double* wvn, int nrt, double w, double h, double g_disp) {
+  if (!wvn) return HM_ERR_ARG - 1;
+  if (nrt < 1) return HM_ERR_ARG - 2;
+  if (!(h > 0.0)) return HM_ERR_ARG - 4;
+  const double pi = hm::kPi;
+  const double x = w * w * h / g_disp;
+  double y;
+  if (x > 0.0 && x <= 2.0) y = sqrt(x) * (0.9994 + 0.1701 * x + F32(0.0305) * x * x);
+  else {
+    const double t = x * exp(-2.0 * x);
+    y = x + 2.0 * t - 6.0 * t * t;
+  }
+  if (fabs(y - x) < F32(1.e-10)) {
+    wvn[0] = x / h;
+  } else {
+    double exx = 1.0;
+    while (exx > 1.0e-10) {
+      const double trial = y, dnm = trial * trial - x * x;
+      const double fun = log((trial + x) / (trial - x)) / 2.0 - trial;
+      const double dfun = -x / dnm - 1.0, d2fun = 2.0 * x * trial / (dnm * dnm);
+      y = trial - fun / dfun * (1.0 + (fun / dfun) * (d2fun / dfun) / 2.0);
+      exx = fabs(y - trial);
+    }
+    wvn[0] = y / h;
+  }
+  const double u = 3.0 * x / (7.0 + 3.0 * x);
+  double t = 0.0159 + 0.1032 * u + 4.3152 * u * u - 2.8768 * u * u * u;
+  for (int i = 2; i <= nrt; ++i) {
+    double exx = 1.0;
+    while (exx > 1.0e-10) {
+      const double trial = t, yy = (i - 1) * pi - trial, dnm = yy * yy + x * x;
+      const double fun = atan2(x, yy) - trial;
+      const double dfun = x / dnm - 1.0, d2fun = 2.0 * x * trial / (dnm * dnm);
+      t = trial - fun / dfun * (1.0 + (fun / dfun) * (d2fun / dfun) / 2.0);
+      exx = fabs(t - trial);
+    }
+    wvn[i - 1] = ((i - 1) * pi - t) / h;
+    t = t - pi * x / (x * x + pi * i * (pi * (i - 1) - t));
+  }
+  return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int hm_create(hm_ctx** out) {
+  if (!out) return HM_ERR_ARG - 1;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HM_ERR_HIP - (int)hipErrorNoDevice;
+  hm_ctx* c = new hm_ctx();
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    delete c;
+    return HM_ERR_HIP - (int)e;
+  }
+  *out = c;
+  return HM_OK;
+}
+
+extern "C" int hm_destroy(hm_ctx* c) {
+  if (!c) return HM_ERR_ARG - 1;
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->d_body, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
+                    &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
+                    &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
+                    &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc};
+  for (DevBuf* b : bufs) b->release();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return HM_OK;
+}
+
+extern "C" const char* hm_error_string(const hm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int hm_set_constants(hm_ctx* c, double rho, double g, double g_disp, double amp, double xw, double yw) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!(rho > 0)) return c->fail(HM_ERR_ARG - 2, "rho must be > 0");
+  if (!(g > 0)) return c->fail(HM_ERR_ARG - 3, "g must be > 0");
+  if (!(g_disp > 0)) return c->fail(HM_ERR_ARG - 4, "g_disp must be > 0");
+  c->rho = rho; c->g = g; c->g_disp = g_disp; c->amp = amp; c->xw = xw; c->yw = yw;
+  c->have_freq = false;
+  return HM_OK;
+}
+
+template <typename T>
+static int upload(hm_ctx* c, DevBuf& b, const T* src, size_t count, const char* what) {
+  HIPCHK(c, b.ensure(count * sizeof(T)), what);
+  HIPCHK(c, hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+  return HM_OK;
+}
+
+extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int ntnd, const double* xyz,
+                               const int* ncn, const int* ncon, const double* xyz_p, const double* ds,
+                               const double* pnsz, const double* dxyz_p, int ni, const double* ixyz_p) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nbody < 1) return c->fail(HM_ERR_ARG - 2, "nbody must be >= 1");
+  if (!body_start) return c->fail(HM_ERR_ARG - 3, "body_start is NULL");
+  const int n = body_start[nbody];
+  if (body_start[0] != 0 || n < 1) return c->fail(HM_ERR_ARG - 3, "body_start must start at 0 and end at n >= 1");
+  for (int b = 0; b < nbody; ++b)
+    if (body_start[b + 1] <= body_start[b]) return c->fail(HM_ERR_ARG - 3, "body %d has no panels", b);
+  if (ntnd < 3 || !xyz) return c->fail(HM_ERR_ARG - 5, "nodes missing");
+  if (!ncn || !ncon || !xyz_p || !ds || !pnsz || !dxyz_p) return c->fail(HM_ERR_ARG - 6, "panel arrays missing");
+  if (ni < 0 || (ni > 0 && !ixyz_p)) return c->fail(HM_ERR_ARG - 12, "bad waterplane field points");
+  if (n + ni >= 65536) return c->fail(HM_ERR_ARG - 3, "n + ni = %d exceeds the 16-bit pair encoding", n + ni);
+  std::vector<double> verts((size_t)n * 12, 0.0);
+  for (int j = 0; j < n; ++j) {
+    if (ncn[j] != 3 && ncn[j] != 4) return c->fail(HM_ERR_ARG - 7, "panel %d has %d vertices", j + 1, ncn[j]);
+    for (int v = 0; v < ncn[j]; ++v) {
+      const int node = ncon[4 * j + v];
+      if (node < 1 || node > ntnd) return c->fail(HM_ERR_ARG - 8, "panel %d vertex %d = %d out of range", j + 1, v + 1, node);
+      for (int k = 0; k < 3; ++k) verts[12 * (size_t)j + 3 * v + k] = xyz[3 * (size_t)(node - 1) + k];
+    }
+  }
+  std::vector<double> fp((size_t)(n + ni) * 3);
+  memcpy(fp.data(), xyz_p, sizeof(double) * 3 * (size_t)n);
+  if (ni) memcpy(fp.data() + 3 * (size_t)n, ixyz_p, sizeof(double) * 3 * (size_t)ni);
+  c->n = n; c->ni = ni; c->nbody = nbody;
+  c->body_start.assign(body_start, body_start + nbody + 1);
+  int rc;
+  if ((rc = upload(c, c->d_body, body_start, nbody + 1, "geometry")) ||
+      (rc = upload(c, c->d_fp, fp.data(), fp.size(), "geometry")) ||
+      (rc = upload(c, c->d_verts, verts.data(), verts.size(), "geometry")) ||
+      (rc = upload(c, c->d_ncn, ncn, n, "geometry")) || (rc = upload(c, c->d_ds, ds, n, "geometry")) ||
+      (rc = upload(c, c->d_pnsz, pnsz, n, "geometry")) || (rc = upload(c, c->d_dxyz, dxyz_p, 6 * (size_t)n, "geometry")))
+    return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "geometry upload");
+  c->have_geom = true;
+  c->have_depth = false;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_set_depth(hm_ctx* c, double h) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_depth before hm_set_geometry");
+  if (h == 0.0) return c->fail(HM_ERR_ARG - 2, "depth must be > 0 (finite) or < 0 (infinite)");
+  c->h = h;
+  c->have_depth = true;
+  c->have_freq = false;
+  c->assembled = c->factored = c->solved = false;
+  for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
+  if (h < 0.0) return HM_OK;
+  // classify the pairs by regime and series length (frequency independent)
+  const hm::GeomDev gd = c->geom();
+  const long long np = (long long)c->nt() * c->n;
+  c->stage_begin();
+  HIPCHK(c, c->d_hist.ensure(sizeof(unsigned) * hm::kNumKeys), "classify");
+  HIPCHK(c, c->d_cursor.ensure(sizeof(unsigned long long) * hm::kNumKeys), "classify");
+  HIPCHK(c, c->d_pairs.ensure(sizeof(uint32_t) * (size_t)np), "classify");
+  HIPCHK(c, hipMemsetAsync(c->d_hist.p, 0, sizeof(unsigned) * hm::kNumKeys, c->stream), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true), "classify");
+  std::vector<unsigned> hist(hm::kNumKeys);
+  HIPCHK(c, hipMemcpyAsync(hist.data(), c->d_hist.p, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, c->stream), "classify");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
+  std::vector<unsigned long long> cur(hm::kNumKeys);
+  unsigned long long acc = 0;
+  for (int k = 0; k < hm::kNumKeys; ++k) {
+    cur[k] = acc;
+    acc += hist[k];
+  }
+  if ((long long)acc != np) return c->fail(HM_ERR_STATE, "pair classification lost pairs (%llu of %lld)", acc, np);
+  for (int r = 0; r < 4; ++r) {
+    c->seg_off[r] = (long long)cur[r * hm::kKeyStride];
+    c->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_cursor.p, cur.data(), sizeof(unsigned long long) * hm::kNumKeys, hipMemcpyHostToDevice, c->stream), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false), "classify");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
+  return c->stage_end(kClassify);
+}
+
+extern "C" int hm_set_headings(hm_ctx* c, int nbeta, const double* beta) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nbeta < 0 || (nbeta > 0 && !beta)) return c->fail(HM_ERR_ARG - 2, "bad headings");
+  c->beta.assign(beta, beta + nbeta);
+  if (nbeta) {
+    int rc = upload(c, c->d_beta, beta, nbeta, "headings");
+    if (rc) return rc;
+  } else {
+    HIPCHK(c, c->d_beta.ensure(8), "headings");
+  }
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_set_frequency(hm_ctx* c, double w1) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_depth) return c->fail(HM_ERR_STATE, "hm_set_frequency before hm_set_depth");
+  if (!(w1 > 0.0)) return c->fail(HM_ERR_ARG - 2, "omega must be > 0");
+  c->w1 = w1;
+  c->v = w1 * w1 / c->g;
+  if (c->h > 0.0) {
+    hm_dispersion(c->wvn, kNk, w1, c->h, c->g_disp);
+    c->wk = c->wvn[0];
+  } else {
+    c->wk = c->v;
+    for (int k = 0; k < kNk; ++k) c->wvn[k] = 0.0;
+  }
+  int rc = upload(c, c->d_wvn, c->wvn, kNk, "frequency");
+  if (rc) return rc;
+  c->have_freq = true;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-frequency chain
+// ---------------------------------------------------------------------------
+extern "C" int hm_assemble(hm_ctx* c) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_freq) return c->fail(HM_ERR_STATE, "hm_assemble before hm_set_frequency");
+  const int n = c->n, nt = c->nt(), ncol = c->ncol();
+  const size_t cplx = 2 * sizeof(double);
+  HIPCHK(c, c->d_A.ensure(cplx * (size_t)nt * n), "assemble alloc A");
+  HIPCHK(c, c->d_S.ensure(cplx * (size_t)nt * n), "assemble alloc S");
+  HIPCHK(c, c->d_W.ensure(cplx * (size_t)n * ncol), "assemble alloc W");
+  HIPCHK(c, c->d_R.ensure(cplx * (size_t)nt * ncol), "assemble alloc RHS");
+  const hm::GeomDev gd = c->geom();
+  const hm::FreqDev f = c->freq();
+  double* A = c->d_A.as<double>();
+  double* S = c->d_S.as<double>();
+  c->stage_begin();
+  const bool limit = fabs(c->v) < F32(1.e-8) || fabs(c->v + 1.0) < F32(1.e-8);
+  if (c->h < 0.0 || limit) {
+    HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, nt), "assemble");
+  } else {
+    const uint32_t* pairs = c->d_pairs.as<uint32_t>();
+    for (int r = 0; r < 4; ++r)
+      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt), "assemble");
+  }
+  int rc = c->stage_end(kAssembly);
+  if (rc) return rc;
+  // right-hand sides RHS = S * W
+  c->stage_begin();
+  HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
+  HIPCHK(c, hm::cgemm<double>(c->stream, false, nt, ncol, n, 1.0, 0.0, S, nt, 0, c->d_W.as<double>(), n, 0, 0.0, 0.0,
+                              c->d_R.as<double>(), nt, 0, 1), "rhs gemm");
+  if ((rc = c->stage_end(kRhs))) return rc;
+  if (c->ni > 0) {
+    // least squares through the normal equations with plain transpose
+    c->stage_begin();
+    HIPCHK(c, c->d_C.ensure(cplx * (size_t)n * n), "irr alloc");
+    HIPCHK(c, c->d_D.ensure(cplx * (size_t)n * ncol), "irr alloc");
+    HIPCHK(c, hm::cgemm<double>(c->stream, true, n, n, nt, 1.0, 0.0, A, nt, 0, A, nt, 0, 0.0, 0.0, c->d_C.as<double>(), n, 0, 1), "irr A^T A");
+    HIPCHK(c, hm::cgemm<double>(c->stream, true, n, ncol, nt, 1.0, 0.0, A, nt, 0, c->d_R.as<double>(), nt, 0, 0.0, 0.0, c->d_D.as<double>(), n, 0, 1), "irr A^T B");
+    if ((rc = c->stage_end(kIrr))) return rc;
+  }
+  c->assembled = true;
+  c->factored = c->solved = false;
+  return HM_OK;
+}
+
+// the n x n system matrix and n x ncol right-hand side the solve works on
+static double* sys_matrix(hm_ctx* c) { return c->ni > 0 ? c->d_C.as<double>() : c->d_A.as<double>(); }
+static double* sys_rhs(hm_ctx* c) { return c->ni > 0 ? c->d_D.as<double>() : c->d_R.as<double>(); }
+static int sys_ld_rhs(hm_ctx* c) { return c->ni > 0 ? c->n : c->nt(); }
+
+extern "C" int hm_factor(hm_ctx* c, int precision) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->assembled) return c->fail(HM_ERR_STATE, "hm_factor before hm_assemble");
+  if (precision != HM_FP64 && precision != HM_FP32_IR) return c->fail(HM_ERR_ARG - 2, "unknown precision %d", precision);
+  const int n = c->n;
+  HIPCHK(c, c->d_ipiv.ensure(sizeof(int) * (size_t)n), "factor alloc");
+  HIPCHK(c, c->d_info.ensure(sizeof(int)), "factor alloc");
+  HIPCHK(c, c->d_counter.ensure(sizeof(unsigned)), "factor alloc");
+  HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream), "factor");
+  HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned), c->stream), "factor");
+  c->stage_begin();
+  if (precision == HM_FP64) {
+    HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
+                                    c->d_counter.as<unsigned>(), 1), "lu fp64");
+  } else {
+    HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
+    HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
+                                   c->d_counter.as<unsigned>(), 1), "lu fp32");
+  }
+  int info = 0;
+  HIPCHK(c, hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "factor");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "factor");
+  int rc = c->stage_end(kLu);
+  if (rc) return rc;
+  c->precision = precision;
+  c->factored = true;
+  c->solved = false;
+  if (info > 0) return c->fail(info, "zero pivot in column %d", info);
+  return HM_OK;
+}
+
+// per-column max(|re|,|im|) of an n x ncol block
+template <typename T>
+__global__ void k_colmax_t(const T* X, int ldx, int n, double* out) {
+  __shared__ double s[256];
+  const int c = blockIdx.x;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const long long off = (long long)i + (long long)c * ldx;
+    m = fmax(m, fmax(fabs((double)X[2 * off]), fabs((double)X[2 * off + 1])));
+  }
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (!threadIdx.x) out[c] = s[0];
+}
+#define k_colmax k_colmax_t<double>
+#define k_colmax_f k_colmax_t<float>
+
+constexpr int kMaxRefine = 30;
+constexpr double kRefineTol = 1e-13;
+
+extern "C" int hm_solve(hm_ctx* c, int* iters) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->factored) return c->fail(HM_ERR_STATE, "hm_solve before hm_factor");
+  const int n = c->n, ncol = c->ncol();
+  const size_t cplx = 2 * sizeof(double);
+  HIPCHK(c, c->d_X.ensure(cplx * (size_t)n * ncol), "solve alloc");
+  double* X = c->d_X.as<double>();
+  const double* B = sys_rhs(c);
+  const int ldb = sys_ld_rhs(c);
+  int it = 0;
+  c->stage_begin();
+  HIPCHK(c, hipMemcpy2DAsync(X, cplx * n, B, cplx * ldb, cplx * n, ncol, hipMemcpyDeviceToDevice, c->stream), "solve");
+  if (c->precision == HM_FP64) {
+    HIPCHK(c, hm::lu_solve<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), X, n, 0, ncol, 1), "getrs fp64");
+  } else {
+    // FP32 solves of the FP64 residual; stop when the last correction is below
+    // 1e-13 of the solution in every column
+    HIPCHK(c, c->d_X32.ensure(2 * sizeof(float) * (size_t)n * ncol), "solve alloc");
+    HIPCHK(c, c->d_Rw.ensure(cplx * (size_t)n * ncol), "solve alloc");
+    HIPCHK(c, c->d_norm.ensure(sizeof(double) * 2 * ncol), "solve alloc");
+    float* X32 = c->d_X32.as<float>();
+    double* Rw = c->d_Rw.as<double>();
+    double* nrm = c->d_norm.as<double>();
+    std::vector<double> hn(2 * (size_t)ncol);
+    const double* A = sys_matrix(c);
+    for (it = 0; it < kMaxRefine;) {
+      const double* res = it == 0 ? X : Rw;       // X holds B before the first pass
+      HIPCHK(c, hm::demote(c->stream, res, n, 0, X32, n, 0, n, ncol, 1), "ir demote");
+      HIPCHK(c, hm::lu_solve<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), X32, n, 0, ncol, 1), "getrs fp32");
+      HIPCHK(c, hm::promote_add(c->stream, X32, n, 0, X, n, 0, n, ncol, 1, it == 0), "ir update");
+      ++it;
+      hipLaunchKernelGGL(k_colmax_f, dim3(ncol), dim3(256), 0, c->stream, X32, n, n, nrm + ncol);
+      hipLaunchKernelGGL(k_colmax, dim3(ncol), dim3(256), 0, c->stream, X, n, n, nrm);
+      HIPCHK(c, hipGetLastError(), "ir norm");
+      HIPCHK(c, hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * ncol, hipMemcpyDeviceToHost, c->stream), "ir norm");
+      HIPCHK(c, hipStreamSynchronize(c->stream), "ir norm");
+      double worst = 0.0;
+      for (int k = 0; k < ncol; ++k) worst = fmax(worst, hn[ncol + k] / (hn[k] > 0.0 ? hn[k] : 1.0));
+      if (it > 1 && worst < kRefineTol) break;
+      // residual R = B - A X in FP64
+      HIPCHK(c, hipMemcpy2DAsync(Rw, cplx * n, B, cplx * ldb, cplx * n, ncol, hipMemcpyDeviceToDevice, c->stream), "ir residual");
+      HIPCHK(c, hm::cgemm<double>(c->stream, false, n, ncol, n, -1.0, 0.0, A, n, 0, X, n, 0, 1.0, 0.0, Rw, n, 0, 1), "ir residual");
+    }
+  }
+  int rc = c->stage_end(kSolve);
+  if (rc) return rc;
+  if (iters) *iters = it;
+  c->solved = true;
+  return HM_OK;
+}
+
+extern "C" int hm_forces(hm_ctx* c, double* amas, double* bdmp, double* exfc) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->solved) return c->fail(HM_ERR_STATE, "hm_forces before hm_solve");
+  const int m = c->m(), nbeta = (int)c->beta.size();
+  HIPCHK(c, c->d_amas.ensure(sizeof(double) * m * m), "forces alloc");
+  HIPCHK(c, c->d_bdmp.ensure(sizeof(double) * m * m), "forces alloc");
+  HIPCHK(c, c->d_exfc.ensure(2 * sizeof(double) * m * (nbeta ? nbeta : 1)), "forces alloc");
+  c->stage_begin();
+  HIPCHK(c, hm::forces(c->stream, c->geom(), c->freq(), c->d_beta.as<double>(), nbeta, c->xw, c->yw, c->d_X.as<double>(),
+                       c->n, c->d_amas.as<double>(), c->d_bdmp.as<double>(), c->d_exfc.as<double>()), "forces");
+  if (amas) HIPCHK(c, hipMemcpyAsync(amas, c->d_amas.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream), "forces");
+  if (bdmp) HIPCHK(c, hipMemcpyAsync(bdmp, c->d_bdmp.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream), "forces");
+  if (exfc && nbeta) HIPCHK(c, hipMemcpyAsync(exfc, c->d_exfc.p, 2 * sizeof(double) * m * nbeta, hipMemcpyDeviceToHost, c->stream), "forces");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "forces");
+  return c->stage_end(kForces);
+}
+
+extern "C" int hm_solve_frequency(hm_ctx* c, double w1, int precision, double* amas, double* bdmp, double* exfc) {
+  int rc;
+  if ((rc = hm_set_frequency(c, w1))) return rc;
+  if ((rc = hm_assemble(c))) return rc;
+  if ((rc = hm_factor(c, precision))) return rc;
+  if ((rc = hm_solve(c, nullptr))) return rc;
+  return hm_forces(c, amas, bdmp, exfc);
+}
+
+extern "C" int hm_run_frequencies(hm_ctx* c, int nfreq, const double* w1, int precision, double* amas, double* bdmp,
+                                  double* exfc) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nfreq < 0 || (nfreq && !w1)) return c->fail(HM_ERR_ARG - 2, "bad frequency list");
+  const size_t mm = (size_t)c->m() * c->m(), mb = 2 * (size_t)c->m() * c->beta.size();
+  for (int k = 0; k < nfreq; ++k) {
+    int rc = hm_solve_frequency(c, w1[k], precision, amas ? amas + k * mm : nullptr, bdmp ? bdmp + k * mm : nullptr,
+                                exfc ? exfc + k * mb : nullptr);
+    if (rc) return rc;
+  }
+  return HM_OK;
+}
+
+extern "C" int hm_set_timing(hm_ctx* c, int enable) {
+  if (!c) return HM_ERR_ARG - 1;
+  c->timing = enable != 0;
+  for (int s = 0; s < kStages; ++s) { c->stage_ms[s] = 0.0; c->stage_cnt[s] = 0; }
+  return HM_OK;
+}
+
+extern "C" int hm_get_timing(hm_ctx* c, double* ms, int* counts, int nstage) {
+  if (!c) return HM_ERR_ARG - 1;
+  for (int s = 0; s < nstage && s < kStages; ++s) {
+    if (ms) ms[s] = c->stage_ms[s];
+    if (counts) counts[s] = c->stage_cnt[s];
+  }
+  return HM_OK;
+}
+
+extern "C" int hm_get_system(hm_ctx* c, double* amat, double* smat, double* rhs) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->assembled || c->factored) return c->fail(HM_ERR_STATE, "hm_get_system needs an assembled, unfactored system");
+  const size_t cplx = 2 * sizeof(double), nt = c->nt();
+  if (amat) HIPCHK(c, hipMemcpyAsync(amat, c->d_A.p, cplx * nt * c->n, hipMemcpyDeviceToHost, c->stream), "get system");
+  if (smat) HIPCHK(c, hipMemcpyAsync(smat, c->d_S.p, cplx * nt * c->n, hipMemcpyDeviceToHost, c->stream), "get system");
+  if (rhs) HIPCHK(c, hipMemcpyAsync(rhs, c->d_R.p, cplx * nt * c->ncol(), hipMemcpyDeviceToHost, c->stream), "get system");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "get system");
+  return HM_OK;
+}
+
+extern "C" int hm_get_potentials(hm_ctx* c, double* mxpot) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->solved) return c->fail(HM_ERR_STATE, "hm_get_potentials before hm_solve");
+  if (!mxpot) return c->fail(HM_ERR_ARG - 2, "mxpot is NULL");
+  HIPCHK(c, hipMemcpyAsync(mxpot, c->d_X.p, 2 * sizeof(double) * (size_t)c->n * c->ncol(), hipMemcpyDeviceToHost, c->stream), "get potentials");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "get potentials");
+  return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// element-wise entry points
+// ---------------------------------------------------------------------------
+static int green_common(hm_ctx* c, int n, bool deep, const double* const* in, double v, const double* wvn, int nk,
+                        double h, const int* flag, double* grn) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");
+  if (n == 0) return HM_OK;
+  for (int k = 0; k < 6; ++k)
+    if (!in[k]) return c->fail(HM_ERR_ARG - 3 - k, "coordinate array %d is NULL", k);
+  if (!flag || !grn) return c->fail(HM_ERR_ARG - 1, "flag/grn NULL");
+  if (!deep && (!wvn || nk < kNk)) return c->fail(HM_ERR_ARG - 10, "wvn must hold %d roots", kNk);
+  if (!deep && !(h > 0.0)) return c->fail(HM_ERR_ARG - 12, "h must be > 0");
+  DevBuf tmp;
+  const size_t sz = (size_t)n * sizeof(double);
+  const size_t flag_bytes = (((size_t)n * sizeof(int) + 15) / 16) * 16;
+  HIPCHK(c, tmp.ensure(6 * sz + flag_bytes + 8 * sz + kNk * sizeof(double)), "green batch alloc");
+  char* base = tmp.as<char>();
+  double* d[6];
+  for (int k = 0; k < 6; ++k) {
+    d[k] = reinterpret_cast<double*>(base + k * sz);
+    HIPCHK(c, hipMemcpyAsync(d[k], in[k], sz, hipMemcpyHostToDevice, c->stream), "green batch");
+  }
+  int* df = reinterpret_cast<int*>(base + 6 * sz);
+  double* dout = reinterpret_cast<double*>(base + 6 * sz + flag_bytes);
+  double* dw = dout + 8 * (size_t)n;
+  HIPCHK(c, hipMemcpyAsync(df, flag, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream), "green batch");
+  if (!deep) HIPCHK(c, hipMemcpyAsync(dw, wvn, kNk * sizeof(double), hipMemcpyHostToDevice, c->stream), "green batch");
+  int rc = c->hip(hm::green_batch(c->stream, n, deep, d[0], d[1], d[2], d[3], d[4], d[5], v, dw, h, df, dout), "green batch");
+  if (!rc) rc = c->hip(hipMemcpyAsync(grn, dout, 8 * sz, hipMemcpyDeviceToHost, c->stream), "green batch");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "green batch");
+  (void)hipStreamSynchronize(c->stream);
+  tmp.release();
+  return rc;
+}
+
+extern "C" int hm_fingreen3d_batch(hm_ctx* c, int n, const double* xf, const double* xp, const double* yf,
+                                   const double* yp, const double* zf, const double* zp, double v, const double* wvn,
+                                   int nk, double h, const int* flag, double* grn) {
+  const double* in[6] = {xf, xp, yf, yp, zf, zp};
+  return green_common(c, n, false, in, v, wvn, nk, h, flag, grn);
+}
+
+extern "C" int hm_infgreen3d_batch(hm_ctx* c, int n, const double* xf, const double* xp, const double* yf,
+                                   const double* yp, const double* zf, const double* zp, double v, const int* flag,
+                                   double* grn) {
+  const double* in[6] = {xf, xp, yf, yp, zf, zp};
+  return green_common(c, n, true, in, v, nullptr, 0, -1.0, flag, grn);
+}
+
+extern "C" int hm_sglintbd_batch(hm_ctx* c, int n, const int* ncn, const double* verts, const double* xf, double* sij,
+                                 double* dij) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");
+  if (n == 0) return HM_OK;
+  if (!ncn || !verts || !xf || !sij || !dij) return c->fail(HM_ERR_ARG - 3, "NULL array");
+  for (int i = 0; i < n; ++i)
+    if (ncn[i] != 3 && ncn[i] != 4) return c->fail(HM_ERR_ARG - 3, "panel %d has %d vertices", i, ncn[i]);
+  DevBuf tmp;
+  const size_t n8 = (size_t)n * sizeof(double);
+  HIPCHK(c, tmp.ensure(((n * sizeof(int) + 15) / 16) * 16 + 12 * n8 + 3 * n8 + 4 * n8), "rankine batch alloc");
+  int* dn = tmp.as<int>();
+  double* dv = reinterpret_cast<double*>(tmp.as<char>() + ((n * sizeof(int) + 15) / 16) * 16);
+  double* dx = dv + 12 * (size_t)n;
+  double* dout = dx + 3 * (size_t)n;
+  HIPCHK(c, hipMemcpyAsync(dn, ncn, n * sizeof(int), hipMemcpyHostToDevice, c->stream), "rankine batch");
+  HIPCHK(c, hipMemcpyAsync(dv, verts, 12 * n8, hipMemcpyHostToDevice, c->stream), "rankine batch");
+  HIPCHK(c, hipMemcpyAsync(dx, xf, 3 * n8, hipMemcpyHostToDevice, c->stream), "rankine batch");
+  HIPCHK(c, hm::rankine_batch(c->stream, n, dn, dv, dx, dout), "rankine batch");
+  std::vector<double> out(4 * (size_t)n);
+  HIPCHK(c, hipMemcpyAsync(out.data(), dout, 4 * n8, hipMemcpyDeviceToHost, c->stream), "rankine batch");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "rankine batch");
+  tmp.release();
+  for (int i = 0; i < n; ++i) {
+    sij[i] = out[4 * i];
+    for (int k = 0; k < 3; ++k) dij[3 * i + k] = out[4 * i + 1 + k];
+  }
+  return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LAPACK drop-ins
+// ---------------------------------------------------------------------------
+extern "C" int hm_zgetrf(hm_ctx* c, int m, int n, double* a, int lda, int* ipiv, int* info) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (m < 0) return c->fail(HM_ERR_ARG - 2, "m < 0");
+  if (n != m) return c->fail(HM_ERR_ARG - 3, "only square matrices are supported (m=%d n=%d)", m, n);
+  if (!a && n) return c->fail(HM_ERR_ARG - 4, "a is NULL");
+  if (lda < (m > 1 ? m : 1)) return c->fail(HM_ERR_ARG - 5, "lda < max(1,m)");
+  if (!ipiv && n) return c->fail(HM_ERR_ARG - 6, "ipiv is NULL");
+  if (!info) return c->fail(HM_ERR_ARG - 7, "info is NULL");
+  *info = 0;
+  if (n == 0) return HM_OK;
+  DevBuf da, dp, di, dc;
+  const size_t cplx = 2 * sizeof(double);
+  int rc = c->hip(da.ensure(cplx * (size_t)n * n), "zgetrf alloc");
+  if (!rc) rc = c->hip(dp.ensure(sizeof(int) * n), "zgetrf alloc");
+  if (!rc) rc = c->hip(di.ensure(sizeof(int)), "zgetrf alloc");
+  if (!rc) rc = c->hip(dc.ensure(sizeof(unsigned)), "zgetrf alloc");
+  if (!rc) rc = c->hip(hipMemcpy2DAsync(da.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hipMemsetAsync(di.p, 0, sizeof(int), c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hipMemsetAsync(dc.p, 0, sizeof(unsigned), c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hm::lu_factor<double>(c->stream, da.as<double>(), n, 0, n, dp.as<int>(), di.as<int>(), dc.as<unsigned>(), 1), "zgetrf");
+  if (!rc) rc = c->hip(hipMemcpy2DAsync(a, cplx * lda, da.p, cplx * n, cplx * n, n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hipMemcpyAsync(ipiv, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hipMemcpyAsync(info, di.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "zgetrf");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "zgetrf");
+  (void)hipStreamSynchronize(c->stream);
+  da.release(); dp.release(); di.release(); dc.release();
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) ipiv[i] += 1;
+  return HM_OK;
+}
+
+extern "C" int hm_zgetrs(hm_ctx* c, char trans, int n, int nrhs, const double* a, int lda, const int* ipiv, double* b,
+                         int ldb, int* info) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (trans != 'N' && trans != 'n') return c->fail(HM_ERR_ARG - 2, "only TRANS='N' is supported");
+  if (n < 0) return c->fail(HM_ERR_ARG - 3, "n < 0");
+  if (nrhs < 0) return c->fail(HM_ERR_ARG - 4, "nrhs < 0");
+  if (lda < (n > 1 ? n : 1)) return c->fail(HM_ERR_ARG - 6, "lda < max(1,n)");
+  if (ldb < (n > 1 ? n : 1)) return c->fail(HM_ERR_ARG - 9, "ldb < max(1,n)");
+  if (!info) return c->fail(HM_ERR_ARG - 10, "info is NULL");
+  *info = 0;
+  if (n == 0 || nrhs == 0) return HM_OK;
+  if (!a || !ipiv || !b) return c->fail(HM_ERR_ARG - 5, "NULL array");
+  std::vector<int> p0(n);
+  for (int i = 0; i < n; ++i) {
+    if (ipiv[i] < 1 || ipiv[i] > n) return c->fail(HM_ERR_ARG - 7, "ipiv[%d] = %d out of range", i, ipiv[i]);
+    p0[i] = ipiv[i] - 1;
+  }
+  DevBuf da, dp, db;
+  const size_t cplx = 2 * sizeof(double);
+  int rc = c->hip(da.ensure(cplx * (size_t)n * n), "zgetrs alloc");
+  if (!rc) rc = c->hip(dp.ensure(sizeof(int) * n), "zgetrs alloc");
+  if (!rc) rc = c->hip(db.ensure(cplx * (size_t)n * nrhs), "zgetrs alloc");
+  if (!rc) rc = c->hip(hipMemcpy2DAsync(da.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, c->stream), "zgetrs");
+  if (!rc) rc = c->hip(hipMemcpyAsync(dp.p, p0.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream), "zgetrs");
+  if (!rc) rc = c->hip(hipMemcpy2DAsync(db.p, cplx * n, b, cplx * ldb, cplx * n, nrhs, hipMemcpyHostToDevice, c->stream), "zgetrs");
+  if (!rc) rc = c->hip(hm::lu_solve<double>(c->stream, da.as<double>(), n, 0, n, dp.as<int>(), db.as<double>(), n, 0, nrhs, 1), "zgetrs");
+  if (!rc) rc = c->hip(hipMemcpy2DAsync(b, cplx * ldb, db.p, cplx * n, cplx * n, nrhs, hipMemcpyDeviceToHost, c->stream), "zgetrs");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "zgetrs");
+  (void)hipStreamSynchronize(c->stream);
+  da.release(); dp.release(); db.release();
+  return rc;
+}

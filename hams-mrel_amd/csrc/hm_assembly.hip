@@ -1,0 +1,337 @@
+// hm_assembly.hip -- influence-matrix assembly and force reductions on gfx950.
+//
+// Replaces the per-frequency table fill CALGREEN_MULTI (CalGreenFuncMulti.f90:112)
+// followed by the element integrations BODINT/RBC/DBC (BodyIntgrMulti.f90:48-173)
+// and the fills of ASSB_LEFT/RBC/DBC (AssbMatxMulti.f90:53-251; IRR rows:
+// AssbMatx_irr.f90:53-466). The 4-plane RKBN/CGRN tables are never stored: one
+// thread evaluates the Rankine panel integral (near pairs only) and the wave
+// Green function of its (field point i, source panel j) pair and writes
+//   A[i,j] = (i==j ? 2*pi : 0) + sum_k (dS_k + dG_k*ds_j) n_jk       (flag = 1)
+//                                 or (sum_k dG_k n_jk) * ds_j          (flag = 0)
+//   S[i,j] = S_ij + G*ds_j  (flag = 1)   or   G*ds_j  (flag = 0)
+// straight into the column-major system matrix A and single-layer matrix S.
+// The radiation/diffraction right-hand sides are then S*W (a GEMM) with W the
+// per-panel normal components and incident normal velocities.
+//
+// Finite depth: pairs are classified once per geometry by regime and series
+// length (both depend on R/h only) and each regime runs its own kernel over a
+// contiguous pair list sorted by series length, so that lanes of a wave follow
+// the same branch and trip count.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hm_assembly.hpp"
+#include "hm_device.hpp"
+
+namespace hm {
+
+__device__ __forceinline__ void load_wvn(double* s, const double* g, int nk) {
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) s[k] = g[k];
+  __syncthreads();
+}
+
+// one pair: field point i (hull centroid, or waterplane centroid when i >= n),
+// source panel j; REG = regime (or -1 for infinite depth, -2 for generic dispatch)
+template <int REG>
+__device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
+                                              double* A, double* S, long long ld) {
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Rk rk{0.0, 0.0, 0.0, 0.0};
+  if (flag) rankine(g.ncn[j], g.verts + 12 * (long long)j, xp, rk);
+  Grn4 gr;
+  if (REG == -1) {
+    infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
+  } else if (REG == -2) {
+    fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, wvn, f.h, flag, gr);
+  } else {
+    const double xpq = xq[0] - xp[0], ypq = xq[1] - xp[1];
+    const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+    FinOut fo{0, 0, 0, 0, 0, 0};
+    if (REG == kLinton) fin_linton(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
+    else if (REG == kPidcock) fin_pidcock(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
+    else if (REG == kEigene) fin_eigene(rr, xq[2], xp[2], wvn, f.h, fo);
+    else fin_eigen(rr, xq[2], xp[2], wvn, f.h, fo);
+    if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+    fin_to_grn(fo, xpq, ypq, xq[2], xp[2], rr, true, flag, gr);
+  }
+  const double ear = g.ds[j];
+  const double* nv = g.dxyz + 6 * (long long)j;
+  double ar, ai, sr, si;
+  if (flag) {
+    ar = (rk.dx + gr.re[1] * ear) * nv[0] + (rk.dy + gr.re[2] * ear) * nv[1] + (rk.dz + gr.re[3] * ear) * nv[2];
+    ai = (gr.im[1] * ear) * nv[0] + (gr.im[2] * ear) * nv[1] + (gr.im[3] * ear) * nv[2];
+    sr = rk.s + gr.re[0] * ear;
+    si = gr.im[0] * ear;
+  } else {
+    ar = (gr.re[1] * nv[0] + gr.re[2] * nv[1] + gr.re[3] * nv[2]) * ear;
+    ai = (gr.im[1] * nv[0] + gr.im[2] * nv[1] + gr.im[3] * nv[2]) * ear;
+    sr = gr.re[0] * ear;
+    si = gr.im[0] * ear;
+  }
+  if (i == j) ar = 2.0 * kPi + ar;
+  const long long off = (long long)i + (long long)j * ld;
+  A[2 * off] = ar;
+  A[2 * off + 1] = ai;
+  S[2 * off] = sr;
+  S[2 * off + 1] = si;
+}
+
+// 2-D grid over all pairs (infinite depth, or finite depth with dispatch per pair)
+template <int REG>
+__global__ __launch_bounds__(256) void k_assemble_grid(GeomDev g, FreqDev f, double* A, double* S, long long ld) {
+  __shared__ double wvn[kNk];
+  if (REG != -1) load_wvn(wvn, f.wvn, kNk);
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i >= g.nt || j >= g.n) return;
+  assemble_pair<REG>(g, f, wvn, i, j, A, S, ld);
+}
+
+// one regime over its classified pair list (i in the high 16 bits, j in the low)
+template <int REG>
+__global__ __launch_bounds__(256) void k_assemble_list(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
+                                                       long long np, double* A, double* S, long long ld) {
+  __shared__ double wvn[kNk];
+  load_wvn(wvn, f.wvn, kNk);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const uint32_t p = pairs[t];
+  assemble_pair<REG>(g, f, wvn, (int)(p >> 16), (int)(p & 0xffffu), A, S, ld);
+}
+
+// ---- classification --------------------------------------------------------
+__device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j) {
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const int reg = fin_regime(rr, h);
+  int len = 0;
+  if (reg == kEigene) len = fin_eigene_terms(rr, h);
+  else if (reg == kPidcock) len = fin_pidcock_terms(rr, h);
+  return reg * kKeyStride + len;
+}
+
+__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist) {
+  __shared__ unsigned sh[kNumKeys];
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
+  __syncthreads();
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i < g.nt && j < g.n) atomicAdd(&sh[pair_key(g, h, i, j)], 1u);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
+    if (sh[k]) atomicAdd(&hist[k], sh[k]);
+}
+
+__global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, unsigned long long* cursor,
+                                                          uint32_t* pairs) {
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i >= g.nt || j >= g.n) return;
+  const unsigned long long pos = atomicAdd(&cursor[pair_key(g, h, i, j)], 1ull);
+  pairs[pos] = ((uint32_t)i << 16) | (uint32_t)j;
+}
+
+hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist,
+                          unsigned long long* cursor, uint32_t* pairs, bool count_only) {
+  dim3 grid((g.n + 15) / 16, (g.nt + 15) / 16);
+  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist);
+  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs);
+  return hipGetLastError();
+}
+
+hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
+                         long long ld) {
+  dim3 grid((g.n + 15) / 16, (g.nt + 15) / 16);
+  if (deep) hipLaunchKernelGGL((k_assemble_grid<-1>), grid, dim3(256), 0, s, g, f, A, S, ld);
+  else hipLaunchKernelGGL((k_assemble_grid<-2>), grid, dim3(256), 0, s, g, f, A, S, ld);
+  return hipGetLastError();
+}
+
+hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
+                           long long np, double* A, double* S, long long ld) {
+  if (np <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((np + 255) / 256);
+  switch (regime) {
+    case kLinton: hipLaunchKernelGGL((k_assemble_list<kLinton>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    case kPidcock: hipLaunchKernelGGL((k_assemble_list<kPidcock>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    case kEigene: hipLaunchKernelGGL((k_assemble_list<kEigene>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    default: hipLaunchKernelGGL((k_assemble_list<kEigen>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+  }
+  return hipGetLastError();
+}
+
+// ---- incident wave (PatclVelct.f90) -----------------------------------------
+struct Incident { double w0, wk0; };
+__device__ __forceinline__ Incident incident_params(const FreqDev& f) {
+  if (fabs(f.v) < F32(1.e-8)) return {F32(1.e-20), F32(1.e-20)};
+  if (fabs(f.v + 1.0) < F32(1.e-8)) return {F32(1.e+20), F32(1.e+20)};
+  return {f.w1, f.wk};
+}
+
+// DINP . n at source panel j for heading beta (DINP :83-123); VINP-style deep
+// test (H < 0) as in the reference
+__device__ void dinp_dot_n(const FreqDev& f, const double* x, const double* n, double beta, double xw, double yw,
+                           double& qr, double& qi) {
+  const Incident p = incident_params(f);
+  double sb, cb;
+  sincos(beta, &sb, &cb);
+  const double wkx = p.wk0 * ((x[0] - xw) * cb + (x[1] - yw) * sb);
+  double se, ce;
+  sincos(wkx, &se, &ce);
+  double hx, vz;   // horizontal and vertical depth factors
+  if (f.h < 0.0) { hx = exp(p.wk0 * x[2]); vz = hx; }
+  else {
+    hx = cosh(p.wk0 * (x[2] + f.h)) / sinh(p.wk0 * f.h);
+    vz = sinh(p.wk0 * (x[2] + f.h)) / sinh(p.wk0 * f.h);
+  }
+  const double a = p.w0 * f.amp;
+  // dx = a*hx*e*cos/i, dy = a*hx*e*sin/i, dz = -i*a*vz*e/i = -a*vz*e; e = ce + i se; (u)/i = -i u
+  const double dxr = a * hx * cb * se, dxi = -a * hx * cb * ce;
+  const double dyr = a * hx * sb * se, dyi = -a * hx * sb * ce;
+  const double dzr = -a * vz * ce, dzi = -a * vz * se;
+  qr = dxr * n[0] + dyr * n[1] + dzr * n[2];
+  qi = dxi * n[0] + dyi * n[1] + dzi * n[2];
+}
+
+// VINP at field panel i (VINP :42-76; deep formula when H < 0 or H > 500)
+__device__ void vinp(const FreqDev& f, const double* x, double beta, double xw, double yw, double& vr, double& vi) {
+  const Incident p = incident_params(f);
+  double sb, cb;
+  sincos(beta, &sb, &cb);
+  const double wkx = p.wk0 * ((x[0] - xw) * cb + (x[1] - yw) * sb);
+  double se, ce;
+  sincos(wkx, &se, &ce);
+  double amp;   // VINP = (i * c * e)/i for finite depth, (-i * c * e)/i for deep water
+  if (f.h < 0.0 || f.h > 500.0) {
+    amp = -f.amp * f.g / p.w0 * exp(p.wk0 * x[2]);
+  } else {
+    const double dum = -f.amp * p.w0 / (p.wk0 * sinh(p.wk0 * f.h));
+    amp = dum * cosh(p.wk0 * (x[2] + f.h));
+    // finite depth: VINP = CI*DUM*cosh*e / CI = DUM*cosh*e
+    vr = amp * ce;
+    vi = amp * se;
+    return;
+  }
+  // deep: VINP = -CI*c*e / CI = -c*e with c = AMP*G/W0*exp(kz)
+  vr = amp * ce;
+  vi = amp * se;
+}
+
+// W[j, 6b+d] = n_jd for j in body b; W[j, m+h] = -(DINP.n)_j,h
+__global__ void k_build_w(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw, double yw, double* W,
+                          int ldw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.n) return;
+  const int m = 6 * g.nbody;
+  int b = 0;
+  while (b + 1 < g.nbody && j >= g.body_start[b + 1]) ++b;
+  for (int c = 0; c < m; ++c) {
+    const long long off = (long long)j + (long long)c * ldw;
+    W[2 * off] = (c / 6 == b) ? g.dxyz[6 * (long long)j + c % 6] : 0.0;
+    W[2 * off + 1] = 0.0;
+  }
+  for (int h = 0; h < nbeta; ++h) {
+    double qr, qi;
+    dinp_dot_n(f, g.xyz_p + 3 * (long long)j, g.dxyz + 6 * (long long)j, beta[h], xw, yw, qr, qi);
+    const long long off = (long long)j + (long long)(m + h) * ldw;
+    W[2 * off] = -qr;
+    W[2 * off + 1] = -qi;
+  }
+}
+
+hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
+                   double yw, double* W, int ldw) {
+  hipLaunchKernelGGL(k_build_w, dim3((g.n + 255) / 256), dim3(256), 0, s, g, f, beta, nbeta, xw, yw, W, ldw);
+  return hipGetLastError();
+}
+
+// Force reductions (RFORCE_MULTI/EFORCE_MULTI, PotentWavForceMulti.f90:45-226):
+// one workgroup per (force row 6*b+d, potential column c).
+__global__ __launch_bounds__(256) void k_forces(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw,
+                                                double yw, const double* X, int ldx, double* amas, double* bdmp,
+                                                double* exfc) {
+  const int row = blockIdx.x, c = blockIdx.y, b = row / 6, d = row % 6, m = 6 * g.nbody;
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  for (int i = g.body_start[b] + threadIdx.x; i < g.body_start[b + 1]; i += blockDim.x) {
+    const long long off = (long long)i + (long long)c * ldx;
+    double pr = X[2 * off], pi = X[2 * off + 1];
+    if (c >= m) {
+      double vr, vi;
+      vinp(f, g.xyz_p + 3 * (long long)i, beta[c - m], xw, yw, vr, vi);
+      pr += vr;
+      pi += vi;
+    }
+    const double w = g.ds[i] * g.dxyz[6 * (long long)i + d];
+    ar += pr * w;
+    ai += pi * w;
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) { sr[threadIdx.x] += sr[threadIdx.x + st]; si[threadIdx.x] += si[threadIdx.x + st]; }
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  if (c < m) {
+    amas[row + (long long)c * m] = f.rho * sr[0];            // Im(i rho z) = rho Re z
+    bdmp[row + (long long)c * m] = f.rho * f.w1 * si[0];     // -Re(i rho w z) = rho w Im z
+  } else {
+    const long long off = row + (long long)(c - m) * m;      // i w rho z
+    exfc[2 * off] = -f.w1 * f.rho * si[0];
+    exfc[2 * off + 1] = f.w1 * f.rho * sr[0];
+  }
+}
+
+hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
+                  double yw, const double* X, int ldx, double* amas, double* bdmp, double* exfc) {
+  const int m = 6 * g.nbody;
+  hipLaunchKernelGGL(k_forces, dim3(m, m + nbeta), dim3(256), 0, s, g, f, beta, nbeta, xw, yw, X, ldx, amas, bdmp, exfc);
+  return hipGetLastError();
+}
+
+// ---- element-wise parity kernels ---------------------------------------------
+__global__ void k_green_batch(int n, int deep, const double* xf, const double* xp, const double* yf, const double* yp,
+                              const double* zf, const double* zp, double v, const double* wvn_g, double h,
+                              const int* flag, double* out) {
+  __shared__ double wvn[kNk];
+  if (!deep) load_wvn(wvn, wvn_g, kNk);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Grn4 g;
+  if (deep) infgreen3d(xf[t], xp[t], yf[t], yp[t], zf[t], zp[t], v, flag[t], g);
+  else fingreen3d(xf[t], xp[t], yf[t], yp[t], zf[t], zp[t], v, wvn, h, flag[t], g);
+  for (int k = 0; k < 4; ++k) {
+    out[8 * (long long)t + 2 * k] = g.re[k];
+    out[8 * (long long)t + 2 * k + 1] = g.im[k];
+  }
+}
+
+__global__ void k_rankine_batch(int n, const int* ncn, const double* verts, const double* xf, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Rk r;
+  rankine(ncn[t], verts + 12 * (long long)t, xf + 3 * (long long)t, r);
+  out[4 * (long long)t] = r.s;
+  out[4 * (long long)t + 1] = r.dx;
+  out[4 * (long long)t + 2] = r.dy;
+  out[4 * (long long)t + 3] = r.dz;
+}
+
+hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const double* xp, const double* yf,
+                       const double* yp, const double* zf, const double* zp, double v, const double* wvn, double h,
+                       const int* flag, double* out) {
+  hipLaunchKernelGGL(k_green_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, deep ? 1 : 0, xf, xp, yf, yp, zf, zp, v,
+                     wvn, h, flag, out);
+  return hipGetLastError();
+}
+hipError_t rankine_batch(hipStream_t s, int n, const int* ncn, const double* verts, const double* xf, double* out) {
+  hipLaunchKernelGGL(k_rankine_batch, dim3((n + 255) / 256), dim3(256), 0, s, n, ncn, verts, xf, out);
+  return hipGetLastError();
+}
+
+}  // namespace hm

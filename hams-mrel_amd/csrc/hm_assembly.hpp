@@ -1,0 +1,46 @@
+// hm_assembly.hpp -- device-side problem description and assembly launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hm {
+
+constexpr int kNk = 200;          // dispersion roots per frequency (WavDynMods.f90:59)
+constexpr int kKeyStride = 128;   // classification key = regime * 128 + series length
+constexpr int kNumKeys = 4 * kKeyStride;
+
+// Panel geometry resident in HBM (all bodies combined, panel-major).
+struct GeomDev {
+  int n, ni, nt, nbody;
+  const int* body_start;   // nbody+1 panel offsets
+  const double* fp;        // nt*3 field points: hull centroids, then waterplane centroids
+  const double* xyz_p;     // n*3 source centroids (= fp)
+  const double* verts;     // n*12 panel vertices
+  const int* ncn;          // n vertex counts
+  const double* ds;        // n areas
+  const double* pnsz;      // n longest edges
+  const double* dxyz;      // n*6 generalised normals
+};
+
+// Per-frequency wave state.
+struct FreqDev {
+  double v, w1, wk, h, rho, g, amp;
+  const double* wvn;       // kNk roots on the device
+};
+
+hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist, unsigned long long* cursor,
+                          uint32_t* pairs, bool count_only);
+hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
+                         long long ld);
+hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
+                           long long np, double* A, double* S, long long ld);
+hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
+                   double yw, double* W, int ldw);
+hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
+                  double yw, const double* X, int ldx, double* amas, double* bdmp, double* exfc);
+hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const double* xp, const double* yf,
+                       const double* yp, const double* zf, const double* zp, double v, const double* wvn, double h,
+                       const int* flag, double* out);
+hipError_t rankine_batch(hipStream_t s, int n, const int* ncn, const double* verts, const double* xf, double* out);
+
+}  // namespace hm

@@ -1,0 +1,465 @@
+// hm_linalg.hip -- dense complex linear algebra of the solve stage on gfx950:
+// the work of LAPACK ZGETRF/ZGETRS at the reference's call sites
+// (AssbMatxMulti.f90:112,:279,:346; AssbMatx_irr.f90:175,:491) plus the complex
+// GEMMs of the irregular-frequency normal equations (AssbMatx_irr.f90:153-169,
+// :300-318) and of the right-hand-side assembly.
+//
+// Storage: column-major complex matrices, interleaved (re, im), leading
+// dimension ld; a batch of matrices is addressed by a per-matrix element stride.
+// Everything is templated on the real type T (double for the plain path, float
+// for the mixed-precision factorisation that FP64 iterative refinement repairs).
+//
+// * cgemm: C = alpha*op(A)*B + beta*C with op = identity or plain transpose.
+//   64x64 output tile per 256-thread workgroup (2x2 waves of 32x32), K staged
+//   through LDS in chunks of 16, inner products on MFMA 16x16x4 (f64 or f32);
+//   a complex product is four real MFMAs into separate Re/Im accumulators.
+// * LU with partial pivoting, right-looking, panel width kNb: per panel column
+//   one launch scales the column, applies the rank-1 update to the rest of the
+//   panel and -- in the last workgroup to finish -- searches the next column's
+//   pivot (first max of |re|+|im|, as izamax) and swaps it into place. Then the
+//   panel's row swaps are applied to the other columns, the block row is solved
+//   with the unit-lower panel block, and the trailing matrix gets one GEMM.
+// * Triangular solves with many right-hand sides: per 64-row block a
+//   diagonal-block solve (64 lanes per right-hand side) and a GEMM update.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hm_linalg.hpp"
+
+namespace hm {
+
+template <typename T> struct vec4;
+template <> struct vec4<double> { typedef double __attribute__((ext_vector_type(4))) type; };
+template <> struct vec4<float> { typedef float __attribute__((ext_vector_type(4))) type; };
+
+// Row of accumulator element v held by `lane` in a 16x16 MFMA result (column is
+// lane % 16). Measured on gfx950: the f32 16x16x4 result holds 4 consecutive rows
+// per lane group, the f64 16x16x4 result interleaves them with stride 4.
+template <typename T> __device__ __forceinline__ int mfma_row(int lane, int v);
+template <> __device__ __forceinline__ int mfma_row<float>(int lane, int v) { return 4 * (lane >> 4) + v; }
+template <> __device__ __forceinline__ int mfma_row<double>(int lane, int v) { return (lane >> 4) + 4 * v; }
+
+__device__ __forceinline__ vec4<double>::type mfma16(double a, double b, vec4<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ vec4<float>::type mfma16(float a, float b, vec4<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int kBM = 64, kBN = 64, kBK = 16;
+
+// C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C; blockIdx.z = batch index.
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, const T* __restrict__ A,
+                                               int lda, long long sA, const T* __restrict__ B, int ldb,
+                                               long long sB, T br, T bi, T* __restrict__ C, int ldc,
+                                               long long sC) {
+  __shared__ T as_re[kBK][kBM + 1], as_im[kBK][kBM + 1], bs_re[kBK][kBN + 1], bs_im[kBK][kBN + 1];
+  typedef typename vec4<T>::type v4;
+  A += 2 * sA * blockIdx.z;
+  B += 2 * sB * blockIdx.z;
+  C += 2 * sC * blockIdx.z;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  v4 cre[2][2], cim[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int v = 0; v < 4; ++v) { cre[i][j][v] = 0; cim[i][j][v] = 0; }
+
+  for (int k0 = 0; k0 < k; k0 += kBK) {
+    // stage op(A)[row0:+64, k0:+16] and B[k0:+16, col0:+64]
+    for (int q = 0; q < 4; ++q) {
+      int r, kk;
+      if (!TA) { r = tid & 63; kk = (tid >> 6) + 4 * q; }
+      else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      const int gr = row0 + r, gk = k0 + kk;
+      T xr = 0, xi = 0;
+      if (gr < m && gk < k) {
+        const long long off = TA ? (long long)gk + (long long)gr * lda : (long long)gr + (long long)gk * lda;
+        xr = A[2 * off];
+        xi = A[2 * off + 1];
+      }
+      as_re[kk][r] = xr;
+      as_im[kk][r] = xi;
+      const int bk = tid & 15, bc = (tid >> 4) + 16 * q;
+      const int gbk = k0 + bk, gbc = col0 + bc;
+      T yr = 0, yi = 0;
+      if (gbk < k && gbc < n) {
+        const long long off = (long long)gbk + (long long)gbc * ldb;
+        yr = B[2 * off];
+        yi = B[2 * off + 1];
+      }
+      bs_re[bk][bc] = yr;
+      bs_im[bk][bc] = yi;
+    }
+    __syncthreads();
+    for (int k4 = 0; k4 < kBK; k4 += 4) {
+      const int kr = k4 + (lane >> 4), li = lane & 15;
+      T a_re[2], a_im[2], b_re[2], b_im[2];
+      for (int s = 0; s < 2; ++s) {
+        a_re[s] = as_re[kr][wr + 16 * s + li];
+        a_im[s] = as_im[kr][wr + 16 * s + li];
+        b_re[s] = bs_re[kr][wc + 16 * s + li];
+        b_im[s] = bs_im[kr][wc + 16 * s + li];
+      }
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+          cre[i][j] = mfma16(a_re[i], b_re[j], cre[i][j]);
+          cre[i][j] = mfma16(-a_im[i], b_im[j], cre[i][j]);
+          cim[i][j] = mfma16(a_re[i], b_im[j], cim[i][j]);
+          cim[i][j] = mfma16(a_im[i], b_re[j], cim[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int v = 0; v < 4; ++v) {
+        const int gr = row0 + wr + 16 * i + mfma_row<T>(lane, v);
+        const int gc = col0 + wc + 16 * j + (lane & 15);
+        if (gr < m && gc < n) {
+          const long long off = (long long)gr + (long long)gc * ldc;
+          const T pr = cre[i][j][v], pi = cim[i][j][v];
+          T outr = ar * pr - ai * pi, outi = ar * pi + ai * pr;
+          if (br != T(0) || bi != T(0)) {
+            const T c_r = C[2 * off], c_i = C[2 * off + 1];
+            outr += br * c_r - bi * c_i;
+            outi += br * c_i + bi * c_r;
+          }
+          C[2 * off] = outr;
+          C[2 * off + 1] = outi;
+        }
+      }
+}
+
+template <typename T>
+hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda,
+                 long long sA, const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc,
+                 long long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return hipSuccess;
+  dim3 grid((m + kBM - 1) / kBM, (n + kBN - 1) / kBN, batch);
+  if (transA)
+    hipLaunchKernelGGL((k_cgemm<T, true>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc, sC);
+  else
+    hipLaunchKernelGGL((k_cgemm<T, false>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc, sC);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// LU
+// ---------------------------------------------------------------------------
+
+// block-wide first-max search of |re|+|im| over rows [r0, n) of column j;
+// returns the winning row in *prow (shared). Tie -> smallest row.
+template <typename T>
+__device__ void block_pivot_search(const T* a, int lda, int n, int r0, int j, int* prow, T* pval) {
+  __shared__ T smax[256];
+  __shared__ int sidx[256];
+  T best = T(-1);
+  int bi = n;
+  for (int r = r0 + threadIdx.x; r < n; r += blockDim.x) {
+    const long long off = (long long)r + (long long)j * lda;
+    const T v = fabs(a[2 * off]) + fabs(a[2 * off + 1]);
+    if (v > best) { best = v; bi = r; }
+  }
+  smax[threadIdx.x] = best;
+  sidx[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const T o = smax[threadIdx.x + s];
+      const int oi = sidx[threadIdx.x + s];
+      if (o > smax[threadIdx.x] || (o == smax[threadIdx.x] && oi < sidx[threadIdx.x])) {
+        smax[threadIdx.x] = o;
+        sidx[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { *prow = sidx[0]; *pval = smax[0]; }
+  __syncthreads();
+}
+
+// swap rows j and p over columns [c0, c1), record the pivot, flag a zero pivot
+template <typename T>
+__device__ void block_apply_pivot(T* a, int lda, int j, int p, T pmax, int c0, int c1, int* ipiv, int* info) {
+  if (p != j)
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+      const long long oj = (long long)j + (long long)c * lda, op = (long long)p + (long long)c * lda;
+      const T xr = a[2 * oj], xi = a[2 * oj + 1];
+      a[2 * oj] = a[2 * op];
+      a[2 * oj + 1] = a[2 * op + 1];
+      a[2 * op] = xr;
+      a[2 * op + 1] = xi;
+    }
+  if (threadIdx.x == 0) {
+    ipiv[j] = p;
+    if (!(pmax > T(0)) && *info == 0) *info = j + 1;
+  }
+}
+
+// pivot of the first column of a panel: one workgroup per matrix
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_pivot(T* A, int lda, long long sA, int n, int j, int c0, int c1,
+                                                  int* ipiv, int* info) {
+  T* a = A + 2 * sA * blockIdx.z;
+  __shared__ int prow;
+  __shared__ T pval;
+  block_pivot_search(a, lda, n, j, j, &prow, &pval);
+  block_apply_pivot(a, lda, j, prow, pval, c0, c1, ipiv + (long long)n * blockIdx.z, info + blockIdx.z);
+}
+
+// one panel column j (pivot already in place): scale L(:,j), rank-1 update of
+// panel columns (j, c1); the last workgroup to finish pivots column j+1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int n, int j, int c0, int c1,
+                                                int* ipiv, int* info, unsigned* counter) {
+  T* a = A + 2 * sA * blockIdx.z;
+  __shared__ T u_re[kLuNb], u_im[kLuNb];
+  __shared__ int last;
+  const int ncu = c1 - j - 1;
+  for (int c = threadIdx.x; c < ncu; c += blockDim.x) {
+    const long long off = (long long)j + (long long)(j + 1 + c) * lda;
+    u_re[c] = a[2 * off];
+    u_im[c] = a[2 * off + 1];
+  }
+  const long long od = (long long)j + (long long)j * lda;
+  const T dr = a[2 * od], di = a[2 * od + 1];
+  const T den = dr * dr + di * di;
+  T rr = 0, ri = 0;
+  if (den > T(0)) { rr = dr / den; ri = -di / den; }
+  __syncthreads();
+  const int r = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) {
+    const long long ol = (long long)r + (long long)j * lda;
+    const T xr = a[2 * ol], xi = a[2 * ol + 1];
+    const T lr = xr * rr - xi * ri, li = xr * ri + xi * rr;
+    a[2 * ol] = lr;
+    a[2 * ol + 1] = li;
+    for (int c = 0; c < ncu; ++c) {
+      const long long oc = (long long)r + (long long)(j + 1 + c) * lda;
+      a[2 * oc] -= lr * u_re[c] - li * u_im[c];
+      a[2 * oc + 1] -= lr * u_im[c] + li * u_re[c];
+    }
+  }
+  if (j + 1 >= c1) return;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(counter + blockIdx.z, 1u) == gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  __shared__ int prow;
+  __shared__ T pval;
+  block_pivot_search(a, lda, n, j + 1, j + 1, &prow, &pval);
+  block_apply_pivot(a, lda, j + 1, prow, pval, c0, c1, ipiv + (long long)n * blockIdx.z, info + blockIdx.z);
+  if (threadIdx.x == 0) counter[blockIdx.z] = 0u;
+}
+
+// apply the panel's swaps (rows [k0,k1)) to columns [0,k0) and [k1,n)
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_laswp(T* A, int lda, long long sA, int n, int k0, int k1,
+                                                  const int* ipiv) {
+  T* a = A + 2 * sA * blockIdx.z;
+  const int* pv = ipiv + (long long)n * blockIdx.z;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n - (k1 - k0)) return;
+  if (c >= k0) c += k1 - k0;
+  T* col = a + 2 * (long long)c * lda;
+  for (int j = k0; j < k1; ++j) {
+    const int p = pv[j];
+    if (p != j) {
+      const T xr = col[2 * j], xi = col[2 * j + 1];
+      col[2 * j] = col[2 * p];
+      col[2 * j + 1] = col[2 * p + 1];
+      col[2 * p] = xr;
+      col[2 * p + 1] = xi;
+    }
+  }
+}
+
+// X[kb x ncols] <- L^{-1} X (unit lower, L = T[0:kb,0:kb]) or U^{-1} X (upper).
+// 64 lanes per right-hand side, 4 right-hand sides per workgroup.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long long sT, int kb, T* X, int ldx,
+                                                    long long sX, int ncols) {
+  __shared__ T l_re[kLuNb][kLuNb + 1], l_im[kLuNb][kLuNb + 1];
+  __shared__ T x_re[4][kLuNb], x_im[4][kLuNb];
+  const T* t = Tm + 2 * sT * blockIdx.z;
+  T* x = X + 2 * sX * blockIdx.z;
+  for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
+    const int r = e % kb, c = e / kb;
+    const long long off = (long long)r + (long long)c * ldt;
+    l_re[r][c] = t[2 * off];
+    l_im[r][c] = t[2 * off + 1];
+  }
+  const int q = threadIdx.x >> 6, r = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + q;
+  const bool act = col < ncols && r < kb;
+  if (act) {
+    const long long off = (long long)r + (long long)col * ldx;
+    x_re[q][r] = x[2 * off];
+    x_im[q][r] = x[2 * off + 1];
+  }
+  __syncthreads();
+  if (!UPPER) {
+    for (int k = 0; k < kb; ++k) {
+      if (act && r > k) {
+        const T lr = l_re[r][k], li = l_im[r][k], br = x_re[q][k], bi = x_im[q][k];
+        x_re[q][r] -= lr * br - li * bi;
+        x_im[q][r] -= lr * bi + li * br;
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int k = kb - 1; k >= 0; --k) {
+      if (act && r == k) {
+        const T dr = l_re[k][k], di = l_im[k][k], den = dr * dr + di * di;
+        const T xr = x_re[q][k], xi = x_im[q][k];
+        x_re[q][k] = (xr * dr + xi * di) / den;
+        x_im[q][k] = (xi * dr - xr * di) / den;
+      }
+      __syncthreads();
+      if (act && r < k) {
+        const T lr = l_re[r][k], li = l_im[r][k], br = x_re[q][k], bi = x_im[q][k];
+        x_re[q][r] -= lr * br - li * bi;
+        x_im[q][r] -= lr * bi + li * br;
+      }
+      __syncthreads();
+    }
+  }
+  if (act) {
+    const long long off = (long long)r + (long long)col * ldx;
+    x[2 * off] = x_re[q][r];
+    x[2 * off + 1] = x_im[q][r];
+  }
+}
+
+template <typename T>
+hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipiv, int* info,
+                     unsigned* counter, int batch) {
+  for (int k0 = 0; k0 < n; k0 += kLuNb) {
+    const int k1 = k0 + kLuNb < n ? k0 + kLuNb : n;
+    hipLaunchKernelGGL((k_lu_pivot<T>), dim3(1, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, k0, k1, ipiv, info);
+    for (int j = k0; j < k1; ++j) {
+      const int rows = n - j - 1;
+      const int gx = rows > 0 ? (rows + 255) / 256 : 1;
+      hipLaunchKernelGGL((k_lu_col<T>), dim3(gx, 1, batch), dim3(256), 0, s, A, lda, sA, n, j, k0, k1, ipiv, info, counter);
+    }
+    const int other = n - (k1 - k0);
+    if (other > 0)
+      hipLaunchKernelGGL((k_lu_laswp<T>), dim3((other + 255) / 256, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, k1, ipiv);
+    if (k1 < n) {
+      const int nc = n - k1;
+      const long long d = (long long)k0 + (long long)k0 * lda;
+      hipLaunchKernelGGL((k_trsm_block<T, false>), dim3((nc + 3) / 4, 1, batch), dim3(256), 0, s,
+                         A + 2 * d, lda, sA, k1 - k0, A + 2 * ((long long)k0 + (long long)k1 * lda), lda, sA, nc);
+      hipError_t e = cgemm<T>(s, false, n - k1, nc, k1 - k0, T(-1), T(0),
+                              A + 2 * ((long long)k1 + (long long)k0 * lda), lda, sA,
+                              A + 2 * ((long long)k0 + (long long)k1 * lda), lda, sA, T(1), T(0),
+                              A + 2 * ((long long)k1 + (long long)k1 * lda), lda, sA, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
+// B[n x nrhs] rows permuted by the pivot sequence (one thread per column)
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pivots(T* B, int ldb, long long sB, int n, int nrhs,
+                                                      const int* ipiv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  T* col = B + 2 * sB * blockIdx.z + 2 * (long long)c * ldb;
+  const int* pv = ipiv + (long long)n * blockIdx.z;
+  for (int j = 0; j < n; ++j) {
+    const int p = pv[j];
+    if (p != j) {
+      const T xr = col[2 * j], xi = col[2 * j + 1];
+      col[2 * j] = col[2 * p];
+      col[2 * j + 1] = col[2 * p + 1];
+      col[2 * p] = xr;
+      col[2 * p + 1] = xi;
+    }
+  }
+}
+
+template <typename T>
+hipError_t lu_solve(hipStream_t s, const T* A, int lda, long long sA, int n, const int* ipiv, T* B, int ldb,
+                    long long sB, int nrhs, int batch) {
+  hipLaunchKernelGGL((k_apply_pivots<T>), dim3((nrhs + 255) / 256, 1, batch), dim3(256), 0, s, B, ldb, sB, n, nrhs, ipiv);
+  // forward: unit lower
+  for (int k0 = 0; k0 < n; k0 += kLuNb) {
+    const int kb = (n - k0) < kLuNb ? n - k0 : kLuNb;
+    hipLaunchKernelGGL((k_trsm_block<T, false>), dim3((nrhs + 3) / 4, 1, batch), dim3(256), 0, s,
+                       A + 2 * ((long long)k0 + (long long)k0 * lda), lda, sA, kb, B + 2 * (long long)k0, ldb, sB, nrhs);
+    if (k0 + kb < n) {
+      hipError_t e = cgemm<T>(s, false, n - k0 - kb, nrhs, kb, T(-1), T(0),
+                              A + 2 * ((long long)(k0 + kb) + (long long)k0 * lda), lda, sA,
+                              B + 2 * (long long)k0, ldb, sB, T(1), T(0), B + 2 * (long long)(k0 + kb), ldb, sB, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  // backward: upper
+  const int nblk = (n + kLuNb - 1) / kLuNb;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * kLuNb, kb = (n - k0) < kLuNb ? n - k0 : kLuNb;
+    hipLaunchKernelGGL((k_trsm_block<T, true>), dim3((nrhs + 3) / 4, 1, batch), dim3(256), 0, s,
+                       A + 2 * ((long long)k0 + (long long)k0 * lda), lda, sA, kb, B + 2 * (long long)k0, ldb, sB, nrhs);
+    if (k0 > 0) {
+      hipError_t e = cgemm<T>(s, false, k0, nrhs, kb, T(-1), T(0), A + 2 * ((long long)k0 * lda), lda, sA,
+                              B + 2 * (long long)k0, ldb, sB, T(1), T(0), B, ldb, sB, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
+// complex128 -> complex64 copy of an m x n column-major block (ld may differ)
+__global__ void k_demote(const double* __restrict__ src, int lds, long long ss, float* __restrict__ dst, int ldd,
+                         long long sd, int m, int n) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  const double* s = src + 2 * ss * blockIdx.z + 2 * ((long long)r + (long long)c * lds);
+  float* d = dst + 2 * sd * blockIdx.z + 2 * ((long long)r + (long long)c * ldd);
+  d[0] = (float)s[0];
+  d[1] = (float)s[1];
+}
+__global__ void k_promote_add(const float* __restrict__ src, int lds, long long ss, double* __restrict__ dst,
+                              int ldd, long long sd, int m, int n, int overwrite) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  const int r = (int)(e % m), c = (int)(e / m);
+  const float* s = src + 2 * ss * blockIdx.z + 2 * ((long long)r + (long long)c * lds);
+  double* d = dst + 2 * sd * blockIdx.z + 2 * ((long long)r + (long long)c * ldd);
+  if (overwrite) { d[0] = s[0]; d[1] = s[1]; }
+  else { d[0] += s[0]; d[1] += s[1]; }
+}
+
+hipError_t demote(hipStream_t s, const double* src, int lds, long long ss, float* dst, int ldd, long long sd, int m,
+                  int n, int batch) {
+  const long long tot = (long long)m * n;
+  hipLaunchKernelGGL(k_demote, dim3((unsigned)((tot + 255) / 256), 1, batch), dim3(256), 0, s, src, lds, ss, dst, ldd, sd, m, n);
+  return hipGetLastError();
+}
+hipError_t promote_add(hipStream_t s, const float* src, int lds, long long ss, double* dst, int ldd, long long sd,
+                       int m, int n, int batch, bool overwrite) {
+  const long long tot = (long long)m * n;
+  hipLaunchKernelGGL(k_promote_add, dim3((unsigned)((tot + 255) / 256), 1, batch), dim3(256), 0, s, src, lds, ss, dst, ldd, sd, m, n, overwrite ? 1 : 0);
+  return hipGetLastError();
+}
+
+template hipError_t cgemm<double>(hipStream_t, bool, int, int, int, double, double, const double*, int, long long,
+                                  const double*, int, long long, double, double, double*, int, long long, int);
+template hipError_t cgemm<float>(hipStream_t, bool, int, int, int, float, float, const float*, int, long long,
+                                 const float*, int, long long, float, float, float*, int, long long, int);
+template hipError_t lu_factor<double>(hipStream_t, double*, int, long long, int, int*, int*, unsigned*, int);
+template hipError_t lu_factor<float>(hipStream_t, float*, int, long long, int, int*, int*, unsigned*, int);
+template hipError_t lu_solve<double>(hipStream_t, const double*, int, long long, int, const int*, double*, int,
+                                     long long, int, int);
+template hipError_t lu_solve<float>(hipStream_t, const float*, int, long long, int, const int*, float*, int,
+                                    long long, int, int);
+
+}  // namespace hm

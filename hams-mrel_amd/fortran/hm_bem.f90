@@ -1,0 +1,318 @@
+! hm_bem.f90 -- Fortran side of the MI355X hot path: ISO_C_BINDING interfaces to
+! include/hm_bem.h and wrappers that keep the reference subroutines' argument
+! lists, so a HAMS-MREL driver loop can call the GPU path in place of
+!   FINGREEN3D / INFGREEN3D            (FinGreen3D.f90:81, InfGreen_Appr.f90:104)
+!   CALGREEN_MULTI + ASSB_LEFT_MULTI   (CalGreenFuncMulti.f90:112, AssbMatxMulti.f90:53)
+!   ASSB_RBC_MULTI / ASSB_DBC_MULTI    (AssbMatxMulti.f90:124, :187)
+!   RADIATION/DIFFRACTION_SOLVER_MULTI (AssbMatxMulti.f90:257, :323)
+!   RFORCE_MULTI / EFORCE_MULTI        (PotentWavForceMulti.f90:138, :45)
+! The reference passes geometry and wave state through module variables
+! (WavDynMods.f90); here the driver hands them over once with HM_SETUP and per
+! frequency with HM_SET_FREQUENCY. NSYS (geometric symmetry) must be 1.
+module hm_bem
+   use iso_c_binding
+   implicit none
+   private
+
+   integer, parameter, public :: HM_FP64 = 0, HM_FP32_IR = 1
+
+   interface
+      integer(c_int) function hm_create(ctx) bind(C, name="hm_create")
+         import :: c_int, c_ptr
+         type(c_ptr), intent(out) :: ctx
+      end function
+      integer(c_int) function hm_destroy(ctx) bind(C, name="hm_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function hm_set_constants(ctx, rho, g, g_disp, amp, xw, yw) bind(C, name="hm_set_constants")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: rho, g, g_disp, amp, xw, yw
+      end function
+      integer(c_int) function hm_set_geometry(ctx, nbody, body_start, ntnd, xyz, ncn, ncon, xyz_p, ds, pnsz, &
+                                              dxyz_p, ni, ixyz_p) bind(C, name="hm_set_geometry")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nbody, ntnd, ni
+         integer(c_int), intent(in) :: body_start(*), ncn(*), ncon(*)
+         real(c_double), intent(in) :: xyz(*), xyz_p(*), ds(*), pnsz(*), dxyz_p(*), ixyz_p(*)
+      end function
+      integer(c_int) function hm_set_depth(ctx, h) bind(C, name="hm_set_depth")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: h
+      end function
+      integer(c_int) function hm_set_headings(ctx, nbeta, beta) bind(C, name="hm_set_headings")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nbeta
+         real(c_double), intent(in) :: beta(*)
+      end function
+      integer(c_int) function hm_set_frequency(ctx, w1) bind(C, name="hm_set_frequency")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), value :: w1
+      end function
+      integer(c_int) function hm_assemble(ctx) bind(C, name="hm_assemble")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function hm_factor(ctx, precision) bind(C, name="hm_factor")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: precision
+      end function
+      integer(c_int) function hm_solve(ctx, iters) bind(C, name="hm_solve")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: iters
+      end function
+      integer(c_int) function hm_forces(ctx, amas, bdmp, exfc) bind(C, name="hm_forces")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: amas(*), bdmp(*)
+         complex(c_double_complex), intent(out) :: exfc(*)
+      end function
+      integer(c_int) function hm_get_system(ctx, amat, smat, rhs) bind(C, name="hm_get_system")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, amat, smat, rhs
+      end function
+      integer(c_int) function hm_get_potentials(ctx, mxpot) bind(C, name="hm_get_potentials")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         complex(c_double_complex), intent(out) :: mxpot(*)
+      end function
+      integer(c_int) function hm_fingreen3d_batch(ctx, n, xf, xp, yf, yp, zf, zp, v, wvn, nk, h, flag, grn) &
+                                                  bind(C, name="hm_fingreen3d_batch")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n, nk
+         real(c_double), intent(in) :: xf(*), xp(*), yf(*), yp(*), zf(*), zp(*), wvn(*)
+         real(c_double), value :: v, h
+         integer(c_int), intent(in) :: flag(*)
+         complex(c_double_complex), intent(out) :: grn(*)
+      end function
+      integer(c_int) function hm_infgreen3d_batch(ctx, n, xf, xp, yf, yp, zf, zp, v, flag, grn) &
+                                                  bind(C, name="hm_infgreen3d_batch")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         real(c_double), intent(in) :: xf(*), xp(*), yf(*), yp(*), zf(*), zp(*)
+         real(c_double), value :: v
+         integer(c_int), intent(in) :: flag(*)
+         complex(c_double_complex), intent(out) :: grn(*)
+      end function
+      integer(c_int) function hm_zgetrs(ctx, trans, n, nrhs, a, lda, ipiv, b, ldb, info) bind(C, name="hm_zgetrs")
+         import :: c_int, c_ptr, c_char, c_double_complex
+         type(c_ptr), value :: ctx
+         character(kind=c_char), value :: trans
+         integer(c_int), value :: n, nrhs, lda, ldb
+         complex(c_double_complex), intent(in) :: a(*)
+         integer(c_int), intent(in) :: ipiv(*)
+         complex(c_double_complex), intent(inout) :: b(*)
+         integer(c_int), intent(out) :: info
+      end function
+      integer(c_int) function hm_zgetrf(ctx, m, n, a, lda, ipiv, info) bind(C, name="hm_zgetrf")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: m, n, lda
+         complex(c_double_complex), intent(inout) :: a(*)
+         integer(c_int), intent(out) :: ipiv(*), info
+      end function
+   end interface
+
+   public :: hm_create, hm_destroy, hm_set_constants, hm_set_geometry, hm_set_depth, hm_set_headings, &
+             hm_set_frequency, hm_assemble, hm_factor, hm_solve, hm_forces, hm_get_system, hm_get_potentials, &
+             hm_fingreen3d_batch, hm_infgreen3d_batch, hm_zgetrf, hm_zgetrs
+   public :: HM_CONTEXT, HM_SETUP, HM_SET_FREQUENCY_F, HM_CHECK
+   public :: FINGREEN3D, INFGREEN3D
+   public :: ASSB_LEFT_MULTI, ASSB_RBC_MULTI, ASSB_DBC_MULTI, RADIATION_SOLVER_MULTI, DIFFRACTION_SOLVER_MULTI
+   public :: RFORCE_EFORCE_MULTI
+
+   type(c_ptr), save :: ctx_ = c_null_ptr
+   integer, save :: n_ = 0, nbody_ = 0, nbeta_ = 0
+   integer, allocatable, save :: body_start_(:)
+   complex(c_double_complex), allocatable, target, save :: rhs_(:, :)
+
+contains
+
+   ! the module's context, created on first use
+   function HM_CONTEXT() result(ctx)
+      type(c_ptr) :: ctx
+      if (.not. c_associated(ctx_)) call HM_CHECK(hm_create(ctx_), 'hm_create')
+      ctx = ctx_
+   end function
+
+   subroutine HM_CHECK(rc, what)
+      integer(c_int), intent(in) :: rc
+      character(*), intent(in) :: what
+      if (rc /= 0) then
+         write (*, '(A,A,A,I0)') ' hm_bem: ', what, ' failed, code ', rc
+         error stop 1
+      end if
+   end subroutine
+
+   ! geometry, depth, headings and constants of the run: the module state the
+   ! reference keeps in WavDynMods.f90 (the *_MULTI_COMB arrays, H, WVHD, RHO, G)
+   subroutine HM_SETUP(NBODY, BODY_START, NTND, XYZ, NCN, NCON, XYZ_P, DS, PNSZ, DXYZ_P, NI, IXYZ_P, H, &
+                       NBETA, BETA, RHO, G, G_DISP, AMP, XW)
+      integer, intent(in) :: NBODY, BODY_START(NBODY + 1), NTND, NCN(:), NCON(:, :), NI, NBETA
+      real(8), intent(in) :: XYZ(:, :), XYZ_P(:, :), DS(:), PNSZ(:), DXYZ_P(:, :), IXYZ_P(:, :), H
+      real(8), intent(in) :: BETA(NBETA), RHO, G, G_DISP, AMP, XW(2)
+      type(c_ptr) :: c
+      integer(c_int), allocatable :: ncon_t(:, :)
+      c = HM_CONTEXT()
+      ! reference layout is (panel, k); the C-ABI is panel-major (k fastest)
+      allocate (ncon_t(4, size(NCN)))
+      ncon_t = transpose(NCON(:, 1:4))
+      call HM_CHECK(hm_set_constants(c, RHO, G, G_DISP, AMP, XW(1), XW(2)), 'hm_set_constants')
+      call HM_CHECK(hm_set_geometry(c, NBODY, BODY_START, NTND, transpose(XYZ), NCN, ncon_t, transpose(XYZ_P), &
+                                    DS, PNSZ, transpose(DXYZ_P), NI, transpose(IXYZ_P)), 'hm_set_geometry')
+      call HM_CHECK(hm_set_depth(c, H), 'hm_set_depth')
+      call HM_CHECK(hm_set_headings(c, NBETA, BETA), 'hm_set_headings')
+      n_ = size(NCN)
+      nbody_ = NBODY
+      if (allocated(body_start_)) deallocate (body_start_)
+      allocate (body_start_(NBODY + 1))
+      body_start_ = BODY_START
+      nbeta_ = NBETA
+   end subroutine
+
+   ! CalWaveProperts(KK) for INFT=3 (ImplementSubs.f90:96): omega -> V, WVN
+   subroutine HM_SET_FREQUENCY_F(W1)
+      real(8), intent(in) :: W1
+      call HM_CHECK(hm_set_frequency(HM_CONTEXT(), W1), 'hm_set_frequency')
+   end subroutine
+
+   ! ---- element-wise kernels with the reference argument lists ----------------
+   subroutine FINGREEN3D(XF, XP, YF, YP, ZF, ZP, V, WVN, NK, H, GRN, FLAG)
+      integer, intent(in) :: NK, FLAG
+      real(8), intent(in) :: XF, XP, YF, YP, ZF, ZP, V, WVN(1:NK), H
+      complex(8), intent(out) :: GRN(4)
+      integer(c_int) :: f(1)
+      f(1) = FLAG
+      call HM_CHECK(hm_fingreen3d_batch(HM_CONTEXT(), 1, [XF], [XP], [YF], [YP], [ZF], [ZP], V, WVN, NK, H, f, GRN), &
+                    'FINGREEN3D')
+   end subroutine
+
+   subroutine INFGREEN3D(XF, XP, YF, YP, ZF, ZP, V, GRN, FLAG)
+      integer, intent(in) :: FLAG
+      real(8), intent(in) :: XF, XP, YF, YP, ZF, ZP, V
+      complex(8), intent(out) :: GRN(4)
+      integer(c_int) :: f(1)
+      f(1) = FLAG
+      call HM_CHECK(hm_infgreen3d_batch(HM_CONTEXT(), 1, [XF], [XP], [YF], [YP], [ZF], [ZP], V, f, GRN), 'INFGREEN3D')
+   end subroutine
+
+   ! ---- the per-frequency chain with the reference argument lists --------------
+   ! CALGREEN_MULTI + ASSB_LEFT_MULTI: assemble on the device and return the LU
+   ! factors and pivots (ZGETRF, AssbMatxMulti.f90:112) as the reference does
+   subroutine ASSB_LEFT_MULTI(AMAT, IPIV, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(out) :: IPIV(NELEM, NSYS)
+      complex(8), intent(out), target :: AMAT(NELEM, NELEM, NSYS)
+      integer(c_int) :: info, ncol
+      type(c_ptr) :: c
+      if (NSYS /= 1) call HM_CHECK(-101_c_int, 'ASSB_LEFT_MULTI: NSYS must be 1')
+      c = HM_CONTEXT()
+      ncol = 6 * nbody_ + nbeta_
+      if (allocated(rhs_)) deallocate (rhs_)
+      allocate (rhs_(NELEM, ncol))
+      call HM_CHECK(hm_assemble(c), 'hm_assemble')
+      call HM_CHECK(hm_get_system(c, c_loc(AMAT), c_null_ptr, c_loc(rhs_)), 'hm_get_system')
+      call HM_CHECK(hm_zgetrf(c, NELEM, NELEM, AMAT, NELEM, IPIV, info), 'ZGETRF')
+   end subroutine
+
+   ! ASSB_RBC_MULTI: the 6 radiation columns of the body whose panels are
+   ! NELEM_START..NELEM_END (the driver calls it once per body)
+   subroutine ASSB_RBC_MULTI(BRMAT, NELEM, NSYS, NELEM_START, NELEM_END)
+      integer, intent(in) :: NELEM, NSYS, NELEM_START, NELEM_END
+      complex(8), intent(inout) :: BRMAT(NELEM, 6, NSYS)
+      integer :: b
+      do b = 0, nbody_ - 1
+         if (body_start_(b + 1) == NELEM_START - 1) exit
+      end do
+      if (b >= nbody_ .or. body_start_(b + 2) /= NELEM_END .or. NSYS /= 1) &
+         call HM_CHECK(-103_c_int, 'ASSB_RBC_MULTI: panel range is not one body')
+      BRMAT(:, :, 1) = BRMAT(:, :, 1) + rhs_(:, 6 * b + 1:6 * b + 6)
+   end subroutine
+
+   ! ASSB_DBC_MULTI: diffraction right-hand side of heading HEADING (1-based)
+   subroutine ASSB_DBC_MULTI(BDMAT, NELEM, NSYS, HEADING)
+      integer, intent(in) :: NELEM, NSYS, HEADING
+      complex(8), intent(out) :: BDMAT(NELEM, NSYS)
+      BDMAT(:, 1) = rhs_(:, 6 * nbody_ + HEADING)
+   end subroutine
+
+   subroutine RADIATION_SOLVER_MULTI(AMAT, BRMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: AMAT(NELEM, NELEM, NSYS), BRMAT(NELEM, 6 * nbody_, NSYS)
+      complex(8), intent(out) :: MXPOT(NELEM, 6 * nbody_ + 1, NSYS)
+      integer(c_int) :: info
+      MXPOT(:, 1:6 * nbody_, 1) = BRMAT(:, :, 1)
+      call HM_CHECK(hm_zgetrs(HM_CONTEXT(), 'N', NELEM, 6 * nbody_, AMAT, NELEM, IPIV, MXPOT, NELEM, info), 'ZGETRS')
+   end subroutine
+
+   subroutine DIFFRACTION_SOLVER_MULTI(AMAT, BDMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: AMAT(NELEM, NELEM, NSYS), BDMAT(NELEM, NSYS)
+      complex(8), intent(inout) :: MXPOT(NELEM, 6 * nbody_ + 1, NSYS)
+      integer(c_int) :: info
+      MXPOT(:, 6 * nbody_ + 1, 1) = BDMAT(:, 1)
+      call HM_CHECK(hm_zgetrs(HM_CONTEXT(), 'N', NELEM, 1, AMAT, NELEM, IPIV, MXPOT(:, 6 * nbody_ + 1, 1), NELEM, &
+                              info), 'ZGETRS')
+   end subroutine
+
+   ! RFORCE_MULTI + EFORCE_MULTI for all bodies and headings at once, from the
+   ! device-resident solve (HM_FP64 or HM_FP32_IR): AMAS/BDMP (6nb x 6nb),
+   ! EXFC (6nb x nbeta)
+   subroutine RFORCE_EFORCE_MULTI(PRECISION, AMAS, BDMP, EXFC, ITERS)
+      integer, intent(in) :: PRECISION
+      real(8), intent(out) :: AMAS(6 * nbody_, 6 * nbody_), BDMP(6 * nbody_, 6 * nbody_)
+      complex(8), intent(out) :: EXFC(6 * nbody_, nbeta_)
+      integer, intent(out) :: ITERS
+      type(c_ptr) :: c
+      integer(c_int) :: it
+      c = HM_CONTEXT()
+      call HM_CHECK(hm_assemble(c), 'hm_assemble')
+      call HM_CHECK(hm_factor(c, PRECISION), 'hm_factor')
+      call HM_CHECK(hm_solve(c, it), 'hm_solve')
+      call HM_CHECK(hm_forces(c, AMAS, BDMP, EXFC), 'hm_forces')
+      ITERS = it
+   end subroutine
+
+end module hm_bem
+
+! LAPACK-named external drop-ins: linking this object in place of MKL routes the
+! reference's own ZGETRF/ZGETRS calls (AssbMatxMulti.f90:112,:279,:346) to the GPU.
+subroutine ZGETRF(M, N, A, LDA, IPIV, INFO)
+   use hm_bem, only: HM_CONTEXT, hm_zgetrf
+   use iso_c_binding
+   implicit none
+   integer, intent(in) :: M, N, LDA
+   complex(8), intent(inout) :: A(LDA, *)
+   integer, intent(out) :: IPIV(*), INFO
+   integer(c_int) :: rc, inf
+   rc = hm_zgetrf(HM_CONTEXT(), M, N, A, LDA, IPIV, inf)
+   INFO = inf
+   if (rc < 0) INFO = rc
+end subroutine
+
+subroutine ZGETRS(TRANS, N, NRHS, A, LDA, IPIV, B, LDB, INFO)
+   use hm_bem, only: HM_CONTEXT, hm_zgetrs
+   use iso_c_binding
+   implicit none
+   character, intent(in) :: TRANS
+   integer, intent(in) :: N, NRHS, LDA, LDB
+   complex(8), intent(in) :: A(LDA, *)
+   integer, intent(in) :: IPIV(*)
+   complex(8), intent(inout) :: B(LDB, *)
+   integer, intent(out) :: INFO
+   integer(c_int) :: rc, inf
+   rc = hm_zgetrs(HM_CONTEXT(), TRANS, N, NRHS, A, LDA, IPIV, B, LDB, inf)
+   INFO = inf
+   if (rc < 0) INFO = rc
+end subroutine

@@ -1,0 +1,235 @@
+"""ctypes binding of the MI355X hot-path library (lib/libhm_bem.so, include/hm_bem.h).
+
+This is the Python view of the same C-ABI the Fortran layer (fortran/hm_bem.f90)
+binds. Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libhm_bem.so")
+
+HM_FP64 = 0
+HM_FP32_IR = 1
+STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces")
+
+_d = ctypes.c_double
+_i = ctypes.c_int
+_vp = ctypes.c_void_p
+
+# every symbol include/hm_bem.h declares
+EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm_set_geometry",
+           "hm_set_depth", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
+           "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
+           "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
+           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C hams-mrel_amd` "
+                               "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hm_error_string.restype = ctypes.c_char_p
+        L.hm_error_string.argtypes = [_vp]
+        for name in EXPORTS:
+            if name != "hm_error_string":
+                getattr(L, name).restype = _i
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+class HMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hm_bem error {code}: {msg}")
+        self.code = code
+
+
+def dispersion(w, h, g_disp, nrt=200):
+    wvn = np.zeros(nrt)
+    rc = load().hm_dispersion(_p(wvn), _i(nrt), _d(w), _d(h), _d(g_disp))
+    if rc:
+        raise HMError(rc, "hm_dispersion")
+    return wvn
+
+
+class Context:
+    """One HIP stream + the HBM-resident state of one problem."""
+
+    def __init__(self):
+        self.L = load()
+        self.ptr = _vp()
+        rc = self.L.hm_create(ctypes.byref(self.ptr))
+        if rc:
+            raise HMError(rc, "hm_create (no GPU?)")
+        self.problem = None
+
+    def close(self):
+        if self.ptr:
+            self.L.hm_destroy(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise HMError(rc, f"{what}: {self.L.hm_error_string(self.ptr).decode()}")
+
+    # ---- problem set-up -------------------------------------------------------
+    def setup(self, prob):
+        """prob: meshes.Problem"""
+        self.problem = prob
+        keep = dict(bs=np.ascontiguousarray(prob.body_start, dtype=np.int32),
+                    xyz=np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                    ncn=np.ascontiguousarray(prob.ncn, dtype=np.int32),
+                    ncon=np.ascontiguousarray(prob.ncon, dtype=np.int32),
+                    xp=np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
+                    ds=np.ascontiguousarray(prob.ds, dtype=np.float64),
+                    pn=np.ascontiguousarray(prob.pnsz, dtype=np.float64),
+                    dx=np.ascontiguousarray(prob.dxyz_p, dtype=np.float64),
+                    ix=np.ascontiguousarray(prob.ixyz_p if prob.ni else np.zeros((1, 3)), dtype=np.float64),
+                    beta=np.ascontiguousarray(prob.beta, dtype=np.float64))
+        self._chk(self.L.hm_set_constants(self.ptr, _d(prob.rho), _d(prob.g), _d(prob.g_disp), _d(prob.amp),
+                                          _d(prob.xw[0]), _d(prob.xw[1])), "hm_set_constants")
+        self._chk(self.L.hm_set_geometry(self.ptr, _i(prob.nbody), _p(keep["bs"]), _i(len(prob.xyz)),
+                                         _p(keep["xyz"]), _p(keep["ncn"]), _p(keep["ncon"]), _p(keep["xp"]),
+                                         _p(keep["ds"]), _p(keep["pn"]), _p(keep["dx"]), _i(prob.ni),
+                                         _p(keep["ix"])), "hm_set_geometry")
+        self._chk(self.L.hm_set_depth(self.ptr, _d(prob.h)), "hm_set_depth")
+        self._chk(self.L.hm_set_headings(self.ptr, _i(prob.nbeta), _p(keep["beta"])), "hm_set_headings")
+
+    def set_frequency(self, w1):
+        self._chk(self.L.hm_set_frequency(self.ptr, _d(w1)), "hm_set_frequency")
+
+    def assemble(self):
+        self._chk(self.L.hm_assemble(self.ptr), "hm_assemble")
+
+    def factor(self, precision=HM_FP64):
+        self._chk(self.L.hm_factor(self.ptr, _i(precision)), "hm_factor")
+
+    def solve(self):
+        it = _i(0)
+        self._chk(self.L.hm_solve(self.ptr, ctypes.byref(it)), "hm_solve")
+        return it.value
+
+    def forces(self):
+        p = self.problem
+        m, nb = p.m, p.nbeta
+        amas = np.zeros((m, m), order="F")
+        bdmp = np.zeros((m, m), order="F")
+        exfc = np.zeros((m, max(nb, 1)), dtype=np.complex128, order="F")
+        self._chk(self.L.hm_forces(self.ptr, _p(amas), _p(bdmp), _p(exfc)), "hm_forces")
+        return amas, bdmp, exfc[:, :nb]
+
+    def solve_frequency(self, w1, precision=HM_FP64):
+        self.set_frequency(w1)
+        self.assemble()
+        self.factor(precision)
+        iters = self.solve()
+        a, b, f = self.forces()
+        return dict(amas=a, bdmp=b, exfc=f, iters=iters)
+
+    def run_frequencies(self, omegas, precision=HM_FP64):
+        p = self.problem
+        m, nb = p.m, p.nbeta
+        w = np.ascontiguousarray(omegas, dtype=np.float64)
+        amas = np.zeros((len(w), m, m))
+        bdmp = np.zeros((len(w), m, m))
+        exfc = np.zeros((len(w), nb, m), dtype=np.complex128)
+        self._chk(self.L.hm_run_frequencies(self.ptr, _i(len(w)), _p(w), _i(precision), _p(amas), _p(bdmp),
+                                            _p(exfc)), "hm_run_frequencies")
+        # each frequency block is column-major (m x m), (m x nb)
+        return amas.transpose(0, 2, 1), bdmp.transpose(0, 2, 1), exfc.transpose(0, 2, 1)
+
+    def get_system(self):
+        p = self.problem
+        nt, n, ncol = p.n + p.ni, p.n, p.m + p.nbeta
+        amat = np.zeros((nt, n), dtype=np.complex128, order="F")
+        smat = np.zeros((nt, n), dtype=np.complex128, order="F")
+        rhs = np.zeros((nt, ncol), dtype=np.complex128, order="F")
+        self._chk(self.L.hm_get_system(self.ptr, _p(amat), _p(smat), _p(rhs)), "hm_get_system")
+        return amat, smat, rhs
+
+    def get_potentials(self):
+        p = self.problem
+        x = np.zeros((p.n, p.m + p.nbeta), dtype=np.complex128, order="F")
+        self._chk(self.L.hm_get_potentials(self.ptr, _p(x)), "hm_get_potentials")
+        return x
+
+    def set_timing(self, on=True):
+        self._chk(self.L.hm_set_timing(self.ptr, _i(1 if on else 0)), "hm_set_timing")
+
+    def timing(self):
+        ms = np.zeros(len(STAGES))
+        cnt = np.zeros(len(STAGES), dtype=np.int32)
+        self._chk(self.L.hm_get_timing(self.ptr, _p(ms), _p(cnt), _i(len(STAGES))), "hm_get_timing")
+        return {s: (float(ms[k]), int(cnt[k])) for k, s in enumerate(STAGES)}
+
+    # ---- element-wise kernels --------------------------------------------------
+    def fingreen3d(self, src, fld, v, wvn, h, flag):
+        """src, fld: (n,3) source/field points; flag: (n,) -> (n,4) complex GRN"""
+        src = np.asarray(src, dtype=np.float64)
+        fld = np.asarray(fld, dtype=np.float64)
+        n = len(src)
+        cols = [np.ascontiguousarray(a) for a in (src[:, 0], fld[:, 0], src[:, 1], fld[:, 1], src[:, 2], fld[:, 2])]
+        fl = np.ascontiguousarray(flag, dtype=np.int32)
+        wvn = np.ascontiguousarray(wvn, dtype=np.float64)
+        out = np.zeros((n, 4), dtype=np.complex128)
+        self._chk(self.L.hm_fingreen3d_batch(self.ptr, _i(n), *[_p(c) for c in cols], _d(v), _p(wvn), _i(len(wvn)),
+                                             _d(h), _p(fl), _p(out)), "hm_fingreen3d_batch")
+        return out
+
+    def infgreen3d(self, src, fld, v, flag):
+        src = np.asarray(src, dtype=np.float64)
+        fld = np.asarray(fld, dtype=np.float64)
+        n = len(src)
+        cols = [np.ascontiguousarray(a) for a in (src[:, 0], fld[:, 0], src[:, 1], fld[:, 1], src[:, 2], fld[:, 2])]
+        fl = np.ascontiguousarray(flag, dtype=np.int32)
+        out = np.zeros((n, 4), dtype=np.complex128)
+        self._chk(self.L.hm_infgreen3d_batch(self.ptr, _i(n), *[_p(c) for c in cols], _d(v), _p(fl), _p(out)),
+                  "hm_infgreen3d_batch")
+        return out
+
+    def sglintbd(self, ncn, verts, xf):
+        ncn = np.ascontiguousarray(ncn, dtype=np.int32)
+        verts = np.ascontiguousarray(verts, dtype=np.float64).reshape(len(ncn), 12)
+        xf = np.ascontiguousarray(xf, dtype=np.float64).reshape(len(ncn), 3)
+        s = np.zeros(len(ncn))
+        d = np.zeros((len(ncn), 3))
+        self._chk(self.L.hm_sglintbd_batch(self.ptr, _i(len(ncn)), _p(ncn), _p(verts), _p(xf), _p(s), _p(d)),
+                  "hm_sglintbd_batch")
+        return s, d
+
+    def zgetrf(self, a):
+        a = np.array(a, dtype=np.complex128, order="F", copy=True)
+        n = a.shape[0]
+        ipiv = np.zeros(n, dtype=np.int32)
+        info = _i(0)
+        self._chk(self.L.hm_zgetrf(self.ptr, _i(n), _i(a.shape[1]), _p(a), _i(n), _p(ipiv), ctypes.byref(info)),
+                  "hm_zgetrf")
+        return a, ipiv, info.value
+
+    def zgetrs(self, lu, ipiv, b):
+        b = np.array(b, dtype=np.complex128, order="F", copy=True)
+        if b.ndim == 1:
+            b = b[:, None]
+        n = lu.shape[0]
+        info = _i(0)
+        self._chk(self.L.hm_zgetrs(self.ptr, ctypes.c_char(b"N"), _i(n), _i(b.shape[1]), _p(lu), _i(n), _p(ipiv),
+                                   _p(b), _i(n), ctypes.byref(info)), "hm_zgetrs")
+        return b

@@ -360,14 +360,19 @@ def c3_random_problems(seed=20261203, count=64, n_freq=32):
     return out
 
 
-def reference_config1(root):
-    """test/data/config1 of the reference: 3 bodies x 234 panels, h=70 m,
-    omega = 0.05, 0.06, 0.07, headings 0, 10, 20 deg, reference constants."""
+def reference_config1_from(inp):
+    """test/data/config1 of the reference (input directory `inp`): 3 bodies x 234
+    panels at y = 0, 54, 108 m, h = 70 m, omega = 0.05, 0.06, 0.07 rad/s,
+    headings 0, 10, 20 deg, with the reference's constants."""
     import os
-    inp = os.path.join(root, "test", "data", "config1", "input")
     ys = (0.0, 54.0, 108.0)
     bodies = [read_pnl(os.path.join(inp, f"HullMesh_{k + 1}.pnl"), lcs=(0.0, y, 0.0, 0.0),
                        xr=(0.0, y, 0.0)) for k, y in enumerate(ys)]
     p = Problem(bodies, h=70.0, omegas=np.array([0.05, 0.06, 0.07]),
                 beta=np.deg2rad(np.array([0.0, 10.0, 20.0])), name="reference config1")
     return reference_parity_constants(p)
+
+
+def reference_config1(root):
+    import os
+    return reference_config1_from(os.path.join(root, "test", "data", "config1", "input"))

@@ -1,0 +1,124 @@
+/*
+ * hm_bem.h -- C-ABI of the MI355X boundary-integral hot path (HAMS-MREL drop-in).
+ *
+ * Plain C types only: int, double, char, and opaque context pointers. Complex
+ * arrays are interleaved (re, im) doubles in Fortran column-major order, i.e.
+ * exactly the memory of a Fortran COMPLEX*16 / complex(c_double_complex) array.
+ * Panel arrays are panel-major (xyz_p[3*i+k]); connectivity is 1-based as in the
+ * reference's .pnl files. One context owns one HIP stream on the current device
+ * and the device buffers of one problem; a context is single-threaded (the
+ * reference's solver state is module-global too, IterativeSolvers.f90:61-74).
+ *
+ * Return codes: HM_OK (0); HM_ERR_ARG - k for a bad k-th argument (1-based);
+ * HM_ERR_STATE when a call comes before the state it needs; HM_ERR_HIP - e for
+ * HIP runtime error e; a positive value is a LAPACK-style INFO (zero pivot at
+ * that column). Nothing in the library calls exit/abort; hm_error_string()
+ * describes the last failure of a context.
+ */
+#ifndef HM_BEM_H
+#define HM_BEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_OK 0
+#define HM_ERR_ARG (-100)
+#define HM_ERR_STATE (-200)
+#define HM_ERR_HIP (-1000)
+
+/* precision of the dense solve */
+#define HM_FP64 0      /* FP64 LU + FP64 triangular solves (plain path)          */
+#define HM_FP32_IR 1   /* FP32 LU on MFMA, FP64 residuals, iterative refinement  */
+
+typedef struct hm_ctx hm_ctx;
+
+/* ---- context ---------------------------------------------------------------- */
+int hm_create(hm_ctx **ctx);
+int hm_destroy(hm_ctx *ctx);
+const char *hm_error_string(const hm_ctx *ctx);
+
+/* Physical constants. The reference hard-wires rho=1000, g=9.80665 and uses
+ * g=9.807 inside DISPERSION (WavDynMods.f90:146, WavDynSubs.f90:58). */
+int hm_set_constants(hm_ctx *ctx, double rho, double g, double g_disp, double amp, double xw, double yw);
+
+/* Geometry of all bodies combined (the *_MULTI_COMB arrays that
+ * BuildCombinedArrays packs, CalGreenFuncMulti.f90:53-106): n hull panels with
+ * nodes xyz[3*ntnd], ncn[n], ncon[4*n] (1-based), centroids xyz_p[3n], areas
+ * ds[n], longest edges pnsz[n], normals dxyz_p[6n]; body b owns panels
+ * [body_start[b], body_start[b+1]). ni > 0 adds interior waterplane field points
+ * ixyz_p[3*ni] for irregular-frequency removal (IRSP=1). */
+int hm_set_geometry(hm_ctx *ctx, int nbody, const int *body_start, int ntnd, const double *xyz, const int *ncn,
+                    const int *ncon, const double *xyz_p, const double *ds, const double *pnsz, const double *dxyz_p,
+                    int ni, const double *ixyz_p);
+/* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
+int hm_set_depth(hm_ctx *ctx, double h);
+/* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */
+int hm_set_headings(hm_ctx *ctx, int nbeta, const double *beta);
+/* Wave frequency omega [rad/s]: V = omega^2/g and the 200 dispersion roots
+ * (CalWaveProperts with INFT=3, ImplementSubs.f90:96-105). */
+int hm_set_frequency(hm_ctx *ctx, double w1);
+
+/* ---- the per-frequency chain (device resident) --------------------------------
+ * hm_assemble   replaces CALGREEN_MULTI(_IRR) + ASSB_LEFT_MULTI's fill +
+ *               ASSB_RBC_MULTI + ASSB_DBC_MULTI (+ the IRR normal equations
+ *               C = A^T A, D = A^T B of AssbMatx_irr.f90:151-169,:300-318,:449-463)
+ *               (CalGreenFuncMulti.f90:112, AssbMatxMulti.f90:53,:124,:187)
+ * hm_factor     replaces ZGETRF on the assembled matrix (AssbMatxMulti.f90:112)
+ * hm_solve      replaces ZGETRS with 6*NBODY + NBETA right-hand sides
+ *               (RADIATION/DIFFRACTION_SOLVER_MULTI, AssbMatxMulti.f90:257,:323);
+ *               *iters receives the refinement steps taken (0 for HM_FP64)
+ * hm_forces     replaces RFORCE_MULTI / EFORCE_MULTI (PotentWavForceMulti.f90:138,:45):
+ *               amas, bdmp: (6nb x 6nb) column-major [force dof, mode];
+ *               exfc: complex (6nb x nbeta) column-major. */
+int hm_assemble(hm_ctx *ctx);
+int hm_factor(hm_ctx *ctx, int precision);
+int hm_solve(hm_ctx *ctx, int *iters);
+int hm_forces(hm_ctx *ctx, double *amas, double *bdmp, double *exfc);
+/* set_frequency + assemble + factor + solve + forces */
+int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, double *bdmp, double *exfc);
+/* nfreq frequencies back to back; outputs stacked per frequency */
+int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, double *amas, double *bdmp,
+                       double *exfc);
+
+/* Stage timing (HIP events on the context's stream): when enabled, each stage's
+ * elapsed milliseconds accumulate into ms[stage] for stages
+ * 0 classify, 1 assembly (Green+Rankine), 2 RHS GEMM, 3 IRR normal equations,
+ * 4 LU, 5 triangular solves + refinement, 6 forces; counts[stage] = launches. */
+int hm_set_timing(hm_ctx *ctx, int enable);
+int hm_get_timing(hm_ctx *ctx, double *ms, int *counts, int nstage);
+
+/* Host copies of the assembled system (after hm_assemble, before hm_factor):
+ * amat, smat: (n+ni) x n complex; rhs: (n+ni) x (6nb+nbeta) complex. Any may be
+ * NULL. hm_get_potentials: n x (6nb+nbeta) complex after hm_solve (MXPOT). */
+int hm_get_system(hm_ctx *ctx, double *amat, double *smat, double *rhs);
+int hm_get_potentials(hm_ctx *ctx, double *mxpot);
+
+/* ---- element-wise kernels (parity entry points, host arrays) ----------------
+ * FINGREEN3D(XF,XP,YF,YP,ZF,ZP,V,WVN,NK,H,GRN,FLAG)  FinGreen3D.f90:81
+ * INFGREEN3D(XF,XP,YF,YP,ZF,ZP,V,GRN,FLAG)           InfGreen_Appr.f90:104
+ * evaluated for n pairs at once; grn is 4 complex per pair (8n doubles). */
+int hm_fingreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, const double *yf, const double *yp,
+                        const double *zf, const double *zp, double v, const double *wvn, int nk, double h,
+                        const int *flag, double *grn);
+int hm_infgreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, const double *yf, const double *yp,
+                        const double *zf, const double *zp, double v, const int *flag, double *grn);
+/* SGLINTBD_QUAD/TRI(IS,IEL,JEL,SIJ,DIJ,IRR)  SingularIntgr.f90:252/:44, with the
+ * source panel's vertices verts[12*i] (ncn[i] of them) and field point xf[3*i]. */
+int hm_sglintbd_batch(hm_ctx *ctx, int n, const int *ncn, const double *verts, const double *xf, double *sij,
+                      double *dij);
+/* DISPERSION(WVN,NRT,W,H)  WavDynSubs.f90:40 (host; g_disp = 9.807 in the reference) */
+int hm_dispersion(double *wvn, int nrt, double w, double h, double g_disp);
+
+/* ---- dense complex LU with LAPACK argument lists (host arrays) --------------
+ * ZGETRF(M,N,A,LDA,IPIV,INFO) and ZGETRS(TRANS,N,NRHS,A,LDA,IPIV,B,LDB,INFO)
+ * as called at AssbMatxMulti.f90:112,:279,:346 (square, TRANS='N'); ipiv is
+ * 1-based. The factorisation runs in FP64 on the GPU. */
+int hm_zgetrf(hm_ctx *ctx, int m, int n, double *a, int lda, int *ipiv, int *info);
+int hm_zgetrs(hm_ctx *ctx, char trans, int n, int nrhs, const double *a, int lda, const int *ipiv, double *b, int ldb,
+              int *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -168,8 +168,8 @@ class Oracle:
         return out
 
     def zgetrf_solve(self, a, b):
-        a = np.asfortranarray(a, dtype=np.complex128).copy()
-        b = np.asfortranarray(b, dtype=np.complex128).copy()
+        a = np.array(a, dtype=np.complex128, order="F", copy=True)
+        b = np.array(b, dtype=np.complex128, order="F", copy=True)
         n = a.shape[0]
         ipiv = np.zeros(n, dtype=np.int32)
         info = self.lib.hmo_zgetrf(_i(n), _ptr(a), _i(n), _ptr(ipiv, _ip))

@@ -1,0 +1,243 @@
+"""The oracle (oracle/hm_oracle.c) pinned before it is trusted.
+
+How each oracle function is pinned:
+  special functions   golden vectors of the reference build (bit-level) + the
+                      known-answer tables in FinGrnExtSubs.f90's comments
+  Limes (Wynn eps)    golden vectors + exact on an alternating series with a
+                      closed-form limit (ln 2)
+  DISPERSION          golden vectors + residual of both dispersion equations
+  FINGREEN3D          golden vectors over all four R/h regimes (<=1e-12) +
+                      continuity across the regime boundaries + reciprocity
+  INFGREEN3D          golden vectors (<=1e-12)
+  SGLINTBD            golden vectors + closed form of a square at its centroid
+                      + far-field limit + brute-force Gauss-Legendre + finite
+                      differences of the potential for the gradient
+  assembly/LU/forces  the reference's own config1 expected outputs (5 printed
+                      digits), A/B symmetry, LU residual
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, relerr
+
+G = 9.80665
+
+
+# ---- special functions ----------------------------------------------------------
+def test_special_functions_golden(oracle, golden):
+    g = golden("special")
+    for x, k in zip(g["x"], g["k01"]):
+        assert oracle.ik01a(x) == pytest.approx(tuple(k), rel=1e-14)
+    for x, j in zip(g["x"], g["jy"]):
+        np.testing.assert_allclose(oracle.jy01b(x), j, rtol=1e-14, atol=1e-300)
+    for x, e in zip(g["x"], g["e1"]):
+        assert oracle.e1xa(x + 1e-3) == pytest.approx(e, rel=1e-14)
+    for x, e in zip(g["x"], g["ei"]):
+        assert oracle.eix(x + 1e-3) == pytest.approx(e, rel=1e-14)
+    for x, row in zip(g["x"], g["en"]):
+        for n in range(6):
+            assert oracle.enxa(n, x + 1e-3) == pytest.approx(row[n], rel=1e-13)
+    for x, e in zip(g["xe"], g["erf"]):
+        assert oracle.error(x) == pytest.approx(e, rel=1e-14, abs=1e-300)
+    for s, n, lim in zip(g["lim_s"], g["lim_n"], g["lim"]):
+        assert oracle.limes(s[: n + 1]) == pytest.approx(lim, rel=1e-14)
+
+
+def test_special_function_known_answers(oracle):
+    # tables in the source comments, FinGrnExtSubs.f90:52-59, :124-132, :187-195, :243-251
+    for x, v in [(1.0, .84270079), (2.0, .99532227), (3.0, .99997791), (4.0, .99999998), (5.0, 1.0)]:
+        assert oracle.error(x) == pytest.approx(v, abs=1e-8)
+    for n, v in enumerate([.45399930e-05, .41569689e-05, .38302405e-05, .35487626e-05, .33041014e-05,
+                           .30897289e-05]):
+        assert oracle.enxa(n, 10.0) == pytest.approx(v, rel=2e-5)   # recurrence on a 1e-7-class E1 fit
+    for x, v in [(1.0, .2193839), (2.0, .4890051e-01), (3.0, .1304838e-01), (4.0, .3779352e-02),
+                 (5.0, .1148296e-02)]:
+        assert oracle.e1xa(x) == pytest.approx(v, rel=1e-6)
+    for x, v in [(1.0, 1.8951178), (2.0, 4.9542344), (3.0, 9.9338326), (4.0, 19.630874), (5.0, 40.185275)]:
+        assert oracle.eix(x) == pytest.approx(v, rel=1e-7)
+
+
+def test_bessel_against_closed_forms(oracle):
+    # K0/K1 are the Zhang-Jin series/asymptotics: full FP64 against scipy
+    from scipy import special as sp
+    for x in (0.01, 0.3, 1.0, 5.0, 9.0, 12.0, 20.0, 40.0):
+        k0, k1 = oracle.ik01a(x)
+        assert k0 == pytest.approx(sp.k0(x), rel=1e-13)
+        assert k1 == pytest.approx(sp.k1(x), rel=1e-13)
+    # J/Y are 1e-8-class polynomial fits (JY01B), so only that accuracy is pinned here
+    for x in (0.5, 2.0, 3.9, 4.1, 10.0, 30.0):
+        j0, j1, y0, y1 = oracle.jy01b(x)
+        assert j0 == pytest.approx(sp.j0(x), abs=1e-8)
+        assert y1 == pytest.approx(sp.y1(x), abs=1e-8)
+
+
+def test_limes_alternating_series(oracle):
+    # partial sums of sum (-1)^k/(k+1) -> ln 2; Wynn epsilon reaches it to 1e-12
+    s = np.cumsum((-1.0) ** np.arange(30) / (1.0 + np.arange(30)))
+    assert oracle.limes(s) == pytest.approx(np.log(2.0), abs=1e-12)
+
+
+# ---- dispersion ----------------------------------------------------------------------
+def test_dispersion_golden_and_residual(oracle, golden):
+    g = golden("dispersion")
+    for a, w in enumerate(g["w"]):
+        for b, h in enumerate(g["h"]):
+            wvn = oracle.dispersion(w, h)
+            np.testing.assert_array_equal(wvn, g["wvn"][a, b])
+            v = w * w / 9.807
+            k = wvn[0]
+            assert k * np.tanh(k * h) == pytest.approx(v, rel=1e-9)
+            km = wvn[1:]
+            np.testing.assert_allclose(km * np.tan(km * h), -v, rtol=1e-7, atol=1e-9)
+
+
+# ---- Green functions -------------------------------------------------------------------
+def test_fingreen3d_golden(oracle, golden):
+    g = golden("green_finite")
+    worst = 0.0
+    for k in range(len(g["w"])):
+        w, h = g["w"][k], g["h"][k]
+        wvn = oracle.dispersion(w, h)
+        got = oracle.fingreen3d(g["src"][k], g["fld"][k], w * w / G, wvn, h, int(g["flag"][k]))
+        worst = max(worst, relerr(got, g["grn"][k]))
+    assert worst < 1e-12
+
+
+def test_infgreen3d_golden(oracle, golden):
+    g = golden("green_infinite")
+    worst = 0.0
+    for k in range(len(g["w"])):
+        got = oracle.infgreen3d(g["src"][k], g["fld"][k], g["w"][k] ** 2 / G, int(g["flag"][k]))
+        worst = max(worst, relerr(got, g["grn"][k]))
+    assert worst < 1e-12
+
+
+def test_fingreen3d_regime_continuity(oracle):
+    # the eigenfunction, epsilon-accelerated and Pidcock series agree where the
+    # regime changes at R/h = 0.05 and 0.5. (The reference is not continuous at
+    # its Linton boundary R/h = 5e-4 -- the compiled reference differs there by
+    # up to O(1) -- so that branch is pinned by the golden vectors instead.)
+    for h, w in ((50.0, 1.0), (50.0, 0.3), (70.0, 2.0)):
+        wvn = oracle.dispersion(w, h)
+        for rb in (0.05, 0.5):
+            lo = oracle.fingreen3d((rb * h * (1 - 1e-9), 0, -10.0), (0, 0, -12.0), w * w / G, wvn, h, 1)
+            hi = oracle.fingreen3d((rb * h * (1 + 1e-9), 0, -10.0), (0, 0, -12.0), w * w / G, wvn, h, 1)
+            assert relerr(lo, hi) < 1e-3
+
+
+def test_fingreen3d_reciprocity(oracle):
+    # G(P,Q) = G(Q,P) for the wave Green function (value), any regime
+    h, w = 40.0, 0.8
+    wvn = oracle.dispersion(w, h)
+    rng = np.random.default_rng(3)
+    for rr in (0.0, 0.1, 1.0, 5.0, 30.0):
+        a = np.array([0.0, 0.0, -rng.uniform(1, 30)])
+        b = np.array([rr, 0.0, -rng.uniform(1, 30)])
+        g1 = oracle.fingreen3d(a, b, w * w / G, wvn, h, 0)
+        g2 = oracle.fingreen3d(b, a, w * w / G, wvn, h, 0)
+        assert abs(g1[0] - g2[0]) / abs(g1[0]) < 1e-4   # the series stop at 1e-6 absolute
+
+
+# ---- Rankine panel integrals ---------------------------------------------------------
+def test_sglintbd_golden(oracle, golden):
+    g = golden("rankine")
+    for k in range(len(g["ncn"])):
+        s, d = oracle.sglintbd(g["verts"][k][: g["ncn"][k]], g["xf"][k])
+        assert s == pytest.approx(g["sij"][k], rel=1e-13, abs=1e-13)
+        np.testing.assert_allclose(d, g["dij"][k], rtol=1e-12, atol=1e-12)
+
+
+def _square(a):
+    return np.array([[0, 0, 0], [a, 0, 0], [a, a, 0], [0, a, 0]], dtype=float)
+
+
+def test_sglintbd_closed_form_and_limits(oracle):
+    a = 1.7
+    q = _square(a)
+    s, _ = oracle.sglintbd(q, q.mean(0))
+    assert abs(s) == pytest.approx(4 * a * np.arcsinh(1.0), rel=1e-12)
+    far = q.mean(0) + np.array([300.0, -200.0, 150.0])
+    s, _ = oracle.sglintbd(q, far)
+    assert abs(s) == pytest.approx(a * a / np.linalg.norm(far - q.mean(0)), rel=1e-5)
+
+
+def test_sglintbd_brute_force_and_gradient(oracle):
+    a = 1.3
+    q = _square(a)
+    x, w = np.polynomial.legendre.leggauss(64)
+    u = 0.5 * a * (x + 1)
+    wu = 0.5 * a * w
+    for xf in (np.array([0.4, 0.9, 0.35]), np.array([2.0, -0.5, -0.8]), np.array([0.65, 0.65, 3.0])):
+        r = np.sqrt((u[:, None] - xf[0]) ** 2 + (u[None, :] - xf[1]) ** 2 + xf[2] ** 2)
+        ref = np.sum(wu[:, None] * wu[None, :] / r)
+        s, d = oracle.sglintbd(q, xf)
+        assert abs(s) == pytest.approx(ref, rel=1e-9)
+        # gradient of the integral w.r.t. the field point, by central differences
+        eps = 1e-5
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = eps
+            sp_, _ = oracle.sglintbd(q, xf + e)
+            sm_, _ = oracle.sglintbd(q, xf - e)
+            fd = (sp_ - sm_) / (2 * eps)
+            assert abs(abs(d[k]) - abs(fd)) < 1e-6 * max(1.0, abs(fd))
+
+
+# ---- panel geometry -----------------------------------------------------------------------
+def test_panel_geometry_matches_numpy_restatement(oracle):
+    import meshes
+    b = meshes.cylinder(4.0, 3.0, 16, 3, 2)
+    mine = meshes.panel_geometry(b.xyz, b.ncn, b.ncon, b.xr)
+    ref = oracle.panel_geometry(b.xyz, b.ncn, b.ncon, b.xr)
+    for x, y in zip(mine, ref):
+        np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-14)
+    # wetted area of the cylinder converges to 2*pi*r*T + pi*r^2 from below
+    assert mine[1].sum() == pytest.approx(2 * np.pi * 4 * 3 + np.pi * 16, rel=0.05)
+
+
+# ---- LU -------------------------------------------------------------------------------------
+def test_textbook_lu_residual(oracle):
+    rng = np.random.default_rng(9)
+    n = 60
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    b = rng.normal(size=(n, 4)) + 1j * rng.normal(size=(n, 4))
+    x, info = oracle.zgetrf_solve(a, b)
+    assert info == 0
+    assert np.max(np.abs(a @ x - b)) / (np.max(np.abs(a)) * np.max(np.abs(x))) < 1e-13
+
+
+# ---- end to end on the reference's own test case -----------------------------------------------
+def _load_expected(kind, k):
+    return np.loadtxt(os.path.join(GOLDEN, "config1_expected", f"{kind}{k}.txt"))
+
+
+@pytest.mark.slow
+def test_config1_expected_outputs(oracle):
+    """test/data/config1 (3 bodies x 234 panels, h = 70 m): added mass, damping and
+    excitation of the oracle against the reference's expected_output files, which
+    print 5 significant digits (E14.5)."""
+    import meshes
+    p = meshes.reference_config1_from(os.path.join(GOLDEN, "config1_input"))
+    r = oracle.solve_frequency(p, 0.05)
+    assert r["info"] == 0
+    amas, bdmp, exfc = r["amas"], r["bdmp"], r["exfc"]
+    for k in range(6):
+        ea = _load_expected("OAMASS", k + 1)[:9]
+        eb = _load_expected("ODAMPING", k + 1)[:9]
+        q = 0
+        for m in range(3):
+            for n in range(3):
+                got_a = amas[6 * m + k, 6 * n:6 * n + 6]
+                got_b = bdmp[6 * m + k, 6 * n:6 * n + 6]
+                assert np.max(np.abs(got_a - ea[q, 2:])) <= 5e-5 * np.max(np.abs(amas)) + 1e-6
+                assert np.max(np.abs(got_b - eb[q, 2:])) <= 5e-5 * np.max(np.abs(bdmp)) + 1e-6
+                q += 1
+        ef = _load_expected("OEXFOR", k + 1)[:9]      # rows: heading-major, body-minor
+        got = np.array([exfc[6 * m + k, hh] for hh in range(3) for m in range(3)])
+        np.testing.assert_allclose(got.real, ef[:, 2], rtol=2e-5, atol=2e-5 * np.max(np.abs(exfc)))
+        np.testing.assert_allclose(got.imag, ef[:, 3], rtol=2e-5, atol=2e-5 * np.max(np.abs(exfc)))
+    # reciprocity of the radiation coefficients (to the discretisation error)
+    assert relerr(amas, amas.T) < 1e-2
+    assert relerr(bdmp, bdmp.T) < 1e-2
