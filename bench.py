@@ -1,0 +1,242 @@
+"""Benchmark: panel-frequency systems solved per second on one MI355X.
+
+One step = one panel-frequency system of the workload (assemble the influence
+matrix and right-hand sides, factor, solve the 6*NBODY radiation + NBETA
+diffraction problems, reduce to added mass / damping / excitation), cycling
+through the workload's wave frequencies. Inputs are resident in HBM when the
+timed region starts (geometry uploaded and pairs classified beforehand).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
+
+Prints one JSON line (see the project design notes for the field definitions).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hams-mrel_amd"))
+
+METRIC = "panel-freq systems solved/s, mixed+surrogate vs plain FP64 GPU at ≤1e-4 rel err"
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def _hip():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return ctypes.CDLL(name)
+        except OSError:
+            pass
+    raise RuntimeError("HIP runtime not found")
+
+
+def sync():
+    _hip().hipDeviceSynchronize()
+
+
+REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen": 3}
+
+
+def flop_model(name):
+    """algorithmic flops per pair by classification key (tests/golden/flop_model.json)"""
+    path = os.path.join(ROOT, "tests", "golden", "flop_model.json")
+    with open(path) as f:
+        t = json.load(f)[name]
+    return {int(k): v for k, v in t["flops_per_pair"].items()}, t["deep"]
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """one device-to-device copy of known size: the reference reading for the
+    FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)"""
+    hip = _hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes))
+    hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes))
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)      # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": nbytes}))
+
+
+def workload(name):
+    import meshes
+    if name == "c1":
+        return meshes.c1_single_cylinder()
+    if name == "c2":
+        return meshes.c2_three_cylinders()
+    if name == "c4":
+        return meshes.c4_eight_cylinders()
+    if name == "c5":
+        return meshes.c5_sixteen_cylinders()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_steps(ctx, omegas, k0, steps, precision):
+    out = []
+    for s in range(steps):
+        w = float(omegas[(k0 + s) % len(omegas)])
+        out.append(ctx.solve_frequency(w, precision))
+    return out
+
+
+def time_path(ctx, prob, precision, steps, warmup, stage_times=True):
+    import hm_bem
+    run_steps(ctx, prob.omegas, 0, warmup, precision)
+    sync()
+    ctx.set_timing(stage_times)
+    t0 = time.perf_counter()
+    res = run_steps(ctx, prob.omegas, warmup, steps, precision)
+    sync()
+    dt = time.perf_counter() - t0
+    stages = ctx.timing()
+    ctx.set_timing(False)
+    return dt, res, stages
+
+
+def cpu_baseline(prob, seconds=20.0, threads=16):
+    """The reference's own CPU kernels timed on the host: the body of its
+    CALGREEN table-fill loop (SGLINTBD + FINGREEN3D/INFGREEN3D, compiled from the
+    reference sources into oracle/_ref with OpenMP, CalGreenFunc.f90:58-115) over
+    a random sample of the workload's hull panel pairs on `threads` cores, plus
+    LAPACK ZGETRF/ZGETRS (numpy) at N <= 3000 scaled by N^3 to the workload's N.
+    Returns systems/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as om
+    if not om.Reference.available():
+        return {"value": None, "unit": "systems/s", "cores": threads, "kind": "reference",
+                "sample": "oracle/_ref/libhamsref.so not built"}
+    R = om.Reference()
+    L = R.lib
+    dp = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+    n, nt = prob.n, prob.n + prob.ni
+    xyz = np.ascontiguousarray(prob.xyz)
+    ncon = np.ascontiguousarray(prob.ncon, dtype=np.int32)
+    ncn = np.ascontiguousarray(prob.ncn, dtype=np.int32)
+    xp = np.ascontiguousarray(prob.xyz_p)
+    pn = np.ascontiguousarray(prob.pnsz)
+    L.ref_set_mesh(ctypes.c_int(n), ctypes.c_int(len(xyz)), dp(xyz), dp(ncn), dp(ncon), dp(xp), dp(pn))
+    w = float(prob.omegas[len(prob.omegas) // 2])
+    v = w * w / prob.g
+    wvn = R.dispersion(w, prob.h) if prob.h > 0 else np.zeros(200)
+    rng = np.random.default_rng(20261203)
+    csum = np.zeros(2)
+
+    def run(np_):
+        iel = np.ascontiguousarray(rng.integers(1, n + 1, np_), dtype=np.int32)
+        jel = np.ascontiguousarray(rng.integers(1, n + 1, np_), dtype=np.int32)
+        t0 = time.perf_counter()
+        L.ref_pairs_omp(ctypes.c_int(np_), dp(iel), dp(jel), ctypes.c_double(prob.h), ctypes.c_double(v), dp(wvn),
+                        ctypes.c_int(200), ctypes.c_int(threads), dp(csum))
+        return time.perf_counter() - t0
+
+    t = run(4000)
+    npairs = int(min(5e7, max(4000, 4000 * 0.7 * seconds / max(t, 1e-6))))
+    t = run(npairs)
+    pair_rate = npairs / t
+    m = min(n, 3000)
+    a = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
+    b = rng.normal(size=(m, prob.m + prob.nbeta)) + 1j * rng.normal(size=(m, prob.m + prob.nbeta))
+    t0 = time.perf_counter()
+    np.linalg.solve(a, b)
+    t_lu = (time.perf_counter() - t0) * (n / m) ** 3
+    t_sys = nt * n / pair_rate + t_lu
+    return {"value": 1.0 / t_sys, "unit": "systems/s", "cores": threads, "kind": "reference",
+            "sample": (f"{npairs} random hull panel pairs of the workload at omega={w:.3f} through the reference's "
+                       f"CALGREEN loop body (oracle/_ref, OpenMP, {threads} threads) in {t:.1f} s = {pair_rate:.3g} "
+                       f"pairs/s, x {nt * n} pairs per system; + LAPACK zgesv (numpy) at N={m} scaled by "
+                       f"(N/{m})^3 = {t_lu:.2f} s; {t_sys:.1f} s per system")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c5")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 path")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run the traffic-counter calibration copy")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return 0
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: frequencies shard across GPUs "
+                                   "without communication, but only --gpus 1 is supported here"}))
+        return 2
+    import hm_bem
+    prob = workload(args.workload)
+    print(f"# workload {prob.name}: n={prob.n} ni={prob.ni} modes={prob.m} headings={prob.nbeta} "
+          f"freqs={len(prob.omegas)}", file=sys.stderr, flush=True)
+    ctx = hm_bem.Context()
+    t0 = time.perf_counter()
+    ctx.setup(prob)
+    sync()
+    print(f"# setup (upload + pair classification) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+
+    # mixed precision path (FP32 LU on MFMA + FP64 refinement)
+    dt_mx, res_mx, st_mx = time_path(ctx, prob, hm_bem.HM_FP32_IR, args.steps, args.warmup)
+    print(f"# fp32+IR: {args.steps} systems in {dt_mx:.3f} s; stages {st_mx}", file=sys.stderr, flush=True)
+    plain = None
+    if not args.no_plain:
+        dt_pl, res_pl, st_pl = time_path(ctx, prob, hm_bem.HM_FP64, args.steps, args.warmup)
+        print(f"# fp64: {args.steps} systems in {dt_pl:.3f} s; stages {st_pl}", file=sys.stderr, flush=True)
+        from_err = []
+        for a, b in zip(res_mx, res_pl):
+            for k in ("amas", "bdmp", "exfc"):
+                den = np.maximum(np.abs(b[k]), 1e-6 * np.max(np.abs(b[k])))
+                from_err.append(float(np.max(np.abs(a[k] - b[k]) / den)))
+        plain = {"value": args.steps / dt_pl, "ms_per_step": 1e3 * dt_pl / args.steps,
+                 "max_rel_err_mixed_vs_fp64": max(from_err)}
+
+    # dominant kernel: the assembly launch (per regime for finite depth) with most time
+    fpp, deep = flop_model(args.workload)
+    hist = ctx.pair_histogram().ravel()
+    asm = {k: v for k, v in st_mx.items() if k.startswith("asm_") and v[1] > 0}
+    if asm:
+        dom = max(asm, key=lambda k: asm[k][0])
+        dom_ms = asm[dom][0] / asm[dom][1]
+        r = REGIME_STAGE[dom]
+        keys = range(128 * r, 128 * (r + 1))
+        flops = sum(float(hist[k]) * fpp.get(k, fpp.get(128 * r, 0.0)) for k in keys)
+        units = int(sum(hist[k] for k in keys))
+    else:
+        dom, dom_ms = "asm_deep", st_mx["assembly"][0] / max(st_mx["assembly"][1], 1)
+        units = (prob.n + prob.ni) * prob.n
+        flops = units * fpp[0]
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "avg_ms": dom_ms, "pairs_per_launch": units, "algorithmic_flops_per_launch": flops,
+                "note": ("FP64 VALU-bound Green-function assembly (one pair per lane); peak = MI355X FP64 "
+                         "vector peak (= FP64 matrix peak); flops per pair from tests/golden/flop_model.json")}
+
+    line = {"metric": METRIC, "value": args.steps / dt_mx, "unit": "systems/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt_mx / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+            "data": "synthetic (deterministic meshes of the named workload)",
+            "config": {"workload": prob.name, "panels": prob.n, "waterplane_panels": prob.ni,
+                       "modes": prob.m, "headings": prob.nbeta, "frequencies": len(prob.omegas),
+                       "path": "FP64 assembly + FP32 LU (MFMA) + FP64 iterative refinement",
+                       "parallelism": "1 GPU"},
+            "stages_ms": {k: v[0] for k, v in st_mx.items()},
+            "plain_fp64": plain, "roofline": roofline}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(prob, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -45,8 +45,9 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr int kStages = 7;
-enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6 };
+constexpr int kStages = 11;
+enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6,
+             kAsmRegime0 = 7 /* + regime: Linton, Pidcock, eps-accelerated, eigenfunction */ };
 
 }  // namespace
 
@@ -64,6 +65,7 @@ struct hm_ctx {
   double h = 0.0;
   bool have_depth = false;
   DevBuf d_pairs, d_hist, d_cursor;
+  std::vector<long long> hist_host;
   long long seg_off[4] = {0, 0, 0, 0}, seg_cnt[4] = {0, 0, 0, 0};
   // headings
   std::vector<double> beta;
@@ -78,11 +80,16 @@ struct hm_ctx {
   DevBuf d_amas, d_bdmp, d_exfc;
   bool assembled = false, factored = false, solved = false;
   int precision = HM_FP64;
-  // timing
+  // timing: event pairs recorded on the stream, resolved by hm_get_timing
   bool timing = false;
   double stage_ms[kStages] = {0};
   int stage_cnt[kStages] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  struct Rec { int stage; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  int open_stage[kStages];
+  hipEvent_t open_ev[kStages];
 
   int nt() const { return n + ni; }
   int m() const { return 6 * nbody; }
@@ -120,18 +127,37 @@ struct hm_ctx {
     (void)hipGetLastError();   // do not let this failure resurface at a later launch check
     return fail(HM_ERR_HIP - (int)e, "%s: %s", what, hipGetErrorString(e));
   }
-  void stage_begin() {
-    if (timing) (void)hipEventRecord(ev0, stream);
+  hipEvent_t take_event() {
+    hipEvent_t e = nullptr;
+    if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+    else (void)hipEventCreate(&e);
+    return e;
+  }
+  void stage_begin(int s) {
+    if (!timing) return;
+    open_ev[s] = take_event();
+    (void)hipEventRecord(open_ev[s], stream);
   }
   int stage_end(int s) {
     if (!timing) return HM_OK;
-    (void)hipEventRecord(ev1, stream);
-    hipError_t e = hipEventSynchronize(ev1);
+    hipEvent_t e = take_event();
+    (void)hipEventRecord(e, stream);
+    recs.push_back({s, open_ev[s], e});
+    return HM_OK;
+  }
+  int resolve_timing() {
+    if (recs.empty()) return HM_OK;
+    hipError_t e = hipEventSynchronize(recs.back().b);
     if (e != hipSuccess) return hip(e, "stage timing");
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ev0, ev1);
-    stage_ms[s] += ms;
-    stage_cnt[s] += 1;
+    for (const Rec& r : recs) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, r.a, r.b);
+      stage_ms[r.stage] += ms;
+      stage_cnt[r.stage] += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    recs.clear();
     return HM_OK;
   }
 };
@@ -215,6 +241,8 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc};
   for (DevBuf* b : bufs) b->release();
+  (void)c->resolve_timing();
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -292,11 +320,12 @@ extern "C" int hm_set_depth(hm_ctx* c, double h) {
   c->have_freq = false;
   c->assembled = c->factored = c->solved = false;
   for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
+  c->hist_host.assign(hm::kNumKeys, 0);
   if (h < 0.0) return HM_OK;
   // classify the pairs by regime and series length (frequency independent)
   const hm::GeomDev gd = c->geom();
   const long long np = (long long)c->nt() * c->n;
-  c->stage_begin();
+  c->stage_begin(kClassify);
   HIPCHK(c, c->d_hist.ensure(sizeof(unsigned) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_cursor.ensure(sizeof(unsigned long long) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_pairs.ensure(sizeof(uint32_t) * (size_t)np), "classify");
@@ -305,6 +334,7 @@ extern "C" int hm_set_depth(hm_ctx* c, double h) {
   std::vector<unsigned> hist(hm::kNumKeys);
   HIPCHK(c, hipMemcpyAsync(hist.data(), c->d_hist.p, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, c->stream), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
+  c->hist_host.assign(hist.begin(), hist.end());
   std::vector<unsigned long long> cur(hm::kNumKeys);
   unsigned long long acc = 0;
   for (int k = 0; k < hm::kNumKeys; ++k) {
@@ -372,26 +402,30 @@ extern "C" int hm_assemble(hm_ctx* c) {
   const hm::FreqDev f = c->freq();
   double* A = c->d_A.as<double>();
   double* S = c->d_S.as<double>();
-  c->stage_begin();
+  c->stage_begin(kAssembly);
   const bool limit = fabs(c->v) < F32(1.e-8) || fabs(c->v + 1.0) < F32(1.e-8);
   if (c->h < 0.0 || limit) {
     HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, nt), "assemble");
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) {
+      if (c->seg_cnt[r] == 0) continue;
+      c->stage_begin(kAsmRegime0 + r);
       HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt), "assemble");
+      c->stage_end(kAsmRegime0 + r);
+    }
   }
   int rc = c->stage_end(kAssembly);
   if (rc) return rc;
   // right-hand sides RHS = S * W
-  c->stage_begin();
+  c->stage_begin(kRhs);
   HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
   HIPCHK(c, hm::cgemm<double>(c->stream, false, nt, ncol, n, 1.0, 0.0, S, nt, 0, c->d_W.as<double>(), n, 0, 0.0, 0.0,
                               c->d_R.as<double>(), nt, 0, 1), "rhs gemm");
   if ((rc = c->stage_end(kRhs))) return rc;
   if (c->ni > 0) {
     // least squares through the normal equations with plain transpose
-    c->stage_begin();
+    c->stage_begin(kIrr);
     HIPCHK(c, c->d_C.ensure(cplx * (size_t)n * n), "irr alloc");
     HIPCHK(c, c->d_D.ensure(cplx * (size_t)n * ncol), "irr alloc");
     HIPCHK(c, hm::cgemm<double>(c->stream, true, n, n, nt, 1.0, 0.0, A, nt, 0, A, nt, 0, 0.0, 0.0, c->d_C.as<double>(), n, 0, 1), "irr A^T A");
@@ -418,7 +452,7 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
   HIPCHK(c, c->d_counter.ensure(sizeof(unsigned)), "factor alloc");
   HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream), "factor");
   HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned), c->stream), "factor");
-  c->stage_begin();
+  c->stage_begin(kLu);
   if (precision == HM_FP64) {
     HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
                                     c->d_counter.as<unsigned>(), 1), "lu fp64");
@@ -474,7 +508,7 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
   const double* B = sys_rhs(c);
   const int ldb = sys_ld_rhs(c);
   int it = 0;
-  c->stage_begin();
+  c->stage_begin(kSolve);
   HIPCHK(c, hipMemcpy2DAsync(X, cplx * n, B, cplx * ldb, cplx * n, ncol, hipMemcpyDeviceToDevice, c->stream), "solve");
   if (c->precision == HM_FP64) {
     HIPCHK(c, hm::lu_solve<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), X, n, 0, ncol, 1), "getrs fp64");
@@ -522,7 +556,7 @@ extern "C" int hm_forces(hm_ctx* c, double* amas, double* bdmp, double* exfc) {
   HIPCHK(c, c->d_amas.ensure(sizeof(double) * m * m), "forces alloc");
   HIPCHK(c, c->d_bdmp.ensure(sizeof(double) * m * m), "forces alloc");
   HIPCHK(c, c->d_exfc.ensure(2 * sizeof(double) * m * (nbeta ? nbeta : 1)), "forces alloc");
-  c->stage_begin();
+  c->stage_begin(kForces);
   HIPCHK(c, hm::forces(c->stream, c->geom(), c->freq(), c->d_beta.as<double>(), nbeta, c->xw, c->yw, c->d_X.as<double>(),
                        c->n, c->d_amas.as<double>(), c->d_bdmp.as<double>(), c->d_exfc.as<double>()), "forces");
   if (amas) HIPCHK(c, hipMemcpyAsync(amas, c->d_amas.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, c->stream), "forces");
@@ -556,17 +590,29 @@ extern "C" int hm_run_frequencies(hm_ctx* c, int nfreq, const double* w1, int pr
 
 extern "C" int hm_set_timing(hm_ctx* c, int enable) {
   if (!c) return HM_ERR_ARG - 1;
+  int rc = c->resolve_timing();
   c->timing = enable != 0;
   for (int s = 0; s < kStages; ++s) { c->stage_ms[s] = 0.0; c->stage_cnt[s] = 0; }
-  return HM_OK;
+  return rc;
 }
 
 extern "C" int hm_get_timing(hm_ctx* c, double* ms, int* counts, int nstage) {
   if (!c) return HM_ERR_ARG - 1;
+  int rc = c->resolve_timing();
+  if (rc) return rc;
   for (int s = 0; s < nstage && s < kStages; ++s) {
     if (ms) ms[s] = c->stage_ms[s];
     if (counts) counts[s] = c->stage_cnt[s];
   }
+  return HM_OK;
+}
+
+extern "C" int hm_get_pair_histogram(hm_ctx* c, long long* counts, int nkeys) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_depth) return c->fail(HM_ERR_STATE, "hm_get_pair_histogram before hm_set_depth");
+  if (!counts) return c->fail(HM_ERR_ARG - 2, "counts is NULL");
+  if (nkeys < hm::kNumKeys) return c->fail(HM_ERR_ARG - 3, "nkeys must be >= %d", hm::kNumKeys);
+  for (int k = 0; k < nkeys; ++k) counts[k] = k < (int)c->hist_host.size() ? c->hist_host[k] : 0;
   return HM_OK;
 }
 

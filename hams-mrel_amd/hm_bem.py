@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libhm_bem.so")
 
 HM_FP64 = 0
 HM_FP32_IR = 1
-STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces")
+STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces",
+          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -25,7 +26,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_depth", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
-           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs")
+           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
+           "hm_get_pair_histogram")
 
 _lib = None
 
@@ -170,6 +172,13 @@ class Context:
         x = np.zeros((p.n, p.m + p.nbeta), dtype=np.complex128, order="F")
         self._chk(self.L.hm_get_potentials(self.ptr, _p(x)), "hm_get_potentials")
         return x
+
+    def pair_histogram(self):
+        """(4, 128) pair counts by regime (Linton, Pidcock, eps-accelerated,
+        eigenfunction) and series length"""
+        h = np.zeros(512, dtype=np.int64)
+        self._chk(self.L.hm_get_pair_histogram(self.ptr, _p(h), _i(512)), "hm_get_pair_histogram")
+        return h.reshape(4, 128)
 
     def set_timing(self, on=True):
         self._chk(self.L.hm_set_timing(self.ptr, _i(1 if on else 0)), "hm_set_timing")

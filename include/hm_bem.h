@@ -81,12 +81,21 @@ int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, doub
 int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, double *amas, double *bdmp,
                        double *exfc);
 
-/* Stage timing (HIP events on the context's stream): when enabled, each stage's
- * elapsed milliseconds accumulate into ms[stage] for stages
- * 0 classify, 1 assembly (Green+Rankine), 2 RHS GEMM, 3 IRR normal equations,
- * 4 LU, 5 triangular solves + refinement, 6 forces; counts[stage] = launches. */
+/* Stage timing: when enabled, a pair of HIP events is recorded on the context's
+ * stream around each stage (no extra synchronisation); hm_get_timing waits for
+ * the last one and accumulates elapsed milliseconds into ms[stage], calls into
+ * counts[stage], for stages 0 classify, 1 assembly (all regimes), 2 RHS GEMM,
+ * 3 IRR normal equations, 4 LU, 5 triangular solves + refinement, 6 forces,
+ * 7..10 the assembly kernel of each finite-depth regime (Linton, Pidcock,
+ * epsilon-accelerated, eigenfunction). hm_set_timing resets the totals. */
 int hm_set_timing(hm_ctx *ctx, int enable);
 int hm_get_timing(hm_ctx *ctx, double *ms, int *counts, int nstage);
+
+/* Pair classification of the current geometry and depth: counts[regime*128 + len]
+ * = number of (field point, source panel) pairs in regime 0 Linton, 1 Pidcock,
+ * 2 epsilon-accelerated, 3 eigenfunction with series length len (0 when the
+ * regime has no fixed length); nkeys >= 512. All zero for infinite depth. */
+int hm_get_pair_histogram(hm_ctx *ctx, long long *counts, int nkeys);
 
 /* Host copies of the assembled system (after hm_assemble, before hm_factor):
  * amat, smat: (n+ni) x n complex; rhs: (n+ni) x (6nb+nbeta) complex. Any may be

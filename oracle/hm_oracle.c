@@ -1265,7 +1265,9 @@ static void sglint_tri(int ncn, const double *xq, const double xf[3], double *si
         ksi[3] = ksi[0];
         eta[3] = eta[0];
         double st = 0.0, sxt = 0.0, syt = 0.0, szt = 0.0;
+        FL(70); TR(0);
         for (int i = 0; i < 3; i++) {
+            FL(40); TR(4);
             double dksi = ksi[i + 1] - ksi[i], deta = eta[i + 1] - eta[i];
             double r1 = sqrt((ksi[i] - x) * (ksi[i] - x) + (eta[i] - y) * (eta[i] - y) + z * z);
             double r2 = sqrt((ksi[i + 1] - x) * (ksi[i + 1] - x) + (eta[i + 1] - y) * (eta[i + 1] - y) + z * z);
@@ -1323,7 +1325,9 @@ static void sglint_quad(const double *xq, const double xf[3], double *sij, doubl
     ksi[4] = ksi[0];
     eta[4] = eta[0];
     double st = 0.0, sxt = 0.0, syt = 0.0, szt = 0.0;
+    FL(110);
     for (int i = 0; i < 4; i++) {
+        FL(40); TR(4);
         double dksi = ksi[i + 1] - ksi[i], deta = eta[i + 1] - eta[i];
         double r1 = sqrt((ksi[i] - x) * (ksi[i] - x) + (eta[i] - y) * (eta[i] - y) + z * z);
         double r2 = sqrt((ksi[i + 1] - x) * (ksi[i + 1] - x) + (eta[i + 1] - y) * (eta[i + 1] - y) + z * z);

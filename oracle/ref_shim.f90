@@ -14,6 +14,11 @@
 !   ref_sglintbd     -> SGLINTBD_QUAD/TRI(src/SingularIntgr.f90:252/:44)
 !   ref_ik01a, ref_jy01b, ref_error, ref_e1xa, ref_enxa, ref_eix, ref_limes
 !                    -> FinGrnExtSubs.f90 routines of the same names
+!   ref_set_mesh + ref_pairs_omp
+!                    -> the body of the CALGREEN table-fill loop
+!                       (CalGreenFunc.f90:58-115) over a list of (field panel,
+!                       source panel) pairs, OpenMP-parallel like the reference;
+!                       used to time the reference's CPU kernels
 
 module ref_shim
    use iso_c_binding
@@ -139,6 +144,73 @@ contains
       integer :: ip, kp
       real(8) :: err
       call Limes(int(n), s, rlim, ip, kp, err)
+   end subroutine
+
+   ! mesh in the reference's module arrays (PanelMesh_mod), as ReadPanelMesh sets them
+   subroutine ref_set_mesh(nelem_in, ntnd_in, xyz_in, ncn_in, ncon_in, xyz_p_in, pnsz_in) bind(C, name="ref_set_mesh")
+      integer(c_int), value :: nelem_in, ntnd_in
+      real(c_double), intent(in) :: xyz_in(3, ntnd_in), xyz_p_in(3, nelem_in), pnsz_in(nelem_in)
+      integer(c_int), intent(in) :: ncn_in(nelem_in), ncon_in(4, nelem_in)
+      integer :: k
+      if (allocated(XYZ)) deallocate(XYZ)
+      if (allocated(NCON)) deallocate(NCON)
+      if (allocated(NCN)) deallocate(NCN)
+      if (allocated(XYZ_P)) deallocate(XYZ_P)
+      if (allocated(PNSZ)) deallocate(PNSZ)
+      allocate(XYZ(ntnd_in, 3), NCON(nelem_in, 4), NCN(nelem_in), XYZ_P(nelem_in, 3), PNSZ(nelem_in))
+      do k = 1, 3
+         XYZ(:, k) = xyz_in(k, :)
+         XYZ_P(:, k) = xyz_p_in(k, :)
+      end do
+      do k = 1, 4
+         NCON(:, k) = ncon_in(k, :)
+      end do
+      NCN = ncn_in
+      PNSZ = pnsz_in
+      NELEM = nelem_in
+      NTND = ntnd_in
+      ISX = 0
+      ISY = 0
+      NSYS = 1
+   end subroutine
+
+   ! the CALGREEN loop body for np listed pairs (1-based panel numbers); returns
+   ! the sum of all GRN(1) values so nothing is optimised away
+   subroutine ref_pairs_omp(np, iel, jel, h, v, wvn, nk, nthreads, csum) bind(C, name="ref_pairs_omp")
+      integer(c_int), value :: np, nk, nthreads
+      integer(c_int), intent(in) :: iel(np), jel(np)
+      real(c_double), value :: h, v
+      real(c_double), intent(in) :: wvn(nk)
+      complex(c_double_complex), intent(out) :: csum
+      integer :: k, i, j, flag
+      real(8) :: dist, sij, dij(3)
+      complex(8) :: grn(4), acc
+      acc = (0.0d0, 0.0d0)
+!$OMP PARALLEL DO NUM_THREADS(nthreads) SCHEDULE(DYNAMIC,16) PRIVATE(k,i,j,flag,dist,sij,dij,grn) REDUCTION(+:acc)
+      do k = 1, np
+         i = iel(k)
+         j = jel(k)
+         dist = sqrt((XYZ_P(i, 1) - XYZ_P(j, 1))**2 + (XYZ_P(i, 2) - XYZ_P(j, 2))**2 + (XYZ_P(i, 3) - XYZ_P(j, 3))**2)
+         if (dist .le. 50.d0 * PNSZ(j)) then
+            flag = 1
+         else
+            flag = 0
+         end if
+         if (NCN(j) .eq. 3) then
+            call SGLINTBD_TRI(1, i, j, sij, dij, 1)
+         else
+            call SGLINTBD_QUAD(1, i, j, sij, dij, 1)
+         end if
+         if (h .lt. 0.d0) then
+            call INFGREEN3D(XYZ_P(j, 1), XYZ_P(i, 1), XYZ_P(j, 2), XYZ_P(i, 2), XYZ_P(j, 3), XYZ_P(i, 3), v, grn, flag)
+         else
+            call FINGREEN3D(XYZ_P(j, 1), XYZ_P(i, 1), XYZ_P(j, 2), XYZ_P(i, 2), XYZ_P(j, 3), XYZ_P(i, 3), v, wvn, &
+                            int(nk), h, grn, flag)
+         end if
+         acc = acc + grn(1) + sij
+      end do
+!$OMP END PARALLEL DO
+      csum = acc
    end subroutine
 
 end module ref_shim
