@@ -166,6 +166,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the traffic-counter calibration copy")
+    ap.add_argument("--assemble-only", action="store_true",
+                    help="only assemble --steps systems (for hardware-counter passes over the assembly kernels)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -183,6 +185,14 @@ def main():
     ctx.setup(prob)
     sync()
     print(f"# setup (upload + pair classification) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    if args.assemble_only:
+        for s in range(args.steps):
+            ctx.set_frequency(float(prob.omegas[s % len(prob.omegas)]))
+            ctx.assemble()
+        sync()
+        print(json.dumps({"assembled_systems": args.steps, "workload": prob.name}))
+        ctx.close()
+        return 0
 
     # mixed precision path (FP32 LU on MFMA + FP64 refinement)
     dt_mx, res_mx, st_mx = time_path(ctx, prob, hm_bem.HM_FP32_IR, args.steps, args.warmup)
