@@ -8,7 +8,9 @@ timed region starts (geometry uploaded and pairs classified beforehand).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5]
 
-Prints one JSON line (see the project design notes for the field definitions).
+Prints one JSON line: metric, value (systems/s of the mixed-precision path),
+unit, the plain FP64 path's rate beside it, per-stage milliseconds, the
+roofline of the dominant assembly kernel and the CPU baseline.
 """
 import argparse
 import ctypes

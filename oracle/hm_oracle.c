@@ -10,7 +10,7 @@
  * term counts.
  *
  * Parity status of each function (how it is pinned) is listed in the header
- * comment of tests/test_oracle.py and in the project design notes.
+ * comment of tests/test_oracle.py.
  */
 #include "hm_oracle.h"
 
