@@ -18,6 +18,7 @@
 #include "hm_assembly.hpp"
 #include "hm_bem.h"
 #include "hm_device.hpp"
+#include "hm_gmres.hpp"
 #include "hm_linalg.hpp"
 
 namespace {
@@ -80,6 +81,13 @@ struct hm_ctx {
   DevBuf d_amas, d_bdmp, d_exfc;
   bool assembled = false, factored = false, solved = false;
   int precision = HM_FP64;
+  // GMRES (ZITER_SOLVE_MULTI parameters, IterativeSolvers.f90:56-58; the
+  // block-diagonal preconditioner of SETUP_BLOCK_PRECONDITIONER :190)
+  int gm_restart = 50, gm_maxiter = 500, gm_precond = -1;
+  double gm_tol = 1e-6;
+  DevBuf d_P32, d_pipiv, d_gwork;
+  std::vector<int> gm_blocks;
+  hm::GmresStats gm_stats;
   // timing: event pairs recorded on the stream, resolved by hm_get_timing
   bool timing = false;
   double stage_ms[kStages] = {0};
@@ -239,7 +247,8 @@ extern "C" int hm_destroy(hm_ctx* c) {
   DevBuf* bufs[] = {&c->d_body, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
-                    &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc};
+                    &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -445,26 +454,54 @@ static int sys_ld_rhs(hm_ctx* c) { return c->ni > 0 ? c->n : c->nt(); }
 extern "C" int hm_factor(hm_ctx* c, int precision) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->assembled) return c->fail(HM_ERR_STATE, "hm_factor before hm_assemble");
-  if (precision != HM_FP64 && precision != HM_FP32_IR) return c->fail(HM_ERR_ARG - 2, "unknown precision %d", precision);
+  if (precision != HM_FP64 && precision != HM_FP32_IR && precision != HM_GMRES)
+    return c->fail(HM_ERR_ARG - 2, "unknown precision %d", precision);
   const int n = c->n;
   HIPCHK(c, c->d_ipiv.ensure(sizeof(int) * (size_t)n), "factor alloc");
-  HIPCHK(c, c->d_info.ensure(sizeof(int)), "factor alloc");
-  HIPCHK(c, c->d_counter.ensure(sizeof(unsigned)), "factor alloc");
-  HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int), c->stream), "factor");
-  HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned), c->stream), "factor");
+  // one info word and one counter per matrix of a batched factorisation (the
+  // block preconditioner factors the bodies' blocks as a batch)
+  const size_t nmat = (size_t)(c->nbody > 1 ? c->nbody : 1);
+  HIPCHK(c, c->d_info.ensure(sizeof(int) * nmat), "factor alloc");
+  HIPCHK(c, c->d_counter.ensure(sizeof(unsigned) * nmat), "factor alloc");
+  HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nmat, c->stream), "factor");
+  HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nmat, c->stream), "factor");
   c->stage_begin(kLu);
   if (precision == HM_FP64) {
     HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
                                     c->d_counter.as<unsigned>(), 1), "lu fp64");
-  } else {
+  } else if (precision == HM_FP32_IR) {
     HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
     HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
     HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
                                    c->d_counter.as<unsigned>(), 1), "lu fp32");
+  } else {
+    // GMRES: FP32 copy for the Krylov products; block-diagonal preconditioner
+    // (one block per body) by default for several bodies, none for one body
+    HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
+    HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    const bool blocks = c->gm_precond < 0 ? c->nbody > 1 : c->gm_precond == 1;
+    c->gm_blocks.clear();
+    if (blocks) {
+      c->gm_blocks = c->body_start;
+      size_t tot = 0;
+      for (int b = 0; b < c->nbody; ++b) {
+        const size_t nb = (size_t)(c->body_start[b + 1] - c->body_start[b]);
+        tot += nb * nb;
+      }
+      HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * tot), "factor alloc");
+      HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)n), "factor alloc");
+      HIPCHK(c, hm::gmres_block_precond_setup(c->stream, sys_matrix(c), n, c->nbody, c->gm_blocks.data(),
+                                              c->d_P32.as<float>(), c->d_pipiv.as<int>(), c->d_info.as<int>(),
+                                              c->d_counter.as<unsigned>()), "block preconditioner");
+    }
   }
-  int info = 0;
-  HIPCHK(c, hipMemcpyAsync(&info, c->d_info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "factor");
+  std::vector<int> infos(nmat, 0);
+  HIPCHK(c, hipMemcpyAsync(infos.data(), c->d_info.p, sizeof(int) * nmat, hipMemcpyDeviceToHost, c->stream), "factor");
   HIPCHK(c, hipStreamSynchronize(c->stream), "factor");
+  int info = infos[0];
+  if (precision == HM_GMRES && !c->gm_blocks.empty())
+    for (size_t b = 0; b < nmat && !info; ++b)
+      if (infos[b] > 0) info = c->body_start[b] + infos[b];   // zero pivot of a preconditioner block
   int rc = c->stage_end(kLu);
   if (rc) return rc;
   c->precision = precision;
@@ -512,6 +549,29 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
   HIPCHK(c, hipMemcpy2DAsync(X, cplx * n, B, cplx * ldb, cplx * n, ncol, hipMemcpyDeviceToDevice, c->stream), "solve");
   if (c->precision == HM_FP64) {
     HIPCHK(c, hm::lu_solve<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), X, n, 0, ncol, 1), "getrs fp64");
+  } else if (c->precision == HM_GMRES) {
+    hm::GmresArgs ga;
+    ga.n = n;
+    ga.ncol = ncol;
+    ga.restart = c->gm_restart;
+    ga.maxiter = c->gm_maxiter;
+    ga.tol = c->gm_tol;
+    ga.inner_tol = 0.5 * c->gm_tol;
+    ga.A64 = sys_matrix(c);
+    ga.lda64 = n;
+    ga.A32 = c->d_A32.as<float>();
+    ga.B = B;
+    ga.ldb = ldb;
+    ga.X = X;
+    if (!c->gm_blocks.empty()) {
+      ga.nblk = (int)c->gm_blocks.size() - 1;
+      ga.blk_start = c->gm_blocks.data();
+      ga.P = c->d_P32.as<float>();
+      ga.pipiv = c->d_pipiv.as<int>();
+    }
+    HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, ncol, c->gm_restart)), "gmres alloc");
+    HIPCHK(c, hm::gmres_solve(c->stream, ga, c->d_gwork.p, &c->gm_stats), "gmres");
+    it = c->gm_stats.iterations;
   } else {
     // FP32 solves of the FP64 residual; stop when the last correction is below
     // 1e-13 of the solution in every column
@@ -546,6 +606,30 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
   if (rc) return rc;
   if (iters) *iters = it;
   c->solved = true;
+  return HM_OK;
+}
+
+extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int precond) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (restart < 1 || restart > hm::kGmresMaxRestart)
+    return c->fail(HM_ERR_ARG - 2, "restart must be in 1..%d", hm::kGmresMaxRestart);
+  if (!(tol > 0.0 && tol < 1.0)) return c->fail(HM_ERR_ARG - 3, "tol must be in (0, 1)");
+  if (maxiter < 1) return c->fail(HM_ERR_ARG - 4, "maxiter must be >= 1");
+  if (precond < -1 || precond > 1) return c->fail(HM_ERR_ARG - 5, "precond must be -1, 0 or 1");
+  c->gm_restart = restart;
+  c->gm_tol = tol;
+  c->gm_maxiter = maxiter;
+  c->gm_precond = precond;
+  c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_get_gmres_stats(hm_ctx* c, int* iters, int* cycles, int* converged, double* rel_residual) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (iters) *iters = c->gm_stats.iterations;
+  if (cycles) *cycles = c->gm_stats.cycles;
+  if (converged) *converged = c->gm_stats.converged;
+  if (rel_residual) *rel_residual = c->gm_stats.rel_residual;
   return HM_OK;
 }
 

@@ -1,0 +1,39 @@
+// hm_gmres.hpp -- restarted GMRES with FP32 MFMA products and FP64 residuals.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace hm {
+
+constexpr int kGmresMaxRestart = 128;
+
+struct GmresArgs {
+  int n = 0, ncol = 0;
+  int restart = 50, maxiter = 500, max_cycles = 50;
+  double tol = 1e-6;        // on the FP64 residual: ||B - A X|| <= tol ||B|| per column
+  double inner_tol = 5e-7;  // on the Arnoldi residual estimate of a cycle
+  const double* A64 = nullptr;  // FP64 system matrix (n x n, leading dimension lda64)
+  int lda64 = 0;
+  const float* A32 = nullptr;   // FP32 copy, leading dimension n
+  const double* B = nullptr;    // right-hand sides, leading dimension ldb
+  int ldb = 0;
+  double* X = nullptr;          // solution (n x ncol, leading dimension n)
+  // block-diagonal preconditioner: nblk blocks [blk_start[b], blk_start[b+1]) (host array)
+  int nblk = 0;
+  const int* blk_start = nullptr;
+  const float* P = nullptr;     // LU factors of the blocks, packed one after the other
+  const int* pipiv = nullptr;   // block-local pivots at the block's row offset
+};
+
+struct GmresStats {
+  int iterations = 0, cycles = 0, converged = 0;
+  double rel_residual = 0.0;
+};
+
+size_t gmres_workspace_bytes(int n, int ncol, int restart);
+// FP32 copies of the diagonal blocks of A, LU-factored in place
+hipError_t gmres_block_precond_setup(hipStream_t s, const double* A, int lda, int nblk, const int* blk_start,
+                                     float* P, int* pipiv, int* info, unsigned* counter);
+hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats* stats);
+
+}  // namespace hm

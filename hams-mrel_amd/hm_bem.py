@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libhm_bem.so")
 
 HM_FP64 = 0
 HM_FP32_IR = 1
+HM_GMRES = 2
 STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces",
           "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
 
@@ -27,7 +28,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
            "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
-           "hm_get_pair_histogram")
+           "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats")
 
 _lib = None
 
@@ -179,6 +180,16 @@ class Context:
         h = np.zeros(512, dtype=np.int64)
         self._chk(self.L.hm_get_pair_histogram(self.ptr, _p(h), _i(512)), "hm_get_pair_histogram")
         return h.reshape(4, 128)
+
+    def set_gmres(self, restart=50, tol=1e-6, maxiter=500, precond=-1):
+        self._chk(self.L.hm_set_gmres(self.ptr, _i(restart), _d(tol), _i(maxiter), _i(precond)), "hm_set_gmres")
+
+    def gmres_stats(self):
+        it, cy, cv = _i(), _i(), _i()
+        rr = _d()
+        self._chk(self.L.hm_get_gmres_stats(self.ptr, ctypes.byref(it), ctypes.byref(cy), ctypes.byref(cv),
+                                            ctypes.byref(rr)), "hm_get_gmres_stats")
+        return dict(iterations=it.value, cycles=cy.value, converged=bool(cv.value), rel_residual=rr.value)
 
     def set_timing(self, on=True):
         self._chk(self.L.hm_set_timing(self.ptr, _i(1 if on else 0)), "hm_set_timing")

@@ -30,6 +30,7 @@ extern "C" {
 /* precision of the dense solve */
 #define HM_FP64 0      /* FP64 LU + FP64 triangular solves (plain path)          */
 #define HM_FP32_IR 1   /* FP32 LU on MFMA, FP64 residuals, iterative refinement  */
+#define HM_GMRES 2     /* restarted GMRES: FP32 MFMA products, FP64 residuals     */
 
 typedef struct hm_ctx hm_ctx;
 
@@ -75,6 +76,18 @@ int hm_assemble(hm_ctx *ctx);
 int hm_factor(hm_ctx *ctx, int precision);
 int hm_solve(hm_ctx *ctx, int *iters);
 int hm_forces(hm_ctx *ctx, double *amas, double *bdmp, double *exfc);
+/* Restarted GMRES (precision HM_GMRES), replacing ZITER_SOLVE_MULTI
+ * (IterativeSolvers.f90:574) with its module parameters GMRES_RESTART, ITER_TOL,
+ * ITER_MAXITER (:56-58). All right-hand sides advance together; each Krylov
+ * step is one FP32 complex GEMM on MFMA and each restart recomputes the FP64
+ * residual; converged when ||B - A X|| <= tol ||B|| in every column (:388).
+ * precond: 1 block-diagonal LU with one block per body (SETUP_BLOCK_PRECONDITIONER
+ * :190), 0 none, -1 block-diagonal for several bodies and none for one.
+ * Defaults: restart 50, tol 1e-6, maxiter 500 (total Krylov steps), precond -1.
+ * hm_get_gmres_stats reports the last solve (converged = 0 is the reference's
+ * INFO = 1). */
+int hm_set_gmres(hm_ctx *ctx, int restart, double tol, int maxiter, int precond);
+int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, double *rel_residual);
 /* set_frequency + assemble + factor + solve + forces */
 int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, double *bdmp, double *exfc);
 /* nfreq frequencies back to back; outputs stacked per frequency */
