@@ -176,6 +176,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--path", choices=("gmres", "fp32ir"), default="gmres",
                     help="the mixed-precision solve that `value` measures")
+    ap.add_argument("--series-variant", type=int, default=1, choices=(0, 1, 2),
+                    help="epsilon-accelerated regimes: 1 lane groups of 8/16 (default), 2 of 4/8, 0 one pair per lane")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the traffic-counter calibration copy")
     ap.add_argument("--assemble-only", action="store_true",
@@ -197,12 +199,17 @@ def main():
     ctx.setup(prob)
     sync()
     print(f"# setup (upload + pair classification) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    ctx.set_series_variant(args.series_variant)
     if args.assemble_only:
+        ctx.set_timing(True)
         for s in range(args.steps):
             ctx.set_frequency(float(prob.omegas[s % len(prob.omegas)]))
             ctx.assemble()
         sync()
-        print(json.dumps({"assembled_systems": args.steps, "workload": prob.name}))
+        st = ctx.timing()
+        ctx.set_timing(False)
+        print(json.dumps({"assembled_systems": args.steps, "workload": prob.name,
+                          "series_variant": args.series_variant, "stages_ms": {k: v[0] for k, v in st.items()}}))
         ctx.close()
         return 0
 

@@ -88,6 +88,9 @@ struct hm_ctx {
   DevBuf d_P32, d_pipiv, d_gwork;
   std::vector<int> gm_blocks;
   hm::GmresStats gm_stats;
+  // epsilon-accelerated regimes: 0 one pair per lane; lane groups of 8 (EIGENE)
+  // and 16 (PIDCOCK) lanes: 1 (default); of 4 and 8 lanes: 2
+  int series_variant = 1;
   // timing: event pairs recorded on the stream, resolved by hm_get_timing
   bool timing = false;
   double stage_ms[kStages] = {0};
@@ -420,7 +423,8 @@ extern "C" int hm_assemble(hm_ctx* c) {
     for (int r = 0; r < 4; ++r) {
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
-      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt), "assemble");
+      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt,
+                                    c->series_variant), "assemble");
       c->stage_end(kAsmRegime0 + r);
     }
   }
@@ -621,6 +625,14 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
   c->gm_maxiter = maxiter;
   c->gm_precond = precond;
   c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_set_series_variant(hm_ctx* c, int variant) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (variant < 0 || variant > 2) return c->fail(HM_ERR_ARG - 2, "variant must be 0, 1 or 2");
+  c->series_variant = variant;
+  c->assembled = c->factored = c->solved = false;
   return HM_OK;
 }
 

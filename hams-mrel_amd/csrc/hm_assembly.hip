@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "hm_assembly.hpp"
+#include "hm_coop.hpp"
 #include "hm_device.hpp"
 
 namespace hm {
@@ -30,32 +31,13 @@ __device__ __forceinline__ void load_wvn(double* s, const double* g, int nk) {
   __syncthreads();
 }
 
-// one pair: field point i (hull centroid, or waterplane centroid when i >= n),
-// source panel j; REG = regime (or -1 for infinite depth, -2 for generic dispatch)
-template <int REG>
-__device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
-                                              double* A, double* S, long long ld) {
-  const double* xp = g.fp + 3 * (long long)i;
-  const double* xq = g.xyz_p + 3 * (long long)j;
-  const int flag = near_flag(xp, xq, g.pnsz[j]);
+// Rankine terms of a near pair and the write of A[i,j], S[i,j] (column-major, ld)
+//   A = (i==j ? 2*pi : 0) + sum_k (dS_k + dG_k*ds_j) n_jk  (flag = 1)  or  (sum_k dG_k n_jk) ds_j
+//   S = S_ij + G*ds_j  (flag = 1)  or  G*ds_j
+__device__ __forceinline__ void write_pair(const GeomDev& g, int i, int j, const double* xp, int flag, const Grn4& gr,
+                                           double* A, double* S, long long ld) {
   Rk rk{0.0, 0.0, 0.0, 0.0};
   if (flag) rankine(g.ncn[j], g.verts + 12 * (long long)j, xp, rk);
-  Grn4 gr;
-  if (REG == -1) {
-    infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
-  } else if (REG == -2) {
-    fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, wvn, f.h, flag, gr);
-  } else {
-    const double xpq = xq[0] - xp[0], ypq = xq[1] - xp[1];
-    const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
-    FinOut fo{0, 0, 0, 0, 0, 0};
-    if (REG == kLinton) fin_linton(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
-    else if (REG == kPidcock) fin_pidcock(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
-    else if (REG == kEigene) fin_eigene(rr, xq[2], xp[2], wvn, f.h, fo);
-    else fin_eigen(rr, xq[2], xp[2], wvn, f.h, fo);
-    if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
-    fin_to_grn(fo, xpq, ypq, xq[2], xp[2], rr, true, flag, gr);
-  }
   const double ear = g.ds[j];
   const double* nv = g.dxyz + 6 * (long long)j;
   double ar, ai, sr, si;
@@ -76,6 +58,33 @@ __device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f
   A[2 * off + 1] = ai;
   S[2 * off] = sr;
   S[2 * off + 1] = si;
+}
+
+// one pair: field point i (hull centroid, or waterplane centroid when i >= n),
+// source panel j; REG = regime (or -1 for infinite depth, -2 for generic dispatch)
+template <int REG>
+__device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
+                                              double* A, double* S, long long ld) {
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  if (REG == -1) {
+    infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
+  } else if (REG == -2) {
+    fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, wvn, f.h, flag, gr);
+  } else {
+    const double xpq = xq[0] - xp[0], ypq = xq[1] - xp[1];
+    const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+    FinOut fo{0, 0, 0, 0, 0, 0};
+    if (REG == kLinton) fin_linton(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
+    else if (REG == kPidcock) fin_pidcock(rr, xq[2], xp[2], f.v, wvn, f.h, fo);
+    else if (REG == kEigene) fin_eigene(rr, xq[2], xp[2], wvn, f.h, fo);
+    else fin_eigen(rr, xq[2], xp[2], wvn, f.h, fo);
+    if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+    fin_to_grn(fo, xpq, ypq, xq[2], xp[2], rr, true, flag, gr);
+  }
+  write_pair(g, i, j, xp, flag, gr, A, S, ld);
 }
 
 // 2-D grid over all pairs (infinite depth, or finite depth with dispatch per pair)
@@ -99,6 +108,82 @@ __global__ __launch_bounds__(256) void k_assemble_list(GeomDev g, FreqDev f, con
   if (t >= np) return;
   const uint32_t p = pairs[t];
   assemble_pair<REG>(g, f, wvn, (int)(p >> 16), (int)(p & 0xffffu), A, S, ld);
+}
+
+// The epsilon-accelerated regimes with G lanes per pair (see hm_coop.hpp): each
+// lane owns one pair of the list; the group evaluates the series and their
+// epsilon limits for its G pairs in turn, then every lane finishes its own pair.
+template <int REG, int G, int K>
+__global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
+                                                       long long np, double* A, double* S, long long ld) {
+  typedef LaneGroup<G> LG;
+  __shared__ double wvn[kNk];
+  load_wvn(wvn, f.wvn, kNk);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < np;
+  const uint32_t p = pairs[valid ? t : np - 1];
+  const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double h = f.h;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const double zf = xq[2], zp = xp[2];
+  const int nt = REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
+  const int l = threadIdx.x & (G - 1);
+  double L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  for (int q = 0; q < G; ++q) {
+    const double r = group_bcast<G>(rr, q);
+    const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
+    const int ntq = group_bcast<G>(nt, q);
+    double s1[K], s2[K], s3[K];
+    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int row = l + G * k;
+      double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+      if (G * k < ntq) {
+        if (row >= 1 && row < ntq) {
+          if (REG == kEigene) fin_evanescent(wvn[row], r, zfh, zph, h, t1, t2, t3);
+          else fin_pidcock_term(row, wvn[row], r, zfh, zph, f.v, h, t1, t2, t3);
+        }
+        t1 = LG::incl_sum(t1, l) + c1;
+        t2 = LG::incl_sum(t2, l) + c2;
+        t3 = LG::incl_sum(t3, l) + c3;
+        c1 = group_bcast<G>(t1, G - 1);
+        c2 = group_bcast<G>(t2, G - 1);
+        c3 = group_bcast<G>(t3, G - 1);
+      } else {
+        t1 = c1;
+        t2 = c2;
+        t3 = c3;
+      }
+      s1[k] = t1;
+      s2[k] = t2;
+      s3[k] = t3;
+    }
+    const int n = ntq - 1;
+    const double l1 = coop_limes<G, K>(n, s1, l);
+    const double l2 = coop_limes<G, K>(n, s2, l);
+    const double l3 = coop_limes<G, K>(n, s3, l);
+    if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
+  }
+  if (!valid) return;
+  FinOut fo{0, 0, 0, 0, 0, 0};
+  if (REG == kEigene) {
+    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+    fo.g_re += L1;
+    fo.gr_re += L2;
+    fo.gz_re += L3;
+    fin_strip(rr, zf, zp, fo);
+  } else {
+    fin_pidcock_head(rr, zf, zp, wvn, h, fo);
+    fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
+  }
+  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
+  write_pair(g, i, j, xp, flag, gr, A, S, ld);
 }
 
 // ---- classification --------------------------------------------------------
@@ -151,13 +236,21 @@ hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool
 }
 
 hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
-                           long long np, double* A, double* S, long long ld) {
+                           long long np, double* A, double* S, long long ld, int variant) {
   if (np <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((np + 255) / 256);
   switch (regime) {
     case kLinton: hipLaunchKernelGGL((k_assemble_list<kLinton>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
-    case kPidcock: hipLaunchKernelGGL((k_assemble_list<kPidcock>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
-    case kEigene: hipLaunchKernelGGL((k_assemble_list<kEigene>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    case kPidcock:
+      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 16, 7>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 8, 13>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      else hipLaunchKernelGGL((k_assemble_list<kPidcock>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      break;
+    case kEigene:
+      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kEigene, 8, 7>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kEigene, 4, 13>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      else hipLaunchKernelGGL((k_assemble_list<kEigene>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      break;
     default: hipLaunchKernelGGL((k_assemble_list<kEigen>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
   }
   return hipGetLastError();

@@ -33,7 +33,7 @@ hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* h
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
                          long long ld);
 hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
-                           long long np, double* A, double* S, long long ld);
+                           long long np, double* A, double* S, long long ld, int variant = 1);
 hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                    double yw, double* W, int ldw);
 hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,

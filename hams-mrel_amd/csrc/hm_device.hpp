@@ -101,8 +101,49 @@ HM_DEV double sf_ei(double x) {
   return exp(x) / x * ei;
 }
 
+// Correctly rounded quotients by the integer divisors of the series below,
+// without the generic division sequence: with y = RN(1/b) from a table rounded
+// at compile time, q = RN(a*y) is within an ulp, the remainder r = a - b*q is
+// exact with an FMA, and RN(q + r*y) is the correctly rounded quotient
+// (Markstein's correction step).
+struct RcpTables { double k[51], kk[51], kk1[51]; };
+constexpr RcpTables make_rcp_tables() {
+  RcpTables t{};
+  for (int i = 1; i <= 50; ++i) {
+    t.k[i] = 1.0 / i;
+    t.kk[i] = 1.0 / ((double)i * i);
+    t.kk1[i] = 1.0 / ((double)i * (i + 1));
+  }
+  return t;
+}
+static __constant__ constexpr RcpTables kRcp = make_rcp_tables();
+
+HM_DEV double div_by(double a, double b, double y) {
+  const double q = a * y;
+  const double r = fma(-b, q, a);
+  return fma(r, y, q);
+}
+// 1/b from the hardware estimate and two Newton steps, then a / b as above
+HM_DEV double div_cr(double a, double b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  return div_by(a, b, y);
+#else
+  return a / b;
+#endif
+}
+
 // K0(x), K1(x) (IK01A :538). The I0/I1 series feed K1 and the large-x K0.
+// Same operations in the same order as the reference, without multiply-add
+// contraction (the K0 series cancels by up to e^(2x), so one rounding more or
+// less shows at 1e-9); the divisions inside the loops are the correctly rounded
+// quotients of div_by and div_cr.
 HM_DEV void sf_k01(double x, double& bk0, double& bk1) {
+#pragma clang fp contract(off)
   if (x == 0.0) { bk0 = 1.0e300; bk1 = 1.0e300; return; }
   const double x2 = x * x;
   double bi0, bi1;
@@ -110,16 +151,16 @@ HM_DEV void sf_k01(double x, double& bk0, double& bk1) {
     double r = 1.0;
     bi0 = 1.0;
     for (int k = 1; k <= 50; ++k) {
-      r = 0.25 * r * x2 / (k * k);
+      r = div_by(0.25 * r * x2, (double)(k * k), kRcp.kk[k]);
       bi0 += r;
-      if (fabs(r / bi0) < 1.0e-15) break;
+      if (fabs(div_cr(r, bi0)) < 1.0e-15) break;
     }
     r = 1.0;
     bi1 = 1.0;
     for (int k = 1; k <= 50; ++k) {
-      r = 0.25 * r * x2 / (k * (k + 1));
+      r = div_by(0.25 * r * x2, (double)(k * (k + 1)), kRcp.kk1[k]);
       bi1 += r;
-      if (fabs(r / bi1) < 1.0e-15) break;
+      if (fabs(div_cr(r, bi1)) < 1.0e-15) break;
     }
     bi1 = 0.5 * x * bi1;
   } else {
@@ -149,10 +190,10 @@ HM_DEV void sf_k01(double x, double& bk0, double& bk1) {
     double w0 = 0.0, ww = 0.0, r = 1.0;
     k0v = 0.0;
     for (int k = 1; k <= 50; ++k) {
-      w0 += 1.0 / k;
-      r = 0.25 * r / (k * k) * x2;
+      w0 += kRcp.k[k];
+      r = div_by(0.25 * r, (double)(k * k), kRcp.kk[k]) * x2;
       k0v += r * (w0 + ct);
-      if (fabs((k0v - ww) / k0v) < 1.0e-15) break;
+      if (fabs(div_cr(k0v - ww, k0v)) < 1.0e-15) break;
       ww = k0v;
     }
     k0v += ct;
@@ -366,10 +407,10 @@ HM_DEV void fin_images(double r, double zf, double zp, double h, double& g, doub
   gz = (-s1z + s2z) / (h * h);
 }
 
-// 0.0005 <= R/h < 0.05: accelerated series + Wynn epsilon + image sums (PIDCOCK :558)
-HM_DEV void fin_pidcock(double r, double zf, double zp, double v, const double* wvn, double h,
-                        FinOut& o) {
-  double s1[kPidcockMaxTerms], s2[kPidcockMaxTerms], s3[kPidcockMaxTerms], sa[kPidcockMaxTerms];
+// 0.0005 <= R/h < 0.05: accelerated series + Wynn epsilon + image sums (PIDCOCK :558).
+// Split into the part before the series, one series term, and the part after
+// the epsilon limits, so that the lane-cooperative assembly kernel can share them.
+HM_DEV void fin_pidcock_head(double r, double zf, double zp, const double* wvn, double h, FinOut& o) {
   const double wk = wvn[0], zfh = zf + h, zph = zp + h;
   const double nm0 = h / 2.0 * (1.0 + sinh(2.0 * wk * h) / (2.0 * wk * h));
   const double p0 = kPi / nm0 * cosh(wk * zfh) * cosh(wk * zph);
@@ -382,35 +423,53 @@ HM_DEV void fin_pidcock(double r, double zf, double zp, double v, const double* 
   o.gr_im = -wk * p0 * j1;
   o.gz_re = dp0 * -y0;
   o.gz_im = dp0 * j0;
-  const int nt = fin_pidcock_terms(r, h);
-  s1[0] = s2[0] = s3[0] = 0.0;
-  for (int i = 1; i < nt; ++i) {
-    const double umt = i * kPi / h, um = wvn[i];
-    double k0, k1, k0t, k1t;
-    sf_k01(um * r, k0, k1);
-    sf_k01(umt * r, k0t, k1t);
-    const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-    double su, cu, sut, cut;
-    sincos(um * zfh, &su, &cu);
-    sincos(umt * zfh, &sut, &cut);
-    const double cup = cos(um * zph), cutp = cos(umt * zph);
-    const double am = ac * cu * cup;
-    const double amt = cut * cutp / h;
-    const double dam = -um * ac * su * cup;
-    const double damt = -umt * sut * cutp / h;
-    s1[i] = s1[i - 1] + (am * k0 - amt * k0t);
-    s2[i] = s2[i - 1] + (-um * am * k1 + umt * amt * k1t);
-    s3[i] = s3[i - 1] + (dam * k0 - damt * k0t);
-  }
-  const double l1 = sf_limes(nt - 1, s1, sa);
-  const double l2 = sf_limes(nt - 1, s2, sa);
-  const double l3 = sf_limes(nt - 1, s3, sa);
+}
+// term i >= 1 of the three accelerated series (mode i minus its large-i form)
+HM_DEV void fin_pidcock_term(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
+                             double& t2, double& t3) {
+  const double umt = i * kPi / h;
+  double k0, k1, k0t, k1t;
+  sf_k01(um * r, k0, k1);
+  sf_k01(umt * r, k0t, k1t);
+  const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
+  double su, cu, sut, cut;
+  sincos(um * zfh, &su, &cu);
+  sincos(umt * zfh, &sut, &cut);
+  const double cup = cos(um * zph), cutp = cos(umt * zph);
+  const double am = ac * cu * cup;
+  const double amt = cut * cutp / h;
+  const double dam = -um * ac * su * cup;
+  const double damt = -umt * sut * cutp / h;
+  t1 = am * k0 - amt * k0t;
+  t2 = -um * am * k1 + umt * amt * k1t;
+  t3 = dam * k0 - damt * k0t;
+}
+HM_DEV void fin_pidcock_tail(double l1, double l2, double l3, double r, double zf, double zp, double h, FinOut& o) {
   double ig, igr, igz;
   fin_images(r, zf, zp, h, ig, igr, igz);
   o.g_re += 4.0 * l1 + ig;
   o.gr_re += 4.0 * l2 + igr;
   o.gz_re += 4.0 * l3 + igz;
   fin_strip(r, zf, zp, o);
+}
+HM_DEV void fin_pidcock(double r, double zf, double zp, double v, const double* wvn, double h,
+                        FinOut& o) {
+  double s1[kPidcockMaxTerms], s2[kPidcockMaxTerms], s3[kPidcockMaxTerms], sa[kPidcockMaxTerms];
+  const double zfh = zf + h, zph = zp + h;
+  fin_pidcock_head(r, zf, zp, wvn, h, o);
+  const int nt = fin_pidcock_terms(r, h);
+  s1[0] = s2[0] = s3[0] = 0.0;
+  for (int i = 1; i < nt; ++i) {
+    double t1, t2, t3;
+    fin_pidcock_term(i, wvn[i], r, zfh, zph, v, h, t1, t2, t3);
+    s1[i] = s1[i - 1] + t1;
+    s2[i] = s2[i - 1] + t2;
+    s3[i] = s3[i - 1] + t3;
+  }
+  const double l1 = sf_limes(nt - 1, s1, sa);
+  const double l2 = sf_limes(nt - 1, s2, sa);
+  const double l3 = sf_limes(nt - 1, s3, sa);
+  fin_pidcock_tail(l1, l2, l3, r, zf, zp, h, o);
 }
 
 // ---- Linton regime (R/h < 0.0005, FinGreen3D.f90:712-861) -------------------

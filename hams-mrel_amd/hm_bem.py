@@ -28,7 +28,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
            "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
-           "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats")
+           "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
+           "hm_set_series_variant")
 
 _lib = None
 
@@ -183,6 +184,9 @@ class Context:
 
     def set_gmres(self, restart=50, tol=1e-6, maxiter=500, precond=-1):
         self._chk(self.L.hm_set_gmres(self.ptr, _i(restart), _d(tol), _i(maxiter), _i(precond)), "hm_set_gmres")
+
+    def set_series_variant(self, variant):
+        self._chk(self.L.hm_set_series_variant(self.ptr, _i(variant)), "hm_set_series_variant")
 
     def gmres_stats(self):
         it, cy, cv = _i(), _i(), _i()

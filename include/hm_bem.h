@@ -94,6 +94,15 @@ int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, doub
 int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, double *amas, double *bdmp,
                        double *exfc);
 
+/* Kernel variant of the two epsilon-accelerated finite-depth regimes (EIGENE,
+ * PIDCOCK; FinGreen3D.f90:413,:558): 0 one pair per lane (series arrays in
+ * scratch memory); 1 (default) a group of 8 (EIGENE) / 16 (PIDCOCK) lanes per
+ * pair with the series and the epsilon table in registers; 2 the same with 4 / 8
+ * lanes. All compute the same algorithm; the lane groups form the partial sums
+ * in a different order and take reciprocals to within two ulps, so they agree
+ * with variant 0 to the conditioning of the epsilon algorithm. */
+int hm_set_series_variant(hm_ctx *ctx, int variant);
+
 /* Stage timing: when enabled, a pair of HIP events is recorded on the context's
  * stream around each stage (no extra synchronisation); hm_get_timing waits for
  * the last one and accumulates elapsed milliseconds into ms[stage], calls into

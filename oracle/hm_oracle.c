@@ -438,6 +438,29 @@ static void eigene(double r, double zf, double zp, double wk, const double *wvn,
     singular_strip(r, zf, zp, g);
 }
 
+/* Test hook: the EIGENE series terms t[0..nt-1] (t[0] = 0) whose running sums
+ * eigene() feeds to Limes; lets a test form the partial sums in another order. */
+int hmo_eigene_terms(double r, double zf, double zp, const double *wvn, double h, double *t1, double *t2,
+                     double *t3)
+{
+    double rhp = fabs(r / h);
+    int nt = (int)(F32(-88.89) * rhp + F32(54.45));
+    double zfh = zf + h, zph = zp + h;
+    t1[0] = t2[0] = t3[0] = 0.0;
+    for (int i = 2; i <= nt; i++) {
+        double km = wvn[i - 1], k0, k1;
+        double twokmh = 2.0 * km * h;
+        double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
+        hmo_ik01a(km * r, &k0, &k1);
+        double kzf = km * zfh, kzp = km * zph;
+        double ckzf = cos(kzf), ckzp = cos(kzp), skzf = sin(kzf);
+        t1[i - 1] = pm * k0 * ckzf * ckzp;
+        t2[i - 1] = -pm * km * k1 * ckzf * ckzp;
+        t3[i - 1] = -pm * km * k0 * skzf * ckzp;
+    }
+    return nt;
+}
+
 static const double AMN[25] = {   /* AMN(0:4,0:4), column-major DATA (FinGreen3D.f90:949) */
     0.0, -0.02525711, 0.00086546, -0.00004063, 0.00000193,
     0.05051418, -0.00692380, 0.00073292, -0.00006636, 0.00000398,

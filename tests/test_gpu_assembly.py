@@ -202,3 +202,28 @@ def test_rejects_bad_geometry():
         c.setup(p)
     assert e.value.code == -108
     c.close()
+
+
+def test_series_variants_agree(finite, oracle):
+    """the lane-group kernels of the epsilon-accelerated regimes against the
+    one-pair-per-lane kernels and the oracle: same algorithm, partial sums added
+    in a different order, so they agree within the reference's own one-ulp band"""
+    p, c = finite
+    hist = c.pair_histogram()
+    assert hist[1].sum() > 0 and hist[2].sum() > 0      # both regimes are exercised
+    for w in p.omegas:
+        r = oracle.solve_frequency(p, w, want_matrices=True)
+        band = _oracle_band(oracle, p, w, r, matrices=True)
+        out = {}
+        for variant in (0, 1):
+            c.set_series_variant(variant)
+            c.set_frequency(w)
+            c.assemble()
+            out[variant] = c.get_system()
+            g = c.solve_frequency(w, hm_bem.HM_FP64)
+            bad = _check(g, r, band, 1e-10, 1e-8)
+            assert not bad, (w, variant, bad)
+        for k, name in ((0, "amat"), (1, "smat")):
+            assert relerr(out[1][k], out[0][k]) <= max(1e-12, 40.0 * band[name][0]), (w, name)
+            assert relerr(out[1][k], r[name]) <= max(1e-12, 20.0 * band[name][0]), (w, name)
+    c.set_series_variant(1)

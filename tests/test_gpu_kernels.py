@@ -7,8 +7,8 @@ Tolerances (relative to the pair's largest GRN entry):
   epsilon extrapolation that amplifies last-bit differences (libm, operation
   order): the reference's own result moves by up to ~1e-2 when one input
   coordinate moves by one ulp. There the bar is the reference's own conditioning,
-  max(1e-12, 50 x band), band = the largest change of the oracle under +-1 ulp
-  changes of each input coordinate, plus 2e-6 absolute because the series and
+  max(1e-12, 50 x band), band = the largest change of the oracle under +-1 and
+  +-2 ulp changes of each input coordinate, plus 2e-6 absolute because the series and
   their Wynn tables stop on absolute 1e-6 tests (FinGreen3D.f90:341,:787).
   Eigenfunction regime (R/h >= 0.5): 1e-12 relative plus 2e-6 absolute for the
   same early-exit test (FinGreen3D.f90:341).
@@ -28,13 +28,15 @@ def _band(oracle, src, fld, v, wvn, h, flag):
     for which in (0, 1):
         for k in range(3):
             for direction in (np.inf, -np.inf):
-                s = np.array(src, dtype=float)
-                f = np.array(fld, dtype=float)
-                if which == 0:
-                    s[k] = np.nextafter(s[k], direction)
-                else:
-                    f[k] = np.nextafter(f[k], direction)
-                worst = max(worst, relerr(oracle.fingreen3d(s, f, v, wvn, h, flag), g0))
+                for steps in (1, 2):
+                    s = np.array(src, dtype=float)
+                    f = np.array(fld, dtype=float)
+                    for _ in range(steps):
+                        if which == 0:
+                            s[k] = np.nextafter(s[k], direction)
+                        else:
+                            f[k] = np.nextafter(f[k], direction)
+                    worst = max(worst, relerr(oracle.fingreen3d(s, f, v, wvn, h, flag), g0))
     return g0, worst
 
 
