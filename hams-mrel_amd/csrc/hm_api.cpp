@@ -12,6 +12,9 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <chrono>
+#include <functional>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -20,6 +23,7 @@
 #include "hm_device.hpp"
 #include "hm_gmres.hpp"
 #include "hm_linalg.hpp"
+#include "hm_surrogate.hpp"
 
 namespace {
 
@@ -46,9 +50,10 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr int kStages = 11;
+constexpr int kStages = 12;
 enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6,
-             kAsmRegime0 = 7 /* + regime: Linton, Pidcock, eps-accelerated, eigenfunction */ };
+             kAsmRegime0 = 7 /* + regime: Linton, Pidcock, eps-accelerated, eigenfunction */,
+             kAsmSurrogate = 11 };
 
 }  // namespace
 
@@ -88,6 +93,16 @@ struct hm_ctx {
   DevBuf d_P32, d_pipiv, d_gwork;
   std::vector<int> gm_blocks;
   hm::GmresStats gm_stats;
+  // learned surrogate of the evanescent sums (R/h >= 0.05)
+  DevBuf d_sur, d_fb, d_fbcnt;
+  bool sur_ready = false, sur_on = false;
+  int sur_quant = 1;
+  double sur_tol = 1e-4;
+  long long sur_counts[3] = {0, 0, 0};
+  hm::SurDev sur_dev() const {
+    const float* b = d_sur.as<float>();
+    return hm::SurDev{b, b + hm::kSurParams, b + 2 * hm::kSurParams, b + 2 * hm::kSurParams + hm::kSurOut, sur_quant};
+  }
   // epsilon-accelerated regimes: 0 one pair per lane; lane groups of 8 (EIGENE)
   // and 16 (PIDCOCK) lanes: 1 (default); of 4 and 8 lanes: 2
   int series_variant = 1;
@@ -251,7 +266,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_sur, &c->d_fb, &c->d_fbcnt};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -420,12 +435,44 @@ extern "C" int hm_assemble(hm_ctx* c) {
     HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, nt), "assemble");
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
-    for (int r = 0; r < 4; ++r) {
+    const bool sur = c->sur_on && c->sur_ready;
+    for (int r = 0; r < (sur ? 2 : 4); ++r) {
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
       HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt,
                                     c->series_variant), "assemble");
       c->stage_end(kAsmRegime0 + r);
+    }
+    if (sur && c->seg_cnt[2] + c->seg_cnt[3] > 0) {
+      // network for the EIGENE + EIGEN segments (contiguous in the key order);
+      // pairs outside its validated cells go to the plain kernels
+      const long long np = c->seg_cnt[2] + c->seg_cnt[3];
+      HIPCHK(c, c->d_fb.ensure(sizeof(uint32_t) * (size_t)np), "surrogate alloc");
+      HIPCHK(c, c->d_fbcnt.ensure(2 * sizeof(unsigned long long)), "surrogate alloc");
+      HIPCHK(c, hipMemsetAsync(c->d_fbcnt.p, 0, 2 * sizeof(unsigned long long), c->stream), "surrogate");
+      uint32_t* fb_e = c->d_fb.as<uint32_t>();
+      uint32_t* fb_g = fb_e + c->seg_cnt[2];
+      const double kh = c->w1 * c->w1 * c->h / c->g_disp;
+      c->stage_begin(kAsmSurrogate);
+      HIPCHK(c, hm::assemble_surrogate(c->stream, gd, f, kh, pairs + c->seg_off[2], np, c->sur_dev(), (float)c->sur_tol,
+                                       A, S, nt, fb_e, fb_g, c->d_fbcnt.as<unsigned long long>()), "assemble surrogate");
+      c->stage_end(kAsmSurrogate);
+      unsigned long long cnt[2] = {0, 0};
+      HIPCHK(c, hipMemcpyAsync(cnt, c->d_fbcnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream), "surrogate");
+      HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate");
+      if ((long long)cnt[0] > c->seg_cnt[2] || (long long)cnt[1] > c->seg_cnt[3])
+        return c->fail(HM_ERR_STATE, "surrogate fallback lists overflow");
+      c->sur_counts[0] += np - (long long)(cnt[0] + cnt[1]);
+      c->sur_counts[1] += (long long)cnt[0];
+      c->sur_counts[2] += (long long)cnt[1];
+      for (int r = 2; r < 4; ++r) {
+        const long long m = (long long)cnt[r - 2];
+        if (!m) continue;
+        c->stage_begin(kAsmRegime0 + r);
+        HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, r == 2 ? fb_e : fb_g, m, A, S, nt, c->series_variant),
+               "assemble fallback");
+        c->stage_end(kAsmRegime0 + r);
+      }
     }
   }
   int rc = c->stage_end(kAssembly);
@@ -642,6 +689,287 @@ extern "C" int hm_get_gmres_stats(hm_ctx* c, int* iters, int* cycles, int* conve
   if (cycles) *cycles = c->gm_stats.cycles;
   if (converged) *converged = c->gm_stats.converged;
   if (rel_residual) *rel_residual = c->gm_stats.rel_residual;
+  return HM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned surrogate of the evanescent sums (hm_surrogate.hpp)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Kh table (h = 1) with the 200 dispersion roots of each entry
+void kh_table(int nkh, uint64_t seed, double g_disp, std::vector<double>& kh, std::vector<double>& wvn) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  kh.resize(nkh);
+  wvn.resize((size_t)nkh * kNk);
+  const double l0 = log(hm::kSurKhMin), l1 = log(hm::kSurKhMax);
+  for (int i = 0; i < nkh; ++i) {
+    const double k = exp(l0 + (i + u(rng)) / nkh * (l1 - l0));
+    const double w = sqrt(k * g_disp);
+    kh[i] = w * w / g_disp;
+    hm_dispersion(wvn.data() + (size_t)kNk * i, kNk, w, 1.0, g_disp);
+  }
+}
+
+struct TrainBufs {
+  DevBuf kh, wvn, X, Y, M, cell, Z, A, D, Yh, grad, m, v, loss, U, bits;
+};
+
+}  // namespace
+
+extern "C" int hm_surrogate_blob_size(void) { return hm::kSurBlob; }
+
+extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
+                                  unsigned long long seed, int quant, double* stats) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nsamples < 1024) return c->fail(HM_ERR_ARG - 2, "need at least 1024 samples");
+  if (epochs < 0) return c->fail(HM_ERR_ARG - 3, "epochs must be >= 0");
+  if (batch < 64 || batch > 65536) return c->fail(HM_ERR_ARG - 4, "batch must be in 64..65536");
+  if (!(lr > 0.0)) return c->fail(HM_ERR_ARG - 5, "lr must be > 0");
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t s = c->stream;
+  TrainBufs tb;
+  auto release = [&]() {
+    DevBuf* all[] = {&tb.kh, &tb.wvn, &tb.X, &tb.Y, &tb.M, &tb.cell, &tb.Z, &tb.A, &tb.D, &tb.Yh, &tb.grad, &tb.m,
+                     &tb.v, &tb.loss, &tb.U, &tb.bits};
+    for (DevBuf* b : all) b->release();
+  };
+  struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{release};
+  const int nkh = 2048;
+  std::vector<double> kh, wvn;
+  kh_table(nkh, seed ^ 0x5eedull, c->g_disp, kh, wvn);
+  const long long n = nsamples;
+  const long long nval = n / 5 > 200000 ? n / 5 : 200000;
+  const long long nmax = n > nval ? n : nval;
+  int rc;
+  HIPCHK(c, tb.kh.ensure(sizeof(double) * nkh), "surrogate alloc");
+  HIPCHK(c, tb.wvn.ensure(sizeof(double) * wvn.size()), "surrogate alloc");
+  HIPCHK(c, tb.X.ensure(4 * sizeof(float) * (size_t)nmax), "surrogate alloc");
+  HIPCHK(c, tb.Y.ensure(3 * sizeof(float) * (size_t)nmax), "surrogate alloc");
+  HIPCHK(c, hipMemcpyAsync(tb.kh.p, kh.data(), sizeof(double) * nkh, hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, hipMemcpyAsync(tb.wvn.p, wvn.data(), sizeof(double) * wvn.size(), hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, hm::sur_generate(s, n, seed, nkh, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
+                             tb.Y.as<float>(), nullptr, nullptr), "surrogate samples");
+  // output scales: RMS of each scaled sum
+  std::vector<float> yh((size_t)n * 3);
+  HIPCHK(c, hipMemcpyAsync(yh.data(), tb.Y.p, sizeof(float) * yh.size(), hipMemcpyDeviceToHost, s), "surrogate");
+  HIPCHK(c, hipStreamSynchronize(s), "surrogate samples");
+  float yscale[3];
+  for (int o = 0; o < 3; ++o) {
+    double acc = 0.0;
+    for (long long i = 0; i < n; ++i) acc += (double)yh[3 * i + o] * yh[3 * i + o];
+    yscale[o] = (float)sqrt(acc / (double)n);
+    if (!(yscale[o] > 0.0f)) yscale[o] = 1.0f;
+  }
+  // Glorot-uniform initial weights, zero biases
+  std::vector<float> p(hm::kSurParams, 0.0f);
+  std::mt19937_64 rng(seed);
+  auto fill = [&](int off, int fo, int fi) {
+    std::uniform_real_distribution<float> u(-sqrtf(6.0f / (fi + fo)), sqrtf(6.0f / (fi + fo)));
+    for (int k = 0; k < fo * fi; ++k) p[off + k] = u(rng);
+  };
+  fill(hm::kSurOffW0, hm::kSurH, hm::kSurIn);
+  for (int l = 0; l < 3; ++l) fill(hm::kSurOffW1 + l * hm::kSurHiddenStride, hm::kSurH, hm::kSurH);
+  fill(hm::kSurOffW4, hm::kSurOut, hm::kSurH);
+  HIPCHK(c, c->d_sur.ensure(sizeof(float) * (2 * (size_t)hm::kSurParams + hm::kSurOut + hm::kSurMap)), "surrogate alloc");
+  float* P = c->d_sur.as<float>();
+  float* Q = P + hm::kSurParams;
+  float* YS = Q + hm::kSurParams;
+  float* MAP = YS + hm::kSurOut;
+  HIPCHK(c, hipMemcpyAsync(P, p.data(), sizeof(float) * hm::kSurParams, hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, hipMemcpyAsync(YS, yscale, sizeof yscale, hipMemcpyHostToDevice, s), "surrogate");
+  const int B = batch;
+  HIPCHK(c, tb.Z.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "surrogate alloc");
+  HIPCHK(c, tb.A.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "surrogate alloc");
+  HIPCHK(c, tb.D.ensure(sizeof(float) * (4 * hm::kSurH + 3) * (size_t)B), "surrogate alloc");
+  HIPCHK(c, tb.Yh.ensure(3 * sizeof(float) * (size_t)B), "surrogate alloc");
+  HIPCHK(c, tb.grad.ensure(sizeof(float) * hm::kSurParams), "surrogate alloc");
+  HIPCHK(c, tb.m.ensure(sizeof(float) * hm::kSurParams), "surrogate alloc");
+  HIPCHK(c, tb.v.ensure(sizeof(float) * hm::kSurParams), "surrogate alloc");
+  HIPCHK(c, tb.loss.ensure(sizeof(double) * 2), "surrogate alloc");
+  HIPCHK(c, hipMemsetAsync(tb.m.p, 0, sizeof(float) * hm::kSurParams, s), "surrogate");
+  HIPCHK(c, hipMemsetAsync(tb.v.p, 0, sizeof(float) * hm::kSurParams, s), "surrogate");
+  HIPCHK(c, hm::sur_round(s, P, Q), "surrogate");
+  const long long steps_per_epoch = n / B;
+  const long long total = steps_per_epoch * epochs;
+  double last_loss = 0.0;
+  long long t = 0;
+  for (int e = 0; e < epochs; ++e) {
+    HIPCHK(c, hipMemsetAsync(tb.loss.p, 0, sizeof(double), s), "surrogate");
+    for (long long b = 0; b < steps_per_epoch; ++b, ++t) {
+      const float* Xb = tb.X.as<float>() + 4 * b * B;
+      const float* Yb = tb.Y.as<float>() + 3 * b * B;
+      const double lrt = lr * (0.01 + 0.99 * 0.5 * (1.0 + cos(hm::kPi * (double)t / (double)(total ? total : 1))));
+      const float* Wt = quant ? Q : P;
+      HIPCHK(c, hm::sur_forward(s, B, Xb, Wt, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(), quant != 0),
+             "surrogate fwd");
+      HIPCHK(c, hm::sur_loss_grad(s, B, Xb, Yb, YS, Wt, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(),
+                                  tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<float>()), "surrogate bwd");
+      HIPCHK(c, hm::sur_adam(s, P, tb.grad.as<float>(), tb.m.as<float>(), tb.v.as<float>(), (float)lrt, (int)(t + 1)),
+             "surrogate adam");
+      HIPCHK(c, hm::sur_round(s, P, Q), "surrogate");
+    }
+    double l = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&l, tb.loss.p, sizeof(double), hipMemcpyDeviceToHost, s), "surrogate");
+    HIPCHK(c, hipStreamSynchronize(s), "surrogate");
+    last_loss = l / (3.0 * (double)(steps_per_epoch * B));
+  }
+  // validation set on other Kh values: error map and RMS error
+  std::vector<double> khv, wvnv;
+  kh_table(512, seed ^ 0xa11da7eull, c->g_disp, khv, wvnv);
+  HIPCHK(c, hipMemcpyAsync(tb.kh.p, khv.data(), sizeof(double) * khv.size(), hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, hipMemcpyAsync(tb.wvn.p, wvnv.data(), sizeof(double) * wvnv.size(), hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, tb.M.ensure(3 * sizeof(float) * (size_t)nval), "surrogate alloc");
+  HIPCHK(c, tb.cell.ensure(sizeof(int) * (size_t)nval), "surrogate alloc");
+  HIPCHK(c, tb.U.ensure(3 * sizeof(float) * (size_t)nval), "surrogate alloc");
+  HIPCHK(c, tb.bits.ensure(sizeof(unsigned) * hm::kSurMap), "surrogate alloc");
+  HIPCHK(c, hm::sur_generate(s, nval, seed ^ 0x7a11dull, 512, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
+                             tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>()), "surrogate validation");
+  c->sur_quant = quant ? 1 : 0;
+  hm::SurDev md{P, Q, YS, MAP, c->sur_quant};
+  HIPCHK(c, hm::sur_predict(s, nval, tb.X.as<float>(), md, tb.U.as<float>()), "surrogate predict");
+  HIPCHK(c, hipMemsetAsync(tb.bits.p, 0, sizeof(unsigned) * hm::kSurMap, s), "surrogate");
+  HIPCHK(c, hipMemsetAsync(tb.loss.p, 0, sizeof(double), s), "surrogate");
+  HIPCHK(c, hm::sur_error_map(s, nval, tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>(), tb.U.as<float>(),
+                              tb.bits.as<unsigned>(), tb.loss.as<double>()), "surrogate error map");
+  std::vector<unsigned> bits(hm::kSurMap);
+  double sq = 0.0;
+  HIPCHK(c, hipMemcpyAsync(bits.data(), tb.bits.p, sizeof(unsigned) * hm::kSurMap, hipMemcpyDeviceToHost, s), "surrogate");
+  HIPCHK(c, hipMemcpyAsync(&sq, tb.loss.p, sizeof(double), hipMemcpyDeviceToHost, s), "surrogate");
+  HIPCHK(c, hipStreamSynchronize(s), "surrogate");
+  // bound per cell: 1.5 x the largest validation error; cells without samples are closed
+  std::vector<float> map(hm::kSurMap);
+  double worst = 0.0;
+  long long below4 = 0, below3 = 0;
+  for (int k = 0; k < hm::kSurMap; ++k) {
+    float e;
+    memcpy(&e, &bits[k], sizeof e);
+    map[k] = bits[k] ? 1.5f * e : 1e30f;
+    if (bits[k]) worst = fmax(worst, (double)e);
+    below4 += map[k] <= 1e-4f;
+    below3 += map[k] <= 1e-3f;
+  }
+  HIPCHK(c, hipMemcpyAsync(MAP, map.data(), sizeof(float) * hm::kSurMap, hipMemcpyHostToDevice, s), "surrogate");
+  HIPCHK(c, hipStreamSynchronize(s), "surrogate");
+  c->sur_ready = true;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    stats[0] = last_loss;
+    stats[1] = sqrt(sq / (3.0 * (double)nval)) / ((yscale[0] + yscale[1] + yscale[2]) / 3.0);
+    stats[2] = worst;
+    stats[3] = secs;
+    stats[4] = (double)below4 / hm::kSurMap;
+    stats[5] = (double)below3 / hm::kSurMap;
+    stats[6] = (double)total;
+    stats[7] = (double)n;
+  }
+  (void)rc;
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_get(hm_ctx* c, float* blob, int n) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!blob || n < hm::kSurBlob) return c->fail(HM_ERR_ARG - 3, "blob needs %d floats", hm::kSurBlob);
+  if (!c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
+  const float* b = c->d_sur.as<float>();
+  HIPCHK(c, hipMemcpyAsync(blob, b, sizeof(float) * hm::kSurParams, hipMemcpyDeviceToHost, c->stream), "surrogate get");
+  HIPCHK(c, hipMemcpyAsync(blob + hm::kSurParams, b + 2 * hm::kSurParams, sizeof(float) * (hm::kSurOut + hm::kSurMap),
+                           hipMemcpyDeviceToHost, c->stream), "surrogate get");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate get");
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_set(hm_ctx* c, const float* blob, int n) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!blob || n < hm::kSurBlob) return c->fail(HM_ERR_ARG - 3, "blob needs %d floats", hm::kSurBlob);
+  HIPCHK(c, c->d_sur.ensure(sizeof(float) * (2 * (size_t)hm::kSurParams + hm::kSurOut + hm::kSurMap)), "surrogate alloc");
+  float* b = c->d_sur.as<float>();
+  HIPCHK(c, hipMemcpyAsync(b, blob, sizeof(float) * hm::kSurParams, hipMemcpyHostToDevice, c->stream), "surrogate set");
+  HIPCHK(c, hipMemcpyAsync(b + 2 * hm::kSurParams, blob + hm::kSurParams, sizeof(float) * (hm::kSurOut + hm::kSurMap),
+                           hipMemcpyHostToDevice, c->stream), "surrogate set");
+  HIPCHK(c, hm::sur_round(c->stream, b, b + hm::kSurParams), "surrogate set");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate set");
+  c->sur_ready = true;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_set_surrogate(hm_ctx* c, int enable, double tol) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (enable && !c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
+  if (!(tol >= 0.0)) return c->fail(HM_ERR_ARG - 3, "tol must be >= 0");
+  c->sur_on = enable != 0;
+  c->sur_tol = tol;
+  for (long long& k : c->sur_counts) k = 0;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_counts(hm_ctx* c, long long* counts) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!counts) return c->fail(HM_ERR_ARG - 2, "counts is NULL");
+  for (int k = 0; k < 3; ++k) counts[k] = c->sur_counts[k];
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_eval(hm_ctx* c, int n, const double* rh, const double* zs, const double* zd,
+                                 const double* kh, double* t_net, double* t_plain, double* bound) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
+  if (!rh || !zs || !zd || !kh) return c->fail(HM_ERR_ARG - 3, "inputs missing");
+  if (!c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
+  for (int i = 0; i < n; ++i) {
+    const double zf = 0.5 * (zs[i] + zd[i]), zp = 0.5 * (zs[i] - zd[i]);
+    if (!(rh[i] >= hm::kSurRhMin && rh[i] <= hm::kSurRhMax) || !(kh[i] >= hm::kSurKhMin && kh[i] <= hm::kSurKhMax) ||
+        !(zf >= -hm::kSurDepth && zf <= 0.0 && zp >= -hm::kSurDepth && zp <= 0.0))
+      return c->fail(HM_ERR_ARG - 3, "point %d outside the surrogate's domain", i);
+  }
+  std::vector<double> wv((size_t)n * kNk);
+  for (int i = 0; i < n; ++i) {
+    const double w = sqrt(kh[i] * c->g_disp);
+    hm_dispersion(wv.data() + (size_t)kNk * i, kNk, w, 1.0, c->g_disp);
+  }
+  return hm::sur_eval_points(c->stream, n, rh, zs, zd, kh, wv.data(), c->g_disp, c->sur_dev(), t_net, t_plain, bound)
+             == hipSuccess ? HM_OK : c->fail(HM_ERR_HIP, "surrogate eval failed");
+}
+
+extern "C" int hm_surrogate_grad(hm_ctx* c, int batch, const float* x, const float* u, int quant, float* grad,
+                                 double* loss) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (batch < 1 || batch > 65536) return c->fail(HM_ERR_ARG - 2, "batch must be in 1..65536");
+  if (!x || !u || !grad) return c->fail(HM_ERR_ARG - 3, "arrays missing");
+  if (!c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
+  hipStream_t s = c->stream;
+  TrainBufs tb;
+  auto release = [&]() {
+    DevBuf* all[] = {&tb.X, &tb.Y, &tb.Z, &tb.A, &tb.D, &tb.Yh, &tb.grad, &tb.loss};
+    for (DevBuf* b : all) b->release();
+  };
+  struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{release};
+  const int B = batch;
+  HIPCHK(c, tb.X.ensure(4 * sizeof(float) * B), "grad alloc");
+  HIPCHK(c, tb.Y.ensure(3 * sizeof(float) * B), "grad alloc");
+  HIPCHK(c, tb.Z.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "grad alloc");
+  HIPCHK(c, tb.A.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "grad alloc");
+  HIPCHK(c, tb.D.ensure(sizeof(float) * (4 * hm::kSurH + 3) * (size_t)B), "grad alloc");
+  HIPCHK(c, tb.Yh.ensure(3 * sizeof(float) * B), "grad alloc");
+  HIPCHK(c, tb.grad.ensure(sizeof(float) * hm::kSurParams), "grad alloc");
+  HIPCHK(c, tb.loss.ensure(sizeof(double)), "grad alloc");
+  HIPCHK(c, hipMemcpyAsync(tb.X.p, x, 4 * sizeof(float) * B, hipMemcpyHostToDevice, s), "grad");
+  HIPCHK(c, hipMemcpyAsync(tb.Y.p, u, 3 * sizeof(float) * B, hipMemcpyHostToDevice, s), "grad");
+  HIPCHK(c, hipMemsetAsync(tb.loss.p, 0, sizeof(double), s), "grad");
+  const hm::SurDev m = c->sur_dev();
+  const float* w = quant ? m.q : m.p;
+  HIPCHK(c, hm::sur_forward(s, B, tb.X.as<float>(), w, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(), quant != 0),
+         "grad fwd");
+  HIPCHK(c, hm::sur_loss_grad(s, B, tb.X.as<float>(), tb.Y.as<float>(), m.yscale, w, tb.Z.as<float>(), tb.A.as<float>(),
+                              tb.Yh.as<float>(), tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<float>()), "grad bwd");
+  double l = 0.0;
+  HIPCHK(c, hipMemcpyAsync(grad, tb.grad.p, sizeof(float) * hm::kSurParams, hipMemcpyDeviceToHost, s), "grad");
+  HIPCHK(c, hipMemcpyAsync(&l, tb.loss.p, sizeof(double), hipMemcpyDeviceToHost, s), "grad");
+  HIPCHK(c, hipStreamSynchronize(s), "grad");
+  if (loss) *loss = l / (3.0 * B);
   return HM_OK;
 }
 

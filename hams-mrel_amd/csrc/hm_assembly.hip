@@ -23,41 +23,13 @@
 #include "hm_assembly.hpp"
 #include "hm_coop.hpp"
 #include "hm_device.hpp"
+#include "hm_pairs.hpp"
 
 namespace hm {
 
 __device__ __forceinline__ void load_wvn(double* s, const double* g, int nk) {
   for (int k = threadIdx.x; k < nk; k += blockDim.x) s[k] = g[k];
   __syncthreads();
-}
-
-// Rankine terms of a near pair and the write of A[i,j], S[i,j] (column-major, ld)
-//   A = (i==j ? 2*pi : 0) + sum_k (dS_k + dG_k*ds_j) n_jk  (flag = 1)  or  (sum_k dG_k n_jk) ds_j
-//   S = S_ij + G*ds_j  (flag = 1)  or  G*ds_j
-__device__ __forceinline__ void write_pair(const GeomDev& g, int i, int j, const double* xp, int flag, const Grn4& gr,
-                                           double* A, double* S, long long ld) {
-  Rk rk{0.0, 0.0, 0.0, 0.0};
-  if (flag) rankine(g.ncn[j], g.verts + 12 * (long long)j, xp, rk);
-  const double ear = g.ds[j];
-  const double* nv = g.dxyz + 6 * (long long)j;
-  double ar, ai, sr, si;
-  if (flag) {
-    ar = (rk.dx + gr.re[1] * ear) * nv[0] + (rk.dy + gr.re[2] * ear) * nv[1] + (rk.dz + gr.re[3] * ear) * nv[2];
-    ai = (gr.im[1] * ear) * nv[0] + (gr.im[2] * ear) * nv[1] + (gr.im[3] * ear) * nv[2];
-    sr = rk.s + gr.re[0] * ear;
-    si = gr.im[0] * ear;
-  } else {
-    ar = (gr.re[1] * nv[0] + gr.re[2] * nv[1] + gr.re[3] * nv[2]) * ear;
-    ai = (gr.im[1] * nv[0] + gr.im[2] * nv[1] + gr.im[3] * nv[2]) * ear;
-    sr = gr.re[0] * ear;
-    si = gr.im[0] * ear;
-  }
-  if (i == j) ar = 2.0 * kPi + ar;
-  const long long off = (long long)i + (long long)j * ld;
-  A[2 * off] = ar;
-  A[2 * off + 1] = ai;
-  S[2 * off] = sr;
-  S[2 * off + 1] = si;
 }
 
 // one pair: field point i (hull centroid, or waterplane centroid when i >= n),
@@ -84,7 +56,7 @@ __device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f
     if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
     fin_to_grn(fo, xpq, ypq, xq[2], xp[2], rr, true, flag, gr);
   }
-  write_pair(g, i, j, xp, flag, gr, A, S, ld);
+  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
 }
 
 // 2-D grid over all pairs (infinite depth, or finite depth with dispatch per pair)
@@ -183,7 +155,7 @@ __global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, con
   const int flag = near_flag(xp, xq, g.pnsz[j]);
   Grn4 gr;
   fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
-  write_pair(g, i, j, xp, flag, gr, A, S, ld);
+  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
 }
 
 // ---- classification --------------------------------------------------------

@@ -326,6 +326,20 @@ HM_DEV void fin_eigen(double r, double zf, double zp, const double* wvn, double 
   fin_strip(r, zf, zp, o);
 }
 
+// The evanescent-mode sums alone (what EIGEN / EIGENE add to the propagating
+// mode before the 1/r + 1/r1 strip): the part a learned surrogate replaces.
+HM_DEV void fin_eigen_evan(double r, double zf, double zp, const double* wvn, double h, double& tg, double& tr,
+                           double& tz) {
+  const double zfh = zf + h, zph = zp + h;
+  tg = tr = tz = 0.0;
+  for (int i = 1; i < 10; ++i) {
+    double t1, t2, t3;
+    fin_evanescent(wvn[i], r, zfh, zph, h, t1, t2, t3);
+    tg += t1; tr += t2; tz += t3;
+    if (fabs(t1) < F32(1.e-6) && fabs(t2) < F32(1.e-6) && fabs(t3) < F32(1.e-6)) break;
+  }
+}
+
 // Series lengths (FinGreen3D.f90:433, :597). Like every discrete decision of the
 // reference (regime, series length, near-panel flag), they are evaluated without
 // multiply-add contraction so that they round exactly as the reference does.
@@ -368,6 +382,24 @@ HM_DEV void fin_eigene(double r, double zf, double zp, const double* wvn, double
   o.gr_re += sf_limes(nt - 1, s2, sa);
   o.gz_re += sf_limes(nt - 1, s3, sa);
   fin_strip(r, zf, zp, o);
+}
+
+HM_DEV void fin_eigene_evan(double r, double zf, double zp, const double* wvn, double h, double& tg, double& tr,
+                            double& tz) {
+  double s1[kEigeneMaxTerms], s2[kEigeneMaxTerms], s3[kEigeneMaxTerms], sa[kEigeneMaxTerms];
+  const int nt = fin_eigene_terms(r, h);
+  const double zfh = zf + h, zph = zp + h;
+  s1[0] = s2[0] = s3[0] = 0.0;
+  for (int i = 1; i < nt; ++i) {
+    double t1, t2, t3;
+    fin_evanescent(wvn[i], r, zfh, zph, h, t1, t2, t3);
+    s1[i] = s1[i - 1] + t1;
+    s2[i] = s2[i - 1] + t2;
+    s3[i] = s3[i - 1] + t3;
+  }
+  tg = sf_limes(nt - 1, s1, sa);
+  tr = sf_limes(nt - 1, s2, sa);
+  tz = sf_limes(nt - 1, s3, sa);
 }
 
 // Rankine image sums by the Chebyshev fit (IMGS/DGSR/DGSZ, FinGreen3D.f90:941-1114)

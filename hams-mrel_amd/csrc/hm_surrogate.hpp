@@ -1,0 +1,102 @@
+// hm_surrogate.hpp -- learned surrogate of the finite-depth evanescent-mode sum.
+//
+// For R/h >= 0.05 the reference's FINGREEN3D (EIGENE / EIGEN,
+// FinGreen3D.f90:281-506) is
+//   wave part = propagating mode (J0, Y0, J1, Y1, cosh)  +  evanescent sum  -  (1/r + 1/r1)
+// The propagating mode carries all of the imaginary part and is cheap and
+// exact in FP64, as is the Rankine strip. The evanescent sum (up to 50 K0/K1
+// terms and three Wynn epsilon limits) is the expensive part; it is a smooth
+// function of four nondimensional inputs
+//   R/h, (z+zeta)/h, (z-zeta)/h, K h          (h = depth, K = omega^2 / g)
+// (the eigenfunctions cos k(z+h) cos k(zeta+h) depend on z+zeta and z-zeta).
+// The evanescent sums T carry the 1/r singularity that the strip S = -(1/r +
+// 1/r1) (and its R-, z-derivatives) removes. The network predicts, per
+// component, Y = T + S e^{-k1 R} (k1 = first evanescent wave number): near
+// R = 0 this is the smooth regular part T + S, for large R both terms decay
+// like e^{-k1 R}. A 4-64-64-64-64-3 SiLU network predicts the dimensionless
+// U = Y R^p e^{k1 R} (p = 1, 2, 2), O(1) over the domain; the assembly adds
+// V = U e^{-k1 R} / R^p + S (1 - e^{-k1 R}) to the exact propagating mode.
+//
+// Precision: the first layer (4 -> 64) runs in FP32 (BF16 inputs would quantize
+// log(R/h) to 1 %); the hidden and output layers take BF16 weights and BF16
+// activations with FP32 accumulation, as the inference kernel computes them.
+// Training is quantization-aware (forward with the BF16-rounded copy, FP32
+// master weights, Adam).
+//
+// Error map: after training, the network is compared with the plain series on
+// an independent validation set; the largest relative error of the Green
+// function value and of its gradient vector (each against the size of the full
+// component: propagating + evanescent + Rankine) is recorded per cell of a 24 x 8 x 8 x 24 grid over the four inputs. The
+// assembly uses the network only in cells whose bound is below the tolerance
+// and falls back to the plain series elsewhere, for R/h < 0.1 (the close pairs
+// of one body, the Pidcock and Linton regimes), for R/h >= 0.5 (EIGEN, at most
+// 9 terms, cheaper than a network evaluation; the network is trained up to
+// R/h = 2) and for points deeper than 0.25 h. The domain (R/h in [0.1, 2],
+// z/h, zeta/h in [-0.25, 0], Kh in [0.01, 50]) covers the inter-body pairs of
+// surface-piercing hull arrays; over the whole water column, or down to R/h =
+// 0.05, a network of this size reached only ~1e-2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hm_assembly.hpp"
+
+namespace hm {
+
+constexpr int kSurIn = 4, kSurH = 64, kSurOut = 3;
+constexpr int kSurOffW0 = 0;
+constexpr int kSurOffB0 = kSurOffW0 + kSurH * kSurIn;
+constexpr int kSurOffW1 = kSurOffB0 + kSurH;                 // hidden layer l = 1..3: W (64x64), b (64)
+constexpr int kSurHiddenStride = kSurH * kSurH + kSurH;
+constexpr int kSurOffW4 = kSurOffW1 + 3 * kSurHiddenStride;
+constexpr int kSurOffB4 = kSurOffW4 + kSurOut * kSurH;
+constexpr int kSurParams = kSurOffB4 + kSurOut;              // 12995
+constexpr int kMapR = 24, kMapS = 8, kMapD = 8, kMapK = 24;
+constexpr int kSurMap = kMapR * kMapS * kMapD * kMapK;       // 36864 cells
+constexpr int kSurBlob = kSurParams + kSurOut + kSurMap;     // params, output scales, error map
+constexpr double kSurRhMin = 0.1, kSurRhMax = 2.0, kSurKhMin = 0.01, kSurKhMax = 50.0;
+constexpr double kSurDepth = 0.25;   // z/h and zeta/h in [-0.25, 0]: the draft range of surface-piercing hulls
+
+// a trained model on the device: FP32 parameters, the BF16-rounded copy the
+// forward pass uses (as floats), output scales and the error map
+struct SurDev {
+  const float* p;     // kSurParams
+  const float* q;     // kSurParams, BF16-rounded
+  const float* yscale;
+  const float* map;
+  int quant;          // 1: BF16 weights and activations; 0: FP32 network
+};
+
+struct SurTrainStats {
+  double train_loss, val_rmse, val_max_err, seconds;
+  long long samples;
+  int epochs, steps;
+};
+
+// training and validation data from the plain kernels (h = 1)
+hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, const double* kh, const double* wvn,
+                        float* X, float* Y, float* M, int* cell);
+hipError_t sur_round(hipStream_t s, const float* p, float* q);
+hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh,
+                       bool quant = true);
+hipError_t sur_loss_grad(hipStream_t s, int B, const float* X, const float* Y, const float* yscale, const float* q,
+                         const float* Z, const float* A, const float* Yh, float* D, float* grad, float* loss);
+hipError_t sur_adam(hipStream_t s, float* p, const float* grad, float* m, float* v, float lr, int t);
+hipError_t sur_error_map(hipStream_t s, long long n, const float* Y, const float* M, const int* cell,
+                         const float* Yh, unsigned* map_bits, double* sq_err);
+// network evaluation of scaled outputs U for normalized inputs (n x 4)
+hipError_t sur_predict(hipStream_t s, long long n, const float* X, SurDev m, float* U);
+
+// network and plain evanescent sums T (unscaled, h = 1) at host points, with
+// the cell bound of each point (test and ablation entry)
+hipError_t sur_eval_points(hipStream_t s, int n, const double* rh, const double* zs, const double* zd, const double* kh,
+                           const double* wvn, double g_disp, SurDev m, double* tn, double* tp, double* bound);
+
+// assembly of the pairs of the EIGENE + EIGEN list segments with the network;
+// pairs whose cell bound exceeds tol (or outside the trained R/h range) are
+// appended to fallback lists (regime 2 -> fb[0..], regime 3 -> fb2) for the plain kernels
+hipError_t assemble_surrogate(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
+                              long long np, SurDev m, float tol, double* A, double* S, long long ld,
+                              uint32_t* fb_eigene, uint32_t* fb_eigen, unsigned long long* fb_count);
+
+}  // namespace hm

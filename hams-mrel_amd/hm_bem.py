@@ -15,8 +15,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libhm_bem.so")
 HM_FP64 = 0
 HM_FP32_IR = 1
 HM_GMRES = 2
+SUR_PARAMS = 12995          # network parameters (see include/hm_bem.h)
 STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces",
-          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen")
+          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen", "asm_surrogate")
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -29,7 +30,9 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
            "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
-           "hm_set_series_variant")
+           "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
+           "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
+           "hm_surrogate_grad")
 
 _lib = None
 
@@ -184,6 +187,53 @@ class Context:
 
     def set_gmres(self, restart=50, tol=1e-6, maxiter=500, precond=-1):
         self._chk(self.L.hm_set_gmres(self.ptr, _i(restart), _d(tol), _i(maxiter), _i(precond)), "hm_set_gmres")
+
+    # ---- learned surrogate ------------------------------------------------------
+    def surrogate_train(self, nsamples=2_000_000, epochs=20, batch=8192, lr=2e-3, seed=20261204, quant=True):
+        st = np.zeros(8)
+        self._chk(self.L.hm_surrogate_train(self.ptr, ctypes.c_longlong(nsamples), _i(epochs), _i(batch), _d(lr),
+                                            ctypes.c_ulonglong(seed), _i(1 if quant else 0), _p(st)),
+                  "hm_surrogate_train")
+        keys = ("train_loss", "val_rmse", "val_max_err", "seconds", "cells_below_1e-4", "cells_below_1e-3",
+                "steps", "samples")
+        return dict(zip(keys, st.tolist()))
+
+    def surrogate_get(self):
+        n = self.L.hm_surrogate_blob_size()
+        b = np.zeros(n, dtype=np.float32)
+        self._chk(self.L.hm_surrogate_get(self.ptr, _p(b), _i(n)), "hm_surrogate_get")
+        return b
+
+    def surrogate_set(self, blob):
+        b = np.ascontiguousarray(blob, dtype=np.float32)
+        self._chk(self.L.hm_surrogate_set(self.ptr, _p(b), _i(len(b))), "hm_surrogate_set")
+
+    def set_surrogate(self, enable=True, tol=1e-4):
+        self._chk(self.L.hm_set_surrogate(self.ptr, _i(1 if enable else 0), _d(tol)), "hm_set_surrogate")
+
+    def surrogate_counts(self):
+        k = np.zeros(3, dtype=np.int64)
+        self._chk(self.L.hm_surrogate_counts(self.ptr, _p(k)), "hm_surrogate_counts")
+        return dict(network=int(k[0]), fallback_eigene=int(k[1]), fallback_eigen=int(k[2]))
+
+    def surrogate_eval(self, rh, zs, zd, kh):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (rh, zs, zd, kh)]
+        n = len(a[0])
+        tn = np.zeros((n, 3))
+        tp = np.zeros((n, 3))
+        bd = np.zeros(n)
+        self._chk(self.L.hm_surrogate_eval(self.ptr, _i(n), *[_p(v) for v in a], _p(tn), _p(tp), _p(bd)),
+                  "hm_surrogate_eval")
+        return tn, tp, bd
+
+    def surrogate_grad(self, x, u, quant=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        u = np.ascontiguousarray(u, dtype=np.float32)
+        g = np.zeros(self.L.hm_surrogate_blob_size(), dtype=np.float32)
+        loss = _d()
+        self._chk(self.L.hm_surrogate_grad(self.ptr, _i(len(x)), _p(x), _p(u), _i(1 if quant else 0), _p(g),
+                                           ctypes.byref(loss)), "hm_surrogate_grad")
+        return g[:SUR_PARAMS], loss.value
 
     def set_series_variant(self, variant):
         self._chk(self.L.hm_set_series_variant(self.ptr, _i(variant)), "hm_set_series_variant")

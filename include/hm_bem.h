@@ -103,13 +103,66 @@ int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, 
  * with variant 0 to the conditioning of the epsilon algorithm. */
 int hm_set_series_variant(hm_ctx *ctx, int variant);
 
+/* ---- learned surrogate of the evanescent-mode sums ----------------------------
+ * Replaces, for 0.1 <= R/h <= 2 and z, zeta in [-0.25 h, 0], the evanescent part of EIGENE / EIGEN
+ * (FinGreen3D.f90:413-506, :281-362): up to 50 K0/K1 terms and three Wynn
+ * epsilon limits per pair. A 4-64-64-64-64-3 SiLU network of (R/h, (z+zeta)/h,
+ * (z-zeta)/h, Kh) predicts the real regular part (G, dG/dR, dG/dz of the
+ * evanescent sums minus 1/r + 1/r1, smooth for R/h >= 0.05); the propagating
+ * mode (the whole imaginary part) stays exact FP64. Hidden and output layers use BF16 weights and activations with
+ * FP32 accumulation; the first layer is FP32.
+ *
+ * hm_surrogate_train  generates nsamples training samples with the plain FP64
+ *                     series on the GPU (h = 1, log-uniform R/h in [0.05, 8] and
+ *                     Kh in [0.01, 50] over 2048 Kh values, uniform z, zeta in
+ *                     [-h, 0]), trains with Adam (cosine learning rate from lr,
+ *                     batch `batch`, `epochs` passes) by the library's own forward
+ *                     and backward kernels (quant = 1: forward with BF16 weights
+ *                     and activations as the inference runs it; 0: FP32), then
+ *                     measures the error against the
+ *                     plain series on an independent validation set (nsamples/5,
+ *                     at least 2e5, on 512 other Kh values) and stores, per cell
+ *                     of a 24 x 8 x 8 x 24 grid over the four inputs, 1.5 x the
+ *                     largest relative error (against the size of the full Green
+ *                     function component). stats[8]: final training loss
+ *                     (normalized MSE), validation RMS error (normalized), largest
+ *                     validation error, seconds, fraction of cells with bound
+ *                     <= 1e-4, <= 1e-3, Adam steps, samples.
+ * hm_surrogate_get/set  the model as hm_surrogate_blob_size() floats
+ *                     (parameters, output scales, error map).
+ * hm_set_surrogate    use it in hm_assemble for finite depth: a pair takes the
+ *                     network when its cell bound is <= tol, otherwise the plain
+ *                     series (also always for R/h < 0.1, R/h >= 0.5 where the
+ *                     plain EIGEN series is cheaper, and points deeper than
+ *                     0.25 h).
+ * hm_surrogate_counts pairs assembled since hm_set_surrogate: [0] by the network,
+ *                     [1] plain EIGENE fallback, [2] plain EIGEN fallback.
+ * hm_surrogate_eval   network and plain regular parts (h = 1) at n points, with
+ *                     the cell bound of each point (t_net, t_plain: 3n; bound: n).
+ * hm_surrogate_grad   loss and gradient of the loss (mean squared error of the
+ *                     normalized outputs against scaled targets u) over one batch
+ *                     for the current parameters; quant = 0 evaluates the network
+ *                     without BF16 rounding. For checks against a CPU reference.
+ * None of these has a counterpart in the reference, which has no surrogate. */
+int hm_surrogate_blob_size(void);
+int hm_surrogate_train(hm_ctx *ctx, long long nsamples, int epochs, int batch, double lr, unsigned long long seed,
+                       int quant, double *stats);
+int hm_surrogate_get(hm_ctx *ctx, float *blob, int n);
+int hm_surrogate_set(hm_ctx *ctx, const float *blob, int n);
+int hm_set_surrogate(hm_ctx *ctx, int enable, double tol);
+int hm_surrogate_counts(hm_ctx *ctx, long long *counts);
+int hm_surrogate_eval(hm_ctx *ctx, int n, const double *rh, const double *zs, const double *zd, const double *kh,
+                      double *t_net, double *t_plain, double *bound);
+int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, int quant, float *grad, double *loss);
+
 /* Stage timing: when enabled, a pair of HIP events is recorded on the context's
  * stream around each stage (no extra synchronisation); hm_get_timing waits for
  * the last one and accumulates elapsed milliseconds into ms[stage], calls into
  * counts[stage], for stages 0 classify, 1 assembly (all regimes), 2 RHS GEMM,
  * 3 IRR normal equations, 4 LU, 5 triangular solves + refinement, 6 forces,
  * 7..10 the assembly kernel of each finite-depth regime (Linton, Pidcock,
- * epsilon-accelerated, eigenfunction). hm_set_timing resets the totals. */
+ * epsilon-accelerated, eigenfunction), 11 the surrogate assembly kernel.
+ * hm_set_timing resets the totals. */
 int hm_set_timing(hm_ctx *ctx, int enable);
 int hm_get_timing(hm_ctx *ctx, double *ms, int *counts, int nstage);
 
