@@ -92,17 +92,17 @@ def test_assembly_with_surrogate(trained):
         a1, s1, _ = ctx.get_system()
         assert np.array_equal(a1, a0) and np.array_equal(s1, s0)
         assert ctx.surrogate_counts()["network"] == 0
-        # tolerance 1e-3: the network takes the cells it was validated on
-        ctx.set_surrogate(True, 1e-3)
+        # tolerance 3e-3: the network takes the cells it was validated on
+        ctx.set_surrogate(True, 3e-3)
         ctx.set_frequency(w)
         ctx.assemble()
         a2, s2, _ = ctx.get_system()
         k = ctx.surrogate_counts()
         assert k["network"] > 0, k
-        assert relerr(a2, a0) < 2e-3 and relerr(s2, s0) < 2e-3, (relerr(a2, a0), relerr(s2, s0), k)
+        assert relerr(a2, a0) < 3e-3 and relerr(s2, s0) < 3e-3, (relerr(a2, a0), relerr(s2, s0), k)
         got = ctx.solve_frequency(w, hm_bem.HM_FP64)
         for key in ("amas", "bdmp", "exfc"):
-            assert relerr(got[key], ref[key]) < 1e-2, (w, key, relerr(got[key], ref[key]), k)
+            assert relerr(got[key], ref[key]) < 2e-3, (w, key, relerr(got[key], ref[key]), k)
         print("omega", w, "counts", k, "A err", relerr(a2, a0), "forces err",
               {key: relerr(got[key], ref[key]) for key in ("amas", "bdmp", "exfc")}, flush=True)
     ctx.close()
