@@ -87,6 +87,11 @@ PATH_DESC = {
               "allows, plain series elsewhere) + restarted GMRES(50), tol 1e-6: FP32 complex GEMM products on "
               "MFMA, FP32 block-diagonal LU preconditioner (one block per body), FP64 residuals"),
     "fp32ir": "FP64 assembly + FP32 LU (MFMA) + FP64 iterative refinement",
+    "hmatrix": ("H-matrix: dense FP64 diagonal block per body (learned BF16 surrogate for the evanescent sums "
+                "where its validated error bound allows), off-diagonal blocks by adaptive cross approximation "
+                "evaluated straight from the FP64 Green function, right-hand sides S W from the H-matrix of S; "
+                "restarted GMRES(50), tol 1e-6, with FP32 MFMA H-matrix products, FP32 block-diagonal LU "
+                "preconditioner and FP64 H-matrix residuals"),
 }
 
 
@@ -175,8 +180,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 path")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--path", choices=("gmres", "fp32ir"), default="gmres",
+    ap.add_argument("--path", choices=("hmatrix", "gmres", "fp32ir"), default="hmatrix",
                     help="the mixed-precision solve that `value` measures")
+    ap.add_argument("--aca-eps", type=float, default=1e-5, help="cross-approximation tolerance of A")
+    ap.add_argument("--aca-eps-s", type=float, default=1e-6, help="cross-approximation tolerance of S")
+    ap.add_argument("--aca-kmax", type=int, default=50, help="largest rank of a cross-approximated block")
     ap.add_argument("--surrogate-tol", type=float, default=2e-3,
                     help="cell error bound up to which the learned surrogate replaces the plain series (0: off)")
     ap.add_argument("--surrogate-samples", type=int, default=8_000_000)
@@ -231,18 +239,25 @@ def main():
         ctx.set_surrogate(True, args.surrogate_tol)
     # mixed precision path: GMRES with FP32 MFMA products and FP64 residuals
     # (default), or FP32 LU on MFMA + FP64 refinement
-    prec = {"gmres": hm_bem.HM_GMRES, "fp32ir": hm_bem.HM_FP32_IR}[args.path]
-    if args.path == "gmres":
+    prec = {"hmatrix": hm_bem.HM_GMRES, "gmres": hm_bem.HM_GMRES, "fp32ir": hm_bem.HM_FP32_IR}[args.path]
+    if prec == hm_bem.HM_GMRES:
         ctx.set_gmres(restart=50, tol=args.gmres_tol)
+    hmat = None
+    if args.path == "hmatrix":
+        ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
     dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup)
-    gm = ctx.gmres_stats() if args.path == "gmres" else None
+    gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
+    if args.path == "hmatrix":
+        hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
+        ctx.set_hmatrix(False)
+        print(f"# hmatrix {hmat}", file=sys.stderr, flush=True)
     if sur is not None:
         sur["pairs"] = ctx.surrogate_counts()
         ctx.set_surrogate(False)
     print(f"# {args.path}: {args.steps} systems in {dt_mx:.3f} s; stages {st_mx}; gmres {gm}; surrogate {sur}",
           file=sys.stderr, flush=True)
     others = {}
-    if args.path == "gmres" and args.with_fp32ir:
+    if args.path != "fp32ir" and args.with_fp32ir:
         dt_ir, _, st_ir = time_path(ctx, prob, hm_bem.HM_FP32_IR, args.steps, args.warmup)
         print(f"# fp32ir: {args.steps} systems in {dt_ir:.3f} s; stages {st_ir}", file=sys.stderr, flush=True)
         others["fp32_lu_ir"] = {"value": args.steps / dt_ir, "ms_per_step": 1e3 * dt_ir / args.steps}
@@ -299,6 +314,8 @@ def main():
             "plain_fp64": plain, "roofline": roofline}
     if gm is not None:
         line["gmres"] = gm
+    if hmat is not None:
+        line["hmatrix"] = hmat
     if sur is not None:
         line["surrogate"] = sur
     if others:

@@ -22,6 +22,7 @@
 #include "hm_bem.h"
 #include "hm_device.hpp"
 #include "hm_gmres.hpp"
+#include "hm_hmatrix.hpp"
 #include "hm_linalg.hpp"
 #include "hm_surrogate.hpp"
 
@@ -50,10 +51,10 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr int kStages = 12;
+constexpr int kStages = 13;
 enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6,
              kAsmRegime0 = 7 /* + regime: Linton, Pidcock, eps-accelerated, eigenfunction */,
-             kAsmSurrogate = 11 };
+             kAsmSurrogate = 11, kAsmCross = 12 };
 
 }  // namespace
 
@@ -103,6 +104,21 @@ struct hm_ctx {
     const float* b = d_sur.as<float>();
     return hm::SurDev{b, b + hm::kSurParams, b + 2 * hm::kSurParams, b + 2 * hm::kSurParams + hm::kSurOut, sur_quant};
   }
+  // H-matrix form (equal bodies, no waterplane points): dense diagonal blocks
+  // (block b at b m (m+1) complex elements, leading dimension m, so that the
+  // pair kernels' i + j ld addressing lands in it) and cross-approximated
+  // off-diagonal blocks
+  bool hm_on = false;
+  double hm_eps = 1e-4, hm_eps_s = 1e-4;
+  int hm_kmax = 50;
+  DevBuf d_hmat, d_hD, d_hDS, d_hD32, d_hU32, d_hV32, d_htb, d_hts;
+  DevBuf d_hUc[2], d_hVc[2];   // factors of A and S packed at their largest rank
+  int hm_kuse[2] = {0, 0};
+  hm::HmatDev hmat{};
+  int hm_unconverged = 0, hm_steps = 0, hm_rank_max = 0;
+  double hm_rank_mean = 0.0;
+  int bm() const { return n / nbody; }
+  size_t hm_diag_elems() const { return (size_t)nbody * bm() * (bm() + 1); }
   // epsilon-accelerated regimes: 0 one pair per lane; lane groups of 8 (EIGENE)
   // and 16 (PIDCOCK) lanes: 1 (default); of 4 and 8 lanes: 2
   int series_variant = 1;
@@ -334,30 +350,25 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   HIPCHK(c, hipStreamSynchronize(c->stream), "geometry upload");
   c->have_geom = true;
   c->have_depth = false;
+  c->hm_on = false;
   c->assembled = c->factored = c->solved = false;
   return HM_OK;
 }
 
-extern "C" int hm_set_depth(hm_ctx* c, double h) {
-  if (!c) return HM_ERR_ARG - 1;
-  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_depth before hm_set_geometry");
-  if (h == 0.0) return c->fail(HM_ERR_ARG - 2, "depth must be > 0 (finite) or < 0 (infinite)");
-  c->h = h;
-  c->have_depth = true;
-  c->have_freq = false;
-  c->assembled = c->factored = c->solved = false;
-  for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
-  c->hist_host.assign(hm::kNumKeys, 0);
-  if (h < 0.0) return HM_OK;
-  // classify the pairs by regime and series length (frequency independent)
+// classify the pairs by regime and series length (frequency independent)
+static int classify(hm_ctx* c) {
   const hm::GeomDev gd = c->geom();
-  const long long np = (long long)c->nt() * c->n;
+  const double h = c->h;
+  // the H-matrix form assembles only the pairs of the diagonal blocks densely
+  const int blk = c->hm_on ? c->bm() : 0;
+  const long long np = blk ? (long long)c->nbody * blk * blk : (long long)c->nt() * c->n;
+  for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
   c->stage_begin(kClassify);
   HIPCHK(c, c->d_hist.ensure(sizeof(unsigned) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_cursor.ensure(sizeof(unsigned long long) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_pairs.ensure(sizeof(uint32_t) * (size_t)np), "classify");
   HIPCHK(c, hipMemsetAsync(c->d_hist.p, 0, sizeof(unsigned) * hm::kNumKeys, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true, blk), "classify");
   std::vector<unsigned> hist(hm::kNumKeys);
   HIPCHK(c, hipMemcpyAsync(hist.data(), c->d_hist.p, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, c->stream), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
@@ -374,9 +385,23 @@ extern "C" int hm_set_depth(hm_ctx* c, double h) {
     c->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
   }
   HIPCHK(c, hipMemcpyAsync(c->d_cursor.p, cur.data(), sizeof(unsigned long long) * hm::kNumKeys, hipMemcpyHostToDevice, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false, blk), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
   return c->stage_end(kClassify);
+}
+
+extern "C" int hm_set_depth(hm_ctx* c, double h) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_depth before hm_set_geometry");
+  if (h == 0.0) return c->fail(HM_ERR_ARG - 2, "depth must be > 0 (finite) or < 0 (infinite)");
+  c->h = h;
+  c->have_depth = true;
+  c->have_freq = false;
+  c->assembled = c->factored = c->solved = false;
+  for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
+  c->hist_host.assign(hm::kNumKeys, 0);
+  if (h < 0.0) return HM_OK;
+  return classify(c);
 }
 
 extern "C" int hm_set_headings(hm_ctx* c, int nbeta, const double* beta) {
@@ -416,30 +441,21 @@ extern "C" int hm_set_frequency(hm_ctx* c, double w1) {
 // ---------------------------------------------------------------------------
 // per-frequency chain
 // ---------------------------------------------------------------------------
-extern "C" int hm_assemble(hm_ctx* c) {
-  if (!c) return HM_ERR_ARG - 1;
-  if (!c->have_freq) return c->fail(HM_ERR_STATE, "hm_assemble before hm_set_frequency");
-  const int n = c->n, nt = c->nt(), ncol = c->ncol();
-  const size_t cplx = 2 * sizeof(double);
-  HIPCHK(c, c->d_A.ensure(cplx * (size_t)nt * n), "assemble alloc A");
-  HIPCHK(c, c->d_S.ensure(cplx * (size_t)nt * n), "assemble alloc S");
-  HIPCHK(c, c->d_W.ensure(cplx * (size_t)n * ncol), "assemble alloc W");
-  HIPCHK(c, c->d_R.ensure(cplx * (size_t)nt * ncol), "assemble alloc RHS");
+// the classified pairs (or the pair grid) into A and S (leading dimension ld);
+// blk > 0: the diagonal blocks of blk x blk pairs only
+static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk) {
   const hm::GeomDev gd = c->geom();
   const hm::FreqDev f = c->freq();
-  double* A = c->d_A.as<double>();
-  double* S = c->d_S.as<double>();
-  c->stage_begin(kAssembly);
   const bool limit = fabs(c->v) < F32(1.e-8) || fabs(c->v + 1.0) < F32(1.e-8);
   if (c->h < 0.0 || limit) {
-    HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, nt), "assemble");
+    HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, ld, blk), "assemble");
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
     const bool sur = c->sur_on && c->sur_ready;
     for (int r = 0; r < (sur ? 2 : 4); ++r) {
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
-      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, nt,
+      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld,
                                     c->series_variant), "assemble");
       c->stage_end(kAsmRegime0 + r);
     }
@@ -455,7 +471,7 @@ extern "C" int hm_assemble(hm_ctx* c) {
       const double kh = c->w1 * c->w1 * c->h / c->g_disp;
       c->stage_begin(kAsmSurrogate);
       HIPCHK(c, hm::assemble_surrogate(c->stream, gd, f, kh, pairs + c->seg_off[2], np, c->sur_dev(), (float)c->sur_tol,
-                                       A, S, nt, fb_e, fb_g, c->d_fbcnt.as<unsigned long long>()), "assemble surrogate");
+                                       A, S, ld, fb_e, fb_g, c->d_fbcnt.as<unsigned long long>()), "assemble surrogate");
       c->stage_end(kAsmSurrogate);
       unsigned long long cnt[2] = {0, 0};
       HIPCHK(c, hipMemcpyAsync(cnt, c->d_fbcnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream), "surrogate");
@@ -469,13 +485,118 @@ extern "C" int hm_assemble(hm_ctx* c) {
         const long long m = (long long)cnt[r - 2];
         if (!m) continue;
         c->stage_begin(kAsmRegime0 + r);
-        HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, r == 2 ? fb_e : fb_g, m, A, S, nt, c->series_variant),
+        HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, r == 2 ? fb_e : fb_g, m, A, S, ld, c->series_variant),
                "assemble fallback");
         c->stage_end(kAsmRegime0 + r);
       }
     }
   }
-  int rc = c->stage_end(kAssembly);
+  return HM_OK;
+}
+
+// H-matrix form: diagonal blocks of A and S densely (the classified pairs of
+// those blocks), the off-diagonal blocks by cross approximation, and the
+// right-hand sides S W through the H-matrix product in FP64
+static int assemble_hmatrix(hm_ctx* c) {
+  const int n = c->n, nb = c->nbody, m = c->bm(), ncol = c->ncol(), kmax = c->hm_kmax;
+  const size_t cplx = 2 * sizeof(double);
+  const size_t tsz = cplx * (size_t)nb * (nb - 1) * kmax * ncol;
+  HIPCHK(c, c->d_hD.ensure(cplx * c->hm_diag_elems()), "hmatrix alloc");
+  HIPCHK(c, c->d_hDS.ensure(cplx * c->hm_diag_elems()), "hmatrix alloc");
+  HIPCHK(c, c->d_hmat.ensure(hm::hmat_bytes(nb, m, kmax)), "hmatrix alloc");
+  HIPCHK(c, c->d_htb.ensure(tsz), "hmatrix alloc");
+  HIPCHK(c, c->d_hts.ensure(tsz), "hmatrix alloc");
+  HIPCHK(c, c->d_W.ensure(cplx * (size_t)n * ncol), "assemble alloc W");
+  HIPCHK(c, c->d_R.ensure(cplx * (size_t)n * ncol), "assemble alloc RHS");
+  c->hmat = hm::hmat_carve(c->d_hmat.p, nb, m, kmax, c->hm_eps, c->hm_eps_s);
+  const hm::GeomDev gd = c->geom();
+  const hm::FreqDev f = c->freq();
+  double* D = c->d_hD.as<double>();
+  double* DS = c->d_hDS.as<double>();
+  c->stage_begin(kAssembly);
+  int rc = assemble_pairs(c, D, DS, m, m);
+  if (rc) return rc;
+  c->stage_begin(kAsmCross);
+  HIPCHK(c, hm::aca_build(c->stream, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
+  c->stage_end(kAsmCross);
+  if ((rc = c->stage_end(kAssembly))) return rc;
+  const int P = hm::hmat_processes(nb);
+  std::vector<int> rank(P);
+  HIPCHK(c, hipMemcpyAsync(rank.data(), c->hmat.rank, sizeof(int) * P, hipMemcpyDeviceToHost, c->stream), "hmatrix");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "hmatrix");
+  long long rs = 0;
+  c->hm_rank_max = 0;
+  for (int r : rank) { rs += r; c->hm_rank_max = r > c->hm_rank_max ? r : c->hm_rank_max; }
+  c->hm_rank_mean = P ? (double)rs / P : 0.0;
+  // pack each matrix's factors at its largest rank, so that the products run
+  // over the ranks the blocks need instead of kmax
+  for (int mat = 0; mat < 2; ++mat) {
+    int ku = 0;
+    for (int p = mat * P / 2; p < (mat + 1) * P / 2; ++p) ku = rank[p] > ku ? rank[p] : ku;
+    c->hm_kuse[mat] = ku;
+    const size_t bytes = cplx * (size_t)nb * (nb - 1) * m * (ku ? ku : 1);
+    HIPCHK(c, c->d_hUc[mat].ensure(bytes), "hmatrix alloc");
+    HIPCHK(c, c->d_hVc[mat].ensure(bytes), "hmatrix alloc");
+    if (ku) HIPCHK(c, hm::hmat_compact(c->stream, c->hmat, mat, ku, c->d_hUc[mat].as<double>(), c->d_hVc[mat].as<double>()),
+                   "hmatrix pack");
+  }
+  // right-hand sides RHS = S * W
+  c->stage_begin(kRhs);
+  HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
+  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_kuse[1], DS, m, (long long)m * (m + 1),
+                                   c->d_hUc[1].as<double>(), c->d_hVc[1].as<double>(), c->d_W.as<double>(), n,
+                                   c->d_R.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
+         "rhs hmatrix product");
+  if ((rc = c->stage_end(kRhs))) return rc;
+  c->assembled = true;
+  c->factored = c->solved = false;
+  return HM_OK;
+}
+
+// H-matrix form of hm_factor: FP32 copies of the diagonal blocks and of A's
+// low-rank factors for the Krylov products, and the block-diagonal
+// preconditioner (one block per body) factored in FP32
+static int factor_hmatrix(hm_ctx* c) {
+  const int nb = c->nbody, m = c->bm(), k = c->hm_kuse[0] ? c->hm_kuse[0] : 1;
+  const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+  const size_t lr = (size_t)nb * (nb - 1) * m * k;   // complex entries of A's packed U (or V)
+  HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+  HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+  HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
+  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "factor alloc");
+  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "factor alloc");
+  const double* D = c->d_hD.as<double>();
+  HIPCHK(c, hm::demote(c->stream, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::demote(c->stream, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1),
+         "demote");
+  HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
+         "demote");
+  HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info.as<int>(),
+                                 c->d_counter.as<unsigned>(), nb), "block preconditioner");
+  c->gm_blocks = c->body_start;
+  if (c->gm_precond == 0) c->gm_blocks.clear();
+  return HM_OK;
+}
+
+extern "C" int hm_assemble(hm_ctx* c) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_freq) return c->fail(HM_ERR_STATE, "hm_assemble before hm_set_frequency");
+  if (c->hm_on) return assemble_hmatrix(c);
+  const int n = c->n, nt = c->nt(), ncol = c->ncol();
+  const size_t cplx = 2 * sizeof(double);
+  HIPCHK(c, c->d_A.ensure(cplx * (size_t)nt * n), "assemble alloc A");
+  HIPCHK(c, c->d_S.ensure(cplx * (size_t)nt * n), "assemble alloc S");
+  HIPCHK(c, c->d_W.ensure(cplx * (size_t)n * ncol), "assemble alloc W");
+  HIPCHK(c, c->d_R.ensure(cplx * (size_t)nt * ncol), "assemble alloc RHS");
+  const hm::GeomDev gd = c->geom();
+  const hm::FreqDev f = c->freq();
+  double* A = c->d_A.as<double>();
+  double* S = c->d_S.as<double>();
+  c->stage_begin(kAssembly);
+  int rc = assemble_pairs(c, A, S, nt, 0);
+  if (rc) return rc;
+  rc = c->stage_end(kAssembly);
   if (rc) return rc;
   // right-hand sides RHS = S * W
   c->stage_begin(kRhs);
@@ -517,7 +638,11 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
   HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nmat, c->stream), "factor");
   HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nmat, c->stream), "factor");
   c->stage_begin(kLu);
-  if (precision == HM_FP64) {
+  if (c->hm_on) {
+    if (precision != HM_GMRES) return c->fail(HM_ERR_ARG - 2, "the H-matrix form solves with HM_GMRES only");
+    int rc = factor_hmatrix(c);
+    if (rc) return rc;
+  } else if (precision == HM_FP64) {
     HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
                                     c->d_counter.as<unsigned>(), 1), "lu fp64");
   } else if (precision == HM_FP32_IR) {
@@ -608,9 +733,28 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
     ga.maxiter = c->gm_maxiter;
     ga.tol = c->gm_tol;
     ga.inner_tol = 0.5 * c->gm_tol;
-    ga.A64 = sys_matrix(c);
-    ga.lda64 = n;
-    ga.A32 = c->d_A32.as<float>();
+    if (c->hm_on) {
+      const int nb = c->nbody, m = c->bm(), k = c->hm_kuse[0];
+      const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+      const float* D32 = c->d_hD32.as<float>();
+      const float* U32 = c->d_hU32.as<float>();
+      const float* V32 = c->d_hV32.as<float>();
+      const double* D = c->d_hD.as<double>();
+      const double* U = c->d_hUc[0].as<double>();
+      const double* V = c->d_hVc[0].as<double>();
+      double* tb = c->d_htb.as<double>();
+      double* ts = c->d_hts.as<double>();
+      ga.op32 = [=](hipStream_t s, const float* Z, float* W) {
+        return hm::hmat_apply<float>(s, nb, m, k, D32, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
+      };
+      ga.op64 = [=](hipStream_t s, const double* Z, double* W) {
+        return hm::hmat_apply<double>(s, nb, m, k, D, m, sD, U, V, Z, n, W, n, ncol, tb, ts);
+      };
+    } else {
+      ga.A64 = sys_matrix(c);
+      ga.lda64 = n;
+      ga.A32 = c->d_A32.as<float>();
+    }
     ga.B = B;
     ga.ldb = ldb;
     ga.X = X;
@@ -673,6 +817,73 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
   c->gm_precond = precond;
   c->factored = c->solved = false;
   return HM_OK;
+}
+
+extern "C" int hm_set_hmatrix(hm_ctx* c, int enable, double eps, double eps_s, int kmax) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_hmatrix before hm_set_geometry");
+  if (enable) {
+    if (c->nbody < 2) return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs at least two bodies");
+    if (c->ni > 0) return c->fail(HM_ERR_ARG - 2, "the H-matrix form takes no interior waterplane points (ni = %d)", c->ni);
+    const int m = c->n / c->nbody;
+    for (int b = 0; b <= c->nbody; ++b)
+      if (c->body_start[b] != b * m)
+        return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs bodies of equal panel counts");
+    if (!(eps > 0.0 && eps < 1.0)) return c->fail(HM_ERR_ARG - 3, "eps must be in (0, 1)");
+    if (!(eps_s > 0.0 && eps_s < 1.0)) return c->fail(HM_ERR_ARG - 4, "eps_s must be in (0, 1)");
+    if (kmax < 1 || kmax > m) return c->fail(HM_ERR_ARG - 5, "kmax must be in 1..%d", m);
+    c->hm_eps = eps;
+    c->hm_eps_s = eps_s;
+    c->hm_kmax = kmax;
+  }
+  const bool was = c->hm_on;
+  c->hm_on = enable != 0;
+  c->assembled = c->factored = c->solved = false;
+  c->hm_unconverged = c->hm_steps = c->hm_rank_max = 0;
+  c->hm_rank_mean = 0.0;
+  if (was != c->hm_on && c->have_depth && c->h > 0.0) return classify(c);
+  return HM_OK;
+}
+
+extern "C" int hm_get_hmatrix_stats(hm_ctx* c, int* unconverged, int* steps, double* mean_rank, int* max_rank,
+                                    double* stored_fraction) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (unconverged) *unconverged = c->hm_unconverged;
+  if (steps) *steps = c->hm_steps;
+  if (mean_rank) *mean_rank = c->hm_rank_mean;
+  if (max_rank) *max_rank = c->hm_rank_max;
+  if (stored_fraction) {
+    // entries held per matrix (dense diagonal blocks + U and V at the actual
+    // ranks) over the n^2 of the dense matrix
+    const double m = c->nbody ? (double)c->n / c->nbody : 0.0;
+    const double off = (double)c->nbody * (c->nbody - 1) * 2.0 * m * c->hm_rank_mean;
+    *stored_fraction = c->hm_on && c->n ? (c->nbody * m * m + off) / ((double)c->n * c->n) : 1.0;
+  }
+  return HM_OK;
+}
+
+extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, double* y) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->hm_on || !c->assembled) return c->fail(HM_ERR_STATE, "hm_hmatrix_apply needs an assembled H-matrix form");
+  if (mat != 0 && mat != 1) return c->fail(HM_ERR_ARG - 2, "mat must be 0 (A) or 1 (S)");
+  if (ncol < 1 || ncol > c->ncol()) return c->fail(HM_ERR_ARG - 3, "ncol must be in 1..%d", c->ncol());
+  if (!x || !y) return c->fail(HM_ERR_ARG - 4, "x or y is NULL");
+  const int n = c->n, nb = c->nbody, m = c->bm();
+  const size_t bytes = 2 * sizeof(double) * (size_t)n * ncol;
+  DevBuf dx, dy;
+  int rc = HM_OK;
+  if ((rc = c->hip(dx.ensure(bytes), "hmatrix apply")) == HM_OK && (rc = c->hip(dy.ensure(bytes), "hmatrix apply")) == HM_OK &&
+      (rc = c->hip(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, c->stream), "hmatrix apply")) == HM_OK &&
+      (rc = c->hip(hm::hmat_apply<double>(c->stream, nb, m, c->hm_kuse[mat],
+                                          mat ? c->d_hDS.as<double>() : c->d_hD.as<double>(), m, (long long)m * (m + 1),
+                                          c->d_hUc[mat].as<double>(), c->d_hVc[mat].as<double>(), dx.as<double>(), n,
+                                          dy.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
+                   "hmatrix apply")) == HM_OK &&
+      (rc = c->hip(hipMemcpyAsync(y, dy.p, bytes, hipMemcpyDeviceToHost, c->stream), "hmatrix apply")) == HM_OK)
+    rc = c->hip(hipStreamSynchronize(c->stream), "hmatrix apply");
+  dx.release();
+  dy.release();
+  return rc;
 }
 
 extern "C" int hm_set_series_variant(hm_ctx* c, int variant) {
@@ -1043,6 +1254,7 @@ extern "C" int hm_get_pair_histogram(hm_ctx* c, long long* counts, int nkeys) {
 extern "C" int hm_get_system(hm_ctx* c, double* amat, double* smat, double* rhs) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->assembled || c->factored) return c->fail(HM_ERR_STATE, "hm_get_system needs an assembled, unfactored system");
+  if (c->hm_on) return c->fail(HM_ERR_STATE, "the H-matrix form holds no dense system");
   const size_t cplx = 2 * sizeof(double), nt = c->nt();
   if (amat) HIPCHK(c, hipMemcpyAsync(amat, c->d_A.p, cplx * nt * c->n, hipMemcpyDeviceToHost, c->stream), "get system");
   if (smat) HIPCHK(c, hipMemcpyAsync(smat, c->d_S.p, cplx * nt * c->n, hipMemcpyDeviceToHost, c->stream), "get system");

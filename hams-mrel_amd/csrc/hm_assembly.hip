@@ -61,12 +61,15 @@ __device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f
 
 // 2-D grid over all pairs (infinite depth, or finite depth with dispatch per pair)
 template <int REG>
-__global__ __launch_bounds__(256) void k_assemble_grid(GeomDev g, FreqDev f, double* A, double* S, long long ld) {
+__global__ __launch_bounds__(256) void k_assemble_grid(GeomDev g, FreqDev f, double* A, double* S, long long ld,
+                                                       int blk) {
   __shared__ double wvn[kNk];
   if (REG != -1) load_wvn(wvn, f.wvn, kNk);
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i >= g.nt || j >= g.n) return;
+  // blk > 0: only the diagonal blocks of blk x blk pairs (blockIdx.z = block)
+  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i >= (blk > 0 ? b0 + blk : g.nt) || j >= (blk > 0 ? b0 + blk : g.n)) return;
   assemble_pair<REG>(g, f, wvn, i, j, A, S, ld);
 }
 
@@ -170,40 +173,47 @@ __device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j
   return reg * kKeyStride + len;
 }
 
-__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist) {
+__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk) {
   __shared__ unsigned sh[kNumKeys];
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
   __syncthreads();
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i < g.nt && j < g.n) atomicAdd(&sh[pair_key(g, h, i, j)], 1u);
+  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n)) atomicAdd(&sh[pair_key(g, h, i, j)], 1u);
   __syncthreads();
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
     if (sh[k]) atomicAdd(&hist[k], sh[k]);
 }
 
 __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, unsigned long long* cursor,
-                                                          uint32_t* pairs) {
-  const int i = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const int j = blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i >= g.nt || j >= g.n) return;
+                                                          uint32_t* pairs, int blk) {
+  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i >= (blk > 0 ? b0 + blk : g.nt) || j >= (blk > 0 ? b0 + blk : g.n)) return;
   const unsigned long long pos = atomicAdd(&cursor[pair_key(g, h, i, j)], 1ull);
   pairs[pos] = ((uint32_t)i << 16) | (uint32_t)j;
 }
 
+static dim3 pair_grid(const GeomDev& g, int blk) {
+  if (blk > 0) return dim3((blk + 15) / 16, (blk + 15) / 16, g.n / blk);
+  return dim3((g.n + 15) / 16, (g.nt + 15) / 16);
+}
+
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist,
-                          unsigned long long* cursor, uint32_t* pairs, bool count_only) {
-  dim3 grid((g.n + 15) / 16, (g.nt + 15) / 16);
-  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist);
-  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs);
+                          unsigned long long* cursor, uint32_t* pairs, bool count_only, int blk) {
+  const dim3 grid = pair_grid(g, blk);
+  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist, blk);
+  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs, blk);
   return hipGetLastError();
 }
 
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
-                         long long ld) {
-  dim3 grid((g.n + 15) / 16, (g.nt + 15) / 16);
-  if (deep) hipLaunchKernelGGL((k_assemble_grid<-1>), grid, dim3(256), 0, s, g, f, A, S, ld);
-  else hipLaunchKernelGGL((k_assemble_grid<-2>), grid, dim3(256), 0, s, g, f, A, S, ld);
+                         long long ld, int blk) {
+  const dim3 grid = pair_grid(g, blk);
+  if (deep) hipLaunchKernelGGL((k_assemble_grid<-1>), grid, dim3(256), 0, s, g, f, A, S, ld, blk);
+  else hipLaunchKernelGGL((k_assemble_grid<-2>), grid, dim3(256), 0, s, g, f, A, S, ld, blk);
   return hipGetLastError();
 }
 

@@ -28,10 +28,11 @@ struct FreqDev {
   const double* wvn;       // kNk roots on the device
 };
 
+// blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0)
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist, unsigned long long* cursor,
-                          uint32_t* pairs, bool count_only);
+                          uint32_t* pairs, bool count_only, int blk = 0);
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
-                         long long ld);
+                         long long ld, int blk = 0);
 hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
                            long long np, double* A, double* S, long long ld, int variant = 1);
 hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,

@@ -298,6 +298,16 @@ static hipError_t for_block_runs(int nblk, const int* blk_start, F&& f) {
   return hipSuccess;
 }
 
+// R = B - W (column c of B at B + c ldb, of W and R at + c n)
+__global__ void k_sub(const double* __restrict__ B, long long ldb, const double* __restrict__ W, int n,
+                      double* __restrict__ R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (i >= n) return;
+  const long long o = (long long)c * n + i, ob = (long long)c * ldb + i;
+  R[2 * o] = B[2 * ob] - W[2 * o];
+  R[2 * o + 1] = B[2 * ob + 1] - W[2 * o + 1];
+}
+
 hipError_t gmres_block_precond_setup(hipStream_t s, const double* A, int lda, int nblk, const int* blk_start,
                                      float* P, int* pipiv, int* info, unsigned* counter) {
   return for_block_runs(nblk, blk_start, [&](int r0, int nb, int cnt, size_t off) {
@@ -374,7 +384,8 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       // W = A * M^{-1} v_j on MFMA in FP32
       GM_CHECK(demote(s, vj, n, 0, Z32, n, 0, n, ncol, 1));
       if (a.nblk > 0) GM_CHECK(apply_precond(s, a, Z32, ncol));
-      GM_CHECK(cgemm<float>(s, false, n, ncol, n, 1.f, 0.f, a.A32, n, 0, Z32, n, 0, 0.f, 0.f, W32, n, 0, 1));
+      if (a.op32) GM_CHECK(a.op32(s, Z32, W32));
+      else GM_CHECK(cgemm<float>(s, false, n, ncol, n, 1.f, 0.f, a.A32, n, 0, Z32, n, 0, 0.f, 0.f, W32, n, 0, 1));
       GM_CHECK(promote_add(s, W32, n, 0, W, n, 0, n, ncol, 1, true));
       // classical Gram-Schmidt, twice
       hipLaunchKernelGGL(k_dots, dim3(j + 1, ncol), dim3(256), 0, s, V, vstride, W, n, active, m1, hc1);
@@ -404,8 +415,13 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       GM_CHECK(hipGetLastError());
     }
     // FP64 residual R = B - A X
-    GM_CHECK(hipMemcpy2DAsync(R, cx * n, a.B, cx * a.ldb, cx * n, ncol, hipMemcpyDeviceToDevice, s));
-    GM_CHECK(cgemm<double>(s, false, n, ncol, n, -1.0, 0.0, a.A64, a.lda64, 0, a.X, n, 0, 1.0, 0.0, R, n, 0, 1));
+    if (a.op64) {
+      GM_CHECK(a.op64(s, a.X, W));
+      hipLaunchKernelGGL(k_sub, gcol, dim3(256), 0, s, a.B, (long long)a.ldb, W, n, R);
+    } else {
+      GM_CHECK(hipMemcpy2DAsync(R, cx * n, a.B, cx * a.ldb, cx * n, ncol, hipMemcpyDeviceToDevice, s));
+      GM_CHECK(cgemm<double>(s, false, n, ncol, n, -1.0, 0.0, a.A64, a.lda64, 0, a.X, n, 0, 1.0, 0.0, R, n, 0, 1));
+    }
     hipLaunchKernelGGL(k_colnorm, dim3(ncol), dim3(256), 0, s, R, (long long)n, n, rnorm);
     GM_CHECK(hipGetLastError());
   }

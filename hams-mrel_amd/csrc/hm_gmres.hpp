@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <functional>
+
 namespace hm {
 
 constexpr int kGmresMaxRestart = 128;
@@ -23,6 +25,10 @@ struct GmresArgs {
   const int* blk_start = nullptr;
   const float* P = nullptr;     // LU factors of the blocks, packed one after the other
   const int* pipiv = nullptr;   // block-local pivots at the block's row offset
+  // operator form (used instead of A32 / A64 when set): op32 writes W = A Z in
+  // FP32, op64 writes W = A X in FP64, both n x ncol with leading dimension n
+  std::function<hipError_t(hipStream_t, const float*, float*)> op32;
+  std::function<hipError_t(hipStream_t, const double*, double*)> op64;
 };
 
 struct GmresStats {

@@ -1,0 +1,403 @@
+// hm_hmatrix.hip -- cross approximation of the off-diagonal influence blocks
+// and the H-matrix products (see hm_hmatrix.hpp).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "hm_device.hpp"
+#include "hm_hmatrix.hpp"
+#include "hm_linalg.hpp"
+#include "hm_pairs.hpp"
+
+namespace hm {
+
+namespace {
+
+struct Proc {
+  int mat, b1, b2;
+  double* U;
+  double* V;
+};
+__device__ __forceinline__ Proc proc(const HmatDev& h, int p) {
+  const int nb = h.nb, per = nb * (nb - 1);
+  Proc q;
+  q.mat = p / per;
+  const int r = p % per;
+  q.b1 = r / (nb - 1);
+  const int b2p = r % (nb - 1);
+  q.b2 = b2p + (b2p >= q.b1 ? 1 : 0);
+  const int b1p = q.b1 - (q.b1 > q.b2 ? 1 : 0);
+  const long long blk = 2LL * h.m * h.kmax;
+  q.U = h.U + ((long long)(q.mat * nb + q.b1) * (nb - 1) + b2p) * blk;
+  q.V = h.V + ((long long)(q.mat * nb + q.b2) * (nb - 1) + b1p) * blk;
+  return q;
+}
+
+__device__ __forceinline__ void entry(const GeomDev& g, const FreqDev& f, const double* wvn, int mat, int i, int j,
+                                      double& er, double& ei) {
+  double ar, ai, sr, si;
+  pair_entry(g, f, wvn, i, j, ar, ai, sr, si);
+  er = mat ? sr : ar;
+  ei = mat ? si : ai;
+}
+
+// block-wide argmax of v (first index on ties); v < 0 marks "not a candidate"
+__device__ void block_argmax(double v, int idx, double* sv, int* si, double& best, int& bidx) {
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = idx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = sv[threadIdx.x + s];
+      const int oi = si[threadIdx.x + s];
+      if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = o;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  best = sv[0];
+  bidx = si[0];
+  __syncthreads();
+}
+
+// block-wide sum of 4 doubles
+__device__ void block_sum4(double* v, double (*sh)[256]) {
+  for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  for (int c = 0; c < 4; ++c) v[c] = sh[c][0];
+  __syncthreads();
+}
+
+// residual of one entry: A(b1 m + i, b2 m + j) - sum_{l<k} U(i, l) V(j, l)
+__device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const Proc& q,
+                                               int m, int k, int i, int j, double& er, double& ei) {
+  entry(g, f, wvn, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
+  for (int l = 0; l < k; ++l) {
+    const double ur = q.U[2 * (i + (long long)l * m)], ui = q.U[2 * (i + (long long)l * m) + 1];
+    const double vr = q.V[2 * (j + (long long)l * m)], vi = q.V[2 * (j + (long long)l * m) + 1];
+    er -= ur * vr - ui * vi;
+    ei -= ur * vi + ui * vr;
+  }
+}
+
+// residual pivot rows of every unconverged process, one entry per thread
+// (grid: column chunks x processes)
+__global__ __launch_bounds__(256) void k_aca_eval_row(GeomDev g, FreqDev f, HmatDev h) {
+  const int p = blockIdx.y;
+  if (h.conv[p]) return;
+  __shared__ double wvn[kNk];
+  if (f.h > 0.0)
+    for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= h.m) return;
+  const Proc q = proc(h, p);
+  double er, ei;
+  residual_entry(g, f, wvn, q, h.m, h.rank[p], h.next_row[p], j, er, ei);
+  double* rb = h.rowbuf + 2LL * p * h.m;
+  rb[2 * j] = er;
+  rb[2 * j + 1] = ei;
+}
+
+// pivot column j* = argmax |r| over unused columns, v_k = r / r(j*); a zero
+// residual row is retired and the pivot row moves to the next unused row
+__global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
+  const int p = blockIdx.x;
+  if (h.conv[p]) return;
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const Proc q = proc(h, p);
+  const int m = h.m, k = h.rank[p], i = h.next_row[p];
+  const double* rb = h.rowbuf + 2LL * p * m;
+  const int* cu = h.col_used + (long long)p * m;
+  double best = -1.0;
+  int bj = m;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const double a = rb[2 * j] * rb[2 * j] + rb[2 * j + 1] * rb[2 * j + 1];
+    if (!cu[j] && a > best) { best = a; bj = j; }
+  }
+  double emax;
+  int js;
+  block_argmax(best, bj, sv, si, emax, js);
+  if (!(emax > 0.0) || js >= m) {
+    if (threadIdx.x == 0) {
+      int* ru = h.row_used + (long long)p * m;
+      ru[i] = 1;
+      int nr = -1;
+      for (int t = 1; t <= m; ++t)
+        if (!ru[(i + t) % m]) { nr = (i + t) % m; break; }
+      if (nr < 0) h.conv[p] = 1;
+      else h.next_row[p] = nr;
+      h.skip[p] = 1;
+    }
+    return;
+  }
+  const double pr = rb[2 * js], pi = rb[2 * js + 1], den = pr * pr + pi * pi;
+  const double ir = pr / den, ii = -pi / den;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const double er = rb[2 * j], ei = rb[2 * j + 1];
+    q.V[2 * (j + (long long)k * m)] = er * ir - ei * ii;
+    q.V[2 * (j + (long long)k * m) + 1] = er * ii + ei * ir;
+  }
+  if (threadIdx.x == 0) {
+    h.pivot[p] = js;
+    h.col_used[(long long)p * m + js] = 1;
+    h.skip[p] = 0;
+  }
+}
+
+// residual pivot columns u_k = A(:, j*) - U V(j*, :)^T, one entry per thread
+__global__ __launch_bounds__(256) void k_aca_eval_col(GeomDev g, FreqDev f, HmatDev h) {
+  const int p = blockIdx.y;
+  if (h.conv[p] || h.skip[p]) return;
+  __shared__ double wvn[kNk];
+  if (f.h > 0.0)
+    for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= h.m) return;
+  const Proc q = proc(h, p);
+  const int k = h.rank[p];
+  double er, ei;
+  residual_entry(g, f, wvn, q, h.m, k, i, h.pivot[p], er, ei);
+  q.U[2 * (i + (long long)k * h.m)] = er;
+  q.U[2 * (i + (long long)k * h.m) + 1] = ei;
+}
+
+// norm estimate, stopping test and next pivot row = argmax |u_k| over unused rows
+__global__ __launch_bounds__(256) void k_aca_finish_col(HmatDev h) {
+  const int p = blockIdx.x;
+  if (h.conv[p] || h.skip[p]) return;
+  __shared__ double sh[4][256];
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const Proc q = proc(h, p);
+  const int m = h.m, k = h.rank[p], ir0 = h.next_row[p];
+  const double* uk = q.U + 2LL * k * m;
+  const double* vk = q.V + 2LL * k * m;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < m; t += blockDim.x) {
+    acc[0] += uk[2 * t] * uk[2 * t] + uk[2 * t + 1] * uk[2 * t + 1];
+    acc[1] += vk[2 * t] * vk[2 * t] + vk[2 * t + 1] * vk[2 * t + 1];
+  }
+  block_sum4(acc, sh);
+  const double nu2 = acc[0], nv2 = acc[1];
+  // |sum_l u_l v_l^T|_F^2 = sum_{l,l'} (u_l^H u_l')(v_l^H v_l'): add the new
+  // diagonal term and twice the real part of the cross terms with l < k, one
+  // wave per l
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double cross = 0.0;
+  for (int l = wv; l < k; l += nw) {
+    const double* ul = q.U + 2LL * l * m;
+    const double* vl = q.V + 2LL * l * m;
+    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+    for (int t = lane; t < m; t += 64) {
+      c0 += ul[2 * t] * uk[2 * t] + ul[2 * t + 1] * uk[2 * t + 1];    // conj(u_l) u_k
+      c1 += ul[2 * t] * uk[2 * t + 1] - ul[2 * t + 1] * uk[2 * t];
+      c2 += vl[2 * t] * vk[2 * t] + vl[2 * t + 1] * vk[2 * t + 1];    // conj(v_l) v_k
+      c3 += vl[2 * t] * vk[2 * t + 1] - vl[2 * t + 1] * vk[2 * t];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      c0 += __shfl_xor(c0, o);
+      c1 += __shfl_xor(c1, o);
+      c2 += __shfl_xor(c2, o);
+      c3 += __shfl_xor(c3, o);
+    }
+    cross += c0 * c2 - c1 * c3;
+  }
+  {
+    double c[4] = {lane == 0 ? cross : 0.0, 0.0, 0.0, 0.0};
+    block_sum4(c, sh);
+    cross = c[0];
+  }
+  const double n2 = h.norm2[p] + nu2 * nv2 + 2.0 * cross;
+  // next pivot row
+  const int* ru = h.row_used + (long long)p * m;
+  double best = -1.0;
+  int bi = m;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double a = uk[2 * i] * uk[2 * i] + uk[2 * i + 1] * uk[2 * i + 1];
+    if (!ru[i] && i != ir0 && a > best) { best = a; bi = i; }
+  }
+  double bmax;
+  int bidx;
+  block_argmax(best, bi, sv, si, bmax, bidx);
+  if (threadIdx.x == 0) {
+    h.row_used[(long long)p * m + ir0] = 1;
+    h.norm2[p] = n2;
+    h.rank[p] = k + 1;
+    const bool small = sqrt(nu2 * nv2) < (q.mat ? h.eps_s : h.eps) * sqrt(fmax(n2, 1e-30));
+    if (small) h.conv[p] = 1;
+    else if (bidx >= m) h.conv[p] = 1;        // every row used: the approximation is exact
+    else if (k + 1 >= h.kmax) h.conv[p] = 2;  // rank cap reached before the tolerance
+    else h.next_row[p] = bidx;
+  }
+}
+
+__global__ void k_count_open(const int* conv, int n, int* out) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) mine += conv[p] != 1;
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = cnt;
+}
+
+__global__ void k_count_active(const int* conv, int n, int* out) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) mine += conv[p] == 0;
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = cnt;
+}
+
+// Tstack[b1][b2' kmax + r, c] = Tbuf[b2][b1' kmax + r, c]
+template <typename T>
+__global__ void k_permute(int nb, int kmax, int ncol, const T* __restrict__ tb, T* __restrict__ ts) {
+  const long long rows = (long long)(nb - 1) * kmax;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nb * rows * ncol;
+  if (e >= total) return;
+  const int b1 = (int)(e / (rows * ncol));
+  const long long rem = e % (rows * ncol);
+  const int c = (int)(rem / rows);
+  const int rr = (int)(rem % rows);
+  const int b2p = rr / kmax, r = rr % kmax;
+  const int b2 = b2p + (b2p >= b1 ? 1 : 0);
+  const int b1p = b1 - (b1 > b2 ? 1 : 0);
+  const long long src = (long long)b2 * rows * ncol + (long long)c * rows + (long long)b1p * kmax + r;
+  const long long dst = (long long)b1 * rows * ncol + (long long)c * rows + rr;
+  ts[2 * dst] = tb[2 * src];
+  ts[2 * dst + 1] = tb[2 * src + 1];
+}
+
+}  // namespace
+
+size_t hmat_bytes(int nb, int m, int kmax) {
+  const size_t P = (size_t)hmat_processes(nb);
+  size_t b = 0;
+  b += 2 * 16 * P * m * kmax;      // U, V (complex FP64)
+  b += 16 * P * m;                 // rowbuf
+  b += 8 * P;                      // norm2
+  b += 4 * 5 * P;                  // rank, conv, next_row, pivot, skip
+  b += 4 * 2 * P * m;              // row_used, col_used
+  b += 4;                          // counter
+  return b + 13 * 256;             // alignment of the 13 arrays
+}
+
+HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s) {
+  const size_t P = (size_t)hmat_processes(nb);
+  char* p = static_cast<char*>(base);
+  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~(size_t)255; return q; };
+  HmatDev h;
+  h.nb = nb;
+  h.m = m;
+  h.kmax = kmax;
+  h.eps = eps;
+  h.eps_s = eps_s;
+  h.U = (double*)take(16 * P * m * kmax);
+  h.V = (double*)take(16 * P * m * kmax);
+  h.rowbuf = (double*)take(16 * P * m);
+  h.norm2 = (double*)take(8 * P);
+  h.rank = (int*)take(4 * P);
+  h.conv = (int*)take(4 * P);
+  h.next_row = (int*)take(4 * P);
+  h.pivot = (int*)take(4 * P);
+  h.skip = (int*)take(4 * P);
+  h.row_used = (int*)take(4 * P * m);
+  h.col_used = (int*)take(4 * P * m);
+  h.counter = (int*)take(4);
+  return h;
+}
+
+hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
+                     int* steps) {
+  const int P = hmat_processes(h.nb);
+  const size_t uv = 16 * (size_t)P * h.m * h.kmax;
+  hipError_t e;
+  if ((e = hipMemsetAsync(h.U, 0, uv, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.V, 0, uv, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.norm2, 0, 8 * (size_t)P, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.rank, 0, 4 * (size_t)P, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.conv, 0, 4 * (size_t)P, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.next_row, 0, 4 * (size_t)P, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.skip, 0, 4 * (size_t)P, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.row_used, 0, 4 * (size_t)P * h.m, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(h.col_used, 0, 4 * (size_t)P * h.m, s)) != hipSuccess) return e;
+  int* cnt = h.counter;
+  int open = P, st = 0;
+  // each row step either adds a rank (then a column step) or retires a zero row;
+  // at most kmax + m row steps per process
+  const int max_steps = h.kmax + h.m;
+  for (; st < max_steps && open > 0; ++st) {
+    const dim3 eval((h.m + 255) / 256, P);
+    hipLaunchKernelGGL(k_aca_eval_row, eval, dim3(256), 0, s, g, f, h);
+    hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
+    hipLaunchKernelGGL(k_aca_eval_col, eval, dim3(256), 0, s, g, f, h);
+    hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (st % 4 == 3) {
+      hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, h.conv, P, cnt);
+      if ((e = hipMemcpyAsync(&open, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+      if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL(k_count_open, dim3(1), dim3(256), 0, s, h.conv, P, cnt);
+  int bad = 0;
+  if ((e = hipMemcpyAsync(&bad, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (unconverged) *unconverged = bad;
+  if (steps) *steps = st;
+  return hipSuccess;
+}
+
+hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, double* Uc, double* Vc) {
+  const size_t nblk = (size_t)h.nb * (h.nb - 1);
+  const size_t src_pitch = 16 * (size_t)h.m * h.kmax, dst_pitch = 16 * (size_t)h.m * kuse;
+  const double* U = h.U + 2 * (size_t)mat * nblk * h.m * h.kmax;
+  const double* V = h.V + 2 * (size_t)mat * nblk * h.m * h.kmax;
+  hipError_t e = hipMemcpy2DAsync(Uc, dst_pitch, U, src_pitch, dst_pitch, nblk, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipMemcpy2DAsync(Vc, dst_pitch, V, src_pitch, dst_pitch, nblk, hipMemcpyDeviceToDevice, s);
+}
+
+template <typename T>
+hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long ldd, long long sD, const T* U,
+                      const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack) {
+  const long long rows = (long long)(nb - 1) * k;
+  hipError_t e;
+  // diagonal blocks
+  e = cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb);
+  if (e != hipSuccess || k == 0) return e;
+  // T_b2 = [V_{b1,b2}]_b1^T x_b2 for every column block b2
+  e = cgemm<T>(s, true, (int)rows, ncol, m, T(1), T(0), V, m, rows * m, X, (int)ldx, m, T(0), T(0), tbuf,
+               (int)rows, rows * ncol, nb);
+  if (e != hipSuccess) return e;
+  const long long tot = (long long)nb * rows * ncol;
+  hipLaunchKernelGGL((k_permute<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nb, k, ncol, tbuf,
+                     tstack);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // y_b1 += [U_{b1,b2}]_b2 [T_{b1,b2}]_b2
+  return cgemm<T>(s, false, m, ncol, (int)rows, T(1), T(0), U, m, rows * m, tstack, (int)rows, rows * ncol, T(1),
+                  T(0), Y, (int)ldy, m, nb);
+}
+
+template hipError_t hmat_apply<double>(hipStream_t, int, int, int, const double*, long long, long long, const double*,
+                                       const double*, const double*, long long, double*, long long, int, double*,
+                                       double*);
+template hipError_t hmat_apply<float>(hipStream_t, int, int, int, const float*, long long, long long, const float*,
+                                      const float*, const float*, long long, float*, long long, int, float*, float*);
+
+}  // namespace hm

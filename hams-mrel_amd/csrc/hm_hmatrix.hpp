@@ -1,0 +1,68 @@
+// hm_hmatrix.hpp -- hierarchical (H-) matrix form of the influence matrices.
+//
+// Replaces HMATRIX_BUILD / HMATVEC (HMatrix.f90:135-196, :760-820) for the
+// multi-body arrays: one cluster per body (bodies of equal panel count m). The
+// diagonal (same-body) blocks are assembled densely; every off-diagonal block
+// is admissible (bodies are separated) and is compressed by adaptive cross
+// approximation with partial pivoting (ADD_LOWRANK_BLOCK :560-748) to
+// A_blk ~= U V^T (plain transpose, as the influence matrices are complex
+// symmetric-free), stopping when |u_k| |v_k| < eps |U V^T|_F (ACA_TOL = 1e-4,
+// RANK_MAX = 50 by default, :58-60). The Frobenius norm estimate carries the
+// exact cross terms 2 Re sum_l (u_l^H u_k)(v_l^H v_k); a zero residual row
+// moves the pivot to the next unused row instead of ending the block.
+//
+// Unlike the reference, which builds the H-matrix from the assembled dense
+// AMAT, the cross approximation here evaluates only the rows and columns it
+// samples, directly with the Green function and Rankine kernels, so the
+// off-diagonal blocks are never assembled. A and S (for the right-hand sides
+// S W) are compressed independently. All processes (matrix x block) advance in
+// lockstep: one launch evaluates the pivot row of every unconverged process,
+// the next its pivot column.
+//
+// Products: y = A x for n x ncol blocks x is
+//   y_b  = D_b x_b                                   (batched GEMM, diagonal blocks)
+//   T_b2 = [V_{b1,b2}]_b1^T x_b2                      (batched GEMM per column block)
+//   y_b1 += [U_{b1,b2}]_b2 [T_{b1,b2}]_b2               (batched GEMM per row block)
+// in FP32 on MFMA for the GMRES Krylov products and in FP64 for the residuals
+// and the right-hand sides.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hm_assembly.hpp"
+
+namespace hm {
+
+struct HmatDev {
+  int nb, m, kmax;
+  double eps, eps_s;   // stopping tolerance for A (mat 0) and S (mat 1)
+  double* U;         // [mat][b1][b2'] m x kmax, ld m   (b2' = b2 without b1)
+  double* V;         // [mat][b2][b1'] m x kmax, ld m   (b1' = b1 without b2)
+  double* rowbuf;    // [process][m] complex
+  double* norm2;     // [process]
+  int *rank, *conv, *next_row, *pivot, *skip;   // [process]
+  int *row_used, *col_used;                     // [process][m]
+  int* counter;                                 // one int for host polls
+};
+
+inline int hmat_processes(int nb) { return 2 * nb * (nb - 1); }
+size_t hmat_bytes(int nb, int m, int kmax);   // device memory for HmatDev arrays
+HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s);
+
+// cross approximation of all off-diagonal blocks of A (mat 0) and S (mat 1);
+// returns the number of unconverged processes in *unconverged
+hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
+                     int* steps);
+
+// the first kuse columns of the U and V factors of matrix `mat` (0: A, 1: S),
+// packed with stride m kuse per block (same block order) into Uc, Vc
+hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, double* Uc, double* Vc);
+
+// Y = H X: diagonal blocks D (block b at D + b sD complex elements,
+// column-major, ld ldd) plus the low-rank blocks of one matrix, packed as by
+// hmat_compact with rank stride k. X, Y: n x ncol (n = nb m); tbuf, tstack:
+// nb (nb-1) k ncol complex each.
+template <typename T>
+hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long ldd, long long sD, const T* U,
+                      const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack);
+
+}  // namespace hm

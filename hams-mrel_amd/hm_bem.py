@@ -17,7 +17,7 @@ HM_FP32_IR = 1
 HM_GMRES = 2
 SUR_PARAMS = 12995          # network parameters (see include/hm_bem.h)
 STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces",
-          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen", "asm_surrogate")
+          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen", "asm_surrogate", "asm_cross")
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -32,7 +32,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
-           "hm_surrogate_grad")
+           "hm_surrogate_grad", "hm_set_hmatrix", "hm_get_hmatrix_stats",
+           "hm_hmatrix_apply")
 
 _lib = None
 
@@ -234,6 +235,29 @@ class Context:
         self._chk(self.L.hm_surrogate_grad(self.ptr, _i(len(x)), _p(x), _p(u), _i(1 if quant else 0), _p(g),
                                            ctypes.byref(loss)), "hm_surrogate_grad")
         return g[:SUR_PARAMS], loss.value
+
+    def set_hmatrix(self, enable=True, eps=1e-4, eps_s=None, kmax=50):
+        """eps: ACA tolerance of A; eps_s: of S (default eps)"""
+        self._chk(self.L.hm_set_hmatrix(self.ptr, _i(1 if enable else 0), _d(eps), _d(eps if eps_s is None else eps_s),
+                                        _i(kmax)), "hm_set_hmatrix")
+
+    def hmatrix_stats(self):
+        un, st, mx = _i(), _i(), _i()
+        mr, fr = _d(), _d()
+        self._chk(self.L.hm_get_hmatrix_stats(self.ptr, ctypes.byref(un), ctypes.byref(st), ctypes.byref(mr),
+                                              ctypes.byref(mx), ctypes.byref(fr)), "hm_get_hmatrix_stats")
+        return dict(unconverged=un.value, steps=st.value, mean_rank=mr.value, max_rank=mx.value,
+                    stored_fraction=fr.value)
+
+    def hmatrix_apply(self, mat, x):
+        """H-matrix product of A (mat 0) or S (mat 1) with x (n x ncol complex)"""
+        x = np.asarray(x, dtype=np.complex128)
+        if x.ndim == 1:
+            x = x[:, None]
+        xf = np.asfortranarray(x)
+        y = np.zeros_like(xf, order="F")
+        self._chk(self.L.hm_hmatrix_apply(self.ptr, _i(mat), _i(x.shape[1]), _p(xf), _p(y)), "hm_hmatrix_apply")
+        return y
 
     def set_series_variant(self, variant):
         self._chk(self.L.hm_set_series_variant(self.ptr, _i(variant)), "hm_set_series_variant")

@@ -88,6 +88,26 @@ int hm_forces(hm_ctx *ctx, double *amas, double *bdmp, double *exfc);
  * INFO = 1). */
 int hm_set_gmres(hm_ctx *ctx, int restart, double tol, int maxiter, int precond);
 int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, double *rel_residual);
+/* H-matrix form of A and S for arrays of equal bodies without interior
+ * waterplane points, replacing HMATRIX_BUILD / HMATVEC (HMatrix.f90:135,:760)
+ * as ZITER_SOLVE_MULTI uses them. One cluster per body: the diagonal blocks are
+ * assembled densely, every off-diagonal block by adaptive cross approximation
+ * with partial pivoting (ADD_LOWRANK_BLOCK :560) that evaluates only the rows
+ * and columns it samples, stopping at |u_k||v_k| < eps |U V^T|_F or rank kmax
+ * (ACA_TOL, RANK_MAX :59-60; defaults eps 1e-4, kmax 50). The right-hand sides
+ * come from the H-matrix of S in FP64, compressed to its own tolerance eps_s
+ * (the reference forms S W densely). Solves with precision HM_GMRES only;
+ * hm_get_system is unavailable. Toggling the form reclassifies the pairs;
+ * hm_set_geometry turns it off. hm_get_hmatrix_stats: blocks that stopped at
+ * kmax before eps, ACA steps, mean and largest rank, and stored entries per
+ * matrix over n^2, for the last assembly. */
+int hm_set_hmatrix(hm_ctx *ctx, int enable, double eps, double eps_s, int kmax);
+int hm_get_hmatrix_stats(hm_ctx *ctx, int *unconverged, int *steps, double *mean_rank, int *max_rank,
+                         double *stored_fraction);
+/* y = H x in FP64 for the assembled H-matrix form of A (mat 0) or S (mat 1);
+ * x, y: n x ncol column-major complex (HMATVEC, HMatrix.f90:760, for several
+ * columns). */
+int hm_hmatrix_apply(hm_ctx *ctx, int mat, int ncol, const double *x, double *y);
 /* set_frequency + assemble + factor + solve + forces */
 int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, double *bdmp, double *exfc);
 /* nfreq frequencies back to back; outputs stacked per frequency */
@@ -161,7 +181,8 @@ int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, in
  * counts[stage], for stages 0 classify, 1 assembly (all regimes), 2 RHS GEMM,
  * 3 IRR normal equations, 4 LU, 5 triangular solves + refinement, 6 forces,
  * 7..10 the assembly kernel of each finite-depth regime (Linton, Pidcock,
- * epsilon-accelerated, eigenfunction), 11 the surrogate assembly kernel.
+ * epsilon-accelerated, eigenfunction), 11 the surrogate assembly kernel, 12
+ * the cross approximation of the H-matrix form.
  * hm_set_timing resets the totals. */
 int hm_set_timing(hm_ctx *ctx, int enable);
 int hm_get_timing(hm_ctx *ctx, double *ms, int *counts, int nstage);
