@@ -438,6 +438,23 @@ extern "C" int hm_set_frequency(hm_ctx* c, double w1) {
   return HM_OK;
 }
 
+// zero (which = 0) or infinite (which = 1) wave frequency: CalWaveProperts with
+// SYBO = 1 (ImplementSubs.f90:45-70) sets V = WK = W1 = 0 or -1 and skips the
+// dispersion roots, which the limits of the Green functions do not read
+extern "C" int hm_set_frequency_limit(hm_ctx* c, int which) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_depth) return c->fail(HM_ERR_STATE, "hm_set_frequency_limit before hm_set_depth");
+  if (which != 0 && which != 1) return c->fail(HM_ERR_ARG - 2, "which must be 0 (zero) or 1 (infinite frequency)");
+  const double val = which ? -1.0 : 0.0;
+  c->w1 = c->v = c->wk = val;
+  for (int k = 0; k < kNk; ++k) c->wvn[k] = 0.0;
+  int rc = upload(c, c->d_wvn, c->wvn, kNk, "frequency");
+  if (rc) return rc;
+  c->have_freq = true;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // per-frequency chain
 // ---------------------------------------------------------------------------

@@ -33,7 +33,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
            "hm_surrogate_grad", "hm_set_hmatrix", "hm_get_hmatrix_stats",
-           "hm_hmatrix_apply")
+           "hm_hmatrix_apply", "hm_set_frequency_limit")
 
 _lib = None
 
@@ -144,8 +144,19 @@ class Context:
         self._chk(self.L.hm_forces(self.ptr, _p(amas), _p(bdmp), _p(exfc)), "hm_forces")
         return amas, bdmp, exfc[:, :nb]
 
+    def set_frequency_limit(self, which):
+        """which: 0 zero frequency, 1 infinite frequency"""
+        self._chk(self.L.hm_set_frequency_limit(self.ptr, _i(which)), "hm_set_frequency_limit")
+
+    def solve_limit(self, which, precision=HM_FP64):
+        self.set_frequency_limit(which)
+        return self._finish(precision)
+
     def solve_frequency(self, w1, precision=HM_FP64):
         self.set_frequency(w1)
+        return self._finish(precision)
+
+    def _finish(self, precision):
         self.assemble()
         self.factor(precision)
         iters = self.solve()

@@ -149,6 +149,40 @@ def cylinder(radius, draft, n_theta, n_z, n_r, center=(0.0, 0.0), waterplane_nr=
     return body
 
 
+def hemisphere(radius, n_theta, n_phi, center=(0.0, 0.0)):
+    """Floating hemisphere (the sphere's lower half, waterline at z = 0): flat
+    quads between n_phi latitude rings from the waterline down, closed by
+    degenerate quads whose first and last vertex are the bottom pole (the
+    bottom disk's convention)."""
+    cx, cy = center
+    nodes = []
+    th = 2.0 * np.pi * np.arange(n_theta) / n_theta
+    for k in range(n_phi):
+        a = 0.5 * np.pi * k / n_phi
+        for t in th:
+            nodes.append((cx + radius * np.cos(a) * np.cos(t), cy + radius * np.cos(a) * np.sin(t),
+                          -radius * np.sin(a)))
+    nodes.append((cx, cy, -radius))
+    pole = len(nodes)                                   # 1-based index of the pole
+    ring = lambda k, t: k * n_theta + (t % n_theta) + 1
+    pan = []
+    for k in range(n_phi - 1):
+        for t in range(n_theta):
+            pan.append((ring(k, t), ring(k, t + 1), ring(k + 1, t + 1), ring(k + 1, t)))
+    for t in range(n_theta):
+        pan.append((pole, ring(n_phi - 1, t), ring(n_phi - 1, t + 1), pole))
+    xyz = np.array(nodes)
+    ncon = np.array(pan, dtype=np.int32)
+    ncn = np.full(len(pan), 4, dtype=np.int32)
+
+    def inside(xp):
+        c = np.empty_like(xp)
+        c[:, 0], c[:, 1], c[:, 2] = cx, cy, 0.0
+        return c
+    ncon = _orient_inward(xyz, ncon, ncn, inside)
+    return Body(xyz=xyz, ncn=ncn, ncon=ncon, xr=(cx, cy, 0.0))
+
+
 def box(length, beam, draft, nx, ny, nz, center=(0.0, 0.0)):
     """Rectangular floating box: sides and bottom meshed with quads."""
     cx, cy = center

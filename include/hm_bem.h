@@ -59,6 +59,13 @@ int hm_set_headings(hm_ctx *ctx, int nbeta, const double *beta);
 /* Wave frequency omega [rad/s]: V = omega^2/g and the 200 dispersion roots
  * (CalWaveProperts with INFT=3, ImplementSubs.f90:96-105). */
 int hm_set_frequency(hm_ctx *ctx, double w1);
+/* Zero (which = 0) or infinite (which = 1) wave frequency: V = WK = W1 = 0 or -1
+ * as CalWaveProperts sets them for SYBO = 1 (ImplementSubs.f90:45-70); the Green
+ * functions take their V = 0 / V = -1 branches (FinGreen3D.f90:100-106,
+ * InfGreen_Appr.f90:130-160) and the incident wave its 1e-20 / 1e+20
+ * substitutes (PatclVelct.f90:91-99). The forces follow the reference's
+ * formulas with W1 = 0 or -1 (zero damping and excitation at zero frequency). */
+int hm_set_frequency_limit(hm_ctx *ctx, int which);
 
 /* ---- the per-frequency chain (device resident) --------------------------------
  * hm_assemble   replaces CALGREEN_MULTI(_IRR) + ASSB_LEFT_MULTI's fill +

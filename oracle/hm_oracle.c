@@ -1555,10 +1555,10 @@ void hmo_zgetrs(int n, int nrhs, const double complex *a, int lda, const int *ip
  *   RFORCE/EFORCE_MULTI      PotentWavForceMulti.f90:45-226
  * NSYS=1 (no geometric symmetry), ISOL=1 (total potential in the excitation).
  * ========================================================================== */
+static int solve_wave(const hmo_problem *p, wave_t w, const double *wvn, hmo_result *out);
+
 int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out)
 {
-    const int n = p->n, ni = p->ni, nt = n + ni, nb = p->nbody, m = 6 * nb, nbeta = p->nbeta;
-    const int ncol = m + nbeta;
     wave_t w;
     double wvn[200];
     w.w1 = w1;
@@ -1573,6 +1573,29 @@ int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out)
         w.wk = w.v;
         for (int k = 0; k < 200; k++) wvn[k] = 0.0;
     }
+    return solve_wave(p, w, wvn, out);
+}
+
+/* Zero (which = 0) and infinite (which = 1) wave frequency, CalWaveProperts
+ * with SYBO = 1 (ImplementSubs.f90:45-70): V = WK = W1 = 0 or -1; DISPERSION is
+ * not called (the limits of FINGREEN3D and INFGREEN3D do not read WVN). */
+int hmo_solve_limit(const hmo_problem *p, int which, hmo_result *out)
+{
+    wave_t w;
+    double wvn[200];
+    const double val = which ? -1.0 : 0.0;
+    w.w1 = w.v = w.wk = val;
+    w.h = p->h;
+    w.amp = p->amp;
+    w.g = p->g;
+    for (int k = 0; k < 200; k++) wvn[k] = 0.0;
+    return solve_wave(p, w, wvn, out);
+}
+
+static int solve_wave(const hmo_problem *p, wave_t w, const double *wvn, hmo_result *out)
+{
+    const int n = p->n, ni = p->ni, nt = n + ni, nb = p->nbody, m = 6 * nb, nbeta = p->nbeta;
+    const int ncol = m + nbeta;
 
     double complex *A = calloc((size_t)nt * n, sizeof(double complex));
     double complex *B = calloc((size_t)nt * ncol, sizeof(double complex));
@@ -1663,7 +1686,7 @@ int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out)
             for (int i = p->body_start[bm]; i < p->body_start[bm + 1]; i++) {
                 double complex phi = X[i + (size_t)c * n];
                 double complex rphi = I * p->rho * phi * p->ds[i];
-                double complex iphi = I * p->rho * w1 * phi * p->ds[i];
+                double complex iphi = I * p->rho * w.w1 * phi * p->ds[i];
                 for (int d = 0; d < 6; d++) {
                     out->amas[(6 * bm + d) + (size_t)c * m] += cimag(rphi * 1.0 * p->dxyz_p[6 * i + d]);
                     out->bdmp[(6 * bm + d) + (size_t)c * m] -= creal(iphi * 1.0 * p->dxyz_p[6 * i + d]);
@@ -1677,7 +1700,7 @@ int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out)
                                       + vinp(&w, x[0], x[1], x[2], p->xw[0], p->xw[1], p->beta[h])) * p->ds[i];
                 for (int d = 0; d < 6; d++) force[d] += phi * 1.0 * p->dxyz_p[6 * i + d];
             }
-            for (int d = 0; d < 6; d++) out->exfc[(6 * bm + d) + (size_t)h * m] = I * w1 * p->rho * force[d];
+            for (int d = 0; d < 6; d++) out->exfc[(6 * bm + d) + (size_t)h * m] = I * w.w1 * p->rho * force[d];
         }
     }
     if (ni > 0) { free(C); }

@@ -91,6 +91,8 @@ typedef struct {
  * IRR normal equations when ni>0), LU solve, force reductions. Returns the
  * LAPACK-style INFO of the factorisation (0 = ok). */
 int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out);
+/* The same chain at zero (which = 0) or infinite (which = 1) wave frequency. */
+int hmo_solve_limit(const hmo_problem *p, int which, hmo_result *out);
 
 /* Textbook partial-pivot LU (pivot by max |re|+|im|, as izamax) + solve. */
 int  hmo_zgetrf(int n, double complex *a, int lda, int *ipiv);

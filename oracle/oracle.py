@@ -134,6 +134,13 @@ class Oracle:
 
     def solve_frequency(self, prob, w1, want_matrices=False):
         """prob: mesh.Problem-like object (see tests/meshes.py). Returns dict."""
+        return self._solve(prob, lambda P, R: self.lib.hmo_solve_frequency(P, _d(w1), R), want_matrices)
+
+    def solve_limit(self, prob, which, want_matrices=False):
+        """zero (which=0) or infinite (which=1) wave frequency (SYBO=1)"""
+        return self._solve(prob, lambda P, R: self.lib.hmo_solve_limit(P, _i(which), R), want_matrices)
+
+    def _solve(self, prob, call, want_matrices):
         n, ni, nb = prob.n, prob.ni, prob.nbody
         m, nbeta = 6 * nb, len(prob.beta)
         keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
@@ -163,7 +170,7 @@ class Oracle:
             out["mxpot"] = np.zeros((n, m + nbeta), dtype=np.complex128, order="F")
             R.amat, R.smat = _ptr(out["amat"]), _ptr(out["smat"])
             R.rhs, R.mxpot = _ptr(out["rhs"]), _ptr(out["mxpot"])
-        info = self.lib.hmo_solve_frequency(ctypes.byref(P), _d(w1), ctypes.byref(R))
+        info = call(ctypes.byref(P), ctypes.byref(R))
         out.update(amas=amas, bdmp=bdmp, exfc=exfc, info=info)
         return out
 
