@@ -45,6 +45,16 @@ def sync():
 
 REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen": 3}
 
+# HBM bytes per launch of the dominant kernel, measured on one MI355X by two
+# counter passes of `bench.py --steps 1 --assemble-only` (rocprofv3 --pmc
+# FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter k_assemble), scaled by the
+# readings of a 1 GiB device-to-device copy (fetch x2, write x1).
+MEASURED_TRAFFIC = {
+    ("c5", "hmatrix", "asm_pidcock"): 772773.0 * 1024 * 2 + 1547392.1 * 1024,
+    ("c5", "gmres", "asm_eigene"): 5494024.2 * 1024 * 2 + 29080699.1 * 1024,
+    ("c5", "fp32ir", "asm_eigene"): 5494024.2 * 1024 * 2 + 29080699.1 * 1024,
+}
+
 
 def flop_model(name):
     """algorithmic flops per pair by classification key (tests/golden/flop_model.json)"""
@@ -215,6 +225,8 @@ def main():
     print(f"# setup (upload + pair classification) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     ctx.set_series_variant(args.series_variant)
     if args.assemble_only:
+        if args.path == "hmatrix":
+            ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
         ctx.set_timing(True)
         for s in range(args.steps):
             ctx.set_frequency(float(prob.omegas[s % len(prob.omegas)]))
@@ -222,7 +234,7 @@ def main():
         sync()
         st = ctx.timing()
         ctx.set_timing(False)
-        print(json.dumps({"assembled_systems": args.steps, "workload": prob.name,
+        print(json.dumps({"assembled_systems": args.steps, "workload": prob.name, "path": args.path,
                           "series_variant": args.series_variant, "stages_ms": {k: v[0] for k, v in st.items()}}))
         ctx.close()
         return 0
@@ -247,6 +259,7 @@ def main():
         ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
     dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup)
     gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
+    hist_mx = ctx.pair_histogram().ravel()     # pairs assembled densely on the measured path
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
         ctx.set_hmatrix(False)
@@ -279,11 +292,11 @@ def main():
                  "max_rel_err_mixed_vs_fp64": max(nw.values()),
                  "stages_ms": {k: v[0] for k, v in st_pl.items()}}
 
-    # dominant kernel: the assembly launch (per regime for finite depth) with most time
+    # dominant kernel of the measured path: the assembly launch (per regime for
+    # finite depth) with most time
     fpp, deep = flop_model(args.workload)
-    hist = ctx.pair_histogram().ravel()
-    # the plain path runs every pair through the FP64 regime kernels
-    st_rf = st_pl if plain is not None else st_mx
+    hist = hist_mx
+    st_rf = st_mx
     asm = {k: v for k, v in st_rf.items() if k in REGIME_STAGE and v[1] > 0}
     if asm:
         dom = max(asm, key=lambda k: asm[k][0])
@@ -297,11 +310,17 @@ def main():
         units = (prob.n + prob.ni) * prob.n
         flops = units * fpp[0]
     achieved = flops / (dom_ms * 1e-3) / 1e12
+    traffic = MEASURED_TRAFFIC.get((args.workload, args.path, dom))
     roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "traffic_gbs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
                 "avg_ms": dom_ms, "pairs_per_launch": units, "algorithmic_flops_per_launch": flops,
-                "note": ("FP64 VALU-bound Green-function assembly (one pair per lane); peak = MI355X FP64 "
-                         "vector peak (= FP64 matrix peak); flops per pair from tests/golden/flop_model.json")}
+                "algorithmic_bytes_per_launch": 36 * units,
+                "note": ("FP64 VALU-bound Green-function assembly over the classified pairs of one regime "
+                         "(PIDCOCK and EIGENE series evaluated by lane groups of 16 and 8); peak = MI355X FP64 "
+                         "vector peak (= FP64 matrix peak); flops per pair counted by the oracle "
+                         "(tests/golden/flop_model.json); traffic = measured HBM bytes per launch, "
+                         "algorithmic bytes = 32 B of A and S + 4 B pair index per pair")}
 
     line = {"metric": METRIC, "value": args.steps / dt_mx, "unit": "systems/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt_mx / args.steps,
