@@ -115,6 +115,10 @@ struct hm_ctx {
   DevBuf d_hUc[2], d_hVc[2];   // factors of A and S packed at their largest rank
   int hm_kuse[2] = {0, 0};
   hm::HmatDev hmat{};
+  // H-matrix built by hm_ziter_solve (ISOLV = 2) from an assembled matrix:
+  // hd_nb clusters of hd_m rows
+  bool hd_ready = false;
+  int hd_nb = 0, hd_m = 0;
   int hm_unconverged = 0, hm_steps = 0, hm_rank_max = 0;
   double hm_rank_mean = 0.0;
   int bm() const { return n / nbody; }
@@ -481,8 +485,8 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
       // pairs outside its validated cells go to the plain kernels
       const long long np = c->seg_cnt[2] + c->seg_cnt[3];
       HIPCHK(c, c->d_fb.ensure(sizeof(uint32_t) * (size_t)np), "surrogate alloc");
-      HIPCHK(c, c->d_fbcnt.ensure(2 * sizeof(unsigned long long)), "surrogate alloc");
-      HIPCHK(c, hipMemsetAsync(c->d_fbcnt.p, 0, 2 * sizeof(unsigned long long), c->stream), "surrogate");
+      HIPCHK(c, c->d_fbcnt.ensure(3 * sizeof(unsigned long long)), "surrogate alloc");
+      HIPCHK(c, hipMemsetAsync(c->d_fbcnt.p, 0, 3 * sizeof(unsigned long long), c->stream), "surrogate");
       uint32_t* fb_e = c->d_fb.as<uint32_t>();
       uint32_t* fb_g = fb_e + c->seg_cnt[2];
       const double kh = c->w1 * c->w1 * c->h / c->g_disp;
@@ -490,14 +494,14 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
       HIPCHK(c, hm::assemble_surrogate(c->stream, gd, f, kh, pairs + c->seg_off[2], np, c->sur_dev(), (float)c->sur_tol,
                                        A, S, ld, fb_e, fb_g, c->d_fbcnt.as<unsigned long long>()), "assemble surrogate");
       c->stage_end(kAsmSurrogate);
-      unsigned long long cnt[2] = {0, 0};
+      unsigned long long cnt[3] = {0, 0, 0};
       HIPCHK(c, hipMemcpyAsync(cnt, c->d_fbcnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream), "surrogate");
       HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate");
       if ((long long)cnt[0] > c->seg_cnt[2] || (long long)cnt[1] > c->seg_cnt[3])
         return c->fail(HM_ERR_STATE, "surrogate fallback lists overflow");
-      c->sur_counts[0] += np - (long long)(cnt[0] + cnt[1]);
+      c->sur_counts[0] += np - (long long)(cnt[0] + cnt[1] + cnt[2]);
       c->sur_counts[1] += (long long)cnt[0];
-      c->sur_counts[2] += (long long)cnt[1];
+      c->sur_counts[2] += (long long)(cnt[1] + cnt[2]);
       for (int r = 2; r < 4; ++r) {
         const long long m = (long long)cnt[r - 2];
         if (!m) continue;
@@ -838,21 +842,23 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
 
 extern "C" int hm_set_hmatrix(hm_ctx* c, int enable, double eps, double eps_s, int kmax) {
   if (!c) return HM_ERR_ARG - 1;
-  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_hmatrix before hm_set_geometry");
+  if (!(eps > 0.0 && eps < 1.0)) return c->fail(HM_ERR_ARG - 3, "eps must be in (0, 1)");
+  if (!(eps_s > 0.0 && eps_s < 1.0)) return c->fail(HM_ERR_ARG - 4, "eps_s must be in (0, 1)");
+  if (kmax < 1) return c->fail(HM_ERR_ARG - 5, "kmax must be >= 1");
   if (enable) {
+    if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_hmatrix before hm_set_geometry");
     if (c->nbody < 2) return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs at least two bodies");
     if (c->ni > 0) return c->fail(HM_ERR_ARG - 2, "the H-matrix form takes no interior waterplane points (ni = %d)", c->ni);
     const int m = c->n / c->nbody;
     for (int b = 0; b <= c->nbody; ++b)
       if (c->body_start[b] != b * m)
         return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs bodies of equal panel counts");
-    if (!(eps > 0.0 && eps < 1.0)) return c->fail(HM_ERR_ARG - 3, "eps must be in (0, 1)");
-    if (!(eps_s > 0.0 && eps_s < 1.0)) return c->fail(HM_ERR_ARG - 4, "eps_s must be in (0, 1)");
-    if (kmax < 1 || kmax > m) return c->fail(HM_ERR_ARG - 5, "kmax must be in 1..%d", m);
-    c->hm_eps = eps;
-    c->hm_eps_s = eps_s;
-    c->hm_kmax = kmax;
+    if (kmax > m) return c->fail(HM_ERR_ARG - 5, "kmax must be in 1..%d", m);
   }
+  // the tolerances and rank cap also serve hm_hmatrix_build
+  c->hm_eps = eps;
+  c->hm_eps_s = eps_s;
+  c->hm_kmax = kmax;
   const bool was = c->hm_on;
   c->hm_on = enable != 0;
   c->assembled = c->factored = c->solved = false;
@@ -881,11 +887,20 @@ extern "C" int hm_get_hmatrix_stats(hm_ctx* c, int* unconverged, int* steps, dou
 
 extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, double* y) {
   if (!c) return HM_ERR_ARG - 1;
-  if (!c->hm_on || !c->assembled) return c->fail(HM_ERR_STATE, "hm_hmatrix_apply needs an assembled H-matrix form");
+  const bool from_assembly = c->hm_on && c->assembled;
+  if (!from_assembly && !c->hd_ready)
+    return c->fail(HM_ERR_STATE, "hm_hmatrix_apply needs an assembled H-matrix form or one built by hm_ziter_solve");
   if (mat != 0 && mat != 1) return c->fail(HM_ERR_ARG - 2, "mat must be 0 (A) or 1 (S)");
-  if (ncol < 1 || ncol > c->ncol()) return c->fail(HM_ERR_ARG - 3, "ncol must be in 1..%d", c->ncol());
+  if (!from_assembly && mat != 0) return c->fail(HM_ERR_ARG - 2, "an H-matrix built from a matrix has only mat 0");
+  if (ncol < 1 || (from_assembly && ncol > c->ncol())) return c->fail(HM_ERR_ARG - 3, "bad ncol %d", ncol);
   if (!x || !y) return c->fail(HM_ERR_ARG - 4, "x or y is NULL");
-  const int n = c->n, nb = c->nbody, m = c->bm();
+  const int nb = from_assembly ? c->nbody : c->hd_nb, m = from_assembly ? c->bm() : c->hd_m, n = nb * m;
+  const size_t tneed = 2 * sizeof(double) * (size_t)nb * (nb - 1) * (c->hm_kuse[mat] ? c->hm_kuse[mat] : 1) * ncol;
+  if (c->d_htb.bytes < tneed || c->d_hts.bytes < tneed) {
+    int e = c->hip(c->d_htb.ensure(tneed), "hmatrix apply");
+    if (e == HM_OK) e = c->hip(c->d_hts.ensure(tneed), "hmatrix apply");
+    if (e) return e;
+  }
   const size_t bytes = 2 * sizeof(double) * (size_t)n * ncol;
   DevBuf dx, dy;
   int rc = HM_OK;
@@ -901,6 +916,200 @@ extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, d
   dx.release();
   dy.release();
   return rc;
+}
+
+// H-matrix of an assembled host matrix (HMATRIX_BUILD(N, AMAT, XYZ_P),
+// HMatrix.f90:135): clusters = the nblocks equal blocks given, diagonal blocks
+// dense, the rest by cross approximation of the matrix's entries; the FP32
+// copies and the block-diagonal preconditioner for GMRES are set up with it
+extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int nblocks, const int* block_sizes) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
+  if (!a || lda < n) return c->fail(HM_ERR_ARG - 3, "bad matrix");
+  if (nblocks < 2 || !block_sizes) return c->fail(HM_ERR_ARG - 5, "the H-matrix needs >= 2 cluster blocks");
+  const int m = block_sizes[0];
+  for (int k = 0; k < nblocks; ++k)
+    if (block_sizes[k] != m) return c->fail(HM_ERR_ARG - 6, "the H-matrix needs cluster blocks of equal size");
+  if ((long long)m * nblocks != n) return c->fail(HM_ERR_ARG - 6, "block sizes sum to %lld, not n = %d", (long long)m * nblocks, n);
+  if (c->hm_kmax > m) return c->fail(HM_ERR_STATE, "kmax %d exceeds the block size %d", c->hm_kmax, m);
+  // the context's matrix buffers are reused: any assembled state is gone
+  c->assembled = c->factored = c->solved = false;
+  c->hd_ready = false;
+  const size_t cplx = 2 * sizeof(double);
+  hipStream_t s = c->stream;
+  const int nb = nblocks, kmax = c->hm_kmax;
+  const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+  HIPCHK(c, c->d_A.ensure(cplx * (size_t)n * n), "hmatrix alloc");
+  HIPCHK(c, hipMemcpy2DAsync(c->d_A.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, s), "hmatrix upload");
+  HIPCHK(c, c->d_info.ensure(sizeof(int) * nb), "hmatrix alloc");
+  HIPCHK(c, c->d_counter.ensure(sizeof(unsigned) * nb), "hmatrix alloc");
+  HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nb, s), "hmatrix");
+  HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nb, s), "hmatrix");
+  HIPCHK(c, c->d_hmat.ensure(hm::hmat_bytes(nb, m, kmax, 1)), "hmatrix alloc");
+  HIPCHK(c, c->d_hD.ensure(cplx * (size_t)nb * sD), "hmatrix alloc");
+  c->hmat = hm::hmat_carve(c->d_hmat.p, nb, m, kmax, c->hm_eps, c->hm_eps, 1, c->d_A.as<double>(), n);
+  for (int k = 0; k < nb; ++k)
+    HIPCHK(c, hipMemcpy2DAsync(c->d_hD.as<double>() + 2 * (size_t)k * sD, cplx * m,
+                               c->d_A.as<double>() + 2 * ((size_t)k * m + (size_t)k * m * n), cplx * n, cplx * m, m,
+                               hipMemcpyDeviceToDevice, s), "hmatrix diagonal blocks");
+  hm::GeomDev g0{};
+  hm::FreqDev f0{};
+  HIPCHK(c, hm::aca_build(s, g0, f0, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
+  const int P = hm::hmat_processes(nb, 1);
+  std::vector<int> rank(P);
+  HIPCHK(c, hipMemcpyAsync(rank.data(), c->hmat.rank, sizeof(int) * P, hipMemcpyDeviceToHost, s), "hmatrix");
+  HIPCHK(c, hipStreamSynchronize(s), "hmatrix");
+  int ku = 0;
+  long long rs = 0;
+  for (int r : rank) { ku = r > ku ? r : ku; rs += r; }
+  c->hm_kuse[0] = ku;
+  c->hm_rank_max = ku;
+  c->hm_rank_mean = P ? (double)rs / P : 0.0;
+  const size_t lr = (size_t)nb * (nb - 1) * m * (ku ? ku : 1);
+  HIPCHK(c, c->d_hUc[0].ensure(cplx * lr), "hmatrix alloc");
+  HIPCHK(c, c->d_hVc[0].ensure(cplx * lr), "hmatrix alloc");
+  if (ku) HIPCHK(c, hm::hmat_compact(s, c->hmat, 0, ku, c->d_hUc[0].as<double>(), c->d_hVc[0].as<double>()), "hmatrix pack");
+  HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "hmatrix alloc");
+  HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "hmatrix alloc");
+  HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)n), "hmatrix alloc");
+  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "hmatrix alloc");
+  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "hmatrix alloc");
+  const double* D = c->d_hD.as<double>();
+  HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::demote(s, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::demote(s, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1), "demote");
+  HIPCHK(c, hm::demote(s, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1), "demote");
+  HIPCHK(c, hm::lu_factor<float>(s, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info.as<int>(),
+                                 c->d_counter.as<unsigned>(), nb), "block preconditioner");
+  std::vector<int> infos(nb, 0);
+  HIPCHK(c, hipMemcpyAsync(infos.data(), c->d_info.p, sizeof(int) * nb, hipMemcpyDeviceToHost, s), "hmatrix");
+  HIPCHK(c, hipStreamSynchronize(s), "hmatrix");
+  for (int k = 0; k < nb; ++k)
+    if (infos[k] > 0) return c->fail(k * m + infos[k], "zero pivot in diagonal block %d", k + 1);
+  c->hd_nb = nb;
+  c->hd_m = m;
+  c->hd_ready = true;
+  return HM_OK;
+}
+
+extern "C" int hm_hmatrix_release(hm_ctx* c) {
+  if (!c) return HM_ERR_ARG - 1;
+  c->hd_ready = false;
+  return HM_OK;
+}
+
+// ZITER_SOLVE_MULTI(ISOLV, N, AMAT, B, X, NRHS, INFO) (IterativeSolvers.f90:574)
+// on an assembled host matrix. isolv = 1: dense GMRES with the block-diagonal
+// preconditioner of SETUP_BLOCK_PRECONDITIONER(N, AMAT, NBLOCKS, BLOCK_SIZES)
+// (:190; nblocks = 0: none); isolv = 2: GMRES on the H-matrix of the last
+// hm_hmatrix_build (as ZGMRES_SOLVE uses HMATVEC once HMATRIX_BUILD has run,
+// :325-343), which also supplies the preconditioner; a and the blocks are then
+// not read.
+extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int lda, const double* b, int ldb,
+                              double* x, int ldx, int nrhs, int nblocks, const int* block_sizes, int* info) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (isolv != 1 && isolv != 2) return c->fail(HM_ERR_ARG - 2, "isolv must be 1 (dense) or 2 (H-matrix)");
+  if (n < 1) return c->fail(HM_ERR_ARG - 3, "n must be >= 1");
+  if (isolv == 1 && (!a || lda < n)) return c->fail(HM_ERR_ARG - 5, "bad matrix");
+  if (!b || ldb < n) return c->fail(HM_ERR_ARG - 7, "bad right-hand sides");
+  if (!x || ldx < n) return c->fail(HM_ERR_ARG - 8, "bad solution array");
+  if (nrhs < 1) return c->fail(HM_ERR_ARG - 10, "nrhs must be >= 1");
+  if (isolv == 2) {
+    if (!c->hd_ready) return c->fail(HM_ERR_STATE, "isolv = 2 before hm_hmatrix_build");
+    if (c->hd_nb * c->hd_m != n) return c->fail(HM_ERR_ARG - 3, "n = %d differs from the H-matrix's %d", n, c->hd_nb * c->hd_m);
+  }
+  std::vector<int> start(1, 0);
+  if (isolv == 1) {
+    if (nblocks < 0 || (nblocks > 0 && !block_sizes)) return c->fail(HM_ERR_ARG - 11, "bad preconditioner blocks");
+    for (int k = 0; k < nblocks; ++k) {
+      if (block_sizes[k] < 1) return c->fail(HM_ERR_ARG - 12, "block %d is empty", k + 1);
+      start.push_back(start.back() + block_sizes[k]);
+    }
+    if (nblocks > 0 && start.back() != n) return c->fail(HM_ERR_ARG - 12, "block sizes sum to %d, not n = %d", start.back(), n);
+    c->assembled = c->factored = c->solved = false;
+    c->hd_ready = false;
+  }
+  const size_t cplx = 2 * sizeof(double);
+  hipStream_t s = c->stream;
+  HIPCHK(c, c->d_R.ensure(cplx * (size_t)n * nrhs), "ziter alloc");
+  HIPCHK(c, c->d_X.ensure(cplx * (size_t)n * nrhs), "ziter alloc");
+  HIPCHK(c, hipMemcpy2DAsync(c->d_R.p, cplx * n, b, cplx * ldb, cplx * n, nrhs, hipMemcpyHostToDevice, s), "ziter upload");
+  hm::GmresArgs ga;
+  ga.n = n;
+  ga.ncol = nrhs;
+  ga.restart = c->gm_restart;
+  ga.maxiter = c->gm_maxiter;
+  ga.tol = c->gm_tol;
+  ga.inner_tol = 0.5 * c->gm_tol;
+  ga.B = c->d_R.as<double>();
+  ga.ldb = n;
+  ga.X = c->d_X.as<double>();
+  c->gm_blocks.clear();
+  if (isolv == 1) {
+    HIPCHK(c, c->d_A.ensure(cplx * (size_t)n * n), "ziter alloc");
+    HIPCHK(c, hipMemcpy2DAsync(c->d_A.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, s), "ziter upload");
+    HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "ziter alloc");
+    HIPCHK(c, hm::demote(s, c->d_A.as<double>(), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    ga.A64 = c->d_A.as<double>();
+    ga.lda64 = n;
+    ga.A32 = c->d_A32.as<float>();
+    if (nblocks > 0) {
+      const size_t nmat = (size_t)nblocks;
+      HIPCHK(c, c->d_info.ensure(sizeof(int) * nmat), "ziter alloc");
+      HIPCHK(c, c->d_counter.ensure(sizeof(unsigned) * nmat), "ziter alloc");
+      HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nmat, s), "ziter");
+      HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nmat, s), "ziter");
+      c->gm_blocks = start;
+      size_t tot = 0;
+      for (int k = 0; k < nblocks; ++k) tot += (size_t)block_sizes[k] * block_sizes[k];
+      HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * tot), "ziter alloc");
+      HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)n), "ziter alloc");
+      HIPCHK(c, hm::gmres_block_precond_setup(s, c->d_A.as<double>(), n, nblocks, c->gm_blocks.data(),
+                                              c->d_P32.as<float>(), c->d_pipiv.as<int>(), c->d_info.as<int>(),
+                                              c->d_counter.as<unsigned>()), "block preconditioner");
+      std::vector<int> infos(nmat, 0);
+      HIPCHK(c, hipMemcpyAsync(infos.data(), c->d_info.p, sizeof(int) * nmat, hipMemcpyDeviceToHost, s), "ziter");
+      HIPCHK(c, hipStreamSynchronize(s), "ziter");
+      for (size_t k = 0; k < nmat; ++k)
+        if (infos[k] > 0) return c->fail(start[k] + infos[k], "zero pivot in preconditioner block %zu", k + 1);
+    }
+  } else {
+    const int nb = c->hd_nb, m = c->hd_m, ku = c->hm_kuse[0];
+    const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+    const size_t tsz = cplx * (size_t)nb * (nb - 1) * (ku ? ku : 1) * nrhs;
+    HIPCHK(c, c->d_htb.ensure(tsz), "ziter alloc");
+    HIPCHK(c, c->d_hts.ensure(tsz), "ziter alloc");
+    const float* D32 = c->d_hD32.as<float>();
+    const float* U32 = c->d_hU32.as<float>();
+    const float* V32 = c->d_hV32.as<float>();
+    const double* D = c->d_hD.as<double>();
+    const double* U = c->d_hUc[0].as<double>();
+    const double* V = c->d_hVc[0].as<double>();
+    double* tb = c->d_htb.as<double>();
+    double* ts = c->d_hts.as<double>();
+    const int ncol = nrhs;
+    ga.op32 = [=](hipStream_t st, const float* Z, float* W) {
+      return hm::hmat_apply<float>(st, nb, m, ku, D32, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
+    };
+    ga.op64 = [=](hipStream_t st, const double* Z, double* W) {
+      return hm::hmat_apply<double>(st, nb, m, ku, D, m, sD, U, V, Z, n, W, n, ncol, tb, ts);
+    };
+    for (int k = 0; k <= nb; ++k) start.push_back(k * m);
+    start.erase(start.begin());
+    c->gm_blocks = start;
+  }
+  if (!c->gm_blocks.empty()) {
+    ga.nblk = (int)c->gm_blocks.size() - 1;
+    ga.blk_start = c->gm_blocks.data();
+    ga.P = c->d_P32.as<float>();
+    ga.pipiv = c->d_pipiv.as<int>();
+  }
+  HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, nrhs, c->gm_restart)), "gmres alloc");
+  HIPCHK(c, hm::gmres_solve(s, ga, c->d_gwork.p, &c->gm_stats), "gmres");
+  HIPCHK(c, hipMemcpy2DAsync(x, cplx * ldx, c->d_X.p, cplx * n, cplx * n, nrhs, hipMemcpyDeviceToHost, s), "ziter download");
+  HIPCHK(c, hipStreamSynchronize(s), "ziter");
+  if (info) *info = c->gm_stats.converged ? 0 : 1;
+  return HM_OK;
 }
 
 extern "C" int hm_set_series_variant(hm_ctx* c, int variant) {

@@ -88,10 +88,73 @@ __global__ __launch_bounds__(256) void k_assemble_list(GeomDev g, FreqDev f, con
 // The epsilon-accelerated regimes with G lanes per pair (see hm_coop.hpp): each
 // lane owns one pair of the list; the group evaluates the series and their
 // epsilon limits for its G pairs in turn, then every lane finishes its own pair.
+
+// series of one pair (broadcast inputs) evaluated by the whole group; every lane
+// returns the three epsilon limits
+template <int REG, int G, int K>
+__device__ __forceinline__ void coop_series(int ntq, double r, double zfh, double zph, double h, double v,
+                                            const double* wvn, int l, double& l1, double& l2, double& l3) {
+  typedef LaneGroup<G> LG;
+  double s1[K], s2[K], s3[K];
+  double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int row = l + G * k;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (G * k < ntq) {
+      if (row >= 1 && row < ntq) {
+        if (REG == kEigene) fin_evanescent(wvn[row], r, zfh, zph, h, t1, t2, t3);
+        else fin_pidcock_term(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3);
+      }
+      t1 = LG::incl_sum(t1, l) + c1;
+      t2 = LG::incl_sum(t2, l) + c2;
+      t3 = LG::incl_sum(t3, l) + c3;
+      c1 = group_bcast<G>(t1, G - 1);
+      c2 = group_bcast<G>(t2, G - 1);
+      c3 = group_bcast<G>(t3, G - 1);
+    } else {
+      t1 = c1;
+      t2 = c2;
+      t3 = c3;
+    }
+    s1[k] = t1;
+    s2[k] = t2;
+    s3[k] = t3;
+  }
+  const int n = ntq - 1;
+  l1 = coop_limes<G, K>(n, s1, l);
+  l2 = coop_limes<G, K>(n, s2, l);
+  l3 = coop_limes<G, K>(n, s3, l);
+}
+
+// the rest of one pair once its epsilon limits are known
+template <int REG>
+__device__ __forceinline__ void coop_finish(const GeomDev& g, const double* wvn, double h, int i, int j, double rr,
+                                            double L1, double L2, double L3, double* A, double* S, long long ld) {
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double zf = xq[2], zp = xp[2];
+  FinOut fo{0, 0, 0, 0, 0, 0};
+  if (REG == kEigene) {
+    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+    fo.g_re += L1;
+    fo.gr_re += L2;
+    fo.gz_re += L3;
+    fin_strip(rr, zf, zp, fo);
+  } else {
+    fin_pidcock_head(rr, zf, zp, wvn, h, fo);
+    fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
+  }
+  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
+  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
+}
+
 template <int REG, int G, int K>
 __global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
                                                        long long np, double* A, double* S, long long ld) {
-  typedef LaneGroup<G> LG;
   __shared__ double wvn[kNk];
   load_wvn(wvn, f.wvn, kNk);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -110,55 +173,12 @@ __global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, con
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
     const int ntq = group_bcast<G>(nt, q);
-    double s1[K], s2[K], s3[K];
-    double c1 = 0.0, c2 = 0.0, c3 = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int row = l + G * k;
-      double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-      if (G * k < ntq) {
-        if (row >= 1 && row < ntq) {
-          if (REG == kEigene) fin_evanescent(wvn[row], r, zfh, zph, h, t1, t2, t3);
-          else fin_pidcock_term(row, wvn[row], r, zfh, zph, f.v, h, t1, t2, t3);
-        }
-        t1 = LG::incl_sum(t1, l) + c1;
-        t2 = LG::incl_sum(t2, l) + c2;
-        t3 = LG::incl_sum(t3, l) + c3;
-        c1 = group_bcast<G>(t1, G - 1);
-        c2 = group_bcast<G>(t2, G - 1);
-        c3 = group_bcast<G>(t3, G - 1);
-      } else {
-        t1 = c1;
-        t2 = c2;
-        t3 = c3;
-      }
-      s1[k] = t1;
-      s2[k] = t2;
-      s3[k] = t3;
-    }
-    const int n = ntq - 1;
-    const double l1 = coop_limes<G, K>(n, s1, l);
-    const double l2 = coop_limes<G, K>(n, s2, l);
-    const double l3 = coop_limes<G, K>(n, s3, l);
+    double l1, l2, l3;
+    coop_series<REG, G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
   }
   if (!valid) return;
-  FinOut fo{0, 0, 0, 0, 0, 0};
-  if (REG == kEigene) {
-    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
-    fo.g_re += L1;
-    fo.gr_re += L2;
-    fo.gz_re += L3;
-    fin_strip(rr, zf, zp, fo);
-  } else {
-    fin_pidcock_head(rr, zf, zp, wvn, h, fo);
-    fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
-  }
-  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
-  const int flag = near_flag(xp, xq, g.pnsz[j]);
-  Grn4 gr;
-  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
-  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
+  coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld);
 }
 
 // ---- classification --------------------------------------------------------
@@ -296,10 +316,8 @@ __device__ void vinp(const FreqDev& f, const double* x, double beta, double xw, 
 }
 
 // W[j, 6b+d] = n_jd for j in body b; W[j, m+h] = -(DINP.n)_j,h
-__global__ void k_build_w(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw, double yw, double* W,
-                          int ldw) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= g.n) return;
+__device__ __forceinline__ void build_w_body(const GeomDev& g, const FreqDev& f, const double* beta, int nbeta,
+                                             double xw, double yw, double* W, long long ldw, int j) {
   const int m = 6 * g.nbody;
   int b = 0;
   while (b + 1 < g.nbody && j >= g.body_start[b + 1]) ++b;
@@ -317,6 +335,13 @@ __global__ void k_build_w(GeomDev g, FreqDev f, const double* beta, int nbeta, d
   }
 }
 
+__global__ void k_build_w(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw, double yw, double* W,
+                          int ldw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.n) return;
+  build_w_body(g, f, beta, nbeta, xw, yw, W, ldw, j);
+}
+
 hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                    double yw, double* W, int ldw) {
   hipLaunchKernelGGL(k_build_w, dim3((g.n + 255) / 256), dim3(256), 0, s, g, f, beta, nbeta, xw, yw, W, ldw);
@@ -325,10 +350,10 @@ hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const doub
 
 // Force reductions (RFORCE_MULTI/EFORCE_MULTI, PotentWavForceMulti.f90:45-226):
 // one workgroup per (force row 6*b+d, potential column c).
-__global__ __launch_bounds__(256) void k_forces(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw,
-                                                double yw, const double* X, int ldx, double* amas, double* bdmp,
-                                                double* exfc) {
-  const int row = blockIdx.x, c = blockIdx.y, b = row / 6, d = row % 6, m = 6 * g.nbody;
+__device__ __forceinline__ void forces_body(const GeomDev& g, const FreqDev& f, const double* beta, int nbeta,
+                                            double xw, double yw, const double* X, long long ldx, double* amas,
+                                            double* bdmp, double* exfc, int row, int c) {
+  const int b = row / 6, d = row % 6, m = 6 * g.nbody;
   __shared__ double sr[256], si[256];
   double ar = 0.0, ai = 0.0;
   for (int i = g.body_start[b] + threadIdx.x; i < g.body_start[b + 1]; i += blockDim.x) {
@@ -362,10 +387,202 @@ __global__ __launch_bounds__(256) void k_forces(GeomDev g, FreqDev f, const doub
   }
 }
 
+__global__ __launch_bounds__(256) void k_forces(GeomDev g, FreqDev f, const double* beta, int nbeta, double xw,
+                                                double yw, const double* X, int ldx, double* amas, double* bdmp,
+                                                double* exfc) {
+  forces_body(g, f, beta, nbeta, xw, yw, X, ldx, amas, bdmp, exfc, blockIdx.x, blockIdx.y);
+}
+
 hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                   double yw, const double* X, int ldx, double* amas, double* bdmp, double* exfc) {
   const int m = 6 * g.nbody;
   hipLaunchKernelGGL(k_forces, dim3(m, m + nbeta), dim3(256), 0, s, g, f, beta, nbeta, xw, yw, X, ldx, amas, bdmp, exfc);
+  return hipGetLastError();
+}
+
+// ---- batches of independent problems ------------------------------------------
+// Pair codes: member << 40 | i << 20 | j. Each lane reads its member's geometry,
+// matrices and wave state from the device descriptors; the kernels are
+// persistent (grid-stride over the pair list, grid sized by the caller).
+__device__ __forceinline__ void decode_pair(uint64_t c, int& p, int& i, int& j) {
+  p = (int)(c >> 40);
+  i = (int)((c >> 20) & 0xfffffu);
+  j = (int)(c & 0xfffffu);
+}
+__device__ __forceinline__ FreqDev batch_freq(const BatchFreq& b) {
+  FreqDev f;
+  f.v = b.v; f.w1 = b.w1; f.wk = b.wk; f.h = b.h; f.rho = b.rho; f.g = b.g; f.amp = b.amp;
+  f.wvn = b.wvn;
+  return f;
+}
+
+__global__ __launch_bounds__(256) void k_bclassify_count(const BatchProb* __restrict__ P,
+                                                         const BatchFreq* __restrict__ F, int p, unsigned* hist) {
+  __shared__ unsigned sh[kNumKeys];
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
+  __syncthreads();
+  const GeomDev g = P[p].g;
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4), j = blockIdx.x * 16 + (threadIdx.x & 15);
+  if (i < g.n && j < g.n) atomicAdd(&sh[pair_key(g, F[p].h, i, j)], 1u);
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
+    if (sh[k]) atomicAdd(&hist[k], sh[k]);
+}
+
+// scatter with one atomic per (workgroup, key): ranks within the workgroup from LDS
+__global__ __launch_bounds__(256) void k_bclassify_scatter(const BatchProb* __restrict__ P,
+                                                           const BatchFreq* __restrict__ F, int p,
+                                                           unsigned long long* cursor, uint64_t* codes) {
+  __shared__ unsigned cnt[kNumKeys];
+  __shared__ unsigned long long base[kNumKeys];
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cnt[k] = 0;
+  __syncthreads();
+  const GeomDev g = P[p].g;
+  const int i = blockIdx.y * 16 + (threadIdx.x >> 4), j = blockIdx.x * 16 + (threadIdx.x & 15);
+  const bool in = i < g.n && j < g.n;
+  const int key = in ? pair_key(g, F[p].h, i, j) : 0;
+  const unsigned rank = in ? atomicAdd(&cnt[key], 1u) : 0u;
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
+    if (cnt[k]) base[k] = atomicAdd(&cursor[k], (unsigned long long)cnt[k]);
+  __syncthreads();
+  if (in) codes[base[key] + rank] = ((uint64_t)p << 40) | ((uint64_t)i << 20) | (uint64_t)j;
+}
+
+template <int REG>
+__global__ __launch_bounds__(256) void k_bassemble_list(const BatchProb* __restrict__ P,
+                                                        const BatchFreq* __restrict__ F,
+                                                        const uint64_t* __restrict__ codes, long long np) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < np;
+       t += (long long)gridDim.x * blockDim.x) {
+    int p, i, j;
+    decode_pair(codes[t], p, i, j);
+    const FreqDev f = batch_freq(F[p]);
+    const BatchProb& bp = P[p];
+    assemble_pair<REG>(bp.g, f, f.wvn, i, j, bp.A, bp.S, bp.ld);
+  }
+}
+
+template <int REG, int G, int K>
+__global__ __launch_bounds__(256) void k_bassemble_coop(const BatchProb* __restrict__ P,
+                                                        const BatchFreq* __restrict__ F,
+                                                        const uint64_t* __restrict__ codes, long long np) {
+  const int l = threadIdx.x & (G - 1);
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < np; base += (long long)gridDim.x * blockDim.x) {
+    const long long t = base + threadIdx.x;
+    const bool valid = t < np;
+    int p, i, j;
+    decode_pair(codes[valid ? t : np - 1], p, i, j);
+    const GeomDev& g = P[p].g;
+    const double* xp = g.fp + 3 * (long long)i;
+    const double* xq = g.xyz_p + 3 * (long long)j;
+    const double h = F[p].h;
+    const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+    const double zf = xq[2], zp = xp[2];
+    const int nt = REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
+    double L1 = 0.0, L2 = 0.0, L3 = 0.0;
+    for (int q = 0; q < G; ++q) {
+      const int pq = group_bcast<G>(p, q);
+      const double hq = F[pq].h;
+      const double r = group_bcast<G>(rr, q);
+      const double zfh = group_bcast<G>(zf, q) + hq, zph = group_bcast<G>(zp, q) + hq;
+      const int ntq = group_bcast<G>(nt, q);
+      double l1, l2, l3;
+      coop_series<REG, G, K>(ntq, r, zfh, zph, hq, F[pq].v, F[pq].wvn, l, l1, l2, l3);
+      if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
+    }
+    if (valid) coop_finish<REG>(g, F[p].wvn, h, i, j, rr, L1, L2, L3, P[p].A, P[p].S, P[p].ld);
+  }
+}
+
+// all pairs of the members in `list` (infinite depth, or a frequency limit):
+// t -> member by the prefix sums of their n^2
+template <bool DEEP>
+__global__ __launch_bounds__(256) void k_bassemble_all(const BatchProb* __restrict__ P,
+                                                       const BatchFreq* __restrict__ F, const int* __restrict__ list,
+                                                       const long long* __restrict__ prefix, int nlist) {
+  const long long total = prefix[nlist];
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nlist - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int p = list[lo];
+    const BatchProb& bp = P[p];
+    const long long e = t - prefix[lo];
+    const int i = (int)(e % bp.g.n), j = (int)(e / bp.g.n);
+    const FreqDev f = batch_freq(F[p]);
+    assemble_pair<DEEP ? -1 : -2>(bp.g, f, f.wvn, i, j, bp.A, bp.S, bp.ld);
+  }
+}
+
+__global__ void k_bbuild_w(const BatchProb* __restrict__ P, const BatchFreq* __restrict__ F) {
+  const BatchProb& bp = P[blockIdx.y];
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= bp.g.n) return;
+  const BatchFreq& fr = F[blockIdx.y];
+  build_w_body(bp.g, batch_freq(fr), bp.beta, bp.nbeta, fr.xw, fr.yw, bp.W, bp.ld, j);
+}
+
+__global__ __launch_bounds__(256) void k_bforces(const BatchProb* __restrict__ P, const BatchFreq* __restrict__ F) {
+  const BatchProb& bp = P[blockIdx.z];
+  const int m = 6 * bp.g.nbody;
+  if ((int)blockIdx.x >= m || (int)blockIdx.y >= m + bp.nbeta) return;
+  const BatchFreq& fr = F[blockIdx.z];
+  forces_body(bp.g, batch_freq(fr), bp.beta, bp.nbeta, fr.xw, fr.yw, bp.X, bp.ld, bp.amas, bp.bdmp, bp.exfc,
+              blockIdx.x, blockIdx.y);
+}
+
+hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int n, unsigned* hist,
+                          unsigned long long* cursor, uint64_t* codes, bool count_only) {
+  const dim3 grid((n + 15) / 16, (n + 15) / 16);
+  if (count_only) hipLaunchKernelGGL(k_bclassify_count, grid, dim3(256), 0, s, P, F, p, hist);
+  else hipLaunchKernelGGL(k_bclassify_scatter, grid, dim3(256), 0, s, P, F, p, cursor, codes);
+  return hipGetLastError();
+}
+
+hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, const BatchFreq* F,
+                                 const uint64_t* codes, long long np, int variant, int grid) {
+  if (np <= 0) return hipSuccess;
+  const long long need = (np + 255) / 256;
+  const dim3 gd((unsigned)(need < grid ? need : grid));
+  switch (regime) {
+    case kLinton: hipLaunchKernelGGL((k_bassemble_list<kLinton>), gd, dim3(256), 0, s, P, F, codes, np); break;
+    case kPidcock:
+      if (variant == 1) hipLaunchKernelGGL((k_bassemble_coop<kPidcock, 16, 7>), gd, dim3(256), 0, s, P, F, codes, np);
+      else if (variant == 2) hipLaunchKernelGGL((k_bassemble_coop<kPidcock, 8, 13>), gd, dim3(256), 0, s, P, F, codes, np);
+      else hipLaunchKernelGGL((k_bassemble_list<kPidcock>), gd, dim3(256), 0, s, P, F, codes, np);
+      break;
+    case kEigene:
+      if (variant == 1) hipLaunchKernelGGL((k_bassemble_coop<kEigene, 8, 7>), gd, dim3(256), 0, s, P, F, codes, np);
+      else if (variant == 2) hipLaunchKernelGGL((k_bassemble_coop<kEigene, 4, 13>), gd, dim3(256), 0, s, P, F, codes, np);
+      else hipLaunchKernelGGL((k_bassemble_list<kEigene>), gd, dim3(256), 0, s, P, F, codes, np);
+      break;
+    default: hipLaunchKernelGGL((k_bassemble_list<kEigen>), gd, dim3(256), 0, s, P, F, codes, np); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t batch_assemble_all(hipStream_t s, bool deep, const BatchProb* P, const BatchFreq* F, const int* list,
+                              const long long* prefix, int nlist, long long total, int grid) {
+  if (nlist <= 0 || total <= 0) return hipSuccess;
+  const long long need = (total + 255) / 256;
+  const dim3 gd((unsigned)(need < grid ? need : grid));
+  if (deep) hipLaunchKernelGGL((k_bassemble_all<true>), gd, dim3(256), 0, s, P, F, list, prefix, nlist);
+  else hipLaunchKernelGGL((k_bassemble_all<false>), gd, dim3(256), 0, s, P, F, list, prefix, nlist);
+  return hipGetLastError();
+}
+
+hipError_t batch_build_w(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int nmax) {
+  hipLaunchKernelGGL(k_bbuild_w, dim3((nmax + 255) / 256, count), dim3(256), 0, s, P, F);
+  return hipGetLastError();
+}
+
+hipError_t batch_forces(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int mmax, int ncolmax) {
+  hipLaunchKernelGGL(k_bforces, dim3(mmax, ncolmax, count), dim3(256), 0, s, P, F);
   return hipGetLastError();
 }
 

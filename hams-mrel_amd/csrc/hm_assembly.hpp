@@ -39,6 +39,28 @@ hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const doub
                    double yw, double* W, int ldw);
 hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                   double yw, const double* X, int ldx, double* amas, double* bdmp, double* exfc);
+// ---- batches of independent problems (one member = one problem) ----
+struct BatchProb {
+  GeomDev g;                   // the member's geometry (ni = 0)
+  double *A, *S, *W, *R, *X;   // n x n matrices and n x ncol blocks, leading dimension ld = n
+  long long ld;
+  int ncol, nbeta;
+  const double* beta;
+  double *amas, *bdmp, *exfc;  // 6nb x 6nb, 6nb x 6nb, 6nb x nbeta
+};
+struct BatchFreq {             // the member's wave state, rewritten per frequency
+  double v, w1, wk, h, rho, g, amp, xw, yw;
+  double wvn[kNk];
+};
+hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int n, unsigned* hist,
+                          unsigned long long* cursor, uint64_t* codes, bool count_only);
+hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, const BatchFreq* F,
+                                 const uint64_t* codes, long long np, int variant, int grid);
+hipError_t batch_assemble_all(hipStream_t s, bool deep, const BatchProb* P, const BatchFreq* F, const int* list,
+                              const long long* prefix, int nlist, long long total, int grid);
+hipError_t batch_build_w(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int nmax);
+hipError_t batch_forces(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int mmax, int ncolmax);
+
 hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const double* xp, const double* yf,
                        const double* yp, const double* zf, const double* zp, double v, const double* wvn, double h,
                        const int* flag, double* out);

@@ -1,0 +1,596 @@
+// hm_batch_api.cpp -- batches of independent problems (include/hm_bem.h, hm_batch_*).
+//
+// The reference solves one problem at a time: its driver loops over the wave
+// frequencies and calls the assembly, LU and force routines serially
+// (HAMS_Prog.f90:615-724). A batch holds many independent problems (members)
+// of different panel counts and depths resident in HBM at once and runs the
+// per-frequency chain for all of them together:
+//   * assembly: one persistent launch per finite-depth regime over the pairs of
+//     every finite-depth member (classified once into one list sorted by regime
+//     and series length across members), one over all pairs of the
+//     infinite-depth members;
+//   * right-hand sides, LU and triangular solves as grouped launches
+//     (blockIdx.z = member, each member at its own order);
+//   * FP64 LU, or FP32 LU on MFMA with a fixed number of FP64 refinement sweeps
+//     inside the chain and more after it where the correction has not yet
+//     fallen below 1e-13 of the solution;
+//   * the whole chain captured once in a HIP graph and replayed per frequency
+//     (the kernels read the members' wave state from device memory, which is
+//     rewritten before each replay).
+// An autotuner times the assembly launches of one frequency for the series
+// variants (lane-group shapes of the epsilon-accelerated regimes) and the
+// persistent grid sizes and keeps the fastest before the graph is captured.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "hm_assembly.hpp"
+#include "hm_bem.h"
+#include "hm_linalg.hpp"
+
+namespace {
+
+struct Member {
+  int n, nbody, nbeta, ncol, m;
+  double h;
+  std::vector<int> body_start;
+  std::vector<double> verts, fp, ds, pnsz, dxyz, beta;
+  std::vector<int> ncn;
+};
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t b) { return (b + kAlign - 1) & ~(kAlign - 1); }
+
+}  // namespace
+
+struct hm_batch {
+  hipStream_t stream = nullptr;
+  std::string err;
+  double rho = 1025.0, g = 9.81, g_disp = 9.81, amp = 1.0, xw = 0.0, yw = 0.0;
+  std::vector<Member> mem;
+  bool prepared = false;
+  int precision = HM_FP64, nrefine = 3, use_graph = 1;
+  // tuning
+  int variant = 1, grid = 0, cus = 256;
+  double tuned_ms = 0.0;
+  // device
+  void* pool = nullptr;
+  size_t pool_bytes = 0;
+  hm::BatchProb* d_prob = nullptr;
+  hm::BatchFreq* d_freq = nullptr;
+  hm::BatchFreq* h_freq = nullptr;    // pinned
+  uint64_t* d_codes = nullptr;
+  long long seg_off[4] = {0, 0, 0, 0}, seg_cnt[4] = {0, 0, 0, 0};
+  int* d_deep = nullptr;
+  long long* d_deep_prefix = nullptr;
+  int ndeep = 0;
+  long long deep_total = 0;
+  hm::GemmDesc<double>* d_rhs = nullptr;   // R = S W
+  hm::GemmDesc<double>* d_res = nullptr;   // Rw = Rw - A X
+  hm::LuDesc<double>* d_lu64 = nullptr;
+  hm::LuDesc<float>* d_lu32 = nullptr;
+  hm::ConvDesc* d_convA = nullptr;         // A -> A32
+  hm::ConvDesc* d_convR = nullptr;         // Rw -> X32 (demote)
+  hm::ConvDesc* d_convX = nullptr;         // X32 -> X (promote); correction ratio
+  hm::CopyDesc* d_copyR = nullptr;         // R -> Rw
+  int* d_info = nullptr;                   // info[2*count]: info, counter per member
+  double* d_ratio = nullptr;
+  std::vector<size_t> off_amas, off_bdmp, off_exfc;
+  std::vector<hm::BatchProb> h_prob;       // host copy of the member descriptors
+  double *d_amas = nullptr, *d_bdmp = nullptr, *d_exfc = nullptr;
+  size_t out_amas = 0, out_exfc = 0;
+  int nmax = 0, ncolmax = 0, mmax = 0;
+  long long max_nn = 0, max_ncol_elems = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double last_ratio = 0.0;
+  int extra_sweeps = 0;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+  int hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return HM_OK;
+    (void)hipGetLastError();
+    return fail(HM_ERR_HIP - (int)e, "%s: %s", what, hipGetErrorString(e));
+  }
+  int count() const { return (int)mem.size(); }
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    if (pool) (void)hipFree(pool);
+    pool = nullptr;
+    if (h_freq) (void)hipHostFree(h_freq);
+    h_freq = nullptr;
+    prepared = false;
+  }
+};
+
+#define BCHK(b, call, what)                     \
+  do {                                          \
+    int _rc = (b)->hip((call), (what));         \
+    if (_rc != HM_OK) return _rc;               \
+  } while (0)
+
+extern "C" int hm_dispersion(double* wvn, int nrt, double w, double h, double g_disp);
+
+extern "C" int hm_batch_create(hm_batch** out) {
+  if (!out) return HM_ERR_ARG - 1;
+  *out = nullptr;
+  hm_batch* b = new hm_batch();
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return HM_ERR_HIP;
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    b->cus = prop.multiProcessorCount;
+  *out = b;
+  return HM_OK;
+}
+
+extern "C" int hm_batch_destroy(hm_batch* b) {
+  if (!b) return HM_ERR_ARG - 1;
+  b->release();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return HM_OK;
+}
+
+extern "C" const char* hm_batch_error_string(const hm_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+extern "C" int hm_batch_set_constants(hm_batch* b, double rho, double g, double g_disp, double amp, double xw,
+                                      double yw) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (b->prepared) return b->fail(HM_ERR_STATE, "hm_batch_set_constants after hm_batch_prepare");
+  if (!(rho > 0.0) || !(g > 0.0) || !(g_disp > 0.0)) return b->fail(HM_ERR_ARG - 2, "rho, g and g_disp must be > 0");
+  b->rho = rho; b->g = g; b->g_disp = g_disp; b->amp = amp; b->xw = xw; b->yw = yw;
+  return HM_OK;
+}
+
+extern "C" int hm_batch_add_problem(hm_batch* b, int nbody, const int* body_start, int ntnd, const double* xyz,
+                                    const int* ncn, const int* ncon, const double* xyz_p, const double* ds,
+                                    const double* pnsz, const double* dxyz_p, double h, int nbeta,
+                                    const double* beta) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (b->prepared) return b->fail(HM_ERR_STATE, "hm_batch_add_problem after hm_batch_prepare");
+  if (nbody < 1) return b->fail(HM_ERR_ARG - 2, "nbody must be >= 1");
+  if (!body_start || body_start[0] != 0) return b->fail(HM_ERR_ARG - 3, "body_start must start at 0");
+  const int n = body_start[nbody];
+  for (int k = 0; k < nbody; ++k)
+    if (body_start[k + 1] <= body_start[k]) return b->fail(HM_ERR_ARG - 3, "body %d has no panels", k);
+  if (n >= (1 << 20)) return b->fail(HM_ERR_ARG - 3, "n = %d exceeds the 20-bit pair encoding", n);
+  if (b->count() >= (1 << 23)) return b->fail(HM_ERR_STATE, "too many members");
+  if (ntnd < 3 || !xyz) return b->fail(HM_ERR_ARG - 5, "nodes missing");
+  if (!ncn || !ncon || !xyz_p || !ds || !pnsz || !dxyz_p) return b->fail(HM_ERR_ARG - 6, "panel arrays missing");
+  if (h == 0.0) return b->fail(HM_ERR_ARG - 12, "depth must be > 0 (finite) or < 0 (infinite)");
+  if (nbeta < 0 || (nbeta > 0 && !beta)) return b->fail(HM_ERR_ARG - 13, "bad headings");
+  Member m;
+  m.n = n; m.nbody = nbody; m.nbeta = nbeta; m.m = 6 * nbody; m.ncol = 6 * nbody + nbeta; m.h = h;
+  m.body_start.assign(body_start, body_start + nbody + 1);
+  m.verts.assign((size_t)n * 12, 0.0);
+  for (int j = 0; j < n; ++j) {
+    if (ncn[j] != 3 && ncn[j] != 4) return b->fail(HM_ERR_ARG - 6, "panel %d has %d vertices", j + 1, ncn[j]);
+    for (int v = 0; v < ncn[j]; ++v) {
+      const int node = ncon[4 * j + v];
+      if (node < 1 || node > ntnd) return b->fail(HM_ERR_ARG - 7, "panel %d vertex %d out of range", j + 1, v + 1);
+      for (int k = 0; k < 3; ++k) m.verts[12 * (size_t)j + 3 * v + k] = xyz[3 * (size_t)(node - 1) + k];
+    }
+  }
+  m.fp.assign(xyz_p, xyz_p + 3 * (size_t)n);
+  m.ds.assign(ds, ds + n);
+  m.pnsz.assign(pnsz, pnsz + n);
+  m.dxyz.assign(dxyz_p, dxyz_p + 6 * (size_t)n);
+  m.ncn.assign(ncn, ncn + n);
+  m.beta.assign(beta, beta + nbeta);
+  b->mem.push_back(std::move(m));
+  return b->count() - 1;
+}
+
+namespace {
+
+// the per-frequency chain (assembly .. forces) on b->stream, with the current
+// tuning; capturable (no memset, no synchronisation, fixed shapes)
+int issue_chain(hm_batch* b) {
+  hipStream_t s = b->stream;
+  const int cnt = b->count();
+  // assembly
+  BCHK(b, hm::batch_assemble_all(s, true, b->d_prob, b->d_freq, b->d_deep, b->d_deep_prefix, b->ndeep,
+                                 b->deep_total, b->grid), "batch assembly (deep)");
+  for (int r = 0; r < 4; ++r)
+    BCHK(b, hm::batch_assemble_regime(s, r, b->d_prob, b->d_freq, b->d_codes + b->seg_off[r], b->seg_cnt[r],
+                                      b->variant, b->grid), "batch assembly");
+  // right-hand sides R = S W
+  BCHK(b, hm::batch_build_w(s, b->d_prob, b->d_freq, cnt, b->nmax), "batch rhs");
+  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->nmax, b->ncolmax), "batch rhs gemm");
+  // factor and solve
+  BCHK(b, hm::fill_i32(s, b->d_info, 2LL * cnt, 0), "batch factor");
+  if (b->precision == HM_FP64) {
+    BCHK(b, hm::lu_factor_grouped<double>(s, b->d_lu64, cnt, b->nmax), "batch lu");
+    BCHK(b, hm::lu_solve_grouped<double>(s, b->d_lu64, cnt, b->nmax, b->ncolmax), "batch solve");
+  } else {
+    BCHK(b, hm::convert_grouped(s, b->d_convA, cnt, b->max_nn, true, false), "batch demote");
+    BCHK(b, hm::lu_factor_grouped<float>(s, b->d_lu32, cnt, b->nmax), "batch lu fp32");
+    for (int it = 0; it < b->nrefine; ++it) {
+      // residual Rw = R - A X (the first sweep has X = 0: Rw = R)
+      BCHK(b, hm::copy_grouped(s, b->d_copyR, cnt, b->max_ncol_elems), "batch residual");
+      if (it > 0) BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax), "batch residual");
+      BCHK(b, hm::convert_grouped(s, b->d_convR, cnt, b->max_ncol_elems, true, false), "batch demote");
+      BCHK(b, hm::lu_solve_grouped<float>(s, b->d_lu32, cnt, b->nmax, b->ncolmax), "batch solve fp32");
+      BCHK(b, hm::convert_grouped(s, b->d_convX, cnt, b->max_ncol_elems, false, it == 0), "batch update");
+    }
+  }
+  BCHK(b, hm::batch_forces(s, b->d_prob, b->d_freq, cnt, b->mmax, b->ncolmax), "batch forces");
+  return HM_OK;
+}
+
+// one more FP32 refinement sweep outside the graph (same launches as a sweep in it)
+int extra_sweep(hm_batch* b) {
+  hipStream_t s = b->stream;
+  const int cnt = b->count();
+  BCHK(b, hm::copy_grouped(s, b->d_copyR, cnt, b->max_ncol_elems), "batch residual");
+  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax), "batch residual");
+  BCHK(b, hm::convert_grouped(s, b->d_convR, cnt, b->max_ncol_elems, true, false), "batch demote");
+  BCHK(b, hm::lu_solve_grouped<float>(s, b->d_lu32, cnt, b->nmax, b->ncolmax), "batch solve fp32");
+  BCHK(b, hm::convert_grouped(s, b->d_convX, cnt, b->max_ncol_elems, false, false), "batch update");
+  return HM_OK;
+}
+
+int write_freq(hm_batch* b, const double* w1) {
+  for (int p = 0; p < b->count(); ++p) {
+    const Member& m = b->mem[p];
+    hm::BatchFreq& f = b->h_freq[p];
+    if (!(w1[p] > 0.0)) return b->fail(HM_ERR_ARG - 2, "w1[%d] must be > 0", p);
+    f.w1 = w1[p];
+    f.v = w1[p] * w1[p] / b->g;
+    f.h = m.h;
+    f.rho = b->rho; f.g = b->g; f.amp = b->amp; f.xw = b->xw; f.yw = b->yw;
+    if (m.h > 0.0) {
+      hm_dispersion(f.wvn, hm::kNk, w1[p], m.h, b->g_disp);
+      f.wk = f.wvn[0];
+    } else {
+      f.wk = f.v;
+      for (int k = 0; k < hm::kNk; ++k) f.wvn[k] = 0.0;
+    }
+  }
+  BCHK(b, hipMemcpyAsync(b->d_freq, b->h_freq, sizeof(hm::BatchFreq) * b->count(), hipMemcpyHostToDevice, b->stream),
+       "batch frequency");
+  return HM_OK;
+}
+
+// time the assembly launches of one frequency for the current tuning
+int time_assembly(hm_batch* b, double* ms) {
+  hipEvent_t e0, e1;
+  BCHK(b, hipEventCreate(&e0), "tune");
+  BCHK(b, hipEventCreate(&e1), "tune");
+  int rc = HM_OK;
+  float best = 1e30f;
+  for (int rep = 0; rep < 2 && rc == HM_OK; ++rep) {
+    (void)hipEventRecord(e0, b->stream);
+    rc = b->hip(hm::batch_assemble_all(b->stream, true, b->d_prob, b->d_freq, b->d_deep, b->d_deep_prefix, b->ndeep,
+                                       b->deep_total, b->grid), "tune");
+    for (int r = 0; r < 4 && rc == HM_OK; ++r)
+      rc = b->hip(hm::batch_assemble_regime(b->stream, r, b->d_prob, b->d_freq, b->d_codes + b->seg_off[r],
+                                            b->seg_cnt[r], b->variant, b->grid), "tune");
+    (void)hipEventRecord(e1, b->stream);
+    if (rc == HM_OK) rc = b->hip(hipEventSynchronize(e1), "tune");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    best = t < best ? t : best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int use_graph, int nrefine) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (b->count() == 0) return b->fail(HM_ERR_STATE, "hm_batch_prepare with no problems");
+  if (precision != HM_FP64 && precision != HM_FP32_IR) return b->fail(HM_ERR_ARG - 2, "precision must be HM_FP64 or HM_FP32_IR");
+  if (nrefine < 1 || nrefine > 20) return b->fail(HM_ERR_ARG - 5, "nrefine must be in 1..20");
+  b->release();
+  b->precision = precision;
+  b->use_graph = use_graph != 0;
+  b->nrefine = nrefine;
+  const int cnt = b->count();
+  const bool ir = precision == HM_FP32_IR;
+  // ---- layout of the pool
+  struct Off { size_t body, fp, verts, ncn, ds, pnsz, dxyz, beta, A, S, W, R, Rw, A32, X32, ipiv, amas, bdmp, exfc; };
+  std::vector<Off> off(cnt);
+  size_t tot = 0;
+  auto take = [&](size_t bytes) { size_t o = tot; tot += align_up(bytes ? bytes : 8); return o; };
+  b->nmax = b->ncolmax = b->mmax = 0;
+  b->max_nn = b->max_ncol_elems = 0;
+  long long finite_pairs = 0;
+  for (int p = 0; p < cnt; ++p) {
+    const Member& m = b->mem[p];
+    const size_t n = m.n, nc = m.ncol, mm = m.m;
+    Off& o = off[p];
+    o.body = take(sizeof(int) * (m.nbody + 1));
+    o.fp = take(24 * n); o.verts = take(96 * n); o.ncn = take(4 * n); o.ds = take(8 * n); o.pnsz = take(8 * n);
+    o.dxyz = take(48 * n); o.beta = take(8 * (size_t)m.nbeta);
+    o.A = take(16 * n * n); o.S = take(16 * n * n); o.W = take(16 * n * nc); o.R = take(16 * n * nc);
+    o.Rw = ir ? take(16 * n * nc) : 0;
+    o.A32 = ir ? take(8 * n * n) : 0;
+    o.X32 = ir ? take(8 * n * nc) : 0;
+    o.ipiv = take(4 * n);
+    o.amas = take(8 * mm * mm); o.bdmp = take(8 * mm * mm); o.exfc = take(16 * mm * (size_t)m.nbeta);
+    b->nmax = m.n > b->nmax ? m.n : b->nmax;
+    b->ncolmax = m.ncol > b->ncolmax ? m.ncol : b->ncolmax;
+    b->mmax = m.m > b->mmax ? m.m : b->mmax;
+    b->max_nn = (long long)n * n > b->max_nn ? (long long)n * n : b->max_nn;
+    b->max_ncol_elems = (long long)n * nc > b->max_ncol_elems ? (long long)n * nc : b->max_ncol_elems;
+    if (m.h > 0.0) finite_pairs += (long long)n * n;
+  }
+  const size_t o_prob = take(sizeof(hm::BatchProb) * cnt), o_freq = take(sizeof(hm::BatchFreq) * cnt);
+  const size_t o_codes = take(8 * (size_t)finite_pairs);
+  const size_t o_deep = take(4 * (size_t)cnt), o_prefix = take(8 * (size_t)(cnt + 1));
+  const size_t o_rhs = take(sizeof(hm::GemmDesc<double>) * cnt), o_res = take(sizeof(hm::GemmDesc<double>) * cnt);
+  const size_t o_lu64 = take(sizeof(hm::LuDesc<double>) * cnt), o_lu32 = take(sizeof(hm::LuDesc<float>) * cnt);
+  const size_t o_convA = take(sizeof(hm::ConvDesc) * cnt), o_convR = take(sizeof(hm::ConvDesc) * cnt);
+  const size_t o_convX = take(sizeof(hm::ConvDesc) * cnt), o_copyR = take(sizeof(hm::CopyDesc) * cnt);
+  const size_t o_info = take(8 * (size_t)cnt), o_ratio = take(8 * (size_t)cnt);
+  const size_t o_hist = take(sizeof(unsigned) * hm::kNumKeys), o_cursor = take(8 * hm::kNumKeys);
+  BCHK(b, hipMalloc(&b->pool, tot), "batch alloc");
+  b->pool_bytes = tot;
+  BCHK(b, hipHostMalloc((void**)&b->h_freq, sizeof(hm::BatchFreq) * cnt, hipHostMallocDefault), "batch alloc");
+  char* base = static_cast<char*>(b->pool);
+  auto at = [&](size_t o) { return static_cast<void*>(base + o); };
+  b->d_prob = (hm::BatchProb*)at(o_prob);
+  b->d_freq = (hm::BatchFreq*)at(o_freq);
+  b->d_codes = (uint64_t*)at(o_codes);
+  b->d_deep = (int*)at(o_deep);
+  b->d_deep_prefix = (long long*)at(o_prefix);
+  b->d_rhs = (hm::GemmDesc<double>*)at(o_rhs);
+  b->d_res = (hm::GemmDesc<double>*)at(o_res);
+  b->d_lu64 = (hm::LuDesc<double>*)at(o_lu64);
+  b->d_lu32 = (hm::LuDesc<float>*)at(o_lu32);
+  b->d_convA = (hm::ConvDesc*)at(o_convA);
+  b->d_convR = (hm::ConvDesc*)at(o_convR);
+  b->d_convX = (hm::ConvDesc*)at(o_convX);
+  b->d_copyR = (hm::CopyDesc*)at(o_copyR);
+  b->d_info = (int*)at(o_info);
+  b->d_ratio = (double*)at(o_ratio);
+  // ---- descriptors and geometry
+  std::vector<hm::BatchProb> prob(cnt);
+  std::vector<hm::GemmDesc<double>> rhs(cnt), res(cnt);
+  std::vector<hm::LuDesc<double>> lu64(cnt);
+  std::vector<hm::LuDesc<float>> lu32(cnt);
+  std::vector<hm::ConvDesc> convA(cnt), convR(cnt), convX(cnt);
+  std::vector<hm::CopyDesc> copyR(cnt);
+  std::vector<int> deep;
+  std::vector<long long> prefix(1, 0);
+  b->off_amas.assign(cnt, 0);
+  b->off_bdmp.assign(cnt, 0);
+  b->off_exfc.assign(cnt, 0);
+  b->out_amas = b->out_exfc = 0;
+  for (int p = 0; p < cnt; ++p) {
+    const Member& m = b->mem[p];
+    const Off& o = off[p];
+    const size_t n = m.n;
+    BCHK(b, hipMemcpyAsync(at(o.body), m.body_start.data(), sizeof(int) * (m.nbody + 1), hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.fp), m.fp.data(), 24 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.verts), m.verts.data(), 96 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.ncn), m.ncn.data(), 4 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.ds), m.ds.data(), 8 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.pnsz), m.pnsz.data(), 8 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.dxyz), m.dxyz.data(), 48 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    if (m.nbeta)
+      BCHK(b, hipMemcpyAsync(at(o.beta), m.beta.data(), 8 * (size_t)m.nbeta, hipMemcpyHostToDevice, b->stream), "batch upload");
+    hm::BatchProb& bp = prob[p];
+    bp.g.n = m.n; bp.g.ni = 0; bp.g.nt = m.n; bp.g.nbody = m.nbody;
+    bp.g.body_start = (const int*)at(o.body);
+    bp.g.fp = (const double*)at(o.fp);
+    bp.g.xyz_p = (const double*)at(o.fp);
+    bp.g.verts = (const double*)at(o.verts);
+    bp.g.ncn = (const int*)at(o.ncn);
+    bp.g.ds = (const double*)at(o.ds);
+    bp.g.pnsz = (const double*)at(o.pnsz);
+    bp.g.dxyz = (const double*)at(o.dxyz);
+    bp.A = (double*)at(o.A); bp.S = (double*)at(o.S); bp.W = (double*)at(o.W); bp.R = (double*)at(o.R);
+    bp.X = bp.R;   // FP64: the solve overwrites the right-hand sides (IR: set below)
+    bp.ld = m.n;
+    bp.ncol = m.ncol;
+    bp.nbeta = m.nbeta;
+    bp.beta = (const double*)at(o.beta);
+    bp.amas = (double*)at(o.amas); bp.bdmp = (double*)at(o.bdmp); bp.exfc = (double*)at(o.exfc);
+    // R = S W
+    rhs[p] = {m.n, m.ncol, m.n, 1.0, 0.0, 0.0, 0.0, bp.S, (long long)n, bp.W, (long long)n, bp.R, (long long)n};
+    int* info = b->d_info + 2 * p;
+    lu64[p] = {bp.A, (long long)n, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), bp.R, (long long)n, m.ncol};
+    if (ir) {
+      // IR: R keeps the right-hand sides, X accumulates in W's slot (W is dead
+      // once R = S W is formed), Rw holds residuals, X32 the FP32 corrections
+      double* X = bp.W;
+      double* Rw = (double*)at(o.Rw);
+      float* A32 = (float*)at(o.A32);
+      float* X32 = (float*)at(o.X32);
+      bp.X = X;
+      lu32[p] = {A32, (long long)n, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), X32, (long long)n, m.ncol};
+      convA[p] = {bp.A, (long long)n, A32, (long long)n, m.n, m.n};
+      // demote Rw -> X32; promote X32 -> X (overwrite on the first sweep, add later)
+      convR[p] = {Rw, (long long)n, X32, (long long)n, m.n, m.ncol};
+      convX[p] = {X, (long long)n, X32, (long long)n, m.n, m.ncol};
+      copyR[p] = {bp.R, (long long)n, Rw, (long long)n, m.n, m.ncol};
+      res[p] = {m.n, m.ncol, m.n, -1.0, 0.0, 1.0, 0.0, bp.A, (long long)n, X, (long long)n, Rw, (long long)n};
+    }
+    if (m.h < 0.0) {
+      deep.push_back(p);
+      prefix.push_back(prefix.back() + (long long)n * n);
+    }
+    b->off_amas[p] = b->out_amas;
+    b->off_bdmp[p] = b->out_amas;
+    b->off_exfc[p] = b->out_exfc;
+    b->out_amas += (size_t)m.m * m.m;
+    b->out_exfc += 2 * (size_t)m.m * m.nbeta;
+  }
+  b->h_prob = prob;
+  b->ndeep = (int)deep.size();
+  b->deep_total = prefix.back();
+  BCHK(b, hipMemcpyAsync(b->d_prob, prob.data(), sizeof(hm::BatchProb) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  BCHK(b, hipMemcpyAsync(b->d_rhs, rhs.data(), sizeof(rhs[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  BCHK(b, hipMemcpyAsync(b->d_lu64, lu64.data(), sizeof(lu64[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  if (ir) {
+    BCHK(b, hipMemcpyAsync(b->d_res, res.data(), sizeof(res[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_lu32, lu32.data(), sizeof(lu32[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_convA, convA.data(), sizeof(convA[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_convR, convR.data(), sizeof(convR[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_convX, convX.data(), sizeof(convX[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_copyR, copyR.data(), sizeof(copyR[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  }
+  if (b->ndeep) {
+    BCHK(b, hipMemcpyAsync(b->d_deep, deep.data(), sizeof(int) * deep.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_deep_prefix, prefix.data(), sizeof(long long) * prefix.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
+  }
+  // wave state needs depths for the classification
+  for (int p = 0; p < cnt; ++p) {
+    memset(&b->h_freq[p], 0, sizeof(hm::BatchFreq));
+    b->h_freq[p].h = b->mem[p].h;
+  }
+  BCHK(b, hipMemcpyAsync(b->d_freq, b->h_freq, sizeof(hm::BatchFreq) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  // ---- classification of the finite-depth members' pairs into one sorted list
+  unsigned* d_hist = (unsigned*)at(o_hist);
+  unsigned long long* d_cursor = (unsigned long long*)at(o_cursor);
+  BCHK(b, hipMemsetAsync(d_hist, 0, sizeof(unsigned) * hm::kNumKeys, b->stream), "batch classify");
+  for (int p = 0; p < cnt; ++p)
+    if (b->mem[p].h > 0.0)
+      BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, d_hist, nullptr, nullptr, true), "batch classify");
+  std::vector<unsigned> hist(hm::kNumKeys);
+  BCHK(b, hipMemcpyAsync(hist.data(), d_hist, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, b->stream), "batch classify");
+  BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
+  std::vector<unsigned long long> cur(hm::kNumKeys);
+  unsigned long long acc = 0;
+  for (int k = 0; k < hm::kNumKeys; ++k) { cur[k] = acc; acc += hist[k]; }
+  if ((long long)acc != finite_pairs) return b->fail(HM_ERR_STATE, "batch classification lost pairs");
+  for (int r = 0; r < 4; ++r) {
+    b->seg_off[r] = (long long)cur[r * hm::kKeyStride];
+    b->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
+  }
+  BCHK(b, hipMemcpyAsync(d_cursor, cur.data(), 8 * hm::kNumKeys, hipMemcpyHostToDevice, b->stream), "batch classify");
+  for (int p = 0; p < cnt; ++p)
+    if (b->mem[p].h > 0.0)
+      BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, nullptr, d_cursor, b->d_codes, false), "batch classify");
+  BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
+  // ---- autotuning on the middle of the frequency range (1 rad/s)
+  b->variant = 1;
+  b->grid = 16 * b->cus;
+  b->tuned_ms = 0.0;
+  if (autotune) {
+    std::vector<double> w(cnt, 1.0);
+    int rc = write_freq(b, w.data());
+    if (rc) return rc;
+    double best = 1e30;
+    int bv = b->variant;
+    for (int v = 0; v <= 2; ++v) {
+      b->variant = v;
+      double ms;
+      if ((rc = time_assembly(b, &ms))) return rc;
+      if (ms < best) { best = ms; bv = v; }
+    }
+    b->variant = bv;
+    int bg = b->grid;
+    for (int mult : {4, 8, 16, 32, 64}) {
+      b->grid = mult * b->cus;
+      double ms;
+      if ((rc = time_assembly(b, &ms))) return rc;
+      if (ms < best) { best = ms; bg = b->grid; }
+    }
+    b->grid = bg;
+    b->tuned_ms = best;
+  }
+  // ---- graph
+  if (b->use_graph) {
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "batch capture");
+    int rc = issue_chain(b);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+    BCHK(b, e, "batch capture");
+    b->graph = gr;
+    BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "batch graph instantiate");
+  }
+  b->prepared = true;
+  return HM_OK;
+}
+
+extern "C" int hm_batch_solve(hm_batch* b, const double* w1, double* amas, double* bdmp, double* exfc) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (!b->prepared) return b->fail(HM_ERR_STATE, "hm_batch_solve before hm_batch_prepare");
+  if (!w1) return b->fail(HM_ERR_ARG - 2, "w1 is NULL");
+  int rc = write_freq(b, w1);
+  if (rc) return rc;
+  if (b->use_graph) BCHK(b, hipGraphLaunch(b->exec, b->stream), "batch graph launch");
+  else if ((rc = issue_chain(b))) return rc;
+  // zero pivots
+  const int cnt = b->count();
+  std::vector<int> info(2 * (size_t)cnt);
+  BCHK(b, hipMemcpyAsync(info.data(), b->d_info, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost, b->stream), "batch info");
+  b->last_ratio = 0.0;
+  b->extra_sweeps = 0;
+  if (b->precision == HM_FP32_IR) {
+    // refinement beyond the captured sweeps until every member's last correction
+    // is below 1e-13 of its solution (at most 10 more)
+    std::vector<double> ratio(cnt);
+    for (;;) {
+      BCHK(b, hm::correction_ratio_grouped(b->stream, b->d_convX, cnt, b->d_ratio), "batch refinement");
+      BCHK(b, hipMemcpyAsync(ratio.data(), b->d_ratio, sizeof(double) * cnt, hipMemcpyDeviceToHost, b->stream), "batch refinement");
+      BCHK(b, hipStreamSynchronize(b->stream), "batch refinement");
+      double worst = 0.0;
+      for (double r : ratio) worst = fmax(worst, r);
+      b->last_ratio = worst;
+      if (worst < 1e-13 || b->extra_sweeps >= 10) break;
+      if ((rc = extra_sweep(b))) return rc;
+      ++b->extra_sweeps;
+    }
+    if (b->extra_sweeps)
+      BCHK(b, hm::batch_forces(b->stream, b->d_prob, b->d_freq, cnt, b->mmax, b->ncolmax), "batch forces");
+  }
+  BCHK(b, hipStreamSynchronize(b->stream), "batch solve");
+  for (int p = 0; p < cnt; ++p)
+    if (info[2 * p] > 0) return b->fail(info[2 * p], "member %d: zero pivot in column %d", p, info[2 * p]);
+  // outputs, stacked by member
+  const std::vector<hm::BatchProb>& prob = b->h_prob;
+  for (int p = 0; p < cnt; ++p) {
+    const Member& m = b->mem[p];
+    const size_t mm = (size_t)m.m * m.m;
+    if (amas) BCHK(b, hipMemcpyAsync(amas + b->off_amas[p], prob[p].amas, 8 * mm, hipMemcpyDeviceToHost, b->stream), "batch output");
+    if (bdmp) BCHK(b, hipMemcpyAsync(bdmp + b->off_bdmp[p], prob[p].bdmp, 8 * mm, hipMemcpyDeviceToHost, b->stream), "batch output");
+    if (exfc && m.nbeta)
+      BCHK(b, hipMemcpyAsync(exfc + b->off_exfc[p], prob[p].exfc, 16 * (size_t)m.m * m.nbeta, hipMemcpyDeviceToHost, b->stream), "batch output");
+  }
+  BCHK(b, hipStreamSynchronize(b->stream), "batch output");
+  return HM_OK;
+}
+
+extern "C" int hm_batch_info(hm_batch* b, int* members, int* nmax, int* series_variant, int* grid_blocks,
+                             double* tuned_assembly_ms, double* last_correction, int* extra_sweeps,
+                             long long* device_bytes) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (members) *members = b->count();
+  if (nmax) *nmax = b->nmax;
+  if (series_variant) *series_variant = b->variant;
+  if (grid_blocks) *grid_blocks = b->grid;
+  if (tuned_assembly_ms) *tuned_assembly_ms = b->tuned_ms;
+  if (last_correction) *last_correction = b->last_ratio;
+  if (extra_sweeps) *extra_sweeps = b->extra_sweeps;
+  if (device_bytes) *device_bytes = (long long)b->pool_bytes;
+  return HM_OK;
+}

@@ -33,8 +33,14 @@ __device__ __forceinline__ Proc proc(const HmatDev& h, int p) {
   return q;
 }
 
-__device__ __forceinline__ void entry(const GeomDev& g, const FreqDev& f, const double* wvn, int mat, int i, int j,
-                                      double& er, double& ei) {
+__device__ __forceinline__ void entry(const GeomDev& g, const FreqDev& f, const double* wvn, const HmatDev& h,
+                                      int mat, int i, int j, double& er, double& ei) {
+  if (h.dense) {
+    const long long o = (long long)i + (long long)j * h.ldd;
+    er = h.dense[2 * o];
+    ei = h.dense[2 * o + 1];
+    return;
+  }
   double ar, ai, sr, si;
   pair_entry(g, f, wvn, i, j, ar, ai, sr, si);
   er = mat ? sr : ar;
@@ -76,9 +82,9 @@ __device__ void block_sum4(double* v, double (*sh)[256]) {
 }
 
 // residual of one entry: A(b1 m + i, b2 m + j) - sum_{l<k} U(i, l) V(j, l)
-__device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const Proc& q,
-                                               int m, int k, int i, int j, double& er, double& ei) {
-  entry(g, f, wvn, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
+__device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const HmatDev& h,
+                                               const Proc& q, int m, int k, int i, int j, double& er, double& ei) {
+  entry(g, f, wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
   for (int l = 0; l < k; ++l) {
     const double ur = q.U[2 * (i + (long long)l * m)], ui = q.U[2 * (i + (long long)l * m) + 1];
     const double vr = q.V[2 * (j + (long long)l * m)], vi = q.V[2 * (j + (long long)l * m) + 1];
@@ -93,14 +99,14 @@ __global__ __launch_bounds__(256) void k_aca_eval_row(GeomDev g, FreqDev f, Hmat
   const int p = blockIdx.y;
   if (h.conv[p]) return;
   __shared__ double wvn[kNk];
-  if (f.h > 0.0)
+  if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= h.m) return;
   const Proc q = proc(h, p);
   double er, ei;
-  residual_entry(g, f, wvn, q, h.m, h.rank[p], h.next_row[p], j, er, ei);
+  residual_entry(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, er, ei);
   double* rb = h.rowbuf + 2LL * p * h.m;
   rb[2 * j] = er;
   rb[2 * j + 1] = ei;
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(256) void k_aca_eval_col(GeomDev g, FreqDev f, Hmat
   const int p = blockIdx.y;
   if (h.conv[p] || h.skip[p]) return;
   __shared__ double wvn[kNk];
-  if (f.h > 0.0)
+  if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,7 +172,7 @@ __global__ __launch_bounds__(256) void k_aca_eval_col(GeomDev g, FreqDev f, Hmat
   const Proc q = proc(h, p);
   const int k = h.rank[p];
   double er, ei;
-  residual_entry(g, f, wvn, q, h.m, k, i, h.pivot[p], er, ei);
+  residual_entry(g, f, wvn, h, q, h.m, k, i, h.pivot[p], er, ei);
   q.U[2 * (i + (long long)k * h.m)] = er;
   q.U[2 * (i + (long long)k * h.m) + 1] = ei;
 }
@@ -285,8 +291,8 @@ __global__ void k_permute(int nb, int kmax, int ncol, const T* __restrict__ tb, 
 
 }  // namespace
 
-size_t hmat_bytes(int nb, int m, int kmax) {
-  const size_t P = (size_t)hmat_processes(nb);
+size_t hmat_bytes(int nb, int m, int kmax, int nmat) {
+  const size_t P = (size_t)hmat_processes(nb, nmat);
   size_t b = 0;
   b += 2 * 16 * P * m * kmax;      // U, V (complex FP64)
   b += 16 * P * m;                 // rowbuf
@@ -297,8 +303,9 @@ size_t hmat_bytes(int nb, int m, int kmax) {
   return b + 13 * 256;             // alignment of the 13 arrays
 }
 
-HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s) {
-  const size_t P = (size_t)hmat_processes(nb);
+HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s, int nmat, const double* dense,
+                   long long ldd) {
+  const size_t P = (size_t)hmat_processes(nb, nmat);
   char* p = static_cast<char*>(base);
   auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) & ~(size_t)255; return q; };
   HmatDev h;
@@ -307,6 +314,9 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.kmax = kmax;
   h.eps = eps;
   h.eps_s = eps_s;
+  h.nmat = nmat;
+  h.dense = dense;
+  h.ldd = ldd;
   h.U = (double*)take(16 * P * m * kmax);
   h.V = (double*)take(16 * P * m * kmax);
   h.rowbuf = (double*)take(16 * P * m);
@@ -324,7 +334,7 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
 
 hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
                      int* steps) {
-  const int P = hmat_processes(h.nb);
+  const int P = hmat_processes(h.nb, h.nmat);
   const size_t uv = 16 * (size_t)P * h.m * h.kmax;
   hipError_t e;
   if ((e = hipMemsetAsync(h.U, 0, uv, s)) != hipSuccess) return e;

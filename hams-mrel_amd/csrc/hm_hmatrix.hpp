@@ -34,6 +34,9 @@ namespace hm {
 
 struct HmatDev {
   int nb, m, kmax;
+  int nmat;          // 2: A and S from the Green function; 1: A only, from `dense`
+  const double* dense;  // when set: entries A(i, j) = dense[i + j ldd] of an assembled matrix
+  long long ldd;
   double eps, eps_s;   // stopping tolerance for A (mat 0) and S (mat 1)
   double* U;         // [mat][b1][b2'] m x kmax, ld m   (b2' = b2 without b1)
   double* V;         // [mat][b2][b1'] m x kmax, ld m   (b1' = b1 without b2)
@@ -44,12 +47,14 @@ struct HmatDev {
   int* counter;                                 // one int for host polls
 };
 
-inline int hmat_processes(int nb) { return 2 * nb * (nb - 1); }
-size_t hmat_bytes(int nb, int m, int kmax);   // device memory for HmatDev arrays
-HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s);
+inline int hmat_processes(int nb, int nmat = 2) { return nmat * nb * (nb - 1); }
+size_t hmat_bytes(int nb, int m, int kmax, int nmat = 2);   // device memory for HmatDev arrays
+HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s, int nmat = 2,
+                   const double* dense = nullptr, long long ldd = 0);
 
-// cross approximation of all off-diagonal blocks of A (mat 0) and S (mat 1);
-// returns the number of unconverged processes in *unconverged
+// cross approximation of all off-diagonal blocks of A (mat 0) and S (mat 1), or
+// of the assembled matrix `dense` (nmat = 1); returns the number of
+// unconverged processes in *unconverged
 hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
                      int* steps);
 

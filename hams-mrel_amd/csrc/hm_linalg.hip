@@ -48,19 +48,15 @@ __device__ __forceinline__ vec4<float>::type mfma16(float a, float b, vec4<float
 
 constexpr int kBM = 64, kBN = 64, kBK = 16;
 
-// C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C; blockIdx.z = batch index.
+// One 64 x 64 tile (rows row0.., columns col0..) of
+// C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C.
 template <typename T, bool TA>
-__global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, const T* __restrict__ A,
-                                               int lda, long long sA, const T* __restrict__ B, int ldb,
-                                               long long sB, T br, T bi, T* __restrict__ C, int ldc,
-                                               long long sC) {
+__device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, const T* __restrict__ A, int lda,
+                                           const T* __restrict__ B, int ldb, T br, T bi, T* __restrict__ C,
+                                           int ldc, int row0, int col0) {
   __shared__ T as_re[kBK][kBM + 1], as_im[kBK][kBM + 1], bs_re[kBK][kBN + 1], bs_im[kBK][kBN + 1];
   typedef typename vec4<T>::type v4;
-  A += 2 * sA * blockIdx.z;
-  B += 2 * sB * blockIdx.z;
-  C += 2 * sC * blockIdx.z;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   v4 cre[2][2], cim[2][2];
   for (int i = 0; i < 2; ++i)
@@ -133,6 +129,26 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, 
       }
 }
 
+// blockIdx.z = batch index
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, const T* __restrict__ A,
+                                               int lda, long long sA, const T* __restrict__ B, int ldb,
+                                               long long sB, T br, T bi, T* __restrict__ C, int ldc,
+                                               long long sC) {
+  cgemm_tile<T, TA>(m, n, k, ar, ai, A + 2 * sA * blockIdx.z, lda, B + 2 * sB * blockIdx.z, ldb, br, bi,
+                    C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
+}
+
+// blockIdx.z = group member; tiles outside the member's m x n exit
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void k_cgemm_grouped(const GemmDesc<T>* __restrict__ d) {
+  const GemmDesc<T> g = d[blockIdx.z];
+  const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
+  if (row0 >= g.m || col0 >= g.n || g.k <= 0) return;
+  cgemm_tile<T, TA>(g.m, g.n, g.k, g.ar, g.ai, g.A, (int)g.lda, g.B, (int)g.ldb, g.br, g.bi, g.C, (int)g.ldc, row0,
+                    col0);
+}
+
 template <typename T>
 hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda,
                  long long sA, const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc,
@@ -143,6 +159,15 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
     hipLaunchKernelGGL((k_cgemm<T, true>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc, sC);
   else
     hipLaunchKernelGGL((k_cgemm<T, false>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc, sC);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax) {
+  if (mmax <= 0 || nmax <= 0 || count <= 0) return hipSuccess;
+  dim3 grid((mmax + kBM - 1) / kBM, (nmax + kBN - 1) / kBN, count);
+  if (transA) hipLaunchKernelGGL((k_cgemm_grouped<T, true>), grid, dim3(256), 0, s, d);
+  else hipLaunchKernelGGL((k_cgemm_grouped<T, false>), grid, dim3(256), 0, s, d);
   return hipGetLastError();
 }
 
@@ -211,11 +236,11 @@ __global__ __launch_bounds__(256) void k_lu_pivot(T* A, int lda, long long sA, i
 }
 
 // one panel column j (pivot already in place): scale L(:,j), rank-1 update of
-// panel columns (j, c1); the last workgroup to finish pivots column j+1.
+// panel columns (j, c1); the last of the nblk workgroups to finish pivots
+// column j+1. bx = this workgroup's index among the nblk.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int n, int j, int c0, int c1,
-                                                int* ipiv, int* info, unsigned* counter) {
-  T* a = A + 2 * sA * blockIdx.z;
+__device__ __forceinline__ void lu_col_body(T* a, int lda, int n, int j, int c0, int c1, int* ipiv, int* info,
+                                            unsigned* counter, int nblk, int bx) {
   __shared__ T u_re[kLuNb], u_im[kLuNb];
   __shared__ int last;
   const int ncu = c1 - j - 1;
@@ -230,7 +255,7 @@ __global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int
   T rr = 0, ri = 0;
   if (den > T(0)) { rr = dr / den; ri = -di / den; }
   __syncthreads();
-  const int r = j + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = j + 1 + bx * blockDim.x + threadIdx.x;
   if (r < n) {
     const long long ol = (long long)r + (long long)j * lda;
     const T xr = a[2 * ol], xi = a[2 * ol + 1];
@@ -246,24 +271,27 @@ __global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int
   if (j + 1 >= c1) return;
   __threadfence();
   __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(counter + blockIdx.z, 1u) == gridDim.x - 1);
+  if (threadIdx.x == 0) last = (atomicAdd(counter, 1u) == (unsigned)nblk - 1u);
   __syncthreads();
   if (!last) return;
   __threadfence();
   __shared__ int prow;
   __shared__ T pval;
   block_pivot_search(a, lda, n, j + 1, j + 1, &prow, &pval);
-  block_apply_pivot(a, lda, j + 1, prow, pval, c0, c1, ipiv + (long long)n * blockIdx.z, info + blockIdx.z);
-  if (threadIdx.x == 0) counter[blockIdx.z] = 0u;
+  block_apply_pivot(a, lda, j + 1, prow, pval, c0, c1, ipiv, info);
+  if (threadIdx.x == 0) *counter = 0u;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int n, int j, int c0, int c1,
+                                                int* ipiv, int* info, unsigned* counter) {
+  lu_col_body<T>(A + 2 * sA * blockIdx.z, lda, n, j, c0, c1, ipiv + (long long)n * blockIdx.z, info + blockIdx.z,
+                 counter + blockIdx.z, gridDim.x, blockIdx.x);
 }
 
 // apply the panel's swaps (rows [k0,k1)) to columns [0,k0) and [k1,n)
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_laswp(T* A, int lda, long long sA, int n, int k0, int k1,
-                                                  const int* ipiv) {
-  T* a = A + 2 * sA * blockIdx.z;
-  const int* pv = ipiv + (long long)n * blockIdx.z;
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void laswp_body(T* a, int lda, int n, int k0, int k1, const int* pv, int c) {
   if (c >= n - (k1 - k0)) return;
   if (c >= k0) c += k1 - k0;
   T* col = a + 2 * (long long)c * lda;
@@ -279,15 +307,19 @@ __global__ __launch_bounds__(256) void k_lu_laswp(T* A, int lda, long long sA, i
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_laswp(T* A, int lda, long long sA, int n, int k0, int k1,
+                                                  const int* ipiv) {
+  laswp_body<T>(A + 2 * sA * blockIdx.z, lda, n, k0, k1, ipiv + (long long)n * blockIdx.z,
+                blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // X[kb x ncols] <- L^{-1} X (unit lower, L = T[0:kb,0:kb]) or U^{-1} X (upper).
 // 64 lanes per right-hand side, 4 right-hand sides per workgroup.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long long sT, int kb, T* X, int ldx,
-                                                    long long sX, int ncols) {
+__device__ __forceinline__ void trsm_body(const T* t, int ldt, int kb, T* x, int ldx, int ncols, int bx) {
   __shared__ T l_re[kLuNb][kLuNb + 1], l_im[kLuNb][kLuNb + 1];
   __shared__ T x_re[4][kLuNb], x_im[4][kLuNb];
-  const T* t = Tm + 2 * sT * blockIdx.z;
-  T* x = X + 2 * sX * blockIdx.z;
   for (int e = threadIdx.x; e < kb * kb; e += blockDim.x) {
     const int r = e % kb, c = e / kb;
     const long long off = (long long)r + (long long)c * ldt;
@@ -295,7 +327,7 @@ __global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long l
     l_im[r][c] = t[2 * off + 1];
   }
   const int q = threadIdx.x >> 6, r = threadIdx.x & 63;
-  const int col = blockIdx.x * 4 + q;
+  const int col = bx * 4 + q;
   const bool act = col < ncols && r < kb;
   if (act) {
     const long long off = (long long)r + (long long)col * ldx;
@@ -336,6 +368,12 @@ __global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long l
   }
 }
 
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long long sT, int kb, T* X, int ldx,
+                                                    long long sX, int ncols) {
+  trsm_body<T, UPPER>(Tm + 2 * sT * blockIdx.z, ldt, kb, X + 2 * sX * blockIdx.z, ldx, ncols, blockIdx.x);
+}
+
 template <typename T>
 hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipiv, int* info,
                      unsigned* counter, int batch) {
@@ -367,12 +405,9 @@ hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipi
 
 // B[n x nrhs] rows permuted by the pivot sequence (one thread per column)
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply_pivots(T* B, int ldb, long long sB, int n, int nrhs,
-                                                      const int* ipiv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void apply_pivots_body(T* B, int ldb, int n, int nrhs, const int* pv, int c) {
   if (c >= nrhs) return;
-  T* col = B + 2 * sB * blockIdx.z + 2 * (long long)c * ldb;
-  const int* pv = ipiv + (long long)n * blockIdx.z;
+  T* col = B + 2 * (long long)c * ldb;
   for (int j = 0; j < n; ++j) {
     const int p = pv[j];
     if (p != j) {
@@ -383,6 +418,13 @@ __global__ __launch_bounds__(256) void k_apply_pivots(T* B, int ldb, long long s
       col[2 * p + 1] = xi;
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pivots(T* B, int ldb, long long sB, int n, int nrhs,
+                                                      const int* ipiv) {
+  apply_pivots_body<T>(B + 2 * sB * blockIdx.z, ldb, n, nrhs, ipiv + (long long)n * blockIdx.z,
+                       blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <typename T>
@@ -413,6 +455,223 @@ hipError_t lu_solve(hipStream_t s, const T* A, int lda, long long sA, int n, con
       if (e != hipSuccess) return e;
     }
   }
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// Grouped factorisation and solve: matrices of different orders in one launch
+// per step (blockIdx.z = member). The host walks the panels of the largest
+// member; a member whose order is exhausted exits, and every grid is sized for
+// the largest member (smaller members' surplus workgroups exit at once).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_pivot_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  if (k0 >= g.n) return;
+  __shared__ int prow;
+  __shared__ T pval;
+  block_pivot_search(g.A, (int)g.lda, g.n, k0, k0, &prow, &pval);
+  block_apply_pivot(g.A, (int)g.lda, k0, prow, pval, k0, imin(k0 + kLuNb, g.n), g.ipiv, g.info);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_col_g(const LuDesc<T>* __restrict__ d, int j, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  if (j >= g.n) return;
+  const int rows = g.n - j - 1, nblk = rows > 0 ? (rows + 255) / 256 : 1;
+  if ((int)blockIdx.x >= nblk) return;
+  lu_col_body<T>(g.A, (int)g.lda, g.n, j, k0, imin(k0 + kLuNb, g.n), g.ipiv, g.info, g.counter, nblk, blockIdx.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_laswp_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  if (k0 >= g.n) return;
+  laswp_body<T>(g.A, (int)g.lda, g.n, k0, imin(k0 + kLuNb, g.n), g.ipiv, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// block row of U: L11^{-1} A12
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_trsm_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  const int k1 = imin(k0 + kLuNb, g.n);
+  if (k1 >= g.n || (int)blockIdx.x * 4 >= g.n - k1) return;
+  trsm_body<T, false>(g.A + 2 * ((long long)k0 + (long long)k0 * g.lda), (int)g.lda, k1 - k0,
+                      g.A + 2 * ((long long)k0 + (long long)k1 * g.lda), (int)g.lda, g.n - k1, blockIdx.x);
+}
+
+// trailing update A22 -= A21 A12
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_update_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  const int k1 = imin(k0 + kLuNb, g.n), nr = g.n - k1;
+  const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
+  if (nr <= 0 || row0 >= nr || col0 >= nr) return;
+  const long long lda = g.lda;
+  cgemm_tile<T, false>(nr, nr, k1 - k0, T(-1), T(0), g.A + 2 * ((long long)k1 + (long long)k0 * lda), (int)lda,
+                       g.A + 2 * ((long long)k0 + (long long)k1 * lda), (int)lda, T(1), T(0),
+                       g.A + 2 * ((long long)k1 + (long long)k1 * lda), (int)lda, row0, col0);
+}
+
+template <typename T>
+hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax) {
+  if (count <= 0 || nmax <= 0) return hipSuccess;
+  for (int k0 = 0; k0 < nmax; k0 += kLuNb) {
+    const int k1 = k0 + kLuNb < nmax ? k0 + kLuNb : nmax;
+    hipLaunchKernelGGL((k_lu_pivot_g<T>), dim3(1, 1, count), dim3(256), 0, s, d, k0);
+    for (int j = k0; j < k1; ++j) {
+      const int rows = nmax - j - 1;
+      const int gx = rows > 0 ? (rows + 255) / 256 : 1;
+      hipLaunchKernelGGL((k_lu_col_g<T>), dim3(gx, 1, count), dim3(256), 0, s, d, j, k0);
+    }
+    hipLaunchKernelGGL((k_lu_laswp_g<T>), dim3((nmax + 255) / 256, 1, count), dim3(256), 0, s, d, k0);
+    if (k1 < nmax) {
+      const int nc = nmax - k1;
+      hipLaunchKernelGGL((k_lu_trsm_g<T>), dim3((nc + 3) / 4, 1, count), dim3(256), 0, s, d, k0);
+      hipLaunchKernelGGL((k_lu_update_g<T>), dim3((nc + kBM - 1) / kBM, (nc + kBN - 1) / kBN, count), dim3(256), 0,
+                         s, d, k0);
+    }
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_pivots_g(const LuDesc<T>* __restrict__ d) {
+  const LuDesc<T> g = d[blockIdx.z];
+  apply_pivots_body<T>(g.B, (int)g.ldb, g.n, g.nrhs, g.ipiv, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void k_solve_trsm_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  if (k0 >= g.n || (int)blockIdx.x * 4 >= g.nrhs) return;
+  const int kb = imin(kLuNb, g.n - k0);
+  trsm_body<T, UPPER>(g.A + 2 * ((long long)k0 + (long long)k0 * g.lda), (int)g.lda, kb, g.B + 2 * (long long)k0,
+                      (int)g.ldb, g.nrhs, blockIdx.x);
+}
+
+// forward: B[k0+kb:, :] -= A[k0+kb:, k0:k0+kb] B[k0:k0+kb, :]
+// backward: B[0:k0, :] -= A[0:k0, k0:k0+kb] B[k0:k0+kb, :]
+template <typename T, bool BACK>
+__global__ __launch_bounds__(256) void k_solve_update_g(const LuDesc<T>* __restrict__ d, int k0) {
+  const LuDesc<T> g = d[blockIdx.z];
+  if (k0 >= g.n) return;
+  const int kb = imin(kLuNb, g.n - k0);
+  const int r0 = BACK ? 0 : k0 + kb, nr = BACK ? k0 : g.n - k0 - kb;
+  const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
+  if (nr <= 0 || row0 >= nr || col0 >= g.nrhs) return;
+  cgemm_tile<T, false>(nr, g.nrhs, kb, T(-1), T(0), g.A + 2 * ((long long)r0 + (long long)k0 * g.lda), (int)g.lda,
+                       g.B + 2 * (long long)k0, (int)g.ldb, T(1), T(0), g.B + 2 * (long long)r0, (int)g.ldb, row0,
+                       col0);
+}
+
+template <typename T>
+hipError_t lu_solve_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax, int nrhs_max) {
+  if (count <= 0 || nmax <= 0 || nrhs_max <= 0) return hipSuccess;
+  const dim3 tg((nrhs_max + 3) / 4, 1, count);
+  hipLaunchKernelGGL((k_apply_pivots_g<T>), dim3((nrhs_max + 255) / 256, 1, count), dim3(256), 0, s, d);
+  for (int k0 = 0; k0 < nmax; k0 += kLuNb) {
+    hipLaunchKernelGGL((k_solve_trsm_g<T, false>), tg, dim3(256), 0, s, d, k0);
+    const int nr = nmax - k0 - kLuNb;
+    if (nr > 0)
+      hipLaunchKernelGGL((k_solve_update_g<T, false>), dim3((nr + kBM - 1) / kBM, (nrhs_max + kBN - 1) / kBN, count),
+                         dim3(256), 0, s, d, k0);
+  }
+  const int nblk = (nmax + kLuNb - 1) / kLuNb;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * kLuNb;
+    hipLaunchKernelGGL((k_solve_trsm_g<T, true>), tg, dim3(256), 0, s, d, k0);
+    if (k0 > 0)
+      hipLaunchKernelGGL((k_solve_update_g<T, true>), dim3((k0 + kBM - 1) / kBM, (nrhs_max + kBN - 1) / kBN, count),
+                         dim3(256), 0, s, d, k0);
+  }
+  return hipGetLastError();
+}
+
+// per member: dst (FP32) <- src (FP64), or dst (FP64) (+)= src (FP32)
+__global__ void k_convert_g(const ConvDesc* __restrict__ d, int to_single, int overwrite) {
+  const ConvDesc g = d[blockIdx.z];
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)g.m * g.n) return;
+  const int r = (int)(e % g.m), c = (int)(e / g.m);
+  const long long o64 = (long long)r + (long long)c * g.ld64, o32 = (long long)r + (long long)c * g.ld32;
+  if (to_single) {
+    g.f32[2 * o32] = (float)g.f64[2 * o64];
+    g.f32[2 * o32 + 1] = (float)g.f64[2 * o64 + 1];
+  } else if (overwrite) {
+    g.f64[2 * o64] = g.f32[2 * o32];
+    g.f64[2 * o64 + 1] = g.f32[2 * o32 + 1];
+  } else {
+    g.f64[2 * o64] += g.f32[2 * o32];
+    g.f64[2 * o64 + 1] += g.f32[2 * o32 + 1];
+  }
+}
+
+hipError_t convert_grouped(hipStream_t s, const ConvDesc* d, int count, long long max_elems, bool to_single,
+                           bool overwrite) {
+  if (count <= 0 || max_elems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_convert_g, dim3((unsigned)((max_elems + 255) / 256), 1, count), dim3(256), 0, s, d,
+                     to_single ? 1 : 0, overwrite ? 1 : 0);
+  return hipGetLastError();
+}
+
+// per member: dst <- src (FP64 m x n blocks)
+__global__ void k_copy_g(const CopyDesc* __restrict__ d) {
+  const CopyDesc g = d[blockIdx.z];
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)g.m * g.n) return;
+  const int r = (int)(e % g.m), c = (int)(e / g.m);
+  const long long os = (long long)r + (long long)c * g.lds, od = (long long)r + (long long)c * g.ldd;
+  g.dst[2 * od] = g.src[2 * os];
+  g.dst[2 * od + 1] = g.src[2 * os + 1];
+}
+hipError_t copy_grouped(hipStream_t s, const CopyDesc* d, int count, long long max_elems) {
+  if (count <= 0 || max_elems <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_g, dim3((unsigned)((max_elems + 255) / 256), 1, count), dim3(256), 0, s, d);
+  return hipGetLastError();
+}
+
+// per member: largest |re|,|im| of the FP32 correction over that of the FP64
+// solution (one workgroup per member)
+__global__ __launch_bounds__(256) void k_correction_ratio_g(const ConvDesc* __restrict__ d, double* out) {
+  const ConvDesc g = d[blockIdx.x];
+  __shared__ double s32[256], s64[256];
+  double m32 = 0.0, m64 = 0.0;
+  for (long long e = threadIdx.x; e < (long long)g.m * g.n; e += blockDim.x) {
+    const int r = (int)(e % g.m), c = (int)(e / g.m);
+    const long long o64 = (long long)r + (long long)c * g.ld64, o32 = (long long)r + (long long)c * g.ld32;
+    m32 = fmax(m32, fmax(fabs((double)g.f32[2 * o32]), fabs((double)g.f32[2 * o32 + 1])));
+    m64 = fmax(m64, fmax(fabs(g.f64[2 * o64]), fabs(g.f64[2 * o64 + 1])));
+  }
+  s32[threadIdx.x] = m32;
+  s64[threadIdx.x] = m64;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      s32[threadIdx.x] = fmax(s32[threadIdx.x], s32[threadIdx.x + st]);
+      s64[threadIdx.x] = fmax(s64[threadIdx.x], s64[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s32[0] / (s64[0] > 0.0 ? s64[0] : 1.0);
+}
+hipError_t correction_ratio_grouped(hipStream_t s, const ConvDesc* d, int count, double* out) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_correction_ratio_g, dim3(count), dim3(256), 0, s, d, out);
+  return hipGetLastError();
+}
+
+// fill n ints with a value (graph-capturable, unlike a memset node)
+__global__ void k_fill_i32(int* p, long long n, int v) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+hipError_t fill_i32(hipStream_t s, int* p, long long n, int v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 
@@ -455,6 +714,12 @@ template hipError_t cgemm<double>(hipStream_t, bool, int, int, int, double, doub
                                   const double*, int, long long, double, double, double*, int, long long, int);
 template hipError_t cgemm<float>(hipStream_t, bool, int, int, int, float, float, const float*, int, long long,
                                  const float*, int, long long, float, float, float*, int, long long, int);
+template hipError_t cgemm_grouped<double>(hipStream_t, bool, const GemmDesc<double>*, int, int, int);
+template hipError_t cgemm_grouped<float>(hipStream_t, bool, const GemmDesc<float>*, int, int, int);
+template hipError_t lu_factor_grouped<double>(hipStream_t, const LuDesc<double>*, int, int);
+template hipError_t lu_factor_grouped<float>(hipStream_t, const LuDesc<float>*, int, int);
+template hipError_t lu_solve_grouped<double>(hipStream_t, const LuDesc<double>*, int, int, int);
+template hipError_t lu_solve_grouped<float>(hipStream_t, const LuDesc<float>*, int, int, int);
 template hipError_t lu_factor<double>(hipStream_t, double*, int, long long, int, int*, int*, unsigned*, int);
 template hipError_t lu_factor<float>(hipStream_t, float*, int, long long, int, int*, int*, unsigned*, int);
 template hipError_t lu_solve<double>(hipStream_t, const double*, int, long long, int, const int*, double*, int,

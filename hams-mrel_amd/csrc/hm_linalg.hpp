@@ -22,6 +22,57 @@ template <typename T>
 hipError_t lu_solve(hipStream_t s, const T* A, int lda, long long sA, int n, const int* ipiv, T* B, int ldb,
                     long long sB, int nrhs, int batch);
 
+// ---- grouped operations: members of different sizes, one launch per step ----
+template <typename T>
+struct GemmDesc {            // C = alpha op(A) B + beta C for one member
+  int m, n, k;
+  T ar, ai, br, bi;
+  const T* A;
+  long long lda;
+  const T* B;
+  long long ldb;
+  T* C;
+  long long ldc;
+};
+template <typename T>
+struct LuDesc {              // one member of a grouped factorisation / solve
+  T* A;                      // n x n, leading dimension lda, factored in place
+  long long lda;
+  int n;
+  int* ipiv;                 // n (0-based)
+  int* info;                 // one word: first zero-pivot column + 1, or 0 (zeroed by the caller)
+  unsigned* counter;         // one word, zeroed by the caller
+  T* B;                      // right-hand sides of lu_solve_grouped (n x nrhs, ld ldb), overwritten by X
+  long long ldb;
+  int nrhs;
+};
+struct ConvDesc {            // FP64 <-> FP32 copy of an m x n block
+  double* f64;
+  long long ld64;
+  float* f32;
+  long long ld32;
+  int m, n;
+};
+struct CopyDesc {            // FP64 m x n block copy
+  const double* src;
+  long long lds;
+  double* dst;
+  long long ldd;
+  int m, n;
+};
+hipError_t copy_grouped(hipStream_t s, const CopyDesc* d, int count, long long max_elems);
+// out[z] = max |correction (f32)| / max |solution (f64)| of member z
+hipError_t correction_ratio_grouped(hipStream_t s, const ConvDesc* d, int count, double* out);
+template <typename T>
+hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax);
+template <typename T>
+hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax);
+template <typename T>
+hipError_t lu_solve_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax, int nrhs_max);
+hipError_t convert_grouped(hipStream_t s, const ConvDesc* d, int count, long long max_elems, bool to_single,
+                           bool overwrite);
+hipError_t fill_i32(hipStream_t s, int* p, long long n, int v);
+
 hipError_t demote(hipStream_t s, const double* src, int lds, long long ss, float* dst, int ldd, long long sd, int m,
                   int n, int batch);
 hipError_t promote_add(hipStream_t s, const float* src, int lds, long long ss, double* dst, int ldd, long long sd,

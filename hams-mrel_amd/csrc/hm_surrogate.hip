@@ -335,13 +335,14 @@ __global__ __launch_bounds__(256) void k_assemble_sur(GeomDev g, FreqDev f, doub
   const bool plain_far = rh >= 0.5;
   // pairs handed to the plain kernels: wave-aggregated append per regime
   const bool to_e = valid && !net && !plain_far && rh < 0.5, to_g = valid && !net && !plain_far && rh >= 0.5;
-  const unsigned long long be = __ballot(to_e), bg = __ballot(to_g);
+  const unsigned long long be = __ballot(to_e), bg = __ballot(to_g), bf = __ballot(valid && plain_far);
   const int lane = threadIdx.x & 63;
   const unsigned long long lt = (1ull << lane) - 1ull;
   unsigned long long base_e = 0, base_g = 0;
   if (lane == 0) {
     if (be) base_e = atomicAdd(&fb_count[0], (unsigned long long)__popcll(be));
     if (bg) base_g = atomicAdd(&fb_count[1], (unsigned long long)__popcll(bg));
+    if (bf) atomicAdd(&fb_count[2], (unsigned long long)__popcll(bf));   // plain series in this kernel
   }
   base_e = __shfl(base_e, 0);
   base_g = __shfl(base_g, 0);

@@ -94,7 +94,9 @@ hipError_t sur_eval_points(hipStream_t s, int n, const double* rh, const double*
 
 // assembly of the pairs of the EIGENE + EIGEN list segments with the network;
 // pairs whose cell bound exceeds tol (or outside the trained R/h range) are
-// appended to fallback lists (regime 2 -> fb[0..], regime 3 -> fb2) for the plain kernels
+// appended to fallback lists (regime 2 -> fb[0..], regime 3 -> fb2) for the plain kernels;
+// fb_count[0], [1]: lengths of those lists, [2]: pairs with R/h >= 0.5 evaluated
+// here by the plain EIGEN series
 hipError_t assemble_surrogate(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
                               long long np, SurDev m, float tol, double* A, double* S, long long ld,
                               uint32_t* fb_eigene, uint32_t* fb_eigen, unsigned long long* fb_count);

@@ -6,6 +6,8 @@
 !   ASSB_RBC_MULTI / ASSB_DBC_MULTI    (AssbMatxMulti.f90:124, :187)
 !   RADIATION/DIFFRACTION_SOLVER_MULTI (AssbMatxMulti.f90:257, :323)
 !   RFORCE_MULTI / EFORCE_MULTI        (PotentWavForceMulti.f90:138, :45)
+!   ZITER_SOLVE_MULTI, SETUP_BLOCK_PRECONDITIONER (IterativeSolvers.f90:574, :190)
+!   HMATRIX_BUILD / HMATVEC / HMATRIX_DESTROY      (HMatrix.f90:135, :760, :852)
 ! The reference passes geometry and wave state through module variables
 ! (WavDynMods.f90); here the driver hands them over once with HM_SETUP and per
 ! frequency with HM_SET_FREQUENCY. NSYS (geometric symmetry) must be 1.
@@ -14,7 +16,7 @@ module hm_bem
    implicit none
    private
 
-   integer, parameter, public :: HM_FP64 = 0, HM_FP32_IR = 1
+   integer, parameter, public :: HM_FP64 = 0, HM_FP32_IR = 1, HM_GMRES = 2
 
    interface
       integer(c_int) function hm_create(ctx) bind(C, name="hm_create")
@@ -120,6 +122,87 @@ module hm_bem
          complex(c_double_complex), intent(inout) :: a(*)
          integer(c_int), intent(out) :: ipiv(*), info
       end function
+      integer(c_int) function hm_set_gmres(ctx, restart, tol, maxiter, precond) bind(C, name="hm_set_gmres")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: restart, maxiter, precond
+         real(c_double), value :: tol
+      end function
+      integer(c_int) function hm_set_hmatrix(ctx, enable, eps, eps_s, kmax) bind(C, name="hm_set_hmatrix")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: enable, kmax
+         real(c_double), value :: eps, eps_s
+      end function
+      integer(c_int) function hm_set_frequency_limit(ctx, which) bind(C, name="hm_set_frequency_limit")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: which
+      end function
+      integer(c_int) function hm_hmatrix_build(ctx, n, a, lda, nblocks, block_sizes) bind(C, name="hm_hmatrix_build")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n, lda, nblocks
+         complex(c_double_complex), intent(in) :: a(lda, *)
+         integer(c_int), intent(in) :: block_sizes(*)
+      end function
+      integer(c_int) function hm_hmatrix_release(ctx) bind(C, name="hm_hmatrix_release")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function hm_hmatrix_apply(ctx, mat, ncol, x, y) bind(C, name="hm_hmatrix_apply")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: mat, ncol
+         complex(c_double_complex), intent(in) :: x(*)
+         complex(c_double_complex), intent(out) :: y(*)
+      end function
+      integer(c_int) function hm_ziter_solve(ctx, isolv, n, a, lda, b, ldb, x, ldx, nrhs, nblocks, block_sizes, info) &
+            bind(C, name="hm_ziter_solve")
+         import :: c_ptr, c_int, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: isolv, n, lda, ldb, ldx, nrhs, nblocks
+         complex(c_double_complex), intent(in) :: a(lda, *), b(ldb, *)
+         complex(c_double_complex), intent(out) :: x(ldx, *)
+         integer(c_int), intent(in) :: block_sizes(*)
+         integer(c_int), intent(out) :: info
+      end function
+      integer(c_int) function hm_batch_create(batch) bind(C, name="hm_batch_create")
+         import :: c_ptr, c_int
+         type(c_ptr) :: batch
+      end function
+      integer(c_int) function hm_batch_destroy(batch) bind(C, name="hm_batch_destroy")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: batch
+      end function
+      integer(c_int) function hm_batch_set_constants(batch, rho, g, g_disp, amp, xw, yw) &
+            bind(C, name="hm_batch_set_constants")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: batch
+         real(c_double), value :: rho, g, g_disp, amp, xw, yw
+      end function
+      integer(c_int) function hm_batch_add_problem(batch, nbody, body_start, ntnd, xyz, ncn, ncon, xyz_p, ds, pnsz, &
+                                                   dxyz_p, h, nbeta, beta) bind(C, name="hm_batch_add_problem")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: batch
+         integer(c_int), value :: nbody, ntnd, nbeta
+         integer(c_int), intent(in) :: body_start(*), ncn(*), ncon(*)
+         real(c_double), intent(in) :: xyz(*), xyz_p(*), ds(*), pnsz(*), dxyz_p(*), beta(*)
+         real(c_double), value :: h
+      end function
+      integer(c_int) function hm_batch_prepare(batch, precision, autotune, use_graph, nrefine) &
+            bind(C, name="hm_batch_prepare")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: batch
+         integer(c_int), value :: precision, autotune, use_graph, nrefine
+      end function
+      integer(c_int) function hm_batch_solve(batch, w1, amas, bdmp, exfc) bind(C, name="hm_batch_solve")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: batch
+         real(c_double), intent(in) :: w1(*)
+         real(c_double), intent(out) :: amas(*), bdmp(*)
+         complex(c_double_complex), intent(out) :: exfc(*)
+      end function
    end interface
 
    public :: hm_create, hm_destroy, hm_set_constants, hm_set_geometry, hm_set_depth, hm_set_headings, &
@@ -129,8 +212,15 @@ module hm_bem
    public :: FINGREEN3D, INFGREEN3D
    public :: ASSB_LEFT_MULTI, ASSB_RBC_MULTI, ASSB_DBC_MULTI, RADIATION_SOLVER_MULTI, DIFFRACTION_SOLVER_MULTI
    public :: RFORCE_EFORCE_MULTI
+   public :: hm_set_gmres, hm_set_hmatrix, hm_set_frequency_limit, hm_hmatrix_build, hm_hmatrix_release, &
+             hm_hmatrix_apply, hm_ziter_solve, hm_batch_create, hm_batch_destroy, hm_batch_set_constants, &
+             hm_batch_add_problem, hm_batch_prepare, hm_batch_solve
+   public :: ZITER_SOLVE_MULTI, SETUP_BLOCK_PRECONDITIONER, HMATRIX_BUILD, HMATVEC, HMATRIX_DESTROY
+   public :: HM_SET_FREQUENCY_LIMIT_F
 
    type(c_ptr), save :: ctx_ = c_null_ptr
+   ! SETUP_BLOCK_PRECONDITIONER's blocks, used by the next ZITER_SOLVE_MULTI
+   integer(c_int), allocatable, save :: pblocks_(:)
    integer, save :: n_ = 0, nbody_ = 0, nbeta_ = 0
    integer, allocatable, save :: body_start_(:)
    complex(c_double_complex), allocatable, target, save :: rhs_(:, :)
@@ -183,6 +273,73 @@ contains
    subroutine HM_SET_FREQUENCY_F(W1)
       real(8), intent(in) :: W1
       call HM_CHECK(hm_set_frequency(HM_CONTEXT(), W1), 'hm_set_frequency')
+   end subroutine
+
+   ! CalWaveProperts(KK) with SYBO = 1 (ImplementSubs.f90:45-70): KK = 1 zero
+   ! frequency, KK = 2 infinite frequency
+   subroutine HM_SET_FREQUENCY_LIMIT_F(KK)
+      integer, intent(in) :: KK
+      call HM_CHECK(hm_set_frequency_limit(HM_CONTEXT(), int(KK - 1, c_int)), 'hm_set_frequency_limit')
+   end subroutine
+
+   ! ---- iterative solver and H-matrix with the reference argument lists -------
+   ! SETUP_BLOCK_PRECONDITIONER (IterativeSolvers.f90:190): the blocks are kept
+   ! and factored on the GPU by the next ZITER_SOLVE_MULTI
+   subroutine SETUP_BLOCK_PRECONDITIONER(N, AMAT, NBLOCKS, BLOCK_SIZES)
+      integer, intent(in) :: N, NBLOCKS
+      complex(8), intent(in) :: AMAT(N, N)
+      integer, intent(in) :: BLOCK_SIZES(NBLOCKS)
+      if (allocated(pblocks_)) deallocate (pblocks_)
+      allocate (pblocks_(NBLOCKS))
+      pblocks_ = BLOCK_SIZES
+      if (sum(BLOCK_SIZES) /= N .or. size(AMAT, 1) /= N) call HM_CHECK(-112_c_int, 'SETUP_BLOCK_PRECONDITIONER')
+   end subroutine
+
+   ! HMATRIX_BUILD (HMatrix.f90:135): clusters are the preconditioner blocks
+   ! (the bodies; equal sizes), the coordinates are not needed
+   subroutine HMATRIX_BUILD(N, AMAT, XYZ_P)
+      integer, intent(in) :: N
+      complex(8), intent(in) :: AMAT(N, N)
+      real(8), intent(in) :: XYZ_P(N, 3)
+      if (.not. allocated(pblocks_)) call HM_CHECK(-115_c_int, 'HMATRIX_BUILD needs SETUP_BLOCK_PRECONDITIONER first')
+      call HM_CHECK(hm_hmatrix_build(HM_CONTEXT(), N, AMAT, N, size(pblocks_), pblocks_), 'HMATRIX_BUILD')
+      if (size(XYZ_P, 1) /= N) call HM_CHECK(-103_c_int, 'HMATRIX_BUILD: XYZ_P')
+   end subroutine
+
+   subroutine HMATVEC(N, X_IN, Y_OUT)
+      integer, intent(in) :: N
+      complex(8), intent(in) :: X_IN(N)
+      complex(8), intent(out) :: Y_OUT(N)
+      call HM_CHECK(hm_hmatrix_apply(HM_CONTEXT(), 0_c_int, 1_c_int, X_IN, Y_OUT), 'HMATVEC')
+   end subroutine
+
+   subroutine HMATRIX_DESTROY()
+      call HM_CHECK(hm_hmatrix_release(HM_CONTEXT()), 'HMATRIX_DESTROY')
+   end subroutine
+
+   ! ZITER_SOLVE_MULTI (IterativeSolvers.f90:574): GMRES on the GPU, on the
+   ! H-matrix when HMATRIX_BUILD has run for this matrix, else on AMAT with the
+   ! blocks of SETUP_BLOCK_PRECONDITIONER; INFO = 0 converged, 1 not
+   subroutine ZITER_SOLVE_MULTI(ISOLV_IN, N, AMAT, B, X, NRHS, INFO)
+      integer, intent(in) :: ISOLV_IN, N, NRHS
+      complex(8), intent(in) :: AMAT(N, N), B(N, NRHS)
+      complex(8), intent(out) :: X(N, NRHS)
+      integer, intent(out) :: INFO
+      integer(c_int) :: inf, isolv, nblk
+      integer(c_int) :: none(1)
+      isolv = 1
+      if (ISOLV_IN == 2) isolv = 2
+      none(1) = 0
+      nblk = 0
+      if (allocated(pblocks_)) nblk = size(pblocks_)
+      if (nblk > 0) then
+         call HM_CHECK(hm_ziter_solve(HM_CONTEXT(), isolv, N, AMAT, N, B, N, X, N, NRHS, nblk, pblocks_, inf), &
+                       'ZITER_SOLVE_MULTI')
+      else
+         call HM_CHECK(hm_ziter_solve(HM_CONTEXT(), isolv, N, AMAT, N, B, N, X, N, NRHS, 0_c_int, none, inf), &
+                       'ZITER_SOLVE_MULTI')
+      end if
+      INFO = inf
    end subroutine
 
    ! ---- element-wise kernels with the reference argument lists ----------------

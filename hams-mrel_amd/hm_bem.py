@@ -33,7 +33,9 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
            "hm_surrogate_grad", "hm_set_hmatrix", "hm_get_hmatrix_stats",
-           "hm_hmatrix_apply", "hm_set_frequency_limit")
+           "hm_hmatrix_apply", "hm_set_frequency_limit", "hm_batch_create", "hm_batch_destroy",
+           "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
+           "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve")
 
 _lib = None
 
@@ -47,8 +49,10 @@ def load():
         L = ctypes.CDLL(LIB_PATH)
         L.hm_error_string.restype = ctypes.c_char_p
         L.hm_error_string.argtypes = [_vp]
+        L.hm_batch_error_string.restype = ctypes.c_char_p
+        L.hm_batch_error_string.argtypes = [_vp]
         for name in EXPORTS:
-            if name != "hm_error_string":
+            if name not in ("hm_error_string", "hm_batch_error_string"):
                 getattr(L, name).restype = _i
         _lib = L
     return _lib
@@ -270,6 +274,31 @@ class Context:
         self._chk(self.L.hm_hmatrix_apply(self.ptr, _i(mat), _i(x.shape[1]), _p(xf), _p(y)), "hm_hmatrix_apply")
         return y
 
+    def hmatrix_build(self, a, block_sizes):
+        """H-matrix of an assembled matrix a (n x n) with the given equal cluster blocks"""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        self._chk(self.L.hm_hmatrix_build(self.ptr, _i(a.shape[0]), _p(a), _i(a.shape[0]), _i(len(bs)), _p(bs)),
+                  "hm_hmatrix_build")
+
+    def hmatrix_release(self):
+        self._chk(self.L.hm_hmatrix_release(self.ptr), "hm_hmatrix_release")
+
+    def ziter_solve(self, isolv, a, b, block_sizes=()):
+        """GMRES solve of a x = b (isolv 1: dense a; 2: the built H-matrix); returns (x, info)"""
+        b = np.asfortranarray(b, dtype=np.complex128)
+        if b.ndim == 1:
+            b = b[:, None]
+        n, nrhs = b.shape
+        x = np.zeros_like(b, order="F")
+        a = np.asfortranarray(a, dtype=np.complex128) if a is not None else None
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        info = _i(0)
+        self._chk(self.L.hm_ziter_solve(self.ptr, _i(isolv), _i(n), _p(a), _i(n), _p(b), _i(n), _p(x), _i(n),
+                                        _i(nrhs), _i(len(bs)), _p(bs) if len(bs) else None, ctypes.byref(info)),
+                  "hm_ziter_solve")
+        return x, info.value
+
     def set_series_variant(self, variant):
         self._chk(self.L.hm_set_series_variant(self.ptr, _i(variant)), "hm_set_series_variant")
 
@@ -342,3 +371,88 @@ class Context:
         self._chk(self.L.hm_zgetrs(self.ptr, ctypes.c_char(b"N"), _i(n), _i(b.shape[1]), _p(lu), _i(n), _p(ipiv),
                                    _p(b), _i(n), ctypes.byref(info)), "hm_zgetrs")
         return b
+
+
+class Batch:
+    """Many independent problems resident at once, solved together per frequency
+    (hm_batch_* in include/hm_bem.h)."""
+
+    def __init__(self, problems, rho=None, g=None, g_disp=None, amp=None, xw=None):
+        self.L = load()
+        self.ptr = _vp()
+        rc = self.L.hm_batch_create(ctypes.byref(self.ptr))
+        if rc:
+            raise HMError(rc, "hm_batch_create (no GPU?)")
+        self.problems = list(problems)
+        p0 = self.problems[0]
+        self._chk(self.L.hm_batch_set_constants(
+            self.ptr, _d(p0.rho if rho is None else rho), _d(p0.g if g is None else g),
+            _d(p0.g_disp if g_disp is None else g_disp), _d(p0.amp if amp is None else amp),
+            _d((p0.xw if xw is None else xw)[0]), _d((p0.xw if xw is None else xw)[1])), "hm_batch_set_constants")
+        for prob in self.problems:
+            if prob.ni:
+                raise ValueError("batch members take no interior waterplane points")
+            keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
+                    np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                    np.ascontiguousarray(prob.ncn, dtype=np.int32),
+                    np.ascontiguousarray(prob.ncon, dtype=np.int32),
+                    np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
+                    np.ascontiguousarray(prob.ds, dtype=np.float64),
+                    np.ascontiguousarray(prob.pnsz, dtype=np.float64),
+                    np.ascontiguousarray(prob.dxyz_p, dtype=np.float64),
+                    np.ascontiguousarray(prob.beta if prob.nbeta else np.zeros(1), dtype=np.float64)]
+            rc = self.L.hm_batch_add_problem(self.ptr, _i(prob.nbody), _p(keep[0]), _i(len(prob.xyz)), _p(keep[1]),
+                                             _p(keep[2]), _p(keep[3]), _p(keep[4]), _p(keep[5]), _p(keep[6]),
+                                             _p(keep[7]), _d(prob.h), _i(prob.nbeta), _p(keep[8]))
+            if rc < 0:
+                self._chk(rc, "hm_batch_add_problem")
+
+    def close(self):
+        if self.ptr:
+            self.L.hm_batch_destroy(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise HMError(rc, f"{what}: {self.L.hm_batch_error_string(self.ptr).decode()}")
+
+    def prepare(self, precision=HM_FP64, autotune=True, use_graph=True, nrefine=3):
+        self._chk(self.L.hm_batch_prepare(self.ptr, _i(precision), _i(1 if autotune else 0),
+                                          _i(1 if use_graph else 0), _i(nrefine)), "hm_batch_prepare")
+
+    def solve(self, w1):
+        """w1: one frequency for all members, or one per member. Returns a list of
+        dict(amas, bdmp, exfc) per member."""
+        w = np.broadcast_to(np.asarray(w1, dtype=np.float64), (len(self.problems),)).copy()
+        na = sum(p.m * p.m for p in self.problems)
+        ne = sum(p.m * p.nbeta for p in self.problems)
+        amas = np.zeros(max(na, 1))
+        bdmp = np.zeros(max(na, 1))
+        exfc = np.zeros(max(ne, 1), dtype=np.complex128)
+        self._chk(self.L.hm_batch_solve(self.ptr, _p(w), _p(amas), _p(bdmp), _p(exfc)), "hm_batch_solve")
+        out, oa, oe = [], 0, 0
+        for p in self.problems:
+            m = p.m
+            out.append(dict(amas=amas[oa:oa + m * m].reshape((m, m), order="F"),
+                            bdmp=bdmp[oa:oa + m * m].reshape((m, m), order="F"),
+                            exfc=exfc[oe:oe + m * p.nbeta].reshape((m, p.nbeta), order="F")))
+            oa += m * m
+            oe += m * p.nbeta
+        return out
+
+    def info(self):
+        mem, nmax, var, grid, es = _i(), _i(), _i(), _i(), _i()
+        tms, corr = _d(), _d()
+        nbytes = ctypes.c_longlong()
+        self._chk(self.L.hm_batch_info(self.ptr, ctypes.byref(mem), ctypes.byref(nmax), ctypes.byref(var),
+                                       ctypes.byref(grid), ctypes.byref(tms), ctypes.byref(corr), ctypes.byref(es),
+                                       ctypes.byref(nbytes)), "hm_batch_info")
+        return dict(members=mem.value, nmax=nmax.value, series_variant=var.value, grid_blocks=grid.value,
+                    tuned_assembly_ms=tms.value, last_correction=corr.value, extra_sweeps=es.value,
+                    device_bytes=nbytes.value)

@@ -111,10 +111,32 @@ int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, dou
 int hm_set_hmatrix(hm_ctx *ctx, int enable, double eps, double eps_s, int kmax);
 int hm_get_hmatrix_stats(hm_ctx *ctx, int *unconverged, int *steps, double *mean_rank, int *max_rank,
                          double *stored_fraction);
-/* y = H x in FP64 for the assembled H-matrix form of A (mat 0) or S (mat 1);
- * x, y: n x ncol column-major complex (HMATVEC, HMatrix.f90:760, for several
- * columns). */
+/* y = H x in FP64 for the assembled H-matrix form of A (mat 0) or S (mat 1),
+ * or for the H-matrix of hm_hmatrix_build (mat 0); x, y: n x ncol column-major
+ * complex (HMATVEC, HMatrix.f90:760, for several columns). */
 int hm_hmatrix_apply(hm_ctx *ctx, int mat, int ncol, const double *x, double *y);
+
+/* Iterative solves on an assembled host matrix, with the reference's argument
+ * lists around them (the Fortran wrappers ZITER_SOLVE_MULTI,
+ * SETUP_BLOCK_PRECONDITIONER, HMATRIX_BUILD, HMATVEC, HMATRIX_DESTROY).
+ * hm_hmatrix_build replaces HMATRIX_BUILD(N, AMAT, XYZ_P) (HMatrix.f90:135):
+ * clusters are the nblocks equal blocks given (the bodies), the diagonal blocks
+ * stay dense and every off-diagonal block is compressed by the cross
+ * approximation of hm_set_hmatrix (eps, kmax) on the matrix's entries; the
+ * block-diagonal FP32 preconditioner is factored with it. hm_hmatrix_release
+ * replaces HMATRIX_DESTROY (:852). hm_ziter_solve replaces
+ * ZITER_SOLVE_MULTI(ISOLV, N, AMAT, B, X, NRHS, INFO) (IterativeSolvers.f90:574):
+ * isolv = 1 restarted GMRES (hm_set_gmres parameters) on the dense matrix a with
+ * the block-diagonal preconditioner of SETUP_BLOCK_PRECONDITIONER(N, AMAT,
+ * NBLOCKS, BLOCK_SIZES) (:190; nblocks = 0: none); isolv = 2 GMRES on the
+ * H-matrix of the last hm_hmatrix_build (a and the blocks are then not read),
+ * as ZGMRES_SOLVE uses HMATVEC once HMATRIX_BUILD has run (:325-343). info = 0
+ * converged, 1 not converged (the reference's INFO). Both reuse the context's
+ * matrix buffers: an assembled system is discarded. */
+int hm_hmatrix_build(hm_ctx *ctx, int n, const double *a, int lda, int nblocks, const int *block_sizes);
+int hm_hmatrix_release(hm_ctx *ctx);
+int hm_ziter_solve(hm_ctx *ctx, int isolv, int n, const double *a, int lda, const double *b, int ldb, double *x,
+                   int ldx, int nrhs, int nblocks, const int *block_sizes, int *info);
 /* set_frequency + assemble + factor + solve + forces */
 int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, double *bdmp, double *exfc);
 /* nfreq frequencies back to back; outputs stacked per frequency */
@@ -181,6 +203,39 @@ int hm_surrogate_counts(hm_ctx *ctx, long long *counts);
 int hm_surrogate_eval(hm_ctx *ctx, int n, const double *rh, const double *zs, const double *zd, const double *kh,
                       double *t_net, double *t_plain, double *bound);
 int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, int quant, float *grad, double *loss);
+
+/* ---- batches of independent problems ------------------------------------------
+ * The reference solves one problem at a time, calling the per-frequency chain
+ * (assembly, ZGETRF/ZGETRS, forces; HAMS_Prog.f90:615-724) serially. A batch
+ * keeps many problems (members; any panel counts, finite or infinite depth, no
+ * interior waterplane points) resident in HBM and runs that chain for all of
+ * them at once per call: one persistent assembly launch per regime over the
+ * pairs of every member, grouped launches (each member at its own order) for
+ * the right-hand sides, the LU and the triangular solves, optionally captured
+ * once in a HIP graph and replayed per frequency.
+ * hm_batch_add_problem takes one member (arguments as hm_set_geometry,
+ * hm_set_depth and hm_set_headings, ni = 0) and returns its index >= 0.
+ * hm_batch_prepare(precision HM_FP64 or HM_FP32_IR, autotune, use_graph,
+ * nrefine): allocates, classifies, times the series variants and persistent
+ * grid sizes of the assembly when autotune != 0 and keeps the fastest, and
+ * captures the chain when use_graph != 0. HM_FP32_IR factors in FP32 and runs
+ * nrefine FP64-residual refinement sweeps inside the chain, then more (at most
+ * 10) until every member's last correction is below 1e-13 of its solution.
+ * hm_batch_solve(w1[member] > 0): outputs stacked by member in member order
+ * (amas, bdmp: (6 nbody)^2 each; exfc: 6 nbody x nbeta complex). A positive
+ * return is a zero pivot of some member (hm_batch_error_string names it). */
+typedef struct hm_batch hm_batch;
+int hm_batch_create(hm_batch **batch);
+int hm_batch_destroy(hm_batch *batch);
+const char *hm_batch_error_string(const hm_batch *batch);
+int hm_batch_set_constants(hm_batch *batch, double rho, double g, double g_disp, double amp, double xw, double yw);
+int hm_batch_add_problem(hm_batch *batch, int nbody, const int *body_start, int ntnd, const double *xyz,
+                         const int *ncn, const int *ncon, const double *xyz_p, const double *ds, const double *pnsz,
+                         const double *dxyz_p, double h, int nbeta, const double *beta);
+int hm_batch_prepare(hm_batch *batch, int precision, int autotune, int use_graph, int nrefine);
+int hm_batch_solve(hm_batch *batch, const double *w1, double *amas, double *bdmp, double *exfc);
+int hm_batch_info(hm_batch *batch, int *members, int *nmax, int *series_variant, int *grid_blocks,
+                  double *tuned_assembly_ms, double *last_correction, int *extra_sweeps, long long *device_bytes);
 
 /* Stage timing: when enabled, a pair of HIP events is recorded on the context's
  * stream around each stage (no extra synchronisation); hm_get_timing waits for

@@ -1,0 +1,128 @@
+"""Batches of independent problems (hm_batch_*) against single-problem solves.
+
+The batch runs the same device code per pair, per LU panel and per force
+reduction as a single-problem context (one kernel body, launched over the pair
+lists of all members or with blockIdx.z = member), so with the same series
+variant its FP64 results are compared with the single-problem FP64 results at
+1e-13 (the two differ only where the compiler fuses a multiply-add differently
+in the two kernels). HIP graph replays are compared bit for bit with the same
+chain issued eagerly, three frequencies in a row with eager work in between.
+FP32 LU + FP64 refinement is held to 1e-10 of FP64 (it refines until the last
+correction is below 1e-13 of the solution).
+"""
+import numpy as np
+import pytest
+
+import hm_bem
+import meshes
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+KEYS = ("amas", "bdmp", "exfc")
+
+
+def _members():
+    beta = np.array([0.0, np.pi / 3])
+    two = [meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(0.0, 0.0)),
+           meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(12.0, 0.0))]
+    return [
+        meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2)], h=-1.0, omegas=[1.0], beta=beta),
+        meshes.Problem([meshes.box(6.0, 4.0, 3.0, 6, 4, 3)], h=40.0, omegas=[1.0], beta=beta[:1]),
+        meshes.Problem([meshes.cylinder(5.0, 5.0, 16, 4, 3)], h=80.0, omegas=[1.0], beta=beta),
+        meshes.Problem([meshes.cylinder(2.0, 3.0, 20, 5, 2)], h=-1.0, omegas=[1.0], beta=beta),
+        meshes.Problem(two, h=60.0, omegas=[1.0], beta=beta),
+    ]
+
+
+def _single(prob, w, variant):
+    c = hm_bem.Context()
+    try:
+        c.setup(prob)
+        c.set_series_variant(variant)
+        return c.solve_frequency(w, hm_bem.HM_FP64)
+    finally:
+        c.close()
+
+
+def test_batch_fp64_matches_single_problems():
+    probs = _members()
+    b = hm_bem.Batch(probs)
+    try:
+        b.prepare(hm_bem.HM_FP64, autotune=True, use_graph=True)
+        info = b.info()
+        print("batch info", info, "sizes", [p.n for p in probs], flush=True)
+        assert info["members"] == len(probs) and info["nmax"] == max(p.n for p in probs)
+        assert info["series_variant"] in (0, 1, 2) and info["grid_blocks"] > 0
+        ws = np.array([0.6, 0.9, 1.3, 1.7, 2.2])          # one frequency per member
+        for w in (1.1, ws):
+            got = b.solve(w)
+            for k, prob in enumerate(probs):
+                wk = float(np.broadcast_to(w, (len(probs),))[k])
+                ref = _single(prob, wk, info["series_variant"])
+                for key in KEYS:
+                    e = relerr(got[k][key], ref[key])
+                    assert e < 1e-13, (k, wk, key, e)
+    finally:
+        b.close()
+
+
+def test_batch_graph_replays_match_eager():
+    probs = _members()
+    g = hm_bem.Batch(probs)
+    e = hm_bem.Batch(probs)
+    try:
+        g.prepare(hm_bem.HM_FP64, autotune=False, use_graph=True)
+        e.prepare(hm_bem.HM_FP64, autotune=False, use_graph=False)
+        for w in (0.8, 1.5, 0.8):
+            ref = e.solve(w)
+            got = g.solve(w)
+            for k in range(len(probs)):
+                for key in KEYS:
+                    assert np.array_equal(got[k][key], ref[k][key]), (w, k, key)
+    finally:
+        g.close()
+        e.close()
+
+
+def test_batch_fp32_refinement():
+    probs = _members()
+    b64 = hm_bem.Batch(probs)
+    b32 = hm_bem.Batch(probs)
+    try:
+        b64.prepare(hm_bem.HM_FP64, autotune=False, use_graph=True)
+        b32.prepare(hm_bem.HM_FP32_IR, autotune=False, use_graph=True, nrefine=3)
+        for w in (0.7, 1.9):
+            ref = b64.solve(w)
+            got = b32.solve(w)
+            info = b32.info()
+            assert info["last_correction"] < 1e-13, info
+            for k in range(len(probs)):
+                for key in KEYS:
+                    err = relerr(got[k][key], ref[k][key])
+                    assert err < 1e-10, (w, k, key, err, info)
+    finally:
+        b64.close()
+        b32.close()
+
+
+def test_batch_argument_errors():
+    probs = _members()[:2]
+    b = hm_bem.Batch(probs)
+    try:
+        with pytest.raises(hm_bem.HMError, match="before hm_batch_prepare"):
+            b.solve(1.0)
+        with pytest.raises(hm_bem.HMError, match="precision"):
+            b.prepare(hm_bem.HM_GMRES)
+        b.prepare(hm_bem.HM_FP64, autotune=False, use_graph=False)
+        with pytest.raises(hm_bem.HMError, match="must be > 0"):
+            b.solve([1.0, 0.0])
+        with pytest.raises(hm_bem.HMError, match="after hm_batch_prepare"):
+            rc = b.L.hm_batch_set_constants(b.ptr, hm_bem._d(1000.0), hm_bem._d(9.81), hm_bem._d(9.81),
+                                            hm_bem._d(1.0), hm_bem._d(0.0), hm_bem._d(0.0))
+            b._chk(rc, "hm_batch_set_constants")
+    finally:
+        b.close()
+    irr = meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, waterplane_nr=2)], h=-1.0, omegas=[1.0],
+                         beta=np.array([0.0]))
+    with pytest.raises(ValueError, match="waterplane"):
+        hm_bem.Batch([irr])

@@ -117,3 +117,58 @@ def test_gmres_argument_errors():
             c.set_gmres(**kw)
         assert e.value.code == code, kw
     c.close()
+
+
+# ---- ZITER_SOLVE_MULTI / HMATRIX_BUILD on an assembled host matrix ----------
+def _assembled(c, w):
+    c.set_frequency(w)
+    c.assemble()
+    a, _, rhs = c.get_system()
+    return a, rhs
+
+
+def test_ziter_dense_with_blocks(quad):
+    p, c = quad
+    c.set_gmres(restart=50, tol=1e-8, maxiter=500, precond=-1)
+    a, rhs = _assembled(c, 0.9)
+    m = p.n // p.nbody
+    x, info = c.ziter_solve(1, a, rhs, block_sizes=[m] * p.nbody)
+    assert info == 0
+    r = np.linalg.norm(rhs - a @ x, axis=0) / np.linalg.norm(rhs, axis=0)
+    assert np.all(r <= 1e-8), r
+    x0, info0 = c.ziter_solve(1, a, rhs)                 # no preconditioner
+    assert info0 == 0
+    assert relerr(x0, x) < 1e-6
+
+
+def test_ziter_hmatrix(quad):
+    p, c = quad
+    m = p.n // p.nbody
+    c.set_gmres(restart=50, tol=1e-9, maxiter=500, precond=-1)
+    c.set_hmatrix(False, eps=1e-9, eps_s=1e-9, kmax=m)   # ACA parameters of the build below
+    a, rhs = _assembled(c, 1.3)
+    c.hmatrix_build(a, [m] * p.nbody)
+    st = c.hmatrix_stats()
+    assert st["unconverged"] == 0 and st["mean_rank"] < m, st
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(p.n) + 1j * rng.standard_normal(p.n)
+    hv = c.hmatrix_apply(0, v)[:, 0]                      # HMATVEC
+    assert np.linalg.norm(hv - a @ v) <= 1e-7 * np.linalg.norm(a @ v)
+    x, info = c.ziter_solve(2, None, rhs)
+    assert info == 0
+    xd = np.linalg.solve(a, rhs)
+    assert relerr(x, xd) < 1e-6
+    c.hmatrix_release()
+    with pytest.raises(hm_bem.HMError, match="before hm_hmatrix_build"):
+        c.ziter_solve(2, None, rhs)
+
+
+def test_ziter_argument_errors(quad):
+    p, c = quad
+    a, rhs = _assembled(c, 0.9)
+    with pytest.raises(hm_bem.HMError, match="isolv"):
+        c.ziter_solve(3, a, rhs)
+    with pytest.raises(hm_bem.HMError, match="sum to"):
+        c.ziter_solve(1, a, rhs, block_sizes=[3, 4])
+    with pytest.raises(hm_bem.HMError, match="equal"):
+        c.hmatrix_build(a, [p.n - 10, 10])
