@@ -181,6 +181,87 @@ def cpu_baseline(prob, seconds=20.0, threads=16):
                        f"(N/{m})^3 = {t_lu:.2f} s; {t_sys:.1f} s per system")}
 
 
+def bench_batch(args):
+    """The batching workload: 64 independent random problems (boxes and
+    cylinders, 512-6144 panels, infinite or finite depth), 4 headings, solved
+    together per frequency by the batch path (assembly launches across members,
+    grouped LU, one HIP graph replay per frequency) against the same problems
+    solved one after another by single-problem contexts (plain FP64)."""
+    import hm_bem
+    import meshes
+    probs = meshes.c3_random_problems()
+    omegas = probs[0].omegas
+    nsys = len(probs)
+    print(f"# workload C3: {nsys} problems, n {min(p.n for p in probs)}..{max(p.n for p in probs)}, "
+          f"{sum(p.h < 0 for p in probs)} infinite depth, sum n^2 = {sum(p.n * p.n for p in probs):.3e}",
+          file=sys.stderr, flush=True)
+    prec = hm_bem.HM_FP32_IR if args.path == "fp32ir" else hm_bem.HM_FP64
+    b = hm_bem.Batch(probs)
+    t0 = time.perf_counter()
+    b.prepare(prec, autotune=True, use_graph=True, nrefine=3)
+    sync()
+    info = b.info()
+    print(f"# batch prepared in {time.perf_counter() - t0:.1f} s: {info}", file=sys.stderr, flush=True)
+    for s in range(args.warmup):
+        b.solve(float(omegas[s % len(omegas)]))
+    sync()
+    t0 = time.perf_counter()
+    res = []
+    for s in range(args.steps):
+        res.append(b.solve(float(omegas[(args.warmup + s) % len(omegas)])))
+    sync()
+    dt = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    value = nsys * args.steps / dt
+    print(f"# batch: {args.steps} frequencies x {nsys} problems in {dt:.3f} s", file=sys.stderr, flush=True)
+    plain = None
+    if not args.no_plain:
+        ctxs = []
+        for p in probs:
+            c = hm_bem.Context()
+            c.setup(p)
+            ctxs.append(c)
+        sync()
+        for c in ctxs:
+            c.solve_frequency(float(omegas[0]), hm_bem.HM_FP64)
+        sync()
+        t0 = time.perf_counter()
+        ref = []
+        for s in range(args.steps):
+            w = float(omegas[(args.warmup + s) % len(omegas)])
+            ref.append([c.solve_frequency(w, hm_bem.HM_FP64) for c in ctxs])
+        sync()
+        dtp = time.perf_counter() - t0
+        for c in ctxs:
+            c.close()
+        err = 0.0
+        for rs, ps in zip(res, ref):
+            for r, p in zip(rs, ps):
+                for k in ("amas", "bdmp", "exfc"):
+                    mx = np.max(np.abs(p[k]))
+                    if mx > 0:
+                        err = max(err, float(np.max(np.abs(r[k] - p[k])) / mx))
+        plain = {"value": nsys * args.steps / dtp, "ms_per_step": 1e3 * dtp / args.steps,
+                 "speedup_batch_vs_sequential": dtp / dt, "max_rel_err_batch_vs_sequential": err}
+        print(f"# sequential single-problem FP64: {dtp:.3f} s; {plain}", file=sys.stderr, flush=True)
+    line = {"metric": "panel-freq systems solved/s (batch of independent problems)", "value": value,
+            "unit": "systems/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if prec == hm_bem.HM_FP64 else "f64+f32",
+            "data": "synthetic (seeded random boxes and cylinders of the named workload)",
+            "config": {"workload": "C3 64 random problems", "problems": nsys,
+                       "panels_min": min(p.n for p in probs), "panels_max": max(p.n for p in probs),
+                       "headings": probs[0].nbeta, "frequencies": len(omegas),
+                       "path": ("batch: one assembly launch per regime over all members' pairs, grouped LU and "
+                                "solves, one HIP graph replay per frequency, autotuned series variant; "
+                                + ("FP64 LU" if prec == hm_bem.HM_FP64 else "FP32 LU + FP64 refinement")),
+                       "parallelism": "1 GPU"},
+            "batch": info, "sequential_fp64": plain}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +295,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only: frequencies shard across GPUs "
                                    "without communication, but only --gpus 1 is supported here"}))
         return 2
+    if args.workload == "c3":
+        return bench_batch(args)
     import hm_bem
     prob = workload(args.workload)
     print(f"# workload {prob.name}: n={prob.n} ni={prob.ni} modes={prob.m} headings={prob.nbeta} "

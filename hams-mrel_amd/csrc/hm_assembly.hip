@@ -206,14 +206,24 @@ __global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, uns
     if (sh[k]) atomicAdd(&hist[k], sh[k]);
 }
 
+// one global atomic per (workgroup, key): ranks within the workgroup come from LDS
 __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, unsigned long long* cursor,
                                                           uint32_t* pairs, int blk) {
+  __shared__ unsigned cnt[kNumKeys];
+  __shared__ unsigned long long base[kNumKeys];
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cnt[k] = 0;
+  __syncthreads();
   const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i >= (blk > 0 ? b0 + blk : g.nt) || j >= (blk > 0 ? b0 + blk : g.n)) return;
-  const unsigned long long pos = atomicAdd(&cursor[pair_key(g, h, i, j)], 1ull);
-  pairs[pos] = ((uint32_t)i << 16) | (uint32_t)j;
+  const bool in = i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n);
+  const int key = in ? pair_key(g, h, i, j) : 0;
+  const unsigned rank = in ? atomicAdd(&cnt[key], 1u) : 0u;
+  __syncthreads();
+  for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
+    if (cnt[k]) base[k] = atomicAdd(&cursor[k], (unsigned long long)cnt[k]);
+  __syncthreads();
+  if (in) pairs[base[key] + rank] = ((uint32_t)i << 16) | (uint32_t)j;
 }
 
 static dim3 pair_grid(const GeomDev& g, int blk) {
@@ -401,9 +411,17 @@ hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const doubl
 }
 
 // ---- batches of independent problems ------------------------------------------
-// Pair codes: member << 40 | i << 20 | j. Each lane reads its member's geometry,
-// matrices and wave state from the device descriptors; the kernels are
-// persistent (grid-stride over the pair list, grid sized by the caller).
+// Pair codes: member << 40 | i << 20 | j. The list is laid out so that every
+// 64-lane wave holds pairs of one member only: each (key, member) run is padded
+// to a multiple of 64 with kNoPair entries, and the deep members' pair ranges
+// are padded likewise. The member index is then wave-uniform
+// (readfirstlane), so its descriptors load into scalar registers as the
+// single-problem kernels' arguments do; a wave stages its member's dispersion
+// roots in LDS. One workgroup per 256 entries: a grid-stride (persistent) loop
+// over the list was measured to double the register allocation of these
+// kernels (occupancy 2 -> 1 wave per SIMD) and is not used.
+constexpr uint64_t kNoPair = ~0ull;
+
 __device__ __forceinline__ void decode_pair(uint64_t c, int& p, int& i, int& j) {
   p = (int)(c >> 40);
   i = (int)((c >> 20) & 0xfffffu);
@@ -414,6 +432,18 @@ __device__ __forceinline__ FreqDev batch_freq(const BatchFreq& b) {
   f.v = b.v; f.w1 = b.w1; f.wk = b.wk; f.h = b.h; f.rho = b.rho; f.g = b.g; f.amp = b.amp;
   f.wvn = b.wvn;
   return f;
+}
+// the wave's member, from its first entry (always a real pair)
+__device__ __forceinline__ int wave_member(const uint64_t* codes, long long wave_base) {
+  return __builtin_amdgcn_readfirstlane((int)(codes[wave_base] >> 40));
+}
+// every wave copies its member's roots into its LDS slice (all waves of the
+// workgroup call it)
+__device__ __forceinline__ const double* stage_wave_wvn(const BatchFreq* F, int p, double (*swvn)[kNk]) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = lane; k < kNk; k += 64) swvn[w][k] = F[p].wvn[k];
+  __syncthreads();
+  return swvn[w];
 }
 
 __global__ __launch_bounds__(256) void k_bclassify_count(const BatchProb* __restrict__ P,
@@ -453,14 +483,19 @@ template <int REG>
 __global__ __launch_bounds__(256) void k_bassemble_list(const BatchProb* __restrict__ P,
                                                         const BatchFreq* __restrict__ F,
                                                         const uint64_t* __restrict__ codes, long long np) {
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < np;
-       t += (long long)gridDim.x * blockDim.x) {
-    int p, i, j;
-    decode_pair(codes[t], p, i, j);
-    const FreqDev f = batch_freq(F[p]);
-    const BatchProb& bp = P[p];
-    assemble_pair<REG>(bp.g, f, f.wvn, i, j, bp.A, bp.S, bp.ld);
-  }
+  __shared__ double swvn[4][kNk];
+  const long long base = (long long)blockIdx.x * blockDim.x;
+  const long long wb = base + (threadIdx.x & ~63);
+  const bool wave_in = wb < np;
+  const int p = wave_in ? wave_member(codes, wb) : 0;
+  const double* wvn = stage_wave_wvn(F, p, swvn);
+  const long long t = base + threadIdx.x;
+  const uint64_t c = t < np ? codes[t] : kNoPair;
+  if (c == kNoPair) return;
+  int pp, i, j;
+  decode_pair(c, pp, i, j);
+  const FreqDev f = batch_freq(F[p]);
+  assemble_pair<REG>(P[p].g, f, wvn, i, j, P[p].A, P[p].S, P[p].ld);
 }
 
 template <int REG, int G, int K>
@@ -468,54 +503,69 @@ __global__ __launch_bounds__(256) void k_bassemble_coop(const BatchProb* __restr
                                                         const BatchFreq* __restrict__ F,
                                                         const uint64_t* __restrict__ codes, long long np) {
   const int l = threadIdx.x & (G - 1);
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < np; base += (long long)gridDim.x * blockDim.x) {
+  __shared__ double swvn[4][kNk];
+  {
+    const long long base = (long long)blockIdx.x * blockDim.x;
+    const long long wb = base + (threadIdx.x & ~63);
+    const bool wave_in = wb < np;
+    const int p = wave_in ? wave_member(codes, wb) : 0;
+    const double* wvn = stage_wave_wvn(F, p, swvn);
+    if (!wave_in) return;                    // the whole wave (no group operations pending)
     const long long t = base + threadIdx.x;
-    const bool valid = t < np;
-    int p, i, j;
-    decode_pair(codes[valid ? t : np - 1], p, i, j);
+    uint64_t c = t < np ? codes[t] : kNoPair;
+    const bool valid = c != kNoPair;
+    if (!valid) c = codes[wb];               // padding: evaluate the wave's first pair, write nothing
+    int pp, i, j;
+    decode_pair(c, pp, i, j);
     const GeomDev& g = P[p].g;
     const double* xp = g.fp + 3 * (long long)i;
     const double* xq = g.xyz_p + 3 * (long long)j;
-    const double h = F[p].h;
+    const double h = F[p].h, v = F[p].v;
     const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
     const double zf = xq[2], zp = xp[2];
     const int nt = REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
     double L1 = 0.0, L2 = 0.0, L3 = 0.0;
     for (int q = 0; q < G; ++q) {
-      const int pq = group_bcast<G>(p, q);
-      const double hq = F[pq].h;
       const double r = group_bcast<G>(rr, q);
-      const double zfh = group_bcast<G>(zf, q) + hq, zph = group_bcast<G>(zp, q) + hq;
+      const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
       const int ntq = group_bcast<G>(nt, q);
       double l1, l2, l3;
-      coop_series<REG, G, K>(ntq, r, zfh, zph, hq, F[pq].v, F[pq].wvn, l, l1, l2, l3);
+      coop_series<REG, G, K>(ntq, r, zfh, zph, h, v, wvn, l, l1, l2, l3);
       if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
     }
-    if (valid) coop_finish<REG>(g, F[p].wvn, h, i, j, rr, L1, L2, L3, P[p].A, P[p].S, P[p].ld);
+    if (valid) coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, P[p].A, P[p].S, P[p].ld);
   }
 }
 
-// all pairs of the members in `list` (infinite depth, or a frequency limit):
-// t -> member by the prefix sums of their n^2
+// all pairs of the members in `list` (infinite depth): member ranges of n^2
+// pairs padded to multiples of 64 in `prefix`
 template <bool DEEP>
 __global__ __launch_bounds__(256) void k_bassemble_all(const BatchProb* __restrict__ P,
                                                        const BatchFreq* __restrict__ F, const int* __restrict__ list,
                                                        const long long* __restrict__ prefix, int nlist) {
+  __shared__ double swvn[4][kNk];
   const long long total = prefix[nlist];
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    int lo = 0, hi = nlist - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= t) lo = mid;
-      else hi = mid - 1;
+  {
+    const long long base = (long long)blockIdx.x * blockDim.x;
+    const long long wb = base + (threadIdx.x & ~63);
+    int lo = 0;
+    if (wb < total) {
+      int hi = nlist - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= wb) lo = mid;
+        else hi = mid - 1;
+      }
     }
+    lo = __builtin_amdgcn_readfirstlane(lo);
     const int p = list[lo];
-    const BatchProb& bp = P[p];
-    const long long e = t - prefix[lo];
-    const int i = (int)(e % bp.g.n), j = (int)(e / bp.g.n);
+    const double* wvn = DEEP ? nullptr : stage_wave_wvn(F, p, swvn);
+    const long long e = base + threadIdx.x - prefix[lo];
+    const int n = P[p].g.n;
+    if (wb >= total || e >= (long long)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
     const FreqDev f = batch_freq(F[p]);
-    assemble_pair<DEEP ? -1 : -2>(bp.g, f, f.wvn, i, j, bp.A, bp.S, bp.ld);
+    assemble_pair<DEEP ? -1 : -2>(P[p].g, f, DEEP ? f.wvn : wvn, i, j, P[p].A, P[p].S, P[p].ld);
   }
 }
 
@@ -545,10 +595,9 @@ hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F,
 }
 
 hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, const BatchFreq* F,
-                                 const uint64_t* codes, long long np, int variant, int grid) {
+                                 const uint64_t* codes, long long np, int variant) {
   if (np <= 0) return hipSuccess;
-  const long long need = (np + 255) / 256;
-  const dim3 gd((unsigned)(need < grid ? need : grid));
+  const dim3 gd((unsigned)((np + 255) / 256));
   switch (regime) {
     case kLinton: hipLaunchKernelGGL((k_bassemble_list<kLinton>), gd, dim3(256), 0, s, P, F, codes, np); break;
     case kPidcock:
@@ -567,10 +616,9 @@ hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, 
 }
 
 hipError_t batch_assemble_all(hipStream_t s, bool deep, const BatchProb* P, const BatchFreq* F, const int* list,
-                              const long long* prefix, int nlist, long long total, int grid) {
+                              const long long* prefix, int nlist, long long total) {
   if (nlist <= 0 || total <= 0) return hipSuccess;
-  const long long need = (total + 255) / 256;
-  const dim3 gd((unsigned)(need < grid ? need : grid));
+  const dim3 gd((unsigned)((total + 255) / 256));
   if (deep) hipLaunchKernelGGL((k_bassemble_all<true>), gd, dim3(256), 0, s, P, F, list, prefix, nlist);
   else hipLaunchKernelGGL((k_bassemble_all<false>), gd, dim3(256), 0, s, P, F, list, prefix, nlist);
   return hipGetLastError();

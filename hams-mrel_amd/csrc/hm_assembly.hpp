@@ -55,9 +55,9 @@ struct BatchFreq {             // the member's wave state, rewritten per frequen
 hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int n, unsigned* hist,
                           unsigned long long* cursor, uint64_t* codes, bool count_only);
 hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, const BatchFreq* F,
-                                 const uint64_t* codes, long long np, int variant, int grid);
+                                 const uint64_t* codes, long long np, int variant);
 hipError_t batch_assemble_all(hipStream_t s, bool deep, const BatchProb* P, const BatchFreq* F, const int* list,
-                              const long long* prefix, int nlist, long long total, int grid);
+                              const long long* prefix, int nlist, long long total);
 hipError_t batch_build_w(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int nmax);
 hipError_t batch_forces(hipStream_t s, const BatchProb* P, const BatchFreq* F, int count, int mmax, int ncolmax);
 

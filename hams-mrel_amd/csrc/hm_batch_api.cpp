@@ -5,7 +5,7 @@
 // (HAMS_Prog.f90:615-724). A batch holds many independent problems (members)
 // of different panel counts and depths resident in HBM at once and runs the
 // per-frequency chain for all of them together:
-//   * assembly: one persistent launch per finite-depth regime over the pairs of
+//   * assembly: one launch per finite-depth regime over the pairs of
 //     every finite-depth member (classified once into one list sorted by regime
 //     and series length across members), one over all pairs of the
 //     infinite-depth members;
@@ -18,8 +18,9 @@
 //     (the kernels read the members' wave state from device memory, which is
 //     rewritten before each replay).
 // An autotuner times the assembly launches of one frequency for the series
-// variants (lane-group shapes of the epsilon-accelerated regimes) and the
-// persistent grid sizes and keeps the fastest before the graph is captured.
+// variants (the lane-group shapes G x K of the epsilon-accelerated regimes:
+// one pair per lane, 16 x 7 / 8 x 7, 8 x 13 / 4 x 13) and keeps the fastest
+// before the graph is captured.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -57,7 +58,7 @@ struct hm_batch {
   bool prepared = false;
   int precision = HM_FP64, nrefine = 3, use_graph = 1;
   // tuning
-  int variant = 1, grid = 0, cus = 256;
+  int variant = 1, grid = 0, cus = 256;   // grid: assembly workgroups per frequency
   double tuned_ms = 0.0;
   // device
   void* pool = nullptr;
@@ -211,10 +212,10 @@ int issue_chain(hm_batch* b) {
   const int cnt = b->count();
   // assembly
   BCHK(b, hm::batch_assemble_all(s, true, b->d_prob, b->d_freq, b->d_deep, b->d_deep_prefix, b->ndeep,
-                                 b->deep_total, b->grid), "batch assembly (deep)");
+                                 b->deep_total), "batch assembly (deep)");
   for (int r = 0; r < 4; ++r)
     BCHK(b, hm::batch_assemble_regime(s, r, b->d_prob, b->d_freq, b->d_codes + b->seg_off[r], b->seg_cnt[r],
-                                      b->variant, b->grid), "batch assembly");
+                                      b->variant), "batch assembly");
   // right-hand sides R = S W
   BCHK(b, hm::batch_build_w(s, b->d_prob, b->d_freq, cnt, b->nmax), "batch rhs");
   BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->nmax, b->ncolmax), "batch rhs gemm");
@@ -283,10 +284,10 @@ int time_assembly(hm_batch* b, double* ms) {
   for (int rep = 0; rep < 2 && rc == HM_OK; ++rep) {
     (void)hipEventRecord(e0, b->stream);
     rc = b->hip(hm::batch_assemble_all(b->stream, true, b->d_prob, b->d_freq, b->d_deep, b->d_deep_prefix, b->ndeep,
-                                       b->deep_total, b->grid), "tune");
+                                       b->deep_total), "tune");
     for (int r = 0; r < 4 && rc == HM_OK; ++r)
       rc = b->hip(hm::batch_assemble_regime(b->stream, r, b->d_prob, b->d_freq, b->d_codes + b->seg_off[r],
-                                            b->seg_cnt[r], b->variant, b->grid), "tune");
+                                            b->seg_cnt[r], b->variant), "tune");
     (void)hipEventRecord(e1, b->stream);
     if (rc == HM_OK) rc = b->hip(hipEventSynchronize(e1), "tune");
     float t = 0.f;
@@ -341,7 +342,11 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     if (m.h > 0.0) finite_pairs += (long long)n * n;
   }
   const size_t o_prob = take(sizeof(hm::BatchProb) * cnt), o_freq = take(sizeof(hm::BatchFreq) * cnt);
-  const size_t o_codes = take(8 * (size_t)finite_pairs);
+  // pair list: every (key, member) run padded to a multiple of 64 entries
+  long long nfinite = 0;
+  for (int p = 0; p < cnt; ++p) nfinite += b->mem[p].h > 0.0;
+  const size_t code_cap = (size_t)finite_pairs + 64 * (size_t)hm::kNumKeys * (size_t)nfinite;
+  const size_t o_codes = take(8 * code_cap);
   const size_t o_deep = take(4 * (size_t)cnt), o_prefix = take(8 * (size_t)(cnt + 1));
   const size_t o_rhs = take(sizeof(hm::GemmDesc<double>) * cnt), o_res = take(sizeof(hm::GemmDesc<double>) * cnt);
   const size_t o_lu64 = take(sizeof(hm::LuDesc<double>) * cnt), o_lu32 = take(sizeof(hm::LuDesc<float>) * cnt);
@@ -434,7 +439,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     }
     if (m.h < 0.0) {
       deep.push_back(p);
-      prefix.push_back(prefix.back() + (long long)n * n);
+      prefix.push_back(prefix.back() + (((long long)n * n + 63) & ~63LL));
     }
     b->off_amas[p] = b->out_amas;
     b->off_bdmp[p] = b->out_amas;
@@ -466,32 +471,51 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     b->h_freq[p].h = b->mem[p].h;
   }
   BCHK(b, hipMemcpyAsync(b->d_freq, b->h_freq, sizeof(hm::BatchFreq) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
-  // ---- classification of the finite-depth members' pairs into one sorted list
+  // ---- classification of the finite-depth members' pairs into one list sorted
+  // by key (regime, series length) and, within a key, by member; each
+  // (key, member) run padded to a multiple of 64 so that no wave mixes members
   unsigned* d_hist = (unsigned*)at(o_hist);
   unsigned long long* d_cursor = (unsigned long long*)at(o_cursor);
-  BCHK(b, hipMemsetAsync(d_hist, 0, sizeof(unsigned) * hm::kNumKeys, b->stream), "batch classify");
-  for (int p = 0; p < cnt; ++p)
-    if (b->mem[p].h > 0.0)
-      BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, d_hist, nullptr, nullptr, true), "batch classify");
-  std::vector<unsigned> hist(hm::kNumKeys);
-  BCHK(b, hipMemcpyAsync(hist.data(), d_hist, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, b->stream), "batch classify");
-  BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
-  std::vector<unsigned long long> cur(hm::kNumKeys);
-  unsigned long long acc = 0;
-  for (int k = 0; k < hm::kNumKeys; ++k) { cur[k] = acc; acc += hist[k]; }
-  if ((long long)acc != finite_pairs) return b->fail(HM_ERR_STATE, "batch classification lost pairs");
-  for (int r = 0; r < 4; ++r) {
-    b->seg_off[r] = (long long)cur[r * hm::kKeyStride];
-    b->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
+  std::vector<std::vector<unsigned>> hist(cnt);
+  for (int p = 0; p < cnt; ++p) {
+    if (!(b->mem[p].h > 0.0)) continue;
+    hist[p].assign(hm::kNumKeys, 0u);
+    BCHK(b, hipMemsetAsync(d_hist, 0, sizeof(unsigned) * hm::kNumKeys, b->stream), "batch classify");
+    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, d_hist, nullptr, nullptr, true), "batch classify");
+    BCHK(b, hipMemcpyAsync(hist[p].data(), d_hist, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, b->stream), "batch classify");
+    BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
   }
-  BCHK(b, hipMemcpyAsync(d_cursor, cur.data(), 8 * hm::kNumKeys, hipMemcpyHostToDevice, b->stream), "batch classify");
+  std::vector<std::vector<unsigned long long>> cur(cnt);
   for (int p = 0; p < cnt; ++p)
-    if (b->mem[p].h > 0.0)
-      BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, nullptr, d_cursor, b->d_codes, false), "batch classify");
-  BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
+    if (!hist[p].empty()) cur[p].assign(hm::kNumKeys, 0ull);
+  unsigned long long acc = 0, real = 0;
+  std::vector<unsigned long long> key_start(hm::kNumKeys + 1, 0);
+  for (int k = 0; k < hm::kNumKeys; ++k) {
+    key_start[k] = acc;
+    for (int p = 0; p < cnt; ++p) {
+      if (hist[p].empty()) continue;
+      cur[p][k] = acc;
+      acc += ((unsigned long long)hist[p][k] + 63ull) & ~63ull;
+      real += hist[p][k];
+    }
+  }
+  key_start[hm::kNumKeys] = acc;
+  if ((long long)real != finite_pairs || acc > code_cap) return b->fail(HM_ERR_STATE, "batch classification lost pairs");
+  for (int r = 0; r < 4; ++r) {
+    b->seg_off[r] = (long long)key_start[r * hm::kKeyStride];
+    b->seg_cnt[r] = (long long)(key_start[(r + 1) * hm::kKeyStride] - key_start[r * hm::kKeyStride]);
+  }
+  BCHK(b, hipMemsetAsync(b->d_codes, 0xFF, 8 * (size_t)acc, b->stream), "batch classify");   // padding = no pair
+  for (int p = 0; p < cnt; ++p) {
+    if (hist[p].empty()) continue;
+    BCHK(b, hipMemcpyAsync(d_cursor, cur[p].data(), 8 * hm::kNumKeys, hipMemcpyHostToDevice, b->stream), "batch classify");
+    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, nullptr, d_cursor, b->d_codes, false), "batch classify");
+    BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
+  }
+  b->grid = (int)((b->deep_total + 255) / 256);
+  for (int r = 0; r < 4; ++r) b->grid += (int)((b->seg_cnt[r] + 255) / 256);
   // ---- autotuning on the middle of the frequency range (1 rad/s)
   b->variant = 1;
-  b->grid = 16 * b->cus;
   b->tuned_ms = 0.0;
   if (autotune) {
     std::vector<double> w(cnt, 1.0);
@@ -506,14 +530,6 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
       if (ms < best) { best = ms; bv = v; }
     }
     b->variant = bv;
-    int bg = b->grid;
-    for (int mult : {4, 8, 16, 32, 64}) {
-      b->grid = mult * b->cus;
-      double ms;
-      if ((rc = time_assembly(b, &ms))) return rc;
-      if (ms < best) { best = ms; bg = b->grid; }
-    }
-    b->grid = bg;
     b->tuned_ms = best;
   }
   // ---- graph

@@ -209,21 +209,25 @@ int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, in
  * (assembly, ZGETRF/ZGETRS, forces; HAMS_Prog.f90:615-724) serially. A batch
  * keeps many problems (members; any panel counts, finite or infinite depth, no
  * interior waterplane points) resident in HBM and runs that chain for all of
- * them at once per call: one persistent assembly launch per regime over the
- * pairs of every member, grouped launches (each member at its own order) for
+ * them at once per call: one assembly launch per regime over the pairs of
+ * every member (each wave's pairs from one member), grouped launches (each
+ * member at its own order) for
  * the right-hand sides, the LU and the triangular solves, optionally captured
  * once in a HIP graph and replayed per frequency.
  * hm_batch_add_problem takes one member (arguments as hm_set_geometry,
  * hm_set_depth and hm_set_headings, ni = 0) and returns its index >= 0.
  * hm_batch_prepare(precision HM_FP64 or HM_FP32_IR, autotune, use_graph,
- * nrefine): allocates, classifies, times the series variants and persistent
- * grid sizes of the assembly when autotune != 0 and keeps the fastest, and
+ * nrefine): allocates, classifies, times the series variants (lane-group
+ * shapes) of the assembly when autotune != 0 and keeps the fastest, and
  * captures the chain when use_graph != 0. HM_FP32_IR factors in FP32 and runs
  * nrefine FP64-residual refinement sweeps inside the chain, then more (at most
  * 10) until every member's last correction is below 1e-13 of its solution.
  * hm_batch_solve(w1[member] > 0): outputs stacked by member in member order
  * (amas, bdmp: (6 nbody)^2 each; exfc: 6 nbody x nbeta complex). A positive
- * return is a zero pivot of some member (hm_batch_error_string names it). */
+ * return is a zero pivot of some member (hm_batch_error_string names it).
+ * hm_batch_info: members, largest order, chosen series variant, assembly
+ * workgroups per frequency, tuned assembly time, the last refinement
+ * correction, refinement sweeps run after the graph, device bytes held. */
 typedef struct hm_batch hm_batch;
 int hm_batch_create(hm_batch **batch);
 int hm_batch_destroy(hm_batch *batch);
