@@ -100,9 +100,11 @@ struct hm_ctx {
   int sur_quant = 1;
   double sur_tol = 1e-4;
   long long sur_counts[3] = {0, 0, 0};
+  void* sur_img() const { return static_cast<char*>(d_sur.p) + hm::kSurImgOffset; }
   hm::SurDev sur_dev() const {
     const float* b = d_sur.as<float>();
-    return hm::SurDev{b, b + hm::kSurParams, b + 2 * hm::kSurParams, b + 2 * hm::kSurParams + hm::kSurOut, sur_quant};
+    return hm::SurDev{b, b + hm::kSurParams, b + 2 * hm::kSurParams, b + 2 * hm::kSurParams + hm::kSurOut, sur_quant,
+                      sur_img()};
   }
   // H-matrix form (equal bodies, no waterplane points): dense diagonal blocks
   // (block b at b m (m+1) complex elements, leading dimension m, so that the
@@ -480,36 +482,33 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
                                     c->series_variant), "assemble");
       c->stage_end(kAsmRegime0 + r);
     }
-    if (sur && c->seg_cnt[2] + c->seg_cnt[3] > 0) {
-      // network for the EIGENE + EIGEN segments (contiguous in the key order);
-      // pairs outside its validated cells go to the plain kernels
-      const long long np = c->seg_cnt[2] + c->seg_cnt[3];
-      HIPCHK(c, c->d_fb.ensure(sizeof(uint32_t) * (size_t)np), "surrogate alloc");
-      HIPCHK(c, c->d_fbcnt.ensure(3 * sizeof(unsigned long long)), "surrogate alloc");
-      HIPCHK(c, hipMemsetAsync(c->d_fbcnt.p, 0, 3 * sizeof(unsigned long long), c->stream), "surrogate");
-      uint32_t* fb_e = c->d_fb.as<uint32_t>();
-      uint32_t* fb_g = fb_e + c->seg_cnt[2];
-      const double kh = c->w1 * c->w1 * c->h / c->g_disp;
-      c->stage_begin(kAsmSurrogate);
-      HIPCHK(c, hm::assemble_surrogate(c->stream, gd, f, kh, pairs + c->seg_off[2], np, c->sur_dev(), (float)c->sur_tol,
-                                       A, S, ld, fb_e, fb_g, c->d_fbcnt.as<unsigned long long>()), "assemble surrogate");
-      c->stage_end(kAsmSurrogate);
-      unsigned long long cnt[3] = {0, 0, 0};
-      HIPCHK(c, hipMemcpyAsync(cnt, c->d_fbcnt.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream), "surrogate");
-      HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate");
-      if ((long long)cnt[0] > c->seg_cnt[2] || (long long)cnt[1] > c->seg_cnt[3])
-        return c->fail(HM_ERR_STATE, "surrogate fallback lists overflow");
-      c->sur_counts[0] += np - (long long)(cnt[0] + cnt[1] + cnt[2]);
-      c->sur_counts[1] += (long long)cnt[0];
-      c->sur_counts[2] += (long long)(cnt[1] + cnt[2]);
-      for (int r = 2; r < 4; ++r) {
-        const long long m = (long long)cnt[r - 2];
-        if (!m) continue;
-        c->stage_begin(kAsmRegime0 + r);
-        HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, r == 2 ? fb_e : fb_g, m, A, S, ld, c->series_variant),
-               "assemble fallback");
-        c->stage_end(kAsmRegime0 + r);
+    if (sur) {
+      // EIGENE segment: the network takes the pairs inside its validated cells
+      // and marks them done; the plain series kernel runs over the segment with
+      // that mask. EIGEN (R/h >= 0.5, at most 9 terms) stays plain.
+      const long long ne = c->seg_cnt[2];
+      if (ne > 0) {
+        HIPCHK(c, c->d_fb.ensure(hm::sur_regular_bytes(ne)), "surrogate alloc");
+        const double kh = c->w1 * c->w1 * c->h / c->g_disp;
+        hm::SurPartition part{};
+        c->stage_begin(kAsmSurrogate);
+        HIPCHK(c, hm::sur_regular_parts(c->stream, gd, f, kh, pairs + c->seg_off[2], ne, c->sur_dev(),
+                                        (float)c->sur_tol, c->d_fb.p, c->d_fbcnt.as<unsigned long long>(), part),
+               "assemble surrogate");
+        c->stage_end(kAsmSurrogate);
+        c->stage_begin(kAsmRegime0 + 2);
+        HIPCHK(c, hm::assemble_regime(c->stream, 2, gd, f, part.pairs, ne, A, S, ld, c->series_variant, part.mask,
+                                      part.vreg), "assemble");
+        c->stage_end(kAsmRegime0 + 2);
       }
+      if (c->seg_cnt[3] > 0) {
+        c->stage_begin(kAsmRegime0 + 3);
+        HIPCHK(c, hm::assemble_regime(c->stream, 3, gd, f, pairs + c->seg_off[3], c->seg_cnt[3], A, S, ld,
+                                      c->series_variant), "assemble");
+        c->stage_end(kAsmRegime0 + 3);
+      }
+      c->sur_counts[1] += ne;
+      c->sur_counts[2] += c->seg_cnt[3];
     }
   }
   return HM_OK;
@@ -1209,7 +1208,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   fill(hm::kSurOffW0, hm::kSurH, hm::kSurIn);
   for (int l = 0; l < 3; ++l) fill(hm::kSurOffW1 + l * hm::kSurHiddenStride, hm::kSurH, hm::kSurH);
   fill(hm::kSurOffW4, hm::kSurOut, hm::kSurH);
-  HIPCHK(c, c->d_sur.ensure(sizeof(float) * (2 * (size_t)hm::kSurParams + hm::kSurOut + hm::kSurMap)), "surrogate alloc");
+  HIPCHK(c, c->d_sur.ensure(hm::kSurBufBytes), "surrogate alloc");
   float* P = c->d_sur.as<float>();
   float* Q = P + hm::kSurParams;
   float* YS = Q + hm::kSurParams;
@@ -1252,6 +1251,8 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
     HIPCHK(c, hipStreamSynchronize(s), "surrogate");
     last_loss = l / (3.0 * (double)(steps_per_epoch * B));
   }
+  // the inference image of the trained network
+  HIPCHK(c, hm::sur_round(s, P, Q, c->sur_img()), "surrogate");
   // validation set on other Kh values: error map and RMS error
   std::vector<double> khv, wvnv;
   kh_table(512, seed ^ 0xa11da7eull, c->g_disp, khv, wvnv);
@@ -1264,8 +1265,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   HIPCHK(c, hm::sur_generate(s, nval, seed ^ 0x7a11dull, 512, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
                              tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>()), "surrogate validation");
   c->sur_quant = quant ? 1 : 0;
-  hm::SurDev md{P, Q, YS, MAP, c->sur_quant};
-  HIPCHK(c, hm::sur_predict(s, nval, tb.X.as<float>(), md, tb.U.as<float>()), "surrogate predict");
+  HIPCHK(c, hm::sur_predict(s, nval, tb.X.as<float>(), c->sur_dev(), tb.U.as<float>()), "surrogate predict");
   HIPCHK(c, hipMemsetAsync(tb.bits.p, 0, sizeof(unsigned) * hm::kSurMap, s), "surrogate");
   HIPCHK(c, hipMemsetAsync(tb.loss.p, 0, sizeof(double), s), "surrogate");
   HIPCHK(c, hm::sur_error_map(s, nval, tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>(), tb.U.as<float>(),
@@ -1320,12 +1320,12 @@ extern "C" int hm_surrogate_get(hm_ctx* c, float* blob, int n) {
 extern "C" int hm_surrogate_set(hm_ctx* c, const float* blob, int n) {
   if (!c) return HM_ERR_ARG - 1;
   if (!blob || n < hm::kSurBlob) return c->fail(HM_ERR_ARG - 3, "blob needs %d floats", hm::kSurBlob);
-  HIPCHK(c, c->d_sur.ensure(sizeof(float) * (2 * (size_t)hm::kSurParams + hm::kSurOut + hm::kSurMap)), "surrogate alloc");
+  HIPCHK(c, c->d_sur.ensure(hm::kSurBufBytes), "surrogate alloc");
   float* b = c->d_sur.as<float>();
   HIPCHK(c, hipMemcpyAsync(b, blob, sizeof(float) * hm::kSurParams, hipMemcpyHostToDevice, c->stream), "surrogate set");
   HIPCHK(c, hipMemcpyAsync(b + 2 * hm::kSurParams, blob + hm::kSurParams, sizeof(float) * (hm::kSurOut + hm::kSurMap),
                            hipMemcpyHostToDevice, c->stream), "surrogate set");
-  HIPCHK(c, hm::sur_round(c->stream, b, b + hm::kSurParams), "surrogate set");
+  HIPCHK(c, hm::sur_round(c->stream, b, b + hm::kSurParams, c->sur_img()), "surrogate set");
   HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate set");
   c->sur_ready = true;
   c->assembled = c->factored = c->solved = false;
@@ -1336,6 +1336,8 @@ extern "C" int hm_set_surrogate(hm_ctx* c, int enable, double tol) {
   if (!c) return HM_ERR_ARG - 1;
   if (enable && !c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
   if (!(tol >= 0.0)) return c->fail(HM_ERR_ARG - 3, "tol must be >= 0");
+  HIPCHK(c, c->d_fbcnt.ensure(sizeof(unsigned long long)), "surrogate alloc");
+  HIPCHK(c, hipMemsetAsync(c->d_fbcnt.p, 0, sizeof(unsigned long long), c->stream), "surrogate");
   c->sur_on = enable != 0;
   c->sur_tol = tol;
   for (long long& k : c->sur_counts) k = 0;
@@ -1346,7 +1348,15 @@ extern "C" int hm_set_surrogate(hm_ctx* c, int enable, double tol) {
 extern "C" int hm_surrogate_counts(hm_ctx* c, long long* counts) {
   if (!c) return HM_ERR_ARG - 1;
   if (!counts) return c->fail(HM_ERR_ARG - 2, "counts is NULL");
-  for (int k = 0; k < 3; ++k) counts[k] = c->sur_counts[k];
+  // the network's count accumulates on the device (no host sync per assembly)
+  unsigned long long net = 0;
+  if (c->d_fbcnt.p) {
+    HIPCHK(c, hipMemcpyAsync(&net, c->d_fbcnt.p, sizeof net, hipMemcpyDeviceToHost, c->stream), "surrogate counts");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate counts");
+  }
+  counts[0] = (long long)net;
+  counts[1] = c->sur_counts[1] - (long long)net;
+  counts[2] = c->sur_counts[2];
   return HM_OK;
 }
 
@@ -1369,6 +1379,24 @@ extern "C" int hm_surrogate_eval(hm_ctx* c, int n, const double* rh, const doubl
   }
   return hm::sur_eval_points(c->stream, n, rh, zs, zd, kh, wv.data(), c->g_disp, c->sur_dev(), t_net, t_plain, bound)
              == hipSuccess ? HM_OK : c->fail(HM_ERR_HIP, "surrogate eval failed");
+}
+
+extern "C" int hm_surrogate_predict(hm_ctx* c, int n, const float* x, float* u) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
+  if (!x || !u) return c->fail(HM_ERR_ARG - 3, "x or u is NULL");
+  if (!c->sur_ready) return c->fail(HM_ERR_STATE, "no surrogate trained or loaded");
+  DevBuf d;
+  int rc = c->hip(d.ensure(sizeof(float) * 7 * (size_t)n), "surrogate predict");
+  float* dx = d.as<float>();
+  float* du = dx + 4 * (size_t)n;
+  if (rc == HM_OK &&
+      (rc = c->hip(hipMemcpyAsync(dx, x, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice, c->stream), "surrogate predict")) == HM_OK &&
+      (rc = c->hip(hm::sur_predict(c->stream, n, dx, c->sur_dev(), du), "surrogate predict")) == HM_OK &&
+      (rc = c->hip(hipMemcpyAsync(u, du, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToHost, c->stream), "surrogate predict")) == HM_OK)
+    rc = c->hip(hipStreamSynchronize(c->stream), "surrogate predict");
+  d.release();
+  return rc;
 }
 
 extern "C" int hm_surrogate_grad(hm_ctx* c, int batch, const float* x, const float* u, int quant, float* grad,

@@ -73,16 +73,56 @@ __global__ __launch_bounds__(256) void k_assemble_grid(GeomDev g, FreqDev f, dou
   assemble_pair<REG>(g, f, wvn, i, j, A, S, ld);
 }
 
-// one regime over its classified pair list (i in the high 16 bits, j in the low)
+// the rest of one EIGENE / PIDCOCK pair once its series limits L are known
+// (regular = true: L is the regular part V = T - (1/r + 1/r1) of an EIGENE
+// pair, the 1/r + 1/r1 strip already applied)
+template <int REG>
+__device__ __forceinline__ void coop_finish(const GeomDev& g, const double* wvn, double h, int i, int j, double rr,
+                                            double L1, double L2, double L3, double* A, double* S, long long ld,
+                                            bool regular = false) {
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double zf = xq[2], zp = xp[2];
+  FinOut fo{0, 0, 0, 0, 0, 0};
+  if (REG == kEigene) {
+    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+    fo.g_re += L1;
+    fo.gr_re += L2;
+    fo.gz_re += L3;
+    if (!regular) fin_strip(rr, zf, zp, fo);
+  } else {
+    fin_pidcock_head(rr, zf, zp, wvn, h, fo);
+    fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
+  }
+  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
+  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
+}
+
+// one regime over its classified pair list (i in the high 16 bits, j in the
+// low). EIGENE with a mask: pairs with done[t] != 0 take the regular part
+// vreg[3 t .. 3 t + 2] instead of the series.
 template <int REG>
 __global__ __launch_bounds__(256) void k_assemble_list(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
-                                                       long long np, double* A, double* S, long long ld) {
+                                                       long long np, double* A, double* S, long long ld,
+                                                       const uint8_t* __restrict__ done,
+                                                       const float* __restrict__ vreg) {
   __shared__ double wvn[kNk];
   load_wvn(wvn, f.wvn, kNk);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= np) return;
   const uint32_t p = pairs[t];
-  assemble_pair<REG>(g, f, wvn, (int)(p >> 16), (int)(p & 0xffffu), A, S, ld);
+  const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
+  if (REG == kEigene && done && done[t]) {
+    const double* xp = g.fp + 3 * (long long)i;
+    const double* xq = g.xyz_p + 3 * (long long)j;
+    coop_finish<kEigene>(g, wvn, f.h, i, j, horiz_dist(xq[0], xp[0], xq[1], xp[1]), vreg[3 * t], vreg[3 * t + 1],
+                         vreg[3 * t + 2], A, S, ld, true);
+    return;
+  }
+  assemble_pair<REG>(g, f, wvn, i, j, A, S, ld);
 }
 
 // The epsilon-accelerated regimes with G lanes per pair (see hm_coop.hpp): each
@@ -127,38 +167,17 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
   l3 = coop_limes<G, K>(n, s3, l);
 }
 
-// the rest of one pair once its epsilon limits are known
-template <int REG>
-__device__ __forceinline__ void coop_finish(const GeomDev& g, const double* wvn, double h, int i, int j, double rr,
-                                            double L1, double L2, double L3, double* A, double* S, long long ld) {
-  const double* xp = g.fp + 3 * (long long)i;
-  const double* xq = g.xyz_p + 3 * (long long)j;
-  const double zf = xq[2], zp = xp[2];
-  FinOut fo{0, 0, 0, 0, 0, 0};
-  if (REG == kEigene) {
-    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
-    fo.g_re += L1;
-    fo.gr_re += L2;
-    fo.gz_re += L3;
-    fin_strip(rr, zf, zp, fo);
-  } else {
-    fin_pidcock_head(rr, zf, zp, wvn, h, fo);
-    fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
-  }
-  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
-  const int flag = near_flag(xp, xq, g.pnsz[j]);
-  Grn4 gr;
-  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
-  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
-}
-
 template <int REG, int G, int K>
 __global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
-                                                       long long np, double* A, double* S, long long ld) {
+                                                       long long np, double* A, double* S, long long ld,
+                                                       const uint8_t* __restrict__ done,
+                                                       const float* __restrict__ vreg) {
   __shared__ double wvn[kNk];
   load_wvn(wvn, f.wvn, kNk);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < np;
+  // a pair whose regular part is given (masked EIGENE list) runs no series
+  const bool given = REG == kEigene && valid && done && done[t];
   const uint32_t p = pairs[valid ? t : np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
   const double* xp = g.fp + 3 * (long long)i;
@@ -166,19 +185,26 @@ __global__ __launch_bounds__(256) void k_assemble_coop(GeomDev g, FreqDev f, con
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const double zf = xq[2], zp = xp[2];
-  const int nt = REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
+  // no series for lanes without a pair to evaluate (end of list, or done)
+  const int nt = !valid || given ? 0 : REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
   const int l = threadIdx.x & (G - 1);
   double L1 = 0.0, L2 = 0.0, L3 = 0.0;
   for (int q = 0; q < G; ++q) {
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
     const int ntq = group_bcast<G>(nt, q);
+    if (ntq == 0) continue;                  // uniform over the group
     double l1, l2, l3;
     coop_series<REG, G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
   }
   if (!valid) return;
-  coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld);
+  if (given) {
+    L1 = vreg[3 * t];
+    L2 = vreg[3 * t + 1];
+    L3 = vreg[3 * t + 2];
+  }
+  coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld, given);
 }
 
 // ---- classification --------------------------------------------------------
@@ -248,22 +274,23 @@ hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool
 }
 
 hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
-                           long long np, double* A, double* S, long long ld, int variant) {
+                           long long np, double* A, double* S, long long ld, int variant, const uint8_t* done,
+                           const float* vreg) {
   if (np <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)((np + 255) / 256);
+  const dim3 gd((unsigned)((np + 255) / 256)), bd(256);
   switch (regime) {
-    case kLinton: hipLaunchKernelGGL((k_assemble_list<kLinton>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    case kLinton: hipLaunchKernelGGL((k_assemble_list<kLinton>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg); break;
     case kPidcock:
-      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 16, 7>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
-      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 8, 13>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
-      else hipLaunchKernelGGL((k_assemble_list<kPidcock>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 16, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kPidcock, 8, 13>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+      else hipLaunchKernelGGL((k_assemble_list<kPidcock>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
       break;
     case kEigene:
-      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kEigene, 8, 7>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
-      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kEigene, 4, 13>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
-      else hipLaunchKernelGGL((k_assemble_list<kEigene>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld);
+      if (variant == 1) hipLaunchKernelGGL((k_assemble_coop<kEigene, 8, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+      else if (variant == 2) hipLaunchKernelGGL((k_assemble_coop<kEigene, 4, 13>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+      else hipLaunchKernelGGL((k_assemble_list<kEigene>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
       break;
-    default: hipLaunchKernelGGL((k_assemble_list<kEigen>), dim3(blocks), dim3(256), 0, s, g, f, pairs, np, A, S, ld); break;
+    default: hipLaunchKernelGGL((k_assemble_list<kEigen>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg); break;
   }
   return hipGetLastError();
 }

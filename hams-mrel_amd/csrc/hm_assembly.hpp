@@ -33,8 +33,12 @@ hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* h
                           uint32_t* pairs, bool count_only, int blk = 0);
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
                          long long ld, int blk = 0);
+// done, vreg (optional, EIGENE only): pairs with done[t] != 0 take the regular
+// part V = T - (1/r + 1/r1) of their evanescent sum T from vreg[3 t .. 3 t + 2]
+// (value, R- and z-derivative) instead of evaluating the series
 hipError_t assemble_regime(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
-                           long long np, double* A, double* S, long long ld, int variant = 1);
+                           long long np, double* A, double* S, long long ld, int variant = 1,
+                           const uint8_t* done = nullptr, const float* vreg = nullptr);
 hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                    double yw, double* W, int ldw);
 hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,

@@ -109,7 +109,144 @@ __device__ __forceinline__ void mlp_eval(const float* w, const float* x, float* 
   }
 }
 
-// ---- data ---------------------------------------------------------------------
+// ---- inference on MFMA ----------------------------------------------------------
+// The three 64 x 64 hidden layers of a wave's 64 pairs are 16 x 16 x 16 BF16
+// MFMA tiles with FP32 accumulation, computed transposed: out[o][pair] =
+// sum_k W[o][k] act[pair][k] (weights as the A operand, 4 output tiles x 4 pair
+// tiles x 4 input steps per layer), so that each lane ends up with 4
+// consecutive output features of one pair and stores them, with bias, SiLU and
+// BF16 rounding applied, as one 8-byte LDS write in the layout the next layer
+// reads. The first layer (FP32 weights) and the output layer run per lane.
+// Operand layout of v_mfma_f32_16x16x16_bf16 (lane l, slot i < 4, k = 4 (l / 16)
+// + i): A[l % 16][k], B[k][l % 16]; result slot v is D[4 (l / 16) + v][l % 16].
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// element (row, col) of a 64 x 64 BF16 tile: the 8-byte chunk index is XORed
+// with the row (mod 16), so the 16 rows of a fragment access fall on distinct
+// LDS banks
+__device__ __forceinline__ int tile_at(int row, int col) {
+  return row * kSurH + ((((col >> 2) ^ (row & 15))) << 2) + (col & 3);
+}
+
+// the network as the inference kernels hold it in LDS (also its image in
+// device memory, built once per parameter change by k_build_img)
+struct alignas(16) SurImg {
+  unsigned short w[3][kSurH * kSurH];        // hidden weights [out][in], BF16 bits, tile_at layout
+  float b[3][kSurH];
+  float w0[kSurH][kSurIn], b0[kSurH];
+  float w4[kSurOut][kSurH], b4[kSurOut], pad;
+};
+static_assert(sizeof(SurImg) == kSurImgBytes, "surrogate image size");
+static_assert(kSurImgBytes % 16 == 0, "surrogate image copies in 16-byte units");
+struct alignas(16) SurLds {
+  SurImg m;
+  unsigned short act[4][kSurH * kSurH];      // per wave: 64 pairs x 64 activations, BF16 bits, tile_at layout
+};
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) { return (unsigned short)(__float_as_uint(x) >> 16); }
+__device__ __forceinline__ float bf16_val(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+// two floats to BF16 (round to nearest even, as bf16r) packed low, high
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{a, b}, bf16x2v));
+}
+// z sigmoid(z) with the hardware exp2 and reciprocal (relative error ~1e-7,
+// far below the BF16 rounding that follows)
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+
+// image of the BF16-rounded parameters q (one workgroup)
+__global__ __launch_bounds__(256) void k_build_img(const float* q, SurImg* img) {
+  for (int e = threadIdx.x; e < 3 * kSurH * kSurH; e += blockDim.x) {
+    const int l = e / (kSurH * kSurH), r = e % (kSurH * kSurH);
+    img->w[l][tile_at(r / kSurH, r % kSurH)] = bf16_bits(q[kSurOffW1 + l * kSurHiddenStride + r]);
+  }
+  for (int e = threadIdx.x; e < 3 * kSurH; e += blockDim.x)
+    img->b[e / kSurH][e % kSurH] = q[kSurOffW1 + (e / kSurH) * kSurHiddenStride + kSurH * kSurH + e % kSurH];
+  for (int e = threadIdx.x; e < kSurH * kSurIn; e += blockDim.x) img->w0[e / kSurIn][e % kSurIn] = q[kSurOffW0 + e];
+  for (int e = threadIdx.x; e < kSurH; e += blockDim.x) img->b0[e] = q[kSurOffB0 + e];
+  for (int e = threadIdx.x; e < kSurOut * kSurH; e += blockDim.x) img->w4[e / kSurH][e % kSurH] = q[kSurOffW4 + e];
+  for (int e = threadIdx.x; e < kSurOut; e += blockDim.x) img->b4[e] = q[kSurOffB4 + e];
+  if (threadIdx.x == 0) img->pad = 0.0f;
+}
+
+__device__ __forceinline__ void load_sur_lds(SurImg& L, const void* img) {
+  const uint4* src = static_cast<const uint4*>(img);
+  uint4* dst = reinterpret_cast<uint4*>(&L);
+  for (int e = threadIdx.x; e < (int)(kSurImgBytes / 16); e += blockDim.x) dst[e] = src[e];
+}
+
+// y = network(x) for this lane's pair; every thread of the (256-thread)
+// workgroup calls it, after L.m is loaded and a barrier
+__device__ void mlp_block_mfma(SurLds& L, const float* x, float* y) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned short* act = L.act[w];
+  for (int c = 0; c < kSurH / 4; ++c) {
+    float a[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = 4 * c + v;
+      float z = L.m.b0[i];
+#pragma unroll
+      for (int j = 0; j < kSurIn; ++j) z = fmaf(L.m.w0[i][j], x[j], z);
+      a[v] = silu_fast(z);
+    }
+    *reinterpret_cast<uint2*>(&act[tile_at(lane, 4 * c)]) = uint2{pack_bf16(a[0], a[1]), pack_bf16(a[2], a[3])};
+  }
+  const int r16 = lane & 15, kq = 4 * (lane >> 4);
+  for (int l = 0; l < 3; ++l) {
+    __syncthreads();
+    f32x4 acc[4][4];                         // [output tile][pair tile]
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) acc[ot][pt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      bf16x4 wf[4], af[4];
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot) wf[ot] = *reinterpret_cast<const bf16x4*>(&L.m.w[l][tile_at(16 * ot + r16, 16 * s + kq)]);
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) af[pt] = *reinterpret_cast<const bf16x4*>(&act[tile_at(16 * pt + r16, 16 * s + kq)]);
+#pragma unroll
+      for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+        for (int pt = 0; pt < 4; ++pt)
+          acc[ot][pt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wf[ot], af[pt], acc[ot][pt], 0, 0, 0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      const float4 bb = *reinterpret_cast<const float4*>(&L.m.b[l][16 * ot + kq]);
+#pragma unroll
+      for (int pt = 0; pt < 4; ++pt) {
+        const float a0 = silu_fast(acc[ot][pt][0] + bb.x), a1 = silu_fast(acc[ot][pt][1] + bb.y);
+        const float a2 = silu_fast(acc[ot][pt][2] + bb.z), a3 = silu_fast(acc[ot][pt][3] + bb.w);
+        *reinterpret_cast<uint2*>(&act[tile_at(16 * pt + r16, 16 * ot + kq)]) = uint2{pack_bf16(a0, a1), pack_bf16(a2, a3)};
+      }
+    }
+  }
+  __syncthreads();
+  float z0 = L.m.b4[0], z1 = L.m.b4[1], z2 = L.m.b4[2];
+  for (int c = 0; c < kSurH / 4; ++c) {
+    const uint2 u = *reinterpret_cast<const uint2*>(&act[tile_at(lane, 4 * c)]);
+    const float a[4] = {bf16_val((unsigned short)u.x), bf16_val((unsigned short)(u.x >> 16)),
+                        bf16_val((unsigned short)u.y), bf16_val((unsigned short)(u.y >> 16))};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      z0 = fmaf(L.m.w4[0][4 * c + v], a[v], z0);
+      z1 = fmaf(L.m.w4[1][4 * c + v], a[v], z1);
+      z2 = fmaf(L.m.w4[2][4 * c + v], a[v], z2);
+    }
+  }
+  y[0] = z0;
+  y[1] = z1;
+  y[2] = z2;
+}
+
 __global__ __launch_bounds__(256) void k_generate(long long n, uint64_t seed, int nkh, const double* kh,
                                                   const double* wvn, float* X, float* Y, float* M, int* cell) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -295,32 +432,38 @@ __global__ __launch_bounds__(256) void k_errmap(long long n, const float* Y, con
   if (threadIdx.x == 0) atomicAdd(sq, red[0]);
 }
 
+// one pair per thread; the BF16 network runs on MFMA (the FP32 network, quant
+// = 0, per lane from global memory)
 __global__ __launch_bounds__(256) void k_predict(long long n, const float* X, SurDev m, float* U) {
-  __shared__ float w[kSurParams];
-  const float* src = m.quant ? m.q : m.p;
-  for (int i = threadIdx.x; i < kSurParams; i += blockDim.x) w[i] = src[i];
+  __shared__ SurLds L;
+  if (m.quant) load_sur_lds(L.m, m.img);
   __syncthreads();
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    float y[kSurOut];
-    mlp_eval(w, X + 4 * i, y, nullptr, nullptr, 0, m.quant != 0);
-    for (int o = 0; o < kSurOut; ++o) U[3 * i + o] = y[o] * m.yscale[o];
-  }
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ic = i < n ? i : n - 1;
+  float y[kSurOut];
+  if (m.quant) mlp_block_mfma(L, X + 4 * ic, y);
+  else mlp_eval(m.p, X + 4 * ic, y, nullptr, nullptr, 0, false);
+  if (i >= n) return;
+  for (int o = 0; o < kSurOut; ++o) U[3 * i + o] = y[o] * m.yscale[o];
 }
 
 // ---- assembly -----------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_assemble_sur(GeomDev g, FreqDev f, double kh, const uint32_t* __restrict__ pairs,
-                                                      long long np, SurDev m, float tol, double* A, double* S,
-                                                      long long ld, uint32_t* fb_e, uint32_t* fb_g,
-                                                      unsigned long long* fb_count) {
-  __shared__ float w[kSurParams];
-  __shared__ double wvn[kNk];
-  const float* src = m.quant ? m.q : m.p;
-  for (int i = threadIdx.x; i < kSurParams; i += blockDim.x) w[i] = src[i];
-  for (int i = threadIdx.x; i < kNk; i += blockDim.x) wvn[i] = f.wvn[i];
-  __syncthreads();
+// One pair of the EIGENE segment per thread. A pair inside the trained range
+// whose error-map cell bound is <= tol takes the network: done[t] = 1 and
+// vreg[3 t ..] = its regular part V (value, R- and z-derivative) from the
+// network; every other pair gets done[t] = 0. blk[b] = pairs taken by workgroup
+// b. The MFMA network needs the whole workgroup, so a workgroup runs it when
+// any of its pairs takes it (the segment is in key order, i.e. by R/h, so whole
+// workgroups mostly agree).
+template <bool QUANT>
+__global__ __launch_bounds__(256) void k_sur_regular(GeomDev g, FreqDev f, double kh, const uint32_t* __restrict__ pairs,
+                                                     long long np, SurDev m, float tol, uint8_t* __restrict__ done,
+                                                     float* __restrict__ vreg, unsigned* __restrict__ blk,
+                                                     unsigned long long* count) {
+  __shared__ SurLds L;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < np;
-  const uint32_t p = valid ? pairs[t] : 0u;
+  const uint32_t p = valid ? pairs[t] : pairs[np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
   const double* xp = g.fp + 3 * (long long)i;
   const double* xq = g.xyz_p + 3 * (long long)j;
@@ -329,60 +472,107 @@ __global__ __launch_bounds__(256) void k_assemble_sur(GeomDev g, FreqDev f, doub
   const double zf = xq[2], zp = xp[2], rh = rr / h;
   float x[4];
   sur_inputs(rh > kSurRhMin ? rh : kSurRhMin, (zf + zp) / h, (zf - zp) / h, kh, x);
-  // R/h >= 0.5 (EIGEN, at most 9 terms) is cheaper as plain series than as a network evaluation
   const bool inside = rh >= kSurRhMin && rh < 0.5 && zf >= -kSurDepth * h && zp >= -kSurDepth * h;
-  const bool net = inside && m.map[sur_cell(x)] <= tol;
-  const bool plain_far = rh >= 0.5;
-  // pairs handed to the plain kernels: wave-aggregated append per regime
-  const bool to_e = valid && !net && !plain_far && rh < 0.5, to_g = valid && !net && !plain_far && rh >= 0.5;
-  const unsigned long long be = __ballot(to_e), bg = __ballot(to_g), bf = __ballot(valid && plain_far);
-  const int lane = threadIdx.x & 63;
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  unsigned long long base_e = 0, base_g = 0;
-  if (lane == 0) {
-    if (be) base_e = atomicAdd(&fb_count[0], (unsigned long long)__popcll(be));
-    if (bg) base_g = atomicAdd(&fb_count[1], (unsigned long long)__popcll(bg));
-    if (bf) atomicAdd(&fb_count[2], (unsigned long long)__popcll(bf));   // plain series in this kernel
+  const bool net = valid && inside && m.map[sur_cell(x)] <= tol;
+  if (valid) done[t] = net ? 1 : 0;
+  const int nb = __syncthreads_count(net);   // uniform over the workgroup
+  if (threadIdx.x == 0) {
+    blk[blockIdx.x] = (unsigned)nb;
+    if (nb) atomicAdd(count, (unsigned long long)nb);
   }
-  base_e = __shfl(base_e, 0);
-  base_g = __shfl(base_g, 0);
-  if (to_e) fb_e[base_e + __popcll(be & lt)] = p;
-  if (to_g) fb_g[base_g + __popcll(bg & lt)] = p;
-  if (!valid || !(net || plain_far)) return;
-  FinOut fo{0, 0, 0, 0, 0, 0};
-  if (plain_far) {
-    fin_eigen(rr, zf, zp, wvn, h, fo);
+  float y[kSurOut];
+  if (QUANT) {
+    if (nb == 0) return;
+    load_sur_lds(L.m, m.img);
+    __syncthreads();
+    mlp_block_mfma(L, x, y);
+    if (!net) return;
   } else {
-    float y[kSurOut];
-    mlp_eval(w, x, y, nullptr, nullptr, 0, m.quant != 0);
-    // U = Y R^p e^{k1 R} is dimensionless for any depth (R, k1 dimensional)
-    const double u[3] = {(double)(y[0] * m.yscale[0]), (double)(y[1] * m.yscale[1]), (double)(y[2] * m.yscale[2])};
-    double v[3];
-    sur_regular_from(rr, zf, zp, wvn[1], u, v);
-    fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
-    fo.g_re += v[0];
-    fo.gr_re += v[1];
-    fo.gz_re += v[2];
+    if (!net) return;
+    mlp_eval(m.p, x, y, nullptr, nullptr, 0, false);
   }
-  if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
-  const int flag = near_flag(xp, xq, g.pnsz[j]);
-  Grn4 gr;
-  fin_to_grn(fo, xq[0] - xp[0], xq[1] - xp[1], zf, zp, rr, true, flag, gr);
-  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
+  // U = Y R^p e^{k1 R} is dimensionless for any depth (R, k1 dimensional)
+  const double u[3] = {(double)(y[0] * m.yscale[0]), (double)(y[1] * m.yscale[1]), (double)(y[2] * m.yscale[2])};
+  double v[3];
+  sur_regular_from(rr, zf, zp, f.wvn[1], u, v);
+  vreg[3 * t] = (float)v[0];
+  vreg[3 * t + 1] = (float)v[1];
+  vreg[3 * t + 2] = (float)v[2];
 }
 
+// exclusive prefix sums of the per-workgroup counts (one workgroup of 1024
+// threads, each over a contiguous run); off[nblk] = total
+__global__ __launch_bounds__(1024) void k_scan_counts(const unsigned* __restrict__ cnt, long long nblk,
+                                                      unsigned long long* __restrict__ off) {
+  __shared__ unsigned long long part[1024];
+  const long long per = (nblk + 1023) / 1024;
+  const long long b0 = threadIdx.x * per, b1 = b0 + per < nblk ? b0 + per : nblk;
+  unsigned long long sum = 0;
+  for (long long b = b0; b < b1; ++b) sum += cnt[b];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {         // inclusive scan of the 1024 run sums
+    const unsigned long long v = threadIdx.x >= d ? part[threadIdx.x - d] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned long long run = part[threadIdx.x] - sum;
+  for (long long b = b0; b < b1; ++b) {
+    off[b] = run;
+    run += cnt[b];
+  }
+  if (threadIdx.x == 1023) off[nblk] = part[1023];
+}
+
+// Stable partition of the segment by done: the series pairs first, then the
+// network pairs, each in segment (key) order, so that every wave of the masked
+// series kernel is (up to the one at the boundary) either all series or all
+// given. Writes out (pair codes), mask and the regular parts of the given pairs
+// in the partitioned order. Same workgroup decomposition as k_sur_regular.
+__global__ __launch_bounds__(256) void k_partition_done(const uint32_t* __restrict__ pairs, long long np,
+                                                        const uint8_t* __restrict__ done, const float* __restrict__ vreg,
+                                                        const unsigned long long* __restrict__ off,
+                                                        long long nblk, uint32_t* __restrict__ out,
+                                                        uint8_t* __restrict__ mask, float* __restrict__ vout) {
+  __shared__ unsigned wsum[4];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < np;
+  const bool d = valid && done[t];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned long long bd = __ballot(d);
+  if (lane == 0) wsum[w] = (unsigned)__popcll(bd);
+  __syncthreads();
+  unsigned before = (unsigned)__popcll(bd & ((1ull << lane) - 1ull));
+  for (int k = 0; k < w; ++k) before += wsum[k];
+  if (!valid) return;
+  const unsigned long long ndone = off[nblk];
+  const unsigned long long pd = off[blockIdx.x] + before;     // given pairs before t
+  const unsigned long long dst = d ? (unsigned long long)np - ndone + pd : (unsigned long long)t - pd;
+  out[dst] = pairs[t];
+  mask[dst] = d ? 1 : 0;
+  if (d) {
+    vout[3 * dst] = vreg[3 * t];
+    vout[3 * dst + 1] = vreg[3 * t + 1];
+    vout[3 * dst + 2] = vreg[3 * t + 2];
+  }
+}
 
 // network and plain evanescent sums (unscaled, h = 1) at given points, with the
 // error-map bound of each point's cell
 __global__ __launch_bounds__(256) void k_eval_points(int n, const double* rh, const double* zs, const double* zd,
                                                      const double* kh, const double* wvn, SurDev m, double* tn,
                                                      double* tp, double* bound) {
-  __shared__ float w[kSurParams];
-  const float* src = m.quant ? m.q : m.p;
-  for (int i = threadIdx.x; i < kSurParams; i += blockDim.x) w[i] = src[i];
+  __shared__ SurLds L;
+  if (m.quant) load_sur_lds(L.m, m.img);
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = i0 < n ? i0 : n - 1;
+  float x[4], y[kSurOut];
+  sur_inputs(rh[i], zs[i], zd[i], kh[i], x);
+  if (m.quant) mlp_block_mfma(L, x, y);
+  else mlp_eval(m.p, x, y, nullptr, nullptr, 0, false);
+  if (i0 >= n) return;
   const double* wv = wvn + (long long)kNk * i;
   const double zf = 0.5 * (zs[i] + zd[i]), zp = 0.5 * (zs[i] - zd[i]);
   double tg, tr, tz, u[3], v[3];
@@ -391,9 +581,6 @@ __global__ __launch_bounds__(256) void k_eval_points(int n, const double* rh, co
   sur_target(rh[i], zf, zp, wv[1], tg, tr, tz, u);
   sur_regular_from(rh[i], zf, zp, wv[1], u, v);
   for (int o = 0; o < kSurOut; ++o) tp[3 * i + o] = v[o];
-  float x[4], y[kSurOut];
-  sur_inputs(rh[i], zs[i], zd[i], kh[i], x);
-  mlp_eval(w, x, y, nullptr, nullptr, 0, m.quant != 0);
   for (int o = 0; o < kSurOut; ++o) u[o] = (double)(y[o] * m.yscale[o]);
   sur_regular_from(rh[i], zf, zp, wv[1], u, v);
   for (int o = 0; o < kSurOut; ++o) tn[3 * i + o] = v[o];
@@ -409,8 +596,9 @@ hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, cons
                      wvn, X, Y, M, cell);
   return hipGetLastError();
 }
-hipError_t sur_round(hipStream_t s, const float* p, float* q) {
+hipError_t sur_round(hipStream_t s, const float* p, float* q, void* img) {
   hipLaunchKernelGGL(k_round, dim3((kSurParams + 255) / 256), dim3(256), 0, s, p, q);
+  if (img) hipLaunchKernelGGL(k_build_img, dim3(1), dim3(256), 0, s, q, static_cast<SurImg*>(img));
   return hipGetLastError();
 }
 hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh, bool quant) {
@@ -434,22 +622,47 @@ hipError_t sur_error_map(hipStream_t s, long long n, const float* Y, const float
   return hipGetLastError();
 }
 hipError_t sur_predict(hipStream_t s, long long n, const float* X, SurDev m, float* U) {
-  const long long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_predict, dim3((unsigned)(blocks < 16384 ? blocks : 16384)), dim3(256), 0, s, n, X, m, U);
+  if (n <= 0) return hipSuccess;
+  if (m.quant && !m.img) return hipErrorInvalidValue;   // the MFMA kernels copy the image into LDS
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, X, m, U);
   return hipGetLastError();
 }
-hipError_t assemble_surrogate(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
-                              long long np, SurDev m, float tol, double* A, double* S, long long ld,
-                              uint32_t* fb_eigene, uint32_t* fb_eigen, unsigned long long* fb_count) {
+size_t sur_regular_bytes(long long np) {
+  const long long nblk = (np + 255) / 256;
+  return 30 * (size_t)np + 4 * (size_t)nblk + 8 * (size_t)(nblk + 1) + 64;
+}
+
+hipError_t sur_regular_parts(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
+                             long long np, SurDev m, float tol, void* work, unsigned long long* count,
+                             SurPartition& part) {
+  part = SurPartition{pairs, nullptr, nullptr};
   if (np <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble_sur, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, f, kh, pairs, np, m, tol,
-                     A, S, ld, fb_eigene, fb_eigen, fb_count);
+  if (m.quant && !m.img) return hipErrorInvalidValue;   // the MFMA kernels copy the image into LDS
+  const long long nblk = (np + 255) / 256;
+  // work: vout (12 np) | vreg (12 np) | out (4 np) | done (np) | mask (np) | off (8 (nblk + 1)) | cnt (4 nblk)
+  char* w = static_cast<char*>(work);
+  float* vout = reinterpret_cast<float*>(w);
+  float* vreg = vout + 3 * np;
+  uint32_t* out = reinterpret_cast<uint32_t*>(vreg + 3 * np);
+  uint8_t* done = reinterpret_cast<uint8_t*>(out + np);
+  uint8_t* mask = done + np;
+  unsigned long long* off = reinterpret_cast<unsigned long long*>(
+      (reinterpret_cast<uintptr_t>(mask + np) + 7) & ~static_cast<uintptr_t>(7));
+  unsigned* cnt = reinterpret_cast<unsigned*>(off + nblk + 1);
+  const dim3 gd((unsigned)nblk), bd(256);
+  if (m.quant) hipLaunchKernelGGL(k_sur_regular<true>, gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  else hipLaunchKernelGGL(k_sur_regular<false>, gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, nblk, off);
+  hipLaunchKernelGGL(k_partition_done, gd, bd, 0, s, pairs, np, done, vreg, off, nblk, out, mask, vout);
+  part = SurPartition{out, mask, vout};
   return hipGetLastError();
 }
 
 hipError_t sur_eval_points(hipStream_t s, int n, const double* rh, const double* zs, const double* zd, const double* kh,
                            const double* wvn, double g_disp, SurDev m, double* tn, double* tp, double* bound) {
   (void)g_disp;
+  if (n <= 0) return hipSuccess;
+  if (m.quant && !m.img) return hipErrorInvalidValue;   // the MFMA kernels copy the image into LDS
   double* d = nullptr;
   const size_t per = 4 + kNk + 3 + 3 + 1;
   hipError_t e = hipMalloc(&d, sizeof(double) * per * (size_t)n);

@@ -54,17 +54,26 @@ constexpr int kSurParams = kSurOffB4 + kSurOut;              // 12995
 constexpr int kMapR = 24, kMapS = 8, kMapD = 8, kMapK = 24;
 constexpr int kSurMap = kMapR * kMapS * kMapD * kMapK;       // 36864 cells
 constexpr int kSurBlob = kSurParams + kSurOut + kSurMap;     // params, output scales, error map
-constexpr double kSurRhMin = 0.1, kSurRhMax = 2.0, kSurKhMin = 0.01, kSurKhMax = 50.0;
+constexpr double kSurRhMin = 0.05, kSurRhMax = 0.5, kSurKhMin = 0.01, kSurKhMax = 50.0;
 constexpr double kSurDepth = 0.25;   // z/h and zeta/h in [-0.25, 0]: the draft range of surface-piercing hulls
 
+// LDS image of the BF16 network for the MFMA inference kernels (hidden weights
+// as BF16 bits, the FP32 first and output layers and the biases), built on the
+// device whenever the parameters change, copied into LDS by each workgroup
+constexpr size_t kSurImgBytes = 27408;
+// byte offset of the image in the surrogate buffer (after p, q, yscale, map)
+constexpr size_t kSurImgOffset = (((size_t)(2 * kSurParams + kSurOut + kSurMap) * 4) + 255) & ~(size_t)255;
+constexpr size_t kSurBufBytes = kSurImgOffset + kSurImgBytes;
+
 // a trained model on the device: FP32 parameters, the BF16-rounded copy the
-// forward pass uses (as floats), output scales and the error map
+// forward pass uses (as floats), output scales, the error map and the image
 struct SurDev {
   const float* p;     // kSurParams
   const float* q;     // kSurParams, BF16-rounded
   const float* yscale;
   const float* map;
   int quant;          // 1: BF16 weights and activations; 0: FP32 network
+  const void* img;    // kSurImgBytes
 };
 
 struct SurTrainStats {
@@ -76,7 +85,8 @@ struct SurTrainStats {
 // training and validation data from the plain kernels (h = 1)
 hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, const double* kh, const double* wvn,
                         float* X, float* Y, float* M, int* cell);
-hipError_t sur_round(hipStream_t s, const float* p, float* q);
+// q = BF16-rounded p; img (may be null) = the LDS image of q
+hipError_t sur_round(hipStream_t s, const float* p, float* q, void* img = nullptr);
 hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh,
                        bool quant = true);
 hipError_t sur_loss_grad(hipStream_t s, int B, const float* X, const float* Y, const float* yscale, const float* q,
@@ -92,13 +102,22 @@ hipError_t sur_predict(hipStream_t s, long long n, const float* X, SurDev m, flo
 hipError_t sur_eval_points(hipStream_t s, int n, const double* rh, const double* zs, const double* zd, const double* kh,
                            const double* wvn, double g_disp, SurDev m, double* tn, double* tp, double* bound);
 
-// assembly of the pairs of the EIGENE + EIGEN list segments with the network;
-// pairs whose cell bound exceeds tol (or outside the trained R/h range) are
-// appended to fallback lists (regime 2 -> fb[0..], regime 3 -> fb2) for the plain kernels;
-// fb_count[0], [1]: lengths of those lists, [2]: pairs with R/h >= 0.5 evaluated
-// here by the plain EIGEN series
-hipError_t assemble_surrogate(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
-                              long long np, SurDev m, float tol, double* A, double* S, long long ld,
-                              uint32_t* fb_eigene, uint32_t* fb_eigen, unsigned long long* fb_count);
+// the network over the pairs of the EIGENE list segment: a pair inside the
+// trained range whose cell bound is <= tol is taken by the network, which gives
+// its regular part V = T - (1/r + 1/r1) (value, R- and z-derivative; T the
+// evanescent sum). The segment comes back partitioned (stable, by key order
+// within each part): part.pairs = the pairs left to the series, then those
+// taken; part.mask[t] = 1 and part.vreg[3 t .. 3 t + 2] = V for a taken pair.
+// The masked EIGENE assembly (assemble_regime with mask and vreg) writes A and
+// S for all of them. count += pairs taken. work: sur_regular_bytes(np) bytes.
+struct SurPartition {
+  const uint32_t* pairs;
+  const uint8_t* mask;
+  const float* vreg;
+};
+size_t sur_regular_bytes(long long np);
+hipError_t sur_regular_parts(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
+                             long long np, SurDev m, float tol, void* work, unsigned long long* count,
+                             SurPartition& part);
 
 }  // namespace hm

@@ -35,7 +35,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_surrogate_grad", "hm_set_hmatrix", "hm_get_hmatrix_stats",
            "hm_hmatrix_apply", "hm_set_frequency_limit", "hm_batch_create", "hm_batch_destroy",
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
-           "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve")
+           "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
+           "hm_surrogate_predict")
 
 _lib = None
 
@@ -298,6 +299,13 @@ class Context:
                                         _i(nrhs), _i(len(bs)), _p(bs) if len(bs) else None, ctypes.byref(info)),
                   "hm_ziter_solve")
         return x, info.value
+
+    def surrogate_predict(self, x):
+        """scaled network outputs u (n x 3) for normalised inputs x (n x 4)"""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        u = np.zeros((x.shape[0], 3), dtype=np.float32)
+        self._chk(self.L.hm_surrogate_predict(self.ptr, _i(x.shape[0]), _p(x), _p(u)), "hm_surrogate_predict")
+        return u
 
     def set_series_variant(self, variant):
         self._chk(self.L.hm_set_series_variant(self.ptr, _i(variant)), "hm_set_series_variant")

@@ -203,6 +203,9 @@ int hm_surrogate_counts(hm_ctx *ctx, long long *counts);
 int hm_surrogate_eval(hm_ctx *ctx, int n, const double *rh, const double *zs, const double *zd, const double *kh,
                       double *t_net, double *t_plain, double *bound);
 int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, int quant, float *grad, double *loss);
+/* network inference (the BF16 network on MFMA, as the assembly evaluates it)
+ * for n normalised inputs x (4n) -> scaled outputs u (3n) */
+int hm_surrogate_predict(hm_ctx *ctx, int n, const float *x, float *u);
 
 /* ---- batches of independent problems ------------------------------------------
  * The reference solves one problem at a time, calling the per-frequency chain

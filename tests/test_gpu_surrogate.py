@@ -46,6 +46,29 @@ def test_gradients_match_cpu_reference(trained):
     g.close()
 
 
+def test_mfma_inference_matches_cpu_reference(trained):
+    """The inference kernels run the BF16 network as MFMA tiles with FP32
+    accumulation; the reference is the float64 network with the same BF16
+    rounding of weights and activations. Accumulation order and the odd
+    activation falling on the other side of a BF16 rounding boundary make them
+    differ at ~1e-3 of the output scale; a wrong tile layout would make them
+    differ at O(1)."""
+    c, _, _, _ = trained
+    blob = c.surrogate_get()
+    p = blob[:M.N_PARAMS].astype(np.float64)
+    ys = blob[M.N_PARAMS:M.N_PARAMS + 3].astype(np.float64)
+    rng = np.random.default_rng(17)
+    x = rng.uniform(-1, 1, (1000, 4)).astype(np.float32)       # not a multiple of the 64-pair wave tile
+    u = c.surrogate_predict(x).astype(np.float64)
+    y_ref, _, _ = M.forward(p, x.astype(np.float64), quant=True)
+    u_ref = y_ref * ys
+    scale = np.max(np.abs(u_ref), axis=0)
+    err = np.abs(u - u_ref) / scale
+    print("MFMA inference vs reference: max", err.max(axis=0), "median", np.median(err, axis=0), flush=True)
+    assert np.all(err.max(axis=0) < 1e-2), err.max(axis=0)
+    assert np.all(np.median(err, axis=0) < 1e-3), np.median(err, axis=0)
+
+
 def test_training_reduces_error(trained):
     _, st0, _, st = trained
     assert st["val_rmse"] < 0.1 * st0["val_rmse"], (st0, st)
@@ -57,7 +80,7 @@ def test_network_against_plain_series(trained):
     c, _, _, _ = trained
     rng = np.random.default_rng(5)
     n = 4000
-    rh = np.exp(rng.uniform(np.log(0.1), np.log(2.0), n))
+    rh = np.exp(rng.uniform(np.log(0.05), np.log(0.5), n))
     zf, zp = -rng.uniform(0, 0.25, n), -rng.uniform(0, 0.25, n)
     kh = np.exp(rng.uniform(np.log(0.01), np.log(50.0), n))
     tn, tp, bound = c.surrogate_eval(rh, zf + zp, zf - zp, kh)
