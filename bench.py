@@ -276,10 +276,10 @@ def main():
     ap.add_argument("--aca-eps", type=float, default=1e-5, help="cross-approximation tolerance of A")
     ap.add_argument("--aca-eps-s", type=float, default=1e-6, help="cross-approximation tolerance of S")
     ap.add_argument("--aca-kmax", type=int, default=50, help="largest rank of a cross-approximated block")
-    ap.add_argument("--surrogate-tol", type=float, default=2e-3,
+    ap.add_argument("--surrogate-tol", type=float, default=1.5e-3,
                     help="cell error bound up to which the learned surrogate replaces the plain series (0: off)")
-    ap.add_argument("--surrogate-samples", type=int, default=8_000_000)
-    ap.add_argument("--surrogate-epochs", type=int, default=24)
+    ap.add_argument("--surrogate-samples", type=int, default=16_000_000)
+    ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--gmres-tol", type=float, default=1e-6)
     ap.add_argument("--series-variant", type=int, default=1, choices=(0, 1, 2),
                     help="epsilon-accelerated regimes: 1 lane groups of 8/16 (default), 2 of 4/8, 0 one pair per lane")

@@ -1149,7 +1149,7 @@ void kh_table(int nkh, uint64_t seed, double g_disp, std::vector<double>& kh, st
 }
 
 struct TrainBufs {
-  DevBuf kh, wvn, X, Y, M, cell, Z, A, D, Yh, grad, m, v, loss, U, bits;
+  DevBuf kh, wvn, X, Y, M, cell, Z, A, D, Yh, grad, m, v, loss, U, bits, work;
 };
 
 }  // namespace
@@ -1168,7 +1168,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   TrainBufs tb;
   auto release = [&]() {
     DevBuf* all[] = {&tb.kh, &tb.wvn, &tb.X, &tb.Y, &tb.M, &tb.cell, &tb.Z, &tb.A, &tb.D, &tb.Yh, &tb.grad, &tb.m,
-                     &tb.v, &tb.loss, &tb.U, &tb.bits};
+                     &tb.v, &tb.loss, &tb.U, &tb.bits, &tb.work};
     for (DevBuf* b : all) b->release();
   };
   struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{release};
@@ -1219,6 +1219,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   HIPCHK(c, tb.Z.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "surrogate alloc");
   HIPCHK(c, tb.A.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "surrogate alloc");
   HIPCHK(c, tb.D.ensure(sizeof(float) * (4 * hm::kSurH + 3) * (size_t)B), "surrogate alloc");
+  HIPCHK(c, tb.work.ensure(hm::sur_grad_work_bytes(B)), "surrogate alloc");
   HIPCHK(c, tb.Yh.ensure(3 * sizeof(float) * (size_t)B), "surrogate alloc");
   HIPCHK(c, tb.grad.ensure(sizeof(float) * hm::kSurParams), "surrogate alloc");
   HIPCHK(c, tb.m.ensure(sizeof(float) * hm::kSurParams), "surrogate alloc");
@@ -1241,7 +1242,8 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
       HIPCHK(c, hm::sur_forward(s, B, Xb, Wt, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(), quant != 0),
              "surrogate fwd");
       HIPCHK(c, hm::sur_loss_grad(s, B, Xb, Yb, YS, Wt, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(),
-                                  tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<float>()), "surrogate bwd");
+                                  tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<double>(), tb.work.as<double>()),
+             "surrogate bwd");
       HIPCHK(c, hm::sur_adam(s, P, tb.grad.as<float>(), tb.m.as<float>(), tb.v.as<float>(), (float)lrt, (int)(t + 1)),
              "surrogate adam");
       HIPCHK(c, hm::sur_round(s, P, Q), "surrogate");
@@ -1408,7 +1410,7 @@ extern "C" int hm_surrogate_grad(hm_ctx* c, int batch, const float* x, const flo
   hipStream_t s = c->stream;
   TrainBufs tb;
   auto release = [&]() {
-    DevBuf* all[] = {&tb.X, &tb.Y, &tb.Z, &tb.A, &tb.D, &tb.Yh, &tb.grad, &tb.loss};
+    DevBuf* all[] = {&tb.X, &tb.Y, &tb.Z, &tb.A, &tb.D, &tb.Yh, &tb.grad, &tb.loss, &tb.work};
     for (DevBuf* b : all) b->release();
   };
   struct Guard { std::function<void()> f; ~Guard() { f(); } } guard{release};
@@ -1418,6 +1420,7 @@ extern "C" int hm_surrogate_grad(hm_ctx* c, int batch, const float* x, const flo
   HIPCHK(c, tb.Z.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "grad alloc");
   HIPCHK(c, tb.A.ensure(4 * sizeof(float) * hm::kSurH * (size_t)B), "grad alloc");
   HIPCHK(c, tb.D.ensure(sizeof(float) * (4 * hm::kSurH + 3) * (size_t)B), "grad alloc");
+  HIPCHK(c, tb.work.ensure(hm::sur_grad_work_bytes(B)), "grad alloc");
   HIPCHK(c, tb.Yh.ensure(3 * sizeof(float) * B), "grad alloc");
   HIPCHK(c, tb.grad.ensure(sizeof(float) * hm::kSurParams), "grad alloc");
   HIPCHK(c, tb.loss.ensure(sizeof(double)), "grad alloc");
@@ -1429,7 +1432,8 @@ extern "C" int hm_surrogate_grad(hm_ctx* c, int batch, const float* x, const flo
   HIPCHK(c, hm::sur_forward(s, B, tb.X.as<float>(), w, tb.Z.as<float>(), tb.A.as<float>(), tb.Yh.as<float>(), quant != 0),
          "grad fwd");
   HIPCHK(c, hm::sur_loss_grad(s, B, tb.X.as<float>(), tb.Y.as<float>(), m.yscale, w, tb.Z.as<float>(), tb.A.as<float>(),
-                              tb.Yh.as<float>(), tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<float>()), "grad bwd");
+                              tb.Yh.as<float>(), tb.D.as<float>(), tb.grad.as<float>(), tb.loss.as<double>(),
+                              tb.work.as<double>()), "grad bwd");
   double l = 0.0;
   HIPCHK(c, hipMemcpyAsync(grad, tb.grad.p, sizeof(float) * hm::kSurParams, hipMemcpyDeviceToHost, s), "grad");
   HIPCHK(c, hipMemcpyAsync(&l, tb.loss.p, sizeof(double), hipMemcpyDeviceToHost, s), "grad");

@@ -295,58 +295,51 @@ __global__ void k_round(const float* p, float* q) {
   q[i] = hidden_w ? bf16r(p[i]) : p[i];
 }
 
-// Z, A: [layer 0..3][feature][sample] with sample stride 1
-__global__ __launch_bounds__(256) void k_forward(int B, const float* X, const float* q, float* Z, float* A, float* Yh,
-                                                 int quant) {
-  __shared__ float w[kSurParams];
-  for (int i = threadIdx.x; i < kSurParams; i += blockDim.x) w[i] = q[i];
-  __syncthreads();
+// Training runs layer by layer with one thread per (sample, feature): the
+// feature is uniform over a workgroup (blockIdx.y), so the weight row is read
+// through scalar loads, and each sum runs over its inputs in the same order as
+// mlp_eval. Z, A: [layer 0..3][feature][sample] with sample stride 1.
+//
+// forward of layer l: 0 (inputs X [sample][4]), 1..3 (hidden), 4 (output -> Yh [sample][3])
+__global__ __launch_bounds__(256) void k_fwd_layer(int B, int l, const float* __restrict__ X, const float* __restrict__ q,
+                                                   float* __restrict__ Z, float* __restrict__ A, float* __restrict__ Yh,
+                                                   int quant) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
   if (s >= B) return;
-  float y[kSurOut];
-  mlp_eval(w, X + 4 * (long long)s, y, Z ? Z + s : nullptr, A ? A + s : nullptr, B, quant != 0);
-  for (int o = 0; o < kSurOut; ++o) Yh[3 * (long long)s + o] = y[o];
+  const long long Bl = B;
+  float z;
+  if (l == 0) {
+    z = q[kSurOffB0 + j];
+    for (int i = 0; i < kSurIn; ++i) z = fmaf(q[kSurOffW0 + j * kSurIn + i], X[4 * (long long)s + i], z);
+  } else {
+    const float* W = l < 4 ? q + kSurOffW1 + (l - 1) * kSurHiddenStride : q + kSurOffW4;
+    const float* in = A + (long long)(l - 1) * kSurH * Bl + s;
+    z = l < 4 ? W[kSurH * kSurH + j] : q[kSurOffB4 + j];
+    for (int i = 0; i < kSurH; ++i) z = fmaf(W[j * kSurH + i], in[i * Bl], z);
+  }
+  if (l == 4) {
+    Yh[3 * (long long)s + j] = z;
+    return;
+  }
+  const long long o = ((long long)l * kSurH + j) * Bl + s;
+  Z[o] = z;
+  A[o] = quant ? bf16r(z * sigm(z)) : z * sigm(z);
 }
 
-// D: d loss / d z for layers 0..3 ([layer][feature][sample]), then the output
-// deltas [sample][3]; loss accumulates sum of squared normalized errors
-__global__ __launch_bounds__(256) void k_backward(int B, const float* Y, const float* yscale, const float* q,
-                                                  const float* Z, const float* Yh, float* D, double* loss) {
-  __shared__ float w[kSurParams];
+// output deltas D[4 * 64 * B + 3 s + o] = d loss / d y; loss accumulates the
+// sum of squared normalized errors
+__global__ __launch_bounds__(256) void k_out_delta(int B, const float* __restrict__ Y, const float* __restrict__ yscale,
+                                                   const float* __restrict__ Yh, float* __restrict__ D, double* loss) {
   __shared__ double red[256];
-  for (int i = threadIdx.x; i < kSurParams; i += blockDim.x) w[i] = q[i];
-  __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0;
   if (s < B) {
     const float inv = 1.0f / (3.0f * (float)B);
-    float dy[kSurOut];
     for (int o = 0; o < kSurOut; ++o) {
       const float e = Yh[3 * (long long)s + o] - Y[3 * (long long)s + o] / yscale[o];
       sq += (double)e * e;
-      dy[o] = 2.0f * e * inv;
-      D[4 * (long long)kSurH * B + 3 * (long long)s + o] = dy[o];
-    }
-    float da[kSurH], dz[kSurH];
-    for (int j = 0; j < kSurH; ++j) {
-      float t = 0.0f;
-      for (int o = 0; o < kSurOut; ++o) t = fmaf(w[kSurOffW4 + o * kSurH + j], dy[o], t);
-      da[j] = t;
-    }
-    for (int l = 3; l >= 0; --l) {
-      for (int j = 0; j < kSurH; ++j) {
-        const float z = Z[((long long)l * kSurH + j) * B + s];
-        const float sg = sigm(z);
-        dz[j] = da[j] * (sg + z * sg * (1.0f - sg));
-        D[((long long)l * kSurH + j) * B + s] = dz[j];
-      }
-      if (l == 0) break;
-      const float* W = w + kSurOffW1 + (l - 1) * kSurHiddenStride;
-      for (int j = 0; j < kSurH; ++j) {
-        float t = 0.0f;
-        for (int i = 0; i < kSurH; ++i) t = fmaf(W[i * kSurH + j], dz[i], t);
-        da[j] = t;
-      }
+      D[4 * (long long)kSurH * B + 3 * (long long)s + o] = 2.0f * e * inv;
     }
   }
   red[threadIdx.x] = sq;
@@ -358,10 +351,79 @@ __global__ __launch_bounds__(256) void k_backward(int B, const float* Y, const f
   if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
-// one thread per parameter: sum over the batch of delta x input
-__global__ __launch_bounds__(256) void k_wgrad(int B, const float* X, const float* A, const float* D, float* grad) {
+// deltas of layer l (3..0): D[l][j][s] = (sum_i Wnext[i][j] Dnext[i][s]) silu'(Z[l][j][s])
+__global__ __launch_bounds__(256) void k_bwd_layer(int B, int l, const float* __restrict__ q, const float* __restrict__ Z,
+                                                   float* __restrict__ D) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (s >= B) return;
+  const long long Bl = B;
+  float t = 0.0f;
+  if (l == 3) {
+    const float* dy = D + 4 * (long long)kSurH * Bl + 3 * (long long)s;
+    for (int o = 0; o < kSurOut; ++o) t = fmaf(q[kSurOffW4 + o * kSurH + j], dy[o], t);
+  } else {
+    const float* W = q + kSurOffW1 + l * kSurHiddenStride;
+    const float* dn = D + (long long)(l + 1) * kSurH * Bl + s;
+    for (int i = 0; i < kSurH; ++i) t = fmaf(W[i * kSurH + j], dn[i * Bl], t);
+  }
+  const long long o = ((long long)l * kSurH + j) * Bl + s;
+  const float z = Z[o];
+  const float sg = sigm(z);
+  D[o] = t * (sg + z * sg * (1.0f - sg));
+}
+
+// weight gradients in two passes: part[c][pi] = sum over samples of chunk c
+// (kSurChunk samples) of delta x input, in FP64; then grad[pi] = sum over
+// chunks. The hidden weight matrices (12288 of the parameters) are 64 x 64 x
+// chunk products, tiled through LDS (k_wgrad_hidden); k_wgrad takes the rest.
+constexpr int kSurChunk = 64;
+__device__ __forceinline__ bool hidden_weight(int pi) {
+  return pi >= kSurOffW1 && pi < kSurOffW4 && (pi - kSurOffW1) % kSurHiddenStride < kSurH * kSurH;
+}
+
+// dW_l[i][j] over one chunk = sum_s D[l + 1][i][s] A[l][j][s]; grid (chunk, l).
+// Operands transposed into LDS ([s][feature], row stride 68 floats) so that a
+// thread's 4 consecutive features of one sample are one 16-byte read; each
+// thread accumulates a 4 x 4 block in FP64.
+__global__ __launch_bounds__(256) void k_wgrad_hidden(int B, const float* __restrict__ A, const float* __restrict__ D,
+                                                      double* __restrict__ part) {
+  constexpr int kLd = kSurH + 4;
+  __shared__ alignas(16) float dt[kSurChunk][kLd];
+  __shared__ alignas(16) float at[kSurChunk][kLd];
+  const int c = blockIdx.x, l = blockIdx.y;
+  const int s0 = c * kSurChunk, ns = B - s0 < kSurChunk ? B - s0 : kSurChunk;
+  const long long Bl = B;
+  const float* Dl = D + (long long)(l + 1) * kSurH * Bl + s0;
+  const float* Al = A + (long long)l * kSurH * Bl + s0;
+  for (int e = threadIdx.x; e < kSurH * kSurChunk; e += blockDim.x) {
+    const int f = e / kSurChunk, sm = e % kSurChunk;
+    dt[sm][f] = sm < ns ? Dl[f * Bl + sm] : 0.0f;
+    at[sm][f] = sm < ns ? Al[f * Bl + sm] : 0.0f;
+  }
+  __syncthreads();
+  const int i0 = 4 * (threadIdx.x >> 4), j0 = 4 * (threadIdx.x & 15);
+  double acc[4][4] = {};
+  for (int sm = 0; sm < ns; ++sm) {
+    const float4 dv = *reinterpret_cast<const float4*>(&dt[sm][i0]);
+    const float4 av = *reinterpret_cast<const float4*>(&at[sm][j0]);
+    const double d4[4] = {dv.x, dv.y, dv.z, dv.w}, a4[4] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = fma(d4[a], a4[b], acc[a][b]);
+  }
+  double* out = part + (long long)c * kSurParams + kSurOffW1 + l * kSurHiddenStride;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) out[(i0 + a) * kSurH + j0 + b] = acc[a][b];
+}
+
+__global__ __launch_bounds__(256) void k_wgrad(int B, const float* X, const float* A, const float* D, double* part) {
   const int pi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pi >= kSurParams) return;
+  const int c = blockIdx.y;
+  if (pi >= kSurParams || hidden_weight(pi)) return;
   const float* dv;      // deltas of the parameter's output feature, sample stride ds
   const float* iv;      // inputs of the parameter's input feature (nullptr for a bias), stride is_
   long long ds = 1, is_ = 1;
@@ -393,8 +455,34 @@ __global__ __launch_bounds__(256) void k_wgrad(int B, const float* X, const floa
     ds = 3;
     iv = nullptr;
   }
+  const int s0 = c * kSurChunk, s1 = s0 + kSurChunk < B ? s0 + kSurChunk : B;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int sm = s0;
+  if (iv) {
+    for (; sm + 4 <= s1; sm += 4) {
+      a0 += (double)dv[sm * ds] * (double)iv[sm * is_];
+      a1 += (double)dv[(sm + 1) * ds] * (double)iv[(sm + 1) * is_];
+      a2 += (double)dv[(sm + 2) * ds] * (double)iv[(sm + 2) * is_];
+      a3 += (double)dv[(sm + 3) * ds] * (double)iv[(sm + 3) * is_];
+    }
+    for (; sm < s1; ++sm) a0 += (double)dv[sm * ds] * (double)iv[sm * is_];
+  } else {
+    for (; sm + 4 <= s1; sm += 4) {
+      a0 += (double)dv[sm * ds];
+      a1 += (double)dv[(sm + 1) * ds];
+      a2 += (double)dv[(sm + 2) * ds];
+      a3 += (double)dv[(sm + 3) * ds];
+    }
+    for (; sm < s1; ++sm) a0 += (double)dv[sm * ds];
+  }
+  part[(long long)c * kSurParams + pi] = (a0 + a1) + (a2 + a3);
+}
+
+__global__ __launch_bounds__(256) void k_wgrad_sum(int nchunk, const double* part, float* grad) {
+  const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pi >= kSurParams) return;
   double acc = 0.0;
-  for (int s = 0; s < B; ++s) acc += (double)dv[s * ds] * (iv ? (double)iv[s * is_] : 1.0);
+  for (int c = 0; c < nchunk; ++c) acc += part[(long long)c * kSurParams + pi];
   grad[pi] = (float)acc;
 }
 
@@ -602,13 +690,25 @@ hipError_t sur_round(hipStream_t s, const float* p, float* q, void* img) {
   return hipGetLastError();
 }
 hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh, bool quant) {
-  hipLaunchKernelGGL(k_forward, dim3((B + 255) / 256), dim3(256), 0, s, B, X, q, Z, A, Yh, quant ? 1 : 0);
+  if (B <= 0) return hipSuccess;
+  const unsigned gx = (unsigned)((B + 255) / 256);
+  for (int l = 0; l < 5; ++l)
+    hipLaunchKernelGGL(k_fwd_layer, dim3(gx, l < 4 ? kSurH : kSurOut), dim3(256), 0, s, B, l, X, q, Z, A, Yh,
+                       quant ? 1 : 0);
   return hipGetLastError();
 }
+size_t sur_grad_work_bytes(int B) { return sizeof(double) * (size_t)kSurParams * (size_t)((B + kSurChunk - 1) / kSurChunk); }
 hipError_t sur_loss_grad(hipStream_t s, int B, const float* X, const float* Y, const float* yscale, const float* q,
-                         const float* Z, const float* A, const float* Yh, float* D, float* grad, float* loss) {
-  hipLaunchKernelGGL(k_backward, dim3((B + 255) / 256), dim3(256), 0, s, B, Y, yscale, q, Z, Yh, D, (double*)loss);
-  hipLaunchKernelGGL(k_wgrad, dim3((kSurParams + 255) / 256), dim3(256), 0, s, B, X, A, D, grad);
+                         const float* Z, const float* A, const float* Yh, float* D, float* grad, double* loss,
+                         double* work) {
+  if (B <= 0) return hipSuccess;
+  const unsigned gx = (unsigned)((B + 255) / 256);
+  const int nchunk = (B + kSurChunk - 1) / kSurChunk;
+  hipLaunchKernelGGL(k_out_delta, dim3(gx), dim3(256), 0, s, B, Y, yscale, Yh, D, loss);
+  for (int l = 3; l >= 0; --l) hipLaunchKernelGGL(k_bwd_layer, dim3(gx, kSurH), dim3(256), 0, s, B, l, q, Z, D);
+  hipLaunchKernelGGL(k_wgrad_hidden, dim3(nchunk, 3), dim3(256), 0, s, B, A, D, work);
+  hipLaunchKernelGGL(k_wgrad, dim3((kSurParams + 255) / 256, nchunk), dim3(256), 0, s, B, X, A, D, work);
+  hipLaunchKernelGGL(k_wgrad_sum, dim3((kSurParams + 255) / 256), dim3(256), 0, s, nchunk, work, grad);
   return hipGetLastError();
 }
 hipError_t sur_adam(hipStream_t s, float* p, const float* grad, float* m, float* v, float lr, int t) {

@@ -89,8 +89,12 @@ hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, cons
 hipError_t sur_round(hipStream_t s, const float* p, float* q, void* img = nullptr);
 hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh,
                        bool quant = true);
+// loss (accumulated into *loss) and parameter gradient of one batch; work:
+// sur_grad_work_bytes(B) bytes of per-chunk partial sums
+size_t sur_grad_work_bytes(int B);
 hipError_t sur_loss_grad(hipStream_t s, int B, const float* X, const float* Y, const float* yscale, const float* q,
-                         const float* Z, const float* A, const float* Yh, float* D, float* grad, float* loss);
+                         const float* Z, const float* A, const float* Yh, float* D, float* grad, double* loss,
+                         double* work);
 hipError_t sur_adam(hipStream_t s, float* p, const float* grad, float* m, float* v, float lr, int t);
 hipError_t sur_error_map(hipStream_t s, long long n, const float* Y, const float* M, const int* cell,
                          const float* Yh, unsigned* map_bits, double* sq_err);

@@ -206,7 +206,7 @@ class Context:
         self._chk(self.L.hm_set_gmres(self.ptr, _i(restart), _d(tol), _i(maxiter), _i(precond)), "hm_set_gmres")
 
     # ---- learned surrogate ------------------------------------------------------
-    def surrogate_train(self, nsamples=2_000_000, epochs=20, batch=8192, lr=2e-3, seed=20261204, quant=True):
+    def surrogate_train(self, nsamples=16_000_000, epochs=60, batch=4096, lr=2e-3, seed=20261204, quant=True):
         st = np.zeros(8)
         self._chk(self.L.hm_surrogate_train(self.ptr, ctypes.c_longlong(nsamples), _i(epochs), _i(batch), _d(lr),
                                             ctypes.c_ulonglong(seed), _i(1 if quant else 0), _p(st)),
