@@ -81,10 +81,36 @@ __device__ void block_sum4(double* v, double (*sh)[256]) {
   __syncthreads();
 }
 
-// residual of one entry: A(b1 m + i, b2 m + j) - sum_{l<k} U(i, l) V(j, l)
+// residual of one entry: A(b1 m + i, b2 m + j) - sum_{l<k} U(i, l) V(j, l).
+// Called by every lane of the workgroup (the Green function's series run in
+// lane groups of 16); a lane with active = false gets nothing. LINTON entries
+// come out wrong here and are redone by residual_entry_direct (k_aca_fix_*).
 __device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const HmatDev& h,
-                                               const Proc& q, int m, int k, int i, int j, double& er, double& ei) {
-  entry(g, f, wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
+                                               const Proc& q, int m, int k, int i, int j, bool active, double& er,
+                                               double& ei) {
+  if (h.dense) {
+    if (!active) return;
+    entry(g, f, wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
+  } else {
+    double ar, ai, sr, si;
+    pair_entry_coop<16>(g, f, wvn, active, q.b1 * m + i, q.b2 * m + j, ar, ai, sr, si);
+    if (!active) return;
+    er = q.mat ? sr : ar;
+    ei = q.mat ? si : ai;
+  }
+  for (int l = 0; l < k; ++l) {
+    const double ur = q.U[2 * (i + (long long)l * m)], ui = q.U[2 * (i + (long long)l * m) + 1];
+    const double vr = q.V[2 * (j + (long long)l * m)], vi = q.V[2 * (j + (long long)l * m) + 1];
+    er -= ur * vr - ui * vi;
+    ei -= ur * vi + ui * vr;
+  }
+}
+
+// the same one lane at a time, every regime (pair_entry)
+__device__ __forceinline__ void residual_entry_direct(const GeomDev& g, const FreqDev& f, const HmatDev& h,
+                                                      const Proc& q, int m, int k, int i, int j, double& er,
+                                                      double& ei) {
+  entry(g, f, f.wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
   for (int l = 0; l < k; ++l) {
     const double ur = q.U[2 * (i + (long long)l * m)], ui = q.U[2 * (i + (long long)l * m) + 1];
     const double vr = q.V[2 * (j + (long long)l * m)], vi = q.V[2 * (j + (long long)l * m) + 1];
@@ -102,11 +128,13 @@ __global__ __launch_bounds__(256) void k_aca_eval_row(GeomDev g, FreqDev f, Hmat
   if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= h.m) return;
+  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = j0 < h.m;
+  const int j = active ? j0 : h.m - 1;
   const Proc q = proc(h, p);
-  double er, ei;
-  residual_entry(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, er, ei);
+  double er = 0.0, ei = 0.0;
+  residual_entry(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
+  if (!active) return;
   double* rb = h.rowbuf + 2LL * p * h.m;
   rb[2 * j] = er;
   rb[2 * j + 1] = ei;
@@ -167,12 +195,44 @@ __global__ __launch_bounds__(256) void k_aca_eval_col(GeomDev g, FreqDev f, Hmat
   if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i0 < h.m;
+  const int i = active ? i0 : h.m - 1;
+  const Proc q = proc(h, p);
+  const int k = h.rank[p];
+  double er = 0.0, ei = 0.0;
+  residual_entry(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
+  if (!active) return;
+  q.U[2 * (i + (long long)k * h.m)] = er;
+  q.U[2 * (i + (long long)k * h.m) + 1] = ei;
+}
+
+// LINTON entries of the rows / columns just evaluated (R/h < 5e-4: vertically
+// aligned panels of two bodies; rare), one lane each
+__global__ __launch_bounds__(256) void k_aca_fix_row(GeomDev g, FreqDev f, HmatDev h) {
+  const int p = blockIdx.y;
+  if (h.dense || h.conv[p]) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= h.m) return;
+  const Proc q = proc(h, p);
+  const int i = h.next_row[p];
+  if (!linton_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j)) return;
+  double er, ei;
+  residual_entry_direct(g, f, h, q, h.m, h.rank[p], i, j, er, ei);
+  double* rb = h.rowbuf + 2LL * p * h.m;
+  rb[2 * j] = er;
+  rb[2 * j + 1] = ei;
+}
+__global__ __launch_bounds__(256) void k_aca_fix_col(GeomDev g, FreqDev f, HmatDev h) {
+  const int p = blockIdx.y;
+  if (h.dense || h.conv[p] || h.skip[p]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= h.m) return;
   const Proc q = proc(h, p);
-  const int k = h.rank[p];
+  const int k = h.rank[p], j = h.pivot[p];
+  if (!linton_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j)) return;
   double er, ei;
-  residual_entry(g, f, wvn, h, q, h.m, k, i, h.pivot[p], er, ei);
+  residual_entry_direct(g, f, h, q, h.m, k, i, j, er, ei);
   q.U[2 * (i + (long long)k * h.m)] = er;
   q.U[2 * (i + (long long)k * h.m) + 1] = ei;
 }
@@ -354,8 +414,10 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   for (; st < max_steps && open > 0; ++st) {
     const dim3 eval((h.m + 255) / 256, P);
     hipLaunchKernelGGL(k_aca_eval_row, eval, dim3(256), 0, s, g, f, h);
+    hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
     hipLaunchKernelGGL(k_aca_eval_col, eval, dim3(256), 0, s, g, f, h);
+    hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (st % 4 == 3) {

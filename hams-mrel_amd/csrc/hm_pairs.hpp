@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "hm_assembly.hpp"
+#include "hm_coop.hpp"
 #include "hm_device.hpp"
 
 namespace hm {
@@ -52,6 +53,79 @@ __device__ __forceinline__ void pair_entry(const GeomDev& g, const FreqDev& f, c
   Grn4 gr;
   if (f.h < 0.0) infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
   else fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, wvn, f.h, flag, gr);
+  pair_values(g, i, j, xp, flag, gr, ar, ai, sr, si);
+}
+
+// pair_entry for a whole workgroup of lanes, one entry per lane: the PIDCOCK
+// and EIGENE series are evaluated by lane groups of G (a group runs the series
+// of its entries in turn, in registers), EIGEN, the frequency limits and
+// infinite depth per lane as in pair_entry. All lanes call it; a lane with
+// active = false (i, j must still name panels) takes part in its group's
+// series and gets zeros. LINTON entries (R/h < 5e-4, whose private arrays
+// would spill the whole kernel) are left out: zeros and linton_entry() true;
+// callers evaluate them with pair_entry.
+__device__ __forceinline__ bool linton_entry(const GeomDev& g, const FreqDev& f, int i, int j) {
+  if (!(f.h > 0.0) || fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8)) return false;
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  return fin_regime(horiz_dist(xq[0], xp[0], xq[1], xp[1]), f.h) == kLinton;
+}
+
+template <int G>
+__device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev& f, const double* wvn, bool active,
+                                                int i, int j, double& ar, double& ai, double& sr, double& si) {
+  constexpr int KP = (kPidcockMaxTerms + G - 1) / G, KE = (kEigeneMaxTerms + G - 1) / G;
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double h = f.h;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const double zf = xq[2], zp = xp[2];
+  const bool zero = fabs(f.v) < F32(1.e-8), inf = !zero && fabs(f.v + 1.0) < F32(1.e-8);
+  const int reg = h > 0.0 && !zero && !inf ? fin_regime(rr, h) : -1;
+  const int nt = !active ? 0 : reg == kPidcock ? fin_pidcock_terms(rr, h) : reg == kEigene ? fin_eigene_terms(rr, h) : 0;
+  const int l = threadIdx.x & (G - 1);
+  double L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  for (int q = 0; q < G; ++q) {
+    const int ntq = group_bcast<G>(nt, q);
+    if (ntq == 0) continue;                  // uniform over the group
+    const int rq = group_bcast<G>(reg, q);
+    const double r = group_bcast<G>(rr, q);
+    const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
+    double l1, l2, l3;
+    if (rq == kPidcock) coop_series<kPidcock, G, KP>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
+    else coop_series<kEigene, G, KE>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
+    if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
+  }
+  ar = ai = sr = si = 0.0;
+  if (!active) return;
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  const double xpq = xq[0] - xp[0], ypq = xq[1] - xp[1];
+  Grn4 gr;
+  if (h < 0.0) {
+    infgreen3d(xq[0], xp[0], xq[1], xp[1], zf, zp, f.v, flag, gr);
+  } else if (inf) {
+    fin_limit_inf(xpq, ypq, zf, zp, rr, flag, gr);
+  } else {
+    FinOut fo{0, 0, 0, 0, 0, 0};
+    if (!zero) {
+      if (reg == kPidcock) {
+        fin_pidcock_head(rr, zf, zp, wvn, h, fo);
+        fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
+      } else if (reg == kEigene) {
+        fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+        fo.g_re += L1;
+        fo.gr_re += L2;
+        fo.gz_re += L3;
+        fin_strip(rr, zf, zp, fo);
+      } else if (reg == kLinton) {
+        return;
+      } else {
+        fin_eigen(rr, zf, zp, wvn, h, fo);
+      }
+      if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+    }
+    fin_to_grn(fo, xpq, ypq, zf, zp, rr, !zero, flag, gr);
+  }
   pair_values(g, i, j, xp, flag, gr, ar, ai, sr, si);
 }
 

@@ -163,3 +163,41 @@ def test_hmatrix_argument_errors():
             c.hmatrix_apply(2, np.zeros(p.n))
     finally:
         c.close()
+
+
+def test_linton_entries_between_stacked_bodies():
+    """Two copies of one cylinder, the second 10 m below the first: their side
+    panels share horizontal positions (R = 0, the LINTON regime) across the
+    off-diagonal blocks. The cross approximation evaluates those entries one
+    lane at a time after its lane-group pass; products against the dense
+    assembly at eps = 1e-7, full rank allowed."""
+    import dataclasses
+    top = meshes.cylinder(3.0, 4.0, 16, 4, 2)
+    low = dataclasses.replace(top, xyz=top.xyz + np.array([0.0, 0.0, -10.0]), xr=(0.0, 0.0, -10.0))
+    p = meshes.Problem([top, low], h=60.0, omegas=[0.5, 1.6], beta=np.array([0.0]))
+    m = p.n // 2
+    xy = p.xyz_p[:, :2]
+    rh = np.linalg.norm(xy[:m, None, :] - xy[None, m:, :], axis=-1) / p.h
+    assert np.count_nonzero(rh < 5e-4) > 0          # LINTON pairs across the two bodies
+    dense, hm = _ctx(p), _ctx(p)
+    try:
+        eps = 1e-7
+        hm.set_hmatrix(True, eps=eps, kmax=m)
+        rng = np.random.default_rng(11)
+        x = rng.standard_normal((p.n, 2)) + 1j * rng.standard_normal((p.n, 2))
+        for w in p.omegas:
+            dense.set_frequency(w)
+            dense.assemble()
+            a, s, _ = dense.get_system()
+            hm.set_frequency(w)
+            hm.assemble()
+            st = hm.hmatrix_stats()
+            assert st["unconverged"] == 0, st
+            for mat, ref in ((0, a), (1, s)):
+                ya = ref @ x
+                err = np.linalg.norm(hm.hmatrix_apply(mat, x) - ya, axis=0) / np.linalg.norm(ya, axis=0)
+                print(f"stacked w={w} mat {mat}: rel err {err} stats {st}", flush=True)
+                assert np.all(err < 20 * eps), (w, mat, err)
+    finally:
+        dense.close()
+        hm.close()
