@@ -121,7 +121,10 @@ __device__ __forceinline__ void residual_entry_direct(const GeomDev& g, const Fr
 
 // residual pivot rows of every unconverged process, one entry per thread
 // (grid: column chunks x processes)
-__global__ __launch_bounds__(256) void k_aca_eval_row(GeomDev g, FreqDev f, HmatDev h) {
+// the entry kernels are compiled for 4 waves per SIMD (128 VGPRs, ~270 bytes a
+// lane of cached spills): 21 % faster than at their natural 208 VGPRs
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_row(GeomDev g, FreqDev f,
+                                                                                          HmatDev h) {
   const int p = blockIdx.y;
   if (h.conv[p]) return;
   __shared__ double wvn[kNk];
@@ -188,7 +191,8 @@ __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
 }
 
 // residual pivot columns u_k = A(:, j*) - U V(j*, :)^T, one entry per thread
-__global__ __launch_bounds__(256) void k_aca_eval_col(GeomDev g, FreqDev f, HmatDev h) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_col(GeomDev g, FreqDev f,
+                                                                                          HmatDev h) {
   const int p = blockIdx.y;
   if (h.conv[p] || h.skip[p]) return;
   __shared__ double wvn[kNk];
