@@ -47,12 +47,12 @@ REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen":
 
 # HBM bytes per launch of the dominant kernel, measured on one MI355X by two
 # counter passes of `bench.py --steps 1 --assemble-only` (rocprofv3 --pmc
-# FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter k_assemble), scaled by the
-# readings of a 1 GiB device-to-device copy (fetch x2, write x1).
+# FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter k_assemble|k_aca_eval), scaled
+# by the readings of a 1 GiB device-to-device copy (fetch x2, write x1). The
+# lane-group kernels run at 5 waves per SIMD with register spills; the spill
+# traffic (scratch reads and writes that miss the L2) is most of these bytes.
 MEASURED_TRAFFIC = {
-    ("c5", "hmatrix", "asm_pidcock"): 772773.0 * 1024 * 2 + 1547392.1 * 1024,
-    ("c5", "gmres", "asm_eigene"): 5494024.2 * 1024 * 2 + 29080699.1 * 1024,
-    ("c5", "fp32ir", "asm_eigene"): 5494024.2 * 1024 * 2 + 29080699.1 * 1024,
+    ("c5", "hmatrix", "asm_pidcock"): 46981128.3 * 1024 * 2 + 56137415.2 * 1024,
 }
 
 
