@@ -28,13 +28,11 @@
 // function value and of its gradient vector (each against the size of the full
 // component: propagating + evanescent + Rankine) is recorded per cell of a 24 x 8 x 8 x 24 grid over the four inputs. The
 // assembly uses the network only in cells whose bound is below the tolerance
-// and falls back to the plain series elsewhere, for R/h < 0.1 (the close pairs
-// of one body, the Pidcock and Linton regimes), for R/h >= 0.5 (EIGEN, at most
-// 9 terms, cheaper than a network evaluation; the network is trained up to
-// R/h = 2) and for points deeper than 0.25 h. The domain (R/h in [0.1, 2],
-// z/h, zeta/h in [-0.25, 0], Kh in [0.01, 50]) covers the inter-body pairs of
-// surface-piercing hull arrays; over the whole water column, or down to R/h =
-// 0.05, a network of this size reached only ~1e-2.
+// and falls back to the plain series elsewhere. The domain is the EIGENE
+// regime, R/h in [0.05, 0.5], with z/h, zeta/h in [-0.25, 0] (the draft range
+// of surface-piercing hulls) and Kh in [0.01, 50]; R/h < 0.05 (PIDCOCK,
+// LINTON), R/h >= 0.5 (EIGEN, at most 9 terms, cheaper than a network
+// evaluation) and deeper points always take the plain series.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

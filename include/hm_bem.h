@@ -153,8 +153,8 @@ int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, 
 int hm_set_series_variant(hm_ctx *ctx, int variant);
 
 /* ---- learned surrogate of the evanescent-mode sums ----------------------------
- * Replaces, for 0.1 <= R/h <= 2 and z, zeta in [-0.25 h, 0], the evanescent part of EIGENE / EIGEN
- * (FinGreen3D.f90:413-506, :281-362): up to 50 K0/K1 terms and three Wynn
+ * Replaces, for 0.05 <= R/h < 0.5 and z, zeta in [-0.25 h, 0], the evanescent part of EIGENE
+ * (FinGreen3D.f90:413-506): up to 50 K0/K1 terms and three Wynn
  * epsilon limits per pair. A 4-64-64-64-64-3 SiLU network of (R/h, (z+zeta)/h,
  * (z-zeta)/h, Kh) predicts the real regular part (G, dG/dR, dG/dz of the
  * evanescent sums minus 1/r + 1/r1, smooth for R/h >= 0.05); the propagating
@@ -162,9 +162,10 @@ int hm_set_series_variant(hm_ctx *ctx, int variant);
  * FP32 accumulation; the first layer is FP32.
  *
  * hm_surrogate_train  generates nsamples training samples with the plain FP64
- *                     series on the GPU (h = 1, log-uniform R/h in [0.05, 8] and
- *                     Kh in [0.01, 50] over 2048 Kh values, uniform z, zeta in
- *                     [-h, 0]), trains with Adam (cosine learning rate from lr,
+ *                     series on the GPU (h = 1, log-uniform R/h in [0.05, 0.5] and
+ *                     Kh in [0.01, 50] over 2048 Kh values, z, zeta in
+ *                     [-0.25 h, 0], denser near the surface), trains with Adam
+ *                     (cosine learning rate from lr,
  *                     batch `batch`, `epochs` passes) by the library's own forward
  *                     and backward kernels (quant = 1: forward with BF16 weights
  *                     and activations as the inference runs it; 0: FP32), then
@@ -179,13 +180,14 @@ int hm_set_series_variant(hm_ctx *ctx, int variant);
  *                     <= 1e-4, <= 1e-3, Adam steps, samples.
  * hm_surrogate_get/set  the model as hm_surrogate_blob_size() floats
  *                     (parameters, output scales, error map).
- * hm_set_surrogate    use it in hm_assemble for finite depth: a pair takes the
- *                     network when its cell bound is <= tol, otherwise the plain
- *                     series (also always for R/h < 0.1, R/h >= 0.5 where the
- *                     plain EIGEN series is cheaper, and points deeper than
- *                     0.25 h).
+ * hm_set_surrogate    use it in hm_assemble for finite depth: an EIGENE pair
+ *                     takes the network when its cell bound is <= tol, otherwise
+ *                     the plain series (also always for points deeper than
+ *                     0.25 h); R/h < 0.05 and R/h >= 0.5 always take the plain
+ *                     series.
  * hm_surrogate_counts pairs assembled since hm_set_surrogate: [0] by the network,
- *                     [1] plain EIGENE fallback, [2] plain EIGEN fallback.
+ *                     [1] EIGENE pairs left to the plain series, [2] EIGEN pairs
+ *                     (plain series).
  * hm_surrogate_eval   network and plain regular parts (h = 1) at n points, with
  *                     the cell bound of each point (t_net, t_plain: 3n; bound: n).
  * hm_surrogate_grad   loss and gradient of the loss (mean squared error of the
