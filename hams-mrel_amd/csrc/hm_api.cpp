@@ -100,6 +100,8 @@ struct hm_ctx {
   int sur_quant = 1;
   double sur_tol = 1e-4;
   long long sur_counts[3] = {0, 0, 0};
+  // H-matrix diagonal blocks: PIDCOCK pairs assembled together with their mirrors
+  bool pair_sym = true;
   void* sur_img() const { return static_cast<char*>(d_sur.p) + hm::kSurImgOffset; }
   hm::SurDev sur_dev() const {
     const float* b = d_sur.as<float>();
@@ -365,8 +367,10 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
 static int classify(hm_ctx* c) {
   const hm::GeomDev gd = c->geom();
   const double h = c->h;
-  // the H-matrix form assembles only the pairs of the diagonal blocks densely
+  // the H-matrix form assembles only the pairs of the diagonal blocks densely,
+  // PIDCOCK pairs with their mirrors (the field points are the panel centroids)
   const int blk = c->hm_on ? c->bm() : 0;
+  const bool sym = blk > 0 && c->pair_sym;
   const long long np = blk ? (long long)c->nbody * blk * blk : (long long)c->nt() * c->n;
   for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
   c->stage_begin(kClassify);
@@ -374,26 +378,40 @@ static int classify(hm_ctx* c) {
   HIPCHK(c, c->d_cursor.ensure(sizeof(unsigned long long) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_pairs.ensure(sizeof(uint32_t) * (size_t)np), "classify");
   HIPCHK(c, hipMemsetAsync(c->d_hist.p, 0, sizeof(unsigned) * hm::kNumKeys, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true, blk), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true, blk, sym), "classify");
   std::vector<unsigned> hist(hm::kNumKeys);
   HIPCHK(c, hipMemcpyAsync(hist.data(), c->d_hist.p, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, c->stream), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
+  // pairs assembled per key (a stored PIDCOCK pair is assembled with its mirror)
   c->hist_host.assign(hist.begin(), hist.end());
+  if (sym)
+    for (int k = hm::kPidcock * hm::kKeyStride; k < (hm::kPidcock + 1) * hm::kKeyStride; ++k) c->hist_host[k] *= 2;
   std::vector<unsigned long long> cur(hm::kNumKeys);
   unsigned long long acc = 0;
   for (int k = 0; k < hm::kNumKeys; ++k) {
     cur[k] = acc;
     acc += hist[k];
   }
-  if ((long long)acc != np) return c->fail(HM_ERR_STATE, "pair classification lost pairs (%llu of %lld)", acc, np);
+  const long long pid = (long long)(cur[(hm::kPidcock + 1) * hm::kKeyStride] - cur[hm::kPidcock * hm::kKeyStride]);
+  if ((long long)acc + (sym ? pid : 0) != np)
+    return c->fail(HM_ERR_STATE, "pair classification lost pairs (%llu of %lld)", acc, np);
   for (int r = 0; r < 4; ++r) {
     c->seg_off[r] = (long long)cur[r * hm::kKeyStride];
     c->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
   }
   HIPCHK(c, hipMemcpyAsync(c->d_cursor.p, cur.data(), sizeof(unsigned long long) * hm::kNumKeys, hipMemcpyHostToDevice, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false, blk), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false, blk, sym), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
   return c->stage_end(kClassify);
+}
+
+extern "C" int hm_set_pair_symmetry(hm_ctx* c, int enable) {
+  if (!c) return HM_ERR_ARG - 1;
+  const bool was = c->pair_sym;
+  c->pair_sym = enable != 0;
+  c->assembled = c->factored = c->solved = false;
+  if (was != c->pair_sym && c->hm_on && c->have_depth && c->h > 0.0) return classify(c);
+  return HM_OK;
 }
 
 extern "C" int hm_set_depth(hm_ctx* c, double h) {
@@ -478,8 +496,12 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
     for (int r = 0; r < (sur ? 2 : 4); ++r) {
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
-      HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld,
-                                    c->series_variant), "assemble");
+      if (r == hm::kPidcock && blk > 0 && c->pair_sym)
+        HIPCHK(c, hm::assemble_pidcock_sym(c->stream, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld),
+               "assemble");
+      else
+        HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld,
+                                      c->series_variant), "assemble");
       c->stage_end(kAsmRegime0 + r);
     }
     if (sur) {

@@ -177,6 +177,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
   coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld, given);
 }
 
+// PIDCOCK pairs (i, j), i < j, of blocks whose field points are their panels'
+// centroids (the H-matrix diagonal blocks): the group evaluates the series of
+// the pair and of its mirror together (four series, four limits instead of six
+// and six), then every lane finishes (i, j) and (j, i).
+template <int G, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K>::value, 8))) void
+k_assemble_pidcock_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long long np, double* A, double* S,
+                       long long ld) {
+  __shared__ double wvn[kNk];
+  load_wvn(wvn, f.wvn, kNk);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < np;
+  const uint32_t p = pairs[valid ? t : np - 1];
+  const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double h = f.h;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const double zf = xq[2], zp = xp[2];
+  const int nt = valid ? fin_pidcock_terms(rr, h) : 0;
+  const int l = threadIdx.x & (G - 1);
+  double L1 = 0.0, L2 = 0.0, L3 = 0.0, L4 = 0.0;
+  for (int q = 0; q < G; ++q) {
+    const int ntq = group_bcast<G>(nt, q);
+    if (ntq == 0) continue;                  // uniform over the group
+    const double r = group_bcast<G>(rr, q);
+    const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
+    double l1, l2, l3, l4;
+    coop_series_sym<G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3, l4);
+    if (l == q) { L1 = l1; L2 = l2; L3 = l3; L4 = l4; }
+  }
+  if (!valid) return;
+  coop_finish<kPidcock>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld);
+  coop_finish<kPidcock>(g, wvn, h, j, i, rr, L1, L2, L4, A, S, ld);
+}
+
+hipError_t assemble_pidcock_sym(hipStream_t s, const GeomDev& g, const FreqDev& f, const uint32_t* pairs, long long np,
+                                double* A, double* S, long long ld) {
+  if (np <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_assemble_pidcock_sym<16, 7>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, f, pairs,
+                     np, A, S, ld);
+  return hipGetLastError();
+}
+
 // ---- classification --------------------------------------------------------
 __device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j) {
   const double* xp = g.fp + 3 * (long long)i;
@@ -189,14 +233,23 @@ __device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j
   return reg * kKeyStride + len;
 }
 
-__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk) {
+// sym: PIDCOCK pairs are kept only as (i, j) with i < j (their mirrors are
+// assembled with them, k_assemble_pidcock_sym)
+__device__ __forceinline__ bool mirrored_out(bool sym, int key, int i, int j) {
+  return sym && key / kKeyStride == kPidcock && i > j;
+}
+
+__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk, int sym) {
   __shared__ unsigned sh[kNumKeys];
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
   __syncthreads();
   const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n)) atomicAdd(&sh[pair_key(g, h, i, j)], 1u);
+  if (i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n)) {
+    const int key = pair_key(g, h, i, j);
+    if (!mirrored_out(sym != 0, key, i, j)) atomicAdd(&sh[key], 1u);
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
     if (sh[k]) atomicAdd(&hist[k], sh[k]);
@@ -204,7 +257,7 @@ __global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, uns
 
 // one global atomic per (workgroup, key): ranks within the workgroup come from LDS
 __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, unsigned long long* cursor,
-                                                          uint32_t* pairs, int blk) {
+                                                          uint32_t* pairs, int blk, int sym) {
   __shared__ unsigned cnt[kNumKeys];
   __shared__ unsigned long long base[kNumKeys];
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cnt[k] = 0;
@@ -212,8 +265,9 @@ __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, u
   const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
-  const bool in = i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n);
-  const int key = in ? pair_key(g, h, i, j) : 0;
+  const bool inb = i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n);
+  const int key = inb ? pair_key(g, h, i, j) : 0;
+  const bool in = inb && !mirrored_out(sym != 0, key, i, j);
   const unsigned rank = in ? atomicAdd(&cnt[key], 1u) : 0u;
   __syncthreads();
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
@@ -228,10 +282,10 @@ static dim3 pair_grid(const GeomDev& g, int blk) {
 }
 
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist,
-                          unsigned long long* cursor, uint32_t* pairs, bool count_only, int blk) {
+                          unsigned long long* cursor, uint32_t* pairs, bool count_only, int blk, bool sym) {
   const dim3 grid = pair_grid(g, blk);
-  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist, blk);
-  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs, blk);
+  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist, blk, sym ? 1 : 0);
+  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs, blk, sym ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -28,9 +28,14 @@ struct FreqDev {
   const double* wvn;       // kNk roots on the device
 };
 
-// blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0)
+// blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0);
+// sym keeps PIDCOCK pairs only as (i, j) with i < j (for assemble_pidcock_sym)
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist, unsigned long long* cursor,
-                          uint32_t* pairs, bool count_only, int blk = 0);
+                          uint32_t* pairs, bool count_only, int blk = 0, bool sym = false);
+// PIDCOCK pairs (i, j), i < j, and their mirrors (j, i) together (field points =
+// panel centroids, ni = 0): one series evaluation serves both
+hipError_t assemble_pidcock_sym(hipStream_t s, const GeomDev& g, const FreqDev& f, const uint32_t* pairs, long long np,
+                                double* A, double* S, long long ld);
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
                          long long ld, int blk = 0);
 // done, vreg (optional, EIGENE only): pairs with done[t] != 0 take the regular

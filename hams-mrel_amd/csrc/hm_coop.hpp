@@ -328,4 +328,46 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
   l3 = coop_limes<G, K>(n, s3, l);
 }
 
+// PIDCOCK series of one pair and of its mirror (depths swapped) by the whole
+// group: the three limits of the pair and l4, the limit of the series of the
+// zph-derivatives (the mirror's l3; its l1, l2 are the pair's)
+template <int G, int K>
+__device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, double zph, double h, double v,
+                                                const double* wvn, int l, double& l1, double& l2, double& l3,
+                                                double& l4) {
+  typedef LaneGroup<G> LG;
+  double s1[K], s2[K], s3[K], s4[K];
+  double c1 = 0.0, c2 = 0.0, c3 = 0.0, c4 = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int row = l + G * k;
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+    if (G * k < ntq) {
+      if (row >= 1 && row < ntq) fin_pidcock_term_sym(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3, t4);
+      t1 = LG::incl_sum(t1, l) + c1;
+      t2 = LG::incl_sum(t2, l) + c2;
+      t3 = LG::incl_sum(t3, l) + c3;
+      t4 = LG::incl_sum(t4, l) + c4;
+      c1 = group_bcast<G>(t1, G - 1);
+      c2 = group_bcast<G>(t2, G - 1);
+      c3 = group_bcast<G>(t3, G - 1);
+      c4 = group_bcast<G>(t4, G - 1);
+    } else {
+      t1 = c1;
+      t2 = c2;
+      t3 = c3;
+      t4 = c4;
+    }
+    s1[k] = t1;
+    s2[k] = t2;
+    s3[k] = t3;
+    s4[k] = t4;
+  }
+  const int n = ntq - 1;
+  l1 = coop_limes<G, K>(n, s1, l);
+  l2 = coop_limes<G, K>(n, s2, l);
+  l3 = coop_limes<G, K>(n, s3, l);
+  l4 = coop_limes<G, K>(n, s4, l);
+}
+
 }  // namespace hm

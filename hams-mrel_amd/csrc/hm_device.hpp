@@ -476,6 +476,33 @@ HM_DEV void fin_pidcock_term(int i, double um, double r, double zfh, double zph,
   t2 = -um * am * k1 + umt * amt * k1t;
   t3 = dam * k0 - damt * k0t;
 }
+// fin_pidcock_term and t4, the derivative of the term with respect to zph:
+// the term of the mirrored pair (field and source depths swapped) has t1, t2
+// unchanged and t4 as its t3 (the series is symmetric in zfh, zph)
+HM_DEV void fin_pidcock_term_sym(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
+                                 double& t2, double& t3, double& t4) {
+  const double umt = i * kPi / h;
+  double k0, k1, k0t, k1t;
+  sf_k01(um * r, k0, k1);
+  sf_k01(umt * r, k0t, k1t);
+  const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
+  // t1..t3 as fin_pidcock_term computes them (bit for bit)
+  double su, cu, sut, cut;
+  sincos(um * zfh, &su, &cu);
+  sincos(umt * zfh, &sut, &cut);
+  const double cup = cos(um * zph), cutp = cos(umt * zph);
+  const double sup = sin(um * zph), sutp = sin(umt * zph);
+  const double am = ac * cu * cup;
+  const double amt = cut * cutp / h;
+  const double dam = -um * ac * su * cup;
+  const double damt = -umt * sut * cutp / h;
+  const double damp = -um * ac * cu * sup;
+  const double damtp = -umt * cut * sutp / h;
+  t1 = am * k0 - amt * k0t;
+  t2 = -um * am * k1 + umt * amt * k1t;
+  t3 = dam * k0 - damt * k0t;
+  t4 = damp * k0 - damtp * k0t;
+}
 HM_DEV void fin_pidcock_tail(double l1, double l2, double l3, double r, double zf, double zp, double h, FinOut& o) {
   double ig, igr, igz;
   fin_images(r, zf, zp, h, ig, igr, igz);

@@ -134,6 +134,11 @@ module hm_bem
          integer(c_int), value :: enable, kmax
          real(c_double), value :: eps, eps_s
       end function
+      integer(c_int) function hm_set_pair_symmetry(ctx, enable) bind(C, name="hm_set_pair_symmetry")
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: enable
+      end function
       integer(c_int) function hm_set_frequency_limit(ctx, which) bind(C, name="hm_set_frequency_limit")
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx
@@ -212,7 +217,7 @@ module hm_bem
    public :: FINGREEN3D, INFGREEN3D
    public :: ASSB_LEFT_MULTI, ASSB_RBC_MULTI, ASSB_DBC_MULTI, RADIATION_SOLVER_MULTI, DIFFRACTION_SOLVER_MULTI
    public :: RFORCE_EFORCE_MULTI
-   public :: hm_set_gmres, hm_set_hmatrix, hm_set_frequency_limit, hm_hmatrix_build, hm_hmatrix_release, &
+   public :: hm_set_gmres, hm_set_hmatrix, hm_set_pair_symmetry, hm_set_frequency_limit, hm_hmatrix_build, hm_hmatrix_release, &
              hm_hmatrix_apply, hm_ziter_solve, hm_batch_create, hm_batch_destroy, hm_batch_set_constants, &
              hm_batch_add_problem, hm_batch_prepare, hm_batch_solve
    public :: ZITER_SOLVE_MULTI, SETUP_BLOCK_PRECONDITIONER, HMATRIX_BUILD, HMATVEC, HMATRIX_DESTROY

@@ -25,7 +25,7 @@ _vp = ctypes.c_void_p
 
 # every symbol include/hm_bem.h declares
 EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm_set_geometry",
-           "hm_set_depth", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
+           "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
            "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
@@ -256,6 +256,10 @@ class Context:
         """eps: ACA tolerance of A; eps_s: of S (default eps)"""
         self._chk(self.L.hm_set_hmatrix(self.ptr, _i(1 if enable else 0), _d(eps), _d(eps if eps_s is None else eps_s),
                                         _i(kmax)), "hm_set_hmatrix")
+
+    def set_pair_symmetry(self, enable=True):
+        """H-matrix diagonal blocks: PIDCOCK pairs assembled with their mirrors"""
+        self._chk(self.L.hm_set_pair_symmetry(self.ptr, _i(1 if enable else 0)), "hm_set_pair_symmetry")
 
     def hmatrix_stats(self):
         un, st, mx = _i(), _i(), _i()

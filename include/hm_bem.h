@@ -111,6 +111,16 @@ int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, dou
 int hm_set_hmatrix(hm_ctx *ctx, int enable, double eps, double eps_s, int kmax);
 int hm_get_hmatrix_stats(hm_ctx *ctx, int *unconverged, int *steps, double *mean_rank, int *max_rank,
                          double *stored_fraction);
+/* The diagonal blocks of the H-matrix form assemble each PIDCOCK pair (i, j),
+ * i < j, together with its mirror (j, i): the reference's series of the pair
+ * is symmetric in the two depths, so one evaluation of four series (value, R-
+ * and both depth derivatives) and four epsilon limits serves both (instead of
+ * six and six). The mirrored entries equal a direct evaluation to rounding,
+ * within the one-ulp sensitivity of the reference's epsilon algorithm (its
+ * exact-repeat exit). enable = 0 assembles every pair directly (bit for bit
+ * the dense path's blocks). Default on; toggling reclassifies the pairs. No
+ * counterpart in the reference (it assembles every pair). */
+int hm_set_pair_symmetry(hm_ctx *ctx, int enable);
 /* y = H x in FP64 for the assembled H-matrix form of A (mat 0) or S (mat 1),
  * or for the H-matrix of hm_hmatrix_build (mat 0); x, y: n x ncol column-major
  * complex (HMATVEC, HMatrix.f90:760, for several columns). */
