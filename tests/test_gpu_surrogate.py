@@ -131,6 +131,46 @@ def test_assembly_with_surrogate(trained):
     ctx.close()
 
 
+def test_hmatrix_with_surrogate(trained):
+    """H-matrix form with the network on: it serves the EIGENE pairs of the
+    diagonal blocks through the same error map and partition as the dense
+    assembly; the cross approximation keeps the plain series (network values
+    switched on and off cell by cell inside a block are not low rank: the
+    blocks then stop at the rank cap). Forces against the plain FP64 dense solve
+    at the bar of the dense assembly test above (2e-3 for this briefly trained
+    network); where the network takes pairs the result must move."""
+    c, _, _, _ = trained
+    blob = c.surrogate_get()
+    p = _quad()
+    dense = hm_bem.Context()
+    hmx = hm_bem.Context()
+    try:
+        dense.setup(p)
+        hmx.setup(p)
+        hmx.surrogate_set(blob)
+        hmx.set_hmatrix(True, eps=1e-5, eps_s=1e-6, kmax=50)
+        hmx.set_gmres(restart=50, tol=1e-8, maxiter=500)
+        for w in p.omegas:
+            ref = dense.solve_frequency(w, hm_bem.HM_FP64)
+            hmx.set_surrogate(False)
+            plain = hmx.solve_frequency(w, hm_bem.HM_GMRES)
+            hmx.set_surrogate(True, 3e-3)
+            got = hmx.solve_frequency(w, hm_bem.HM_GMRES)
+            k = hmx.surrogate_counts()
+            st = hmx.hmatrix_stats()
+            errs = {key: relerr(got[key], ref[key]) for key in ("amas", "bdmp", "exfc")}
+            moved = max(relerr(got[key], plain[key]) for key in ("amas", "bdmp", "exfc"))
+            print("omega", w, "counts", k, "errors", errs, "network moved results by", moved, "hmatrix", st, flush=True)
+            assert st["unconverged"] == 0, st
+            if k["network"] > 0:
+                assert moved > 0.0
+            for key in errs:
+                assert errs[key] < 2e-3, (w, key, errs)
+    finally:
+        dense.close()
+        hmx.close()
+
+
 def test_surrogate_argument_errors():
     c = hm_bem.Context()
     with pytest.raises(hm_bem.HMError) as e:
