@@ -385,15 +385,16 @@ static int classify(hm_ctx* c) {
   // pairs assembled per key (a stored PIDCOCK pair is assembled with its mirror)
   c->hist_host.assign(hist.begin(), hist.end());
   if (sym)
-    for (int k = hm::kPidcock * hm::kKeyStride; k < (hm::kPidcock + 1) * hm::kKeyStride; ++k) c->hist_host[k] *= 2;
+    for (int k = hm::kPidcock * hm::kKeyStride; k < (hm::kEigene + 1) * hm::kKeyStride; ++k) c->hist_host[k] *= 2;
   std::vector<unsigned long long> cur(hm::kNumKeys);
   unsigned long long acc = 0;
   for (int k = 0; k < hm::kNumKeys; ++k) {
     cur[k] = acc;
     acc += hist[k];
   }
-  const long long pid = (long long)(cur[(hm::kPidcock + 1) * hm::kKeyStride] - cur[hm::kPidcock * hm::kKeyStride]);
-  if ((long long)acc + (sym ? pid : 0) != np)
+  // with mirrors, every stored PIDCOCK / EIGENE pair (i < j) stands for two
+  const long long mir = (long long)(cur[(hm::kEigene + 1) * hm::kKeyStride] - cur[hm::kPidcock * hm::kKeyStride]);
+  if ((long long)acc + (sym ? mir : 0) != np)
     return c->fail(HM_ERR_STATE, "pair classification lost pairs (%llu of %lld)", acc, np);
   for (int r = 0; r < 4; ++r) {
     c->seg_off[r] = (long long)cur[r * hm::kKeyStride];
@@ -493,12 +494,13 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
     const bool sur = c->sur_on && c->sur_ready;
+    // the pair lists hold PIDCOCK and EIGENE pairs with their mirrors (classify)
+    const bool sym = blk > 0 && c->pair_sym;
     for (int r = 0; r < (sur ? 2 : 4); ++r) {
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
-      if (r == hm::kPidcock && blk > 0 && c->pair_sym)
-        HIPCHK(c, hm::assemble_pidcock_sym(c->stream, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld),
-               "assemble");
+      if (sym && (r == hm::kPidcock || r == hm::kEigene))
+        HIPCHK(c, hm::assemble_sym(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld), "assemble");
       else
         HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld,
                                       c->series_variant), "assemble");
@@ -510,17 +512,20 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
       // that mask. EIGEN (R/h >= 0.5, at most 9 terms) stays plain.
       const long long ne = c->seg_cnt[2];
       if (ne > 0) {
-        HIPCHK(c, c->d_fb.ensure(hm::sur_regular_bytes(ne)), "surrogate alloc");
+        HIPCHK(c, c->d_fb.ensure(hm::sur_regular_bytes(ne, sym)), "surrogate alloc");
         const double kh = c->w1 * c->w1 * c->h / c->g_disp;
         hm::SurPartition part{};
         c->stage_begin(kAsmSurrogate);
         HIPCHK(c, hm::sur_regular_parts(c->stream, gd, f, kh, pairs + c->seg_off[2], ne, c->sur_dev(),
-                                        (float)c->sur_tol, c->d_fb.p, c->d_fbcnt.as<unsigned long long>(), part),
+                                        (float)c->sur_tol, c->d_fb.p, c->d_fbcnt.as<unsigned long long>(), part, sym),
                "assemble surrogate");
         c->stage_end(kAsmSurrogate);
         c->stage_begin(kAsmRegime0 + 2);
-        HIPCHK(c, hm::assemble_regime(c->stream, 2, gd, f, part.pairs, ne, A, S, ld, c->series_variant, part.mask,
-                                      part.vreg), "assemble");
+        if (sym)
+          HIPCHK(c, hm::assemble_sym(c->stream, 2, gd, f, part.pairs, ne, A, S, ld, part.mask, part.vreg), "assemble");
+        else
+          HIPCHK(c, hm::assemble_regime(c->stream, 2, gd, f, part.pairs, ne, A, S, ld, c->series_variant, part.mask,
+                                        part.vreg), "assemble");
         c->stage_end(kAsmRegime0 + 2);
       }
       if (c->seg_cnt[3] > 0) {
@@ -529,7 +534,7 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
                                       c->series_variant), "assemble");
         c->stage_end(kAsmRegime0 + 3);
       }
-      c->sur_counts[1] += ne;
+      c->sur_counts[1] += sym ? 2 * ne : ne;     // pairs assembled (mirrors included)
       c->sur_counts[2] += c->seg_cnt[3];
     }
   }

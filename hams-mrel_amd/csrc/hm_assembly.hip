@@ -177,18 +177,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
   coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld, given);
 }
 
-// PIDCOCK pairs (i, j), i < j, of blocks whose field points are their panels'
-// centroids (the H-matrix diagonal blocks): the group evaluates the series of
-// the pair and of its mirror together (four series, four limits instead of six
-// and six), then every lane finishes (i, j) and (j, i).
-template <int G, int K>
+// PIDCOCK or EIGENE pairs (i, j), i < j, of blocks whose field points are
+// their panels' centroids (the H-matrix diagonal blocks): the group evaluates
+// the series of the pair and of its mirror together (four series, four limits
+// instead of six and six), then every lane finishes (i, j) and (j, i). EIGENE
+// with a mask: pairs with done[t] != 0 take the regular parts of both
+// directions from vreg[6 t .. 6 t + 5] instead.
+template <int REG, int G, int K>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K>::value, 8))) void
-k_assemble_pidcock_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long long np, double* A, double* S,
-                       long long ld) {
+k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long long np, double* A, double* S,
+               long long ld, const uint8_t* __restrict__ done, const float* __restrict__ vreg) {
   __shared__ double wvn[kNk];
   load_wvn(wvn, f.wvn, kNk);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < np;
+  const bool given = REG == kEigene && valid && done && done[t];
   const uint32_t p = pairs[valid ? t : np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
   const double* xp = g.fp + 3 * (long long)i;
@@ -196,7 +199,7 @@ k_assemble_pidcock_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const double zf = xq[2], zp = xp[2];
-  const int nt = valid ? fin_pidcock_terms(rr, h) : 0;
+  const int nt = !valid || given ? 0 : REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
   const int l = threadIdx.x & (G - 1);
   double L1 = 0.0, L2 = 0.0, L3 = 0.0, L4 = 0.0;
   for (int q = 0; q < G; ++q) {
@@ -205,19 +208,30 @@ k_assemble_pidcock_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs,
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
     double l1, l2, l3, l4;
-    coop_series_sym<G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3, l4);
+    coop_series_sym<REG, G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3, l4);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; L4 = l4; }
   }
   if (!valid) return;
-  coop_finish<kPidcock>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld);
-  coop_finish<kPidcock>(g, wvn, h, j, i, rr, L1, L2, L4, A, S, ld);
+  if (given) {
+    const float* v = vreg + 6 * t;
+    coop_finish<REG>(g, wvn, h, i, j, rr, v[0], v[1], v[2], A, S, ld, true);
+    coop_finish<REG>(g, wvn, h, j, i, rr, v[3], v[4], v[5], A, S, ld, true);
+    return;
+  }
+  coop_finish<REG>(g, wvn, h, i, j, rr, L1, L2, L3, A, S, ld);
+  coop_finish<REG>(g, wvn, h, j, i, rr, L1, L2, L4, A, S, ld);
 }
 
-hipError_t assemble_pidcock_sym(hipStream_t s, const GeomDev& g, const FreqDev& f, const uint32_t* pairs, long long np,
-                                double* A, double* S, long long ld) {
+hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
+                        long long np, double* A, double* S, long long ld, const uint8_t* done, const float* vreg) {
   if (np <= 0) return hipSuccess;
-  hipLaunchKernelGGL((k_assemble_pidcock_sym<16, 7>), dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, g, f, pairs,
-                     np, A, S, ld);
+  const dim3 gd((unsigned)((np + 255) / 256)), bd(256);
+  if (regime == kPidcock)
+    hipLaunchKernelGGL((k_assemble_sym<kPidcock, 16, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, nullptr, nullptr);
+  else if (regime == kEigene)
+    hipLaunchKernelGGL((k_assemble_sym<kEigene, 8, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -233,10 +247,11 @@ __device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j
   return reg * kKeyStride + len;
 }
 
-// sym: PIDCOCK pairs are kept only as (i, j) with i < j (their mirrors are
-// assembled with them, k_assemble_pidcock_sym)
+// sym: PIDCOCK and EIGENE pairs are kept only as (i, j) with i < j (their
+// mirrors are assembled with them, k_assemble_sym)
 __device__ __forceinline__ bool mirrored_out(bool sym, int key, int i, int j) {
-  return sym && key / kKeyStride == kPidcock && i > j;
+  const int reg = key / kKeyStride;
+  return sym && (reg == kPidcock || reg == kEigene) && i > j;
 }
 
 __global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk, int sym) {

@@ -29,13 +29,16 @@ struct FreqDev {
 };
 
 // blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0);
-// sym keeps PIDCOCK pairs only as (i, j) with i < j (for assemble_pidcock_sym)
+// sym keeps PIDCOCK and EIGENE pairs only as (i, j) with i < j (for assemble_sym)
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist, unsigned long long* cursor,
                           uint32_t* pairs, bool count_only, int blk = 0, bool sym = false);
-// PIDCOCK pairs (i, j), i < j, and their mirrors (j, i) together (field points =
-// panel centroids, ni = 0): one series evaluation serves both
-hipError_t assemble_pidcock_sym(hipStream_t s, const GeomDev& g, const FreqDev& f, const uint32_t* pairs, long long np,
-                                double* A, double* S, long long ld);
+// PIDCOCK or EIGENE pairs (i, j), i < j, and their mirrors (j, i) together
+// (field points = panel centroids, ni = 0): one series evaluation serves both.
+// done, vreg (EIGENE, optional): pairs with done[t] != 0 take the regular parts
+// of (i, j) and (j, i) from vreg[6 t .. 6 t + 5]
+hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
+                        long long np, double* A, double* S, long long ld, const uint8_t* done = nullptr,
+                        const float* vreg = nullptr);
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
                          long long ld, int blk = 0);
 // done, vreg (optional, EIGENE only): pairs with done[t] != 0 take the regular

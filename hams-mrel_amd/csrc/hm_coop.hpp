@@ -328,10 +328,10 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
   l3 = coop_limes<G, K>(n, s3, l);
 }
 
-// PIDCOCK series of one pair and of its mirror (depths swapped) by the whole
-// group: the three limits of the pair and l4, the limit of the series of the
-// zph-derivatives (the mirror's l3; its l1, l2 are the pair's)
-template <int G, int K>
+// PIDCOCK or EIGENE series of one pair and of its mirror (depths swapped) by
+// the whole group: the three limits of the pair and l4, the limit of the series
+// of the zph-derivatives (the mirror's l3; its l1, l2 are the pair's)
+template <int REG, int G, int K>
 __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, double zph, double h, double v,
                                                 const double* wvn, int l, double& l1, double& l2, double& l3,
                                                 double& l4) {
@@ -343,7 +343,10 @@ __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, d
     const int row = l + G * k;
     double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
     if (G * k < ntq) {
-      if (row >= 1 && row < ntq) fin_pidcock_term_sym(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3, t4);
+      if (row >= 1 && row < ntq) {
+        if (REG == kEigene) fin_evanescent_sym(wvn[row], r, zfh, zph, h, t1, t2, t3, t4);
+        else fin_pidcock_term_sym(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3, t4);
+      }
       t1 = LG::incl_sum(t1, l) + c1;
       t2 = LG::incl_sum(t2, l) + c2;
       t3 = LG::incl_sum(t3, l) + c3;

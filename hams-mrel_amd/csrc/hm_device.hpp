@@ -313,6 +313,23 @@ HM_DEV void fin_evanescent(double km, double r, double zfh, double zph, double h
   t3 = -pm * km * k0 * skzf * ckzp;
 }
 
+// fin_evanescent and t4, the derivative of the term with respect to zph (the
+// mirrored pair's t3; t1, t2 are symmetric in zfh, zph)
+HM_DEV void fin_evanescent_sym(double km, double r, double zfh, double zph, double h, double& t1, double& t2,
+                               double& t3, double& t4) {
+  const double twokmh = 2.0 * km * h;
+  const double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
+  double k0, k1;
+  sf_k01(km * r, k0, k1);
+  double skzf, ckzf;
+  sincos(km * zfh, &skzf, &ckzf);
+  const double ckzp = cos(km * zph), skzp = sin(km * zph);
+  t1 = pm * k0 * ckzf * ckzp;
+  t2 = -pm * km * k1 * ckzf * ckzp;
+  t3 = -pm * km * k0 * skzf * ckzp;
+  t4 = -pm * km * k0 * ckzf * skzp;
+}
+
 // R/h >= 0.5: at most 9 evanescent terms, early exit at 1e-6 (EIGEN :281)
 HM_DEV void fin_eigen(double r, double zf, double zp, const double* wvn, double h, FinOut& o) {
   const double zfh = zf + h, zph = zp + h;

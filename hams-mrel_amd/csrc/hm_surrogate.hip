@@ -542,8 +542,10 @@ __global__ __launch_bounds__(256) void k_predict(long long n, const float* X, Su
 // network; every other pair gets done[t] = 0. blk[b] = pairs taken by workgroup
 // b. The MFMA network needs the whole workgroup, so a workgroup runs it when
 // any of its pairs takes it (the segment is in key order, i.e. by R/h, so whole
-// workgroups mostly agree).
-template <bool QUANT>
+// workgroups mostly agree). SYM: the pair (i, j) stands for its mirror (j, i)
+// too (depths swapped); it is taken only if both directions are, and vreg holds
+// 6 floats a pair (the pair's V, then the mirror's).
+template <bool QUANT, bool SYM>
 __global__ __launch_bounds__(256) void k_sur_regular(GeomDev g, FreqDev f, double kh, const uint32_t* __restrict__ pairs,
                                                      long long np, SurDev m, float tol, uint8_t* __restrict__ done,
                                                      float* __restrict__ vreg, unsigned* __restrict__ blk,
@@ -558,34 +560,49 @@ __global__ __launch_bounds__(256) void k_sur_regular(GeomDev g, FreqDev f, doubl
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const double zf = xq[2], zp = xp[2], rh = rr / h;
-  float x[4];
+  float x[4], xm[4];
   sur_inputs(rh > kSurRhMin ? rh : kSurRhMin, (zf + zp) / h, (zf - zp) / h, kh, x);
+  if (SYM) sur_inputs(rh > kSurRhMin ? rh : kSurRhMin, (zp + zf) / h, (zp - zf) / h, kh, xm);
   const bool inside = rh >= kSurRhMin && rh < 0.5 && zf >= -kSurDepth * h && zp >= -kSurDepth * h;
-  const bool net = valid && inside && m.map[sur_cell(x)] <= tol;
+  const bool net = valid && inside && m.map[sur_cell(x)] <= tol && (!SYM || m.map[sur_cell(xm)] <= tol);
   if (valid) done[t] = net ? 1 : 0;
   const int nb = __syncthreads_count(net);   // uniform over the workgroup
   if (threadIdx.x == 0) {
     blk[blockIdx.x] = (unsigned)nb;
-    if (nb) atomicAdd(count, (unsigned long long)nb);
+    if (nb) atomicAdd(count, (unsigned long long)nb * (SYM ? 2u : 1u));
   }
-  float y[kSurOut];
+  constexpr int nv = SYM ? 6 : 3;
+  float y[kSurOut], ym[kSurOut];
   if (QUANT) {
     if (nb == 0) return;
     load_sur_lds(L.m, m.img);
     __syncthreads();
     mlp_block_mfma(L, x, y);
+    if (SYM) {
+      __syncthreads();
+      mlp_block_mfma(L, xm, ym);
+    }
     if (!net) return;
   } else {
     if (!net) return;
     mlp_eval(m.p, x, y, nullptr, nullptr, 0, false);
+    if (SYM) mlp_eval(m.p, xm, ym, nullptr, nullptr, 0, false);
   }
   // U = Y R^p e^{k1 R} is dimensionless for any depth (R, k1 dimensional)
   const double u[3] = {(double)(y[0] * m.yscale[0]), (double)(y[1] * m.yscale[1]), (double)(y[2] * m.yscale[2])};
   double v[3];
   sur_regular_from(rr, zf, zp, f.wvn[1], u, v);
-  vreg[3 * t] = (float)v[0];
-  vreg[3 * t + 1] = (float)v[1];
-  vreg[3 * t + 2] = (float)v[2];
+  vreg[nv * t] = (float)v[0];
+  vreg[nv * t + 1] = (float)v[1];
+  vreg[nv * t + 2] = (float)v[2];
+  if (SYM) {
+    const double um[3] = {(double)(ym[0] * m.yscale[0]), (double)(ym[1] * m.yscale[1]),
+                          (double)(ym[2] * m.yscale[2])};
+    sur_regular_from(rr, zp, zf, f.wvn[1], um, v);
+    vreg[nv * t + 3] = (float)v[0];
+    vreg[nv * t + 4] = (float)v[1];
+    vreg[nv * t + 5] = (float)v[2];
+  }
 }
 
 // exclusive prefix sums of the per-workgroup counts (one workgroup of 1024
@@ -621,7 +638,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const unsigned* __restrict
 __global__ __launch_bounds__(256) void k_partition_done(const uint32_t* __restrict__ pairs, long long np,
                                                         const uint8_t* __restrict__ done, const float* __restrict__ vreg,
                                                         const unsigned long long* __restrict__ off,
-                                                        long long nblk, uint32_t* __restrict__ out,
+                                                        long long nblk, int nv, uint32_t* __restrict__ out,
                                                         uint8_t* __restrict__ mask, float* __restrict__ vout) {
   __shared__ unsigned wsum[4];
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -639,11 +656,8 @@ __global__ __launch_bounds__(256) void k_partition_done(const uint32_t* __restri
   const unsigned long long dst = d ? (unsigned long long)np - ndone + pd : (unsigned long long)t - pd;
   out[dst] = pairs[t];
   mask[dst] = d ? 1 : 0;
-  if (d) {
-    vout[3 * dst] = vreg[3 * t];
-    vout[3 * dst + 1] = vreg[3 * t + 1];
-    vout[3 * dst + 2] = vreg[3 * t + 2];
-  }
+  if (d)
+    for (int c = 0; c < nv; ++c) vout[nv * dst + c] = vreg[nv * t + c];
 }
 
 // network and plain evanescent sums (unscaled, h = 1) at given points, with the
@@ -727,33 +741,41 @@ hipError_t sur_predict(hipStream_t s, long long n, const float* X, SurDev m, flo
   hipLaunchKernelGGL(k_predict, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, X, m, U);
   return hipGetLastError();
 }
-size_t sur_regular_bytes(long long np) {
+size_t sur_regular_bytes(long long np, bool sym) {
   const long long nblk = (np + 255) / 256;
-  return 30 * (size_t)np + 4 * (size_t)nblk + 8 * (size_t)(nblk + 1) + 64;
+  const size_t nv = sym ? 6 : 3;
+  return (8 * nv + 6) * (size_t)np + 4 * (size_t)nblk + 8 * (size_t)(nblk + 1) + 64;
 }
 
 hipError_t sur_regular_parts(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
                              long long np, SurDev m, float tol, void* work, unsigned long long* count,
-                             SurPartition& part) {
+                             SurPartition& part, bool sym) {
   part = SurPartition{pairs, nullptr, nullptr};
   if (np <= 0) return hipSuccess;
   if (m.quant && !m.img) return hipErrorInvalidValue;   // the MFMA kernels copy the image into LDS
   const long long nblk = (np + 255) / 256;
-  // work: vout (12 np) | vreg (12 np) | out (4 np) | done (np) | mask (np) | off (8 (nblk + 1)) | cnt (4 nblk)
+  const int nv = sym ? 6 : 3;
+  // work: vout (4 nv np) | vreg (4 nv np) | out (4 np) | done (np) | mask (np) | off (8 (nblk + 1)) | cnt (4 nblk)
   char* w = static_cast<char*>(work);
   float* vout = reinterpret_cast<float*>(w);
-  float* vreg = vout + 3 * np;
-  uint32_t* out = reinterpret_cast<uint32_t*>(vreg + 3 * np);
+  float* vreg = vout + (long long)nv * np;
+  uint32_t* out = reinterpret_cast<uint32_t*>(vreg + (long long)nv * np);
   uint8_t* done = reinterpret_cast<uint8_t*>(out + np);
   uint8_t* mask = done + np;
   unsigned long long* off = reinterpret_cast<unsigned long long*>(
       (reinterpret_cast<uintptr_t>(mask + np) + 7) & ~static_cast<uintptr_t>(7));
   unsigned* cnt = reinterpret_cast<unsigned*>(off + nblk + 1);
   const dim3 gd((unsigned)nblk), bd(256);
-  if (m.quant) hipLaunchKernelGGL(k_sur_regular<true>, gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
-  else hipLaunchKernelGGL(k_sur_regular<false>, gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  if (m.quant && sym)
+    hipLaunchKernelGGL((k_sur_regular<true, true>), gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  else if (m.quant)
+    hipLaunchKernelGGL((k_sur_regular<true, false>), gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  else if (sym)
+    hipLaunchKernelGGL((k_sur_regular<false, true>), gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
+  else
+    hipLaunchKernelGGL((k_sur_regular<false, false>), gd, bd, 0, s, g, f, kh, pairs, np, m, tol, done, vreg, cnt, count);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, nblk, off);
-  hipLaunchKernelGGL(k_partition_done, gd, bd, 0, s, pairs, np, done, vreg, off, nblk, out, mask, vout);
+  hipLaunchKernelGGL(k_partition_done, gd, bd, 0, s, pairs, np, done, vreg, off, nblk, nv, out, mask, vout);
   part = SurPartition{out, mask, vout};
   return hipGetLastError();
 }

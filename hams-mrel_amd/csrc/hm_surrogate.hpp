@@ -109,17 +109,20 @@ hipError_t sur_eval_points(hipStream_t s, int n, const double* rh, const double*
 // its regular part V = T - (1/r + 1/r1) (value, R- and z-derivative; T the
 // evanescent sum). The segment comes back partitioned (stable, by key order
 // within each part): part.pairs = the pairs left to the series, then those
-// taken; part.mask[t] = 1 and part.vreg[3 t .. 3 t + 2] = V for a taken pair.
-// The masked EIGENE assembly (assemble_regime with mask and vreg) writes A and
-// S for all of them. count += pairs taken. work: sur_regular_bytes(np) bytes.
+// taken; part.mask[t] = 1 and part.vreg[nv t .. nv t + 2] = V for a taken pair
+// (nv = 3). sym: each pair also stands for its mirror (depths swapped, for
+// assemble_sym): taken only if both directions are, nv = 6 with the mirror's V
+// in vreg[nv t + 3 ..]. The masked EIGENE assembly (assemble_regime /
+// assemble_sym with mask and vreg) writes A and S. count += pairs taken
+// (mirrors counted). work: sur_regular_bytes(np, sym) bytes.
 struct SurPartition {
   const uint32_t* pairs;
   const uint8_t* mask;
   const float* vreg;
 };
-size_t sur_regular_bytes(long long np);
+size_t sur_regular_bytes(long long np, bool sym = false);
 hipError_t sur_regular_parts(hipStream_t s, const GeomDev& g, const FreqDev& f, double kh, const uint32_t* pairs,
                              long long np, SurDev m, float tol, void* work, unsigned long long* count,
-                             SurPartition& part);
+                             SurPartition& part, bool sym = false);
 
 }  // namespace hm
