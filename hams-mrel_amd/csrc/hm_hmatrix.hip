@@ -119,6 +119,44 @@ __device__ __forceinline__ void residual_entry_direct(const GeomDev& g, const Fr
   }
 }
 
+// The entries of one workgroup in the order this lane evaluates: the 256
+// entries (row i, columns c0 + t; or rows c0 + t, column j) sorted by near flag
+// (the exact Rankine panel integral), regime and series length (bitonic sort in
+// LDS), so that the lane groups and waves take entries with equal trip counts.
+// Returns the entry index t this lane takes.
+__device__ __forceinline__ int sorted_entry(const GeomDev& g, const FreqDev& f, const HmatDev& h, const Proc& q,
+                                            bool row, int fixed, int c0, unsigned* sk) {
+  const int t = threadIdx.x, m = h.m;
+  unsigned key = 0xffffu;                   // past the end: last
+  if (c0 + t < m) {
+    key = 0;
+    if (f.h > 0.0 && !h.dense) {
+      const int i = q.b1 * m + (row ? fixed : c0 + t), j = q.b2 * m + (row ? c0 + t : fixed);
+      const double* xp = g.fp + 3 * (long long)i;
+      const double* xq = g.xyz_p + 3 * (long long)j;
+      const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+      const int reg = fin_regime(rr, f.h);
+      const int nt = reg == kEigene ? fin_eigene_terms(rr, f.h) : reg == kPidcock ? fin_pidcock_terms(rr, f.h) : 0;
+      key = (unsigned)(near_flag(xp, xq, g.pnsz[j]) * 512 + reg * 128 + nt);
+    }
+  }
+  sk[t] = (key << 8) | (unsigned)t;
+  __syncthreads();
+  for (int k = 2; k <= 256; k <<= 1)
+    for (int d = k >> 1; d > 0; d >>= 1) {
+      const int o = t ^ d;
+      if (o > t) {
+        const unsigned a = sk[t], b = sk[o];
+        if ((a > b) == ((t & k) == 0)) {
+          sk[t] = b;
+          sk[o] = a;
+        }
+      }
+      __syncthreads();
+    }
+  return (int)(sk[t] & 255u);
+}
+
 // residual pivot rows of every unconverged process, one entry per thread
 // (grid: column chunks x processes)
 // the entry kernels are compiled for 4 waves per SIMD (128 VGPRs, ~270 bytes a
@@ -131,10 +169,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
-  const int j0 = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned sk[256];
+  const Proc q = proc(h, p);
+  const int c0 = blockIdx.x * blockDim.x;
+  const int j0 = c0 + sorted_entry(g, f, h, q, true, h.next_row[p], c0, sk);
   const bool active = j0 < h.m;
   const int j = active ? j0 : h.m - 1;
-  const Proc q = proc(h, p);
   double er = 0.0, ei = 0.0;
   residual_entry(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
   if (!active) return;
@@ -199,10 +239,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   if (f.h > 0.0 && !h.dense)
     for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
   __syncthreads();
-  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ unsigned sk[256];
+  const Proc q = proc(h, p);
+  const int c0 = blockIdx.x * blockDim.x;
+  const int i0 = c0 + sorted_entry(g, f, h, q, false, h.pivot[p], c0, sk);
   const bool active = i0 < h.m;
   const int i = active ? i0 : h.m - 1;
-  const Proc q = proc(h, p);
   const int k = h.rank[p];
   double er = 0.0, ei = 0.0;
   residual_entry(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
