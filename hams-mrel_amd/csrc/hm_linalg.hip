@@ -289,6 +289,257 @@ __global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int
                  counter + blockIdx.z, gridDim.x, blockIdx.x);
 }
 
+// A whole panel [k0, k1) of a small matrix by one workgroup per matrix: for
+// each column the pivot search, the row swap, the scaling of L(:, j) and the
+// rank-1 update of the panel's columns (j, k1), with the panel in L2. Same
+// arithmetic per row as lu_col_body; one launch per panel instead of one per
+// column (the per-column launches of n <= 4096 matrices are latency bound).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_panel(T* A, int lda, long long sA, int n, int k0, int k1, int* ipiv,
+                                                  int* info) {
+  T* a = A + 2 * sA * blockIdx.z;
+  int* pv = ipiv + (long long)n * blockIdx.z;
+  int* inf = info + blockIdx.z;
+  __shared__ int prow;
+  __shared__ T pval;
+  __shared__ T u_re[kLuNb], u_im[kLuNb];
+  for (int j = k0; j < k1; ++j) {
+    block_pivot_search(a, lda, n, j, j, &prow, &pval);
+    block_apply_pivot(a, lda, j, prow, pval, k0, k1, pv, inf);
+    __syncthreads();
+    const int ncu = k1 - j - 1;
+    for (int c = threadIdx.x; c < ncu; c += blockDim.x) {
+      const long long off = (long long)j + (long long)(j + 1 + c) * lda;
+      u_re[c] = a[2 * off];
+      u_im[c] = a[2 * off + 1];
+    }
+    const long long od = (long long)j + (long long)j * lda;
+    const T dr = a[2 * od], di = a[2 * od + 1];
+    const T den = dr * dr + di * di;
+    T rr = 0, ri = 0;
+    if (den > T(0)) { rr = dr / den; ri = -di / den; }
+    __syncthreads();
+    for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
+      const long long ol = (long long)r + (long long)j * lda;
+      const T xr = a[2 * ol], xi = a[2 * ol + 1];
+      const T lr = xr * rr - xi * ri, li = xr * ri + xi * rr;
+      a[2 * ol] = lr;
+      a[2 * ol + 1] = li;
+      for (int c = 0; c < ncu; ++c) {
+        const long long oc = (long long)r + (long long)(j + 1 + c) * lda;
+        a[2 * oc] -= lr * u_re[c] - li * u_im[c];
+        a[2 * oc + 1] -= lr * u_im[c] + li * u_re[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// A whole panel [k0, k1) of an FP32 matrix of n <= 1024 R rows, one 1024-thread
+// workgroup per matrix, in sub-panels of 8 columns held in registers (thread t
+// owns rows t, t + 1024, ...): per column the pivot search (largest |re| + |im|,
+// smallest row on ties, as block_pivot_search), the swap and the rank-1 update
+// in registers; then the sub-panel's swaps on the panel's other columns, U12 =
+// L11^-1 A12 and one rank-8 update of the panel's remaining columns. The panel
+// goes through memory once per sub-panel instead of once per column.
+template <int R>
+__global__ __launch_bounds__(1024) void k_lu_panel_reg(float* A, int lda, long long sA, int n, int k0, int k1,
+                                                       int* ipiv, int* info) {
+  constexpr int SP = 8, NT = 1024;
+  float* a = A + 2 * sA * blockIdx.z;
+  int* pv = ipiv + (long long)n * blockIdx.z;
+  int* inf = info + blockIdx.z;
+  __shared__ float wv[NT / 64];
+  __shared__ int wi[NT / 64];
+  __shared__ int piv_s[SP];
+  __shared__ float pr_re[SP], pr_im[SP], xr_re[SP], xr_im[SP];
+  __shared__ float l11_re[SP][SP], l11_im[SP][SP];
+  __shared__ float u12_re[SP][kLuNb], u12_im[SP][kLuNb];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j0 = k0; j0 < k1; j0 += SP) {
+    const int w = k1 - j0 < SP ? k1 - j0 : SP;
+    float x_re[R][SP], x_im[R][SP];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = tid + NT * q;
+#pragma unroll
+      for (int c = 0; c < SP; ++c) {
+        const bool in = r < n && c < w;
+        const long long o = (long long)r + (long long)(j0 + c) * lda;
+        x_re[q][c] = in ? a[2 * o] : 0.0f;
+        x_im[q][c] = in ? a[2 * o + 1] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < SP; ++c) {
+      if (c >= w) break;
+      const int j = j0 + c;
+      // pivot: the largest |re| + |im| over rows >= j, the smallest row on ties
+      float best = -1.0f;
+      int bi = n;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = tid + NT * q;
+        if (r >= j && r < n) {
+          const float v = fabsf(x_re[q][c]) + fabsf(x_im[q][c]);
+          if (v > best) { best = v; bi = r; }
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o);
+        const int oi = __shfl_xor(bi, o);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+      __syncthreads();
+      if (tid == 0) {
+        float b = wv[0];
+        int i = wi[0];
+        for (int k = 1; k < NT / 64; ++k)
+          if (wv[k] > b || (wv[k] == b && wi[k] < i)) { b = wv[k]; i = wi[k]; }
+        piv_s[c] = i;
+        pv[j] = i;
+        if (!(b > 0.0f) && *inf == 0) *inf = j + 1;
+      }
+      __syncthreads();
+      const int p = piv_s[c];
+      // swap rows j and p of the sub-panel through LDS
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = tid + NT * q;
+        if (r == p) {
+#pragma unroll
+          for (int cc = 0; cc < SP; ++cc) { pr_re[cc] = x_re[q][cc]; pr_im[cc] = x_im[q][cc]; }
+        }
+        if (r == j) {
+#pragma unroll
+          for (int cc = 0; cc < SP; ++cc) { xr_re[cc] = x_re[q][cc]; xr_im[cc] = x_im[q][cc]; }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = tid + NT * q;
+        if (p != j && r == j) {
+#pragma unroll
+          for (int cc = 0; cc < SP; ++cc) { x_re[q][cc] = pr_re[cc]; x_im[q][cc] = pr_im[cc]; }
+        }
+        if (p != j && r == p) {
+#pragma unroll
+          for (int cc = 0; cc < SP; ++cc) { x_re[q][cc] = xr_re[cc]; x_im[q][cc] = xr_im[cc]; }
+        }
+      }
+      // the pivot row (row j after the swap) is pr; scale L(:, j), rank-1 update of the sub-panel
+      const float dr = pr_re[c], di = pr_im[c];
+      const float den = dr * dr + di * di;
+      float rr = 0.0f, ri = 0.0f;
+      if (den > 0.0f) { rr = dr / den; ri = -di / den; }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = tid + NT * q;
+        if (r > j && r < n) {
+          const float xr = x_re[q][c], xi = x_im[q][c];
+          const float lr = xr * rr - xi * ri, li = xr * ri + xi * rr;
+          x_re[q][c] = lr;
+          x_im[q][c] = li;
+#pragma unroll
+          for (int cc = c + 1; cc < SP; ++cc) {
+            x_re[q][cc] -= lr * pr_re[cc] - li * pr_im[cc];
+            x_im[q][cc] -= lr * pr_im[cc] + li * pr_re[cc];
+          }
+        }
+      }
+      __syncthreads();                        // pr / xr / wv reused by the next column
+    }
+    // the factored sub-panel back to memory; L11 (rows j0 .. j0 + w - 1) to LDS
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = tid + NT * q;
+      if (r >= j0 && r < n)
+#pragma unroll
+        for (int c = 0; c < SP; ++c)
+          if (c < w) {
+            const long long o = (long long)r + (long long)(j0 + c) * lda;
+            a[2 * o] = x_re[q][c];
+            a[2 * o + 1] = x_im[q][c];
+          }
+      if (r >= j0 && r < j0 + w)
+#pragma unroll
+        for (int c = 0; c < SP; ++c) { l11_re[r - j0][c] = x_re[q][c]; l11_im[r - j0][c] = x_im[q][c]; }
+    }
+    __syncthreads();
+    // the sub-panel's swaps on the panel's other columns, in order
+    const int nother = (k1 - k0) - w;
+    if (tid < nother) {
+      const int col = tid < j0 - k0 ? k0 + tid : j0 + w + (tid - (j0 - k0));
+      float* cp = a + 2 * (long long)col * lda;
+      for (int c = 0; c < w; ++c) {
+        const int j = j0 + c, p = piv_s[c];
+        if (p != j) {
+          const float tr = cp[2 * j], ti = cp[2 * j + 1];
+          cp[2 * j] = cp[2 * p];
+          cp[2 * j + 1] = cp[2 * p + 1];
+          cp[2 * p] = tr;
+          cp[2 * p + 1] = ti;
+        }
+      }
+    }
+    __syncthreads();
+    // U12 = L11^-1 A12 (unit lower L11), one thread per remaining panel column
+    const int nrest = k1 - j0 - w;
+    if (tid < nrest) {
+      const int col = j0 + w + tid;
+      float ur[SP], ui[SP];
+#pragma unroll
+      for (int c = 0; c < SP; ++c) {
+        if (c < w) {
+          const long long o = (long long)(j0 + c) + (long long)col * lda;
+          ur[c] = a[2 * o];
+          ui[c] = a[2 * o + 1];
+        } else {
+          ur[c] = ui[c] = 0.0f;
+        }
+      }
+#pragma unroll
+      for (int c = 1; c < SP; ++c)
+#pragma unroll
+        for (int k = 0; k < c; ++k) {
+          ur[c] -= l11_re[c][k] * ur[k] - l11_im[c][k] * ui[k];
+          ui[c] -= l11_re[c][k] * ui[k] + l11_im[c][k] * ur[k];
+        }
+#pragma unroll
+      for (int c = 0; c < SP; ++c) {
+        if (c < w) {
+          const long long o = (long long)(j0 + c) + (long long)col * lda;
+          a[2 * o] = ur[c];
+          a[2 * o + 1] = ui[c];
+        }
+        u12_re[c][tid] = ur[c];
+        u12_im[c][tid] = ui[c];
+      }
+    }
+    __syncthreads();
+    // rank-w update of rows >= j0 + w, columns [j0 + w, k1): A22 -= L21 U12
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = tid + NT * q;
+      if (r >= j0 + w && r < n)
+        for (int t = 0; t < nrest; ++t) {
+          const long long o = (long long)r + (long long)(j0 + w + t) * lda;
+          float vr = a[2 * o], vi = a[2 * o + 1];
+#pragma unroll
+          for (int c = 0; c < SP; ++c) {
+            vr -= x_re[q][c] * u12_re[c][t] - x_im[q][c] * u12_im[c][t];
+            vi -= x_re[q][c] * u12_im[c][t] + x_im[q][c] * u12_re[c][t];
+          }
+          a[2 * o] = vr;
+          a[2 * o + 1] = vi;
+        }
+    }
+    __syncthreads();
+  }
+}
+
 // apply the panel's swaps (rows [k0,k1)) to columns [0,k0) and [k1,n)
 template <typename T>
 __device__ __forceinline__ void laswp_body(T* a, int lda, int n, int k0, int k1, const int* pv, int c) {
@@ -379,11 +630,20 @@ hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipi
                      unsigned* counter, int batch) {
   for (int k0 = 0; k0 < n; k0 += kLuNb) {
     const int k1 = k0 + kLuNb < n ? k0 + kLuNb : n;
-    hipLaunchKernelGGL((k_lu_pivot<T>), dim3(1, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, k0, k1, ipiv, info);
-    for (int j = k0; j < k1; ++j) {
-      const int rows = n - j - 1;
-      const int gx = rows > 0 ? (rows + 255) / 256 : 1;
-      hipLaunchKernelGGL((k_lu_col<T>), dim3(gx, 1, batch), dim3(256), 0, s, A, lda, sA, n, j, k0, k1, ipiv, info, counter);
+    if (sizeof(T) == sizeof(float) && n <= 2048) {
+      float* Af = reinterpret_cast<float*>(A);
+      if (n <= 1024) hipLaunchKernelGGL((k_lu_panel_reg<1>), dim3(1, 1, batch), dim3(1024), 0, s, Af, lda, sA, n, k0, k1, ipiv, info);
+      else hipLaunchKernelGGL((k_lu_panel_reg<2>), dim3(1, 1, batch), dim3(1024), 0, s, Af, lda, sA, n, k0, k1, ipiv, info);
+    } else if (sizeof(T) == sizeof(float) && n <= 4096) {
+      hipLaunchKernelGGL((k_lu_panel<T>), dim3(1, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, k1, ipiv, info);
+    } else {
+      hipLaunchKernelGGL((k_lu_pivot<T>), dim3(1, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, k0, k1, ipiv, info);
+      for (int j = k0; j < k1; ++j) {
+        const int rows = n - j - 1;
+        const int gx = rows > 0 ? (rows + 255) / 256 : 1;
+        hipLaunchKernelGGL((k_lu_col<T>), dim3(gx, 1, batch), dim3(256), 0, s, A, lda, sA, n, j, k0, k1, ipiv, info,
+                           counter);
+      }
     }
     const int other = n - (k1 - k0);
     if (other > 0)
