@@ -338,13 +338,24 @@ def main():
     if prec == hm_bem.HM_GMRES:
         ctx.set_gmres(restart=50, tol=args.gmres_tol)
     hmat = None
+    hm_setup_s = None
     if args.path == "hmatrix":
+        # once per geometry and depth, outside the timed region: the diagonal
+        # blocks' pair classification and the frequency-independent halves of
+        # their PIDCOCK terms
+        sync()
+        t0 = time.perf_counter()
         ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
+        sync()
+        hm_setup_s = time.perf_counter() - t0
+        print(f"# hmatrix set-up (diagonal-block classification, PIDCOCK term cache) {hm_setup_s:.2f} s",
+              file=sys.stderr, flush=True)
     dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup)
     gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
     hist_mx = ctx.pair_histogram().ravel()     # pairs assembled densely on the measured path
     if args.path == "hmatrix":
-        hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
+        hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
+                    setup_s=hm_setup_s)
         ctx.set_hmatrix(False)
         print(f"# hmatrix {hmat}", file=sys.stderr, flush=True)
     if sur is not None:

@@ -102,6 +102,13 @@ struct hm_ctx {
   long long sur_counts[3] = {0, 0, 0};
   // H-matrix diagonal blocks: PIDCOCK pairs assembled together with their mirrors
   bool pair_sym = true;
+  // the frequency-independent halves of those pairs' PIDCOCK terms (built with
+  // the classification; off when the device memory for it is not available)
+  DevBuf d_pcache, d_ptab;
+  bool pcache_on = false;
+  hm::PidCache pcache() const {
+    return pcache_on ? hm::PidCache{d_pcache.as<double>(), d_ptab.as<long long>()} : hm::PidCache{nullptr, nullptr};
+  }
   void* sur_img() const { return static_cast<char*>(d_sur.p) + hm::kSurImgOffset; }
   hm::SurDev sur_dev() const {
     const float* b = d_sur.as<float>();
@@ -290,7 +297,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_sur, &c->d_fb, &c->d_fbcnt};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -402,6 +409,29 @@ static int classify(hm_ctx* c) {
   }
   HIPCHK(c, hipMemcpyAsync(c->d_cursor.p, cur.data(), sizeof(unsigned long long) * hm::kNumKeys, hipMemcpyHostToDevice, c->stream), "classify");
   HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false, blk, sym), "classify");
+  // mixed path (mirrored pairs): cache the i pi / h halves of the PIDCOCK terms
+  c->pcache_on = false;
+  if (sym && c->seg_cnt[hm::kPidcock] > 0) {
+    std::vector<long long> tab(2 * hm::kKeyStride, 0);
+    long long rec = 0;
+    for (int len = 0; len < hm::kKeyStride; ++len) {
+      const int k = hm::kPidcock * hm::kKeyStride + len;
+      tab[2 * len] = (long long)cur[k] - c->seg_off[hm::kPidcock];
+      tab[2 * len + 1] = rec;
+      rec += (long long)hist[k] * len;
+    }
+    const size_t bytes = 32 * (size_t)rec;
+    if (c->d_pcache.ensure(bytes) == hipSuccess && c->d_ptab.ensure(sizeof(long long) * tab.size()) == hipSuccess) {
+      HIPCHK(c, hipMemcpyAsync(c->d_ptab.p, tab.data(), sizeof(long long) * tab.size(), hipMemcpyHostToDevice,
+                               c->stream), "classify");
+      c->pcache_on = true;
+      HIPCHK(c, hm::pidcock_cache_build(c->stream, gd, h, c->d_pairs.as<uint32_t>() + c->seg_off[hm::kPidcock],
+                                        c->seg_cnt[hm::kPidcock], c->pcache()), "classify");
+    } else {
+      (void)hipGetLastError();              // not enough device memory: evaluate the terms in full
+      c->d_pcache.release();
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
   return c->stage_end(kClassify);
 }
@@ -500,7 +530,9 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
       if (c->seg_cnt[r] == 0) continue;
       c->stage_begin(kAsmRegime0 + r);
       if (sym && (r == hm::kPidcock || r == hm::kEigene))
-        HIPCHK(c, hm::assemble_sym(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld), "assemble");
+        HIPCHK(c, hm::assemble_sym(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld, nullptr,
+                                   nullptr, r == hm::kPidcock ? c->pcache() : hm::PidCache{nullptr, nullptr}),
+               "assemble");
       else
         HIPCHK(c, hm::assemble_regime(c->stream, r, gd, f, pairs + c->seg_off[r], c->seg_cnt[r], A, S, ld,
                                       c->series_variant), "assemble");

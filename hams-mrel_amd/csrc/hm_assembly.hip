@@ -183,10 +183,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
 // instead of six and six), then every lane finishes (i, j) and (j, i). EIGENE
 // with a mask: pairs with done[t] != 0 take the regular parts of both
 // directions from vreg[6 t .. 6 t + 5] instead.
-template <int REG, int G, int K>
+template <int REG, int G, int K, bool CACHED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K>::value, 8))) void
 k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long long np, double* A, double* S,
-               long long ld, const uint8_t* __restrict__ done, const float* __restrict__ vreg) {
+               long long ld, const uint8_t* __restrict__ done, const float* __restrict__ vreg, PidCache pc) {
   __shared__ double wvn[kNk];
   load_wvn(wvn, f.wvn, kNk);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -200,6 +200,9 @@ k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long lo
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const double zf = xq[2], zp = xp[2];
   const int nt = !valid || given ? 0 : REG == kEigene ? fin_eigene_terms(rr, h) : fin_pidcock_terms(rr, h);
+  // the pair's record in the PIDCOCK cache (rows of nt x 4 doubles; pairs sorted by nt)
+  long long rec = 0;
+  if (CACHED && valid) rec = pc.tab[2 * nt + 1] + (t - pc.tab[2 * nt]) * nt;
   const int l = threadIdx.x & (G - 1);
   double L1 = 0.0, L2 = 0.0, L3 = 0.0, L4 = 0.0;
   for (int q = 0; q < G; ++q) {
@@ -207,8 +210,13 @@ k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long lo
     if (ntq == 0) continue;                  // uniform over the group
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
+    const double* tc = nullptr;
+    if (CACHED) {
+      const unsigned lo = __shfl((unsigned)(rec & 0xffffffffll), q, G), hi = __shfl((unsigned)(rec >> 32), q, G);
+      tc = pc.data + 4 * (long long)(((unsigned long long)hi << 32) | lo);
+    }
     double l1, l2, l3, l4;
-    coop_series_sym<REG, G, K>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3, l4);
+    coop_series_sym<REG, G, K, CACHED>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3, l4, tc);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; L4 = l4; }
   }
   if (!valid) return;
@@ -222,14 +230,47 @@ k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long lo
   coop_finish<REG>(g, wvn, h, j, i, rr, L1, L2, L4, A, S, ld);
 }
 
+// the i pi / h halves of every term of every pair of a PIDCOCK segment sorted
+// by series length (thread per pair and row)
+__global__ __launch_bounds__(256) void k_pidcock_cache(GeomDev g, double h, const uint32_t* __restrict__ pairs,
+                                                       long long np, PidCache pc) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = e / kPidcockMaxTerms;
+  const int row = (int)(e % kPidcockMaxTerms);
+  if (t >= np) return;
+  const uint32_t p = pairs[t];
+  const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const int nt = fin_pidcock_terms(rr, h);
+  if (row < 1 || row >= nt) return;
+  const long long rec = pc.tab[2 * nt + 1] + (t - pc.tab[2 * nt]) * nt;
+  fin_pidcock_tilde(row, rr, xq[2] + h, xp[2] + h, h, pc.data + 4 * (rec + row));
+}
+
+hipError_t pidcock_cache_build(hipStream_t s, const GeomDev& g, double h, const uint32_t* pairs, long long np,
+                               PidCache pc) {
+  if (np <= 0 || !pc.data) return hipSuccess;
+  const long long work = np * kPidcockMaxTerms;
+  hipLaunchKernelGGL(k_pidcock_cache, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, g, h, pairs, np, pc);
+  return hipGetLastError();
+}
+
 hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
-                        long long np, double* A, double* S, long long ld, const uint8_t* done, const float* vreg) {
+                        long long np, double* A, double* S, long long ld, const uint8_t* done, const float* vreg,
+                        PidCache pc) {
   if (np <= 0) return hipSuccess;
   const dim3 gd((unsigned)((np + 255) / 256)), bd(256);
-  if (regime == kPidcock)
-    hipLaunchKernelGGL((k_assemble_sym<kPidcock, 16, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, nullptr, nullptr);
+  if (regime == kPidcock && pc.data)
+    hipLaunchKernelGGL((k_assemble_sym<kPidcock, 16, 7, true>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, nullptr,
+                       nullptr, pc);
+  else if (regime == kPidcock)
+    hipLaunchKernelGGL((k_assemble_sym<kPidcock, 16, 7, false>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, nullptr,
+                       nullptr, pc);
   else if (regime == kEigene)
-    hipLaunchKernelGGL((k_assemble_sym<kEigene, 8, 7>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg);
+    hipLaunchKernelGGL((k_assemble_sym<kEigene, 8, 7, false>), gd, bd, 0, s, g, f, pairs, np, A, S, ld, done, vreg,
+                       PidCache{nullptr, nullptr});
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -36,9 +36,20 @@ hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* h
 // (field points = panel centroids, ni = 0): one series evaluation serves both.
 // done, vreg (EIGENE, optional): pairs with done[t] != 0 take the regular parts
 // of (i, j) and (j, i) from vreg[6 t .. 6 t + 5]
+// The frequency-independent halves of the PIDCOCK terms (the i pi / h modes:
+// K0, K1 of i pi R / h and the depth cosines) of a PIDCOCK segment sorted by
+// series length nt, once per geometry and depth: 4 doubles per (pair, row),
+// pair t of length nt at record tab[2 nt + 1] + (t - tab[2 nt]) nt (tab[2 nt]:
+// the first pair of length nt; tab: 2 x 128 entries on the device).
+struct PidCache {
+  double* data;
+  const long long* tab;
+};
+hipError_t pidcock_cache_build(hipStream_t s, const GeomDev& g, double h, const uint32_t* pairs, long long np,
+                               PidCache pc);
 hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqDev& f, const uint32_t* pairs,
                         long long np, double* A, double* S, long long ld, const uint8_t* done = nullptr,
-                        const float* vreg = nullptr);
+                        const float* vreg = nullptr, PidCache pc = PidCache{nullptr, nullptr});
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
                          long long ld, int blk = 0);
 // done, vreg (optional, EIGENE only): pairs with done[t] != 0 take the regular

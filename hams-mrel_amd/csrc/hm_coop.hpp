@@ -331,10 +331,11 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
 // PIDCOCK or EIGENE series of one pair and of its mirror (depths swapped) by
 // the whole group: the three limits of the pair and l4, the limit of the series
 // of the zph-derivatives (the mirror's l3; its l1, l2 are the pair's)
-template <int REG, int G, int K>
+// tc (PIDCOCK, optional): the pair's cached i pi / h halves, 4 doubles per row
+template <int REG, int G, int K, bool CACHED = false>
 __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, double zph, double h, double v,
                                                 const double* wvn, int l, double& l1, double& l2, double& l3,
-                                                double& l4) {
+                                                double& l4, const double* tc = nullptr) {
   typedef LaneGroup<G> LG;
   double s1[K], s2[K], s3[K], s4[K];
   double c1 = 0.0, c2 = 0.0, c3 = 0.0, c4 = 0.0;
@@ -345,6 +346,7 @@ __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, d
     if (G * k < ntq) {
       if (row >= 1 && row < ntq) {
         if (REG == kEigene) fin_evanescent_sym(wvn[row], r, zfh, zph, h, t1, t2, t3, t4);
+        else if (CACHED) fin_pidcock_term_cached(wvn[row], r, zfh, zph, v, h, tc + 4 * row, t1, t2, t3, t4);
         else fin_pidcock_term_sym(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3, t4);
       }
       t1 = LG::incl_sum(t1, l) + c1;

@@ -520,6 +520,41 @@ HM_DEV void fin_pidcock_term_sym(int i, double um, double r, double zfh, double 
   t3 = dam * k0 - damt * k0t;
   t4 = damp * k0 - damtp * k0t;
 }
+// The frequency-independent half of the PIDCOCK term of row i (the i pi / h
+// mode): c[0] = amt k0t, c[1] = umt amt k1t, c[2] = damt k0t, c[3] = damtp k0t
+// (depends on R, the depths and h only; cached once per geometry)
+HM_DEV void fin_pidcock_tilde(int i, double r, double zfh, double zph, double h, double* c) {
+  const double umt = i * kPi / h;
+  double k0t, k1t;
+  sf_k01(umt * r, k0t, k1t);
+  double sut, cut;
+  sincos(umt * zfh, &sut, &cut);
+  const double cutp = cos(umt * zph), sutp = sin(umt * zph);
+  const double amt = cut * cutp / h;
+  const double damt = -umt * sut * cutp / h;
+  const double damtp = -umt * cut * sutp / h;
+  c[0] = amt * k0t;
+  c[1] = umt * amt * k1t;
+  c[2] = damt * k0t;
+  c[3] = damtp * k0t;
+}
+// fin_pidcock_term_sym with the i pi / h half from the cache
+HM_DEV void fin_pidcock_term_cached(double um, double r, double zfh, double zph, double v, double h, const double* c,
+                                    double& t1, double& t2, double& t3, double& t4) {
+  double k0, k1;
+  sf_k01(um * r, k0, k1);
+  const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
+  double su, cu;
+  sincos(um * zfh, &su, &cu);
+  const double cup = cos(um * zph), sup = sin(um * zph);
+  const double am = ac * cu * cup;
+  const double dam = -um * ac * su * cup;
+  const double damp = -um * ac * cu * sup;
+  t1 = am * k0 - c[0];
+  t2 = -um * am * k1 + c[1];
+  t3 = dam * k0 - c[2];
+  t4 = damp * k0 - c[3];
+}
 HM_DEV void fin_pidcock_tail(double l1, double l2, double l3, double r, double zf, double zp, double h, FinOut& o) {
   double ig, igr, igz;
   fin_images(r, zf, zp, h, ig, igr, igz);
