@@ -49,10 +49,12 @@ REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen":
 # counter passes of `bench.py --steps 1 --assemble-only` (rocprofv3 --pmc
 # FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter k_assemble|k_aca_eval), scaled
 # by the readings of a 1 GiB device-to-device copy (fetch x2, write x1). The
-# lane-group kernels run at 5 waves per SIMD with register spills; the spill
-# traffic (scratch reads and writes that miss the L2) is most of these bytes.
+# H-matrix path's PIDCOCK kernel assembles pairs with their mirrors from the
+# cached frequency-independent term halves (31 GB read per launch); it runs at 5
+# waves per SIMD, and register-spill traffic (scratch that misses the L2) is
+# most of the rest.
 MEASURED_TRAFFIC = {
-    ("c5", "hmatrix", "asm_pidcock"): 46981128.3 * 1024 * 2 + 56137415.2 * 1024,
+    ("c5", "hmatrix", "asm_pidcock"): 48999487.7 * 1024 * 2 + 50866974.7 * 1024,
 }
 
 
@@ -410,11 +412,14 @@ def main():
                 "traffic_gbs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
                 "avg_ms": dom_ms, "pairs_per_launch": units, "algorithmic_flops_per_launch": flops,
                 "algorithmic_bytes_per_launch": 36 * units,
-                "note": ("FP64 VALU-bound Green-function assembly over the classified pairs of one regime "
-                         "(PIDCOCK and EIGENE series evaluated by lane groups of 16 and 8); peak = MI355X FP64 "
-                         "vector peak (= FP64 matrix peak); flops per pair counted by the oracle "
-                         "(tests/golden/flop_model.json); traffic = measured HBM bytes per launch, "
-                         "algorithmic bytes = 32 B of A and S + 4 B pair index per pair")}
+                "note": ("FP64 VALU-bound Green-function assembly over the classified pairs of one regime; "
+                         "peak = MI355X FP64 vector peak (= FP64 matrix peak); algorithmic flops = the reference's "
+                         "per-pair work counted by the oracle (tests/golden/flop_model.json) for every pair the "
+                         "launch assembles; on the H-matrix path the PIDCOCK kernel assembles each pair with its "
+                         "mirror and reads the frequency-independent halves of its terms from a per-geometry "
+                         "cache, executing 41 % of the FP64 instructions of a pair-by-pair evaluation; traffic = "
+                         "measured HBM bytes per launch, algorithmic bytes = 32 B of A and S + 4 B pair index "
+                         "per pair")}
 
     line = {"metric": METRIC, "value": args.steps / dt_mx, "unit": "systems/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt_mx / args.steps,
