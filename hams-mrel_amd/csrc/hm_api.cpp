@@ -65,6 +65,8 @@ struct hm_ctx {
   double rho = 1025.0, g = 9.81, g_disp = 9.81, amp = 1.0, xw = 0.0, yw = 0.0;
   // geometry
   int n = 0, ni = 0, nbody = 0;
+  // smallest horizontal gap between the bodies' bounding circles (panel centroids)
+  double min_body_gap = 0.0;
   std::vector<int> body_start;
   DevBuf d_body, d_fp, d_verts, d_ncn, d_ds, d_pnsz, d_dxyz;
   bool have_geom = false;
@@ -355,6 +357,21 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   if (ni) memcpy(fp.data() + 3 * (size_t)n, ixyz_p, sizeof(double) * 3 * (size_t)ni);
   c->n = n; c->ni = ni; c->nbody = nbody;
   c->body_start.assign(body_start, body_start + nbody + 1);
+  {
+    std::vector<double> cx(nbody, 0.0), cy(nbody, 0.0), rad(nbody, 0.0);
+    for (int b = 0; b < nbody; ++b) {
+      const int j0 = body_start[b], j1 = body_start[b + 1];
+      for (int j = j0; j < j1; ++j) { cx[b] += xyz_p[3 * (size_t)j]; cy[b] += xyz_p[3 * (size_t)j + 1]; }
+      cx[b] /= (j1 - j0);
+      cy[b] /= (j1 - j0);
+      for (int j = j0; j < j1; ++j)
+        rad[b] = std::max(rad[b], std::hypot(xyz_p[3 * (size_t)j] - cx[b], xyz_p[3 * (size_t)j + 1] - cy[b]));
+    }
+    c->min_body_gap = nbody > 1 ? 1e300 : 0.0;
+    for (int b1 = 0; b1 < nbody; ++b1)
+      for (int b2 = b1 + 1; b2 < nbody; ++b2)
+        c->min_body_gap = std::min(c->min_body_gap, std::hypot(cx[b1] - cx[b2], cy[b1] - cy[b2]) - rad[b1] - rad[b2]);
+  }
   int rc;
   if ((rc = upload(c, c->d_body, body_start, nbody + 1, "geometry")) ||
       (rc = upload(c, c->d_fp, fp.data(), fp.size(), "geometry")) ||
@@ -595,6 +612,8 @@ static int assemble_hmatrix(hm_ctx* c) {
   c->stage_begin(kAssembly);
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
+  // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
+  c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
   c->stage_begin(kAsmCross);
   HIPCHK(c, hm::aca_build(c->stream, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
   c->stage_end(kAsmCross);

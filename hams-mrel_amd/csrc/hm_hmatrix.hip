@@ -85,6 +85,7 @@ __device__ void block_sum4(double* v, double (*sh)[256]) {
 // Called by every lane of the workgroup (the Green function's series run in
 // lane groups of 16); a lane with active = false gets nothing. LINTON entries
 // come out wrong here and are redone by residual_entry_direct (k_aca_fix_*).
+template <int G, bool PID>
 __device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const HmatDev& h,
                                                const Proc& q, int m, int k, int i, int j, bool active, double& er,
                                                double& ei) {
@@ -93,7 +94,7 @@ __device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& 
     entry(g, f, wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
   } else {
     double ar, ai, sr, si;
-    pair_entry_coop<16>(g, f, wvn, active, q.b1 * m + i, q.b2 * m + j, ar, ai, sr, si);
+    pair_entry_coop<G, PID>(g, f, wvn, active, q.b1 * m + i, q.b2 * m + j, ar, ai, sr, si);
     if (!active) return;
     er = q.mat ? sr : ar;
     ei = q.mat ? si : ai;
@@ -161,6 +162,7 @@ __device__ __forceinline__ int sorted_entry(const GeomDev& g, const FreqDev& f, 
 // (grid: column chunks x processes)
 // the entry kernels are compiled for 4 waves per SIMD (128 VGPRs, ~270 bytes a
 // lane of cached spills): 21 % faster than at their natural 208 VGPRs
+template <int G, bool PID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_row(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
   const int p = blockIdx.y;
@@ -176,7 +178,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const bool active = j0 < h.m;
   const int j = active ? j0 : h.m - 1;
   double er = 0.0, ei = 0.0;
-  residual_entry(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
+  residual_entry<G, PID>(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
   if (!active) return;
   double* rb = h.rowbuf + 2LL * p * h.m;
   rb[2 * j] = er;
@@ -231,6 +233,7 @@ __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
 }
 
 // residual pivot columns u_k = A(:, j*) - U V(j*, :)^T, one entry per thread
+template <int G, bool PID>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_col(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
   const int p = blockIdx.y;
@@ -247,14 +250,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const int i = active ? i0 : h.m - 1;
   const int k = h.rank[p];
   double er = 0.0, ei = 0.0;
-  residual_entry(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
+  residual_entry<G, PID>(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
   if (!active) return;
   q.U[2 * (i + (long long)k * h.m)] = er;
   q.U[2 * (i + (long long)k * h.m) + 1] = ei;
 }
 
 // LINTON entries of the rows / columns just evaluated (R/h < 5e-4: vertically
-// aligned panels of two bodies; rare), one lane each
+// aligned panels of two bodies; rare), and PIDCOCK entries when the
+// EIGENE-only entry kernels ran, one lane each
 __global__ __launch_bounds__(256) void k_aca_fix_row(GeomDev g, FreqDev f, HmatDev h) {
   const int p = blockIdx.y;
   if (h.dense || h.conv[p]) return;
@@ -262,7 +266,7 @@ __global__ __launch_bounds__(256) void k_aca_fix_row(GeomDev g, FreqDev f, HmatD
   if (j >= h.m) return;
   const Proc q = proc(h, p);
   const int i = h.next_row[p];
-  if (!linton_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j)) return;
+  if (!deferred_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j, !h.narrow)) return;
   double er, ei;
   residual_entry_direct(g, f, h, q, h.m, h.rank[p], i, j, er, ei);
   double* rb = h.rowbuf + 2LL * p * h.m;
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(256) void k_aca_fix_col(GeomDev g, FreqDev f, HmatD
   if (i >= h.m) return;
   const Proc q = proc(h, p);
   const int k = h.rank[p], j = h.pivot[p];
-  if (!linton_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j)) return;
+  if (!deferred_entry(g, f, q.b1 * h.m + i, q.b2 * h.m + j, !h.narrow)) return;
   double er, ei;
   residual_entry_direct(g, f, h, q, h.m, k, i, j, er, ei);
   q.U[2 * (i + (long long)k * h.m)] = er;
@@ -435,6 +439,7 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.row_used = (int*)take(4 * P * m);
   h.col_used = (int*)take(4 * P * m);
   h.counter = (int*)take(4);
+  h.narrow = 0;
   return h;
 }
 
@@ -459,10 +464,12 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   const int max_steps = h.kmax + h.m;
   for (; st < max_steps && open > 0; ++st) {
     const dim3 eval((h.m + 255) / 256, P);
-    hipLaunchKernelGGL(k_aca_eval_row, eval, dim3(256), 0, s, g, f, h);
+    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false>), eval, dim3(256), 0, s, g, f, h);
+    else hipLaunchKernelGGL((k_aca_eval_row<16, true>), eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
-    hipLaunchKernelGGL(k_aca_eval_col, eval, dim3(256), 0, s, g, f, h);
+    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false>), eval, dim3(256), 0, s, g, f, h);
+    else hipLaunchKernelGGL((k_aca_eval_col<16, true>), eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -45,6 +45,10 @@ struct HmatDev {
   int *rank, *conv, *next_row, *pivot, *skip;   // [process]
   int *row_used, *col_used;                     // [process][m]
   int* counter;                                 // one int for host polls
+  // 1: the bodies' horizontal bounding circles are >= 0.05 h apart, so no
+  // off-diagonal entry is PIDCOCK or LINTON; the entry kernels then take the
+  // EIGENE-only lane groups of 8 (any such entry would still be redone per lane)
+  int narrow;
 };
 
 inline int hmat_processes(int nb, int nmat = 2) { return nmat * nb * (nb - 1); }

@@ -62,16 +62,19 @@ __device__ __forceinline__ void pair_entry(const GeomDev& g, const FreqDev& f, c
 // infinite depth per lane as in pair_entry. All lanes call it; a lane with
 // active = false (i, j must still name panels) takes part in its group's
 // series and gets zeros. LINTON entries (R/h < 5e-4, whose private arrays
-// would spill the whole kernel) are left out: zeros and linton_entry() true;
-// callers evaluate them with pair_entry.
-__device__ __forceinline__ bool linton_entry(const GeomDev& g, const FreqDev& f, int i, int j) {
+// would spill the whole kernel) are left out: zeros and deferred_entry() true;
+// callers evaluate them with pair_entry. PID = false: PIDCOCK entries are left
+// out too (an EIGENE-only kernel, for entries that are PIDCOCK only by
+// exception).
+__device__ __forceinline__ bool deferred_entry(const GeomDev& g, const FreqDev& f, int i, int j, bool pid) {
   if (!(f.h > 0.0) || fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8)) return false;
   const double* xp = g.fp + 3 * (long long)i;
   const double* xq = g.xyz_p + 3 * (long long)j;
-  return fin_regime(horiz_dist(xq[0], xp[0], xq[1], xp[1]), f.h) == kLinton;
+  const int reg = fin_regime(horiz_dist(xq[0], xp[0], xq[1], xp[1]), f.h);
+  return reg == kLinton || (!pid && reg == kPidcock);
 }
 
-template <int G>
+template <int G, bool PID = true>
 __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev& f, const double* wvn, bool active,
                                                 int i, int j, double& ar, double& ai, double& sr, double& si) {
   constexpr int KP = (kPidcockMaxTerms + G - 1) / G, KE = (kEigeneMaxTerms + G - 1) / G;
@@ -82,7 +85,9 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
   const double zf = xq[2], zp = xp[2];
   const bool zero = fabs(f.v) < F32(1.e-8), inf = !zero && fabs(f.v + 1.0) < F32(1.e-8);
   const int reg = h > 0.0 && !zero && !inf ? fin_regime(rr, h) : -1;
-  const int nt = !active ? 0 : reg == kPidcock ? fin_pidcock_terms(rr, h) : reg == kEigene ? fin_eigene_terms(rr, h) : 0;
+  const int nt = !active ? 0
+                 : reg == kPidcock ? (PID ? fin_pidcock_terms(rr, h) : 0)
+                 : reg == kEigene ? fin_eigene_terms(rr, h) : 0;
   const int l = threadIdx.x & (G - 1);
   double L1 = 0.0, L2 = 0.0, L3 = 0.0;
   for (int q = 0; q < G; ++q) {
@@ -92,7 +97,7 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
     double l1, l2, l3;
-    if (rq == kPidcock) coop_series<kPidcock, G, KP>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
+    if (PID && rq == kPidcock) coop_series<kPidcock, G, KP>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
     else coop_series<kEigene, G, KE>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
   }
@@ -109,6 +114,7 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
     FinOut fo{0, 0, 0, 0, 0, 0};
     if (!zero) {
       if (reg == kPidcock) {
+        if (!PID) return;                     // deferred
         fin_pidcock_head(rr, zf, zp, wvn, h, fo);
         fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
       } else if (reg == kEigene) {
