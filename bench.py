@@ -283,6 +283,9 @@ def main():
     ap.add_argument("--surrogate-samples", type=int, default=16_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--gmres-tol", type=float, default=1e-6)
+    ap.add_argument("--gmres-precond", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="block preconditioner: 1 triangular solves, 2 explicit FP32 inverses, 0 none, "
+                         "-1 library default")
     ap.add_argument("--series-variant", type=int, default=1, choices=(0, 1, 2),
                     help="epsilon-accelerated regimes: 1 lane groups of 8/16 (default), 2 of 4/8, 0 one pair per lane")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
@@ -338,7 +341,7 @@ def main():
     # (default), or FP32 LU on MFMA + FP64 refinement
     prec = {"hmatrix": hm_bem.HM_GMRES, "gmres": hm_bem.HM_GMRES, "fp32ir": hm_bem.HM_FP32_IR}[args.path]
     if prec == hm_bem.HM_GMRES:
-        ctx.set_gmres(restart=50, tol=args.gmres_tol)
+        ctx.set_gmres(restart=50, tol=args.gmres_tol, precond=args.gmres_precond)
     hmat = None
     hm_setup_s = None
     if args.path == "hmatrix":
@@ -374,17 +377,26 @@ def main():
     if not args.no_plain:
         dt_pl, res_pl, st_pl = time_path(ctx, prob, hm_bem.HM_FP64, args.steps, args.warmup)
         print(f"# fp64: {args.steps} systems in {dt_pl:.3f} s; stages {st_pl}", file=sys.stderr, flush=True)
-        nw, ew = {}, {}
+        nw, ew, ew3, inbar, cnt = {}, {}, {}, {}, {}
         for a, b in zip(res_mx, res_pl):
             for k in ("amas", "bdmp", "exfc"):
                 mx = np.max(np.abs(b[k]))
-                nw[k] = max(nw.get(k, 0.0), float(np.max(np.abs(a[k] - b[k])) / mx))
+                d = np.abs(a[k] - b[k])
+                nw[k] = max(nw.get(k, 0.0), float(np.max(d) / mx))
                 # entries below 1e-6 of the largest are compared absolutely, against 1e-6 * max
                 den = np.maximum(np.abs(b[k]), 1e-6 * mx)
-                ew[k] = max(ew.get(k, 0.0), float(np.max(np.abs(a[k] - b[k]) / den)))
+                ew[k] = max(ew.get(k, 0.0), float(np.max(d / den)))
+                # entries of at least 1e-3 of the largest (the cross couplings that are
+                # not near a zero crossing)
+                big = np.abs(b[k]) >= 1e-3 * mx
+                ew3[k] = max(ew3.get(k, 0.0), float(np.max(d[big] / np.abs(b[k][big]))))
+                inbar[k] = inbar.get(k, 0) + int(np.count_nonzero(d <= 1e-4 * den))
+                cnt[k] = cnt.get(k, 0) + d.size
         plain = {"value": args.steps / dt_pl, "ms_per_step": 1e3 * dt_pl / args.steps,
                  "speedup_mixed_vs_plain": (args.steps / dt_mx) / (args.steps / dt_pl),
                  "rel_err_mixed_vs_fp64_normwise": nw, "rel_err_mixed_vs_fp64_entrywise": ew,
+                 "rel_err_mixed_vs_fp64_entries_above_1e-3_max": ew3,
+                 "fraction_of_entries_within_1e-4": {k: inbar[k] / cnt[k] for k in cnt},
                  "max_rel_err_mixed_vs_fp64": max(nw.values()),
                  "stages_ms": {k: v[0] for k, v in st_pl.items()}}
 

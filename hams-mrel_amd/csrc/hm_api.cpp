@@ -94,6 +94,9 @@ struct hm_ctx {
   int gm_restart = 50, gm_maxiter = 500, gm_precond = -1;
   double gm_tol = 1e-6;
   DevBuf d_P32, d_pipiv, d_gwork;
+  DevBuf d_Pinv;              // explicit FP32 inverses of the preconditioner blocks
+  DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
+  bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
   std::vector<int> gm_blocks;
   hm::GmresStats gm_stats;
   // learned surrogate of the evanescent sums (R/h >= 0.05)
@@ -299,7 +302,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -651,6 +654,33 @@ static int assemble_hmatrix(hm_ctx* c) {
   return HM_OK;
 }
 
+// precond 2 and -1 (the default): the preconditioner blocks applied as their
+// explicit inverses (one batched GEMM per Krylov step); precond 1: by their
+// triangular solves
+static bool block_inverse(const hm_ctx* c) { return c->gm_precond == 2 || c->gm_precond < 0; }
+
+// after the factorisation of the preconditioner blocks (c->gm_blocks), for
+// block_inverse(c): their explicit FP32 inverses without the row interchanges
+// (U^-1 L^-1) and their pivot permutations
+static int precond_prepare(hm_ctx* c, hipStream_t s) {
+  c->pinv_ready = false;
+  if (c->gm_blocks.size() < 2 || !block_inverse(c)) return HM_OK;
+  const int nblk = (int)c->gm_blocks.size() - 1;
+  size_t tot = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const size_t nb = (size_t)(c->gm_blocks[b + 1] - c->gm_blocks[b]);
+    tot += nb * nb;
+  }
+  HIPCHK(c, c->d_pperm.ensure(sizeof(int) * (size_t)c->gm_blocks[nblk]), "preconditioner alloc");
+  HIPCHK(c, c->d_Pinv.ensure(2 * sizeof(float) * tot), "preconditioner alloc");
+  HIPCHK(c, hm::gmres_block_perm(s, nblk, c->gm_blocks.data(), c->d_pipiv.as<int>(), c->d_pperm.as<int>()),
+         "preconditioner permutation");
+  HIPCHK(c, hm::gmres_block_inverse(s, nblk, c->gm_blocks.data(), c->d_P32.as<float>(), c->d_Pinv.as<float>()),
+         "preconditioner inverse");
+  c->pinv_ready = true;
+  return HM_OK;
+}
+
 // H-matrix form of hm_factor: FP32 copies of the diagonal blocks and of A's
 // low-rank factors for the Krylov products, and the block-diagonal
 // preconditioner (one block per body) factored in FP32
@@ -674,7 +704,7 @@ static int factor_hmatrix(hm_ctx* c) {
                                  c->d_counter.as<unsigned>(), nb), "block preconditioner");
   c->gm_blocks = c->body_start;
   if (c->gm_precond == 0) c->gm_blocks.clear();
-  return HM_OK;
+  return precond_prepare(c, c->stream);
 }
 
 extern "C" int hm_assemble(hm_ctx* c) {
@@ -753,7 +783,7 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
     // (one block per body) by default for several bodies, none for one body
     HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
     HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
-    const bool blocks = c->gm_precond < 0 ? c->nbody > 1 : c->gm_precond == 1;
+    const bool blocks = c->gm_precond < 0 ? c->nbody > 1 : c->gm_precond >= 1;
     c->gm_blocks.clear();
     if (blocks) {
       c->gm_blocks = c->body_start;
@@ -768,6 +798,8 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
                                               c->d_P32.as<float>(), c->d_pipiv.as<int>(), c->d_info.as<int>(),
                                               c->d_counter.as<unsigned>()), "block preconditioner");
     }
+    int rc = precond_prepare(c, c->stream);
+    if (rc != HM_OK) return rc;
   }
   std::vector<int> infos(nmat, 0);
   HIPCHK(c, hipMemcpyAsync(infos.data(), c->d_info.p, sizeof(int) * nmat, hipMemcpyDeviceToHost, c->stream), "factor");
@@ -861,6 +893,10 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
       ga.blk_start = c->gm_blocks.data();
       ga.P = c->d_P32.as<float>();
       ga.pipiv = c->d_pipiv.as<int>();
+      if (c->pinv_ready && block_inverse(c)) {
+        ga.pperm = c->d_pperm.as<int>();
+        ga.Pinv = c->d_Pinv.as<float>();
+      }
     }
     HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, ncol, c->gm_restart)), "gmres alloc");
     HIPCHK(c, hm::gmres_solve(c->stream, ga, c->d_gwork.p, &c->gm_stats), "gmres");
@@ -908,12 +944,13 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
     return c->fail(HM_ERR_ARG - 2, "restart must be in 1..%d", hm::kGmresMaxRestart);
   if (!(tol > 0.0 && tol < 1.0)) return c->fail(HM_ERR_ARG - 3, "tol must be in (0, 1)");
   if (maxiter < 1) return c->fail(HM_ERR_ARG - 4, "maxiter must be >= 1");
-  if (precond < -1 || precond > 1) return c->fail(HM_ERR_ARG - 5, "precond must be -1, 0 or 1");
+  if (precond < -1 || precond > 2) return c->fail(HM_ERR_ARG - 5, "precond must be -1, 0, 1 or 2");
   c->gm_restart = restart;
   c->gm_tol = tol;
   c->gm_maxiter = maxiter;
   c->gm_precond = precond;
   c->factored = c->solved = false;
+  c->pinv_ready = false;
   return HM_OK;
 }
 
@@ -1063,6 +1100,7 @@ extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int 
   HIPCHK(c, hipStreamSynchronize(s), "hmatrix");
   for (int k = 0; k < nb; ++k)
     if (infos[k] > 0) return c->fail(k * m + infos[k], "zero pivot in diagonal block %d", k + 1);
+  c->pinv_ready = false;   // new factors: their permutations (and inverses) follow at the first solve
   c->hd_nb = nb;
   c->hd_m = m;
   c->hd_ready = true;
@@ -1149,6 +1187,8 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
       HIPCHK(c, hipStreamSynchronize(s), "ziter");
       for (size_t k = 0; k < nmat; ++k)
         if (infos[k] > 0) return c->fail(start[k] + infos[k], "zero pivot in preconditioner block %zu", k + 1);
+      int rc = precond_prepare(c, s);
+      if (rc != HM_OK) return rc;
     }
   } else {
     const int nb = c->hd_nb, m = c->hd_m, ku = c->hm_kuse[0];
@@ -1174,12 +1214,20 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
     for (int k = 0; k <= nb; ++k) start.push_back(k * m);
     start.erase(start.begin());
     c->gm_blocks = start;
+    if (!c->pinv_ready) {
+      int rc = precond_prepare(c, s);
+      if (rc != HM_OK) return rc;
+    }
   }
   if (!c->gm_blocks.empty()) {
     ga.nblk = (int)c->gm_blocks.size() - 1;
     ga.blk_start = c->gm_blocks.data();
     ga.P = c->d_P32.as<float>();
     ga.pipiv = c->d_pipiv.as<int>();
+    if (c->pinv_ready && block_inverse(c)) {
+      ga.pperm = c->d_pperm.as<int>();
+      ga.Pinv = c->d_Pinv.as<float>();
+    }
   }
   HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, nrhs, c->gm_restart)), "gmres alloc");
   HIPCHK(c, hm::gmres_solve(s, ga, c->d_gwork.p, &c->gm_stats), "gmres");

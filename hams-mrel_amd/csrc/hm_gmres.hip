@@ -23,7 +23,11 @@
 //
 // Right preconditioning as in the reference: the Krylov vectors are
 // v -> M^{-1} v before the product, and the correction is M^{-1} (V y). M is
-// block diagonal with one block per body, LU-factored in FP32.
+// block diagonal with one block per body, LU-factored in FP32; M^{-1} is applied
+// by the blocks' triangular solves, or, when the explicit FP32 inverses of the
+// blocks are given, by one batched MFMA GEMM after the pivot permutation (the
+// triangular solves of a hundred columns are latency bound: 64 small GEMMs and
+// 64 block solves per application for a block of 2048).
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -318,10 +322,81 @@ hipError_t gmres_block_precond_setup(hipStream_t s, const double* A, int lda, in
   });
 }
 
+// batch of nb x nb identity matrices, one after the other
+__global__ void k_identity(float* __restrict__ I, int nb) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x, nn = (long long)nb * nb;
+  if (e >= nn) return;
+  float* b = I + 2 * nn * blockIdx.y;
+  const int r = (int)(e % nb), c = (int)(e / nb);
+  b[2 * e] = r == c ? 1.f : 0.f;
+  b[2 * e + 1] = 0.f;
+}
+
+hipError_t gmres_block_inverse(hipStream_t s, int nblk, const int* blk_start, const float* P, float* Pinv) {
+  return for_block_runs(nblk, blk_start, [&](int r0, int nb, int cnt, size_t off) {
+    const long long nn = (long long)nb * nb;
+    hipLaunchKernelGGL(k_identity, dim3((unsigned)((nn + 255) / 256), cnt), dim3(256), 0, s, Pinv + 2 * off, nb);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return lu_inverse_nopiv<float>(s, P + 2 * off, nb, nn, nb, Pinv + 2 * off, nb, nn, cnt);
+  });
+}
+
+// row permutation of each block from its LAPACK-style pivot sequence: perm =
+// the sequential swaps j <-> ipiv[j] applied to the identity, stored as
+// absolute rows (row r of the permuted vector is row perm[r] of the vector)
+__global__ void k_pivot_perm(const int* __restrict__ ipiv, int* __restrict__ perm, int nb, int cnt, int r0) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= cnt) return;
+  const int* pv = ipiv + (long long)nb * z;
+  int* pm = perm + (long long)nb * z;
+  for (int j = 0; j < nb; ++j) pm[j] = j;
+  for (int j = 0; j < nb; ++j) {
+    const int q = pv[j];
+    if (q != j) {
+      const int a = pm[j];
+      pm[j] = pm[q];
+      pm[q] = a;
+    }
+  }
+  const int base = r0 + nb * z;
+  for (int j = 0; j < nb; ++j) pm[j] += base;
+}
+
+hipError_t gmres_block_perm(hipStream_t s, int nblk, const int* blk_start, const int* pipiv, int* perm) {
+  return for_block_runs(nblk, blk_start, [&](int r0, int nb, int cnt, size_t) {
+    hipLaunchKernelGGL(k_pivot_perm, dim3((cnt + 63) / 64), dim3(64), 0, s, pipiv + r0, perm + r0, nb, cnt, r0);
+    return hipGetLastError();
+  });
+}
+
+// W = FP32 copy of the rows perm of V (n x ncol, leading dimensions ldv and n)
+__global__ void k_demote_perm(const double* __restrict__ V, long long ldv, const int* __restrict__ perm, int n,
+                              float* __restrict__ W) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= n) return;
+  const long long o = 2 * ((long long)perm[r] + (long long)c * ldv), d = 2 * ((long long)r + (long long)c * n);
+  W[d] = (float)V[o];
+  W[d + 1] = (float)V[o + 1];
+}
+
+// Z = M^{-1} Z in place with the blocks' LU factors
 static hipError_t apply_precond(hipStream_t s, const GmresArgs& a, float* Z, int ncol) {
   return for_block_runs(a.nblk, a.blk_start, [&](int r0, int nb, int cnt, size_t off) {
     return lu_solve<float>(s, a.P + 2 * off, nb, (long long)nb * nb, nb, a.pipiv + r0, Z + 2 * (long long)r0, a.n,
                            nb, ncol, cnt);
+  });
+}
+
+// Z = M^{-1} v = (U^{-1} L^{-1}) (P v): the pivot permutation applied while v is
+// demoted to FP32 (into W), then one batched GEMM per run of equal blocks
+static hipError_t apply_inverse(hipStream_t s, const GmresArgs& a, const double* v, float* W, float* Z, int ncol) {
+  hipLaunchKernelGGL(k_demote_perm, dim3((a.n + 255) / 256, ncol), dim3(256), 0, s, v, (long long)a.n, a.pperm, a.n, W);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return for_block_runs(a.nblk, a.blk_start, [&](int r0, int nb, int cnt, size_t off) {
+    return cgemm<float>(s, false, nb, ncol, nb, 1.f, 0.f, a.Pinv + 2 * off, nb, (long long)nb * nb,
+                        W + 2 * (long long)r0, a.n, nb, 0.f, 0.f, Z + 2 * (long long)r0, a.n, nb, cnt);
   });
 }
 
@@ -382,8 +457,12 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       ++total;
       const double* vj = V + 2 * vstride * j;
       // W = A * M^{-1} v_j on MFMA in FP32
-      GM_CHECK(demote(s, vj, n, 0, Z32, n, 0, n, ncol, 1));
-      if (a.nblk > 0) GM_CHECK(apply_precond(s, a, Z32, ncol));
+      if (a.nblk > 0 && a.Pinv) {
+        GM_CHECK(apply_inverse(s, a, vj, W32, Z32, ncol));
+      } else {
+        GM_CHECK(demote(s, vj, n, 0, Z32, n, 0, n, ncol, 1));
+        if (a.nblk > 0) GM_CHECK(apply_precond(s, a, Z32, ncol));
+      }
       if (a.op32) GM_CHECK(a.op32(s, Z32, W32));
       else GM_CHECK(cgemm<float>(s, false, n, ncol, n, 1.f, 0.f, a.A32, n, 0, Z32, n, 0, 0.f, 0.f, W32, n, 0, 1));
       GM_CHECK(promote_add(s, W32, n, 0, W, n, 0, n, ncol, 1, true));
@@ -407,8 +486,12 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
     if (a.nblk > 0) {
       hipLaunchKernelGGL(k_combine, gcol, dim3(256), 0, s, V, vstride, n, m, y, steps, W, (long long)n, 0);
       GM_CHECK(hipGetLastError());
-      GM_CHECK(demote(s, W, n, 0, Z32, n, 0, n, ncol, 1));
-      GM_CHECK(apply_precond(s, a, Z32, ncol));
+      if (a.Pinv) {
+        GM_CHECK(apply_inverse(s, a, W, W32, Z32, ncol));
+      } else {
+        GM_CHECK(demote(s, W, n, 0, Z32, n, 0, n, ncol, 1));
+        GM_CHECK(apply_precond(s, a, Z32, ncol));
+      }
       GM_CHECK(promote_add(s, Z32, n, 0, a.X, n, 0, n, ncol, 1, false));
     } else {
       hipLaunchKernelGGL(k_combine, gcol, dim3(256), 0, s, V, vstride, n, m, y, steps, a.X, (long long)n, 1);

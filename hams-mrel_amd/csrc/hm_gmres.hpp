@@ -25,6 +25,8 @@ struct GmresArgs {
   const int* blk_start = nullptr;
   const float* P = nullptr;     // LU factors of the blocks, packed one after the other
   const int* pipiv = nullptr;   // block-local pivots at the block's row offset
+  const float* Pinv = nullptr;  // (U^-1 L^-1) of the blocks, packed like P (gmres_block_inverse), or null
+  const int* pperm = nullptr;   // the blocks' pivot permutations (gmres_block_perm), needed with Pinv
   // operator form (used instead of A32 / A64 when set): op32 writes W = A Z in
   // FP32, op64 writes W = A X in FP64, both n x ncol with leading dimension n
   std::function<hipError_t(hipStream_t, const float*, float*)> op32;
@@ -40,6 +42,12 @@ size_t gmres_workspace_bytes(int n, int ncol, int restart);
 // FP32 copies of the diagonal blocks of A, LU-factored in place
 hipError_t gmres_block_precond_setup(hipStream_t s, const double* A, int lda, int nblk, const int* blk_start,
                                      float* P, int* pipiv, int* info, unsigned* counter);
+// row permutations of the factored blocks from their pivots, as absolute rows:
+// row r of P v is row perm[r] of v
+hipError_t gmres_block_perm(hipStream_t s, int nblk, const int* blk_start, const int* pipiv, int* perm);
+// explicit inverses of the blocks without their row interchanges, U^-1 L^-1,
+// from the LU factors P, packed like P (the block's inverse is U^-1 L^-1 P_b)
+hipError_t gmres_block_inverse(hipStream_t s, int nblk, const int* blk_start, const float* P, float* Pinv);
 hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats* stats);
 
 }  // namespace hm

@@ -719,6 +719,38 @@ hipError_t lu_solve(hipStream_t s, const T* A, int lda, long long sA, int n, con
 }
 
 
+template <typename T>
+hipError_t lu_inverse_nopiv(hipStream_t s, const T* A, int lda, long long sA, int n, T* B, int ldb, long long sB,
+                            int batch) {
+  // forward: L Y = I; Y is unit lower triangular, so rows [k0, k0 + kb) hold
+  // nonzeros in columns < k0 + kb only
+  for (int k0 = 0; k0 < n; k0 += kLuNb) {
+    const int kb = (n - k0) < kLuNb ? n - k0 : kLuNb;
+    const int nc = k0 + kb;
+    hipLaunchKernelGGL((k_trsm_block<T, false>), dim3((nc + 3) / 4, 1, batch), dim3(256), 0, s,
+                       A + 2 * ((long long)k0 + (long long)k0 * lda), lda, sA, kb, B + 2 * (long long)k0, ldb, sB, nc);
+    if (k0 + kb < n) {
+      hipError_t e = cgemm<T>(s, false, n - k0 - kb, nc, kb, T(-1), T(0),
+                              A + 2 * ((long long)(k0 + kb) + (long long)k0 * lda), lda, sA,
+                              B + 2 * (long long)k0, ldb, sB, T(1), T(0), B + 2 * (long long)(k0 + kb), ldb, sB, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  // backward: U X = Y, all columns
+  const int nblk = (n + kLuNb - 1) / kLuNb;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int k0 = b * kLuNb, kb = (n - k0) < kLuNb ? n - k0 : kLuNb;
+    hipLaunchKernelGGL((k_trsm_block<T, true>), dim3((n + 3) / 4, 1, batch), dim3(256), 0, s,
+                       A + 2 * ((long long)k0 + (long long)k0 * lda), lda, sA, kb, B + 2 * (long long)k0, ldb, sB, n);
+    if (k0 > 0) {
+      hipError_t e = cgemm<T>(s, false, k0, n, kb, T(-1), T(0), A + 2 * ((long long)k0 * lda), lda, sA,
+                              B + 2 * (long long)k0, ldb, sB, T(1), T(0), B, ldb, sB, batch);
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Grouped factorisation and solve: matrices of different orders in one launch
 // per step (blockIdx.z = member). The host walks the panels of the largest
@@ -986,5 +1018,7 @@ template hipError_t lu_solve<double>(hipStream_t, const double*, int, long long,
                                      long long, int, int);
 template hipError_t lu_solve<float>(hipStream_t, const float*, int, long long, int, const int*, float*, int,
                                     long long, int, int);
+template hipError_t lu_inverse_nopiv<float>(hipStream_t, const float*, int, long long, int, float*, int, long long,
+                                            int);
 
 }  // namespace hm

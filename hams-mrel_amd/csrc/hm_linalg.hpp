@@ -22,6 +22,14 @@ template <typename T>
 hipError_t lu_solve(hipStream_t s, const T* A, int lda, long long sA, int n, const int* ipiv, T* B, int ldb,
                     long long sB, int nrhs, int batch);
 
+// B = (L U)^{-1} B without the row interchanges, for B = the identity on entry
+// (the caller sets it): the forward sweep touches only the columns that
+// L^{-1} fills (column block k0 needs columns < k0 + 64), 2/3 of the work of
+// lu_solve with n right-hand sides
+template <typename T>
+hipError_t lu_inverse_nopiv(hipStream_t s, const T* A, int lda, long long sA, int n, T* B, int ldb, long long sB,
+                            int batch);
+
 // ---- grouped operations: members of different sizes, one launch per step ----
 template <typename T>
 struct GemmDesc {            // C = alpha op(A) B + beta C for one member

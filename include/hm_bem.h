@@ -89,7 +89,11 @@ int hm_forces(hm_ctx *ctx, double *amas, double *bdmp, double *exfc);
  * step is one FP32 complex GEMM on MFMA and each restart recomputes the FP64
  * residual; converged when ||B - A X|| <= tol ||B|| in every column (:388).
  * precond: 1 block-diagonal LU with one block per body (SETUP_BLOCK_PRECONDITIONER
- * :190), 0 none, -1 block-diagonal for several bodies and none for one.
+ * :190), applied by the blocks' triangular solves (APPLY_PRECONDITIONER :266);
+ * 2 the same blocks applied as their explicit FP32 inverses (one batched GEMM
+ * per Krylov step; the inverses, U^-1 L^-1 with the pivots applied to the
+ * vector, are built after each factorisation); 0 none; -1 (default) as 2 for
+ * several bodies, none for one.
  * Defaults: restart 50, tol 1e-6, maxiter 500 (total Krylov steps), precond -1.
  * hm_get_gmres_stats reports the last solve (converged = 0 is the reference's
  * INFO = 1). */

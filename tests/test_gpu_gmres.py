@@ -45,7 +45,7 @@ def _residuals(c, w, precision):
     return np.linalg.norm(r, axis=0) / np.linalg.norm(rhs, axis=0)
 
 
-@pytest.mark.parametrize("precond", [-1, 0, 1])
+@pytest.mark.parametrize("precond", [-1, 0, 1, 2])
 def test_gmres_residual_and_forces(quad, precond):
     p, c = quad
     c.set_gmres(restart=50, tol=1e-6, maxiter=500, precond=precond)
@@ -69,6 +69,31 @@ def test_block_preconditioner_saves_iterations(quad):
         c.solve_frequency(1.6, hm_bem.HM_GMRES)
         its[pc] = c.gmres_stats()["iterations"]
     assert its[1] <= its[0], its
+
+
+def test_explicit_block_inverse_matches_triangular_solves(quad):
+    """The preconditioner applied as the blocks' explicit FP32 inverses (one
+    batched GEMM a Krylov step, precond 2 and the default) against the blocks'
+    LU triangular solves (precond 1), dense and H-matrix: both are FP32
+    approximations of the same M^-1, so GMRES takes the same number of steps
+    (within one) and converges to the same solution."""
+    p, c = quad
+    try:
+        for hmat in (False, True):
+            c.set_hmatrix(hmat, eps=1e-7, eps_s=1e-8, kmax=50)
+            got, its = {}, {}
+            for pc in (1, 2):
+                c.set_gmres(restart=50, tol=1e-9, maxiter=500, precond=pc)
+                got[pc] = c.solve_frequency(1.1, hm_bem.HM_GMRES)
+                st = c.gmres_stats()
+                assert st["converged"], (hmat, pc, st)
+                its[pc] = st["iterations"]
+            assert abs(its[1] - its[2]) <= 1, (hmat, its)
+            for k in KEYS:
+                assert relerr(got[2][k], got[1][k]) < 1e-7, (hmat, k, relerr(got[2][k], got[1][k]))
+    finally:
+        c.set_hmatrix(False, eps=1e-5, eps_s=1e-6, kmax=50)
+        c.set_gmres()
 
 
 def test_short_restart_cycles(quad):
@@ -112,7 +137,7 @@ def test_gmres_deep_irr_single_body(oracle):
 def test_gmres_argument_errors():
     c = hm_bem.Context()
     for kw, code in ((dict(restart=0), -102), (dict(restart=129), -102), (dict(tol=0.0), -103),
-                     (dict(tol=1.5), -103), (dict(maxiter=0), -104), (dict(precond=2), -105)):
+                     (dict(tol=1.5), -103), (dict(maxiter=0), -104), (dict(precond=3), -105)):
         with pytest.raises(hm_bem.HMError) as e:
             c.set_gmres(**kw)
         assert e.value.code == code, kw
