@@ -94,6 +94,7 @@ struct hm_ctx {
   int gm_restart = 50, gm_maxiter = 500, gm_precond = -1;
   double gm_tol = 1e-6;
   DevBuf d_P32, d_pipiv, d_gwork;
+  DevBuf d_luws;              // workspace of the cooperative LU panel kernel
   DevBuf d_Pinv;              // explicit FP32 inverses of the preconditioner blocks
   DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
   bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
@@ -302,7 +303,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
@@ -771,13 +772,15 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
     int rc = factor_hmatrix(c);
     if (rc) return rc;
   } else if (precision == HM_FP64) {
+    HIPCHK(c, c->d_luws.ensure(hm::lu_coop_ws_bytes(n)), "factor alloc");
     HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
-                                    c->d_counter.as<unsigned>(), 1), "lu fp64");
+                                    c->d_counter.as<unsigned>(), 1, c->d_luws.p), "lu fp64");
   } else if (precision == HM_FP32_IR) {
     HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
     HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    HIPCHK(c, c->d_luws.ensure(hm::lu_coop_ws_bytes(n)), "factor alloc");
     HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
-                                   c->d_counter.as<unsigned>(), 1), "lu fp32");
+                                   c->d_counter.as<unsigned>(), 1, c->d_luws.p), "lu fp32");
   } else {
     // GMRES: FP32 copy for the Krylov products; block-diagonal preconditioner
     // (one block per body) by default for several bodies, none for one body
@@ -1755,10 +1758,12 @@ extern "C" int hm_zgetrf(hm_ctx* c, int m, int n, double* a, int lda, int* ipiv,
   if (!rc) rc = c->hip(dp.ensure(sizeof(int) * n), "zgetrf alloc");
   if (!rc) rc = c->hip(di.ensure(sizeof(int)), "zgetrf alloc");
   if (!rc) rc = c->hip(dc.ensure(sizeof(unsigned)), "zgetrf alloc");
+  if (!rc) rc = c->hip(c->d_luws.ensure(hm::lu_coop_ws_bytes(n)), "zgetrf alloc");
   if (!rc) rc = c->hip(hipMemcpy2DAsync(da.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemsetAsync(di.p, 0, sizeof(int), c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemsetAsync(dc.p, 0, sizeof(unsigned), c->stream), "zgetrf");
-  if (!rc) rc = c->hip(hm::lu_factor<double>(c->stream, da.as<double>(), n, 0, n, dp.as<int>(), di.as<int>(), dc.as<unsigned>(), 1), "zgetrf");
+  if (!rc) rc = c->hip(hm::lu_factor<double>(c->stream, da.as<double>(), n, 0, n, dp.as<int>(), di.as<int>(), dc.as<unsigned>(), 1,
+                                                c->d_luws.p), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpy2DAsync(a, cplx * lda, da.p, cplx * n, cplx * n, n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpyAsync(ipiv, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpyAsync(info, di.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "zgetrf");

@@ -21,6 +21,7 @@
 //   with the unit-lower panel block, and the trailing matrix gets one GEMM.
 // * Triangular solves with many right-hand sides: per 64-row block a
 //   diagonal-block solve (64 lanes per right-hand side) and a GEMM update.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -287,6 +288,151 @@ __global__ __launch_bounds__(256) void k_lu_col(T* A, int lda, long long sA, int
                                                 int* ipiv, int* info, unsigned* counter) {
   lu_col_body<T>(A + 2 * sA * blockIdx.z, lda, n, j, c0, c1, ipiv + (long long)n * blockIdx.z, info + blockIdx.z,
                  counter + blockIdx.z, gridDim.x, blockIdx.x);
+}
+
+// A whole panel [k0, k1) of one large matrix in one cooperative launch: the
+// panel's rows [k0, n) are split into slabs of kCpRows rows, one per workgroup,
+// held in LDS for the whole panel. Per column j: each workgroup finds its
+// candidate pivot (largest |re| + |im| at rows >= j, smallest row on ties, as
+// block_pivot_search) and publishes its value, row and panel row; workgroup 0,
+// which holds rows k0 .. k0 + kCpRows - 1, publishes row j; one grid-wide
+// barrier; then every workgroup picks the same winner from the candidates, the
+// owners of rows j and p swap them in their slabs, and each workgroup scales
+// L(:, j) and applies the rank-1 update to its own rows, in LDS. One barrier a
+// column instead of one launch a column and a separate pivot search, and the
+// panel goes through HBM once. The exchange area (two column parities, so that
+// the next column's candidates never overwrite ones still being read):
+// cval[2][W], crow[2][W], cdat[2][W][64] complex, jdat[2][64] complex.
+constexpr int kCpRows = 128;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_panel_coop(T* A, int lda, int n, int k0, int k1, int* ipiv, int* info,
+                                                       T* cval, int* crow, T* cdat, T* jdat) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int W = gridDim.x, w = blockIdx.x, t = threadIdx.x;
+  const int kb = k1 - k0;
+  const int rbeg = k0 + w * kCpRows;
+  const int nr = n - rbeg < kCpRows ? n - rbeg : kCpRows;
+  __shared__ T S[kCpRows * kLuNb * 2];   // the slab, column-major: row r of column c at 2 (c kCpRows + r)
+  __shared__ T U[kLuNb * 2];             // the pivot row (row j after the swap)
+  __shared__ T rv[256];
+  __shared__ int rx[256];
+  for (int e = t; e < nr * kb; e += 256) {
+    const int r = e % nr, c = e / nr;
+    const long long o = 2 * ((long long)(rbeg + r) + (long long)(k0 + c) * lda);
+    S[2 * (c * kCpRows + r)] = A[o];
+    S[2 * (c * kCpRows + r) + 1] = A[o + 1];
+  }
+  __syncthreads();
+  for (int j = k0; j < k1; ++j) {
+    const int jj = j - k0, par = jj & 1;
+    // this slab's candidate
+    T best = T(-1);
+    int bi = n;
+    for (int r = t; r < nr; r += 256) {
+      if (rbeg + r < j) continue;
+      const T v = fabs(S[2 * (jj * kCpRows + r)]) + fabs(S[2 * (jj * kCpRows + r) + 1]);
+      if (v > best) { best = v; bi = rbeg + r; }
+    }
+    rv[t] = best;
+    rx[t] = bi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) {
+        const T o = rv[t + h];
+        const int oi = rx[t + h];
+        if (o > rv[t] || (o == rv[t] && oi < rx[t])) { rv[t] = o; rx[t] = oi; }
+      }
+      __syncthreads();
+    }
+    const int mine = rx[0];
+    if (t == 0) {
+      cval[par * W + w] = rv[0];
+      crow[par * W + w] = mine;
+    }
+    if (mine < n)
+      for (int c = t; c < kb; c += 256) {
+        const int r = mine - rbeg;
+        cdat[2 * ((par * W + w) * kLuNb + c)] = S[2 * (c * kCpRows + r)];
+        cdat[2 * ((par * W + w) * kLuNb + c) + 1] = S[2 * (c * kCpRows + r) + 1];
+      }
+    if (w == 0)
+      for (int c = t; c < kb; c += 256) {
+        jdat[2 * (par * kLuNb + c)] = S[2 * (c * kCpRows + jj)];
+        jdat[2 * (par * kLuNb + c) + 1] = S[2 * (c * kCpRows + jj) + 1];
+      }
+    __syncthreads();
+    grid.sync();
+    // the winner, the same in every workgroup
+    T b2 = T(-1);
+    int i2 = n;
+    for (int q = t; q < W; q += 256) {
+      const T v = cval[par * W + q];
+      const int r = crow[par * W + q];
+      if (v > b2 || (v == b2 && r < i2)) { b2 = v; i2 = r; }
+    }
+    rv[t] = b2;
+    rx[t] = i2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (t < h) {
+        const T o = rv[t + h];
+        const int oi = rx[t + h];
+        if (o > rv[t] || (o == rv[t] && oi < rx[t])) { rv[t] = o; rx[t] = oi; }
+      }
+      __syncthreads();
+    }
+    const T pmax = rv[0];
+    const int p = rx[0] < n ? rx[0] : j;   // a column without a comparable entry keeps its row
+    const int wp = (p - k0) / kCpRows;
+    for (int c = t; c < kb; c += 256) {
+      const T* src = p == j ? jdat + 2 * (par * kLuNb + c) : cdat + 2 * ((par * W + wp) * kLuNb + c);
+      U[2 * c] = src[0];
+      U[2 * c + 1] = src[1];
+    }
+    if (w == 0 && t == 0) {
+      ipiv[j] = p;
+      if (!(pmax > T(0)) && *info == 0) *info = j + 1;
+    }
+    __syncthreads();
+    if (p != j) {
+      if (w == 0)
+        for (int c = t; c < kb; c += 256) {
+          S[2 * (c * kCpRows + jj)] = U[2 * c];
+          S[2 * (c * kCpRows + jj) + 1] = U[2 * c + 1];
+        }
+      if (w == wp)
+        for (int c = t; c < kb; c += 256) {
+          S[2 * (c * kCpRows + p - rbeg)] = jdat[2 * (par * kLuNb + c)];
+          S[2 * (c * kCpRows + p - rbeg) + 1] = jdat[2 * (par * kLuNb + c) + 1];
+        }
+      __syncthreads();
+    }
+    // L(:, j) and the rank-1 update of the panel's columns (j, k1), own rows > j
+    const T dr = U[2 * jj], di = U[2 * jj + 1];
+    const T den = dr * dr + di * di;
+    T qr = 0, qi = 0;
+    if (den > T(0)) { qr = dr / den; qi = -di / den; }
+    for (int r = t; r < nr; r += 256) {
+      if (rbeg + r <= j) continue;
+      T* col = S + 2 * r;
+      const T xr = col[2 * jj * kCpRows], xi = col[2 * jj * kCpRows + 1];
+      const T lr = xr * qr - xi * qi, li = xr * qi + xi * qr;
+      col[2 * jj * kCpRows] = lr;
+      col[2 * jj * kCpRows + 1] = li;
+      for (int c = jj + 1; c < kb; ++c) {
+        col[2 * c * kCpRows] -= lr * U[2 * c] - li * U[2 * c + 1];
+        col[2 * c * kCpRows + 1] -= lr * U[2 * c + 1] + li * U[2 * c];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nr * kb; e += 256) {
+    const int r = e % nr, c = e / nr;
+    const long long o = 2 * ((long long)(rbeg + r) + (long long)(k0 + c) * lda);
+    A[o] = S[2 * (c * kCpRows + r)];
+    A[o + 1] = S[2 * (c * kCpRows + r) + 1];
+  }
 }
 
 // A whole panel [k0, k1) of a small matrix by one workgroup per matrix: for
@@ -625,12 +771,54 @@ __global__ __launch_bounds__(256) void k_trsm_block(const T* Tm, int ldt, long l
   trsm_body<T, UPPER>(Tm + 2 * sT * blockIdx.z, ldt, kb, X + 2 * sX * blockIdx.z, ldx, ncols, blockIdx.x);
 }
 
+size_t lu_coop_ws_bytes(int n) {
+  const size_t W = (size_t)((n + kCpRows - 1) / kCpRows);
+  return 2 * W * sizeof(double) + 2 * W * sizeof(int) + 2 * W * kLuNb * 2 * sizeof(double) +
+         2 * kLuNb * 2 * sizeof(double) + 1024;
+}
+
+// workgroups of k_lu_panel_coop<T> the device holds at once (0: no cooperative launch)
+template <typename T>
+static int lu_coop_capacity() {
+  static int cap = -1;
+  if (cap < 0) {
+    int dev = 0, cus = 0, per = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_lu_panel_coop<T>, 256, 0) != hipSuccess)
+      coop = 0;
+    cap = coop ? cus * per : 0;
+  }
+  return cap;
+}
+
 template <typename T>
 hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipiv, int* info,
-                     unsigned* counter, int batch) {
+                     unsigned* counter, int batch, void* coop_ws) {
+  const bool reg = sizeof(T) == sizeof(float) && n <= 2048;
+  bool coop = coop_ws && batch == 1 && !reg && (n + kCpRows - 1) / kCpRows <= lu_coop_capacity<T>();
   for (int k0 = 0; k0 < n; k0 += kLuNb) {
     const int k1 = k0 + kLuNb < n ? k0 + kLuNb : n;
-    if (sizeof(T) == sizeof(float) && n <= 2048) {
+    if (coop) {
+      const int W = (n - k0 + kCpRows - 1) / kCpRows, Wmax = (n + kCpRows - 1) / kCpRows;
+      char* b = static_cast<char*>(coop_ws);
+      T* cval = reinterpret_cast<T*>(b);
+      int* crow = reinterpret_cast<int*>(b + 2 * (size_t)Wmax * sizeof(double));
+      T* cdat = reinterpret_cast<T*>(b + ((2 * (size_t)Wmax * (sizeof(double) + sizeof(int)) + 255) & ~(size_t)255));
+      T* jdat = cdat + 2 * (size_t)Wmax * kLuNb * 2;
+      int kk0 = k0, kk1 = k1, nn = n, ld = lda;
+      void* args[] = {&A, &ld, &nn, &kk0, &kk1, &ipiv, &info, &cval, &crow, &cdat, &jdat};
+      hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_lu_panel_coop<T>), dim3(W), dim3(256),
+                                                args, 0, s);
+      if (e != hipSuccess) {
+        if (k0 > 0) return e;
+        (void)hipGetLastError();   // refused before anything ran: per-column panels instead
+        coop = false;
+      }
+    }
+    if (coop) {
+    } else if (reg) {
       float* Af = reinterpret_cast<float*>(A);
       if (n <= 1024) hipLaunchKernelGGL((k_lu_panel_reg<1>), dim3(1, 1, batch), dim3(1024), 0, s, Af, lda, sA, n, k0, k1, ipiv, info);
       else hipLaunchKernelGGL((k_lu_panel_reg<2>), dim3(1, 1, batch), dim3(1024), 0, s, Af, lda, sA, n, k0, k1, ipiv, info);
@@ -1012,8 +1200,8 @@ template hipError_t lu_factor_grouped<double>(hipStream_t, const LuDesc<double>*
 template hipError_t lu_factor_grouped<float>(hipStream_t, const LuDesc<float>*, int, int);
 template hipError_t lu_solve_grouped<double>(hipStream_t, const LuDesc<double>*, int, int, int);
 template hipError_t lu_solve_grouped<float>(hipStream_t, const LuDesc<float>*, int, int, int);
-template hipError_t lu_factor<double>(hipStream_t, double*, int, long long, int, int*, int*, unsigned*, int);
-template hipError_t lu_factor<float>(hipStream_t, float*, int, long long, int, int*, int*, unsigned*, int);
+template hipError_t lu_factor<double>(hipStream_t, double*, int, long long, int, int*, int*, unsigned*, int, void*);
+template hipError_t lu_factor<float>(hipStream_t, float*, int, long long, int, int*, int*, unsigned*, int, void*);
 template hipError_t lu_solve<double>(hipStream_t, const double*, int, long long, int, const int*, double*, int,
                                      long long, int, int);
 template hipError_t lu_solve<float>(hipStream_t, const float*, int, long long, int, const int*, float*, int,

@@ -15,7 +15,12 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
 // info[b] = first zero-pivot column + 1 or 0). counter: `batch` zeroed uints.
 template <typename T>
 hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipiv, int* info, unsigned* counter,
-                     int batch);
+                     int batch, void* coop_ws = nullptr);
+// workspace of lu_factor's cooperative panel kernel (one matrix of order <= n):
+// with it, the panels of a single matrix are factored one cooperative launch
+// each (the panel in LDS across workgroups, one grid barrier a column) where
+// the device can hold the launch
+size_t lu_coop_ws_bytes(int n);
 
 // Solve A X = B with the factors of lu_factor; B is overwritten by X.
 template <typename T>

@@ -142,7 +142,7 @@ def test_dispersion_matches_oracle(oracle):
 
 
 # ---- dense LU -------------------------------------------------------------------------
-@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 257, 700])
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 257, 700, 1337])
 def test_zgetrf_zgetrs_vs_textbook(ctx, oracle, n):
     rng = np.random.default_rng(n)
     a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
