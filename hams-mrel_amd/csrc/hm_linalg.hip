@@ -64,33 +64,56 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
     for (int j = 0; j < 2; ++j)
       for (int v = 0; v < 4; ++v) { cre[i][j][v] = 0; cim[i][j][v] = 0; }
 
-  for (int k0 = 0; k0 < k; k0 += kBK) {
-    // stage op(A)[row0:+64, k0:+16] and B[k0:+16, col0:+64]
+  // op(A)[row0:+64, k0:+16] and B[k0:+16, col0:+64] go through registers into
+  // LDS; the next chunk's global loads are issued before the current chunk's
+  // MFMAs, so their latency hides behind the arithmetic
+  T pa_r[4], pa_i[4], pb_r[4], pb_i[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       int r, kk;
       if (!TA) { r = tid & 63; kk = (tid >> 6) + 4 * q; }
       else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
       const int gr = row0 + r, gk = k0 + kk;
-      T xr = 0, xi = 0;
+      pa_r[q] = 0;
+      pa_i[q] = 0;
       if (gr < m && gk < k) {
         const long long off = TA ? (long long)gk + (long long)gr * lda : (long long)gr + (long long)gk * lda;
-        xr = A[2 * off];
-        xi = A[2 * off + 1];
+        pa_r[q] = A[2 * off];
+        pa_i[q] = A[2 * off + 1];
       }
-      as_re[kk][r] = xr;
-      as_im[kk][r] = xi;
       const int bk = tid & 15, bc = (tid >> 4) + 16 * q;
       const int gbk = k0 + bk, gbc = col0 + bc;
-      T yr = 0, yi = 0;
+      pb_r[q] = 0;
+      pb_i[q] = 0;
       if (gbk < k && gbc < n) {
         const long long off = (long long)gbk + (long long)gbc * ldb;
-        yr = B[2 * off];
-        yi = B[2 * off + 1];
+        pb_r[q] = B[2 * off];
+        pb_i[q] = B[2 * off + 1];
       }
-      bs_re[bk][bc] = yr;
-      bs_im[bk][bc] = yi;
     }
-    __syncthreads();
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int r, kk;
+      if (!TA) { r = tid & 63; kk = (tid >> 6) + 4 * q; }
+      else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      as_re[kk][r] = pa_r[q];
+      as_im[kk][r] = pa_i[q];
+      const int bk = tid & 15, bc = (tid >> 4) + 16 * q;
+      bs_re[bk][bc] = pb_r[q];
+      bs_im[bk][bc] = pb_i[q];
+    }
+  };
+  if (k > 0) {
+    fetch(0);
+    stash();
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < k; k0 += kBK) {
+    const bool more = k0 + kBK < k;
+    if (more) fetch(k0 + kBK);
     for (int k4 = 0; k4 < kBK; k4 += 4) {
       const int kr = k4 + (lane >> 4), li = lane & 15;
       T a_re[2], a_im[2], b_re[2], b_im[2];
@@ -109,6 +132,10 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
         }
     }
     __syncthreads();
+    if (more) {
+      stash();
+      __syncthreads();
+    }
   }
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
