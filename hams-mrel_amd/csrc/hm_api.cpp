@@ -1709,6 +1709,32 @@ extern "C" int hm_infgreen3d_batch(hm_ctx* c, int n, const double* xf, const dou
   return green_common(c, n, true, in, v, nullptr, 0, -1.0, flag, grn);
 }
 
+extern "C" int hm_ik01a_batch(hm_ctx* c, int n, const double* x, double* bk0, double* bk1, int mode) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");
+  if (n == 0) return HM_OK;
+  if (!x || !bk0 || !bk1) return c->fail(HM_ERR_ARG - 3, "NULL array");
+  if (mode != 0 && mode != 1) return c->fail(HM_ERR_ARG - 6, "mode must be 0 or 1");
+  DevBuf tmp;
+  const size_t sz = (size_t)n * sizeof(double);
+  HIPCHK(c, tmp.ensure(3 * sz), "ik01a alloc");
+  double* dx = tmp.as<double>();
+  double* dout = dx + n;
+  std::vector<double> out(2 * (size_t)n);
+  int rc = c->hip(hipMemcpyAsync(dx, x, sz, hipMemcpyHostToDevice, c->stream), "ik01a");
+  if (!rc) rc = c->hip(hm::ik01a_batch(c->stream, n, dx, mode, dout), "ik01a");
+  if (!rc) rc = c->hip(hipMemcpyAsync(out.data(), dout, 2 * sz, hipMemcpyDeviceToHost, c->stream), "ik01a");
+  if (!rc) rc = c->hip(hipStreamSynchronize(c->stream), "ik01a");
+  (void)hipStreamSynchronize(c->stream);
+  tmp.release();
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) {
+    bk0[i] = out[2 * (size_t)i];
+    bk1[i] = out[2 * (size_t)i + 1];
+  }
+  return HM_OK;
+}
+
 extern "C" int hm_sglintbd_batch(hm_ctx* c, int n, const int* ncn, const double* verts, const double* xf, double* sij,
                                  double* dij) {
   if (!c) return HM_ERR_ARG - 1;

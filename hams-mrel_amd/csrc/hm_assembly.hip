@@ -769,6 +769,23 @@ __global__ void k_rankine_batch(int n, const int* ncn, const double* verts, cons
   out[4 * (long long)t + 3] = r.dz;
 }
 
+// K0, K1 at n points: mode 0 the reference's IK01A (sf_k01), 1 the Horner-form
+// evaluation of the mixed-precision kernels (sf_k01_fast)
+__global__ void k_ik01a(int n, const double* __restrict__ x, int mode, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double k0, k1;
+  if (mode) sf_k01_fast(x[t], k0, k1);
+  else sf_k01(x[t], k0, k1);
+  out[2 * (long long)t] = k0;
+  out[2 * (long long)t + 1] = k1;
+}
+
+hipError_t ik01a_batch(hipStream_t s, int n, const double* x, int mode, double* out) {
+  hipLaunchKernelGGL(k_ik01a, dim3((n + 255) / 256), dim3(256), 0, s, n, x, mode, out);
+  return hipGetLastError();
+}
+
 hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const double* xp, const double* yf,
                        const double* yp, const double* zf, const double* zp, double v, const double* wvn, double h,
                        const int* flag, double* out) {

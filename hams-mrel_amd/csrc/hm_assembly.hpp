@@ -88,5 +88,7 @@ hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const 
                        const double* yp, const double* zf, const double* zp, double v, const double* wvn, double h,
                        const int* flag, double* out);
 hipError_t rankine_batch(hipStream_t s, int n, const int* ncn, const double* verts, const double* xf, double* out);
+// K0, K1 at n points (out: 2 per point); mode 0 IK01A as the reference, 1 the fast form
+hipError_t ik01a_batch(hipStream_t s, int n, const double* x, int mode, double* out);
 
 }  // namespace hm

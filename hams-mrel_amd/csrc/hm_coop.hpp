@@ -291,8 +291,8 @@ __device__ double coop_limes(int n, double (&s)[K], int l) {
 }
 
 // series of one pair (broadcast inputs) evaluated by the whole group; every lane
-// returns the three epsilon limits
-template <int REG, int G, int K>
+// returns the three epsilon limits. FAST: K0/K1 by sf_k01_fast (mixed paths)
+template <int REG, int G, int K, bool FAST = false>
 __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, double zph, double h, double v,
                                             const double* wvn, int l, double& l1, double& l2, double& l3) {
   typedef LaneGroup<G> LG;
@@ -304,8 +304,8 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
     double t1 = 0.0, t2 = 0.0, t3 = 0.0;
     if (G * k < ntq) {
       if (row >= 1 && row < ntq) {
-        if (REG == kEigene) fin_evanescent(wvn[row], r, zfh, zph, h, t1, t2, t3);
-        else fin_pidcock_term(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3);
+        if (REG == kEigene) fin_evanescent<FAST>(wvn[row], r, zfh, zph, h, t1, t2, t3);
+        else fin_pidcock_term<FAST>(row, wvn[row], r, zfh, zph, v, h, t1, t2, t3);
       }
       t1 = LG::incl_sum(t1, l) + c1;
       t2 = LG::incl_sum(t2, l) + c2;

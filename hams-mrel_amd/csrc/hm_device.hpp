@@ -213,6 +213,102 @@ HM_DEV void sf_k01(double x, double& bk0, double& bk1) {
   bk1 = (1.0 / x - bi1 * k0v) / bi0;
 }
 
+// K0(x), K1(x) for the mixed-precision paths: the same series and expansions as
+// sf_k01 (IK01A), with the power series evaluated as fixed-degree polynomials
+// in t = x^2 / 4 by Horner's rule instead of term-by-term loops with a
+// division and a convergence test per term (A&S 9.6.10, 9.6.11, 9.6.13):
+//   I0 = sum t^k / (k!)^2,  I1 = (x / 2) sum t^k / (k! (k+1)!),
+//   K0 = -(ln(x/2) + gamma) I0 + sum_{k>=1} H_k t^k / (k!)^2   (x <= 9),
+// K0 = (1 / 2x)(1 + sum a_k x^-2k) / I0 (the I0 K0 product expansion) above 9,
+// and K1 = (1/x - I1 K0) / I0 from the Wronskian. Degree 28 covers x <= 9
+// (t <= 20.25) and degree 42 x <= 18 to below 1e-17 of I0; above 18, I0 and I1
+// come from their asymptotic expansions as in sf_k01. The results differ from
+// sf_k01 by rounding only (the K0 series cancels near x = 9 in both).
+struct BesselSeries { double i0[51], i1[51], k0[51]; };
+constexpr BesselSeries make_bessel_series() {
+  BesselSeries b{};
+  double f2 = 1.0, f1 = 1.0, hk = 0.0;   // (k!)^2, k! (k+1)!, H_k
+  b.i0[0] = 1.0;
+  b.i1[0] = 1.0;
+  b.k0[0] = 0.0;
+  for (int k = 1; k <= 50; ++k) {
+    f2 *= (double)k * k;
+    f1 *= (double)k * (k + 1);
+    hk += 1.0 / k;
+    b.i0[k] = 1.0 / f2;
+    b.i1[k] = 1.0 / f1;
+    b.k0[k] = hk / f2;
+  }
+  return b;
+}
+static __constant__ constexpr BesselSeries kBes = make_bessel_series();
+
+template <int N>
+HM_DEV double horner(const double* c, double t) {
+  double s = c[N];
+#pragma unroll
+  for (int k = N - 1; k >= 0; --k) s = fma(s, t, c[k]);
+  return s;
+}
+
+HM_DEV void sf_k01_fast(double x, double& bk0, double& bk1) {
+  if (x == 0.0) { bk0 = 1.0e300; bk1 = 1.0e300; return; }
+  const double t = 0.25 * x * x;
+  double bi0, bi1, k0v;
+  if (x <= 9.0) {
+    bi0 = horner<28>(kBes.i0, t);
+    bi1 = 0.5 * x * horner<28>(kBes.i1, t);
+    k0v = -(log(0.5 * x) + 0.5772156649015329) * bi0 + horner<28>(kBes.k0, t);
+  } else {
+    if (x <= 18.0) {
+      bi0 = horner<42>(kBes.i0, t);
+      bi1 = 0.5 * x * horner<42>(kBes.i1, t);
+    } else {
+      const double a[12] = {0.125, 7.03125e-2, 7.32421875e-2, 1.1215209960938e-1,
+                            2.2710800170898e-1, 5.7250142097473e-1, 1.7277275025845,
+                            6.0740420012735, 2.4380529699556e01, 1.1001714026925e02,
+                            5.5133589612202e02, 3.0380905109224e03};
+      const double b[12] = {-0.375, -1.171875e-1, -1.025390625e-1, -1.4419555664063e-1,
+                            -2.7757644653320e-1, -6.7659258842468e-1, -1.9935317337513,
+                            -6.8839142681099, -2.7248827311269e01, -1.2159789187654e02,
+                            -6.0384407670507e02, -3.3022722944809e03};
+      const int kn = x >= 50.0 ? 7 : (x >= 35.0 ? 9 : 12);
+      const double ca = exp(x) / sqrt(2.0 * kPi * x), xr = 1.0 / x;
+      double p = 1.0;
+      bi0 = 1.0;
+      bi1 = 1.0;
+      for (int k = 0; k < kn; ++k) {
+        p *= xr;
+        bi0 += a[k] * p;
+        bi1 += b[k] * p;
+      }
+      bi0 *= ca;
+      bi1 *= ca;
+    }
+    const double a1[8] = {0.125, 0.2109375, 1.0986328125, 1.1775970458984e01,
+                          2.1461706161499e02, 5.9511522710323e03, 2.3347645606175e05,
+                          1.2312234987631e07};
+    const double xr2 = 1.0 / (x * x);
+    double p = 1.0;
+    k0v = 1.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      p *= xr2;
+      k0v += a1[k] * p;
+    }
+    k0v = 0.5 / x * k0v / bi0;
+  }
+  bk0 = k0v;
+  bk1 = (1.0 / x - bi1 * k0v) / bi0;
+}
+
+// the reference's K0/K1 (plain path) or the fast evaluation (mixed paths)
+template <bool FAST>
+HM_DEV void sf_k01_t(double x, double& bk0, double& bk1) {
+  if (FAST) sf_k01_fast(x, bk0, bk1);
+  else sf_k01(x, bk0, bk1);
+}
+
 // J0, J1, Y0, Y1 polynomial fits (JY01B :663; 1.0766115157 is a REAL(4) literal)
 HM_DEV void sf_jy01(double x, double& j0, double& j1, double& y0, double& y1) {
   if (x == 0.0) { j0 = 1.0; j1 = 0.0; y0 = -1.0e300; y1 = -1.0e300; return; }
@@ -298,13 +394,15 @@ HM_DEV void fin_strip(double r, double zf, double zp, FinOut& o) {
   o.gz_re += ((zf - zp) / sr) / (sr * sr) + ((zf + zp) / sr1) / (sr1 * sr1);
 }
 
-// one evanescent term of the eigenfunction series (EIGEN/EIGENE loop body)
+// one evanescent term of the eigenfunction series (EIGEN/EIGENE loop body);
+// FAST: K0/K1 by sf_k01_fast (mixed-precision paths)
+template <bool FAST = false>
 HM_DEV void fin_evanescent(double km, double r, double zfh, double zph, double h, double& t1,
                            double& t2, double& t3) {
   const double twokmh = 2.0 * km * h;
   const double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
   double k0, k1;
-  sf_k01(km * r, k0, k1);
+  sf_k01_t<FAST>(km * r, k0, k1);
   double skzf, ckzf;
   sincos(km * zfh, &skzf, &ckzf);
   const double ckzp = cos(km * zph);
@@ -314,13 +412,14 @@ HM_DEV void fin_evanescent(double km, double r, double zfh, double zph, double h
 }
 
 // fin_evanescent and t4, the derivative of the term with respect to zph (the
-// mirrored pair's t3; t1, t2 are symmetric in zfh, zph)
+// mirrored pair's t3; t1, t2 are symmetric in zfh, zph); mixed paths only, so
+// K0/K1 by sf_k01_fast
 HM_DEV void fin_evanescent_sym(double km, double r, double zfh, double zph, double h, double& t1, double& t2,
                                double& t3, double& t4) {
   const double twokmh = 2.0 * km * h;
   const double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
   double k0, k1;
-  sf_k01(km * r, k0, k1);
+  sf_k01_fast(km * r, k0, k1);
   double skzf, ckzf;
   sincos(km * zfh, &skzf, &ckzf);
   const double ckzp = cos(km * zph), skzp = sin(km * zph);
@@ -473,13 +572,15 @@ HM_DEV void fin_pidcock_head(double r, double zf, double zp, const double* wvn, 
   o.gz_re = dp0 * -y0;
   o.gz_im = dp0 * j0;
 }
-// term i >= 1 of the three accelerated series (mode i minus its large-i form)
+// term i >= 1 of the three accelerated series (mode i minus its large-i form);
+// FAST: K0/K1 by sf_k01_fast (mixed-precision paths)
+template <bool FAST = false>
 HM_DEV void fin_pidcock_term(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
                              double& t2, double& t3) {
   const double umt = i * kPi / h;
   double k0, k1, k0t, k1t;
-  sf_k01(um * r, k0, k1);
-  sf_k01(umt * r, k0t, k1t);
+  sf_k01_t<FAST>(um * r, k0, k1);
+  sf_k01_t<FAST>(umt * r, k0t, k1t);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
   double su, cu, sut, cut;
   sincos(um * zfh, &su, &cu);
@@ -495,15 +596,16 @@ HM_DEV void fin_pidcock_term(int i, double um, double r, double zfh, double zph,
 }
 // fin_pidcock_term and t4, the derivative of the term with respect to zph:
 // the term of the mirrored pair (field and source depths swapped) has t1, t2
-// unchanged and t4 as its t3 (the series is symmetric in zfh, zph)
+// unchanged and t4 as its t3 (the series is symmetric in zfh, zph); mixed paths
+// only, so K0/K1 by sf_k01_fast
 HM_DEV void fin_pidcock_term_sym(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
                                  double& t2, double& t3, double& t4) {
   const double umt = i * kPi / h;
   double k0, k1, k0t, k1t;
-  sf_k01(um * r, k0, k1);
-  sf_k01(umt * r, k0t, k1t);
+  sf_k01_fast(um * r, k0, k1);
+  sf_k01_fast(umt * r, k0t, k1t);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-  // t1..t3 as fin_pidcock_term computes them (bit for bit)
+  // t1..t3 with the operations of fin_pidcock_term
   double su, cu, sut, cut;
   sincos(um * zfh, &su, &cu);
   sincos(umt * zfh, &sut, &cut);
@@ -542,7 +644,7 @@ HM_DEV void fin_pidcock_tilde(int i, double r, double zfh, double zph, double h,
 HM_DEV void fin_pidcock_term_cached(double um, double r, double zfh, double zph, double v, double h, const double* c,
                                     double& t1, double& t2, double& t3, double& t4) {
   double k0, k1;
-  sf_k01(um * r, k0, k1);
+  sf_k01_fast(um * r, k0, k1);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
   double su, cu;
   sincos(um * zfh, &su, &cu);

@@ -65,7 +65,8 @@ __device__ __forceinline__ void pair_entry(const GeomDev& g, const FreqDev& f, c
 // would spill the whole kernel) are left out: zeros and deferred_entry() true;
 // callers evaluate them with pair_entry. PID = false: PIDCOCK entries are left
 // out too (an EIGENE-only kernel, for entries that are PIDCOCK only by
-// exception).
+// exception). The cross approximation (mixed path) is its caller: the series
+// take K0/K1 from sf_k01_fast.
 __device__ __forceinline__ bool deferred_entry(const GeomDev& g, const FreqDev& f, int i, int j, bool pid) {
   if (!(f.h > 0.0) || fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8)) return false;
   const double* xp = g.fp + 3 * (long long)i;
@@ -97,8 +98,8 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
     const double r = group_bcast<G>(rr, q);
     const double zfh = group_bcast<G>(zf, q) + h, zph = group_bcast<G>(zp, q) + h;
     double l1, l2, l3;
-    if (PID && rq == kPidcock) coop_series<kPidcock, G, KP>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
-    else coop_series<kEigene, G, KE>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
+    if (PID && rq == kPidcock) coop_series<kPidcock, G, KP, true>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
+    else coop_series<kEigene, G, KE, true>(ntq, r, zfh, zph, h, f.v, wvn, l, l1, l2, l3);
     if (l == q) { L1 = l1; L2 = l2; L3 = l3; }
   }
   ar = ai = sr = si = 0.0;

@@ -28,7 +28,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
-           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
+           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
@@ -364,6 +364,15 @@ class Context:
         self._chk(self.L.hm_sglintbd_batch(self.ptr, _i(len(ncn)), _p(ncn), _p(verts), _p(xf), _p(s), _p(d)),
                   "hm_sglintbd_batch")
         return s, d
+
+    def ik01a(self, x, fast=False):
+        """K0(x), K1(x) on the GPU: the reference's IK01A evaluation, or (fast) the
+        mixed-precision kernels' one"""
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        k0, k1 = np.zeros(len(x)), np.zeros(len(x))
+        self._chk(self.L.hm_ik01a_batch(self.ptr, _i(len(x)), _p(x), _p(k0), _p(k1), _i(1 if fast else 0)),
+                  "hm_ik01a_batch")
+        return k0, k1
 
     def zgetrf(self, a):
         a = np.array(a, dtype=np.complex128, order="F", copy=True)

@@ -297,6 +297,11 @@ int hm_infgreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, 
  * source panel's vertices verts[12*i] (ncn[i] of them) and field point xf[3*i]. */
 int hm_sglintbd_batch(hm_ctx *ctx, int n, const int *ncn, const double *verts, const double *xf, double *sij,
                       double *dij);
+/* IK01A(X,BI0,DI0,BI1,DI1,BK0,DK0,BK1,DK1)  FinGrnExtSubs.f90:538, the K0 and K1
+ * outputs, at n points: mode 0 evaluates as the reference does (the plain
+ * path's kernels), mode 1 as the mixed-precision kernels do (the same series
+ * as fixed-degree polynomials, equal to rounding). */
+int hm_ik01a_batch(hm_ctx *ctx, int n, const double *x, double *bk0, double *bk1, int mode);
 /* DISPERSION(WVN,NRT,W,H)  WavDynSubs.f90:40 (host; g_disp = 9.807 in the reference) */
 int hm_dispersion(double *wvn, int nrt, double w, double h, double g_disp);
 

@@ -141,6 +141,58 @@ def test_dispersion_matches_oracle(oracle):
             np.testing.assert_array_equal(hm_bem.dispersion(w, h, 9.807), oracle.dispersion(w, h))
 
 
+# ---- IK01A: the reference's K0/K1 and the mixed kernels' Horner form ---------------------
+def test_ik01a_reference_and_fast_forms(ctx, oracle):
+    """Mode 0 is the reference's IK01A as the plain kernels evaluate it: the
+    oracle's restatement up to the device's log/exp roundings, which the K0
+    series' cancellation near x = 9 amplifies (1e-10; nearly all values equal
+    to the last bit). Mode 1 (the mixed-precision
+    kernels: the same series as fixed-degree Horner polynomials) is checked
+    against scipy's K0/K1 (an independent implementation): to 1e-14 where the
+    K0 series does not cancel, and as the reference itself is near x = 9, where
+    -(ln(x/2) + gamma) I0 and the sum cancel by ~e^(2x) (both forms lose the same
+    digits there)."""
+    import scipy.special as sp
+    x = np.concatenate([np.geomspace(1e-4, 2.0, 300), np.linspace(2.0, 18.0, 801)[1:],
+                        np.geomspace(18.0, 80.0, 200)[1:]])
+    r0, r1 = ctx.ik01a(x)
+    ref = np.array([oracle.ik01a(float(v)) for v in x])
+    for got, want in ((r0, ref[:, 0]), (r1, ref[:, 1])):
+        assert np.max(np.abs(got / want - 1)) < 1e-10
+        assert np.count_nonzero(got != want) < 0.05 * len(x)
+    f0, f1 = ctx.ik01a(x, fast=True)
+    s0, s1 = sp.k0(x), sp.k1(x)
+    e0, e1 = np.abs(f0 / s0 - 1), np.abs(f1 / s1 - 1)
+    g0, g1 = np.abs(r0 / s0 - 1), np.abs(r1 / s1 - 1)
+    small = x <= 2.0
+    for e in (e0, e1):
+        assert e[small].max() < 1e-14, e[small].max()
+        assert e.max() < 1e-7, e.max()
+    # x > 2, where the K0 series cancel (to 18) and the asymptotic expansions of
+    # the reference take over (above 18, ~1e-12 at x = 18): per window of x, no
+    # worse than the reference's own evaluation (a wrong coefficient would show
+    # at O(1) there)
+    edges = np.concatenate([np.linspace(2.0, 18.0, 9), [30.0, 50.0, 80.0]])
+    for a, b in zip(edges[:-1], edges[1:]):
+        w = (x > a) & (x <= b)
+        for e, g in ((e0, g0), (e1, g1)):
+            assert e[w].max() <= 4 * g[w].max() + 1e-13, (a, b, e[w].max(), g[w].max())
+    print("fast K0/K1 vs scipy: max", e0.max(), e1.max(), "reference IK01A vs scipy: max", g0.max(), g1.max(),
+          flush=True)
+
+
+def test_ik01a_argument_errors(ctx):
+    import ctypes
+
+    import hm_bem
+    L = hm_bem.load()
+    x = np.ones(3)
+    k = np.zeros(3)
+    rc = L.hm_ik01a_batch(ctx.ptr, 3, x.ctypes.data_as(ctypes.c_void_p), k.ctypes.data_as(ctypes.c_void_p),
+                          k.ctypes.data_as(ctypes.c_void_p), 2)
+    assert rc == -106
+
+
 # ---- dense LU -------------------------------------------------------------------------
 @pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 130, 257, 700, 1337])
 def test_zgetrf_zgetrs_vs_textbook(ctx, oracle, n):
