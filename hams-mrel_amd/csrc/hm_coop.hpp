@@ -192,6 +192,10 @@ template <> __device__ __forceinline__ int group_min<16>(int v) {
   return min(v, dpp_i<0x140>(v));
 }
 
+// column pairs of the epsilon table on the mixed-precision paths (the plain
+// kernels build the whole table, as Limes does)
+constexpr int kMixedEpsPairs = 12;
+
 // Wynn epsilon limit of the partial sums s[0..n] held as rows l + G*k of the
 // group (Limes :370-503: same columns, same exact-equality tests, and the same
 // result of its row-order running-maximum selection). s is overwritten.
@@ -202,7 +206,7 @@ template <> __device__ __forceinline__ int group_min<16>(int v) {
 // maximum, found by a group max and a group min of row numbers. An exit needs a
 // row with an exact repeat (s_old[i+1] == s_new[i]) and |a| > M0; only columns
 // that have such a row are scanned row by row.
-template <int G, int K, bool FAST_RCP = true>
+template <int G, int K, bool FAST_RCP = true, int KKMAX = 1 << 20>
 __device__ double coop_limes(int n, double (&s)[K], int l) {
   typedef LaneGroup<G> LG;
   auto inv = [](double x) { return FAST_RCP ? rcp_nr(x) : 1.0 / x; };
@@ -216,7 +220,7 @@ __device__ double coop_limes(int n, double (&s)[K], int l) {
   }
   double rl = group_bcast<G>(rl_own, n % G);
   double amax = 0.0;
-  for (int kk = 1;; ++kk) {
+  for (int kk = 1; kk <= KKMAX; ++kk) {
     const int lim_a = n - 2 * kk + 1;
     if (lim_a < 0) break;
     // odd column: a[i] = a[i+1] + 1/(s[i+1]-s[i]) for i <= lim_a (slot k+1 is
@@ -323,9 +327,9 @@ __device__ __forceinline__ void coop_series(int ntq, double r, double zfh, doubl
     s3[k] = t3;
   }
   const int n = ntq - 1;
-  l1 = coop_limes<G, K>(n, s1, l);
-  l2 = coop_limes<G, K>(n, s2, l);
-  l3 = coop_limes<G, K>(n, s3, l);
+  l1 = coop_limes<G, K, true, FAST ? kMixedEpsPairs : (1 << 20)>(n, s1, l);
+  l2 = coop_limes<G, K, true, FAST ? kMixedEpsPairs : (1 << 20)>(n, s2, l);
+  l3 = coop_limes<G, K, true, FAST ? kMixedEpsPairs : (1 << 20)>(n, s3, l);
 }
 
 // PIDCOCK or EIGENE series of one pair and of its mirror (depths swapped) by
@@ -369,10 +373,10 @@ __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, d
     s4[k] = t4;
   }
   const int n = ntq - 1;
-  l1 = coop_limes<G, K>(n, s1, l);
-  l2 = coop_limes<G, K>(n, s2, l);
-  l3 = coop_limes<G, K>(n, s3, l);
-  l4 = coop_limes<G, K>(n, s4, l);
+  l1 = coop_limes<G, K, true, kMixedEpsPairs>(n, s1, l);
+  l2 = coop_limes<G, K, true, kMixedEpsPairs>(n, s2, l);
+  l3 = coop_limes<G, K, true, kMixedEpsPairs>(n, s3, l);
+  l4 = coop_limes<G, K, true, kMixedEpsPairs>(n, s4, l);
 }
 
 }  // namespace hm
