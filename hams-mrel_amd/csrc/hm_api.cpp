@@ -148,6 +148,20 @@ struct hm_ctx {
   double stage_ms[kStages] = {0};
   int stage_cnt[kStages] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // H-matrix path: the block preconditioner is factored (and inverted) on a
+  // second stream while the cross approximation runs on the first
+  hipStream_t side = nullptr;
+  hipEvent_t ev_diag = nullptr, ev_prec = nullptr;
+  bool side_pending = false;     // work on `side` the main stream has not joined yet
+  bool prec_early = false;       // that work is the preconditioner of the current assembly and settings
+  DevBuf d_info2, d_counter2;    // its info words and counters
+  // make the main stream wait for the side stream's work
+  void join_side() {
+    if (side_pending) {
+      (void)hipStreamWaitEvent(stream, ev_prec, 0);
+      side_pending = false;
+    }
+  }
   struct Rec { int stage; hipEvent_t a, b; };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
@@ -286,8 +300,11 @@ extern "C" int hm_create(hm_ctx** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HM_ERR_HIP - (int)hipErrorNoDevice;
   hm_ctx* c = new hm_ctx();
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_diag, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_prec, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return HM_ERR_HIP - (int)e;
@@ -298,17 +315,21 @@ extern "C" int hm_create(hm_ctx** out) {
 
 extern "C" int hm_destroy(hm_ctx* c) {
   if (!c) return HM_ERR_ARG - 1;
+  if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->d_body, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2};
   for (DevBuf* b : bufs) b->release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_diag) (void)hipEventDestroy(c->ev_diag);
+  if (c->ev_prec) (void)hipEventDestroy(c->ev_prec);
+  if (c->side) (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return HM_OK;
@@ -594,6 +615,8 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
   return HM_OK;
 }
 
+static int precond_early(hm_ctx* c);
+
 // H-matrix form: diagonal blocks of A and S densely (the classified pairs of
 // those blocks), the off-diagonal blocks by cross approximation, and the
 // right-hand sides S W through the H-matrix product in FP64
@@ -613,9 +636,12 @@ static int assemble_hmatrix(hm_ctx* c) {
   const hm::FreqDev f = c->freq();
   double* D = c->d_hD.as<double>();
   double* DS = c->d_hDS.as<double>();
+  c->join_side();          // the previous preconditioner still reads D
+  c->prec_early = false;
   c->stage_begin(kAssembly);
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
+  if ((rc = precond_early(c))) return rc;
   // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
   c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
   c->stage_begin(kAsmCross);
@@ -682,6 +708,42 @@ static int precond_prepare(hm_ctx* c, hipStream_t s) {
   return HM_OK;
 }
 
+// H-matrix form, right after the diagonal blocks are assembled: their FP32
+// copies, the block preconditioner's LU and (precond 2, -1) its inverses on the
+// side stream, overlapping the cross approximation on the main stream. Buffers
+// are sized first, so that no allocation happens while the side stream runs.
+static int precond_early(hm_ctx* c) {
+  if (c->gm_precond == 0) return HM_OK;
+  const int nb = c->nbody, m = c->bm();
+  const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+  HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+  HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+  HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
+  HIPCHK(c, c->d_info2.ensure(sizeof(int) * (size_t)nb), "factor alloc");
+  HIPCHK(c, c->d_counter2.ensure(sizeof(unsigned) * (size_t)nb), "factor alloc");
+  if (block_inverse(c)) {
+    HIPCHK(c, c->d_pperm.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
+    HIPCHK(c, c->d_Pinv.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+  }
+  const double* D = c->d_hD.as<double>();
+  hipStream_t s = c->side;
+  HIPCHK(c, hipEventRecord(c->ev_diag, c->stream), "preconditioner");
+  HIPCHK(c, hipStreamWaitEvent(s, c->ev_diag, 0), "preconditioner");
+  c->side_pending = true;
+  HIPCHK(c, hipMemsetAsync(c->d_info2.p, 0, sizeof(int) * nb, s), "preconditioner");
+  HIPCHK(c, hipMemsetAsync(c->d_counter2.p, 0, sizeof(unsigned) * nb, s), "preconditioner");
+  HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::demote(s, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
+  HIPCHK(c, hm::lu_factor<float>(s, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info2.as<int>(),
+                                 c->d_counter2.as<unsigned>(), nb), "block preconditioner");
+  c->gm_blocks = c->body_start;
+  int rc = precond_prepare(c, s);
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev_prec, s), "preconditioner");
+  c->prec_early = true;
+  return HM_OK;
+}
+
 // H-matrix form of hm_factor: FP32 copies of the diagonal blocks and of A's
 // low-rank factors for the Krylov products, and the block-diagonal
 // preconditioner (one block per body) factored in FP32
@@ -689,6 +751,20 @@ static int factor_hmatrix(hm_ctx* c) {
   const int nb = c->nbody, m = c->bm(), k = c->hm_kuse[0] ? c->hm_kuse[0] : 1;
   const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
   const size_t lr = (size_t)nb * (nb - 1) * m * k;   // complex entries of A's packed U (or V)
+  c->join_side();
+  if (c->prec_early) {
+    // the diagonal blocks were demoted, factored and inverted on the side
+    // stream during the cross approximation: its info words, then A's factors
+    c->prec_early = false;
+    HIPCHK(c, hipMemcpyAsync(c->d_info.p, c->d_info2.p, sizeof(int) * nb, hipMemcpyDeviceToDevice, c->stream), "factor");
+    HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "factor alloc");
+    HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "factor alloc");
+    HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1),
+           "demote");
+    HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
+           "demote");
+    return HM_OK;
+  }
   HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
   HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
   HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
@@ -755,6 +831,7 @@ static int sys_ld_rhs(hm_ctx* c) { return c->ni > 0 ? c->n : c->nt(); }
 extern "C" int hm_factor(hm_ctx* c, int precision) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->assembled) return c->fail(HM_ERR_STATE, "hm_factor before hm_assemble");
+  c->join_side();
   if (precision != HM_FP64 && precision != HM_FP32_IR && precision != HM_GMRES)
     return c->fail(HM_ERR_ARG - 2, "unknown precision %d", precision);
   const int n = c->n;
@@ -954,6 +1031,7 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
   c->gm_precond = precond;
   c->factored = c->solved = false;
   c->pinv_ready = false;
+  c->prec_early = false;
   return HM_OK;
 }
 
@@ -1041,6 +1119,8 @@ extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, d
 // copies and the block-diagonal preconditioner for GMRES are set up with it
 extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int nblocks, const int* block_sizes) {
   if (!c) return HM_ERR_ARG - 1;
+  c->join_side();
+  c->prec_early = false;
   if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
   if (!a || lda < n) return c->fail(HM_ERR_ARG - 3, "bad matrix");
   if (nblocks < 2 || !block_sizes) return c->fail(HM_ERR_ARG - 5, "the H-matrix needs >= 2 cluster blocks");
@@ -1126,6 +1206,8 @@ extern "C" int hm_hmatrix_release(hm_ctx* c) {
 extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int lda, const double* b, int ldb,
                               double* x, int ldx, int nrhs, int nblocks, const int* block_sizes, int* info) {
   if (!c) return HM_ERR_ARG - 1;
+  c->join_side();
+  c->prec_early = false;
   if (isolv != 1 && isolv != 2) return c->fail(HM_ERR_ARG - 2, "isolv must be 1 (dense) or 2 (H-matrix)");
   if (n < 1) return c->fail(HM_ERR_ARG - 3, "n must be >= 1");
   if (isolv == 1 && (!a || lda < n)) return c->fail(HM_ERR_ARG - 5, "bad matrix");
