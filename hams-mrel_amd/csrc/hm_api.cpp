@@ -148,17 +148,19 @@ struct hm_ctx {
   double stage_ms[kStages] = {0};
   int stage_cnt[kStages] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // H-matrix path: the block preconditioner is factored (and inverted) on a
-  // second stream while the cross approximation runs on the first
+  // H-matrix path: the cross approximation runs on a second stream while the
+  // main stream assembles the diagonal blocks and factors (and inverts) the
+  // block preconditioner
   hipStream_t side = nullptr;
-  hipEvent_t ev_diag = nullptr, ev_prec = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool side_pending = false;     // work on `side` the main stream has not joined yet
-  bool prec_early = false;       // that work is the preconditioner of the current assembly and settings
+  bool prec_early = false;       // the preconditioner of the current assembly and settings is built
   DevBuf d_info2, d_counter2;    // its info words and counters
   // make the main stream wait for the side stream's work
   void join_side() {
     if (side_pending) {
-      (void)hipStreamWaitEvent(stream, ev_prec, 0);
+      (void)hipEventRecord(ev_join, side);
+      (void)hipStreamWaitEvent(stream, ev_join, 0);
       side_pending = false;
     }
   }
@@ -210,22 +212,24 @@ struct hm_ctx {
     else (void)hipEventCreate(&e);
     return e;
   }
-  void stage_begin(int s) {
+  void stage_begin(int s, hipStream_t on = nullptr) {
     if (!timing) return;
     open_ev[s] = take_event();
-    (void)hipEventRecord(open_ev[s], stream);
+    (void)hipEventRecord(open_ev[s], on ? on : stream);
   }
-  int stage_end(int s) {
+  int stage_end(int s, hipStream_t on = nullptr) {
     if (!timing) return HM_OK;
     hipEvent_t e = take_event();
-    (void)hipEventRecord(e, stream);
+    (void)hipEventRecord(e, on ? on : stream);
     recs.push_back({s, open_ev[s], e});
     return HM_OK;
   }
   int resolve_timing() {
     if (recs.empty()) return HM_OK;
-    hipError_t e = hipEventSynchronize(recs.back().b);
-    if (e != hipSuccess) return hip(e, "stage timing");
+    for (const Rec& r : recs) {   // stages may end on either stream
+      hipError_t e = hipEventSynchronize(r.b);
+      if (e != hipSuccess) return hip(e, "stage timing");
+    }
     for (const Rec& r : recs) {
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, r.a, r.b);
@@ -303,8 +307,8 @@ extern "C" int hm_create(hm_ctx** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_diag, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_prec, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return HM_ERR_HIP - (int)e;
@@ -327,8 +331,8 @@ extern "C" int hm_destroy(hm_ctx* c) {
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->ev_diag) (void)hipEventDestroy(c->ev_diag);
-  if (c->ev_prec) (void)hipEventDestroy(c->ev_prec);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->side) (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -636,17 +640,25 @@ static int assemble_hmatrix(hm_ctx* c) {
   const hm::FreqDev f = c->freq();
   double* D = c->d_hD.as<double>();
   double* DS = c->d_hDS.as<double>();
-  c->join_side();          // the previous preconditioner still reads D
+  c->join_side();
   c->prec_early = false;
   c->stage_begin(kAssembly);
+  // the off-diagonal blocks by cross approximation on the side stream (from the
+  // state the main stream has reached), the diagonal blocks and the block
+  // preconditioner on the main stream: everything on the main stream is
+  // enqueued before the cross approximation's host loop starts
+  HIPCHK(c, hipEventRecord(c->ev_fork, c->stream), "hmatrix");
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "hmatrix");
+  c->side_pending = true;
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
   if ((rc = precond_early(c))) return rc;
   // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
   c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
-  c->stage_begin(kAsmCross);
-  HIPCHK(c, hm::aca_build(c->stream, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
-  c->stage_end(kAsmCross);
+  c->stage_begin(kAsmCross, c->side);
+  HIPCHK(c, hm::aca_build(c->side, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
+  c->stage_end(kAsmCross, c->side);
+  c->join_side();
   if ((rc = c->stage_end(kAssembly))) return rc;
   const int P = hm::hmat_processes(nb);
   std::vector<int> rank(P);
@@ -710,8 +722,9 @@ static int precond_prepare(hm_ctx* c, hipStream_t s) {
 
 // H-matrix form, right after the diagonal blocks are assembled: their FP32
 // copies, the block preconditioner's LU and (precond 2, -1) its inverses on the
-// side stream, overlapping the cross approximation on the main stream. Buffers
-// are sized first, so that no allocation happens while the side stream runs.
+// main stream, overlapping the cross approximation on the side stream (own info
+// words and counters: hm_factor clears the main ones). Buffers are sized
+// first, so that no allocation happens while the side stream runs.
 static int precond_early(hm_ctx* c) {
   if (c->gm_precond == 0) return HM_OK;
   const int nb = c->nbody, m = c->bm();
@@ -726,10 +739,7 @@ static int precond_early(hm_ctx* c) {
     HIPCHK(c, c->d_Pinv.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
   }
   const double* D = c->d_hD.as<double>();
-  hipStream_t s = c->side;
-  HIPCHK(c, hipEventRecord(c->ev_diag, c->stream), "preconditioner");
-  HIPCHK(c, hipStreamWaitEvent(s, c->ev_diag, 0), "preconditioner");
-  c->side_pending = true;
+  hipStream_t s = c->stream;
   HIPCHK(c, hipMemsetAsync(c->d_info2.p, 0, sizeof(int) * nb, s), "preconditioner");
   HIPCHK(c, hipMemsetAsync(c->d_counter2.p, 0, sizeof(unsigned) * nb, s), "preconditioner");
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
@@ -739,7 +749,6 @@ static int precond_early(hm_ctx* c) {
   c->gm_blocks = c->body_start;
   int rc = precond_prepare(c, s);
   if (rc) return rc;
-  HIPCHK(c, hipEventRecord(c->ev_prec, s), "preconditioner");
   c->prec_early = true;
   return HM_OK;
 }
@@ -753,8 +762,8 @@ static int factor_hmatrix(hm_ctx* c) {
   const size_t lr = (size_t)nb * (nb - 1) * m * k;   // complex entries of A's packed U (or V)
   c->join_side();
   if (c->prec_early) {
-    // the diagonal blocks were demoted, factored and inverted on the side
-    // stream during the cross approximation: its info words, then A's factors
+    // the diagonal blocks were demoted, factored and inverted during the
+    // assembly, beside the cross approximation: their info words, then A's factors
     c->prec_early = false;
     HIPCHK(c, hipMemcpyAsync(c->d_info.p, c->d_info2.p, sizeof(int) * nb, hipMemcpyDeviceToDevice, c->stream), "factor");
     HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "factor alloc");
