@@ -966,6 +966,10 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
       ga.op32 = [=](hipStream_t s, const float* Z, float* W) {
         return hm::hmat_apply<float>(s, nb, m, k, D32, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
       };
+      // the preconditioner blocks are the FP32 LU of D32's blocks (factor_hmatrix)
+      ga.op32_off = [=](hipStream_t s, const float* Z, float* W) {
+        return hm::hmat_apply<float>(s, nb, m, k, nullptr, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
+      };
       ga.op64 = [=](hipStream_t s, const double* Z, double* W) {
         return hm::hmat_apply<double>(s, nb, m, k, D, m, sD, U, V, Z, n, W, n, ncol, tb, ts);
       };
@@ -1301,6 +1305,10 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
     const int ncol = nrhs;
     ga.op32 = [=](hipStream_t st, const float* Z, float* W) {
       return hm::hmat_apply<float>(st, nb, m, ku, D32, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
+    };
+    // the preconditioner blocks are the FP32 LU of D32's blocks (hm_hmatrix_build)
+    ga.op32_off = [=](hipStream_t st, const float* Z, float* W) {
+      return hm::hmat_apply<float>(st, nb, m, ku, nullptr, m, sc, U32, V32, Z, n, W, n, ncol, (float*)tb, (float*)ts);
     };
     ga.op64 = [=](hipStream_t st, const double* Z, double* W) {
       return hm::hmat_apply<double>(st, nb, m, ku, D, m, sD, U, V, Z, n, W, n, ncol, tb, ts);

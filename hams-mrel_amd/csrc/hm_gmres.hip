@@ -463,8 +463,15 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
         GM_CHECK(demote(s, vj, n, 0, Z32, n, 0, n, ncol, 1));
         if (a.nblk > 0) GM_CHECK(apply_precond(s, a, Z32, ncol));
       }
-      if (a.op32) GM_CHECK(a.op32(s, Z32, W32));
-      else GM_CHECK(cgemm<float>(s, false, n, ncol, n, 1.f, 0.f, a.A32, n, 0, Z32, n, 0, 0.f, 0.f, W32, n, 0, 1));
+      if (a.nblk > 0 && a.op32_off) {
+        // the diagonal blocks times their own inverses: v itself
+        GM_CHECK(demote(s, vj, n, 0, W32, n, 0, n, ncol, 1));
+        GM_CHECK(a.op32_off(s, Z32, W32));
+      } else if (a.op32) {
+        GM_CHECK(a.op32(s, Z32, W32));
+      } else {
+        GM_CHECK(cgemm<float>(s, false, n, ncol, n, 1.f, 0.f, a.A32, n, 0, Z32, n, 0, 0.f, 0.f, W32, n, 0, 1));
+      }
       GM_CHECK(promote_add(s, W32, n, 0, W, n, 0, n, ncol, 1, true));
       // classical Gram-Schmidt, twice
       hipLaunchKernelGGL(k_dots, dim3(j + 1, ncol), dim3(256), 0, s, V, vstride, W, n, active, m1, hc1);

@@ -30,6 +30,10 @@ struct GmresArgs {
   // operator form (used instead of A32 / A64 when set): op32 writes W = A Z in
   // FP32, op64 writes W = A X in FP64, both n x ncol with leading dimension n
   std::function<hipError_t(hipStream_t, const float*, float*)> op32;
+  // the blocks of P are the FP32 LU factors of A's own diagonal blocks and
+  // op32_off adds the rest of A (W += (A - D) Z): a Krylov step then forms
+  // A M^-1 v as v + (A - D) M^-1 v, without the diagonal blocks' product
+  std::function<hipError_t(hipStream_t, const float*, float*)> op32_off;
   std::function<hipError_t(hipStream_t, const double*, double*)> op64;
 };
 

@@ -503,8 +503,10 @@ hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long
                       const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack) {
   const long long rows = (long long)(nb - 1) * k;
   hipError_t e;
-  // diagonal blocks
-  e = cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb);
+  // diagonal blocks (D null: Y holds the caller's values and only the
+  // off-diagonal part is added)
+  e = D ? cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
+        : hipSuccess;
   if (e != hipSuccess || k == 0) return e;
   // T_b2 = [V_{b1,b2}]_b1^T x_b2 for every column block b2
   e = cgemm<T>(s, true, (int)rows, ncol, m, T(1), T(0), V, m, rows * m, X, (int)ldx, m, T(0), T(0), tbuf,

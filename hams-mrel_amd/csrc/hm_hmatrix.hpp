@@ -69,7 +69,7 @@ hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, doub
 // Y = H X: diagonal blocks D (block b at D + b sD complex elements,
 // column-major, ld ldd) plus the low-rank blocks of one matrix, packed as by
 // hmat_compact with rank stride k. X, Y: n x ncol (n = nb m); tbuf, tstack:
-// nb (nb-1) k ncol complex each.
+// nb (nb-1) k ncol complex each. D null: Y += the low-rank blocks' product only.
 template <typename T>
 hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long ldd, long long sD, const T* U,
                       const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack);
