@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--path", choices=("hmatrix", "gmres", "fp32ir"), default="hmatrix",
                     help="the mixed-precision solve that `value` measures")
-    ap.add_argument("--aca-eps", type=float, default=1e-5, help="cross-approximation tolerance of A")
+    ap.add_argument("--aca-eps", type=float, default=2e-5, help="cross-approximation tolerance of A")
     ap.add_argument("--aca-eps-s", type=float, default=1e-6, help="cross-approximation tolerance of S")
     ap.add_argument("--aca-kmax", type=int, default=50, help="largest rank of a cross-approximated block")
     ap.add_argument("--surrogate-tol", type=float, default=1.5e-3,
