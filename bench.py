@@ -289,6 +289,9 @@ def main():
     ap.add_argument("--series-variant", type=int, default=1, choices=(0, 1, 2),
                     help="epsilon-accelerated regimes: 1 lane groups of 8/16 (default), 2 of 4/8, 0 one pair per lane")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
+    ap.add_argument("--field-points", type=int, default=0,
+                    help="also time the potential, pressure and elevation at this many field points after the "
+                         "last measured system (CalPotentialMulti)")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the traffic-counter calibration copy")
     ap.add_argument("--assemble-only", action="store_true",
                     help="only assemble --steps systems (for hardware-counter passes over the assembly kernels)")
@@ -357,6 +360,23 @@ def main():
               file=sys.stderr, flush=True)
     dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup)
     gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
+    field = None
+    if args.field_points > 0:
+        # a grid of points around the bodies, from the free surface down to the draft
+        k = args.field_points
+        lo, hi = prob.xyz_p.min(axis=0), prob.xyz_p.max(axis=0)
+        rng = np.random.default_rng(20261205)
+        xfp = np.column_stack([rng.uniform(lo[0] - 5.0, hi[0] + 5.0, k), rng.uniform(lo[1] - 5.0, hi[1] + 5.0, k),
+                               rng.uniform(lo[2], 0.0, k)])
+        ctx.field_points(xfp[:8])
+        sync()
+        t0 = time.perf_counter()
+        ctx.field_points(xfp)
+        sync()
+        dt_f = time.perf_counter() - t0
+        field = {"points": k, "ms": 1e3 * dt_f, "point_panel_pairs_per_s": k * prob.n / dt_f,
+                 "columns": prob.m + prob.nbeta}
+        print(f"# field points: {field}", file=sys.stderr, flush=True)
     hist_mx = ctx.pair_histogram().ravel()     # pairs assembled densely on the measured path
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
@@ -446,6 +466,8 @@ def main():
         line["gmres"] = gm
     if hmat is not None:
         line["hmatrix"] = hmat
+    if field is not None:
+        line["field_points"] = field
     if sur is not None:
         line["surrogate"] = sur
     if others:

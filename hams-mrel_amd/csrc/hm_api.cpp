@@ -1808,6 +1808,51 @@ extern "C" int hm_infgreen3d_batch(hm_ctx* c, int n, const double* xf, const dou
   return green_common(c, n, true, in, v, nullptr, 0, -1.0, flag, grn);
 }
 
+// Field-point potential, pressure and elevation of every radiation mode and
+// heading (CalPotentialMulti, CalPressureMulti, CalElevationMulti:
+// PressureElevationMulti.f90:55-348) after hm_solve: the field points' S- and
+// A-like rows against the hull, pot = (Sf W - Af X) / (2 pi or 4 pi) with W the
+// right-hand sides' source terms and X the potentials, plus the incident
+// potential on the diffraction columns
+extern "C" int hm_field_points(hm_ctx* c, int nfp, const double* xfp, double* pot, double* prs, double* elv) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nfp < 0) return c->fail(HM_ERR_ARG - 2, "nfp < 0");
+  if (nfp == 0) return HM_OK;
+  if (!xfp) return c->fail(HM_ERR_ARG - 3, "xfp is NULL");
+  if (!c->solved) return c->fail(HM_ERR_STATE, "hm_field_points before hm_solve");
+  c->join_side();
+  const int n = c->n, m = 6 * c->nbody, nbeta = (int)c->beta.size(), ncol = c->ncol();
+  const size_t cplx = 2 * sizeof(double);
+  const size_t rows = cplx * (size_t)nfp * n, outs = cplx * (size_t)nfp * ncol;
+  DevBuf tmp;
+  HIPCHK(c, tmp.ensure(3 * sizeof(double) * (size_t)nfp + 256 + 2 * rows + 3 * outs), "field points alloc");
+  char* base = tmp.as<char>();
+  double* dx = reinterpret_cast<double*>(base);
+  double* Af = reinterpret_cast<double*>(base + ((3 * sizeof(double) * (size_t)nfp + 255) & ~(size_t)255));
+  double* Sf = Af + 2 * (size_t)nfp * n;
+  double* P = Sf + 2 * (size_t)nfp * n;
+  double* Pr = P + 2 * (size_t)nfp * ncol;
+  double* El = Pr + 2 * (size_t)nfp * ncol;
+  hipStream_t s = c->stream;
+  const hm::GeomDev gd = c->geom();
+  const hm::FreqDev f = c->freq();
+  int rc = c->hip(hipMemcpyAsync(dx, xfp, 3 * sizeof(double) * (size_t)nfp, hipMemcpyHostToDevice, s), "field points");
+  if (!rc) rc = c->hip(hm::field_rows(s, gd, f, dx, nfp, Af, Sf), "field points");
+  if (!rc) rc = c->hip(hm::cgemm<double>(s, false, nfp, ncol, n, 1.0, 0.0, Sf, nfp, 0, c->d_W.as<double>(), n, 0, 0.0,
+                                         0.0, P, nfp, 0, 1), "field points");
+  if (!rc) rc = c->hip(hm::cgemm<double>(s, false, nfp, ncol, n, -1.0, 0.0, Af, nfp, 0, c->d_X.as<double>(), n, 0, 1.0,
+                                         0.0, P, nfp, 0, 1), "field points");
+  if (!rc) rc = c->hip(hm::field_finish(s, gd, f, dx, nfp, m, nbeta, c->d_beta.as<double>(), c->xw, c->yw, P, Pr, El),
+                       "field points");
+  if (!rc && pot) rc = c->hip(hipMemcpyAsync(pot, P, outs, hipMemcpyDeviceToHost, s), "field points");
+  if (!rc && prs) rc = c->hip(hipMemcpyAsync(prs, Pr, outs, hipMemcpyDeviceToHost, s), "field points");
+  if (!rc && elv) rc = c->hip(hipMemcpyAsync(elv, El, outs, hipMemcpyDeviceToHost, s), "field points");
+  if (!rc) rc = c->hip(hipStreamSynchronize(s), "field points");
+  (void)hipStreamSynchronize(s);
+  tmp.release();
+  return rc;
+}
+
 extern "C" int hm_ik01a_batch(hm_ctx* c, int n, const double* x, double* bk0, double* bk1, int mode) {
   if (!c) return HM_ERR_ARG - 1;
   if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");

@@ -798,4 +798,108 @@ hipError_t rankine_batch(hipStream_t s, int n, const int* ncn, const double* ver
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Field points (CalPotentialMulti, PressureElevationMulti.f90:55-272)
+// ---------------------------------------------------------------------------
+
+// S-like (DUM(1)) and A-like (DUM(2)) entries of nfp field points against every
+// hull panel, as the assembly's rows but without the free term: Sf, Af are
+// nfp x n, column-major (entry (p, j) at p + j nfp), one thread per entry
+__global__ __launch_bounds__(256) void k_field_rows(GeomDev g, FreqDev f, const double* __restrict__ xfp, int nfp,
+                                                    double* __restrict__ Af, double* __restrict__ Sf) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nfp * g.n) return;
+  const int p = (int)(t % nfp), j = (int)(t / nfp);
+  const double* xp = xfp + 3 * (long long)p;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  if (f.h < 0.0) infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
+  else fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, f.wvn, f.h, flag, gr);
+  double ar, ai, sr, si;
+  pair_values(g, -1, j, xp, flag, gr, ar, ai, sr, si);   // -1: no panel is the field point's own
+  Af[2 * t] = ar;
+  Af[2 * t + 1] = ai;
+  Sf[2 * t] = sr;
+  Sf[2 * t + 1] = si;
+}
+
+// the field point lies on hull panel j: within max(1e-6, 1e-3 PNSZ) of its plane
+// and inside its polygon (the cross products of the edges with the point keep
+// one sign, to 1e-3 PNSZ^2) -- CalPotentialMulti's free-term test
+__device__ bool on_panel(const GeomDev& g, int j, const double* x) {
+  const double* v = g.verts + 12 * (long long)j;
+  const double* en = g.dxyz + 6 * (long long)j;
+  const int nv = g.ncn[j];
+  const double tolp = fmax(1.0e-6, 1.0e-3 * g.pnsz[j]), tola = 1.0e-3 * g.pnsz[j] * g.pnsz[j];
+  const double dpln = (x[0] - v[0]) * en[0] + (x[1] - v[1]) * en[1] + (x[2] - v[2]) * en[2];
+  if (fabs(dpln) > tolp) return false;
+  double smin = 1.0e30, smax = -1.0e30;
+  for (int k = 0; k < nv; ++k) {
+    const double* a = v + 3 * k;
+    const double* b = v + 3 * (k + 1 < nv ? k + 1 : 0);
+    const double ev0 = b[0] - a[0], ev1 = b[1] - a[1], ev2 = b[2] - a[2];
+    const double wv0 = x[0] - a[0], wv1 = x[1] - a[1], wv2 = x[2] - a[2];
+    const double sv = (ev1 * wv2 - ev2 * wv1) * en[0] + (ev2 * wv0 - ev0 * wv2) * en[1] + (ev0 * wv1 - ev1 * wv0) * en[2];
+    smin = fmin(smin, sv);
+    smax = fmax(smax, sv);
+  }
+  return smin >= -tola || smax <= tola;
+}
+
+// one workgroup per field point: the free term (2 pi on the hull, 4 pi in the
+// fluid), the incident potential on the diffraction columns (ISOL = 1), and
+// pressure i w rho phi and elevation i w / g phi (rho phi and phi at zero or
+// infinite frequency); pot holds the integral sums on entry
+__global__ __launch_bounds__(256) void k_field_finish(GeomDev g, FreqDev f, const double* __restrict__ xfp, int nfp,
+                                                      int m, int nbeta, const double* __restrict__ beta, double xw,
+                                                      double yw, double* __restrict__ pot, double* __restrict__ prs,
+                                                      double* __restrict__ elv) {
+  const int p = blockIdx.x;
+  const double* x = xfp + 3 * (long long)p;
+  int on = 0;
+  for (int j = threadIdx.x; j < g.n && !on; j += blockDim.x) on = on_panel(g, j, x) ? 1 : 0;
+  on = __syncthreads_or(on);
+  const double sld = on ? 2.0 * kPi : 4.0 * kPi;
+  const bool limit = fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8);
+  for (int c = threadIdx.x; c < m + nbeta; c += blockDim.x) {
+    const long long o = 2 * ((long long)p + (long long)c * nfp);
+    double pr = pot[o] / sld, pi = pot[o + 1] / sld;
+    if (c >= m && !limit) {
+      double vr, vi;
+      vinp(f, x, beta[c - m], xw, yw, vr, vi);
+      pr += vr;
+      pi += vi;
+    }
+    pot[o] = pr;
+    pot[o + 1] = pi;
+    if (limit) {
+      prs[o] = f.rho * pr;
+      prs[o + 1] = f.rho * pi;
+      elv[o] = pr;
+      elv[o + 1] = pi;
+    } else {
+      prs[o] = -f.w1 * f.rho * pi;
+      prs[o + 1] = f.w1 * f.rho * pr;
+      elv[o] = -f.w1 / f.g * pi;
+      elv[o + 1] = f.w1 / f.g * pr;
+    }
+  }
+}
+
+hipError_t field_rows(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* xfp, int nfp, double* Af,
+                      double* Sf) {
+  const long long tot = (long long)nfp * g.n;
+  hipLaunchKernelGGL(k_field_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, g, f, xfp, nfp, Af, Sf);
+  return hipGetLastError();
+}
+
+hipError_t field_finish(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* xfp, int nfp, int m,
+                        int nbeta, const double* beta, double xw, double yw, double* pot, double* prs, double* elv) {
+  hipLaunchKernelGGL(k_field_finish, dim3(nfp), dim3(256), 0, s, g, f, xfp, nfp, m, nbeta, beta, xw, yw, pot, prs,
+                     elv);
+  return hipGetLastError();
+}
+
 }  // namespace hm

@@ -90,5 +90,13 @@ hipError_t green_batch(hipStream_t s, int n, bool deep, const double* xf, const 
 hipError_t rankine_batch(hipStream_t s, int n, const int* ncn, const double* verts, const double* xf, double* out);
 // K0, K1 at n points (out: 2 per point); mode 0 IK01A as the reference, 1 the fast form
 hipError_t ik01a_batch(hipStream_t s, int n, const double* x, int mode, double* out);
+// field points (CalPotentialMulti): the S- and A-like rows of nfp points xfp
+// (3 per point) against the hull panels (nfp x n, column-major, no free term),
+// then, on pot = Sf W - Af X (nfp x (m + nbeta)), the free term, the incident
+// potential of the diffraction columns, pressure and elevation
+hipError_t field_rows(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* xfp, int nfp, double* Af,
+                      double* Sf);
+hipError_t field_finish(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* xfp, int nfp, int m,
+                        int nbeta, const double* beta, double xw, double yw, double* pot, double* prs, double* elv);
 
 }  // namespace hm

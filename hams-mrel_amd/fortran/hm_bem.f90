@@ -70,6 +70,15 @@ module hm_bem
          type(c_ptr), value :: ctx
          integer(c_int), intent(out) :: iters
       end function
+      ! CalPotentialMulti / CalPressureMulti / CalElevationMulti (PressureElevationMulti.f90:55,
+      ! :305, :329) for every mode and heading at nfp points after the solve
+      integer(c_int) function hm_field_points(ctx, nfp, xfp, pot, prs, elv) bind(C, name="hm_field_points")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nfp
+         real(c_double), intent(in) :: xfp(*)
+         complex(c_double_complex), intent(out) :: pot(*), prs(*), elv(*)
+      end function
       integer(c_int) function hm_forces(ctx, amas, bdmp, exfc) bind(C, name="hm_forces")
          import :: c_int, c_ptr, c_double, c_double_complex
          type(c_ptr), value :: ctx
@@ -211,7 +220,8 @@ module hm_bem
    end interface
 
    public :: hm_create, hm_destroy, hm_set_constants, hm_set_geometry, hm_set_depth, hm_set_headings, &
-             hm_set_frequency, hm_assemble, hm_factor, hm_solve, hm_forces, hm_get_system, hm_get_potentials, &
+             hm_set_frequency, hm_assemble, hm_factor, hm_solve, hm_forces, hm_field_points, hm_get_system, &
+             hm_get_potentials, &
              hm_fingreen3d_batch, hm_infgreen3d_batch, hm_zgetrf, hm_zgetrs
    public :: HM_CONTEXT, HM_SETUP, HM_SET_FREQUENCY_F, HM_CHECK
    public :: FINGREEN3D, INFGREEN3D

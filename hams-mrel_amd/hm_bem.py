@@ -28,7 +28,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
-           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
+           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
@@ -364,6 +364,18 @@ class Context:
         self._chk(self.L.hm_sglintbd_batch(self.ptr, _i(len(ncn)), _p(ncn), _p(verts), _p(xf), _p(s), _p(d)),
                   "hm_sglintbd_batch")
         return s, d
+
+    def field_points(self, xfp):
+        """potential, pressure and elevation of every radiation mode and heading at
+        the points xfp (nfp x 3) after solve (CalPotentialMulti and its pressure and
+        elevation forms); each nfp x (6 nbody + nbeta) complex"""
+        xfp = np.ascontiguousarray(xfp, dtype=np.float64).reshape(-1, 3)
+        nfp = len(xfp)
+        ncol = self.problem.m + self.problem.nbeta
+        out = {k: np.zeros((nfp, ncol), dtype=np.complex128, order="F") for k in ("pot", "prs", "elv")}
+        self._chk(self.L.hm_field_points(self.ptr, _i(nfp), _p(xfp), _p(out["pot"]), _p(out["prs"]), _p(out["elv"])),
+                  "hm_field_points")
+        return out
 
     def ik01a(self, x, fast=False):
         """K0(x), K1(x) on the GPU: the reference's IK01A evaluation, or (fast) the

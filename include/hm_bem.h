@@ -297,6 +297,15 @@ int hm_infgreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, 
  * source panel's vertices verts[12*i] (ncn[i] of them) and field point xf[3*i]. */
 int hm_sglintbd_batch(hm_ctx *ctx, int n, const int *ncn, const double *verts, const double *xf, double *sij,
                       double *dij);
+/* CalPotentialMulti / CalPressureMulti / CalElevationMulti(XP,RDFLG,MD,...)
+ * PressureElevationMulti.f90:55, :305, :329, for every radiation mode
+ * (column 6 b + d: body b, degree of freedom d) and heading at once, after
+ * hm_solve: potential, pressure (i w rho phi) and free-surface elevation
+ * (i w / g phi) at nfp field points xfp (3 per point), each an nfp x (6 nbody +
+ * nbeta) column-major complex array (any may be NULL). The free term is 2 pi on a
+ * hull panel and 4 pi elsewhere; the diffraction columns include the incident
+ * potential (ISOL = 1). */
+int hm_field_points(hm_ctx *ctx, int nfp, const double *xfp, double *pot, double *prs, double *elv);
 /* IK01A(X,BI0,DI0,BI1,DI1,BK0,DK0,BK1,DK1)  FinGrnExtSubs.f90:538, the K0 and K1
  * outputs, at n points: mode 0 evaluates as the reference does (the plain
  * path's kernels), mode 1 as the mixed-precision kernels do (the same series

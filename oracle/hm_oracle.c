@@ -1707,3 +1707,111 @@ static int solve_wave(const hmo_problem *p, wave_t w, const double *wvn, hmo_res
     free(X); free(A); free(B); free(q); free(body_of); free(ipiv);
     return info;
 }
+
+/* =============================================================================
+ * Field-point potential, pressure and elevation -- src/PressureElevationMulti.f90
+ * CalPotentialMulti :55-272, CalPressureMulti :305-324, CalElevationMulti
+ * :329-348, for every radiation mode (column 6 b + d: body b, dof d; the
+ * reference's PHIDOF = b + 1, MD = d + 1) and heading, NSYS = 1, ISOL = 1, after
+ * the frequency solve. pot, prs, elv: nfp x (6 nbody + nbeta), column-major.
+ * ========================================================================== */
+int hmo_field_points(const hmo_problem *p, double w1, int nfp, const double *xfp, double complex *pot,
+                     double complex *prs, double complex *elv)
+{
+    const int n = p->n, nb = p->nbody, m = 6 * nb, nbeta = p->nbeta, ncol = m + nbeta;
+    wave_t w;
+    double wvn[200];
+    w.w1 = w1;
+    w.v = w1 * w1 / p->g;
+    w.h = p->h;
+    w.amp = p->amp;
+    w.g = p->g;
+    if (p->h > 0.0) {
+        hmo_dispersion(wvn, 200, w1, p->h, p->g_disp);
+        w.wk = wvn[0];
+    } else {
+        w.wk = w.v;
+        for (int k = 0; k < 200; k++) wvn[k] = 0.0;
+    }
+    double *amas = calloc((size_t)m * m, sizeof(double)), *bdmp = calloc((size_t)m * m, sizeof(double));
+    double complex *exfc = calloc((size_t)m * nbeta, sizeof(double complex));
+    double complex *mx = calloc((size_t)n * ncol, sizeof(double complex));
+    hmo_result r = {amas, bdmp, exfc, NULL, NULL, NULL, mx};
+    int info = solve_wave(p, w, wvn, &r);
+    const int limit = fabs(w.v) < F32(1.e-8) || fabs(w.v + 1.0) < F32(1.e-8);
+    for (int ip = 0; ip < nfp; ip++) {
+        const double *xp = xfp + 3 * ip;
+        double complex *acc = calloc((size_t)ncol, sizeof(double complex));
+        for (int j = 0; j < n; j++) {
+            const double *xq = p->xyz_p + 3 * j, *env = p->dxyz_p + 6 * j;
+            double d0 = xp[0] - xq[0], d1 = xp[1] - xq[1], d2 = xp[2] - xq[2];
+            int flag = sqrt(d0 * d0 + d1 * d1 + d2 * d2) <= 50.0 * p->pnsz[j];
+            double rk[4] = {0, 0, 0, 0};
+            if (flag) {                 /* SGLINTBD_MULTI_*2 at the field point, IRR = 1 */
+                double verts[12] = {0};
+                for (int v = 0; v < p->ncn[j]; v++)
+                    for (int k = 0; k < 3; k++) verts[3 * v + k] = p->xyz[3 * (p->ncon[4 * j + v] - 1) + k];
+                hmo_sglintbd(p->ncn[j], verts, xp, &rk[0], rk + 1);
+            }
+            double complex grn[4];
+            if (p->h < 0.0) hmo_infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], w.v, grn, flag);
+            else hmo_fingreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], w.v, wvn, 200, p->h, grn, flag);
+            const double ear = p->ds[j];
+            double complex dum1, dum2;
+            if (flag) {
+                dum1 = rk[0] + grn[0] * ear;
+                dum2 = (rk[1] + grn[1] * ear) * env[0] + (rk[2] + grn[2] * ear) * env[1] + (rk[3] + grn[3] * ear) * env[2];
+            } else {
+                dum1 = grn[0] * ear;
+                dum2 = (grn[1] * env[0] + grn[2] * env[1] + grn[3] * env[2]) * ear;
+            }
+            int bj = 0;
+            while (j >= p->body_start[bj + 1]) bj++;
+            for (int b = 0; b < nb; b++)       /* FACTOR_NORMAL: the moving body's panels */
+                for (int d = 0; d < 6; d++) {
+                    const int c = 6 * b + d;
+                    const double complex t1 = (b == bj) ? dum1 * env[d] : 0.0;
+                    acc[c] += t1 - dum2 * mx[j + (size_t)c * n];
+                }
+            for (int h = 0; h < nbeta; h++) {
+                double complex dx, dy, dz;
+                dinp(&w, xq[0], xq[1], xq[2], p->xw[0], p->xw[1], p->beta[h], &dx, &dy, &dz);
+                const double complex dincp = dx * env[0] + dy * env[1] + dz * env[2];
+                acc[m + h] += -dum1 * dincp - dum2 * mx[j + (size_t)(m + h) * n];
+            }
+        }
+        /* free term: 2 pi on a hull panel (in its plane and inside its polygon), 4 pi elsewhere */
+        int onbody = 0;
+        for (int j = 0; j < n && !onbody; j++) {
+            const double *en = p->dxyz_p + 6 * j;
+            const double tolp = fmax(1.0e-6, 1.0e-3 * p->pnsz[j]), tola = 1.0e-3 * p->pnsz[j] * p->pnsz[j];
+            const double *va = p->xyz + 3 * (p->ncon[4 * j] - 1);
+            const double dpln = (xp[0] - va[0]) * en[0] + (xp[1] - va[1]) * en[1] + (xp[2] - va[2]) * en[2];
+            if (fabs(dpln) > tolp) continue;
+            double smin = 1.0e30, smax = -1.0e30;
+            for (int k = 0; k < p->ncn[j]; k++) {
+                const double *a = p->xyz + 3 * (p->ncon[4 * j + k] - 1);
+                const double *b = p->xyz + 3 * (p->ncon[4 * j + (k + 1 < p->ncn[j] ? k + 1 : 0)] - 1);
+                const double ev[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                const double wv[3] = {xp[0] - a[0], xp[1] - a[1], xp[2] - a[2]};
+                const double cx = ev[1] * wv[2] - ev[2] * wv[1], cy = ev[2] * wv[0] - ev[0] * wv[2],
+                             cz = ev[0] * wv[1] - ev[1] * wv[0];
+                const double sv = cx * en[0] + cy * en[1] + cz * en[2];
+                smin = fmin(smin, sv);
+                smax = fmax(smax, sv);
+            }
+            if (smin >= -tola || smax <= tola) onbody = 1;
+        }
+        const double sld = onbody ? 2.0 * PI : 4.0 * PI;
+        for (int c = 0; c < ncol; c++) {
+            double complex ph = acc[c] / sld;
+            if (c >= m && !limit) ph += vinp(&w, xp[0], xp[1], xp[2], p->xw[0], p->xw[1], p->beta[c - m]);
+            pot[ip + (size_t)c * nfp] = ph;
+            if (prs) prs[ip + (size_t)c * nfp] = limit ? p->rho * ph : I * w1 * p->rho * ph;
+            if (elv) elv[ip + (size_t)c * nfp] = limit ? ph : I * w1 / p->g * ph;
+        }
+        free(acc);
+    }
+    free(amas); free(bdmp); free(exfc); free(mx);
+    return info;
+}

@@ -91,6 +91,12 @@ typedef struct {
  * IRR normal equations when ni>0), LU solve, force reductions. Returns the
  * LAPACK-style INFO of the factorisation (0 = ok). */
 int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out);
+/* Field-point potential, pressure and elevation (CalPotentialMulti,
+ * CalPressureMulti, CalElevationMulti: PressureElevationMulti.f90:55-348) of
+ * every radiation mode and heading at nfp points xfp (3 per point), after the
+ * frequency solve; outputs nfp x (6 nbody + nbeta), column-major. */
+int hmo_field_points(const hmo_problem *p, double w1, int nfp, const double *xfp, double complex *pot,
+                     double complex *prs, double complex *elv);
 /* The same chain at zero (which = 0) or infinite (which = 1) wave frequency. */
 int hmo_solve_limit(const hmo_problem *p, int which, hmo_result *out);
 

@@ -140,6 +140,38 @@ class Oracle:
         """zero (which=0) or infinite (which=1) wave frequency (SYBO=1)"""
         return self._solve(prob, lambda P, R: self.lib.hmo_solve_limit(P, _i(which), R), want_matrices)
 
+    def field_points(self, prob, w1, xfp):
+        """potential, pressure and elevation of every radiation mode and heading
+        at the points xfp (nfp x 3) after the frequency solve (CalPotentialMulti,
+        CalPressureMulti, CalElevationMulti)"""
+        P, keep = self._problem(prob)
+        xfp = np.ascontiguousarray(xfp, dtype=np.float64).reshape(-1, 3)
+        nfp, ncol = len(xfp), 6 * prob.nbody + len(prob.beta)
+        out = {k: np.zeros((nfp, ncol), dtype=np.complex128, order="F") for k in ("pot", "prs", "elv")}
+        info = self.lib.hmo_field_points(ctypes.byref(P), _d(w1), _i(nfp), _ptr(xfp), _ptr(out["pot"]),
+                                         _ptr(out["prs"]), _ptr(out["elv"]))
+        out["info"] = info
+        return out
+
+    def _problem(self, prob):
+        n, ni, nb = prob.n, prob.ni, prob.nbody
+        nbeta = len(prob.beta)
+        keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                np.ascontiguousarray(prob.ncn, dtype=np.int32),
+                np.ascontiguousarray(prob.ncon, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
+                np.ascontiguousarray(prob.ds, dtype=np.float64),
+                np.ascontiguousarray(prob.pnsz, dtype=np.float64),
+                np.ascontiguousarray(prob.dxyz_p, dtype=np.float64),
+                np.ascontiguousarray(prob.ixyz_p if ni else np.zeros((1, 3)), dtype=np.float64),
+                np.ascontiguousarray(prob.beta, dtype=np.float64)]
+        P = _Problem(n, ni, nb, _ptr(keep[0], _ip), _ptr(keep[1]), _ptr(keep[2], _ip),
+                     _ptr(keep[3], _ip), _ptr(keep[4]), _ptr(keep[5]), _ptr(keep[6]),
+                     _ptr(keep[7]), _ptr(keep[8]), _d(prob.h), _d(prob.rho), _d(prob.g),
+                     _d(prob.g_disp), (_d * 2)(*prob.xw), _d(prob.amp), nbeta, _ptr(keep[9]))
+        return P, keep
+
     def _solve(self, prob, call, want_matrices):
         n, ni, nb = prob.n, prob.ni, prob.nbody
         m, nbeta = 6 * nb, len(prob.beta)

@@ -241,3 +241,30 @@ def test_config1_expected_outputs(oracle):
     # reciprocity of the radiation coefficients (to the discretisation error)
     assert relerr(amas, amas.T) < 1e-2
     assert relerr(bdmp, bdmp.T) < 1e-2
+
+
+def test_field_points_reproduce_the_hull_potentials(oracle):
+    """hmo_field_points (CalPotentialMulti) pinned by what the boundary-integral
+    representation fixes: at a hull panel's centroid it is the collocation
+    equation the solve satisfied, so it returns that panel's solved potential
+    for the radiation columns (the 2 pi free term; 4 pi would give half) and the
+    solved scattered potential plus the incident one for the diffraction
+    columns; pressure and elevation are i w rho phi and i w phi / g."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__)))
+    import meshes
+    bs = [meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(12.0 * i, 0.0)) for i in range(2)]
+    p = meshes.Problem(bs, h=40.0, omegas=[0.9], beta=np.array([0.0, np.pi / 3]))
+    j = 7
+    xfp = np.array([p.xyz_p[j], p.xyz_p[p.n - 3]])
+    r = oracle.field_points(p, 0.9, xfp)
+    s = oracle.solve_frequency(p, 0.9, want_matrices=True)
+    assert r["info"] == 0 and s["info"] == 0
+    mx, m = s["mxpot"], 6 * p.nbody
+    for k, jj in enumerate((j, p.n - 3)):
+        np.testing.assert_allclose(r["pot"][k, :m], mx[jj, :m], rtol=1e-10)
+        # diffraction: scattered + incident; the incident part alone is the difference
+        inc = r["pot"][k, m:] - mx[jj, m:]
+        assert np.all(np.abs(inc) > 1e-3 * np.max(np.abs(mx[:, m:])))
+    np.testing.assert_allclose(r["prs"], 1j * 0.9 * p.rho * r["pot"], rtol=1e-14)
+    np.testing.assert_allclose(r["elv"], 1j * 0.9 / p.g * r["pot"], rtol=1e-14)
