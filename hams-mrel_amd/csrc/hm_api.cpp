@@ -1853,6 +1853,95 @@ extern "C" int hm_field_points(hm_ctx* c, int nfp, const double* xfp, double* po
   return rc;
 }
 
+// Motion response (SolveMotionMulti, SolveMotionMulti.f90:46-101): the body
+// matrices (nbody x 6 x 6, Fortran order, element (b, i, j) at b + nbody (i +
+// 6 j)) on the block diagonal (LOCALTOGLOBAL :24-40); LEFT = -w^2 (MATX + AMAS)
+// + i w (BDMP + BLNR) + CRS + KSTF, RIGHT = -Im(EXFC) - i Re(EXFC); with a
+// quadratic damping (largest |BQDR| >= 1e-6) the equivalent linear damping
+// BQDR(i, j) w |x_j| is added and the solve repeated until the summed relative
+// change of x is <= 1e-6. Each solve is the FP64 LU on the device.
+extern "C" int hm_solve_motion(hm_ctx* c, int nbody, double w1, const double* amas, const double* bdmp,
+                               const double* blnr, const double* bqdr, const double* matx, const double* crs,
+                               const double* kstf, const double* exfc, double* dspl, int* iters) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nbody < 1) return c->fail(HM_ERR_ARG - 2, "nbody must be >= 1");
+  if (!(w1 >= 0.0) || !isfinite(w1)) return c->fail(HM_ERR_ARG - 3, "w1 must be finite and >= 0");
+  if (!amas || !bdmp || !blnr || !bqdr || !matx || !crs || !kstf || !exfc || !dspl)
+    return c->fail(HM_ERR_ARG - 4, "NULL array");
+  const int m = 6 * nbody;
+  const size_t mm = (size_t)m * m;
+  auto global = [&](const double* loc, std::vector<double>& g) {   // LOCALTOGLOBAL
+    g.assign(mm, 0.0);
+    for (int b = 0; b < nbody; ++b)
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j)
+          g[(size_t)(6 * b + i) + (size_t)(6 * b + j) * m] = loc[b + (size_t)nbody * (i + 6 * j)];
+  };
+  std::vector<double> bl, bq, mx, cr, ks;
+  global(blnr, bl);
+  global(bqdr, bq);
+  global(matx, mx);
+  global(crs, cr);
+  global(kstf, ks);
+  double norm = 0.0;
+  for (double v : bq) norm = fmax(norm, fabs(v));
+  std::vector<double> left(2 * mm), right(2 * (size_t)m), x(2 * (size_t)m, 0.0);
+  DevBuf da, db, dp, di, dc;
+  HIPCHK(c, da.ensure(2 * sizeof(double) * mm), "motion alloc");
+  HIPCHK(c, db.ensure(2 * sizeof(double) * m), "motion alloc");
+  HIPCHK(c, dp.ensure(sizeof(int) * m), "motion alloc");
+  HIPCHK(c, di.ensure(sizeof(int)), "motion alloc");
+  HIPCHK(c, dc.ensure(sizeof(unsigned)), "motion alloc");
+  hipStream_t s = c->stream;
+  // one solve with the equivalent damping of the current x (none on the first)
+  auto solve = [&](bool quad) -> int {
+    for (size_t e = 0; e < mm; ++e) {
+      const int j = (int)(e / m);
+      const double vd = quad ? bq[e] * w1 * hypot(x[2 * j], x[2 * j + 1]) : 0.0;
+      left[2 * e] = -w1 * w1 * (mx[e] + amas[e]) + cr[e] + ks[e];
+      left[2 * e + 1] = w1 * (bdmp[e] + bl[e] + vd);
+    }
+    for (int i = 0; i < m; ++i) {
+      right[2 * i] = -exfc[2 * i + 1];
+      right[2 * i + 1] = -exfc[2 * i];
+    }
+    int info = 0;
+    int rc = c->hip(hipMemcpyAsync(da.p, left.data(), 2 * sizeof(double) * mm, hipMemcpyHostToDevice, s), "motion");
+    if (!rc) rc = c->hip(hipMemcpyAsync(db.p, right.data(), 2 * sizeof(double) * m, hipMemcpyHostToDevice, s), "motion");
+    if (!rc) rc = c->hip(hipMemsetAsync(di.p, 0, sizeof(int), s), "motion");
+    if (!rc) rc = c->hip(hipMemsetAsync(dc.p, 0, sizeof(unsigned), s), "motion");
+    if (!rc) rc = c->hip(hm::lu_factor<double>(s, da.as<double>(), m, 0, m, dp.as<int>(), di.as<int>(),
+                                               dc.as<unsigned>(), 1), "motion");
+    if (!rc) rc = c->hip(hm::lu_solve<double>(s, da.as<double>(), m, 0, m, dp.as<int>(), db.as<double>(), m, 0, 1, 1),
+                         "motion");
+    if (!rc) rc = c->hip(hipMemcpyAsync(right.data(), db.p, 2 * sizeof(double) * m, hipMemcpyDeviceToHost, s), "motion");
+    if (!rc) rc = c->hip(hipMemcpyAsync(&info, di.p, sizeof(int), hipMemcpyDeviceToHost, s), "motion");
+    if (!rc) rc = c->hip(hipStreamSynchronize(s), "motion");
+    if (rc) return rc;
+    if (info > 0) return c->fail(info, "zero pivot in column %d of the motion equations", info);
+    return HM_OK;
+  };
+  int rc = solve(false);
+  if (rc) return rc;
+  x = right;
+  int it = 0;
+  if (norm >= 1.0e-6) {
+    for (double rerr = 100.0; rerr > 1.0e-6;) {
+      if (++it > 500) return c->fail(HM_ERR_STATE, "quadratic damping iteration: no convergence in 500 steps");
+      if ((rc = solve(true))) return rc;
+      rerr = 0.0;
+      for (int j = 0; j < m; ++j) {
+        const double a = hypot(right[2 * j], right[2 * j + 1]);
+        if (a > 0.0) rerr += hypot(right[2 * j] - x[2 * j], right[2 * j + 1] - x[2 * j + 1]) / a;
+      }
+      x = right;
+    }
+  }
+  for (int j = 0; j < 2 * m; ++j) dspl[j] = x[j];
+  if (iters) *iters = it;
+  return HM_OK;
+}
+
 extern "C" int hm_ik01a_batch(hm_ctx* c, int n, const double* x, double* bk0, double* bk1, int mode) {
   if (!c) return HM_ERR_ARG - 1;
   if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");

@@ -79,6 +79,19 @@ module hm_bem
          real(c_double), intent(in) :: xfp(*)
          complex(c_double_complex), intent(out) :: pot(*), prs(*), elv(*)
       end function
+      ! SolveMotionMulti (SolveMotionMulti.f90:46) for one heading; the body matrices
+      ! MATX_MULTI, CRS_MULTI, KSTF_MULTI are passed instead of read from the module
+      integer(c_int) function hm_solve_motion(ctx, nbody, w1, amas, bdmp, blnr, bqdr, matx, crs, kstf, exfc, &
+                                              dspl, iters) bind(C, name="hm_solve_motion")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nbody
+         real(c_double), value :: w1
+         real(c_double), intent(in) :: amas(*), bdmp(*), blnr(*), bqdr(*), matx(*), crs(*), kstf(*)
+         complex(c_double_complex), intent(in) :: exfc(*)
+         complex(c_double_complex), intent(out) :: dspl(*)
+         integer(c_int), intent(out) :: iters
+      end function
       integer(c_int) function hm_forces(ctx, amas, bdmp, exfc) bind(C, name="hm_forces")
          import :: c_int, c_ptr, c_double, c_double_complex
          type(c_ptr), value :: ctx
@@ -220,7 +233,7 @@ module hm_bem
    end interface
 
    public :: hm_create, hm_destroy, hm_set_constants, hm_set_geometry, hm_set_depth, hm_set_headings, &
-             hm_set_frequency, hm_assemble, hm_factor, hm_solve, hm_forces, hm_field_points, hm_get_system, &
+             hm_set_frequency, hm_assemble, hm_factor, hm_solve, hm_forces, hm_field_points, hm_solve_motion, hm_get_system, &
              hm_get_potentials, &
              hm_fingreen3d_batch, hm_infgreen3d_batch, hm_zgetrf, hm_zgetrs
    public :: HM_CONTEXT, HM_SETUP, HM_SET_FREQUENCY_F, HM_CHECK

@@ -28,7 +28,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
-           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
+           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_solve_motion", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
@@ -376,6 +376,20 @@ class Context:
         self._chk(self.L.hm_field_points(self.ptr, _i(nfp), _p(xfp), _p(out["pot"]), _p(out["prs"]), _p(out["elv"])),
                   "hm_field_points")
         return out
+
+    def solve_motion(self, w1, amas, bdmp, exfc, matx, crs, kstf=None, blnr=None, bqdr=None):
+        """motion response of one heading (SolveMotionMulti): amas, bdmp (6nb x 6nb),
+        exfc (6nb complex), body matrices matx, crs, kstf, blnr, bqdr (nbody x 6 x 6)"""
+        nb = np.asarray(matx).shape[0]
+        f = lambda a: np.asfortranarray(np.zeros((nb, 6, 6)) if a is None else a, dtype=np.float64)  # noqa: E731
+        args = [np.asfortranarray(amas, dtype=np.float64), np.asfortranarray(bdmp, dtype=np.float64),
+                f(blnr), f(bqdr), f(matx), f(crs), f(kstf)]
+        ex = np.ascontiguousarray(exfc, dtype=np.complex128)
+        x = np.zeros(6 * nb, dtype=np.complex128)
+        it = ctypes.c_int(0)
+        self._chk(self.L.hm_solve_motion(self.ptr, _i(nb), _d(w1), *[_p(a) for a in args], _p(ex), _p(x),
+                                         ctypes.byref(it)), "hm_solve_motion")
+        return x, it.value
 
     def ik01a(self, x, fast=False):
         """K0(x), K1(x) on the GPU: the reference's IK01A evaluation, or (fast) the

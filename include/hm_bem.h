@@ -306,6 +306,16 @@ int hm_sglintbd_batch(hm_ctx *ctx, int n, const int *ncn, const double *verts, c
  * hull panel and 4 pi elsewhere; the diffraction columns include the incident
  * potential (ISOL = 1). */
 int hm_field_points(hm_ctx *ctx, int nfp, const double *xfp, double *pot, double *prs, double *elv);
+/* SolveMotionMulti(W1,TP,OUFR,BETA,AMP,AMAS,BDMP,BLNR,BQDR,EXFC,DSPL,NBODY)
+ * SolveMotionMulti.f90:46, with the body matrices the reference keeps in its
+ * module (MATX_MULTI, CRS_MULTI, KSTF_MULTI) passed explicitly: amas, bdmp
+ * (6nbody x 6nbody), blnr, bqdr, matx, crs, kstf (nbody x 6 x 6, Fortran order),
+ * exfc (6nbody complex, one heading) -> dspl (6nbody complex). With quadratic
+ * damping the fixed-point iterations taken are returned in *iters (0 without);
+ * the solves are the FP64 LU on the device; a zero pivot returns its column. */
+int hm_solve_motion(hm_ctx *ctx, int nbody, double w1, const double *amas, const double *bdmp, const double *blnr,
+                    const double *bqdr, const double *matx, const double *crs, const double *kstf, const double *exfc,
+                    double *dspl, int *iters);
 /* IK01A(X,BI0,DI0,BI1,DI1,BK0,DK0,BK1,DK1)  FinGrnExtSubs.f90:538, the K0 and K1
  * outputs, at n points: mode 0 evaluates as the reference does (the plain
  * path's kernels), mode 1 as the mixed-precision kernels do (the same series

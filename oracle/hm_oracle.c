@@ -1815,3 +1815,60 @@ int hmo_field_points(const hmo_problem *p, double w1, int nfp, const double *xfp
     free(amas); free(bdmp); free(exfc); free(mx);
     return info;
 }
+
+/* =============================================================================
+ * Motion response -- src/SolveMotionMulti.f90 (LOCALTOGLOBAL :24-40,
+ * SolveMotionMulti :46-101): body matrices nbody x 6 x 6 in Fortran order on the
+ * block diagonal, the linear solve, then with quadratic damping (max |BQDR| >=
+ * 1e-6) the equivalent damping BQDR(i, j) w |x_j| until the summed relative
+ * change <= 1e-6 (at most 500 steps here; -1 if not reached). Returns the
+ * LAPACK INFO of the last factorisation, or -1.
+ * ========================================================================== */
+int hmo_solve_motion(int nbody, double w1, const double *amas, const double *bdmp, const double *blnr,
+                     const double *bqdr, const double *matx, const double *crs, const double *kstf,
+                     const double complex *exfc, double complex *dspl, int *iters)
+{
+    const int m = 6 * nbody;
+    const size_t mm = (size_t)m * m;
+    double *g[5];
+    const double *loc[5] = {blnr, bqdr, matx, crs, kstf};
+    for (int k = 0; k < 5; k++) {
+        g[k] = calloc(mm, sizeof(double));
+        for (int b = 0; b < nbody; b++)
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++)
+                    g[k][(6 * b + i) + (size_t)(6 * b + j) * m] = loc[k][b + (size_t)nbody * (i + 6 * j)];
+    }
+    double norm = 0.0;
+    for (size_t e = 0; e < mm; e++) norm = fmax(norm, fabs(g[1][e]));
+    double complex *left = malloc(sizeof(double complex) * mm), *x = calloc(m, sizeof(double complex)),
+                   *r = malloc(sizeof(double complex) * m);
+    int *ipiv = malloc(sizeof(int) * m), info = 0, it = 0;
+    for (int pass = 0;; pass++) {
+        for (size_t e = 0; e < mm; e++) {
+            const int j = (int)(e / m);
+            const double vd = pass ? g[1][e] * w1 * cabs(x[j]) : 0.0;
+            left[e] = -w1 * w1 * (g[2][e] + amas[e]) + I * w1 * (bdmp[e] + g[0][e] + vd) + g[3][e] + g[4][e];
+        }
+        for (int i = 0; i < m; i++) r[i] = -cimag(exfc[i]) - I * creal(exfc[i]);
+        info = hmo_zgetrf(m, left, m, ipiv);
+        hmo_zgetrs(m, 1, left, m, ipiv, r, m);
+        if (pass == 0) {
+            memcpy(x, r, sizeof(double complex) * m);
+            if (norm < 1.0e-6) break;
+            continue;
+        }
+        it++;
+        double rerr = 0.0;
+        for (int j = 0; j < m; j++)
+            if (cabs(r[j]) > 0.0) rerr += cabs(r[j] - x[j]) / cabs(r[j]);
+        memcpy(x, r, sizeof(double complex) * m);
+        if (rerr <= 1.0e-6) break;
+        if (it >= 500) { info = -1; break; }
+    }
+    memcpy(dspl, x, sizeof(double complex) * m);
+    if (iters) *iters = it;
+    for (int k = 0; k < 5; k++) free(g[k]);
+    free(left); free(x); free(r); free(ipiv);
+    return info;
+}

@@ -97,6 +97,12 @@ int hmo_solve_frequency(const hmo_problem *p, double w1, hmo_result *out);
  * frequency solve; outputs nfp x (6 nbody + nbeta), column-major. */
 int hmo_field_points(const hmo_problem *p, double w1, int nfp, const double *xfp, double complex *pot,
                      double complex *prs, double complex *elv);
+/* Motion response of one heading (SolveMotionMulti, SolveMotionMulti.f90:46-101):
+ * amas, bdmp (6nbody x 6nbody), body matrices nbody x 6 x 6 (Fortran order),
+ * exfc -> dspl (6nbody); iters = quadratic-damping iterations. */
+int hmo_solve_motion(int nbody, double w1, const double *amas, const double *bdmp, const double *blnr,
+                     const double *bqdr, const double *matx, const double *crs, const double *kstf,
+                     const double complex *exfc, double complex *dspl, int *iters);
 /* The same chain at zero (which = 0) or infinite (which = 1) wave frequency. */
 int hmo_solve_limit(const hmo_problem *p, int which, hmo_result *out);
 

@@ -153,6 +153,19 @@ class Oracle:
         out["info"] = info
         return out
 
+    def solve_motion(self, w1, amas, bdmp, exfc, matx, crs, kstf=None, blnr=None, bqdr=None):
+        """motion response of one heading (SolveMotionMulti); body matrices nbody x 6 x 6"""
+        nb = np.asarray(matx).shape[0]
+        f = lambda a: np.asfortranarray(np.zeros((nb, 6, 6)) if a is None else a, dtype=np.float64)  # noqa: E731
+        args = [np.asfortranarray(amas, dtype=np.float64), np.asfortranarray(bdmp, dtype=np.float64),
+                f(blnr), f(bqdr), f(matx), f(crs), f(kstf)]
+        ex = np.ascontiguousarray(exfc, dtype=np.complex128)
+        x = np.zeros(6 * nb, dtype=np.complex128)
+        it = ctypes.c_int(0)
+        info = self.lib.hmo_solve_motion(_i(nb), _d(w1), *[_ptr(a) for a in args], _ptr(ex), _ptr(x),
+                                         ctypes.byref(it))
+        return x, it.value, info
+
     def _problem(self, prob):
         n, ni, nb = prob.n, prob.ni, prob.nbody
         nbeta = len(prob.beta)

@@ -268,3 +268,74 @@ def test_field_points_reproduce_the_hull_potentials(oracle):
         assert np.all(np.abs(inc) > 1e-3 * np.max(np.abs(mx[:, m:])))
     np.testing.assert_allclose(r["prs"], 1j * 0.9 * p.rho * r["pot"], rtol=1e-14)
     np.testing.assert_allclose(r["elv"], 1j * 0.9 / p.g * r["pot"], rtol=1e-14)
+
+
+def _motion_system(nb, rng):
+    """a plausible multi-body system: symmetric added mass and damping with
+    cross-body coupling, body masses, hydrostatics and a weak mooring"""
+    m = 6 * nb
+    q = rng.normal(size=(m, m))
+    amas = 2e4 * (q @ q.T / m + np.eye(m))
+    q = rng.normal(size=(m, m))
+    bdmp = 3e3 * (q @ q.T / m + 0.5 * np.eye(m))
+    matx = np.zeros((nb, 6, 6))
+    crs = np.zeros((nb, 6, 6))
+    kstf = np.zeros((nb, 6, 6))
+    for b in range(nb):
+        matx[b] = np.diag([1.16e5, 1.16e5, 1.16e5, 4e5, 4e5, 5e5])
+        crs[b] = np.diag([0.0, 0.0, 2.84e5, 6e5, 6e5, 0.0])
+        kstf[b] = np.diag([2e3, 2e3, 0.0, 0.0, 0.0, 3e3])
+    exfc = 1e4 * (rng.normal(size=m) + 1j * rng.normal(size=m))
+    return amas, bdmp, matx, crs, kstf, exfc
+
+
+def _global(loc):
+    nb = loc.shape[0]
+    g = np.zeros((6 * nb, 6 * nb))
+    for b in range(nb):
+        g[6 * b:6 * b + 6, 6 * b:6 * b + 6] = loc[b]
+    return g
+
+
+def test_solve_motion_closed_forms(oracle):
+    """hmo_solve_motion (SolveMotionMulti) pinned by: a single degree of freedom
+    at resonance, |x| = |F| / (w B); the coupled multi-body system against
+    numpy's solve of the block-diagonal assembly (the body matrices' Fortran
+    layout, the sign convention RIGHT = -i conj(F)); and, with quadratic
+    damping, the converged x satisfying its own fixed-point equation to the
+    stopping tolerance"""
+    rng = np.random.default_rng(21)
+    # one body, heave only: resonance of K - w^2 (M + A)
+    nb, w = 1, 0.8
+    amas, bdmp = np.zeros((6, 6)), np.zeros((6, 6))
+    matx, crs = np.zeros((1, 6, 6)), np.zeros((1, 6, 6))
+    for k in range(6):
+        matx[0, k, k] = 1.0
+        crs[0, k, k] = 1.0 if k != 2 else 0.0
+        bdmp[k, k] = 0.1
+    amas[2, 2] = 5.0e4
+    matx[0, 2, 2] = 1.0e5
+    crs[0, 2, 2] = w * w * (1.5e5)
+    bdmp[2, 2] = 2.0e4
+    exfc = np.zeros(6, dtype=np.complex128)
+    exfc[2] = 3.0e4 + 4.0e4j
+    x, it, info = oracle.solve_motion(w, amas, bdmp, exfc, matx, crs)
+    assert info == 0 and it == 0
+    assert abs(abs(x[2]) - abs(exfc[2]) / (w * 2.0e4)) < 1e-12 * abs(x[2])
+    # coupled bodies against numpy
+    nb, w = 3, 0.7
+    amas, bdmp, matx, crs, kstf, exfc = _motion_system(nb, rng)
+    x, it, info = oracle.solve_motion(w, amas, bdmp, exfc, matx, crs, kstf)
+    lhs = -w * w * (_global(matx) + amas) + 1j * w * bdmp + _global(crs) + _global(kstf)
+    ref = np.linalg.solve(lhs, -1j * np.conj(exfc))
+    assert info == 0 and it == 0
+    assert np.max(np.abs(x - ref)) < 1e-12 * np.max(np.abs(ref))
+    # quadratic damping: the fixed point
+    bqdr = np.zeros((nb, 6, 6))
+    for b in range(nb):
+        bqdr[b] = np.diag([4e3, 4e3, 8e3, 1e4, 1e4, 2e3])
+    x, it, info = oracle.solve_motion(w, amas, bdmp, exfc, matx, crs, kstf, bqdr=bqdr)
+    assert info == 0 and it >= 2
+    vd = _global(bqdr) * w * np.abs(x)[None, :]
+    res = (lhs + 1j * w * vd) @ x + 1j * np.conj(exfc)
+    assert np.linalg.norm(res) < 1e-5 * np.linalg.norm(exfc)
