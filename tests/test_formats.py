@@ -1,0 +1,197 @@
+"""On-disk formats (hams-mrel_amd/formats.py): the control file of the
+reference's own test case (test/data/config1), the frequency conventions
+against their closed forms, and the HAMS writer against that case's
+expected_output files text for text (the files are read back and rewritten:
+the layout, the order of records and the Fortran edit descriptors must all
+match for the text to come back identical)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import formats
+import meshes
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+INP = os.path.join(GOLDEN, "config1_input")
+EXP = os.path.join(GOLDEN, "config1_expected")
+
+
+def test_control_file_of_config1():
+    cf = formats.read_control_file(INP)
+    assert cf.h == 70.0 and cf.sybo == 0 and cf.inft == 3 and cf.ouft == 3
+    np.testing.assert_allclose(cf.values, [0.05, 0.06, 0.07], rtol=0, atol=1e-15)
+    assert cf.nbody == 3
+    np.testing.assert_array_equal(cf.lcs, [[0, 0, 0, 0], [0, 54, 0, 0], [0, 108, 0, 0]])
+    np.testing.assert_array_equal(cf.headings_deg, [0.0, 10.0, 20.0])
+    np.testing.assert_array_equal(cf.xr, [[0, 0, 0], [0, 54, 0], [0, 108, 0]])
+    assert (cf.refl, cf.isol, cf.irsp, cf.nthread, cf.isolv) == (1.0, 1, 0, 1, 1)
+    assert cf.xfp.shape == (0, 3) and not cf.separate_wamit
+    p, _ = formats.problem_from_control(INP)
+    q = meshes.reference_config1_from(INP)
+    np.testing.assert_array_equal(p.xyz_p, q.xyz_p)
+    np.testing.assert_array_equal(p.ds, q.ds)
+    np.testing.assert_allclose(p.omegas, q.omegas, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(p.beta, q.beta, rtol=0, atol=1e-15)
+    assert (p.h, p.rho, p.g, p.g_disp) == (q.h, q.rho, q.g, q.g_disp)
+
+
+SINGLE = """   --------------HAMS Control file---------------
+
+   Waterdepth  -1.0D0
+
+   #Start Definition of Wave Frequencies
+    0_inf_frequency_limits      1
+    Input_frequency_type        4
+    Output_frequency_type       4
+    Number_of_frequencies       4
+    12.0 10.0
+    8.0  6.0
+   #End Definition of Wave Frequencies
+
+   #Start Definition of Wave Headings
+    Number_of_headings          2
+    0.0 45.0
+   #End Definition of Wave Headings
+
+    Reference_body_center     0.000     0.000    -1.500
+    Reference_body_length   1.0D0
+    Wave_diffrac_solution    2
+    If_remove_irr_freq       1
+    Number of threads       8
+    Solver_type              2
+
+   #Start Definition of Pressure and/or Elevation (PE)
+    Number_of_field_points     2
+    10.0 0.0 -2.0
+    0.0  20.0  0.0
+   #End Definition of Pressure and/or Elevation
+
+   ----------End HAMS Control file---------------
+"""
+
+
+def test_control_file_single_body_lists(tmp_path):
+    """the other branches: explicit frequency and heading lists spread over
+    records, one body (no LCS records), the optional Solver_type record and
+    field points"""
+    (tmp_path / "ControlFile.in").write_text(SINGLE)
+    cf = formats.read_control_file(str(tmp_path))
+    assert cf.h == -1.0 and cf.sybo == 1 and cf.inft == 4
+    np.testing.assert_array_equal(cf.values, [12.0, 10.0, 8.0, 6.0])
+    np.testing.assert_array_equal(cf.headings_deg, [0.0, 45.0])
+    assert cf.nbody == 1
+    np.testing.assert_array_equal(cf.xr, [[0.0, 0.0, -1.5]])
+    assert (cf.isol, cf.irsp, cf.nthread, cf.isolv) == (2, 1, 8, 2)
+    np.testing.assert_array_equal(cf.xfp, [[10.0, 0.0, -2.0], [0.0, 20.0, 0.0]])
+    with pytest.raises(FileNotFoundError):
+        formats.read_control_file(str(tmp_path / "absent"))
+
+
+def test_wave_properties_closed_forms():
+    g = formats.G_MODULE
+    # deep water: k = w^2 / g whatever the input type
+    for inft, val in ((3, 0.8), (4, 2 * math.pi / 0.8), (1, 0.8 ** 2 / g), (2, 0.8 ** 2 / g),
+                      (5, 2 * math.pi * g / 0.8 ** 2)):
+        r = formats.wave_properties(val, inft, 3, -1.0)
+        assert abs(r["w1"] - 0.8) < 1e-12 and abs(r["wk"] - 0.8 ** 2 / g) < 1e-12
+        assert abs(r["tp"] - 2 * math.pi / 0.8) < 1e-12
+    # finite depth: w^2 = g k tanh(k h) (the dispersion solver's own g differs by 3e-5)
+    r = formats.wave_properties(0.6, 3, 2, 30.0)
+    k = r["wk"]
+    assert abs(formats.G_DISPERSION * k * math.tanh(k * 30.0) - 0.36) < 1e-10
+    assert r["oufr"] == k
+    r2 = formats.wave_properties(k, 2, 3, 30.0)
+    assert abs(r2["w1"] - math.sqrt(g * k * math.tanh(k * 30.0))) < 1e-14
+    r5 = formats.wave_properties(2 * math.pi / k, 5, 5, 30.0)
+    assert abs(r5["wk"] - k) < 1e-14 and abs(r5["oufr"] - 2 * math.pi / k) < 1e-12
+    with pytest.raises(ValueError):
+        formats.wave_properties(1.0, 6, 3, 30.0)
+
+
+def test_edit_descriptors():
+    assert formats.fortran_e(132840.0, 14, 5) == "   0.13284E+06"
+    assert formats.fortran_e(-73.595, 14, 5) == "  -0.73595E+02"
+    assert formats.fortran_e(0.0, 14, 5) == "   0.00000E+00"
+    assert formats.fortran_e(-6753.14, 14, 6) == " -0.675314E+04"
+    assert formats.fortran_e(9.999996e-3, 14, 5) == "   0.10000E-01"
+    assert formats.fortran_es(132840.0, 14, 6) == "  1.328400E+05"
+    assert formats.fortran_f(0.05, 7, 3) == "  0.050"
+
+
+def _read_expected(nb=3, nbeta=3, nf=3):
+    m = 6 * nb
+    amas, bdmp = np.zeros((nf, m, m)), np.zeros((nf, m, m))
+    exfc = np.zeros((nf, m, nbeta), dtype=np.complex128)
+    wk = w1 = None
+    for md in range(6):
+        a = np.loadtxt(os.path.join(EXP, "OAMASS%d.txt" % (md + 1)))
+        b = np.loadtxt(os.path.join(EXP, "ODAMPING%d.txt" % (md + 1)))
+        e = np.loadtxt(os.path.join(EXP, "OEXFOR%d.txt" % (md + 1)))
+        wk, w1 = a[::nb * nb, 0], a[::nb * nb, 1]
+        for f in range(nf):
+            q = 0
+            for b1 in range(nb):
+                for b2 in range(nb):
+                    amas[f, 6 * b1 + md, 6 * b2:6 * b2 + 6] = a[f * nb * nb + q, 2:]
+                    bdmp[f, 6 * b1 + md, 6 * b2:6 * b2 + 6] = b[f * nb * nb + q, 2:]
+                    q += 1
+            q = 0
+            for ih in range(nbeta):
+                for b1 in range(nb):
+                    exfc[f, 6 * b1 + md, ih] = e[f * nbeta * nb + q, 2] + 1j * e[f * nbeta * nb + q, 3]
+                    q += 1
+    return wk, w1, amas, bdmp, exfc
+
+
+def test_hams_writer_reproduces_config1_expected_text(tmp_path):
+    wk, w1, amas, bdmp, exfc = _read_expected()
+    # the wave numbers the writer is given come from the dispersion relation, as in the reference
+    cf = formats.read_control_file(INP)
+    props = [formats.wave_properties(v, cf.inft, cf.ouft, cf.h) for v in cf.values]
+    wk_solved = np.array([r["wk"] for r in props])
+    np.testing.assert_allclose(wk_solved, wk, atol=5e-4)
+    formats.write_hams(str(tmp_path), wk_solved, [r["w1"] for r in props], amas, bdmp, exfc)
+    for stem in ("OAMASS", "ODAMPING", "OEXFOR"):
+        for md in range(6):
+            name = "%s%d.txt" % (stem, md + 1)
+            want = open(os.path.join(EXP, name)).read().rstrip("\n").split("\n")
+            got = open(os.path.join(str(tmp_path), name)).read().rstrip("\n").split("\n")
+            assert got == [s.rstrip() for s in want], name
+
+
+def test_wamit_writers(tmp_path):
+    rng = np.random.default_rng(3)
+    nf, nb, nbeta = 2, 2, 2
+    m = 6 * nb
+    amas = rng.normal(size=(nf, m, m)) * 1e5
+    bdmp = rng.normal(size=(nf, m, m)) * 1e4
+    exfc = (rng.normal(size=(nf, m, nbeta)) + 1j * rng.normal(size=(nf, m, nbeta))) * 1e4
+    w1 = np.array([0.5, 1.1])
+    rho, g = 1000.0, 9.80665
+    p1 = str(tmp_path / "x.1")
+    formats.write_wamit_1(p1, w1, w1, amas, bdmp, rho, limit=[True, False], row_major=True)
+    rows = [ln.split() for ln in open(p1).read().strip().split("\n")]
+    assert len(rows) == 2 * m * m
+    assert all(len(r) == 4 for r in rows[:m * m]) and all(len(r) == 5 for r in rows[m * m:])
+    for q, r in enumerate(rows[m * m:]):
+        i, j = q // m, q % m
+        assert (int(r[1]), int(r[2])) == (i + 1, j + 1)
+        assert abs(float(r[3]) - amas[1, i, j] / rho) <= 1e-6 * abs(amas[1, i, j] / rho)
+        assert abs(float(r[4]) - bdmp[1, i, j] / (rho * 1.1)) <= 1e-6 * abs(bdmp[1, i, j] / (rho * 1.1))
+    formats.write_wamit_1(p1, w1, w1, amas, bdmp, rho, row_major=False)
+    rows = [ln.split() for ln in open(p1).read().strip().split("\n")]
+    assert [(int(r[1]), int(r[2])) for r in rows[:3]] == [(1, 1), (1, 2), (1, 3)]
+    assert (int(rows[36][1]), int(rows[36][2])) == (1, 7)          # second body pair (1, 2)
+    p3 = str(tmp_path / "x.3")
+    formats.write_wamit_3(p3, w1, [0.0, 30.0], exfc, rho, g)
+    rows = [[float(t) for t in ln.split()] for ln in open(p3).read().strip().split("\n")]
+    assert len(rows) == nf * nbeta * m
+    for q, r in enumerate(rows):
+        f, ih, k = q // (nbeta * m), (q // m) % nbeta, q % m
+        z = exfc[f, k, ih] / (rho * g)
+        assert r[1] == [0.0, 30.0][ih] and int(r[2]) == k + 1
+        assert abs(r[3] - abs(z)) <= 1e-6 * abs(z)
+        assert abs(r[5] + z.imag) <= 1e-6 * abs(z) and abs(r[6] + z.real) <= 1e-6 * abs(z)
+        assert abs(r[4] - math.degrees(math.atan2(-z.real, -z.imag)) * 180 / math.pi) <= 1e-5 * 180 * 180 / math.pi
