@@ -1,0 +1,53 @@
+"""The reference's own integration case run as its driver runs it: the input
+directory test/data/config1 (ControlFile.in + three .pnl hulls) read by
+formats.problem_from_control, solved on the GPU (plain FP64 path), and written
+by formats.write_hams as OAMASS/ODAMPING/OEXFOR files, which are compared with
+the case's expected_output files record by record: identical frequency
+columns, and values within the bar of the coefficient test on the same case
+(5e-5 of the frequency's largest added-mass or damping value, 2e-5 of its
+largest excitation)."""
+import os
+
+import numpy as np
+import pytest
+
+import formats
+import hm_bem
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_config1_input_directory_to_hams_files(tmp_path):
+    inp = os.path.join(GOLDEN, "config1_input")
+    p, cf = formats.problem_from_control(inp)
+    props = [formats.wave_properties(v, cf.inft, cf.ouft, cf.h) for v in cf.values]
+    c = hm_bem.Context()
+    try:
+        c.setup(p)
+        amas, bdmp, exfc = c.run_frequencies(p.omegas, hm_bem.HM_FP64)
+    finally:
+        c.close()
+    formats.write_hams(str(tmp_path), [r["wk"] for r in props], [r["w1"] for r in props], amas, bdmp, exfc)
+    same = total = 0
+    for stem in ("OAMASS", "ODAMPING", "OEXFOR"):
+        want_all, got_all = [], []
+        for md in range(6):
+            name = "%s%d.txt" % (stem, md + 1)
+            want = [ln.split() for ln in open(os.path.join(GOLDEN, "config1_expected", name)).read().split("\n") if ln.strip()]
+            got = [ln.split() for ln in open(os.path.join(str(tmp_path), name)).read().split("\n") if ln.strip()]
+            assert len(got) == len(want) == 27, name
+            assert [r[:2] for r in got] == [r[:2] for r in want], name
+            want_all.append(np.array([[float(t) for t in r[2:]] for r in want]))
+            got_all.append(np.array([[float(t) for t in r[2:]] for r in got]))
+            same += sum(a == b for r, q in zip(got, want) for a, b in zip(r[2:], q[2:]))
+            total += want_all[-1].size
+        wv, gv = np.stack(want_all), np.stack(got_all)          # (mode, record, value)
+        for f in range(3):                                      # 9 records per frequency
+            blk = slice(9 * f, 9 * f + 9)
+            scale = np.max(np.abs(wv[:, blk]))                  # the frequency's whole matrix / force set
+            bar = (5e-5 + 1e-6 / scale) if stem != "OEXFOR" else 2e-5
+            err = np.max(np.abs(gv[:, blk] - wv[:, blk])) / scale
+            assert err <= bar, (stem, f, err)
+    print("printed values identical to the expected files: %d of %d" % (same, total), flush=True)
