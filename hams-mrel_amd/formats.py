@@ -208,7 +208,38 @@ def problem_from_control(inp, hull="HullMesh_{k}.pnl", single_hull="HullMesh.pnl
     w = [wave_properties(v, cf.inft, cf.ouft, cf.h)["w1"] for v in cf.values]
     p = meshes.Problem(bodies, h=cf.h, omegas=np.array(w), beta=np.deg2rad(cf.headings_deg),
                        name=os.path.basename(os.path.normpath(inp)))
+    apply_generalized_modes(p, inp)
     return meshes.reference_parity_constants(p), cf
+
+
+def apply_generalized_modes(p, inp, log=print):
+    """The generalized-mode override of CalNormalsMulti (ReadPanelMeshMulti.f90:
+    166-203): gen_mod_{body}_{dof}.txt (body and dof 1-based), one record
+    "panel_id normal_value" per hull panel of the body, replaces that dof's
+    normal component of those panels; a record that does not read ends the
+    file with a warning, as in the reference. Returns the (body, dof) pairs
+    overridden."""
+    done = []
+    for b in range(p.nbody):
+        n0, nb = int(p.body_start[b]), int(p.body_start[b + 1] - p.body_start[b])
+        for dof in range(1, 7):
+            path = os.path.join(inp, "gen_mod_%d_%d.txt" % (b + 1, dof))
+            if not os.path.exists(path):
+                continue
+            with open(path) as fh:
+                recs = fh.read().split("\n")
+            for k in range(nb):
+                try:
+                    tok = recs[k].replace(",", " ").split()
+                    pid, val = int(tok[0]), _num(tok[1])
+                    if not 1 <= pid <= nb:
+                        raise ValueError
+                except (IndexError, ValueError):
+                    log("  Warning: Could not read line %d from %s" % (k + 1, path))
+                    break
+                p.dxyz_p[n0 + pid - 1, dof - 1] = val
+            done.append((b + 1, dof))
+    return done
 
 
 # ---- Fortran edit descriptors ------------------------------------------------------------------

@@ -195,3 +195,23 @@ def test_wamit_writers(tmp_path):
         assert abs(r[3] - abs(z)) <= 1e-6 * abs(z)
         assert abs(r[5] + z.imag) <= 1e-6 * abs(z) and abs(r[6] + z.real) <= 1e-6 * abs(z)
         assert abs(r[4] - math.degrees(math.atan2(-z.real, -z.imag)) * 180 / math.pi) <= 1e-5 * 180 * 180 / math.pi
+
+
+def test_generalized_mode_files(tmp_path):
+    """gen_mod_{body}_{dof}.txt replaces that dof's normals of the listed panels
+    of that body only; an unreadable record ends the file (the records before it
+    stay applied)"""
+    import shutil
+    for f in os.listdir(INP):
+        shutil.copy(os.path.join(INP, f), str(tmp_path))
+    base, _ = formats.problem_from_control(INP)
+    nb = int(base.body_start[2] - base.body_start[1])
+    with open(str(tmp_path / "gen_mod_2_4.txt"), "w") as fh:
+        fh.write("3  0.25\n1 -1.5D0\nbad record\n7 9.0\n")
+    p, _ = formats.problem_from_control(str(tmp_path))
+    want = base.dxyz_p.copy()
+    n0 = int(base.body_start[1])
+    want[n0 + 2, 3] = 0.25
+    want[n0 + 0, 3] = -1.5
+    np.testing.assert_array_equal(p.dxyz_p, want)
+    assert nb == 234

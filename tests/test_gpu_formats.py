@@ -51,3 +51,32 @@ def test_config1_input_directory_to_hams_files(tmp_path):
             err = np.max(np.abs(gv[:, blk] - wv[:, blk])) / scale
             assert err <= bar, (stem, f, err)
     print("printed values identical to the expected files: %d of %d" % (same, total), flush=True)
+
+
+def test_generalized_mode_feeds_the_solve(tmp_path):
+    """a generalized mode whose normals are body 1's heave normals
+    (gen_mod_1_4.txt) radiates and integrates exactly as heave does: its added
+    mass and damping rows and columns and its excitation equal heave's"""
+    import shutil
+    inp = os.path.join(GOLDEN, "config1_input")
+    for f in os.listdir(inp):
+        shutil.copy(os.path.join(inp, f), str(tmp_path))
+    base, _ = formats.problem_from_control(inp)
+    nb = int(base.body_start[1])
+    with open(str(tmp_path / "gen_mod_1_4.txt"), "w") as fh:
+        fh.write("".join("%d %r\n" % (k + 1, float(base.dxyz_p[k, 2])) for k in range(nb)))
+    p, _ = formats.problem_from_control(str(tmp_path))
+    c = hm_bem.Context()
+    try:
+        c.setup(p)
+        r = c.solve_frequency(0.06, hm_bem.HM_FP64)
+    finally:
+        c.close()
+    for k in ("amas", "bdmp"):
+        a = r[k]
+        s = np.max(np.abs(a))
+        assert np.max(np.abs(a[:, 3] - a[:, 2])) <= 1e-12 * s, k
+        assert np.max(np.abs(a[3, :] - a[2, :])) <= 1e-12 * s, k
+    f = r["exfc"]
+    assert np.max(np.abs(f[3] - f[2])) <= 1e-12 * np.max(np.abs(f))
+    assert np.max(np.abs(r["amas"][:, 3] - r["amas"][:, 4])) > 1e-3 * np.max(np.abs(r["amas"]))
