@@ -466,11 +466,15 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
     const dim3 eval((h.m + 255) / 256, P);
     if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false>), eval, dim3(256), 0, s, g, f, h);
     else hipLaunchKernelGGL((k_aca_eval_row<16, true>), eval, dim3(256), 0, s, g, f, h);
-    hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
+    // narrow: every pair of bodies is more than 0.05 h apart (bounding circles of
+    // the panel centroids), so no entry is LINTON or PIDCOCK and the fix-up
+    // kernels would only test and return; skipping them keeps the stream from
+    // waiting for CUs behind the other stream's launches for nothing
+    if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
     if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false>), eval, dim3(256), 0, s, g, f, h);
     else hipLaunchKernelGGL((k_aca_eval_col<16, true>), eval, dim3(256), 0, s, g, f, h);
-    hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
+    if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (st % 4 == 3) {
