@@ -1239,9 +1239,11 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
       start.push_back(start.back() + block_sizes[k]);
     }
     if (nblocks > 0 && start.back() != n) return c->fail(HM_ERR_ARG - 12, "block sizes sum to %d, not n = %d", start.back(), n);
-    c->assembled = c->factored = c->solved = false;
     c->hd_ready = false;
   }
+  // both forms overwrite the context's right-hand sides and solution (d_R, d_X):
+  // the per-frequency chain must assemble again before hm_solve / hm_forces
+  c->assembled = c->factored = c->solved = false;
   const size_t cplx = 2 * sizeof(double);
   hipStream_t s = c->stream;
   HIPCHK(c, c->d_R.ensure(cplx * (size_t)n * nrhs), "ziter alloc");

@@ -192,8 +192,9 @@ template <> __device__ __forceinline__ int group_min<16>(int v) {
   return min(v, dpp_i<0x140>(v));
 }
 
-// column pairs of the epsilon table on the mixed-precision paths (the plain
-// kernels build the whole table, as Limes does)
+// column pairs of the epsilon table in the cross approximation's entries (the
+// plain kernels and the mirrored diagonal-block kernels build the whole table,
+// as Limes does)
 constexpr int kMixedEpsPairs = 12;
 
 // Wynn epsilon limit of the partial sums s[0..n] held as rows l + G*k of the
@@ -373,10 +374,10 @@ __device__ __forceinline__ void coop_series_sym(int ntq, double r, double zfh, d
     s4[k] = t4;
   }
   const int n = ntq - 1;
-  l1 = coop_limes<G, K, true, kMixedEpsPairs>(n, s1, l);
-  l2 = coop_limes<G, K, true, kMixedEpsPairs>(n, s2, l);
-  l3 = coop_limes<G, K, true, kMixedEpsPairs>(n, s3, l);
-  l4 = coop_limes<G, K, true, kMixedEpsPairs>(n, s4, l);
+  l1 = coop_limes<G, K, true>(n, s1, l);
+  l2 = coop_limes<G, K, true>(n, s2, l);
+  l3 = coop_limes<G, K, true>(n, s3, l);
+  l4 = coop_limes<G, K, true>(n, s4, l);
 }
 
 }  // namespace hm

@@ -395,38 +395,83 @@ HM_DEV void fin_strip(double r, double zf, double zp, FinOut& o) {
 }
 
 // one evanescent term of the eigenfunction series (EIGEN/EIGENE loop body);
-// FAST: K0/K1 by sf_k01_fast (mixed-precision paths)
+// FAST: K0/K1 by sf_k01_fast (mixed-precision paths). The depth factors are
+// multiplied together before the mode factor, and both come from sincos, so
+// that the term of the mirrored pair (zfh, zph swapped) is the same number:
+// a pair and its mirror evaluated one by one, or together by fin_evanescent_sym,
+// give identical partial sums.
 template <bool FAST = false>
 HM_DEV void fin_evanescent(double km, double r, double zfh, double zph, double h, double& t1,
                            double& t2, double& t3) {
+#pragma clang fp contract(off)
   const double twokmh = 2.0 * km * h;
   const double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
   double k0, k1;
   sf_k01_t<FAST>(km * r, k0, k1);
-  double skzf, ckzf;
+  double skzf, ckzf, skzp, ckzp;
   sincos(km * zfh, &skzf, &ckzf);
-  const double ckzp = cos(km * zph);
-  t1 = pm * k0 * ckzf * ckzp;
-  t2 = -pm * km * k1 * ckzf * ckzp;
-  t3 = -pm * km * k0 * skzf * ckzp;
+  sincos(km * zph, &skzp, &ckzp);
+  const double cc = ckzf * ckzp;
+  t1 = (pm * k0) * cc;
+  t2 = (-pm * km * k1) * cc;
+  t3 = (-pm * km * k0) * (skzf * ckzp);
 }
 
-// fin_evanescent and t4, the derivative of the term with respect to zph (the
-// mirrored pair's t3; t1, t2 are symmetric in zfh, zph); mixed paths only, so
-// K0/K1 by sf_k01_fast
+// fin_evanescent and t4, the derivative of the term with respect to zph: the
+// mirrored pair's t3 (t1, t2 are symmetric in zfh, zph), bit for bit what
+// fin_evanescent gives for the mirror. FAST as there.
+template <bool FAST = false>
 HM_DEV void fin_evanescent_sym(double km, double r, double zfh, double zph, double h, double& t1, double& t2,
                                double& t3, double& t4) {
+#pragma clang fp contract(off)
   const double twokmh = 2.0 * km * h;
   const double pm = 4.0 / (1.0 + sin(twokmh) / twokmh) / h;
   double k0, k1;
-  sf_k01_fast(km * r, k0, k1);
-  double skzf, ckzf;
+  sf_k01_t<FAST>(km * r, k0, k1);
+  double skzf, ckzf, skzp, ckzp;
   sincos(km * zfh, &skzf, &ckzf);
-  const double ckzp = cos(km * zph), skzp = sin(km * zph);
-  t1 = pm * k0 * ckzf * ckzp;
-  t2 = -pm * km * k1 * ckzf * ckzp;
-  t3 = -pm * km * k0 * skzf * ckzp;
-  t4 = -pm * km * k0 * ckzf * skzp;
+  sincos(km * zph, &skzp, &ckzp);
+  const double cc = ckzf * ckzp;
+  const double d0 = -pm * km * k0;
+  t1 = (pm * k0) * cc;
+  t2 = (-pm * km * k1) * cc;
+  t3 = d0 * (skzf * ckzp);
+  t4 = d0 * (skzp * ckzf);
+}
+
+// Evanescent terms summed directly, without the epsilon acceleration, for the
+// cross approximation's entries between bodies (R/h >= 0.05; mixed path only).
+// With rho = R/h and kappa_n = k_n h >= (n - 1/2) pi, the terms of the three
+// series are bounded by (1/h^2) (5.72/rho) sqrt(pi x / 2) exp(-x) at
+// x = kappa_n rho (pm_n <= 4.77/h for kappa_n >= pi/2 + pi, K0, K1 <= 1.2
+// sqrt(pi/(2x)) exp(-x) for x >= 2), and the terms after n decrease at least
+// by exp(-pi rho) each, so stopping where that bound over (1 - exp(-pi rho))
+// falls below 1e-10 / h^2 leaves a tail below 1e-10 of the scale of dG/dR at
+// R = h (and about 1e-12 of the entries themselves at R/h <= 0.5). The count
+// comes from three fixed-point steps of x = ln(7.17 / (rho eps (1 - q))) +
+// ln(x) / 2 and is capped by the 199 evanescent roots held per frequency.
+HM_DEV int fin_direct_terms(double rr, double h) {
+  const float rho = (float)(fabs(rr) / h);
+  const float q = __expf(-3.14159265f * rho);
+  const float c = __logf(7.17f / (rho * 1.0e-10f * (1.0f - q)));
+  float x = 30.0f;
+  for (int k = 0; k < 3; ++k) x = c + 0.5f * __logf(x);
+  // kappa_N = x / rho >= (N - 1/2) pi: N = x / (pi rho) + 1/2, n = N + 1
+  const int n = (int)(x / (3.14159265f * rho) + 1.5f) + 1;
+  return n < 2 ? 2 : (n > 200 ? 200 : n);       // 200 roots per frequency: modes 1..199
+}
+
+// the evanescent sums of modes 1 .. n-1 (n = fin_direct_terms), FAST K0/K1
+HM_DEV void fin_evanescent_direct(double r, double zfh, double zph, const double* wvn, double h, int n,
+                                  double& s1, double& s2, double& s3) {
+  s1 = s2 = s3 = 0.0;
+  for (int i = 1; i < n; ++i) {
+    double t1, t2, t3;
+    fin_evanescent<true>(wvn[i], r, zfh, zph, h, t1, t2, t3);
+    s1 += t1;
+    s2 += t2;
+    s3 += t3;
+  }
 }
 
 // R/h >= 0.5: at most 9 evanescent terms, early exit at 1e-6 (EIGEN :281)
@@ -573,50 +618,56 @@ HM_DEV void fin_pidcock_head(double r, double zf, double zp, const double* wvn, 
   o.gz_im = dp0 * j0;
 }
 // term i >= 1 of the three accelerated series (mode i minus its large-i form);
-// FAST: K0/K1 by sf_k01_fast (mixed-precision paths)
+// FAST: K0/K1 by sf_k01_fast (mixed-precision paths). As in fin_evanescent the
+// depth factors are multiplied together first, every sine and cosine comes
+// from sincos and nothing is contracted, so that the mirrored pair's terms
+// (fin_pidcock_term_sym's t4, and the cached form) are the same numbers.
 template <bool FAST = false>
 HM_DEV void fin_pidcock_term(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
                              double& t2, double& t3) {
+#pragma clang fp contract(off)
   const double umt = i * kPi / h;
   double k0, k1, k0t, k1t;
   sf_k01_t<FAST>(um * r, k0, k1);
   sf_k01_t<FAST>(umt * r, k0t, k1t);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-  double su, cu, sut, cut;
+  double su, cu, sut, cut, sup, cup, sutp, cutp;
   sincos(um * zfh, &su, &cu);
   sincos(umt * zfh, &sut, &cut);
-  const double cup = cos(um * zph), cutp = cos(umt * zph);
-  const double am = ac * cu * cup;
-  const double amt = cut * cutp / h;
-  const double dam = -um * ac * su * cup;
-  const double damt = -umt * sut * cutp / h;
+  sincos(um * zph, &sup, &cup);
+  sincos(umt * zph, &sutp, &cutp);
+  const double am = ac * (cu * cup);
+  const double amt = (cut * cutp) / h;
+  const double dam = (-um * ac) * (su * cup);
+  const double damt = (-umt * (sut * cutp)) / h;
   t1 = am * k0 - amt * k0t;
   t2 = -um * am * k1 + umt * amt * k1t;
   t3 = dam * k0 - damt * k0t;
 }
 // fin_pidcock_term and t4, the derivative of the term with respect to zph:
 // the term of the mirrored pair (field and source depths swapped) has t1, t2
-// unchanged and t4 as its t3 (the series is symmetric in zfh, zph); mixed paths
-// only, so K0/K1 by sf_k01_fast
+// unchanged and t4 as its t3 (the series is symmetric in zfh, zph), bit for
+// bit. FAST as in fin_pidcock_term.
+template <bool FAST = false>
 HM_DEV void fin_pidcock_term_sym(int i, double um, double r, double zfh, double zph, double v, double h, double& t1,
                                  double& t2, double& t3, double& t4) {
+#pragma clang fp contract(off)
   const double umt = i * kPi / h;
   double k0, k1, k0t, k1t;
-  sf_k01_fast(um * r, k0, k1);
-  sf_k01_fast(umt * r, k0t, k1t);
+  sf_k01_t<FAST>(um * r, k0, k1);
+  sf_k01_t<FAST>(umt * r, k0t, k1t);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-  // t1..t3 with the operations of fin_pidcock_term
-  double su, cu, sut, cut;
+  double su, cu, sut, cut, sup, cup, sutp, cutp;
   sincos(um * zfh, &su, &cu);
   sincos(umt * zfh, &sut, &cut);
-  const double cup = cos(um * zph), cutp = cos(umt * zph);
-  const double sup = sin(um * zph), sutp = sin(umt * zph);
-  const double am = ac * cu * cup;
-  const double amt = cut * cutp / h;
-  const double dam = -um * ac * su * cup;
-  const double damt = -umt * sut * cutp / h;
-  const double damp = -um * ac * cu * sup;
-  const double damtp = -umt * cut * sutp / h;
+  sincos(um * zph, &sup, &cup);
+  sincos(umt * zph, &sutp, &cutp);
+  const double am = ac * (cu * cup);
+  const double amt = (cut * cutp) / h;
+  const double dam = (-um * ac) * (su * cup);
+  const double damt = (-umt * (sut * cutp)) / h;
+  const double damp = (-um * ac) * (sup * cu);
+  const double damtp = (-umt * (sutp * cut)) / h;
   t1 = am * k0 - amt * k0t;
   t2 = -um * am * k1 + umt * amt * k1t;
   t3 = dam * k0 - damt * k0t;
@@ -624,34 +675,37 @@ HM_DEV void fin_pidcock_term_sym(int i, double um, double r, double zfh, double 
 }
 // The frequency-independent half of the PIDCOCK term of row i (the i pi / h
 // mode): c[0] = amt k0t, c[1] = umt amt k1t, c[2] = damt k0t, c[3] = damtp k0t
-// (depends on R, the depths and h only; cached once per geometry)
+// (depends on R, the depths and h only; cached once per geometry), with the
+// operations of fin_pidcock_term_sym
 HM_DEV void fin_pidcock_tilde(int i, double r, double zfh, double zph, double h, double* c) {
+#pragma clang fp contract(off)
   const double umt = i * kPi / h;
   double k0t, k1t;
   sf_k01(umt * r, k0t, k1t);
-  double sut, cut;
+  double sut, cut, sutp, cutp;
   sincos(umt * zfh, &sut, &cut);
-  const double cutp = cos(umt * zph), sutp = sin(umt * zph);
-  const double amt = cut * cutp / h;
-  const double damt = -umt * sut * cutp / h;
-  const double damtp = -umt * cut * sutp / h;
+  sincos(umt * zph, &sutp, &cutp);
+  const double amt = (cut * cutp) / h;
+  const double damt = (-umt * (sut * cutp)) / h;
+  const double damtp = (-umt * (sutp * cut)) / h;
   c[0] = amt * k0t;
   c[1] = umt * amt * k1t;
   c[2] = damt * k0t;
   c[3] = damtp * k0t;
 }
-// fin_pidcock_term_sym with the i pi / h half from the cache
+// fin_pidcock_term_sym with the i pi / h half from the cache (the same numbers)
 HM_DEV void fin_pidcock_term_cached(double um, double r, double zfh, double zph, double v, double h, const double* c,
                                     double& t1, double& t2, double& t3, double& t4) {
+#pragma clang fp contract(off)
   double k0, k1;
-  sf_k01_fast(um * r, k0, k1);
+  sf_k01(um * r, k0, k1);
   const double ac = (um * um + v * v) / ((um * um + v * v) * h - v);
-  double su, cu;
+  double su, cu, sup, cup;
   sincos(um * zfh, &su, &cu);
-  const double cup = cos(um * zph), sup = sin(um * zph);
-  const double am = ac * cu * cup;
-  const double dam = -um * ac * su * cup;
-  const double damp = -um * ac * cu * sup;
+  sincos(um * zph, &sup, &cup);
+  const double am = ac * (cu * cup);
+  const double dam = (-um * ac) * (su * cup);
+  const double damp = (-um * ac) * (sup * cu);
   t1 = am * k0 - c[0];
   t2 = -um * am * k1 + c[1];
   t3 = dam * k0 - c[2];

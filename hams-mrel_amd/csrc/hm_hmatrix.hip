@@ -85,7 +85,7 @@ __device__ void block_sum4(double* v, double (*sh)[256]) {
 // Called by every lane of the workgroup (the Green function's series run in
 // lane groups of 16); a lane with active = false gets nothing. LINTON entries
 // come out wrong here and are redone by residual_entry_direct (k_aca_fix_*).
-template <int G, bool PID>
+template <int G, bool PID, bool DIRECT>
 __device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& f, const double* wvn, const HmatDev& h,
                                                const Proc& q, int m, int k, int i, int j, bool active, double& er,
                                                double& ei) {
@@ -94,7 +94,7 @@ __device__ __forceinline__ void residual_entry(const GeomDev& g, const FreqDev& 
     entry(g, f, wvn, h, q.mat, q.b1 * m + i, q.b2 * m + j, er, ei);
   } else {
     double ar, ai, sr, si;
-    pair_entry_coop<G, PID>(g, f, wvn, active, q.b1 * m + i, q.b2 * m + j, ar, ai, sr, si);
+    pair_entry_coop<G, PID, DIRECT>(g, f, wvn, active, q.b1 * m + i, q.b2 * m + j, ar, ai, sr, si);
     if (!active) return;
     er = q.mat ? sr : ar;
     ei = q.mat ? si : ai;
@@ -162,7 +162,7 @@ __device__ __forceinline__ int sorted_entry(const GeomDev& g, const FreqDev& f, 
 // (grid: column chunks x processes)
 // the entry kernels are compiled for 4 waves per SIMD (128 VGPRs, ~270 bytes a
 // lane of cached spills): 21 % faster than at their natural 208 VGPRs
-template <int G, bool PID>
+template <int G, bool PID, bool DIRECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_row(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
   const int p = blockIdx.y;
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const bool active = j0 < h.m;
   const int j = active ? j0 : h.m - 1;
   double er = 0.0, ei = 0.0;
-  residual_entry<G, PID>(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
+  residual_entry<G, PID, DIRECT>(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
   if (!active) return;
   double* rb = h.rowbuf + 2LL * p * h.m;
   rb[2 * j] = er;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
 }
 
 // residual pivot columns u_k = A(:, j*) - U V(j*, :)^T, one entry per thread
-template <int G, bool PID>
+template <int G, bool PID, bool DIRECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_col(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
   const int p = blockIdx.y;
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const int i = active ? i0 : h.m - 1;
   const int k = h.rank[p];
   double er = 0.0, ei = 0.0;
-  residual_entry<G, PID>(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
+  residual_entry<G, PID, DIRECT>(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
   if (!active) return;
   q.U[2 * (i + (long long)k * h.m)] = er;
   q.U[2 * (i + (long long)k * h.m) + 1] = ei;
@@ -464,16 +464,16 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   const int max_steps = h.kmax + h.m;
   for (; st < max_steps && open > 0; ++st) {
     const dim3 eval((h.m + 255) / 256, P);
-    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false>), eval, dim3(256), 0, s, g, f, h);
-    else hipLaunchKernelGGL((k_aca_eval_row<16, true>), eval, dim3(256), 0, s, g, f, h);
+    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);
+    else hipLaunchKernelGGL((k_aca_eval_row<16, true, false>), eval, dim3(256), 0, s, g, f, h);
     // narrow: every pair of bodies is more than 0.05 h apart (bounding circles of
     // the panel centroids), so no entry is LINTON or PIDCOCK and the fix-up
     // kernels would only test and return; skipping them keeps the stream from
     // waiting for CUs behind the other stream's launches for nothing
     if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
-    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false>), eval, dim3(256), 0, s, g, f, h);
-    else hipLaunchKernelGGL((k_aca_eval_col<16, true>), eval, dim3(256), 0, s, g, f, h);
+    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false, true>), eval, dim3(256), 0, s, g, f, h);
+    else hipLaunchKernelGGL((k_aca_eval_col<16, true, false>), eval, dim3(256), 0, s, g, f, h);
     if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
     if ((e = hipGetLastError()) != hipSuccess) return e;

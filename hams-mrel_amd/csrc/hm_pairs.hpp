@@ -75,7 +75,9 @@ __device__ __forceinline__ bool deferred_entry(const GeomDev& g, const FreqDev& 
   return reg == kLinton || (!pid && reg == kPidcock);
 }
 
-template <int G, bool PID = true>
+// DIRECT: EIGENE entries take the evanescent sums of fin_evanescent_direct,
+// each lane its own entry (no epsilon table), instead of the lane groups.
+template <int G, bool PID = true, bool DIRECT = false>
 __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev& f, const double* wvn, bool active,
                                                 int i, int j, double& ar, double& ai, double& sr, double& si) {
   constexpr int KP = (kPidcockMaxTerms + G - 1) / G, KE = (kEigeneMaxTerms + G - 1) / G;
@@ -88,9 +90,11 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
   const int reg = h > 0.0 && !zero && !inf ? fin_regime(rr, h) : -1;
   const int nt = !active ? 0
                  : reg == kPidcock ? (PID ? fin_pidcock_terms(rr, h) : 0)
-                 : reg == kEigene ? fin_eigene_terms(rr, h) : 0;
+                 : reg == kEigene && !DIRECT ? fin_eigene_terms(rr, h) : 0;
   const int l = threadIdx.x & (G - 1);
   double L1 = 0.0, L2 = 0.0, L3 = 0.0;
+  // with DIRECT and no PIDCOCK entries no lane group runs a series
+  if constexpr (PID || !DIRECT)
   for (int q = 0; q < G; ++q) {
     const int ntq = group_bcast<G>(nt, q);
     if (ntq == 0) continue;                  // uniform over the group
@@ -120,6 +124,7 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
         fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
       } else if (reg == kEigene) {
         fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+        if (DIRECT) fin_evanescent_direct(rr, zf + h, zp + h, wvn, h, fin_direct_terms(rr, h), L1, L2, L3);
         fo.g_re += L1;
         fo.gr_re += L2;
         fo.gz_re += L3;

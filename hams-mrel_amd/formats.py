@@ -9,7 +9,15 @@ input directory runs unchanged and its outputs compare line by line.
   frequency types 1-5 to (w1, wk, tp, wl, v) and the output value OUFR.
 * ``problem_from_control``: the bodies of ReadPanelMeshMulti
   (ReadPanelMeshMulti.f90:46-111) placed by the control file's LCS and
-  reference centres, with the reference's constants.
+  reference centres, with the reference's constants; with If_remove_irr_freq
+  = 1 the interior waterplane meshes too (WaterPlaneMesh.pnl for one body,
+  WaterPlaneMesh_k.pnl per body where present, HAMS_Prog.f90:169-173,
+  :399-418).
+* ``frequency_records``: the frequency loop's records in the reference's
+  order, the zero- and infinite-frequency limits first when
+  0_inf_frequency_limits = 1 (InputFiles.f90:88-110, ImplementSubs.f90:45-70).
+* ``run_input_directory``: an input directory solved on the GPU record by
+  record and written as the HAMS and WAMIT files.
 * ``write_hams``: the OAMASS/ODAMPING/OEXFOR files (PotentWavForceMulti.f90:
   119, :202-205; FORMAT 1000 F7.3,1x,F7.3,1x,6E14.5 and 1010 ... 6E14.6).
 * ``write_wamit_1`` / ``write_wamit_3``: the WAMIT-style .1 and .3 files
@@ -196,15 +204,49 @@ def wave_properties(value, inft, ouft, h, g=G_MODULE, g_disp=G_DISPERSION):
     return out
 
 
-def problem_from_control(inp, hull="HullMesh_{k}.pnl", single_hull="HullMesh.pnl"):
+def _waterplane(path, lcs, log):
+    """interior waterplane mesh of one body (ReadWTPLMesh, ReadPanelMesh.f90:74-
+    98: every node on z = 0, else the reference stops)"""
+    wp = meshes.read_pnl(path, lcs=lcs)
+    bad = np.nonzero(np.abs(wp.xyz[:, 2]) > 1e-10)[0]
+    if len(bad):
+        raise ValueError("Error: Z Coordinate is not zero at Node No. %d of %s" % (bad[0] + 1, path))
+    return wp
+
+
+def problem_from_control(inp, hull="HullMesh_{k}.pnl", single_hull="HullMesh.pnl", log=print):
     """meshes.Problem of a HAMS input directory: the bodies' .pnl meshes placed by
     the control file's LCS and reference centres, the frequencies converted by
-    wave_properties, the headings in radians, the reference's constants"""
+    wave_properties, the headings in radians, the reference's constants.
+    If_remove_irr_freq = 1 adds the interior waterplane meshes (one body:
+    WaterPlaneMesh.pnl, which must exist; several: WaterPlaneMesh_k.pnl for the
+    bodies that have one, HAMS_Prog.f90:399-418). Wave_diffrac_solution must be
+    1 (total potential); 2 (scattered potential only) is not provided by the
+    C-ABI and raises NotImplementedError. The frequency list holds the regular
+    frequencies only; frequency_records adds the zero- and infinite-frequency
+    limits of 0_inf_frequency_limits = 1."""
     cf = read_control_file(inp)
+    if cf.isol not in (1, 2):
+        raise ValueError("Error: The input for ISOL should be either 1 or 2.")
+    if cf.isol == 2:
+        raise NotImplementedError("Wave_diffrac_solution 2 (scattered potential, PotentWavForceMulti.f90:72) "
+                                  "is not provided: the excitation of the GPU path includes the incident potential")
+    if cf.irsp not in (0, 1):
+        raise ValueError("If_remove_irr_freq must be 0 or 1, got %r" % cf.irsp)
     bodies = []
     for b in range(cf.nbody):
         name = single_hull if cf.nbody == 1 else hull.format(k=b + 1)
-        bodies.append(meshes.read_pnl(os.path.join(inp, name), lcs=tuple(cf.lcs[b]), xr=tuple(cf.xr[b])))
+        lcs = tuple(cf.lcs[b])
+        body = meshes.read_pnl(os.path.join(inp, name), lcs=lcs, xr=tuple(cf.xr[b]))
+        if cf.irsp == 1:
+            wname = "WaterPlaneMesh.pnl" if cf.nbody == 1 else "WaterPlaneMesh_%d.pnl" % (b + 1)
+            wpath = os.path.join(inp, wname)
+            if os.path.exists(wpath) or cf.nbody == 1:
+                wp = _waterplane(wpath, lcs, log)
+                body.ixyz, body.incn, body.incon = wp.xyz, wp.ncn, wp.ncon
+            else:
+                log(" Warning: The waterplane mesh file does not exist for Body %d" % (b + 1))
+        bodies.append(body)
     w = [wave_properties(v, cf.inft, cf.ouft, cf.h)["w1"] for v in cf.values]
     p = meshes.Problem(bodies, h=cf.h, omegas=np.array(w), beta=np.deg2rad(cf.headings_deg),
                        name=os.path.basename(os.path.normpath(inp)))
@@ -212,12 +254,92 @@ def problem_from_control(inp, hull="HullMesh_{k}.pnl", single_hull="HullMesh.pnl
     return meshes.reference_parity_constants(p), cf
 
 
+def frequency_records(cf, g=G_MODULE, g_disp=G_DISPERSION):
+    """The records of the reference's frequency loop (NPER of them, InputFiles.f90:
+    88-110): with 0_inf_frequency_limits = 1 the zero-frequency record (V = WK =
+    W1 = 0, TP = WL = -1) and the infinite-frequency record (V = WK = W1 = -1,
+    TP = WL = 0) come first (ImplementSubs.f90:45-70), then the listed values
+    through wave_properties. Each record: dict(kind 'zero' | 'inf' | 'wave',
+    w1, wk, tp, wl, v, oufr)."""
+    out = []
+    if cf.sybo == 1:
+        for kind, val, per in (("zero", 0.0, -1.0), ("inf", -1.0, 0.0)):
+            r = dict(kind=kind, w1=val, wk=val, tp=per, wl=per, v=val)
+            r["oufr"] = r[("v", "wk", "w1", "tp", "wl")[cf.ouft - 1]]
+            out.append(r)
+    for v in cf.values:
+        r = wave_properties(v, cf.inft, cf.ouft, cf.h, g, g_disp)
+        r["kind"] = "wave"
+        out.append(r)
+    return out
+
+
+def run_input_directory(inp, outdir=None, precision=None, ctx=None):
+    """A HAMS input directory solved on the GPU in the reference's record order
+    (HAMS_Prog.f90:271-300, :629-724): each record is a frequency solve or, for
+    0_inf_frequency_limits = 1, a zero- or infinite-frequency limit solve; with
+    outdir the HAMS files (outdir/Hams_format) and the WAMIT .1/.3 files
+    (outdir/Wamit_format, the limits' records as the reference writes them)
+    are written. Returns (records, amas, bdmp, exfc) with amas, bdmp
+    (nrec, 6 nbody, 6 nbody) and exfc (nrec, 6 nbody, nbeta)."""
+    import hm_bem
+    if precision is None:
+        precision = hm_bem.HM_FP64
+    p, cf = problem_from_control(inp)
+    recs = frequency_records(cf, p.g, p.g_disp)
+    own = ctx is None
+    if own:
+        ctx = hm_bem.Context()
+    try:
+        ctx.setup(p)
+        res = []
+        for r in recs:
+            if r["kind"] == "wave":
+                res.append(ctx.solve_frequency(r["w1"], precision))
+            else:
+                res.append(ctx.solve_limit(0 if r["kind"] == "zero" else 1, precision))
+    finally:
+        if own:
+            ctx.close()
+    amas = np.array([x["amas"] for x in res])
+    bdmp = np.array([x["bdmp"] for x in res])
+    exfc = np.array([x["exfc"] for x in res])
+    if outdir is not None:
+        wk = [r["wk"] for r in recs]
+        w1 = [r["w1"] for r in recs]
+        write_hams(os.path.join(outdir, "Hams_format"), wk, w1, amas, bdmp, exfc)
+        wam = os.path.join(outdir, "Wamit_format")
+        os.makedirs(wam, exist_ok=True)
+        oufr = [r["oufr"] for r in recs]
+        limit = [r["kind"] != "wave" for r in recs]
+        write_wamit_1(os.path.join(wam, "AmssDamp.1"), oufr, w1, amas, bdmp, p.rho, limit=limit)
+        write_wamit_3(os.path.join(wam, "ExcForce.3"), oufr, cf.headings_deg, exfc, p.rho, p.g, p.amp, limit=limit)
+    return recs, amas, bdmp, exfc
+
+
+def _list_directed(recs, start, count):
+    """count values of one list-directed READ beginning at record `start`: blank
+    records are skipped, values run across records, and the rest of the record
+    holding the last value is dropped. Returns (tokens, next record); fewer
+    tokens than count at the end of the file."""
+    out, r = [], start
+    while len(out) < count and r < len(recs):
+        tok = recs[r].replace(",", " ").split()
+        r += 1
+        out += tok[:count - len(out)]
+    return out, r
+
+
 def apply_generalized_modes(p, inp, log=print):
     """The generalized-mode override of CalNormalsMulti (ReadPanelMeshMulti.f90:
-    166-203): gen_mod_{body}_{dof}.txt (body and dof 1-based), one record
-    "panel_id normal_value" per hull panel of the body, replaces that dof's
-    normal component of those panels; a record that does not read ends the
-    file with a warning, as in the reference. Returns the (body, dof) pairs
+    166-203): gen_mod_{body}_{dof}.txt (body and dof 1-based) holds a pair
+    "panel_id normal_value" per hull panel of the body, each read by one
+    list-directed READ(99,*) (:189): blank records are skipped, a pair may run
+    across records, and what follows the pair on its record is dropped. Each
+    pair replaces that dof's normal component of the panel; a value that does
+    not read, the end of the file before every panel has a pair, or a panel
+    number out of range (which the reference would write out of bounds) ends
+    the file with a warning, as in the reference. Returns the (body, dof) pairs
     overridden."""
     done = []
     for b in range(p.nbody):
@@ -228,9 +350,10 @@ def apply_generalized_modes(p, inp, log=print):
                 continue
             with open(path) as fh:
                 recs = fh.read().split("\n")
+            r = 0
             for k in range(nb):
+                tok, r = _list_directed(recs, r, 2)
                 try:
-                    tok = recs[k].replace(",", " ").split()
                     pid, val = int(tok[0]), _num(tok[1])
                     if not 1 <= pid <= nb:
                         raise ValueError
@@ -320,16 +443,20 @@ def write_wamit_1(path, oufr, w1, amas, bdmp, rho, limit=None, row_major=True):
         fh.write("\n".join(out) + "\n")
 
 
-def write_wamit_3(path, oufr, headings_deg, exfc, rho, g, amp=1.0, phase_degrees_twice=True):
+def write_wamit_3(path, oufr, headings_deg, exfc, rho, g, amp=1.0, phase_degrees_twice=True, limit=None):
     """WAMIT .3: OUFR, heading [deg], i, |F|, phase, Re, Im of the excitation
     normalised by rho g A, with Re, Im the reference's (-Im F, -Re F) / (rho g A)
     and the phase ATAN2D(Im, Re) (times 180/pi again by default, as the reference
-    writes it). Records per frequency: heading by heading, body by body."""
+    writes it). Records per frequency: heading by heading, body by body.
+    limit[f] true marks a zero- or infinite-frequency record (period -1 or 0),
+    for which the reference writes no excitation (PotentWavForceMulti.f90:120)."""
     exfc = np.asarray(exfc)
     nf, m, nbeta = exfc.shape
     nb = m // 6
     out = []
     for f in range(nf):
+        if limit is not None and limit[f]:
+            continue
         for ih in range(nbeta):
             for b in range(nb):
                 for md in range(6):

@@ -215,3 +215,101 @@ def test_generalized_mode_files(tmp_path):
     want[n0 + 0, 3] = -1.5
     np.testing.assert_array_equal(p.dxyz_p, want)
     assert nb == 234
+
+
+def test_generalized_mode_list_directed_records(tmp_path):
+    """each pair is one list-directed READ (ReadPanelMeshMulti.f90:189): blank
+    records are skipped, a pair may wrap onto the next record, and values after
+    the pair on its record are dropped"""
+    import shutil
+    for f in os.listdir(INP):
+        shutil.copy(os.path.join(INP, f), str(tmp_path))
+    base, _ = formats.problem_from_control(INP)
+    with open(str(tmp_path / "gen_mod_1_5.txt"), "w") as fh:
+        fh.write("\n\n2 0.5\n\n4\n  -0.75\n6 1.25 99 -8.0\n8, 2.0\n")
+    p, _ = formats.problem_from_control(str(tmp_path))
+    want = base.dxyz_p.copy()
+    for pid, val in ((2, 0.5), (4, -0.75), (6, 1.25), (8, 2.0)):
+        want[pid - 1, 4] = val
+    np.testing.assert_array_equal(p.dxyz_p, want)
+    assert formats._list_directed(["", "1", "", "2 3", "4"], 0, 2) == (["1", "2"], 4)
+    assert formats._list_directed(["1"], 0, 2) == (["1"], 1)
+
+
+def test_wamit_3_skips_limit_records(tmp_path):
+    """the zero- and infinite-frequency records write no excitation (the
+    reference writes .3 records only when TP is neither -1 nor 0,
+    PotentWavForceMulti.f90:120)"""
+    rng = np.random.default_rng(5)
+    m, nbeta = 6, 2
+    exfc = rng.normal(size=(3, m, nbeta)) + 1j * rng.normal(size=(3, m, nbeta))
+    p3 = str(tmp_path / "x.3")
+    formats.write_wamit_3(p3, [0.0, -1.0, 0.7], [0.0, 90.0], exfc, 1000.0, 9.80665, limit=[True, True, False])
+    rows = [[float(t) for t in ln.split()] for ln in open(p3).read().strip().split("\n")]
+    assert len(rows) == nbeta * m
+    assert all(r[0] == 0.7 for r in rows)
+    z = exfc[2, 0, 0] / (1000.0 * 9.80665)
+    assert abs(rows[0][3] - abs(z)) <= 1e-6 * abs(z)
+
+
+def test_example5_input_directory():
+    """the reference's single-body IRR example read as its driver reads it: the
+    hull and the interior waterplane mesh (If_remove_irr_freq 1,
+    HAMS_Prog.f90:169-173), 10 frequencies from 0.05 by 0.01 rad/s, 36 headings
+    by 10 degrees"""
+    p, cf = formats.problem_from_control(os.path.join(GOLDEN, "example5_input"))
+    assert (cf.irsp, cf.sybo, cf.isol, cf.nbody) == (1, 0, 1, 1)
+    assert (p.n, p.ni) == (234, 383)
+    assert np.all(np.abs(p.ixyz_p[:, 2]) == 0.0)
+    np.testing.assert_allclose(p.omegas, 0.05 + 0.01 * np.arange(10), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(np.rad2deg(p.beta), 10.0 * np.arange(36), atol=1e-12)
+    recs = formats.frequency_records(cf)
+    assert [r["kind"] for r in recs] == ["wave"] * 10
+
+
+def _edit_control(src, dst, **kv):
+    """copy of a control file with the named records' values replaced"""
+    out = []
+    for ln in open(src).read().split("\n"):
+        for key, val in kv.items():
+            if ln.strip().startswith(key):
+                ln = ln[:ln.index(key) + len(key)] + "   " + str(val)
+        out.append(ln)
+    with open(dst, "w") as fh:
+        fh.write("\n".join(out))
+
+
+def test_limits_and_solver_options(tmp_path):
+    """0_inf_frequency_limits 1 puts the zero- and infinite-frequency records
+    first (ImplementSubs.f90:45-70: W1 = WK = V = 0, TP = -1; then -1 and TP = 0),
+    with OUFR taken from those values; Wave_diffrac_solution 2 is refused, not
+    ignored; a waterplane node off z = 0 stops the read as in ReadWTPLMesh"""
+    import shutil
+    src = os.path.join(GOLDEN, "example5_input")
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), str(tmp_path))
+    ctl = str(tmp_path / "ControlFile.in")
+    _edit_control(os.path.join(src, "ControlFile.in"), ctl, **{"0_inf_frequency_limits": 1})
+    p, cf = formats.problem_from_control(str(tmp_path))
+    recs = formats.frequency_records(cf)
+    assert [r["kind"] for r in recs] == ["zero", "inf"] + ["wave"] * 10
+    assert (recs[0]["w1"], recs[0]["tp"], recs[0]["oufr"]) == (0.0, -1.0, 0.0)
+    assert (recs[1]["w1"], recs[1]["tp"], recs[1]["oufr"]) == (-1.0, 0.0, -1.0)
+    assert abs(recs[2]["w1"] - 0.05) < 1e-15
+    _edit_control(os.path.join(src, "ControlFile.in"), ctl, Wave_diffrac_solution=2)
+    with pytest.raises(NotImplementedError):
+        formats.problem_from_control(str(tmp_path))
+    _edit_control(os.path.join(src, "ControlFile.in"), ctl, Wave_diffrac_solution=3)
+    with pytest.raises(ValueError, match="ISOL"):
+        formats.problem_from_control(str(tmp_path))
+    _edit_control(os.path.join(src, "ControlFile.in"), ctl)
+    lines = open(str(tmp_path / "WaterPlaneMesh.pnl")).read().split("\n")
+    k = next(i for i, ln in enumerate(lines) if "Node Coordinates" in ln) + 3
+    t = lines[k].split()
+    lines[k] = "    %s  %s  %s  -0.001" % (t[0], t[1], t[2])
+    (tmp_path / "WaterPlaneMesh.pnl").write_text("\n".join(lines))
+    with pytest.raises(ValueError, match="Z Coordinate is not zero at Node No. 3"):
+        formats.problem_from_control(str(tmp_path))
+    _edit_control(os.path.join(src, "ControlFile.in"), ctl, If_remove_irr_freq=0)
+    p, cf = formats.problem_from_control(str(tmp_path))
+    assert (cf.irsp, p.ni) == (0, 0)

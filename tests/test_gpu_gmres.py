@@ -188,6 +188,32 @@ def test_ziter_hmatrix(quad):
         c.ziter_solve(2, None, rhs)
 
 
+def test_ziter_hmatrix_after_solve_needs_a_new_assembly(quad):
+    """hm_ziter_solve overwrites the context's right-hand sides and solution, so
+    after hm_solve -> hm_ziter_solve(2) the forces are refused (HM_ERR_STATE)
+    instead of being reduced from the GMRES solution of the host system; a new
+    assembly restores the chain and its forces"""
+    p, c = quad
+    m = p.n // p.nbody
+    c.set_gmres(restart=50, tol=1e-9, maxiter=500, precond=-1)
+    c.set_hmatrix(False, eps=1e-9, eps_s=1e-9, kmax=m)
+    w = 1.1
+    ref = c.solve_frequency(w, hm_bem.HM_FP64)
+    a, rhs = _assembled(c, w)
+    c.factor(hm_bem.HM_FP64)
+    c.solve()
+    c.hmatrix_build(a, [m] * p.nbody)
+    x, info = c.ziter_solve(2, None, 2.0 * rhs)
+    assert info == 0
+    with pytest.raises(hm_bem.HMError) as e:
+        c.forces()
+    assert e.value.code == -200 and "before hm_solve" in str(e.value)
+    again = c.solve_frequency(w, hm_bem.HM_FP64)
+    for k in KEYS:
+        assert np.array_equal(again[k], ref[k]), k
+    c.hmatrix_release()
+
+
 def test_ziter_argument_errors(quad):
     p, c = quad
     a, rhs = _assembled(c, 0.9)

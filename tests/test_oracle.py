@@ -243,6 +243,38 @@ def test_config1_expected_outputs(oracle):
     assert relerr(bdmp, bdmp.T) < 1e-2
 
 
+def _example5_expected(kind, k):
+    return np.loadtxt(os.path.join(GOLDEN, "example5_expected", f"{kind}{k}.txt"))
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("f", [0, 9])
+def test_example5_irr_expected_outputs(oracle, f):
+    """examples/example5_single_body of the reference (234 hull + 383 interior
+    waterplane panels, h = 70 m, irregular-frequency removal on, 36 headings),
+    read from its input directory: the oracle's whole IRR chain (waterplane
+    rows of A, the normal equations C = A^T A and D = A^T B, the solve and the
+    force reductions) against the case's Hams_format files, which print 5
+    significant digits for A and B (E14.5) and 6 for F (E14.6); the bar is
+    that of the config1 case, 5e-5 of the largest entry."""
+    import formats
+    p, cf = formats.problem_from_control(os.path.join(GOLDEN, "example5_input"))
+    assert (p.n, p.ni, cf.irsp, p.nbeta) == (234, 383, 1, 36)
+    w = float(p.omegas[f])
+    r = oracle.solve_frequency(p, w)
+    assert r["info"] == 0
+    amas, bdmp, exfc = r["amas"], r["bdmp"], r["exfc"]
+    for k in range(6):
+        ea = _example5_expected("OAMASS", k + 1)[f]
+        eb = _example5_expected("ODAMPING", k + 1)[f]
+        assert abs(ea[1] - w) < 5e-4
+        assert np.max(np.abs(amas[k] - ea[2:])) <= 5e-5 * np.max(np.abs(amas)) + 1e-6, (k, amas[k], ea[2:])
+        assert np.max(np.abs(bdmp[k] - eb[2:])) <= 5e-5 * np.max(np.abs(bdmp)) + 1e-6, (k, bdmp[k], eb[2:])
+        ef = _example5_expected("OEXFOR", k + 1)[36 * f:36 * (f + 1)]
+        np.testing.assert_allclose(exfc[k].real, ef[:, 2], rtol=2e-5, atol=2e-5 * np.max(np.abs(exfc)))
+        np.testing.assert_allclose(exfc[k].imag, ef[:, 3], rtol=2e-5, atol=2e-5 * np.max(np.abs(exfc)))
+
+
 def test_field_points_reproduce_the_hull_potentials(oracle):
     """hmo_field_points (CalPotentialMulti) pinned by what the boundary-integral
     representation fixes: at a hull panel's centroid it is the collocation
