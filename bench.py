@@ -44,6 +44,10 @@ def sync():
 
 
 REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen": 3}
+KERNEL_OF_STAGE = {"asm_linton": "k_assemble_list<0>", "asm_pidcock": "k_assemble_sym<1,16,7,true>",
+                   "asm_eigene": "k_assemble_sym<2,8,7,false>", "asm_eigen": "k_assemble_list<3>",
+                   "aca_entries": "k_aca_eval_row/col<8,false,true>"}
+ACCURACY_BAR = 1e-4
 
 # HBM bytes per launch of the dominant kernel, measured on one MI355X by two
 # counter passes of `bench.py --steps 1 --assemble-only` (rocprofv3 --pmc
@@ -53,10 +57,7 @@ REGIME_STAGE = {"asm_linton": 0, "asm_pidcock": 1, "asm_eigene": 2, "asm_eigen":
 # cached frequency-independent term halves (31 GB read per launch); it runs at 5
 # waves per SIMD, and register-spill traffic (scratch that misses the L2) is
 # most of the rest.
-MEASURED_TRAFFIC = {
-    ("c5", "hmatrix", "asm_pidcock"): 57449828.7 * 1024 * 2 + 71234060.3 * 1024,
-}
-
+MEASURED_TRAFFIC = {}
 
 def flop_model(name):
     """algorithmic flops per pair by classification key (tests/golden/flop_model.json)"""
@@ -99,29 +100,30 @@ PATH_DESC = {
               "allows, plain series elsewhere) + restarted GMRES(50), tol 1e-6: FP32 complex GEMM products on "
               "MFMA, FP32 block-diagonal LU preconditioner (one block per body), FP64 residuals"),
     "fp32ir": "FP64 assembly + FP32 LU (MFMA) + FP64 iterative refinement",
-    "hmatrix": ("H-matrix: dense FP64 diagonal block per body (learned BF16 surrogate for the evanescent sums "
-                "where its validated error bound allows), off-diagonal blocks by adaptive cross approximation "
-                "evaluated straight from the FP64 Green function, right-hand sides S W from the H-matrix of S; "
-                "restarted GMRES(50), tol 1e-6, with FP32 MFMA H-matrix products, FP32 block-diagonal LU "
-                "preconditioner and FP64 H-matrix residuals"),
+    "hmatrix": ("H-matrix: dense FP64 diagonal block per body (each pair with its mirror, bit for bit the plain "
+                "path's blocks), off-diagonal blocks by adaptive cross approximation (tolerance 1e-6 for A, 1e-7 "
+                "for S) whose entries come straight from the FP64 Green function with directly summed evanescent "
+                "modes, right-hand sides S W from the H-matrix of S; restarted GMRES(50), tol 1e-6, with FP32 "
+                "MFMA H-matrix products, FP32 block-diagonal LU preconditioner and FP64 H-matrix residuals"),
 }
 
 
-def run_steps(ctx, omegas, k0, steps, precision):
+def run_steps(ctx, omegas, k0, steps, precision, ranks=None):
     out = []
     for s in range(steps):
         w = float(omegas[(k0 + s) % len(omegas)])
         out.append(ctx.solve_frequency(w, precision))
+        if ranks is not None:
+            ranks.append(ctx.hmatrix_stats()["mean_rank"])
     return out
 
 
-def time_path(ctx, prob, precision, steps, warmup, stage_times=True):
-    import hm_bem
+def time_path(ctx, prob, precision, steps, warmup, stage_times=True, ranks=None):
     run_steps(ctx, prob.omegas, 0, warmup, precision)
     sync()
     ctx.set_timing(stage_times)
     t0 = time.perf_counter()
-    res = run_steps(ctx, prob.omegas, warmup, steps, precision)
+    res = run_steps(ctx, prob.omegas, warmup, steps, precision, ranks)
     sync()
     dt = time.perf_counter() - t0
     stages = ctx.timing()
@@ -129,15 +131,62 @@ def time_path(ctx, prob, precision, steps, warmup, stage_times=True):
     return dt, res, stages
 
 
-def cpu_baseline(prob, seconds=20.0, threads=16):
-    """The reference's own CPU kernels timed on the host: the body of its
-    CALGREEN table-fill loop (SGLINTBD + FINGREEN3D/INFGREEN3D, compiled from the
-    reference sources into oracle/_ref with OpenMP, CalGreenFunc.f90:58-115) over
-    a random sample of the workload's hull panel pairs on `threads` cores, plus
-    LAPACK ZGETRF/ZGETRS (numpy) at N <= 3000 scaled by N^3 to the workload's N.
+def host_cores():
+    """physical cores among the CPUs this process may run on (lscpu's core ids,
+    one thread per core); the affinity count where lscpu is unavailable"""
+    import subprocess
+    cpus = sorted(os.sched_getaffinity(0))
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        core = {}
+        for ln in out.split("\n"):
+            if ln and not ln.startswith("#"):
+                c, k, so = ln.split(",")[:3]
+                core[int(c)] = (so, k)
+        phys = {core[c] for c in cpus if c in core}
+        return max(1, len(phys)) if phys else len(cpus)
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        return len(cpus)
+
+
+def offdiag_flops_per_entry(prob, fpp, step=7):
+    """average counted flops (reference algorithm, tests/golden/flop_model.json)
+    of the entries between different bodies, over every step-th pair"""
+    bs = prob.body_start
+    tot, cnt = 0.0, 0
+    h = prob.h
+    for b1 in range(prob.nbody):
+        xi = prob.xyz_p[bs[b1]:bs[b1 + 1]:step]
+        for b2 in range(prob.nbody):
+            if b1 == b2:
+                continue
+            xj = prob.xyz_p[bs[b2]:bs[b2 + 1]:step]
+            rr = np.hypot(xi[:, None, 0] - xj[None, :, 0], xi[:, None, 1] - xj[None, :, 1]).ravel()
+            rh = rr / h
+            reg = np.where(rh < 5e-4, 0, np.where(rh < 0.05, 1, np.where(rh < 0.5, 2, 3)))
+            nt = np.where(reg == 2, (np.float32(-88.89) * rh + np.float32(54.45)).astype(int),
+                          np.where(reg == 1, (np.float32(-1010.10) * rh + np.float32(100.50)).astype(int), 0))
+            keys = reg * 128 + nt
+            u, c = np.unique(keys, return_counts=True)
+            for k, cc in zip(u, c):
+                tot += cc * fpp.get(int(k), fpp.get(128 * int(k // 128), 0.0))
+            cnt += len(keys)
+    return tot / max(cnt, 1)
+
+
+def cpu_baseline(prob, seconds=20.0, threads=None):
+    """The reference's own CPU kernels timed on this host's physical cores: the
+    body of its CALGREEN table-fill loop (SGLINTBD + FINGREEN3D/INFGREEN3D,
+    compiled from the reference sources into oracle/_ref with OpenMP,
+    CalGreenFunc.f90:58-115, CalGreenFuncMulti.f90:123-180) over a random
+    sample of the workload's panel pairs, and the rate of a LAPACK ZGETRF/ZGETRS
+    (numpy's, at a stated N <= 6144) applied to the workload's 8N^3/3 LU flops.
+    GRN_KIND (WavDynMods.f90:47) only sets the precision the table is stored in;
+    the loop body computes in FP64 either way and is timed without the store.
     Returns systems/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as om
+    threads = threads or host_cores()
     if not om.Reference.available():
         return {"value": None, "unit": "systems/s", "cores": threads, "kind": "reference",
                 "sample": "oracle/_ref/libhamsref.so not built"}
@@ -165,22 +214,28 @@ def cpu_baseline(prob, seconds=20.0, threads=16):
                         ctypes.c_int(200), ctypes.c_int(threads), dp(csum))
         return time.perf_counter() - t0
 
-    t = run(4000)
-    npairs = int(min(5e7, max(4000, 4000 * 0.7 * seconds / max(t, 1e-6))))
+    t = run(20000)
+    npairs = int(min(1e8, max(20000, 20000 * 0.75 * seconds / max(t, 1e-6))))
     t = run(npairs)
     pair_rate = npairs / t
-    m = min(n, 3000)
+    m = min(n, 6144)
     a = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
     b = rng.normal(size=(m, prob.m + prob.nbeta)) + 1j * rng.normal(size=(m, prob.m + prob.nbeta))
+    np.linalg.solve(a[:256, :256], b[:256])
     t0 = time.perf_counter()
     np.linalg.solve(a, b)
-    t_lu = (time.perf_counter() - t0) * (n / m) ** 3
+    t_small = time.perf_counter() - t0
+    lu_rate = (8.0 * m ** 3 / 3.0 + 8.0 * m * m * (prob.m + prob.nbeta)) / t_small
+    t_lu = (8.0 * n ** 3 / 3.0 + 8.0 * n * n * (prob.m + prob.nbeta)) / lu_rate
     t_sys = nt * n / pair_rate + t_lu
     return {"value": 1.0 / t_sys, "unit": "systems/s", "cores": threads, "kind": "reference",
-            "sample": (f"{npairs} random hull panel pairs of the workload at omega={w:.3f} through the reference's "
-                       f"CALGREEN loop body (oracle/_ref, OpenMP, {threads} threads) in {t:.1f} s = {pair_rate:.3g} "
-                       f"pairs/s, x {nt * n} pairs per system; + LAPACK zgesv (numpy) at N={m} scaled by "
-                       f"(N/{m})^3 = {t_lu:.2f} s; {t_sys:.1f} s per system")}
+            "sample": (f"{npairs} random panel pairs of the workload at omega={w:.3f} through the reference's "
+                       f"CALGREEN loop body (oracle/_ref, OpenMP, {threads} threads = the physical cores this "
+                       f"process may use) in {t:.1f} s = {pair_rate:.3g} pairs/s, x {nt * n} pairs per system "
+                       f"= {nt * n / pair_rate:.1f} s; LU + solves at the rate of LAPACK zgesv (numpy) measured "
+                       f"at N={m}: {lu_rate / 1e9:.1f} GFLOP/s, for 8N^3/3 + 8N^2(M+NB) at N={n}: {t_lu:.1f} s; "
+                       f"{t_sys:.1f} s per system. GRN_KIND sets only the table's storage precision; the loop "
+                       f"body is FP64 either way")}
 
 
 def bench_batch(args):
@@ -275,11 +330,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--path", choices=("hmatrix", "gmres", "fp32ir"), default="hmatrix",
                     help="the mixed-precision solve that `value` measures")
-    ap.add_argument("--aca-eps", type=float, default=2e-5, help="cross-approximation tolerance of A")
-    ap.add_argument("--aca-eps-s", type=float, default=1e-6, help="cross-approximation tolerance of S")
+    ap.add_argument("--aca-eps", type=float, default=1e-6, help="cross-approximation tolerance of A")
+    ap.add_argument("--aca-eps-s", type=float, default=1e-7, help="cross-approximation tolerance of S")
     ap.add_argument("--aca-kmax", type=int, default=50, help="largest rank of a cross-approximated block")
-    ap.add_argument("--surrogate-tol", type=float, default=1.5e-3,
-                    help="cell error bound up to which the learned surrogate replaces the plain series (0: off)")
+    ap.add_argument("--surrogate-tol", type=float, default=None,
+                    help="cell error bound up to which the learned surrogate replaces the plain series (0: off; "
+                         "default: off for C5, whose near-trapped-mode frequencies amplify any change of the "
+                         "diagonal blocks' entries above 1e-4 in A/B/F; 1.5e-3 for C4)")
     ap.add_argument("--surrogate-samples", type=int, default=16_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--gmres-tol", type=float, default=1e-6)
@@ -333,6 +390,8 @@ def main():
     # learned surrogate of the evanescent sums: trained on the GPU before the
     # timed region (finite depth only)
     sur = None
+    if args.surrogate_tol is None:
+        args.surrogate_tol = 1.5e-3 if args.workload == "c4" else 0.0
     if args.surrogate_tol > 0 and prob.h > 0:
         t0 = time.perf_counter()
         st = ctx.surrogate_train(nsamples=args.surrogate_samples, epochs=args.surrogate_epochs, batch=4096,
@@ -358,7 +417,9 @@ def main():
         hm_setup_s = time.perf_counter() - t0
         print(f"# hmatrix set-up (diagonal-block classification, PIDCOCK term cache) {hm_setup_s:.2f} s",
               file=sys.stderr, flush=True)
-    dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup)
+    ranks_mx = []
+    dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup,
+                                     ranks=ranks_mx if args.path == "hmatrix" else None)
     gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
     field = None
     if args.field_points > 0:
@@ -378,6 +439,19 @@ def main():
                  "columns": prob.m + prob.nbeta}
         print(f"# field points: {field}", file=sys.stderr, flush=True)
     hist_mx = ctx.pair_histogram().ravel()     # pairs assembled densely on the measured path
+    aca_alone = None
+    if args.path == "hmatrix" and args.steps > 0:
+        # the cross approximation's entry kernels without the diagonal blocks
+        # beside them (on the context's stream after them), for the timed
+        # region's first two frequencies once more, outside it: their launch
+        # durations are their own, not shared with the other stream's kernels
+        ctx.set_hmatrix_overlap(False)
+        ranks_alone = []
+        _, _, st_alone = time_path(ctx, prob, prec, min(2, args.steps), args.warmup, ranks=ranks_alone)
+        ctx.set_hmatrix_overlap(True)
+        aca_alone = {"systems": min(2, args.steps), "stages_ms": {k: v for k, v in st_alone.items() if v[1]},
+                     "ranks": ranks_alone}
+        print(f"# cross approximation alone: {aca_alone}", file=sys.stderr, flush=True)
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
                     setup_s=hm_setup_s)
@@ -398,7 +472,11 @@ def main():
         dt_pl, res_pl, st_pl = time_path(ctx, prob, hm_bem.HM_FP64, args.steps, args.warmup)
         print(f"# fp64: {args.steps} systems in {dt_pl:.3f} s; stages {st_pl}", file=sys.stderr, flush=True)
         nw, ew, ew3, inbar, cnt = {}, {}, {}, {}, {}
-        for a, b in zip(res_mx, res_pl):
+        per_freq = []
+        for s_, (a, b) in enumerate(zip(res_mx, res_pl)):
+            w = float(prob.omegas[(args.warmup + s_) % len(prob.omegas)])
+            per_freq.append([round(w, 6)] + [float(np.max(np.abs(a[k] - b[k])) / np.max(np.abs(b[k])))
+                                              for k in ("amas", "bdmp", "exfc")])
             for k in ("amas", "bdmp", "exfc"):
                 mx = np.max(np.abs(b[k]))
                 d = np.abs(a[k] - b[k])
@@ -414,54 +492,95 @@ def main():
                 cnt[k] = cnt.get(k, 0) + d.size
         plain = {"value": args.steps / dt_pl, "ms_per_step": 1e3 * dt_pl / args.steps,
                  "speedup_mixed_vs_plain": (args.steps / dt_mx) / (args.steps / dt_pl),
-                 "rel_err_mixed_vs_fp64_normwise": nw, "rel_err_mixed_vs_fp64_entrywise": ew,
+                 "rel_err_mixed_vs_fp64_normwise": nw,
+                 "rel_err_per_frequency": {"columns": ["omega", "amas", "bdmp", "exfc"], "rows": per_freq},
+                 "rel_err_mixed_vs_fp64_entrywise": ew,
                  "rel_err_mixed_vs_fp64_entries_above_1e-3_max": ew3,
                  "fraction_of_entries_within_1e-4": {k: inbar[k] / cnt[k] for k in cnt},
                  "max_rel_err_mixed_vs_fp64": max(nw.values()),
                  "stages_ms": {k: v[0] for k, v in st_pl.items()}}
 
-    # dominant kernel of the measured path: the assembly launch (per regime for
-    # finite depth) with most time
+    # dominant kernel of the measured path: the kernel with the most time over
+    # the timed region, among the diagonal-block (or dense) regime launches and
+    # the cross approximation's entry launches
     fpp, deep = flop_model(args.workload)
     hist = hist_mx
     st_rf = st_mx
-    asm = {k: v for k, v in st_rf.items() if k in REGIME_STAGE and v[1] > 0}
-    if asm:
-        dom = max(asm, key=lambda k: asm[k][0])
-        dom_ms = asm[dom][0] / asm[dom][1]
-        r = REGIME_STAGE[dom]
-        keys = range(128 * r, 128 * (r + 1))
-        flops = sum(float(hist[k]) * fpp.get(k, fpp.get(128 * r, 0.0)) for k in keys)
-        units = int(sum(hist[k] for k in keys))
+    # per system: the regime launches' own times (one launch each) and the
+    # cross approximation's entry launches measured alone
+    cand = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in st_rf.items()
+            if k in REGIME_STAGE and v[1] > 0}
+    if aca_alone is not None and "aca_entries" in aca_alone["stages_ms"]:
+        ms, cnt = aca_alone["stages_ms"]["aca_entries"]
+        cand["aca_entries"] = (ms / aca_alone["systems"], cnt / aca_alone["systems"])
+    if cand:
+        dom = max(cand, key=lambda k: cand[k][0])
+        dom_ms = cand[dom][0] / cand[dom][1]
+        if dom == "aca_entries":
+            # entries evaluated: one row and one column of m entries per rank step
+            # of each of the P processes (A and S of every ordered pair of bodies)
+            m = prob.n // prob.nbody
+            procs = 2 * prob.nbody * (prob.nbody - 1)
+            entries = sum(2.0 * m * procs * r for r in aca_alone["ranks"]) / aca_alone["systems"]
+            units = entries / cand[dom][1]
+            per_entry = offdiag_flops_per_entry(prob, fpp)
+            flops = units * per_entry
+            abytes = 16.0 * units
+            what = (f"cross-approximation entry kernels (residual rows and columns of all open A and S "
+                    f"processes), measured alone after the timed region: {cand[dom][1]:.0f} launches and "
+                    f"{entries:.4g} entries per system (2 m P x the mean rank), {per_entry:.0f} counted flop per entry (the "
+                    f"reference's FINGREEN3D + SGLINTBD work for the off-diagonal pairs of this geometry); the "
+                    f"kernels sum the evanescent modes directly to a bounded tail instead of the epsilon tables")
+        else:
+            r = REGIME_STAGE[dom]
+            keys = range(128 * r, 128 * (r + 1))
+            flops = sum(float(hist[k]) * fpp.get(k, fpp.get(128 * r, 0.0)) for k in keys)
+            units = int(sum(hist[k] for k in keys))
+            abytes = 36.0 * units
+            what = ("Green-function assembly over the classified pairs of one regime (on the H-matrix path the "
+                    "diagonal blocks, each stored pair assembled with its mirror); algorithmic flops = the "
+                    "reference's per-pair work counted by the oracle for every pair the launch assembles; "
+                    "algorithmic bytes = 32 B of A and S + 4 B pair index per pair")
     else:
         dom, dom_ms = "asm_deep", st_rf["assembly"][0] / max(st_rf["assembly"][1], 1)
         units = (prob.n + prob.ni) * prob.n
         flops = units * fpp[0]
+        abytes = 36.0 * units
+        what = "infinite-depth pair grid"
     achieved = flops / (dom_ms * 1e-3) / 1e12
     traffic = MEASURED_TRAFFIC.get((args.workload, args.path, dom))
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                "traffic_gbs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
-                "avg_ms": dom_ms, "pairs_per_launch": units, "algorithmic_flops_per_launch": flops,
-                "algorithmic_bytes_per_launch": 36 * units,
-                "note": ("FP64 VALU-bound Green-function assembly over the classified pairs of one regime; "
-                         "peak = MI355X FP64 vector peak (= FP64 matrix peak); algorithmic flops = the reference's "
-                         "per-pair work counted by the oracle (tests/golden/flop_model.json) for every pair the "
-                         "launch assembles; on the H-matrix path the PIDCOCK kernel assembles each pair with its "
-                         "mirror and reads the frequency-independent halves of its terms from a per-geometry "
-                         "cache, executing 41 % of the FP64 instructions of a pair-by-pair evaluation; traffic = "
-                         "measured HBM bytes per launch, algorithmic bytes = 32 B of A and S + 4 B pair index "
-                         "per pair")}
+    roofline = {"bound": "valu", "kernel": KERNEL_OF_STAGE.get(dom, dom), "stage": dom, "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                "traffic": traffic, "traffic_gbs": traffic / (dom_ms * 1e-3) / 1e9 if traffic else None,
+                "avg_ms": dom_ms, "launches_per_system": cand[dom][1] if cand else 1, "units_per_launch": units,
+                "kernels_ms_per_system": {k: round(v[0], 2) for k, v in cand.items()},
+                "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": abytes,
+                "note": ("FP64 vector (VALU) bound; peak = MI355X FP64 vector peak, equal to its FP64 matrix peak; "
+                         "counted flops: each addition, multiplication, division and square root 1, each "
+                         "transcendental 20 (oracle operation counter, tests/golden/flop_model.json); avg_ms = "
+                         "HIP events around each launch on its own stream. " + what)}
 
-    line = {"metric": METRIC, "value": args.steps / dt_mx, "unit": "systems/s", "n_gpus": 1,
+    acc_ok = None
+    if plain is not None:
+        acc_ok = bool(plain["max_rel_err_mixed_vs_fp64"] <= ACCURACY_BAR)
+    value = args.steps / dt_mx if acc_ok is not False else None
+    line = {"metric": METRIC, "value": value, "unit": "systems/s", "n_gpus": 1,
+            "accuracy_ok": acc_ok, "accuracy_bar": ACCURACY_BAR,
+            "accuracy_note": ("largest normwise deviation (max |mixed - FP64| / max |FP64| per output) over every "
+                              "timed frequency; the line's value is null when it exceeds the bar"),
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt_mx / args.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32+bf16",
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64+f32+bf16" if sur is not None else "f64+f32",
             "data": "synthetic (deterministic meshes of the named workload)",
             "config": {"workload": prob.name, "panels": prob.n, "waterplane_panels": prob.ni,
                        "modes": prob.m, "headings": prob.nbeta, "frequencies": len(prob.omegas),
                        "path": PATH_DESC[args.path], "parallelism": "1 GPU"},
             "stages_ms": {k: v[0] for k, v in st_mx.items()},
             "plain_fp64": plain, "roofline": roofline}
+    if value is None:
+        line["value_without_accuracy"] = args.steps / dt_mx
+    if aca_alone is not None:
+        line["cross_approximation_alone"] = aca_alone
     if gm is not None:
         line["gmres"] = gm
     if hmat is not None:

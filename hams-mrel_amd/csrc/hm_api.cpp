@@ -51,10 +51,10 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-constexpr int kStages = 13;
+constexpr int kStages = 14;
 enum Stage { kClassify = 0, kAssembly = 1, kRhs = 2, kIrr = 3, kLu = 4, kSolve = 5, kForces = 6,
              kAsmRegime0 = 7 /* + regime: Linton, Pidcock, eps-accelerated, eigenfunction */,
-             kAsmSurrogate = 11, kAsmCross = 12 };
+             kAsmSurrogate = 11, kAsmCross = 12, kAcaEntries = 13 };
 
 }  // namespace
 
@@ -152,6 +152,8 @@ struct hm_ctx {
   // main stream assembles the diagonal blocks and factors (and inverts) the
   // block preconditioner
   hipStream_t side = nullptr;
+  hipStream_t aca_stream = nullptr;   // the stream the current cross approximation runs on
+  bool hm_overlap = true;             // cross approximation beside the diagonal blocks
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool side_pending = false;     // work on `side` the main stream has not joined yet
   bool prec_early = false;       // the preconditioner of the current assembly and settings is built
@@ -643,21 +645,33 @@ static int assemble_hmatrix(hm_ctx* c) {
   c->join_side();
   c->prec_early = false;
   c->stage_begin(kAssembly);
-  // the off-diagonal blocks by cross approximation on the side stream (from the
-  // state the main stream has reached), the diagonal blocks and the block
-  // preconditioner on the main stream: everything on the main stream is
-  // enqueued before the cross approximation's host loop starts
-  HIPCHK(c, hipEventRecord(c->ev_fork, c->stream), "hmatrix");
-  HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "hmatrix");
-  c->side_pending = true;
+  // overlap (the default): the off-diagonal blocks by cross approximation on the
+  // side stream (from the state the main stream has reached), the diagonal
+  // blocks and the block preconditioner on the main stream: everything on the
+  // main stream is enqueued before the cross approximation's host loop starts.
+  // Without overlap the cross approximation follows on the main stream.
+  hipStream_t acs = c->hm_overlap ? c->side : c->stream;
+  if (c->hm_overlap) {
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream), "hmatrix");
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0), "hmatrix");
+    c->side_pending = true;
+  }
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
   if ((rc = precond_early(c))) return rc;
   // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
   c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
-  c->stage_begin(kAsmCross, c->side);
-  HIPCHK(c, hm::aca_build(c->side, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps), "cross approximation");
-  c->stage_end(kAsmCross, c->side);
+  c->stage_begin(kAsmCross, acs);
+  // each entry-kernel launch of the cross approximation timed on its stream
+  c->aca_stream = acs;
+  hm::AcaTimer tm{c, [](void* u, int b) {
+                    hm_ctx* cc = static_cast<hm_ctx*>(u);
+                    if (b) cc->stage_begin(kAcaEntries, cc->aca_stream);
+                    else cc->stage_end(kAcaEntries, cc->aca_stream);
+                  }};
+  HIPCHK(c, hm::aca_build(acs, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps, c->timing ? &tm : nullptr),
+         "cross approximation");
+  c->stage_end(kAsmCross, acs);
   c->join_side();
   if ((rc = c->stage_end(kAssembly))) return rc;
   const int P = hm::hmat_processes(nb);
@@ -1044,6 +1058,14 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
   c->gm_precond = precond;
   c->factored = c->solved = false;
   c->pinv_ready = false;
+  c->prec_early = false;
+  return HM_OK;
+}
+
+extern "C" int hm_set_hmatrix_overlap(hm_ctx* c, int enable) {
+  if (!c) return HM_ERR_ARG - 1;
+  c->join_side();
+  c->hm_overlap = enable != 0;
   c->prec_early = false;
   return HM_OK;
 }

@@ -444,7 +444,8 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
 }
 
 hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
-                     int* steps) {
+                     int* steps, const AcaTimer* tm) {
+  auto mark = [tm](int b) { if (tm) tm->mark(tm->user, b); };
   const int P = hmat_processes(h.nb, h.nmat);
   const size_t uv = 16 * (size_t)P * h.m * h.kmax;
   hipError_t e;
@@ -464,16 +465,20 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   const int max_steps = h.kmax + h.m;
   for (; st < max_steps && open > 0; ++st) {
     const dim3 eval((h.m + 255) / 256, P);
+    mark(1);
     if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);
     else hipLaunchKernelGGL((k_aca_eval_row<16, true, false>), eval, dim3(256), 0, s, g, f, h);
+    mark(0);
     // narrow: every pair of bodies is more than 0.05 h apart (bounding circles of
     // the panel centroids), so no entry is LINTON or PIDCOCK and the fix-up
     // kernels would only test and return; skipping them keeps the stream from
     // waiting for CUs behind the other stream's launches for nothing
     if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
+    mark(1);
     if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false, true>), eval, dim3(256), 0, s, g, f, h);
     else hipLaunchKernelGGL((k_aca_eval_col<16, true, false>), eval, dim3(256), 0, s, g, f, h);
+    mark(0);
     if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
     hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -56,11 +56,18 @@ size_t hmat_bytes(int nb, int m, int kmax, int nmat = 2);   // device memory for
 HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s, int nmat = 2,
                    const double* dense = nullptr, long long ldd = 0);
 
+// optional marks around each entry-kernel launch (row and column evaluation):
+// mark(user, 1) before the launch, mark(user, 0) after it, on the same stream
+struct AcaTimer {
+  void* user;
+  void (*mark)(void* user, int begin);
+};
+
 // cross approximation of all off-diagonal blocks of A (mat 0) and S (mat 1), or
 // of the assembled matrix `dense` (nmat = 1); returns the number of
 // unconverged processes in *unconverged
 hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
-                     int* steps);
+                     int* steps, const AcaTimer* tm = nullptr);
 
 // the first kuse columns of the U and V factors of matrix `mat` (0: A, 1: S),
 // packed with stride m kuse per block (same block order) into Uc, Vc

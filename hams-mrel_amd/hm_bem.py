@@ -17,7 +17,8 @@ HM_FP32_IR = 1
 HM_GMRES = 2
 SUR_PARAMS = 12995          # network parameters (see include/hm_bem.h)
 STAGES = ("classify", "assembly", "rhs_gemm", "irr_normal_eq", "lu", "solve", "forces",
-          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen", "asm_surrogate", "asm_cross")
+          "asm_linton", "asm_pidcock", "asm_eigene", "asm_eigen", "asm_surrogate", "asm_cross",
+          "aca_entries")
 
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -32,7 +33,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",
-           "hm_surrogate_grad", "hm_set_hmatrix", "hm_get_hmatrix_stats",
+           "hm_surrogate_grad", "hm_set_hmatrix", "hm_set_hmatrix_overlap", "hm_get_hmatrix_stats",
            "hm_hmatrix_apply", "hm_set_frequency_limit", "hm_batch_create", "hm_batch_destroy",
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
@@ -256,6 +257,10 @@ class Context:
         """eps: ACA tolerance of A; eps_s: of S (default eps)"""
         self._chk(self.L.hm_set_hmatrix(self.ptr, _i(1 if enable else 0), _d(eps), _d(eps if eps_s is None else eps_s),
                                         _i(kmax)), "hm_set_hmatrix")
+
+    def set_hmatrix_overlap(self, enable=True):
+        """cross approximation beside the diagonal blocks (True) or after them"""
+        self._chk(self.L.hm_set_hmatrix_overlap(self.ptr, _i(1 if enable else 0)), "hm_set_hmatrix_overlap")
 
     def set_pair_symmetry(self, enable=True):
         """H-matrix diagonal blocks: PIDCOCK pairs assembled with their mirrors"""

@@ -113,6 +113,11 @@ int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, dou
  * kmax before eps, ACA steps, mean and largest rank, and stored entries per
  * matrix over n^2, for the last assembly. */
 int hm_set_hmatrix(hm_ctx *ctx, int enable, double eps, double eps_s, int kmax);
+/* No reference counterpart (a scheduling choice of this library): enable = 1
+ * (the default) runs the cross approximation on a second stream of the context
+ * beside the diagonal blocks and the block preconditioner; 0 runs it after them
+ * on the context's stream, so that its kernels' timings are their own. */
+int hm_set_hmatrix_overlap(hm_ctx *ctx, int enable);
 int hm_get_hmatrix_stats(hm_ctx *ctx, int *unconverged, int *steps, double *mean_rank, int *max_rank,
                          double *stored_fraction);
 /* The diagonal blocks of the H-matrix form assemble each PIDCOCK pair (i, j),
@@ -267,7 +272,8 @@ int hm_batch_info(hm_batch *batch, int *members, int *nmax, int *series_variant,
  * 3 IRR normal equations, 4 LU, 5 triangular solves + refinement, 6 forces,
  * 7..10 the assembly kernel of each finite-depth regime (Linton, Pidcock,
  * epsilon-accelerated, eigenfunction), 11 the surrogate assembly kernel, 12
- * the cross approximation of the H-matrix form.
+ * the cross approximation of the H-matrix form (its stream, start to end), 13
+ * its entry kernels (one count per launch, on that stream).
  * hm_set_timing resets the totals. */
 int hm_set_timing(hm_ctx *ctx, int enable);
 int hm_get_timing(hm_ctx *ctx, double *ms, int *counts, int nstage);

@@ -253,3 +253,28 @@ def test_diagonal_blocks_match_dense(pair):
                     assert err < bar, (sym, w, b, mat, err)
         print(f"h={p.h} mirrored={sym}: largest block product error {worst:.2e}", flush=True)
     hm.set_pair_symmetry(True)
+
+
+def test_overlap_switch_gives_the_same_solution(pair):
+    """the cross approximation on the second stream beside the diagonal blocks
+    (the default) or after them on the context's stream: the same kernels on
+    the same inputs, so the same A/B/F bit for bit; the stage timer counts one
+    entry-kernel record per launch either way"""
+    p, dense, hm = pair
+    hm.set_hmatrix(True, eps=1e-6, eps_s=1e-7, kmax=50)
+    hm.set_pair_symmetry(True)
+    hm.set_gmres(restart=50, tol=1e-8, maxiter=500, precond=-1)
+    w = p.omegas[1]
+    got = {}
+    for ov in (True, False):
+        hm.set_hmatrix_overlap(ov)
+        hm.set_timing(True)
+        got[ov] = hm.solve_frequency(w, hm_bem.HM_GMRES)
+        st = hm.timing()
+        hm.set_timing(False)
+        steps = hm.hmatrix_stats()["steps"]
+        assert st["aca_entries"][1] == 2 * steps, (ov, st["aca_entries"], steps)
+        assert st["asm_cross"][1] == 1
+    hm.set_hmatrix_overlap(True)
+    for k in KEYS:
+        assert np.array_equal(got[True][k], got[False][k]), k
