@@ -336,7 +336,7 @@ def main():
     ap.add_argument("--surrogate-tol", type=float, default=None,
                     help="cell error bound up to which the learned surrogate replaces the plain series (0: off; "
                          "default: off for C5, whose near-trapped-mode frequencies amplify any change of the "
-                         "diagonal blocks' entries above 1e-4 in A/B/F; 1.5e-3 for C4)")
+                         "diagonal blocks' entries above 1e-4 in A/B/F; 1e-3 for C4)")
     ap.add_argument("--surrogate-samples", type=int, default=16_000_000)
     ap.add_argument("--surrogate-epochs", type=int, default=60)
     ap.add_argument("--gmres-tol", type=float, default=1e-6)
@@ -391,7 +391,7 @@ def main():
     # timed region (finite depth only)
     sur = None
     if args.surrogate_tol is None:
-        args.surrogate_tol = 1.5e-3 if args.workload == "c4" else 0.0
+        args.surrogate_tol = 1e-3 if args.workload == "c4" else 0.0
     if args.surrogate_tol > 0 and prob.h > 0:
         t0 = time.perf_counter()
         st = ctx.surrogate_train(nsamples=args.surrogate_samples, epochs=args.surrogate_epochs, batch=4096,

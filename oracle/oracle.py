@@ -305,3 +305,63 @@ class Reference:
         r = _d()
         self.lib.ref_limes(_i(len(s) - 1), _ptr(s), ctypes.byref(r))
         return r.value
+
+    # ---- the multi-body chain (ref_shim_multi.f90) ---------------------------
+    def multi_set(self, prob, nthreads=1):
+        """the reference's *_MULTI_COMB module arrays from a meshes.Problem (hull panels)"""
+        keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                np.ascontiguousarray(prob.ncn, dtype=np.int32),
+                np.ascontiguousarray(prob.ncon, dtype=np.int32),
+                np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
+                np.ascontiguousarray(prob.ds, dtype=np.float64),
+                np.ascontiguousarray(prob.pnsz, dtype=np.float64),
+                np.ascontiguousarray(prob.dxyz_p, dtype=np.float64)]
+        self.lib.ref_multi_set(_i(prob.nbody), _ptr(keep[0], _ip), _i(prob.n), _i(len(prob.xyz)), _ptr(keep[1]),
+                               _ptr(keep[2], _ip), _ptr(keep[3], _ip), _ptr(keep[4]), _ptr(keep[5]),
+                               _ptr(keep[6]), _ptr(keep[7]), _d(prob.h), _i(nthreads))
+        self._multi = (prob.n, prob.nbody)
+
+    def multi_freq(self, v, wk, w1, wvn):
+        wvn = np.ascontiguousarray(wvn, dtype=np.float64)
+        assert len(wvn) == 200
+        self.lib.ref_multi_freq(_d(v), _d(wk), _d(w1), _ptr(wvn))
+
+    def multi_calgreen(self):
+        """CALGREEN_MULTI, then its tables RKBN (n, n, 4) and CGRN (n, n, 4)"""
+        n, _ = self._multi
+        self.lib.ref_multi_calgreen()
+        rk = np.zeros((n, n, 4), order="F")
+        cg = np.zeros((n, n, 4), dtype=np.complex128, order="F")
+        self.lib.ref_multi_tables(_i(n), _ptr(rk), _ptr(cg))
+        return rk, cg
+
+    def multi_left(self):
+        n, _ = self._multi
+        a = np.zeros((n, n), dtype=np.complex128, order="F")
+        self.lib.ref_multi_left(_i(n), _ptr(a))
+        return a
+
+    def multi_rbc(self):
+        n, nb = self._multi
+        b = np.zeros((n, 6 * nb), dtype=np.complex128, order="F")
+        self.lib.ref_multi_rbc(_i(n), _i(nb), _ptr(b))
+        return b
+
+    def multi_dbc(self, beta):
+        n, _ = self._multi
+        b = np.zeros(n, dtype=np.complex128)
+        self.lib.ref_multi_dbc(_i(n), _d(beta), _ptr(b))
+        return b
+
+    def multi_forces(self, mxpot, beta):
+        """RFORCE_MULTI / EFORCE_MULTI for potentials mxpot (n, 6 nbody + 1: the
+        radiation columns and the diffraction column of heading beta)"""
+        n, nb = self._multi
+        mx = np.asfortranarray(mxpot, dtype=np.complex128)
+        assert mx.shape == (n, 6 * nb + 1)
+        amas = np.zeros((6 * nb, 6 * nb), order="F")
+        bdmp = np.zeros((6 * nb, 6 * nb), order="F")
+        exfc = np.zeros(6 * nb, dtype=np.complex128)
+        self.lib.ref_multi_forces(_i(n), _i(nb), _ptr(mx), _d(beta), _ptr(amas), _ptr(bdmp), _ptr(exfc))
+        return amas, bdmp, exfc

@@ -80,3 +80,42 @@ def test_generalized_mode_feeds_the_solve(tmp_path):
     f = r["exfc"]
     assert np.max(np.abs(f[3] - f[2])) <= 1e-12 * np.max(np.abs(f))
     assert np.max(np.abs(r["amas"][:, 3] - r["amas"][:, 4])) > 1e-3 * np.max(np.abs(r["amas"]))
+
+
+def test_example5_irr_input_directory_to_hams_files(tmp_path):
+    """the reference's single-body example with irregular-frequency removal
+    (examples/example5_single_body: 234 hull + 383 interior waterplane panels,
+    h = 70 m, 10 frequencies, 36 headings) run as its driver runs it
+    (formats.run_input_directory: read, solve on the GPU record by record,
+    write), against its Hams_format files: identical frequency columns and
+    values within 1e-4 of the frequency's largest added-mass, damping or
+    excitation value. The files were written by the reference at GRN_KIND = 4
+    (its Green-function tables stored in FP32) and the IRR normal equations
+    C = A^T A square the system's conditioning: the FP64 oracle itself sits
+    2.2e-6 .. 4.4e-5 from them at these ten frequencies
+    (tests/test_oracle.py::test_example5_irr_expected_outputs)"""
+    inp = os.path.join(GOLDEN, "example5_input")
+    recs, amas, bdmp, exfc = formats.run_input_directory(inp, str(tmp_path))
+    assert len(recs) == 10 and amas.shape == (10, 6, 6) and exfc.shape == (10, 6, 36)
+    for stem in ("OAMASS", "ODAMPING", "OEXFOR"):
+        per = 1 if stem != "OEXFOR" else 36
+        want_all, got_all = [], []
+        for md in range(6):
+            name = "%s%d.txt" % (stem, md + 1)
+            want = [ln.split() for ln in open(os.path.join(GOLDEN, "example5_expected", name)).read().split("\n")
+                    if ln.strip()]
+            got = [ln.split() for ln in open(os.path.join(str(tmp_path), "Hams_format", name)).read().split("\n")
+                   if ln.strip()]
+            assert len(got) == len(want) == 10 * per, name
+            assert [r[:2] for r in got] == [r[:2] for r in want], name
+            want_all.append(np.array([[float(t) for t in r[2:]] for r in want]))
+            got_all.append(np.array([[float(t) for t in r[2:]] for r in got]))
+        wv, gv = np.stack(want_all), np.stack(got_all)
+        errs = []
+        for f in range(10):
+            blk = slice(per * f, per * f + per)
+            scale = np.max(np.abs(wv[:, blk]))
+            errs.append(float(np.max(np.abs(gv[:, blk] - wv[:, blk])) / scale))
+        print(stem, " ".join("%.1e" % e for e in errs), flush=True)
+        assert max(errs) <= 1e-4, (stem, errs)
+    assert os.path.exists(os.path.join(str(tmp_path), "Wamit_format", "AmssDamp.1"))

@@ -243,6 +243,52 @@ def test_config1_expected_outputs(oracle):
     assert relerr(bdmp, bdmp.T) < 1e-2
 
 
+@pytest.mark.parametrize("h", [31.7, -1.0])
+def test_multibody_chain_against_the_reference(oracle, h):
+    """The oracle's assembled system and force reductions against the
+    reference's own multi-body code (oracle/_ref, ref_shim_multi.f90): its
+    CALGREEN_MULTI table fill (CalGreenFuncMulti.f90:112-183, with
+    SGLINTBD_MULTI and FINGREEN3D/INFGREEN3D), its element integration
+    BODINT_MULTI_LEFT / RBC_MULTI_RIGHT / DBC_MULTI_RIGHT with DINP
+    (BodyIntgrMulti.f90:48-173, PatclVelct.f90:83) in the loops of
+    ASSB_LEFT/RBC/DBC_MULTI, and its RFORCE_MULTI / EFORCE_MULTI
+    (PotentWavForceMulti.f90:45-226) fed with the oracle's potentials. Two
+    cylinders 12 m apart: PIDCOCK, EIGENE and EIGEN pairs at h = 31.7 m, and
+    infinite depth. The reference stores its tables at GRN_KIND = 4 (FP32,
+    WavDynMods.f90:47), so A and the right-hand sides agree to that storage
+    (measured ~1e-8 of the largest entry; bar 2e-7); the force reductions read
+    the potentials in FP64 and agree to rounding (bar 1e-13)."""
+    import meshes
+    from oracle import Reference
+    if not Reference.available():
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    R = Reference()
+    bs = [meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(12.0 * i, 0.0)) for i in range(2)]
+    p = meshes.reference_parity_constants(meshes.Problem(bs, h=h, omegas=[0.9], beta=np.array([0.0, np.pi / 3])))
+    w = 0.9
+    r = oracle.solve_frequency(p, w, want_matrices=True)
+    assert r["info"] == 0
+    R.multi_set(p)
+    v = w * w / p.g
+    wvn = R.dispersion(w, h) if h > 0 else np.zeros(200)
+    R.multi_freq(v, wvn[0] if h > 0 else v, w, wvn)
+    rk, cg = R.multi_calgreen()
+    assert np.all(np.isfinite(cg)) and np.max(np.abs(cg)) > 0
+    n, m = p.n, p.m
+    a = R.multi_left()
+    assert relerr(r["amat"][:n], a) <= 2e-7
+    br = R.multi_rbc()
+    assert relerr(r["rhs"][:n, :m], br) <= 2e-7
+    for ib, beta in enumerate(p.beta):
+        bd = R.multi_dbc(beta)
+        assert relerr(r["rhs"][:n, m + ib], bd) <= 2e-7
+        mx = np.column_stack([r["mxpot"][:, :m], r["mxpot"][:, m + ib]])
+        amas, bdmp, exfc = R.multi_forces(mx, beta)
+        assert relerr(r["amas"], amas) <= 1e-13
+        assert relerr(r["bdmp"], bdmp) <= 1e-13
+        assert relerr(r["exfc"][:, ib], exfc) <= 1e-13
+
+
 def _example5_expected(kind, k):
     return np.loadtxt(os.path.join(GOLDEN, "example5_expected", f"{kind}{k}.txt"))
 
