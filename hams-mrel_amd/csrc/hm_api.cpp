@@ -1773,6 +1773,17 @@ extern "C" int hm_get_system(hm_ctx* c, double* amat, double* smat, double* rhs)
   return HM_OK;
 }
 
+extern "C" int hm_get_normal_equations(hm_ctx* c, double* cmat, double* drhs) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->assembled || c->factored) return c->fail(HM_ERR_STATE, "hm_get_normal_equations needs an assembled, unfactored system");
+  if (c->ni == 0) return c->fail(HM_ERR_STATE, "no waterplane panels: the system has no normal equations");
+  const size_t cplx = 2 * sizeof(double), n = c->n;
+  if (cmat) HIPCHK(c, hipMemcpyAsync(cmat, c->d_C.p, cplx * n * n, hipMemcpyDeviceToHost, c->stream), "get normal equations");
+  if (drhs) HIPCHK(c, hipMemcpyAsync(drhs, c->d_D.p, cplx * n * c->ncol(), hipMemcpyDeviceToHost, c->stream), "get normal equations");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "get normal equations");
+  return HM_OK;
+}
+
 extern "C" int hm_get_potentials(hm_ctx* c, double* mxpot) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->solved) return c->fail(HM_ERR_STATE, "hm_get_potentials before hm_solve");

@@ -2,15 +2,20 @@
 ! include/hm_bem.h and wrappers that keep the reference subroutines' argument
 ! lists, so a HAMS-MREL driver loop can call the GPU path in place of
 !   FINGREEN3D / INFGREEN3D            (FinGreen3D.f90:81, InfGreen_Appr.f90:104)
-!   CALGREEN_MULTI + ASSB_LEFT_MULTI   (CalGreenFuncMulti.f90:112, AssbMatxMulti.f90:53)
-!   ASSB_RBC_MULTI / ASSB_DBC_MULTI    (AssbMatxMulti.f90:124, :187)
-!   RADIATION/DIFFRACTION_SOLVER_MULTI (AssbMatxMulti.f90:257, :323)
+!   SGLINTBD_QUAD / SGLINTBD_TRI       (SingularIntgr.f90:252, :44)
+!   CALGREEN(_MULTI)(_IRR)             (CalGreenFunc.f90:49/:124, CalGreenFuncMulti.f90:112/:189)
+!   ASSB_LEFT(_MULTI), ASSB_RBC(_MULTI), ASSB_DBC(_MULTI)   (AssbMatx.f90:53-187, AssbMatxMulti.f90:53-187)
+!   RADIATION/DIFFRACTION_SOLVER(_MULTI)                     (AssbMatx.f90:257/:318, AssbMatxMulti.f90:257/:323)
+!   ASSB_LEFT/RBC/DBC_IRR(_MULTI), RADIATION/DIFFRACTION_SOLVER_IRR(_MULTI)
+!                                      (AssbMatx_irr.f90:53-527, AssbMatx_irrMulti.f90:53-529)
 !   RFORCE_MULTI / EFORCE_MULTI        (PotentWavForceMulti.f90:138, :45)
 !   ZITER_SOLVE_MULTI, SETUP_BLOCK_PRECONDITIONER (IterativeSolvers.f90:574, :190)
 !   HMATRIX_BUILD / HMATVEC / HMATRIX_DESTROY      (HMatrix.f90:135, :760, :852)
 ! The reference passes geometry and wave state through module variables
-! (WavDynMods.f90); here the driver hands them over once with HM_SETUP and per
-! frequency with HM_SET_FREQUENCY. NSYS (geometric symmetry) must be 1.
+! (WavDynMods.f90); here the driver hands them over once with HM_SETUP, per
+! frequency with HM_SET_FREQUENCY, and per heading through this module's BETA
+! (radians; the driver's BETA = WVHD(II)*PI/180, HAMS_Prog.f90:696, sets it).
+! NSYS (geometric symmetry) must be 1.
 module hm_bem
    use iso_c_binding
    implicit none
@@ -101,6 +106,18 @@ module hm_bem
       integer(c_int) function hm_get_system(ctx, amat, smat, rhs) bind(C, name="hm_get_system")
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx, amat, smat, rhs
+      end function
+      integer(c_int) function hm_get_normal_equations(ctx, cmat, drhs) bind(C, name="hm_get_normal_equations")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, cmat, drhs
+      end function
+      integer(c_int) function hm_sglintbd_batch(ctx, n, ncn, verts, xf, sij, dij) bind(C, name="hm_sglintbd_batch")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         integer(c_int), intent(in) :: ncn(*)
+         real(c_double), intent(in) :: verts(*), xf(*)
+         real(c_double), intent(out) :: sij(*), dij(*)
       end function
       integer(c_int) function hm_get_potentials(ctx, mxpot) bind(C, name="hm_get_potentials")
          import :: c_int, c_ptr, c_double_complex
@@ -239,6 +256,12 @@ module hm_bem
    public :: HM_CONTEXT, HM_SETUP, HM_SET_FREQUENCY_F, HM_CHECK
    public :: FINGREEN3D, INFGREEN3D
    public :: ASSB_LEFT_MULTI, ASSB_RBC_MULTI, ASSB_DBC_MULTI, RADIATION_SOLVER_MULTI, DIFFRACTION_SOLVER_MULTI
+   public :: ASSB_LEFT, ASSB_RBC, ASSB_DBC, RADIATION_SOLVER, DIFFRACTION_SOLVER
+   public :: ASSB_LEFT_IRR, ASSB_RBC_IRR, ASSB_DBC_IRR, RADIATION_SOLVER_IRR, DIFFRACTION_SOLVER_IRR
+   public :: ASSB_LEFT_IRR_MULTI, ASSB_RBC_IRR_MULTI, ASSB_DBC_IRR_MULTI, RADIATION_SOLVER_IRR_MULTI, &
+             DIFFRACTION_SOLVER_IRR_MULTI
+   public :: CALGREEN, CALGREEN_MULTI, CALGREEN_IRR, CALGREEN_MULTI_IRR, SGLINTBD_QUAD, SGLINTBD_TRI
+   public :: BETA, hm_get_normal_equations, hm_sglintbd_batch
    public :: RFORCE_EFORCE_MULTI
    public :: hm_set_gmres, hm_set_hmatrix, hm_set_pair_symmetry, hm_set_frequency_limit, hm_hmatrix_build, hm_hmatrix_release, &
              hm_hmatrix_apply, hm_ziter_solve, hm_batch_create, hm_batch_destroy, hm_batch_set_constants, &
@@ -249,9 +272,15 @@ module hm_bem
    type(c_ptr), save :: ctx_ = c_null_ptr
    ! SETUP_BLOCK_PRECONDITIONER's blocks, used by the next ZITER_SOLVE_MULTI
    integer(c_int), allocatable, save :: pblocks_(:)
-   integer, save :: n_ = 0, nbody_ = 0, nbeta_ = 0
+   integer, save :: n_ = 0, ni_ = 0, nbody_ = 0, nbeta_ = 0
    integer, allocatable, save :: body_start_(:)
-   complex(c_double_complex), allocatable, target, save :: rhs_(:, :)
+   complex(c_double_complex), allocatable, target, save :: rhs_(:, :), drhs_(:, :)
+   ! host copies of the geometry for SGLINTBD_*: nodes (3, ntnd), connectivity
+   ! (4, n), panel count per panel, hull and waterplane centroids (3, n / ni)
+   real(8), allocatable, save :: xyz_(:, :), xyzp_(:, :), ixyzp_(:, :), beta_(:)
+   integer, allocatable, save :: ncn_(:), ncon_(:, :)
+   ! the current heading (radians) of ASSB_DBC*, as the reference's module BETA
+   real(8), save :: BETA = 0.d0
 
 contains
 
@@ -290,12 +319,36 @@ contains
       call HM_CHECK(hm_set_depth(c, H), 'hm_set_depth')
       call HM_CHECK(hm_set_headings(c, NBETA, BETA), 'hm_set_headings')
       n_ = size(NCN)
+      ni_ = NI
       nbody_ = NBODY
       if (allocated(body_start_)) deallocate (body_start_)
       allocate (body_start_(NBODY + 1))
       body_start_ = BODY_START
       nbeta_ = NBETA
+      if (allocated(xyz_)) deallocate (xyz_, xyzp_, ixyzp_, ncn_, ncon_, beta_)
+      allocate (xyz_(3, NTND), xyzp_(3, n_), ixyzp_(3, max(NI, 1)), ncn_(n_), ncon_(4, n_), beta_(max(NBETA, 1)))
+      xyz_ = transpose(XYZ(1:NTND, 1:3))
+      xyzp_ = transpose(XYZ_P(1:n_, 1:3))
+      ixyzp_ = 0.d0
+      if (NI > 0) ixyzp_ = transpose(IXYZ_P(1:NI, 1:3))
+      ncn_ = NCN
+      ncon_ = ncon_t
+      beta_ = 0.d0
+      if (NBETA > 0) beta_(1:NBETA) = BETA(1:NBETA)
    end subroutine
+
+   ! the heading of the reference's module BETA among those of HM_SETUP (1-based)
+   integer function heading_index()
+      integer :: k
+      do k = 1, nbeta_
+         if (abs(beta_(k) - BETA) <= 1.d-12 * max(1.d0, abs(BETA))) then
+            heading_index = k
+            return
+         end if
+      end do
+      heading_index = 0
+      call HM_CHECK(-104_c_int, 'ASSB_DBC*: BETA is not one of the headings given to HM_SETUP')
+   end function
 
    ! CalWaveProperts(KK) for INFT=3 (ImplementSubs.f90:96): omega -> V, WVN
    subroutine HM_SET_FREQUENCY_F(W1)
@@ -390,6 +443,62 @@ contains
       call HM_CHECK(hm_infgreen3d_batch(HM_CONTEXT(), 1, [XF], [XP], [YF], [YP], [ZF], [ZP], V, f, GRN), 'INFGREEN3D')
    end subroutine
 
+   ! SGLINTBD_QUAD / SGLINTBD_TRI (SingularIntgr.f90:252, :44): the Rankine
+   ! integral over hull panel JEL seen from the centroid of hull panel IEL
+   ! (IRR = 1) or of waterplane panel IEL (IRR = 3). The waterplane panels are
+   ! field points only on this path (IRR = 2, 4 take them as sources: refused).
+   subroutine sglintbd_one(IS, IEL, JEL, SIJ, DIJ, IRR, nv)
+      integer, intent(in) :: IS, IEL, JEL, IRR, nv
+      real(8), intent(out) :: SIJ, DIJ(3)
+      real(c_double) :: verts(12), xf(3), s(1), d(3)
+      integer(c_int) :: cn(1)
+      integer :: k
+      if (IS /= 1) call HM_CHECK(-101_c_int, 'SGLINTBD: IS must be 1 (no symmetry)')
+      if (IRR /= 1 .and. IRR /= 3) call HM_CHECK(-106_c_int, 'SGLINTBD: IRR 2/4 (waterplane source panels) not provided')
+      if (JEL < 1 .or. JEL > n_) call HM_CHECK(-103_c_int, 'SGLINTBD: JEL out of range')
+      if (ncn_(JEL) /= nv) call HM_CHECK(-103_c_int, 'SGLINTBD: panel JEL has another vertex count')
+      if (IRR == 1) then
+         if (IEL < 1 .or. IEL > n_) call HM_CHECK(-102_c_int, 'SGLINTBD: IEL out of range')
+         xf = xyzp_(:, IEL)
+      else
+         if (IEL < 1 .or. IEL > ni_) call HM_CHECK(-102_c_int, 'SGLINTBD: IEL out of range')
+         xf = ixyzp_(:, IEL)
+      end if
+      verts = 0.d0
+      do k = 1, nv
+         verts(3 * k - 2:3 * k) = xyz_(:, ncon_(k, JEL))
+      end do
+      cn(1) = nv
+      call HM_CHECK(hm_sglintbd_batch(HM_CONTEXT(), 1_c_int, cn, verts, xf, s, d), 'SGLINTBD')
+      SIJ = s(1)
+      DIJ = d
+   end subroutine
+
+   subroutine SGLINTBD_QUAD(IS, IEL, JEL, SIJ, DIJ, IRR)
+      integer, intent(in) :: IS, IEL, JEL, IRR
+      real(8), intent(out) :: SIJ, DIJ(3)
+      call sglintbd_one(IS, IEL, JEL, SIJ, DIJ, IRR, 4)
+   end subroutine
+
+   subroutine SGLINTBD_TRI(IS, IEL, JEL, SIJ, DIJ, IRR)
+      integer, intent(in) :: IS, IEL, JEL, IRR
+      real(8), intent(out) :: SIJ, DIJ(3)
+      call sglintbd_one(IS, IEL, JEL, SIJ, DIJ, IRR, 3)
+   end subroutine
+
+   ! CALGREEN (CalGreenFunc.f90:49), CALGREEN_IRR (:124), CALGREEN_MULTI
+   ! (CalGreenFuncMulti.f90:112), CALGREEN_MULTI_IRR (:189): the table fill is
+   ! fused into the device assembly that the ASSB_LEFT* wrappers run, so the
+   ! driver's calls return at once and leave no table on the host
+   subroutine CALGREEN()
+   end subroutine
+   subroutine CALGREEN_IRR()
+   end subroutine
+   subroutine CALGREEN_MULTI()
+   end subroutine
+   subroutine CALGREEN_MULTI_IRR()
+   end subroutine
+
    ! ---- the per-frequency chain with the reference argument lists --------------
    ! CALGREEN_MULTI + ASSB_LEFT_MULTI: assemble on the device and return the LU
    ! factors and pivots (ZGETRF, AssbMatxMulti.f90:112) as the reference does
@@ -423,11 +532,161 @@ contains
       BRMAT(:, :, 1) = BRMAT(:, :, 1) + rhs_(:, 6 * b + 1:6 * b + 6)
    end subroutine
 
-   ! ASSB_DBC_MULTI: diffraction right-hand side of heading HEADING (1-based)
-   subroutine ASSB_DBC_MULTI(BDMAT, NELEM, NSYS, HEADING)
-      integer, intent(in) :: NELEM, NSYS, HEADING
+   ! ASSB_DBC_MULTI: diffraction right-hand side of the heading in BETA
+   subroutine ASSB_DBC_MULTI(BDMAT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
       complex(8), intent(out) :: BDMAT(NELEM, NSYS)
-      BDMAT(:, 1) = rhs_(:, 6 * nbody_ + HEADING)
+      if (NSYS /= 1) call HM_CHECK(-101_c_int, 'ASSB_DBC_MULTI: NSYS must be 1')
+      BDMAT(:, 1) = rhs_(1:NELEM, 6 * nbody_ + heading_index())
+   end subroutine
+
+   ! ---- one body (AssbMatx.f90): the same chain with six modes -------------------
+   subroutine ASSB_LEFT(AMAT, IPIV, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(out) :: IPIV(NELEM, NSYS)
+      complex(8), intent(out), target :: AMAT(NELEM, NELEM, NSYS)
+      call ASSB_LEFT_MULTI(AMAT, IPIV, NELEM, NSYS)
+   end subroutine
+
+   subroutine ASSB_RBC(BRMAT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      complex(8), intent(out) :: BRMAT(NELEM, 6, NSYS)
+      if (NSYS /= 1 .or. nbody_ /= 1) call HM_CHECK(-101_c_int, 'ASSB_RBC: one body, NSYS = 1')
+      BRMAT(:, :, 1) = rhs_(1:NELEM, 1:6)
+   end subroutine
+
+   subroutine ASSB_DBC(BDMAT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      complex(8), intent(out) :: BDMAT(NELEM, NSYS)
+      call ASSB_DBC_MULTI(BDMAT, NELEM, NSYS)
+   end subroutine
+
+   subroutine RADIATION_SOLVER(AMAT, BRMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: AMAT(NELEM, NELEM, NSYS), BRMAT(NELEM, 6, NSYS)
+      complex(8), intent(out) :: MXPOT(NELEM, 7, NSYS)
+      call RADIATION_SOLVER_MULTI(AMAT, BRMAT, IPIV, MXPOT, NELEM, NSYS)
+   end subroutine
+
+   subroutine DIFFRACTION_SOLVER(AMAT, BDMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: AMAT(NELEM, NELEM, NSYS), BDMAT(NELEM, NSYS)
+      complex(8), intent(inout) :: MXPOT(NELEM, 7, NSYS)
+      call DIFFRACTION_SOLVER_MULTI(AMAT, BDMAT, IPIV, MXPOT, NELEM, NSYS)
+   end subroutine
+
+   ! ---- irregular-frequency removal (AssbMatx_irr.f90, AssbMatx_irrMulti.f90) ---
+   ! ASSB_LEFT_IRR: AMAT (TNELEM x TNELEM, the first NELEM columns used: the hull
+   ! rows and the waterplane rows), CMAT = A^T A factored (ZGETRF) with IPIV
+   subroutine ASSB_LEFT_IRR(AMAT, CMAT, IPIV, NELEM, TNELEM, NSYS)
+      integer, intent(in) :: NELEM, TNELEM, NSYS
+      complex(8), intent(out) :: AMAT(TNELEM, TNELEM, NSYS)
+      complex(8), intent(out), target :: CMAT(NELEM, NELEM, NSYS)
+      integer, intent(out) :: IPIV(NELEM, NSYS)
+      complex(c_double_complex), allocatable, target :: a(:, :)
+      integer(c_int) :: info, ncol
+      type(c_ptr) :: c
+      if (NSYS /= 1) call HM_CHECK(-101_c_int, 'ASSB_LEFT_IRR: NSYS must be 1')
+      if (NELEM /= n_ .or. TNELEM /= n_ + ni_ .or. ni_ == 0) call HM_CHECK(-102_c_int, 'ASSB_LEFT_IRR: NELEM, TNELEM')
+      c = HM_CONTEXT()
+      ncol = 6 * nbody_ + nbeta_
+      if (allocated(rhs_)) deallocate (rhs_)
+      if (allocated(drhs_)) deallocate (drhs_)
+      allocate (rhs_(TNELEM, ncol), drhs_(NELEM, ncol), a(TNELEM, NELEM))
+      call HM_CHECK(hm_assemble(c), 'hm_assemble')
+      call HM_CHECK(hm_get_system(c, c_loc(a), c_null_ptr, c_loc(rhs_)), 'hm_get_system')
+      call HM_CHECK(hm_get_normal_equations(c, c_loc(CMAT), c_loc(drhs_)), 'hm_get_normal_equations')
+      AMAT = (0.d0, 0.d0)
+      AMAT(:, 1:NELEM, 1) = a
+      call HM_CHECK(hm_zgetrf(c, NELEM, NELEM, CMAT, NELEM, IPIV, info), 'ZGETRF')
+   end subroutine
+
+   ! ASSB_RBC_IRR: BRMAT (TNELEM x 6) and DRMAT = A^T BRMAT (NELEM x 6)
+   subroutine ASSB_RBC_IRR(BRMAT, DRMAT, AMAT, NELEM, TNELEM, NSYS)
+      integer, intent(in) :: NELEM, TNELEM, NSYS
+      complex(8), intent(out) :: BRMAT(TNELEM, 6, NSYS), DRMAT(NELEM, 6, NSYS)
+      complex(8), intent(in) :: AMAT(TNELEM, TNELEM, NSYS)
+      if (NSYS /= 1 .or. nbody_ /= 1) call HM_CHECK(-101_c_int, 'ASSB_RBC_IRR: one body, NSYS = 1')
+      BRMAT(:, :, 1) = rhs_(:, 1:6)
+      DRMAT(:, :, 1) = drhs_(:, 1:6)
+   end subroutine
+
+   ! ASSB_DBC_IRR: BDMAT (TNELEM) and DDMAT = A^T BDMAT (NELEM) of the heading in BETA
+   subroutine ASSB_DBC_IRR(BDMAT, DDMAT, AMAT, NELEM, TNELEM, NSYS)
+      integer, intent(in) :: NELEM, TNELEM, NSYS
+      complex(8), intent(out) :: BDMAT(TNELEM, NSYS), DDMAT(NELEM, NSYS)
+      complex(8), intent(in) :: AMAT(TNELEM, TNELEM, NSYS)
+      integer :: k
+      if (NSYS /= 1) call HM_CHECK(-101_c_int, 'ASSB_DBC_IRR: NSYS must be 1')
+      k = 6 * nbody_ + heading_index()
+      BDMAT(:, 1) = rhs_(:, k)
+      DDMAT(:, 1) = drhs_(:, k)
+   end subroutine
+
+   ! RADIATION_SOLVER_IRR / DIFFRACTION_SOLVER_IRR: ZGETRS with the factored CMAT
+   subroutine RADIATION_SOLVER_IRR(CMAT, DRMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: CMAT(NELEM, NELEM, NSYS), DRMAT(NELEM, 6 * nbody_, NSYS)
+      complex(8), intent(out) :: MXPOT(NELEM, 6 * nbody_ + 1, NSYS)
+      call RADIATION_SOLVER_MULTI(CMAT, DRMAT, IPIV, MXPOT, NELEM, NSYS)
+   end subroutine
+
+   subroutine DIFFRACTION_SOLVER_IRR(CMAT, DDMAT, IPIV, MXPOT, NELEM, NSYS)
+      integer, intent(in) :: NELEM, NSYS
+      integer, intent(in) :: IPIV(NELEM, NSYS)
+      complex(8), intent(in) :: CMAT(NELEM, NELEM, NSYS), DDMAT(NELEM, NSYS)
+      complex(8), intent(inout) :: MXPOT(NELEM, 6 * nbody_ + 1, NSYS)
+      call DIFFRACTION_SOLVER_MULTI(CMAT, DDMAT, IPIV, MXPOT, NELEM, NSYS)
+   end subroutine
+
+   ! the multi-body forms (NELEM_TOTAL hull panels, iNELEM_TOTAL waterplane panels)
+   subroutine ASSB_LEFT_IRR_MULTI(AMAT, CMAT, IPIV, NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS)
+      integer, intent(in) :: NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS
+      complex(8), intent(out) :: AMAT(TNELEM, TNELEM, NSYS)
+      complex(8), intent(out), target :: CMAT(NELEM_TOTAL, NELEM_TOTAL, NSYS)
+      integer, intent(out) :: IPIV(NELEM_TOTAL, NSYS)
+      if (INELEM_TOTAL /= ni_) call HM_CHECK(-102_c_int, 'ASSB_LEFT_IRR_MULTI: iNELEM_TOTAL')
+      call ASSB_LEFT_IRR(AMAT, CMAT, IPIV, NELEM_TOTAL, TNELEM, NSYS)
+   end subroutine
+
+   subroutine ASSB_RBC_IRR_MULTI(BRMAT, DRMAT, AMAT, NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS, NELEM_START, NELEM_END)
+      integer, intent(in) :: NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS, NELEM_START, NELEM_END
+      complex(8), intent(inout) :: BRMAT(TNELEM, 6, NSYS), DRMAT(NELEM_TOTAL, 6, NSYS)
+      complex(8), intent(in) :: AMAT(TNELEM, TNELEM, NSYS)
+      integer :: b
+      do b = 0, nbody_ - 1
+         if (body_start_(b + 1) == NELEM_START - 1) exit
+      end do
+      if (b >= nbody_ .or. body_start_(b + 2) /= NELEM_END .or. NSYS /= 1) &
+         call HM_CHECK(-103_c_int, 'ASSB_RBC_IRR_MULTI: panel range is not one body')
+      BRMAT(:, :, 1) = BRMAT(:, :, 1) + rhs_(:, 6 * b + 1:6 * b + 6)
+      DRMAT(:, :, 1) = DRMAT(:, :, 1) + drhs_(:, 6 * b + 1:6 * b + 6)
+   end subroutine
+
+   subroutine ASSB_DBC_IRR_MULTI(BDMAT, DDMAT, AMAT, NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS)
+      integer, intent(in) :: NELEM_TOTAL, INELEM_TOTAL, TNELEM, NSYS
+      complex(8), intent(out) :: BDMAT(TNELEM, NSYS), DDMAT(NELEM_TOTAL, NSYS)
+      complex(8), intent(in) :: AMAT(TNELEM, TNELEM, NSYS)
+      call ASSB_DBC_IRR(BDMAT, DDMAT, AMAT, NELEM_TOTAL, TNELEM, NSYS)
+   end subroutine
+
+   subroutine RADIATION_SOLVER_IRR_MULTI(CMAT, DRMAT, IPIV, MXPOT, NELEM_TOTAL, NSYS)
+      integer, intent(in) :: NELEM_TOTAL, NSYS
+      integer, intent(in) :: IPIV(NELEM_TOTAL, NSYS)
+      complex(8), intent(in) :: CMAT(NELEM_TOTAL, NELEM_TOTAL, NSYS), DRMAT(NELEM_TOTAL, 6 * nbody_, NSYS)
+      complex(8), intent(out) :: MXPOT(NELEM_TOTAL, 6 * nbody_ + 1, NSYS)
+      call RADIATION_SOLVER_MULTI(CMAT, DRMAT, IPIV, MXPOT, NELEM_TOTAL, NSYS)
+   end subroutine
+
+   subroutine DIFFRACTION_SOLVER_IRR_MULTI(CMAT, DDMAT, IPIV, MXPOT, NELEM_TOTAL, NSYS)
+      integer, intent(in) :: NELEM_TOTAL, NSYS
+      integer, intent(in) :: IPIV(NELEM_TOTAL, NSYS)
+      complex(8), intent(in) :: CMAT(NELEM_TOTAL, NELEM_TOTAL, NSYS), DDMAT(NELEM_TOTAL, NSYS)
+      complex(8), intent(inout) :: MXPOT(NELEM_TOTAL, 6 * nbody_ + 1, NSYS)
+      call DIFFRACTION_SOLVER_MULTI(CMAT, DDMAT, IPIV, MXPOT, NELEM_TOTAL, NSYS)
    end subroutine
 
    subroutine RADIATION_SOLVER_MULTI(AMAT, BRMAT, IPIV, MXPOT, NELEM, NSYS)

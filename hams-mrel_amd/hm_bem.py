@@ -28,7 +28,7 @@ _vp = ctypes.c_void_p
 EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm_set_geometry",
            "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
-           "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_fingreen3d_batch",
+           "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_get_normal_equations", "hm_fingreen3d_batch",
            "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_solve_motion", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
@@ -189,6 +189,14 @@ class Context:
         rhs = np.zeros((nt, ncol), dtype=np.complex128, order="F")
         self._chk(self.L.hm_get_system(self.ptr, _p(amat), _p(smat), _p(rhs)), "hm_get_system")
         return amat, smat, rhs
+
+    def get_normal_equations(self):
+        """(C, D): A^T A (n x n) and A^T of the right-hand sides, with waterplane panels"""
+        p = self.problem
+        cm = np.zeros((p.n, p.n), dtype=np.complex128, order="F")
+        d = np.zeros((p.n, p.m + p.nbeta), dtype=np.complex128, order="F")
+        self._chk(self.L.hm_get_normal_equations(self.ptr, _p(cm), _p(d)), "hm_get_normal_equations")
+        return cm, d
 
     def get_potentials(self):
         p = self.problem

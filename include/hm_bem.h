@@ -289,6 +289,11 @@ int hm_get_pair_histogram(hm_ctx *ctx, long long *counts, int nkeys);
  * NULL. hm_get_potentials: n x (6nb+nbeta) complex after hm_solve (MXPOT). */
 int hm_get_system(hm_ctx *ctx, double *amat, double *smat, double *rhs);
 int hm_get_potentials(hm_ctx *ctx, double *mxpot);
+/* With waterplane panels (irregular-frequency removal), after hm_assemble: the
+ * normal equations the solve uses, CMAT = A^T A (n x n, AssbMatx_irr.f90:153-169)
+ * and A^T of the right-hand sides (n x (6nb+nbeta): DRMAT, DDMAT,
+ * AssbMatx_irr.f90:300-318, :448-462), column-major complex; either may be NULL. */
+int hm_get_normal_equations(hm_ctx *ctx, double *cmat, double *drhs);
 
 /* ---- element-wise kernels (parity entry points, host arrays) ----------------
  * FINGREEN3D(XF,XP,YF,YP,ZF,ZP,V,WVN,NK,H,GRN,FLAG)  FinGreen3D.f90:81

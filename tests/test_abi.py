@@ -40,4 +40,12 @@ def test_dispersion_entry_runs_on_host(oracle):
 
 
 def test_fortran_layer_built():
+    """the drop-in module object and its Fortran test caller, which links the
+    reference-named wrappers against libhm_bem.so (run on the GPU by
+    tests/test_gpu_fortran.py); every wrapper the caller uses resolves"""
+    import subprocess
     assert os.path.exists(os.path.join(ROOT, "hams-mrel_amd", "lib", "hm_bem_f.o"))
+    exe = os.path.join(ROOT, "tests", "fortran", "hm_fortran_check")
+    assert os.path.exists(exe)
+    r = subprocess.run(["ldd", exe], capture_output=True, text=True)
+    assert "libhm_bem.so" in r.stdout and "not found" not in r.stdout, r.stdout
