@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "hm_noalloc.hpp"
 #include "hm_assembly.hpp"
 #include "hm_bem.h"
 #include "hm_device.hpp"
@@ -311,6 +312,9 @@ extern "C" int hm_create(hm_ctx** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  // the fixed device array of hm_fingreen3d_batch_noalloc: its address on this
+  // device (no allocation)
+  if (e == hipSuccess) e = hm::noalloc_setup();
   if (e != hipSuccess) {
     delete c;
     return HM_ERR_HIP - (int)e;
@@ -1834,6 +1838,23 @@ extern "C" int hm_fingreen3d_batch(hm_ctx* c, int n, const double* xf, const dou
                                    int nk, double h, const int* flag, double* grn) {
   const double* in[6] = {xf, xp, yf, yp, zf, zp};
   return green_common(c, n, false, in, v, wvn, nk, h, flag, grn);
+}
+
+extern "C" int hm_fingreen3d_batch_noalloc(hm_ctx* c, int n, const double* xf, const double* xp, const double* yf,
+                                           const double* yp, const double* zf, const double* zp, double v,
+                                           const double* wvn, int nk, double h, const int* flag, double* grn) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 0) return c->fail(HM_ERR_ARG - 2, "n < 0");
+  if (n > hm::kNoAllocMaxPoints)
+    return c->fail(HM_ERR_ARG - 2, "n = %d exceeds the %d pairs of the fixed device array", n, hm::kNoAllocMaxPoints);
+  if (n == 0) return HM_OK;
+  const double* in[6] = {xf, xp, yf, yp, zf, zp};
+  for (int k = 0; k < 6; ++k)
+    if (!in[k]) return c->fail(HM_ERR_ARG - 3 - k, "coordinate array %d is NULL", k);
+  if (!flag || !grn) return c->fail(HM_ERR_ARG - 1, "flag/grn NULL");
+  if (!wvn || nk < kNk) return c->fail(HM_ERR_ARG - 10, "wvn must hold %d roots", kNk);
+  if (!(h > 0.0)) return c->fail(HM_ERR_ARG - 12, "h must be > 0");
+  return c->hip(hm::noalloc_green(c->stream, n, false, in, v, wvn, h, flag, grn), "green batch (no allocation)");
 }
 
 extern "C" int hm_infgreen3d_batch(hm_ctx* c, int n, const double* xf, const double* xp, const double* yf,

@@ -29,7 +29,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_set_depth", "hm_set_pair_symmetry", "hm_set_headings", "hm_set_frequency", "hm_assemble", "hm_factor",
            "hm_solve", "hm_forces", "hm_solve_frequency", "hm_run_frequencies", "hm_set_timing",
            "hm_get_timing", "hm_get_system", "hm_get_potentials", "hm_get_normal_equations", "hm_fingreen3d_batch",
-           "hm_infgreen3d_batch", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_solve_motion", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
+           "hm_infgreen3d_batch", "hm_fingreen3d_batch_noalloc", "hm_sglintbd_batch", "hm_ik01a_batch", "hm_field_points", "hm_solve_motion", "hm_dispersion", "hm_zgetrf", "hm_zgetrs",
            "hm_get_pair_histogram", "hm_set_gmres", "hm_get_gmres_stats",
            "hm_set_series_variant", "hm_surrogate_blob_size", "hm_surrogate_train", "hm_surrogate_get",
            "hm_surrogate_set", "hm_set_surrogate", "hm_surrogate_counts", "hm_surrogate_eval",

@@ -304,6 +304,16 @@ int hm_fingreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, 
                         const int *flag, double *grn);
 int hm_infgreen3d_batch(hm_ctx *ctx, int n, const double *xf, const double *xp, const double *yf, const double *yp,
                         const double *zf, const double *zp, double v, const int *flag, double *grn);
+/* No counterpart in the reference: hm_fingreen3d_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same arguments, results bit for bit the
+ * same; the inputs and outputs are staged in a fixed device array that each GPU
+ * holds from the first context created on it until the process ends
+ * (30,410,496 bytes), so n is at most 262144 pairs per call (larger n returns
+ * HM_ERR_ARG - 2). Calls that overlap on one GPU share that array and race. */
+int hm_fingreen3d_batch_noalloc(hm_ctx *ctx, int n, const double *xf, const double *xp, const double *yf,
+                                const double *yp, const double *zf, const double *zp, double v, const double *wvn,
+                                int nk, double h, const int *flag, double *grn);
 /* SGLINTBD_QUAD/TRI(IS,IEL,JEL,SIJ,DIJ,IRR)  SingularIntgr.f90:252/:44, with the
  * source panel's vertices verts[12*i] (ncn[i] of them) and field point xf[3*i]. */
 int hm_sglintbd_batch(hm_ctx *ctx, int n, const int *ncn, const double *verts, const double *xf, double *sij,
