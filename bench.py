@@ -57,7 +57,10 @@ ACCURACY_BAR = 1e-4
 # cached frequency-independent term halves (31 GB read per launch); it runs at 5
 # waves per SIMD, and register-spill traffic (scratch that misses the L2) is
 # most of the rest.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {
+    ("c5", "hmatrix", "asm_pidcock"): 43902678.5 * 1024 * 2 + 44113979.84375 * 1024,
+    ("c5", "hmatrix", "asm_eigene"): 18188861.6875 * 1024 * 2 + 42729947.96875 * 1024,
+}
 
 def flop_model(name):
     """algorithmic flops per pair by classification key (tests/golden/flop_model.json)"""
