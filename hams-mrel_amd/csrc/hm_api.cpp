@@ -24,6 +24,7 @@
 #include "hm_device.hpp"
 #include "hm_gmres.hpp"
 #include "hm_hmatrix.hpp"
+#include "hm_htree.hpp"
 #include "hm_linalg.hpp"
 #include "hm_surrogate.hpp"
 
@@ -137,6 +138,18 @@ struct hm_ctx {
   // hd_nb clusters of hd_m rows
   bool hd_ready = false;
   int hd_nb = 0, hd_m = 0;
+  // H-matrix over the geometric cluster tree (hm_hmatrix_build_tree, or the
+  // assembly-aware form where the body clusters do not apply: one body, bodies
+  // of unequal panel counts, interior waterplane points, or on request)
+  hm::HTree htree;
+  bool ht_ready = false;       // htree holds the H-matrix of the current system
+  bool hm_tree = false;        // hm_set_hmatrix chose the cluster-tree form
+  int ht_mode = 0, ht_leaf = 32;
+  double ht_eta = 2.0;
+  DevBuf d_htw;                // workspace of the tree products
+  bool ht_from_build = false;  // htree came from hm_hmatrix_build_tree (serves hm_ziter_solve)
+  std::vector<int> ht_blocks;  // its preconditioner blocks (empty: none)
+  std::vector<double> xyz_host;   // panel centroids, x, y, z per panel
   int hm_unconverged = 0, hm_steps = 0, hm_rank_max = 0;
   double hm_rank_mean = 0.0;
   int bm() const { return n / nbody; }
@@ -331,8 +344,11 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2};
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
+                    &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
+                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw};
   for (DevBuf* b : bufs) b->release();
+  c->htree.release();
   (void)c->resolve_timing();
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -415,9 +431,12 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
       (rc = upload(c, c->d_pnsz, pnsz, n, "geometry")) || (rc = upload(c, c->d_dxyz, dxyz_p, 6 * (size_t)n, "geometry")))
     return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream), "geometry upload");
+  c->xyz_host.assign(xyz_p, xyz_p + 3 * (size_t)n);
   c->have_geom = true;
   c->have_depth = false;
   c->hm_on = false;
+  c->hm_tree = false;
+  c->ht_ready = false;
   c->assembled = c->factored = c->solved = false;
   return HM_OK;
 }
@@ -855,6 +874,34 @@ static double* sys_matrix(hm_ctx* c) { return c->ni > 0 ? c->d_C.as<double>() : 
 static double* sys_rhs(hm_ctx* c) { return c->ni > 0 ? c->d_D.as<double>() : c->d_R.as<double>(); }
 static int sys_ld_rhs(hm_ctx* c) { return c->ni > 0 ? c->n : c->nt(); }
 
+// the cluster-tree H-matrix of the n x n matrix A (device, leading dimension
+// lda) over the points (x[i], y[i], z[i])
+static int build_tree(hm_ctx* c, const double* A, long long lda, int n, const std::vector<double>& x,
+                      const std::vector<double>& y, const std::vector<double>& z) {
+  hm::HTreeHost host;
+  hm::htree_partition(n, x.data(), y.data(), z.data(), c->ht_leaf, c->ht_eta, host);
+  c->ht_ready = false;
+  HIPCHK(c, c->htree.build(c->stream, host, A, lda, c->hm_eps, c->hm_kmax), "cluster-tree H-matrix");
+  const hm::HTreeStats& st = c->htree.stats();
+  c->hm_unconverged = st.fallback_dense;
+  c->hm_steps = st.steps;
+  c->hm_rank_max = st.max_rank;
+  c->hm_rank_mean = st.mean_rank;
+  c->ht_ready = true;
+  return HM_OK;
+}
+
+// GMRES products through the cluster-tree H-matrix (FP32 for the Krylov steps,
+// FP64 for the residuals); the workspace is sized for ncol columns
+static int tree_ops(hm_ctx* c, hm::GmresArgs& ga, int n, int ncol) {
+  HIPCHK(c, c->d_htw.ensure(c->htree.workspace_bytes(ncol, sizeof(double))), "tree product alloc");
+  const hm::HTree* t = &c->htree;
+  void* w = c->d_htw.p;
+  ga.op32 = [=](hipStream_t st, const float* Z, float* W) { return t->apply<float>(st, Z, n, W, n, ncol, w); };
+  ga.op64 = [=](hipStream_t st, const double* Z, double* W) { return t->apply<double>(st, Z, n, W, n, ncol, w); };
+  return HM_OK;
+}
+
 extern "C" int hm_factor(hm_ctx* c, int precision) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->assembled) return c->fail(HM_ERR_STATE, "hm_factor before hm_assemble");
@@ -870,6 +917,10 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
   HIPCHK(c, c->d_counter.ensure(sizeof(unsigned) * nmat), "factor alloc");
   HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nmat, c->stream), "factor");
   HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nmat, c->stream), "factor");
+  // the factorisation overwrites the preconditioner an earlier H-matrix build set up
+  c->hd_ready = false;
+  c->ht_from_build = false;
+  if (c->hm_tree && precision != HM_GMRES) return c->fail(HM_ERR_ARG - 2, "the H-matrix form solves with HM_GMRES only");
   c->stage_begin(kLu);
   if (c->hm_on) {
     if (precision != HM_GMRES) return c->fail(HM_ERR_ARG - 2, "the H-matrix form solves with HM_GMRES only");
@@ -886,10 +937,13 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
     HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
                                    c->d_counter.as<unsigned>(), 1, c->d_luws.p), "lu fp32");
   } else {
-    // GMRES: FP32 copy for the Krylov products; block-diagonal preconditioner
-    // (one block per body) by default for several bodies, none for one body
-    HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
-    HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    // GMRES: FP32 copy for the Krylov products (the cluster-tree form keeps its
+    // own FP32 factors); block-diagonal preconditioner (one block per body) by
+    // default for several bodies, none for one body
+    if (!c->hm_tree) {
+      HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
+      HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
+    }
     const bool blocks = c->gm_precond < 0 ? c->nbody > 1 : c->gm_precond >= 1;
     c->gm_blocks.clear();
     if (blocks) {
@@ -904,6 +958,18 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
       HIPCHK(c, hm::gmres_block_precond_setup(c->stream, sys_matrix(c), n, c->nbody, c->gm_blocks.data(),
                                               c->d_P32.as<float>(), c->d_pipiv.as<int>(), c->d_info.as<int>(),
                                               c->d_counter.as<unsigned>()), "block preconditioner");
+    }
+    if (c->hm_tree) {
+      // the H-matrix of the system matrix (A, or C = A^T A with waterplane rows)
+      // over the hull panels' centroids
+      std::vector<double> x(n), y(n), z(n);
+      for (int i = 0; i < n; ++i) {
+        x[i] = c->xyz_host[3 * (size_t)i];
+        y[i] = c->xyz_host[3 * (size_t)i + 1];
+        z[i] = c->xyz_host[3 * (size_t)i + 2];
+      }
+      int rc = build_tree(c, sys_matrix(c), n, n, x, y, z);
+      if (rc != HM_OK) return rc;
     }
     int rc = precond_prepare(c, c->stream);
     if (rc != HM_OK) return rc;
@@ -991,6 +1057,10 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
       ga.op64 = [=](hipStream_t s, const double* Z, double* W) {
         return hm::hmat_apply<double>(s, nb, m, k, D, m, sD, U, V, Z, n, W, n, ncol, tb, ts);
       };
+    } else if (c->hm_tree) {
+      if (!c->ht_ready) return c->fail(HM_ERR_STATE, "the cluster-tree H-matrix was released; factor again");
+      int rc = tree_ops(c, ga, n, ncol);
+      if (rc != HM_OK) return rc;
     } else {
       ga.A64 = sys_matrix(c);
       ga.lda64 = n;
@@ -1079,26 +1149,57 @@ extern "C" int hm_set_hmatrix(hm_ctx* c, int enable, double eps, double eps_s, i
   if (!(eps > 0.0 && eps < 1.0)) return c->fail(HM_ERR_ARG - 3, "eps must be in (0, 1)");
   if (!(eps_s > 0.0 && eps_s < 1.0)) return c->fail(HM_ERR_ARG - 4, "eps_s must be in (0, 1)");
   if (kmax < 1) return c->fail(HM_ERR_ARG - 5, "kmax must be >= 1");
+  bool tree = false;
   if (enable) {
     if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_hmatrix before hm_set_geometry");
-    if (c->nbody < 2) return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs at least two bodies");
-    if (c->ni > 0) return c->fail(HM_ERR_ARG - 2, "the H-matrix form takes no interior waterplane points (ni = %d)", c->ni);
+    // body clusters (assembly-aware cross approximation) for two or more bodies
+    // of equal panel counts without waterplane points; the cluster tree over the
+    // assembled system otherwise, or when hm_set_cluster_tree asks for it
     const int m = c->n / c->nbody;
-    for (int b = 0; b <= c->nbody; ++b)
-      if (c->body_start[b] != b * m)
-        return c->fail(HM_ERR_ARG - 2, "the H-matrix form needs bodies of equal panel counts");
-    if (kmax > m) return c->fail(HM_ERR_ARG - 5, "kmax must be in 1..%d", m);
+    bool bodies = c->ht_mode == 0 && c->nbody >= 2 && c->ni == 0 && kmax <= m;
+    for (int b = 0; bodies && b <= c->nbody; ++b) bodies = c->body_start[b] == b * m;
+    tree = !bodies;
   }
-  // the tolerances and rank cap also serve hm_hmatrix_build
+  c->join_side();
+  // the tolerances and rank cap also serve hm_hmatrix_build and hm_hmatrix_build_tree
   c->hm_eps = eps;
   c->hm_eps_s = eps_s;
   c->hm_kmax = kmax;
   const bool was = c->hm_on;
-  c->hm_on = enable != 0;
+  c->hm_on = enable != 0 && !tree;
+  c->hm_tree = enable != 0 && tree;
+  if (c->hm_tree || !c->ht_from_build) {
+    c->ht_ready = false;
+    c->ht_from_build = false;
+  }
   c->assembled = c->factored = c->solved = false;
   c->hm_unconverged = c->hm_steps = c->hm_rank_max = 0;
   c->hm_rank_mean = 0.0;
   if (was != c->hm_on && c->have_depth && c->h > 0.0) return classify(c);
+  return HM_OK;
+}
+
+extern "C" int hm_set_cluster_tree(hm_ctx* c, int mode, int leaf, double eta) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (mode != 0 && mode != 1) return c->fail(HM_ERR_ARG - 2, "mode must be 0 (body clusters where they apply) or 1 (tree)");
+  if (leaf < 2 || leaf > 64) return c->fail(HM_ERR_ARG - 3, "leaf must be in 2..64");
+  if (!(eta > 0.0)) return c->fail(HM_ERR_ARG - 4, "eta must be > 0");
+  c->ht_mode = mode;
+  c->ht_leaf = leaf;
+  c->ht_eta = eta;
+  return HM_OK;
+}
+
+extern "C" int hm_get_cluster_tree_stats(hm_ctx* c, int* clusters, int* leaves, int* dense_blocks, int* lowrank_blocks,
+                                         int* dense_fallbacks) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->ht_ready) return c->fail(HM_ERR_STATE, "no cluster-tree H-matrix is built");
+  const hm::HTreeStats& st = c->htree.stats();
+  if (clusters) *clusters = st.clusters;
+  if (leaves) *leaves = st.leaves;
+  if (dense_blocks) *dense_blocks = st.dense_blocks;
+  if (lowrank_blocks) *lowrank_blocks = st.lowrank_blocks;
+  if (dense_fallbacks) *dense_fallbacks = st.fallback_dense;
   return HM_OK;
 }
 
@@ -1115,6 +1216,7 @@ extern "C" int hm_get_hmatrix_stats(hm_ctx* c, int* unconverged, int* steps, dou
     const double m = c->nbody ? (double)c->n / c->nbody : 0.0;
     const double off = (double)c->nbody * (c->nbody - 1) * 2.0 * m * c->hm_rank_mean;
     *stored_fraction = c->hm_on && c->n ? (c->nbody * m * m + off) / ((double)c->n * c->n) : 1.0;
+    if (!c->hm_on && !c->hd_ready && c->ht_ready) *stored_fraction = c->htree.stats().stored_fraction;
   }
   return HM_OK;
 }
@@ -1122,8 +1224,28 @@ extern "C" int hm_get_hmatrix_stats(hm_ctx* c, int* unconverged, int* steps, dou
 extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, double* y) {
   if (!c) return HM_ERR_ARG - 1;
   const bool from_assembly = c->hm_on && c->assembled;
+  if (!from_assembly && !c->hd_ready && c->ht_ready) {
+    // the cluster-tree H-matrix (of A, or of C = A^T A with waterplane rows)
+    if (mat != 0) return c->fail(HM_ERR_ARG - 2, "the cluster-tree H-matrix has only mat 0");
+    if (ncol < 1) return c->fail(HM_ERR_ARG - 3, "bad ncol %d", ncol);
+    if (!x || !y) return c->fail(HM_ERR_ARG - 4, "x or y is NULL");
+    const int n = c->htree.n();
+    const size_t bytes = 2 * sizeof(double) * (size_t)n * ncol;
+    DevBuf dx, dy;
+    int rc = HM_OK;
+    if ((rc = c->hip(dx.ensure(bytes), "hmatrix apply")) == HM_OK && (rc = c->hip(dy.ensure(bytes), "hmatrix apply")) == HM_OK &&
+        (rc = c->hip(c->d_htw.ensure(c->htree.workspace_bytes(ncol, sizeof(double))), "hmatrix apply")) == HM_OK &&
+        (rc = c->hip(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, c->stream), "hmatrix apply")) == HM_OK &&
+        (rc = c->hip(c->htree.apply<double>(c->stream, dx.as<double>(), n, dy.as<double>(), n, ncol, c->d_htw.p),
+                     "hmatrix apply")) == HM_OK &&
+        (rc = c->hip(hipMemcpyAsync(y, dy.p, bytes, hipMemcpyDeviceToHost, c->stream), "hmatrix apply")) == HM_OK)
+      rc = c->hip(hipStreamSynchronize(c->stream), "hmatrix apply");
+    dx.release();
+    dy.release();
+    return rc;
+  }
   if (!from_assembly && !c->hd_ready)
-    return c->fail(HM_ERR_STATE, "hm_hmatrix_apply needs an assembled H-matrix form or one built by hm_ziter_solve");
+    return c->fail(HM_ERR_STATE, "hm_hmatrix_apply needs an assembled H-matrix form or one built by hm_hmatrix_build(_tree)");
   if (mat != 0 && mat != 1) return c->fail(HM_ERR_ARG - 2, "mat must be 0 (A) or 1 (S)");
   if (!from_assembly && mat != 0) return c->fail(HM_ERR_ARG - 2, "an H-matrix built from a matrix has only mat 0");
   if (ncol < 1 || (from_assembly && ncol > c->ncol())) return c->fail(HM_ERR_ARG - 3, "bad ncol %d", ncol);
@@ -1171,6 +1293,7 @@ extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int 
   // the context's matrix buffers are reused: any assembled state is gone
   c->assembled = c->factored = c->solved = false;
   c->hd_ready = false;
+  c->ht_ready = c->ht_from_build = false;
   const size_t cplx = 2 * sizeof(double);
   hipStream_t s = c->stream;
   const int nb = nblocks, kmax = c->hm_kmax;
@@ -1231,7 +1354,97 @@ extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int 
 
 extern "C" int hm_hmatrix_release(hm_ctx* c) {
   if (!c) return HM_ERR_ARG - 1;
+  c->join_side();
   c->hd_ready = false;
+  if (c->ht_ready) {
+    (void)hipStreamSynchronize(c->stream);
+    c->htree.release();
+    c->ht_ready = false;
+    c->ht_from_build = false;
+    if (c->hm_tree) c->factored = c->solved = false;   // the solve needs it: factor again
+  }
+  return HM_OK;
+}
+
+// the cluster tree and block partition of hm_hmatrix_build_tree, on the host
+// only: perm (n, 0-based cluster position -> point) and 5 ints per block (rs,
+// nr, cs, nc, admissible); blocks = NULL asks for the count only
+extern "C" int hm_cluster_tree_partition(int n, const double* xyz_p, int ldxyz, int leaf, double eta, int* perm,
+                                         int max_blocks, int* nblocks, int* blocks) {
+  if (n < 1) return HM_ERR_ARG - 1;
+  if (!xyz_p || ldxyz < n) return HM_ERR_ARG - 2;
+  if (leaf < 2 || leaf > 64) return HM_ERR_ARG - 4;
+  if (!(eta > 0.0)) return HM_ERR_ARG - 5;
+  if (!perm || !nblocks) return HM_ERR_ARG - 6;
+  hm::HTreeHost t;
+  hm::htree_partition(n, xyz_p, xyz_p + ldxyz, xyz_p + 2 * (size_t)ldxyz, leaf, eta, t);
+  std::copy(t.perm.begin(), t.perm.end(), perm);
+  *nblocks = (int)t.blocks.size();
+  if (!blocks) return HM_OK;
+  if ((int)t.blocks.size() > max_blocks) return HM_ERR_ARG - 7;
+  for (size_t k = 0; k < t.blocks.size(); ++k) {
+    const hm::HTreeBlock& b = t.blocks[k];
+    int* o = blocks + 5 * k;
+    o[0] = b.rs; o[1] = b.nr; o[2] = b.cs; o[3] = b.nc; o[4] = b.lowrank;
+  }
+  return HM_OK;
+}
+
+// HMATRIX_BUILD(N, AMAT, XYZ_P) (HMatrix.f90:135) with the reference's cluster
+// tree and block partition over the points XYZ_P (column-major n x 3, leading
+// dimension ldxyz), the admissible blocks compressed by cross approximation of
+// the matrix's entries (tolerance and rank cap of hm_set_hmatrix, leaf size
+// and eta of hm_set_cluster_tree); with nblocks > 0 also the block-diagonal
+// FP32 preconditioner of SETUP_BLOCK_PRECONDITIONER (IterativeSolvers.f90:190)
+// over those blocks, nblocks = 0 none (the reference's single-body case)
+extern "C" int hm_hmatrix_build_tree(hm_ctx* c, int n, const double* a, int lda, const double* xyz_p, int ldxyz,
+                                     int nblocks, const int* block_sizes) {
+  if (!c) return HM_ERR_ARG - 1;
+  c->join_side();
+  c->prec_early = false;
+  if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
+  if (!a || lda < n) return c->fail(HM_ERR_ARG - 3, "bad matrix");
+  if (!xyz_p || ldxyz < n) return c->fail(HM_ERR_ARG - 5, "bad coordinates");
+  if (nblocks < 0 || (nblocks > 0 && !block_sizes)) return c->fail(HM_ERR_ARG - 7, "bad preconditioner blocks");
+  std::vector<int> start(1, 0);
+  for (int k = 0; k < nblocks; ++k) {
+    if (block_sizes[k] < 1) return c->fail(HM_ERR_ARG - 8, "block %d is empty", k + 1);
+    start.push_back(start.back() + block_sizes[k]);
+  }
+  if (nblocks > 0 && start.back() != n) return c->fail(HM_ERR_ARG - 8, "block sizes sum to %d, not n = %d", start.back(), n);
+  // the context's matrix buffers are reused: any assembled state is gone
+  c->assembled = c->factored = c->solved = false;
+  c->hd_ready = c->ht_ready = c->ht_from_build = false;
+  const size_t cplx = 2 * sizeof(double);
+  hipStream_t s = c->stream;
+  HIPCHK(c, c->d_A.ensure(cplx * (size_t)n * n), "hmatrix alloc");
+  HIPCHK(c, hipMemcpy2DAsync(c->d_A.p, cplx * n, a, cplx * lda, cplx * n, n, hipMemcpyHostToDevice, s), "hmatrix upload");
+  c->ht_blocks.clear();
+  if (nblocks > 0) {
+    const size_t nmat = (size_t)nblocks;
+    HIPCHK(c, c->d_info.ensure(sizeof(int) * nmat), "hmatrix alloc");
+    HIPCHK(c, c->d_counter.ensure(sizeof(unsigned) * nmat), "hmatrix alloc");
+    HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nmat, s), "hmatrix");
+    HIPCHK(c, hipMemsetAsync(c->d_counter.p, 0, sizeof(unsigned) * nmat, s), "hmatrix");
+    size_t tot = 0;
+    for (int k = 0; k < nblocks; ++k) tot += (size_t)block_sizes[k] * block_sizes[k];
+    HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * tot), "hmatrix alloc");
+    HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)n), "hmatrix alloc");
+    HIPCHK(c, hm::gmres_block_precond_setup(s, c->d_A.as<double>(), n, nblocks, start.data(), c->d_P32.as<float>(),
+                                            c->d_pipiv.as<int>(), c->d_info.as<int>(), c->d_counter.as<unsigned>()),
+           "block preconditioner");
+    std::vector<int> infos(nmat, 0);
+    HIPCHK(c, hipMemcpyAsync(infos.data(), c->d_info.p, sizeof(int) * nmat, hipMemcpyDeviceToHost, s), "hmatrix");
+    HIPCHK(c, hipStreamSynchronize(s), "hmatrix");
+    for (size_t k = 0; k < nmat; ++k)
+      if (infos[k] > 0) return c->fail(start[k] + infos[k], "zero pivot in preconditioner block %zu", k + 1);
+    c->ht_blocks = start;
+  }
+  std::vector<double> x(xyz_p, xyz_p + n), y(xyz_p + ldxyz, xyz_p + ldxyz + n), z(xyz_p + 2 * (size_t)ldxyz, xyz_p + 2 * (size_t)ldxyz + n);
+  int rc = build_tree(c, c->d_A.as<double>(), n, n, x, y, z);
+  if (rc != HM_OK) return rc;
+  c->pinv_ready = false;   // new factors: their permutations (and inverses) follow at the first solve
+  c->ht_from_build = true;
   return HM_OK;
 }
 
@@ -1253,9 +1466,11 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
   if (!b || ldb < n) return c->fail(HM_ERR_ARG - 7, "bad right-hand sides");
   if (!x || ldx < n) return c->fail(HM_ERR_ARG - 8, "bad solution array");
   if (nrhs < 1) return c->fail(HM_ERR_ARG - 10, "nrhs must be >= 1");
+  const bool tree = isolv == 2 && !c->hd_ready && c->ht_from_build;
   if (isolv == 2) {
-    if (!c->hd_ready) return c->fail(HM_ERR_STATE, "isolv = 2 before hm_hmatrix_build");
-    if (c->hd_nb * c->hd_m != n) return c->fail(HM_ERR_ARG - 3, "n = %d differs from the H-matrix's %d", n, c->hd_nb * c->hd_m);
+    if (!c->hd_ready && !tree) return c->fail(HM_ERR_STATE, "isolv = 2 before hm_hmatrix_build or hm_hmatrix_build_tree");
+    const int hn = tree ? c->htree.n() : c->hd_nb * c->hd_m;
+    if (hn != n) return c->fail(HM_ERR_ARG - 3, "n = %d differs from the H-matrix's %d", n, hn);
   }
   std::vector<int> start(1, 0);
   if (isolv == 1) {
@@ -1266,6 +1481,10 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
     }
     if (nblocks > 0 && start.back() != n) return c->fail(HM_ERR_ARG - 12, "block sizes sum to %d, not n = %d", start.back(), n);
     c->hd_ready = false;
+    if (c->ht_from_build) {
+      c->ht_from_build = false;   // its preconditioner factors are about to be overwritten
+      c->ht_ready = false;
+    }
   }
   // both forms overwrite the context's right-hand sides and solution (d_R, d_X):
   // the per-frequency chain must assemble again before hm_solve / hm_forces
@@ -1314,6 +1533,14 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
       for (size_t k = 0; k < nmat; ++k)
         if (infos[k] > 0) return c->fail(start[k] + infos[k], "zero pivot in preconditioner block %zu", k + 1);
       int rc = precond_prepare(c, s);
+      if (rc != HM_OK) return rc;
+    }
+  } else if (tree) {
+    int rc = tree_ops(c, ga, n, nrhs);
+    if (rc != HM_OK) return rc;
+    c->gm_blocks = c->ht_blocks;
+    if (!c->gm_blocks.empty() && !c->pinv_ready) {
+      rc = precond_prepare(c, s);
       if (rc != HM_OK) return rc;
     }
   } else {

@@ -190,6 +190,15 @@ module hm_bem
          complex(c_double_complex), intent(in) :: a(lda, *)
          integer(c_int), intent(in) :: block_sizes(*)
       end function
+      integer(c_int) function hm_hmatrix_build_tree(ctx, n, a, lda, xyz_p, ldxyz, nblocks, block_sizes) &
+            bind(C, name="hm_hmatrix_build_tree")
+         import :: c_ptr, c_int, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n, lda, ldxyz, nblocks
+         complex(c_double_complex), intent(in) :: a(lda, *)
+         real(c_double), intent(in) :: xyz_p(ldxyz, 3)
+         integer(c_int), intent(in) :: block_sizes(*)
+      end function
       integer(c_int) function hm_hmatrix_release(ctx) bind(C, name="hm_hmatrix_release")
          import :: c_ptr, c_int
          type(c_ptr), value :: ctx
@@ -263,7 +272,8 @@ module hm_bem
    public :: CALGREEN, CALGREEN_MULTI, CALGREEN_IRR, CALGREEN_MULTI_IRR, SGLINTBD_QUAD, SGLINTBD_TRI
    public :: BETA, hm_get_normal_equations, hm_sglintbd_batch
    public :: RFORCE_EFORCE_MULTI
-   public :: hm_set_gmres, hm_set_hmatrix, hm_set_pair_symmetry, hm_set_frequency_limit, hm_hmatrix_build, hm_hmatrix_release, &
+   public :: hm_set_gmres, hm_set_hmatrix, hm_set_pair_symmetry, hm_set_frequency_limit, hm_hmatrix_build, &
+             hm_hmatrix_build_tree, hm_hmatrix_release, &
              hm_hmatrix_apply, hm_ziter_solve, hm_batch_create, hm_batch_destroy, hm_batch_set_constants, &
              hm_batch_add_problem, hm_batch_prepare, hm_batch_solve
    public :: ZITER_SOLVE_MULTI, SETUP_BLOCK_PRECONDITIONER, HMATRIX_BUILD, HMATVEC, HMATRIX_DESTROY
@@ -376,15 +386,24 @@ contains
       if (sum(BLOCK_SIZES) /= N .or. size(AMAT, 1) /= N) call HM_CHECK(-112_c_int, 'SETUP_BLOCK_PRECONDITIONER')
    end subroutine
 
-   ! HMATRIX_BUILD (HMatrix.f90:135): clusters are the preconditioner blocks
-   ! (the bodies; equal sizes), the coordinates are not needed
+   ! HMATRIX_BUILD (HMatrix.f90:135): the reference's cluster tree over XYZ_P,
+   ! with the blocks of the last SETUP_BLOCK_PRECONDITIONER of this order as the
+   ! preconditioner (none otherwise: the reference's single body)
    subroutine HMATRIX_BUILD(N, AMAT, XYZ_P)
       integer, intent(in) :: N
       complex(8), intent(in) :: AMAT(N, N)
       real(8), intent(in) :: XYZ_P(N, 3)
-      if (.not. allocated(pblocks_)) call HM_CHECK(-115_c_int, 'HMATRIX_BUILD needs SETUP_BLOCK_PRECONDITIONER first')
-      call HM_CHECK(hm_hmatrix_build(HM_CONTEXT(), N, AMAT, N, size(pblocks_), pblocks_), 'HMATRIX_BUILD')
+      integer(c_int) :: none(1)
+      none = 0
       if (size(XYZ_P, 1) /= N) call HM_CHECK(-103_c_int, 'HMATRIX_BUILD: XYZ_P')
+      if (allocated(pblocks_)) then
+         if (sum(pblocks_) == N) then
+            call HM_CHECK(hm_hmatrix_build_tree(HM_CONTEXT(), N, AMAT, N, XYZ_P, N, size(pblocks_), pblocks_), &
+                          'HMATRIX_BUILD')
+            return
+         end if
+      end if
+      call HM_CHECK(hm_hmatrix_build_tree(HM_CONTEXT(), N, AMAT, N, XYZ_P, N, 0_c_int, none), 'HMATRIX_BUILD')
    end subroutine
 
    subroutine HMATVEC(N, X_IN, Y_OUT)

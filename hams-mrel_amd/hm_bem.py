@@ -37,7 +37,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_hmatrix_apply", "hm_set_frequency_limit", "hm_batch_create", "hm_batch_destroy",
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
-           "hm_surrogate_predict")
+           "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
+           "hm_cluster_tree_partition")
 
 _lib = None
 
@@ -76,6 +77,26 @@ def dispersion(w, h, g_disp, nrt=200):
     if rc:
         raise HMError(rc, "hm_dispersion")
     return wvn
+
+
+def cluster_tree_partition(xyz, leaf=32, eta=2.0):
+    """The cluster tree's permutation and block partition over the points xyz
+    (n x 3), on the host: (perm, blocks) with blocks rows (rs, nr, cs, nc,
+    admissible) in cluster positions"""
+    x = np.asfortranarray(xyz, dtype=np.float64)
+    n = x.shape[0]
+    perm = np.zeros(n, dtype=np.int32)
+    nb = _i(0)
+    L = load()
+    rc = L.hm_cluster_tree_partition(_i(n), _p(x), _i(n), _i(leaf), _d(eta), _p(perm), _i(0), ctypes.byref(nb), None)
+    if rc:
+        raise HMError(rc, "hm_cluster_tree_partition")
+    blocks = np.zeros((nb.value, 5), dtype=np.int32)
+    rc = L.hm_cluster_tree_partition(_i(n), _p(x), _i(n), _i(leaf), _d(eta), _p(perm), _i(nb.value),
+                                     ctypes.byref(nb), _p(blocks))
+    if rc:
+        raise HMError(rc, "hm_cluster_tree_partition")
+    return perm, blocks
 
 
 class Context:
@@ -298,6 +319,28 @@ class Context:
         bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
         self._chk(self.L.hm_hmatrix_build(self.ptr, _i(a.shape[0]), _p(a), _i(a.shape[0]), _i(len(bs)), _p(bs)),
                   "hm_hmatrix_build")
+
+    def hmatrix_build_tree(self, a, xyz, block_sizes=()):
+        """H-matrix of an assembled matrix a (n x n) over the cluster tree of the
+        points xyz (n x 3), with the block-diagonal preconditioner of block_sizes
+        (none if empty)"""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        x = np.asfortranarray(xyz, dtype=np.float64)
+        n = a.shape[0]
+        assert x.shape == (n, 3)
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        self._chk(self.L.hm_hmatrix_build_tree(self.ptr, _i(n), _p(a), _i(n), _p(x), _i(n), _i(len(bs)),
+                                               _p(bs) if len(bs) else None), "hm_hmatrix_build_tree")
+
+    def set_cluster_tree(self, mode=0, leaf=32, eta=2.0):
+        self._chk(self.L.hm_set_cluster_tree(self.ptr, _i(mode), _i(leaf), _d(eta)), "hm_set_cluster_tree")
+
+    def cluster_tree_stats(self):
+        v = [_i() for _ in range(5)]
+        self._chk(self.L.hm_get_cluster_tree_stats(self.ptr, *[ctypes.byref(t) for t in v]),
+                  "hm_get_cluster_tree_stats")
+        return dict(zip(("clusters", "leaves", "dense_blocks", "lowrank_blocks", "dense_fallbacks"),
+                        (t.value for t in v)))
 
     def hmatrix_release(self):
         self._chk(self.L.hm_hmatrix_release(self.ptr), "hm_hmatrix_release")

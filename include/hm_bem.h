@@ -99,9 +99,17 @@ int hm_forces(hm_ctx *ctx, double *amas, double *bdmp, double *exfc);
  * INFO = 1). */
 int hm_set_gmres(hm_ctx *ctx, int restart, double tol, int maxiter, int precond);
 int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, double *rel_residual);
-/* H-matrix form of A and S for arrays of equal bodies without interior
- * waterplane points, replacing HMATRIX_BUILD / HMATVEC (HMatrix.f90:135,:760)
- * as ZITER_SOLVE_MULTI uses them. One cluster per body: the diagonal blocks are
+/* H-matrix form of the system, replacing HMATRIX_BUILD / HMATVEC
+ * (HMatrix.f90:135,:760) as ZITER_SOLVE_MULTI uses them. For two or more bodies
+ * of equal panel counts without interior waterplane points (and kmax no larger
+ * than a body), A and S are compressed as they are assembled; otherwise (one
+ * body, bodies of unequal panel counts, waterplane points, or
+ * hm_set_cluster_tree mode 1) the system matrix (A, or C = A^T A with
+ * waterplane points) is assembled densely and compressed at hm_factor over the
+ * reference's geometric cluster tree (see hm_hmatrix_build_tree), with the
+ * dense right-hand sides, the block-diagonal preconditioner of the dense GMRES
+ * and the FP32 tree products in its Krylov steps; hm_hmatrix_release then
+ * frees it (factor again to solve). The equal-body form: one cluster per body: the diagonal blocks are
  * assembled densely, every off-diagonal block by adaptive cross approximation
  * with partial pivoting (ADD_LOWRANK_BLOCK :560) that evaluates only the rows
  * and columns it samples, stopping at |u_k||v_k| < eps |U V^T|_F or rank kmax
@@ -113,6 +121,17 @@ int hm_get_gmres_stats(hm_ctx *ctx, int *iters, int *cycles, int *converged, dou
  * kmax before eps, ACA steps, mean and largest rank, and stored entries per
  * matrix over n^2, for the last assembly. */
 int hm_set_hmatrix(hm_ctx *ctx, int enable, double eps, double eps_s, int kmax);
+/* The cluster tree of the reference's HMATRIX_BUILD (BUILD_CLUSTER_TREE,
+ * SPLIT_CLUSTER_ITERATIVE, BUILD_BLOCK_TREE_ITERATIVE, HMatrix.f90:203-497):
+ * leaf = NLEAF_DEFAULT (32, at most 64 here), eta = ETA_DEFAULT (2.0, :57-58).
+ * mode 0 (default): hm_set_hmatrix takes the body clusters where they apply;
+ * mode 1: always the tree. hm_get_cluster_tree_stats reports the last tree
+ * H-matrix: tree nodes, leaves, dense and low-rank blocks, and the admissible
+ * blocks stored dense after their cross approximation (rank cap reached, or
+ * rank (nr + nc) >= nr nc as in ADD_LOWRANK_BLOCK :731). */
+int hm_set_cluster_tree(hm_ctx *ctx, int mode, int leaf, double eta);
+int hm_get_cluster_tree_stats(hm_ctx *ctx, int *clusters, int *leaves, int *dense_blocks, int *lowrank_blocks,
+                              int *dense_fallbacks);
 /* No reference counterpart (a scheduling choice of this library): enable = 1
  * (the default) runs the cross approximation on a second stream of the context
  * beside the diagonal blocks and the block preconditioner; 0 runs it after them
@@ -131,7 +150,8 @@ int hm_get_hmatrix_stats(hm_ctx *ctx, int *unconverged, int *steps, double *mean
  * counterpart in the reference (it assembles every pair). */
 int hm_set_pair_symmetry(hm_ctx *ctx, int enable);
 /* y = H x in FP64 for the assembled H-matrix form of A (mat 0) or S (mat 1),
- * or for the H-matrix of hm_hmatrix_build (mat 0); x, y: n x ncol column-major
+ * or for the H-matrix of hm_hmatrix_build / hm_hmatrix_build_tree or of the
+ * tree form's last hm_factor (mat 0); x, y: n x ncol column-major
  * complex (HMATVEC, HMatrix.f90:760, for several columns). */
 int hm_hmatrix_apply(hm_ctx *ctx, int mat, int ncol, const double *x, double *y);
 
@@ -153,6 +173,24 @@ int hm_hmatrix_apply(hm_ctx *ctx, int mat, int ncol, const double *x, double *y)
  * converged, 1 not converged (the reference's INFO). Both reuse the context's
  * matrix buffers: an assembled system is discarded. */
 int hm_hmatrix_build(hm_ctx *ctx, int n, const double *a, int lda, int nblocks, const int *block_sizes);
+/* HMATRIX_BUILD(N, AMAT, XYZ_P) (HMatrix.f90:135) as the reference builds it,
+ * for any matrix: the cluster tree and block partition over the points XYZ_P
+ * (column-major n x 3, leading dimension ldxyz), each admissible block
+ * compressed by cross approximation of a's entries (eps and kmax of
+ * hm_set_hmatrix), each inadmissible pair of leaves stored dense; with
+ * nblocks > 0 the block-diagonal FP32 preconditioner over those blocks
+ * (SETUP_BLOCK_PRECONDITIONER, IterativeSolvers.f90:190), with nblocks = 0 none
+ * (the reference's single body, AssbMatx.f90:282). hm_ziter_solve(isolv = 2)
+ * and hm_hmatrix_apply (mat 0) then use it. */
+int hm_hmatrix_build_tree(hm_ctx *ctx, int n, const double *a, int lda, const double *xyz_p, int ldxyz, int nblocks,
+                          const int *block_sizes);
+/* The cluster tree and block partition hm_hmatrix_build_tree uses, on the host
+ * (no device work): perm[k] = the point (0-based) at cluster position k; per
+ * block 5 ints (first row and rows, first column and columns, in cluster
+ * positions; 1 if admissible, 0 if stored dense). blocks = NULL returns the
+ * block count in *nblocks only. */
+int hm_cluster_tree_partition(int n, const double *xyz_p, int ldxyz, int leaf, double eta, int *perm, int max_blocks,
+                              int *nblocks, int *blocks);
 int hm_hmatrix_release(hm_ctx *ctx);
 int hm_ziter_solve(hm_ctx *ctx, int isolv, int n, const double *a, int lda, const double *b, int ldb, double *x,
                    int ldx, int nrhs, int nblocks, const int *block_sizes, int *info);

@@ -13,11 +13,16 @@
 !
 ! output (stream): complex128 grn(4,npts); real64 sij(nsgl), dij(3,nsgl);
 ! complex128 mxpot(n, 6 nbody + nbeta) (radiation columns, then the diffraction
-! potential of each heading); complex128 x(4) = lua \ (1,2,3,4)^T; int32 info
+! potential of each heading); complex128 x(4) = lua \ (1,2,3,4)^T; int32 info;
+! then the H-matrix route of ISOLV = 2 on the test kernel K(i,i) = 4 + i,
+! K(i,j) = exp(0.3 i r_ij) / r_ij over xyz_p: complex128 y(n) = HMATVEC(K e),
+! e(j) = 1 + i j / n; complex128 z(n,2) = ZITER_SOLVE_MULTI(2, ...) of
+! K z = (e, conj(e)); int32 info of that solve
 program hm_fortran_check
    use hm_bem
+   use iso_c_binding, only: c_int
    implicit none
-   integer :: n, ni, nbody, ntnd, nbeta, npts, nsgl, k, b, ih, m, tn, info, u
+   integer :: n, ni, nbody, ntnd, nbeta, npts, nsgl, k, b, ih, m, tn, info, u, i, j, hinfo
    integer, allocatable :: body_start(:), ncn(:), ncon(:, :), iel(:), jel(:), irr(:), ipiv(:, :), ip4(:)
    real(8), allocatable :: xyz(:, :), xyz_p(:, :), ds(:), pnsz(:), dxyz_p(:, :), ixyz_p(:, :), beta_in(:), pts(:, :)
    real(8), allocatable :: flags(:), sij(:), dij(:, :), wvn(:)
@@ -25,6 +30,9 @@ program hm_fortran_check
    complex(8), allocatable :: grn(:, :), amat(:, :, :), cmat(:, :, :), brmat(:, :, :), drmat(:, :, :)
    complex(8), allocatable :: bdmat(:, :), ddmat(:, :), mxpot(:, :, :), out(:, :)
    complex(8) :: lua(4, 4), x(4)
+   complex(8), allocatable :: kmat(:, :), hb(:, :), hz(:, :), hy(:)
+   integer, allocatable :: bsizes(:)
+   real(8) :: r
    character(len=512) :: fin, fout
 
    call get_command_argument(1, fin)
@@ -103,6 +111,37 @@ program hm_fortran_check
       end do
    end if
 
+   ! the H-matrix route of ISOLV = 2 (RADIATION_SOLVER_MULTI, AssbMatxMulti.f90:283-291:
+   ! SETUP_BLOCK_PRECONDITIONER for several bodies, HMATRIX_BUILD, ZITER_SOLVE_MULTI,
+   ! HMATRIX_DESTROY) on a smooth test kernel over the panel centroids
+   allocate (kmat(n, n), hb(n, 2), hz(n, 2), hy(n), bsizes(nbody))
+   do j = 1, n
+      do i = 1, n
+         if (i == j) then
+            kmat(i, j) = (4.d0, 1.d0)
+         else
+            r = sqrt((xyz_p(i, 1) - xyz_p(j, 1))**2 + (xyz_p(i, 2) - xyz_p(j, 2))**2 + (xyz_p(i, 3) - xyz_p(j, 3))**2)
+            kmat(i, j) = exp(cmplx(0.d0, 0.3d0 * r, kind=8)) / r
+         end if
+      end do
+   end do
+   do j = 1, n
+      hb(j, 1) = cmplx(1.d0, dble(j) / dble(n), kind=8)
+   end do
+   hb(:, 2) = conjg(hb(:, 1))
+   call HM_CHECK(hm_set_hmatrix(HM_CONTEXT(), 0_c_int, 1.d-8, 1.d-8, 50_c_int), 'hm_set_hmatrix')
+   call HM_CHECK(hm_set_gmres(HM_CONTEXT(), 50_c_int, 1.d-10, 500_c_int, -1_c_int), 'hm_set_gmres')
+   if (nbody > 1) then
+      do b = 1, nbody
+         bsizes(b) = body_start(b + 1) - body_start(b)
+      end do
+      call SETUP_BLOCK_PRECONDITIONER(n, kmat, nbody, bsizes)
+   end if
+   call HMATRIX_BUILD(n, kmat, xyz_p)
+   call HMATVEC(n, hb(:, 1), hy)
+   call ZITER_SOLVE_MULTI(2, n, kmat, hb, hz, 2, hinfo)
+   call HMATRIX_DESTROY()
+
    ! LAPACK names
    allocate (ip4(4))
    call ZGETRF(4, 4, lua, 4, ip4, info)
@@ -110,7 +149,7 @@ program hm_fortran_check
    if (info == 0) call ZGETRS('N', 4, 1, lua, 4, ip4, x, 4, info)
 
    open (newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace', action='write')
-   write (u) grn, sij, dij, out, x, info
+   write (u) grn, sij, dij, out, x, info, hy, hz, hinfo
    close (u)
 
 contains

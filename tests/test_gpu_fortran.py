@@ -4,12 +4,15 @@ driver's frequency chain (CALGREEN_MULTI, ASSB_LEFT_MULTI, ASSB_RBC_MULTI per
 body, RADIATION_SOLVER_MULTI, then per heading ASSB_DBC_MULTI and
 DIFFRACTION_SOLVER_MULTI; for one body with irregular-frequency removal
 CALGREEN_IRR, ASSB_LEFT_IRR, ASSB_RBC_IRR, RADIATION_SOLVER_IRR, ASSB_DBC_IRR,
-DIFFRACTION_SOLVER_IRR) and LAPACK's ZGETRF / ZGETRS with the reference's
+DIFFRACTION_SOLVER_IRR), the H-matrix route of ISOLV = 2
+(SETUP_BLOCK_PRECONDITIONER, HMATRIX_BUILD, HMATVEC, ZITER_SOLVE_MULTI,
+HMATRIX_DESTROY) on a smooth test kernel, and LAPACK's ZGETRF / ZGETRS with the reference's
 argument lists (hams-mrel_amd/fortran/hm_bem.f90), against the same quantities
 from the C-ABI through ctypes: the element kernels bit for bit, the potentials
 within 1e-12 (the Fortran chain factors the host copy of AMAT or CMAT through
 ZGETRF, the ctypes chain the device-resident system), the 4 x 4 solve within
-1e-13 of numpy."""
+1e-13 of numpy, the H-matrix product within 20 x its 1e-8 tolerance of the
+dense product and its GMRES solution (tol 1e-10) within 1e-7 of numpy's."""
 import os
 import subprocess
 
@@ -62,8 +65,11 @@ def _read(path, p, npts, nsgl):
     out = take(np.complex128, p.n * ncol).reshape(ncol, p.n).T
     x = take(np.complex128, 4)
     info = int(take(np.int32, 1)[0])
+    hy = take(np.complex128, p.n)
+    hz = take(np.complex128, 2 * p.n).reshape(2, p.n).T
+    hinfo = int(take(np.int32, 1)[0])
     assert off == len(raw)
-    return grn, sij, dij, out, x, info
+    return grn, sij, dij, out, x, info, (hy, hz, hinfo)
 
 
 def _case(tmp_path, p, w1):
@@ -84,7 +90,7 @@ def _case(tmp_path, p, w1):
     print(f"fortran caller: n={p.n} ni={p.ni} nbody={p.nbody} h={p.h} w={w1}", flush=True)
     r = subprocess.run([EXE, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    grn, sij, dij, out, x, info = _read(fout, p, len(pts), len(sgl))
+    grn, sij, dij, out, x, info, (hy, hz, hinfo) = _read(fout, p, len(pts), len(sgl))
     c = hm_bem.Context()
     try:
         c.setup(p)
@@ -111,6 +117,19 @@ def _case(tmp_path, p, w1):
     assert e <= 1e-12
     assert info == 0
     assert relerr(x, np.linalg.solve(lua, np.arange(1, 5))) <= 1e-13
+    # the H-matrix route (HMATRIX_BUILD over the cluster tree at eps 1e-8,
+    # HMATVEC, ZITER_SOLVE_MULTI(2), HMATRIX_DESTROY) on the smooth test kernel
+    xp = np.asarray(p.xyz_p)
+    r = np.sqrt(((xp[:, None, :] - xp[None, :, :]) ** 2).sum(axis=2))
+    np.fill_diagonal(r, 1.0)
+    k = np.exp(0.3j * r) / r
+    np.fill_diagonal(k, 4.0 + 1.0j)
+    e = 1.0 + 1j * np.arange(1, p.n + 1) / p.n
+    ey, ez = relerr(hy, k @ e), relerr(hz, np.linalg.solve(k, np.column_stack([e, np.conj(e)])))
+    print(f"  H-matrix route: HMATVEC {ey:.2e}, ZITER_SOLVE_MULTI(2) {ez:.2e}, info {hinfo}", flush=True)
+    assert hinfo == 0
+    assert ey <= 2e-7
+    assert ez <= 1e-7
 
 
 def test_fortran_multibody_chain(tmp_path):

@@ -134,38 +134,24 @@ def test_toggle_back_to_dense(pair):
 
 
 def test_hmatrix_argument_errors():
+    # geometries the body clusters do not fit take the cluster tree
+    # (tests/test_gpu_htree.py checks their solutions)
     single = _ctx(_array(1, 1, 60.0))
     try:
-        with pytest.raises(hm_bem.HMError, match="two bodies"):
-            single.set_hmatrix(True)
+        single.set_hmatrix(True)
     finally:
         single.close()
-    uneven = meshes.Problem([meshes.cylinder(3.0, 4.0, 16, 4, 2, center=(0.0, 0.0)),
-                             meshes.cylinder(3.0, 4.0, 12, 4, 2, center=(12.0, 0.0))],
-                            h=60.0, omegas=[0.5], beta=np.array([0.0]))
-    c = _ctx(uneven)
-    try:
-        with pytest.raises(hm_bem.HMError, match="equal panel counts"):
-            c.set_hmatrix(True)
-    finally:
-        c.close()
-    irr = meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(0.0, 0.0), waterplane_nr=2),
-                          meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(12.0, 0.0), waterplane_nr=2)],
-                         h=60.0, omegas=[0.5], beta=np.array([0.0]))
-    c = _ctx(irr)
-    try:
-        with pytest.raises(hm_bem.HMError, match="waterplane"):
-            c.set_hmatrix(True)
-    finally:
-        c.close()
     p = _array(2, 1, 60.0)
     c = _ctx(p)
     try:
-        m = p.n // p.nbody
         with pytest.raises(hm_bem.HMError, match="eps"):
             c.set_hmatrix(True, eps=0.0)
         with pytest.raises(hm_bem.HMError, match="kmax"):
-            c.set_hmatrix(True, kmax=m + 1)
+            c.set_hmatrix(True, kmax=0)
+        with pytest.raises(hm_bem.HMError, match="leaf"):
+            c.set_cluster_tree(0, leaf=65)
+        with pytest.raises(hm_bem.HMError, match="no cluster-tree"):
+            c.cluster_tree_stats()
         c.set_hmatrix(True)
         c.set_frequency(0.8)
         c.assemble()
