@@ -348,6 +348,9 @@ def main():
                          "-1 library default")
     ap.add_argument("--series-variant", type=int, default=1, choices=(0, 1, 2),
                     help="epsilon-accelerated regimes: 1 lane groups of 8/16 (default), 2 of 4/8, 0 one pair per lane")
+    ap.add_argument("--lu-panel", choices=("coop", "column"), default="coop",
+                    help="dense LU panels: one cooperative launch each (default) or per-column launches (for "
+                         "profilers that do not handle cooperative launches)")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
     ap.add_argument("--field-points", type=int, default=0,
                     help="also time the potential, pressure and elevation at this many field points after the "
@@ -375,6 +378,7 @@ def main():
     sync()
     print(f"# setup (upload + pair classification) {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     ctx.set_series_variant(args.series_variant)
+    ctx.set_lu_panel(args.lu_panel == "coop")
     if args.assemble_only:
         if args.path == "hmatrix":
             ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)

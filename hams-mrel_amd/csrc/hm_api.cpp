@@ -97,6 +97,8 @@ struct hm_ctx {
   double gm_tol = 1e-6;
   DevBuf d_P32, d_pipiv, d_gwork;
   DevBuf d_luws;              // workspace of the cooperative LU panel kernel
+  bool lu_coop = true;        // dense LU panels by one cooperative launch each where the grid fits
+  void* luws() const { return lu_coop ? d_luws.p : nullptr; }
   DevBuf d_Pinv;              // explicit FP32 inverses of the preconditioner blocks
   DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
   bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
@@ -929,13 +931,13 @@ extern "C" int hm_factor(hm_ctx* c, int precision) {
   } else if (precision == HM_FP64) {
     HIPCHK(c, c->d_luws.ensure(hm::lu_coop_ws_bytes(n)), "factor alloc");
     HIPCHK(c, hm::lu_factor<double>(c->stream, sys_matrix(c), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
-                                    c->d_counter.as<unsigned>(), 1, c->d_luws.p), "lu fp64");
+                                    c->d_counter.as<unsigned>(), 1, c->luws()), "lu fp64");
   } else if (precision == HM_FP32_IR) {
     HIPCHK(c, c->d_A32.ensure(2 * sizeof(float) * (size_t)n * n), "factor alloc");
     HIPCHK(c, hm::demote(c->stream, sys_matrix(c), n, 0, c->d_A32.as<float>(), n, 0, n, n, 1), "demote");
     HIPCHK(c, c->d_luws.ensure(hm::lu_coop_ws_bytes(n)), "factor alloc");
     HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_A32.as<float>(), n, 0, n, c->d_ipiv.as<int>(), c->d_info.as<int>(),
-                                   c->d_counter.as<unsigned>(), 1, c->d_luws.p), "lu fp32");
+                                   c->d_counter.as<unsigned>(), 1, c->luws()), "lu fp32");
   } else {
     // GMRES: FP32 copy for the Krylov products (the cluster-tree form keeps its
     // own FP32 factors); block-diagonal preconditioner (one block per body) by
@@ -1591,6 +1593,13 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
   HIPCHK(c, hipMemcpy2DAsync(x, cplx * ldx, c->d_X.p, cplx * n, cplx * n, nrhs, hipMemcpyDeviceToHost, s), "ziter download");
   HIPCHK(c, hipStreamSynchronize(s), "ziter");
   if (info) *info = c->gm_stats.converged ? 0 : 1;
+  return HM_OK;
+}
+
+extern "C" int hm_set_lu_panel(hm_ctx* c, int cooperative) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (cooperative != 0 && cooperative != 1) return c->fail(HM_ERR_ARG - 2, "cooperative must be 0 or 1");
+  c->lu_coop = cooperative != 0;
   return HM_OK;
 }
 
@@ -2305,7 +2314,7 @@ extern "C" int hm_zgetrf(hm_ctx* c, int m, int n, double* a, int lda, int* ipiv,
   if (!rc) rc = c->hip(hipMemsetAsync(di.p, 0, sizeof(int), c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemsetAsync(dc.p, 0, sizeof(unsigned), c->stream), "zgetrf");
   if (!rc) rc = c->hip(hm::lu_factor<double>(c->stream, da.as<double>(), n, 0, n, dp.as<int>(), di.as<int>(), dc.as<unsigned>(), 1,
-                                                c->d_luws.p), "zgetrf");
+                                                c->luws()), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpy2DAsync(a, cplx * lda, da.p, cplx * n, cplx * n, n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpyAsync(ipiv, dp.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream), "zgetrf");
   if (!rc) rc = c->hip(hipMemcpyAsync(info, di.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "zgetrf");

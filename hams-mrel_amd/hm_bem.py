@@ -38,7 +38,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
-           "hm_cluster_tree_partition")
+           "hm_cluster_tree_partition", "hm_set_lu_panel")
 
 _lib = None
 
@@ -331,6 +331,10 @@ class Context:
         bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
         self._chk(self.L.hm_hmatrix_build_tree(self.ptr, _i(n), _p(a), _i(n), _p(x), _i(n), _i(len(bs)),
                                                _p(bs) if len(bs) else None), "hm_hmatrix_build_tree")
+
+    def set_lu_panel(self, cooperative=True):
+        """dense LU panels: one cooperative launch each (True) or per-column launches"""
+        self._chk(self.L.hm_set_lu_panel(self.ptr, _i(1 if cooperative else 0)), "hm_set_lu_panel")
 
     def set_cluster_tree(self, mode=0, leaf=32, eta=2.0):
         self._chk(self.L.hm_set_cluster_tree(self.ptr, _i(mode), _i(leaf), _d(eta)), "hm_set_cluster_tree")

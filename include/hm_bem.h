@@ -208,6 +208,16 @@ int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, 
  * in a different order and take reciprocals to within two ulps, so they agree
  * with variant 0 to the conditioning of the epsilon algorithm. */
 int hm_set_series_variant(hm_ctx *ctx, int variant);
+/* Panels of the dense LU (ZGETRF's panel factorisation, lu_factor): 1 (the
+ * default) one cooperative launch per 64-column panel where its grid fits the
+ * device (the panel's rows in LDS across workgroups, one grid barrier a
+ * column); 0 the per-column launches (pivot search, scale and rank-1 update
+ * per column; FP32 matrices up to 2048 rows keep their one-launch register
+ * panels either way). Same pivots and results; 0 exists for tools that do not
+ * handle cooperative launches (rocprofv3 counter collection ends the process
+ * with a segmentation fault inside the HIP runtime's cooperative launch). No
+ * counterpart in the reference (MKL's ZGETRF). */
+int hm_set_lu_panel(hm_ctx *ctx, int cooperative);
 
 /* ---- learned surrogate of the evanescent-mode sums ----------------------------
  * Replaces, for 0.05 <= R/h < 0.5 and z, zeta in [-0.25 h, 0], the evanescent part of EIGENE

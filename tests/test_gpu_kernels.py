@@ -213,6 +213,24 @@ def test_zgetrf_zgetrs_vs_textbook(ctx, oracle, n):
     assert relerr(lu, ref) < 1e-12
 
 
+@pytest.mark.parametrize("n", [700, 2500])
+def test_lu_panel_switch(ctx, n):
+    """per-column panels (hm_set_lu_panel 0) against the cooperative panels: the
+    same pivot sequence and the same factors to rounding (both apply the rank-1
+    updates of a panel column by column in the same order)"""
+    rng = np.random.default_rng(7 + n)
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    try:
+        lu_c, piv_c, info_c = ctx.zgetrf(a)
+        ctx.set_lu_panel(False)
+        lu_p, piv_p, info_p = ctx.zgetrf(a)
+    finally:
+        ctx.set_lu_panel(True)
+    assert info_c == info_p == 0
+    np.testing.assert_array_equal(piv_c, piv_p)
+    assert relerr(lu_p, lu_c) < 1e-13
+
+
 def test_zgetrf_reports_zero_pivot(ctx):
     a = np.eye(70, dtype=np.complex128)
     a[:, 33] = 0.0
