@@ -560,7 +560,7 @@ __global__ __launch_bounds__(256) void k_bclassify_count(const BatchProb* __rest
   __syncthreads();
   const GeomDev g = P[p].g;
   const int i = blockIdx.y * 16 + (threadIdx.x >> 4), j = blockIdx.x * 16 + (threadIdx.x & 15);
-  if (i < g.n && j < g.n) atomicAdd(&sh[pair_key(g, F[p].h, i, j)], 1u);
+  if (i < g.nt && j < g.n) atomicAdd(&sh[pair_key(g, F[p].h, i, j)], 1u);
   __syncthreads();
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x)
     if (sh[k]) atomicAdd(&hist[k], sh[k]);
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(256) void k_bclassify_scatter(const BatchProb* __re
   __syncthreads();
   const GeomDev g = P[p].g;
   const int i = blockIdx.y * 16 + (threadIdx.x >> 4), j = blockIdx.x * 16 + (threadIdx.x & 15);
-  const bool in = i < g.n && j < g.n;
+  const bool in = i < g.nt && j < g.n;
   const int key = in ? pair_key(g, F[p].h, i, j) : 0;
   const unsigned rank = in ? atomicAdd(&cnt[key], 1u) : 0u;
   __syncthreads();
@@ -644,8 +644,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
   }
 }
 
-// all pairs of the members in `list` (infinite depth): member ranges of n^2
-// pairs padded to multiples of 64 in `prefix`
+// all pairs of the members in `list` (infinite depth): member ranges of nt n
+// pairs (nt = hull + waterplane field points) padded to multiples of 64 in `prefix`
 template <bool DEEP>
 __global__ __launch_bounds__(256) void k_bassemble_all(const BatchProb* __restrict__ P,
                                                        const BatchFreq* __restrict__ F, const int* __restrict__ list,
@@ -668,9 +668,9 @@ __global__ __launch_bounds__(256) void k_bassemble_all(const BatchProb* __restri
     const int p = list[lo];
     const double* wvn = DEEP ? nullptr : stage_wave_wvn(F, p, swvn);
     const long long e = base + threadIdx.x - prefix[lo];
-    const int n = P[p].g.n;
-    if (wb >= total || e >= (long long)n * n) return;
-    const int i = (int)(e % n), j = (int)(e / n);
+    const int n = P[p].g.n, nt = P[p].g.nt;
+    if (wb >= total || e >= (long long)nt * n) return;
+    const int i = (int)(e % nt), j = (int)(e / nt);
     const FreqDev f = batch_freq(F[p]);
     assemble_pair<DEEP ? -1 : -2>(P[p].g, f, DEEP ? f.wvn : wvn, i, j, P[p].A, P[p].S, P[p].ld);
   }
@@ -693,9 +693,9 @@ __global__ __launch_bounds__(256) void k_bforces(const BatchProb* __restrict__ P
               blockIdx.x, blockIdx.y);
 }
 
-hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int n, unsigned* hist,
+hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int nt, int n, unsigned* hist,
                           unsigned long long* cursor, uint64_t* codes, bool count_only) {
-  const dim3 grid((n + 15) / 16, (n + 15) / 16);
+  const dim3 grid((n + 15) / 16, (nt + 15) / 16);
   if (count_only) hipLaunchKernelGGL(k_bclassify_count, grid, dim3(256), 0, s, P, F, p, hist);
   else hipLaunchKernelGGL(k_bclassify_scatter, grid, dim3(256), 0, s, P, F, p, cursor, codes);
   return hipGetLastError();

@@ -64,8 +64,8 @@ hipError_t forces(hipStream_t s, const GeomDev& g, const FreqDev& f, const doubl
                   double yw, const double* X, int ldx, double* amas, double* bdmp, double* exfc);
 // ---- batches of independent problems (one member = one problem) ----
 struct BatchProb {
-  GeomDev g;                   // the member's geometry (ni = 0)
-  double *A, *S, *W, *R, *X;   // n x n matrices and n x ncol blocks, leading dimension ld = n
+  GeomDev g;                   // the member's geometry (ni waterplane field points after the n hull centroids)
+  double *A, *S, *W, *R, *X;   // nt x n matrices and nt x ncol blocks (nt = n + ni), leading dimension ld = nt
   long long ld;
   int ncol, nbeta;
   const double* beta;
@@ -75,7 +75,7 @@ struct BatchFreq {             // the member's wave state, rewritten per frequen
   double v, w1, wk, h, rho, g, amp, xw, yw;
   double wvn[kNk];
 };
-hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int n, unsigned* hist,
+hipError_t batch_classify(hipStream_t s, const BatchProb* P, const BatchFreq* F, int p, int nt, int n, unsigned* hist,
                           unsigned long long* cursor, uint64_t* codes, bool count_only);
 hipError_t batch_assemble_regime(hipStream_t s, int regime, const BatchProb* P, const BatchFreq* F,
                                  const uint64_t* codes, long long np, int variant);

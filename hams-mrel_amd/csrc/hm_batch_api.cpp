@@ -39,10 +39,12 @@ namespace {
 
 struct Member {
   int n, nbody, nbeta, ncol, m;
+  int ni = 0;                 // interior waterplane field points (irregular-frequency removal)
   double h;
   std::vector<int> body_start;
-  std::vector<double> verts, fp, ds, pnsz, dxyz, beta;
+  std::vector<double> verts, fp, ds, pnsz, dxyz, beta;   // fp: the n hull centroids, then the ni waterplane points
   std::vector<int> ncn;
+  int nt() const { return n + ni; }
 };
 
 constexpr size_t kAlign = 256;
@@ -73,6 +75,9 @@ struct hm_batch {
   int ndeep = 0;
   long long deep_total = 0;
   hm::GemmDesc<double>* d_rhs = nullptr;   // R = S W
+  hm::GemmDesc<double>* d_nrmC = nullptr;  // C = A^T A of the waterplane members
+  hm::GemmDesc<double>* d_nrmD = nullptr;  // D = A^T R of the waterplane members
+  int nirr = 0, nmax_irr = 0, ncolmax_irr = 0, ntmax = 0;
   hm::GemmDesc<double>* d_res = nullptr;   // Rw = Rw - A X
   hm::LuDesc<double>* d_lu64 = nullptr;
   hm::LuDesc<float>* d_lu32 = nullptr;
@@ -203,6 +208,19 @@ extern "C" int hm_batch_add_problem(hm_batch* b, int nbody, const int* body_star
   return b->count() - 1;
 }
 
+extern "C" int hm_batch_set_waterplane(hm_batch* b, int member, int ni, const double* ixyz_p) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (b->prepared) return b->fail(HM_ERR_STATE, "hm_batch_set_waterplane after hm_batch_prepare");
+  if (member < 0 || member >= b->count()) return b->fail(HM_ERR_ARG - 2, "no member %d", member);
+  if (ni < 0 || (ni > 0 && !ixyz_p)) return b->fail(HM_ERR_ARG - 3, "bad waterplane points");
+  Member& m = b->mem[member];
+  if (m.n + ni >= (1 << 20)) return b->fail(HM_ERR_ARG - 3, "n + ni = %d exceeds the 20-bit pair encoding", m.n + ni);
+  m.fp.resize(3 * (size_t)m.n);
+  if (ni) m.fp.insert(m.fp.end(), ixyz_p, ixyz_p + 3 * (size_t)ni);
+  m.ni = ni;
+  return HM_OK;
+}
+
 namespace {
 
 // the per-frequency chain (assembly .. forces) on b->stream, with the current
@@ -218,7 +236,13 @@ int issue_chain(hm_batch* b) {
                                       b->variant), "batch assembly");
   // right-hand sides R = S W
   BCHK(b, hm::batch_build_w(s, b->d_prob, b->d_freq, cnt, b->nmax), "batch rhs");
-  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->nmax, b->ncolmax), "batch rhs gemm");
+  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->ntmax, b->ncolmax), "batch rhs gemm");
+  // waterplane members: least squares through the normal equations with plain
+  // transpose, C = A^T A and D = A^T R (AssbMatx_irr.f90:53-180)
+  if (b->nirr) {
+    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmC, b->nirr, b->nmax_irr, b->nmax_irr), "batch A^T A");
+    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmD, b->nirr, b->nmax_irr, b->ncolmax_irr), "batch A^T R");
+  }
   // factor and solve
   BCHK(b, hm::fill_i32(s, b->d_info, 2LL * cnt, 0), "batch factor");
   if (b->precision == HM_FP64) {
@@ -314,32 +338,43 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   const int cnt = b->count();
   const bool ir = precision == HM_FP32_IR;
   // ---- layout of the pool
-  struct Off { size_t body, fp, verts, ncn, ds, pnsz, dxyz, beta, A, S, W, R, Rw, A32, X32, ipiv, amas, bdmp, exfc; };
+  struct Off { size_t body, fp, verts, ncn, ds, pnsz, dxyz, beta, A, S, W, R, Rw, A32, X32, C, D, ipiv, amas, bdmp, exfc; };
   std::vector<Off> off(cnt);
   size_t tot = 0;
   auto take = [&](size_t bytes) { size_t o = tot; tot += align_up(bytes ? bytes : 8); return o; };
-  b->nmax = b->ncolmax = b->mmax = 0;
+  b->nmax = b->ncolmax = b->mmax = b->ntmax = 0;
+  b->nirr = b->nmax_irr = b->ncolmax_irr = 0;
   b->max_nn = b->max_ncol_elems = 0;
   long long finite_pairs = 0;
   for (int p = 0; p < cnt; ++p) {
     const Member& m = b->mem[p];
-    const size_t n = m.n, nc = m.ncol, mm = m.m;
+    const size_t n = m.n, nt = m.nt(), nc = m.ncol, mm = m.m;
     Off& o = off[p];
     o.body = take(sizeof(int) * (m.nbody + 1));
-    o.fp = take(24 * n); o.verts = take(96 * n); o.ncn = take(4 * n); o.ds = take(8 * n); o.pnsz = take(8 * n);
+    o.fp = take(24 * nt); o.verts = take(96 * n); o.ncn = take(4 * n); o.ds = take(8 * n); o.pnsz = take(8 * n);
     o.dxyz = take(48 * n); o.beta = take(8 * (size_t)m.nbeta);
-    o.A = take(16 * n * n); o.S = take(16 * n * n); o.W = take(16 * n * nc); o.R = take(16 * n * nc);
-    o.Rw = ir ? take(16 * n * nc) : 0;
+    // A, S: nt x n; W, R, Rw: nt x ncol (leading dimension nt for all of them)
+    o.A = take(16 * nt * n); o.S = take(16 * nt * n); o.W = take(16 * nt * nc); o.R = take(16 * nt * nc);
+    o.Rw = ir ? take(16 * nt * nc) : 0;
     o.A32 = ir ? take(8 * n * n) : 0;
     o.X32 = ir ? take(8 * n * nc) : 0;
+    // waterplane members: C = A^T A (n x n), D = A^T R (n x ncol, leading dimension nt)
+    o.C = m.ni ? take(16 * n * n) : 0;
+    o.D = m.ni ? take(16 * nt * nc) : 0;
     o.ipiv = take(4 * n);
+    b->ntmax = m.nt() > b->ntmax ? m.nt() : b->ntmax;
+    if (m.ni) {
+      ++b->nirr;
+      b->nmax_irr = m.n > b->nmax_irr ? m.n : b->nmax_irr;
+      b->ncolmax_irr = m.ncol > b->ncolmax_irr ? m.ncol : b->ncolmax_irr;
+    }
     o.amas = take(8 * mm * mm); o.bdmp = take(8 * mm * mm); o.exfc = take(16 * mm * (size_t)m.nbeta);
     b->nmax = m.n > b->nmax ? m.n : b->nmax;
     b->ncolmax = m.ncol > b->ncolmax ? m.ncol : b->ncolmax;
     b->mmax = m.m > b->mmax ? m.m : b->mmax;
     b->max_nn = (long long)n * n > b->max_nn ? (long long)n * n : b->max_nn;
     b->max_ncol_elems = (long long)n * nc > b->max_ncol_elems ? (long long)n * nc : b->max_ncol_elems;
-    if (m.h > 0.0) finite_pairs += (long long)n * n;
+    if (m.h > 0.0) finite_pairs += (long long)nt * n;
   }
   const size_t o_prob = take(sizeof(hm::BatchProb) * cnt), o_freq = take(sizeof(hm::BatchFreq) * cnt);
   // pair list: every (key, member) run padded to a multiple of 64 entries
@@ -349,6 +384,8 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   const size_t o_codes = take(8 * code_cap);
   const size_t o_deep = take(4 * (size_t)cnt), o_prefix = take(8 * (size_t)(cnt + 1));
   const size_t o_rhs = take(sizeof(hm::GemmDesc<double>) * cnt), o_res = take(sizeof(hm::GemmDesc<double>) * cnt);
+  const size_t o_nrmC = take(sizeof(hm::GemmDesc<double>) * (size_t)b->nirr);
+  const size_t o_nrmD = take(sizeof(hm::GemmDesc<double>) * (size_t)b->nirr);
   const size_t o_lu64 = take(sizeof(hm::LuDesc<double>) * cnt), o_lu32 = take(sizeof(hm::LuDesc<float>) * cnt);
   const size_t o_convA = take(sizeof(hm::ConvDesc) * cnt), o_convR = take(sizeof(hm::ConvDesc) * cnt);
   const size_t o_convX = take(sizeof(hm::ConvDesc) * cnt), o_copyR = take(sizeof(hm::CopyDesc) * cnt);
@@ -366,6 +403,8 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   b->d_deep_prefix = (long long*)at(o_prefix);
   b->d_rhs = (hm::GemmDesc<double>*)at(o_rhs);
   b->d_res = (hm::GemmDesc<double>*)at(o_res);
+  b->d_nrmC = (hm::GemmDesc<double>*)at(o_nrmC);
+  b->d_nrmD = (hm::GemmDesc<double>*)at(o_nrmD);
   b->d_lu64 = (hm::LuDesc<double>*)at(o_lu64);
   b->d_lu32 = (hm::LuDesc<float>*)at(o_lu32);
   b->d_convA = (hm::ConvDesc*)at(o_convA);
@@ -376,7 +415,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   b->d_ratio = (double*)at(o_ratio);
   // ---- descriptors and geometry
   std::vector<hm::BatchProb> prob(cnt);
-  std::vector<hm::GemmDesc<double>> rhs(cnt), res(cnt);
+  std::vector<hm::GemmDesc<double>> rhs(cnt), res(cnt), nrmC, nrmD;
   std::vector<hm::LuDesc<double>> lu64(cnt);
   std::vector<hm::LuDesc<float>> lu32(cnt);
   std::vector<hm::ConvDesc> convA(cnt), convR(cnt), convX(cnt);
@@ -390,9 +429,9 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   for (int p = 0; p < cnt; ++p) {
     const Member& m = b->mem[p];
     const Off& o = off[p];
-    const size_t n = m.n;
+    const size_t n = m.n, nt = m.nt();
     BCHK(b, hipMemcpyAsync(at(o.body), m.body_start.data(), sizeof(int) * (m.nbody + 1), hipMemcpyHostToDevice, b->stream), "batch upload");
-    BCHK(b, hipMemcpyAsync(at(o.fp), m.fp.data(), 24 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(at(o.fp), m.fp.data(), 24 * nt, hipMemcpyHostToDevice, b->stream), "batch upload");
     BCHK(b, hipMemcpyAsync(at(o.verts), m.verts.data(), 96 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
     BCHK(b, hipMemcpyAsync(at(o.ncn), m.ncn.data(), 4 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
     BCHK(b, hipMemcpyAsync(at(o.ds), m.ds.data(), 8 * n, hipMemcpyHostToDevice, b->stream), "batch upload");
@@ -401,7 +440,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     if (m.nbeta)
       BCHK(b, hipMemcpyAsync(at(o.beta), m.beta.data(), 8 * (size_t)m.nbeta, hipMemcpyHostToDevice, b->stream), "batch upload");
     hm::BatchProb& bp = prob[p];
-    bp.g.n = m.n; bp.g.ni = 0; bp.g.nt = m.n; bp.g.nbody = m.nbody;
+    bp.g.n = m.n; bp.g.ni = m.ni; bp.g.nt = m.nt(); bp.g.nbody = m.nbody;
     bp.g.body_start = (const int*)at(o.body);
     bp.g.fp = (const double*)at(o.fp);
     bp.g.xyz_p = (const double*)at(o.fp);
@@ -411,35 +450,48 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     bp.g.pnsz = (const double*)at(o.pnsz);
     bp.g.dxyz = (const double*)at(o.dxyz);
     bp.A = (double*)at(o.A); bp.S = (double*)at(o.S); bp.W = (double*)at(o.W); bp.R = (double*)at(o.R);
-    bp.X = bp.R;   // FP64: the solve overwrites the right-hand sides (IR: set below)
-    bp.ld = m.n;
+    bp.ld = (long long)nt;
     bp.ncol = m.ncol;
     bp.nbeta = m.nbeta;
     bp.beta = (const double*)at(o.beta);
     bp.amas = (double*)at(o.amas); bp.bdmp = (double*)at(o.bdmp); bp.exfc = (double*)at(o.exfc);
     // R = S W
-    rhs[p] = {m.n, m.ncol, m.n, 1.0, 0.0, 0.0, 0.0, bp.S, (long long)n, bp.W, (long long)n, bp.R, (long long)n};
+    rhs[p] = {(int)nt, m.ncol, m.n, 1.0, 0.0, 0.0, 0.0, bp.S, (long long)nt, bp.W, (long long)nt, bp.R, (long long)nt};
+    // the n x n system and its n x ncol right-hand sides: A and R, or with
+    // waterplane points C = A^T A and D = A^T R
+    double* sysA = bp.A;
+    long long lds = (long long)nt;
+    double* sysB = bp.R;
+    if (m.ni) {
+      sysA = (double*)at(o.C);
+      lds = (long long)n;
+      sysB = (double*)at(o.D);
+      nrmC.push_back({m.n, m.n, (int)nt, 1.0, 0.0, 0.0, 0.0, bp.A, (long long)nt, bp.A, (long long)nt, sysA, lds});
+      nrmD.push_back({m.n, m.ncol, (int)nt, 1.0, 0.0, 0.0, 0.0, bp.A, (long long)nt, bp.R, (long long)nt, sysB,
+                      (long long)nt});
+    }
+    bp.X = sysB;   // FP64: the solve overwrites the right-hand sides (IR: set below)
     int* info = b->d_info + 2 * p;
-    lu64[p] = {bp.A, (long long)n, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), bp.R, (long long)n, m.ncol};
+    lu64[p] = {sysA, lds, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), sysB, (long long)nt, m.ncol};
     if (ir) {
-      // IR: R keeps the right-hand sides, X accumulates in W's slot (W is dead
-      // once R = S W is formed), Rw holds residuals, X32 the FP32 corrections
+      // IR: sysB keeps the right-hand sides, X accumulates in W's slot (W is
+      // dead once R = S W is formed), Rw holds residuals, X32 the FP32 corrections
       double* X = bp.W;
       double* Rw = (double*)at(o.Rw);
       float* A32 = (float*)at(o.A32);
       float* X32 = (float*)at(o.X32);
       bp.X = X;
       lu32[p] = {A32, (long long)n, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), X32, (long long)n, m.ncol};
-      convA[p] = {bp.A, (long long)n, A32, (long long)n, m.n, m.n};
+      convA[p] = {sysA, lds, A32, (long long)n, m.n, m.n};
       // demote Rw -> X32; promote X32 -> X (overwrite on the first sweep, add later)
-      convR[p] = {Rw, (long long)n, X32, (long long)n, m.n, m.ncol};
-      convX[p] = {X, (long long)n, X32, (long long)n, m.n, m.ncol};
-      copyR[p] = {bp.R, (long long)n, Rw, (long long)n, m.n, m.ncol};
-      res[p] = {m.n, m.ncol, m.n, -1.0, 0.0, 1.0, 0.0, bp.A, (long long)n, X, (long long)n, Rw, (long long)n};
+      convR[p] = {Rw, (long long)nt, X32, (long long)n, m.n, m.ncol};
+      convX[p] = {X, (long long)nt, X32, (long long)n, m.n, m.ncol};
+      copyR[p] = {sysB, (long long)nt, Rw, (long long)nt, m.n, m.ncol};
+      res[p] = {m.n, m.ncol, m.n, -1.0, 0.0, 1.0, 0.0, sysA, lds, X, (long long)nt, Rw, (long long)nt};
     }
     if (m.h < 0.0) {
       deep.push_back(p);
-      prefix.push_back(prefix.back() + (((long long)n * n + 63) & ~63LL));
+      prefix.push_back(prefix.back() + (((long long)nt * n + 63) & ~63LL));
     }
     b->off_amas[p] = b->out_amas;
     b->off_bdmp[p] = b->out_amas;
@@ -453,6 +505,10 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   BCHK(b, hipMemcpyAsync(b->d_prob, prob.data(), sizeof(hm::BatchProb) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
   BCHK(b, hipMemcpyAsync(b->d_rhs, rhs.data(), sizeof(rhs[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
   BCHK(b, hipMemcpyAsync(b->d_lu64, lu64.data(), sizeof(lu64[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+  if (b->nirr) {
+    BCHK(b, hipMemcpyAsync(b->d_nrmC, nrmC.data(), sizeof(nrmC[0]) * nrmC.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
+    BCHK(b, hipMemcpyAsync(b->d_nrmD, nrmD.data(), sizeof(nrmD[0]) * nrmD.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
+  }
   if (ir) {
     BCHK(b, hipMemcpyAsync(b->d_res, res.data(), sizeof(res[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
     BCHK(b, hipMemcpyAsync(b->d_lu32, lu32.data(), sizeof(lu32[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
@@ -481,7 +537,8 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     if (!(b->mem[p].h > 0.0)) continue;
     hist[p].assign(hm::kNumKeys, 0u);
     BCHK(b, hipMemsetAsync(d_hist, 0, sizeof(unsigned) * hm::kNumKeys, b->stream), "batch classify");
-    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, d_hist, nullptr, nullptr, true), "batch classify");
+    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].nt(), b->mem[p].n, d_hist, nullptr, nullptr,
+                               true), "batch classify");
     BCHK(b, hipMemcpyAsync(hist[p].data(), d_hist, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, b->stream), "batch classify");
     BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
   }
@@ -509,7 +566,8 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   for (int p = 0; p < cnt; ++p) {
     if (hist[p].empty()) continue;
     BCHK(b, hipMemcpyAsync(d_cursor, cur[p].data(), 8 * hm::kNumKeys, hipMemcpyHostToDevice, b->stream), "batch classify");
-    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].n, nullptr, d_cursor, b->d_codes, false), "batch classify");
+    BCHK(b, hm::batch_classify(b->stream, b->d_prob, b->d_freq, p, b->mem[p].nt(), b->mem[p].n, nullptr, d_cursor,
+                               b->d_codes, false), "batch classify");
     BCHK(b, hipStreamSynchronize(b->stream), "batch classify");
   }
   b->grid = (int)((b->deep_total + 255) / 256);

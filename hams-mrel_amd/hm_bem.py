@@ -38,7 +38,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
-           "hm_cluster_tree_partition", "hm_set_lu_panel")
+           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane")
 
 _lib = None
 
@@ -497,8 +497,6 @@ class Batch:
             _d(p0.g_disp if g_disp is None else g_disp), _d(p0.amp if amp is None else amp),
             _d((p0.xw if xw is None else xw)[0]), _d((p0.xw if xw is None else xw)[1])), "hm_batch_set_constants")
         for prob in self.problems:
-            if prob.ni:
-                raise ValueError("batch members take no interior waterplane points")
             keep = [np.ascontiguousarray(prob.body_start, dtype=np.int32),
                     np.ascontiguousarray(prob.xyz, dtype=np.float64),
                     np.ascontiguousarray(prob.ncn, dtype=np.int32),
@@ -513,6 +511,10 @@ class Batch:
                                              _p(keep[7]), _d(prob.h), _i(prob.nbeta), _p(keep[8]))
             if rc < 0:
                 self._chk(rc, "hm_batch_add_problem")
+            if prob.ni:
+                ix = np.ascontiguousarray(prob.ixyz_p, dtype=np.float64)
+                self._chk(self.L.hm_batch_set_waterplane(self.ptr, _i(rc), _i(prob.ni), _p(ix)),
+                          "hm_batch_set_waterplane")
 
     def close(self):
         if self.ptr:

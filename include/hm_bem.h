@@ -287,7 +287,12 @@ int hm_surrogate_predict(hm_ctx *ctx, int n, const float *x, float *u);
  * the right-hand sides, the LU and the triangular solves, optionally captured
  * once in a HIP graph and replayed per frequency.
  * hm_batch_add_problem takes one member (arguments as hm_set_geometry,
- * hm_set_depth and hm_set_headings, ni = 0) and returns its index >= 0.
+ * hm_set_depth and hm_set_headings) and returns its index >= 0;
+ * hm_batch_set_waterplane gives that member ni interior waterplane field points
+ * (ixyz_p, 3 per point, as hm_set_geometry's): it then solves with
+ * irregular-frequency removal, the least squares of ASSB_LEFT_IRR /
+ * RADIATION_SOLVER_IRR (AssbMatx_irr.f90:53-180, :472) through the normal
+ * equations C = A^T A, D = A^T B (plain transpose), as hm_assemble does.
  * hm_batch_prepare(precision HM_FP64 or HM_FP32_IR, autotune, use_graph,
  * nrefine): allocates, classifies, times the series variants (lane-group
  * shapes) of the assembly when autotune != 0 and keeps the fastest, and
@@ -308,6 +313,7 @@ int hm_batch_set_constants(hm_batch *batch, double rho, double g, double g_disp,
 int hm_batch_add_problem(hm_batch *batch, int nbody, const int *body_start, int ntnd, const double *xyz,
                          const int *ncn, const int *ncon, const double *xyz_p, const double *ds, const double *pnsz,
                          const double *dxyz_p, double h, int nbeta, const double *beta);
+int hm_batch_set_waterplane(hm_batch *batch, int member, int ni, const double *ixyz_p);
 int hm_batch_prepare(hm_batch *batch, int precision, int autotune, int use_graph, int nrefine);
 int hm_batch_solve(hm_batch *batch, const double *w1, double *amas, double *bdmp, double *exfc);
 int hm_batch_info(hm_batch *batch, int *members, int *nmax, int *series_variant, int *grid_blocks,

@@ -8,7 +8,8 @@ variant its FP64 results are compared with the single-problem FP64 results at
 in the two kernels). HIP graph replays are compared bit for bit with the same
 chain issued eagerly, three frequencies in a row with eager work in between.
 FP32 LU + FP64 refinement is held to 1e-10 of FP64 (it refines until the last
-correction is below 1e-13 of the solution).
+correction is below 1e-13 of the solution). Members with interior waterplane
+points (irregular-frequency removal) are held to the same bars.
 """
 import numpy as np
 import pytest
@@ -122,7 +123,50 @@ def test_batch_argument_errors():
             b._chk(rc, "hm_batch_set_constants")
     finally:
         b.close()
-    irr = meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, waterplane_nr=2)], h=-1.0, omegas=[1.0],
-                         beta=np.array([0.0]))
-    with pytest.raises(ValueError, match="waterplane"):
-        hm_bem.Batch([irr])
+    b = hm_bem.Batch(probs)
+    try:
+        rc = b.L.hm_batch_set_waterplane(b.ptr, hm_bem._i(7), hm_bem._i(0), None)
+        with pytest.raises(hm_bem.HMError, match="no member"):
+            b._chk(rc, "hm_batch_set_waterplane")
+    finally:
+        b.close()
+
+
+def _waterplane_members():
+    """members with interior waterplane points (irregular-frequency removal) among plain ones"""
+    beta = np.array([0.0, np.pi / 4])
+    return [
+        meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, waterplane_nr=2)], h=-1.0, omegas=[1.0], beta=beta),
+        meshes.Problem([meshes.cylinder(4.0, 3.0, 16, 3, 2)], h=50.0, omegas=[1.0], beta=beta),
+        meshes.Problem([meshes.cylinder(5.0, 5.0, 16, 4, 3, waterplane_nr=3)], h=70.0, omegas=[1.0], beta=beta[:1]),
+        meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(0.0, 0.0), waterplane_nr=2),
+                        meshes.cylinder(3.0, 4.0, 12, 3, 2, center=(12.0, 0.0), waterplane_nr=2)],
+                       h=60.0, omegas=[1.0], beta=beta),
+    ]
+
+
+def test_batch_waterplane_members():
+    """members with waterplane points solve through the normal equations C =
+    A^T A, D = A^T B in the batch as in a single context (the same kernels:
+    1e-13 in FP64), and FP32 LU + refinement on C stays within 1e-10 of FP64"""
+    probs = _waterplane_members()
+    assert [p.ni > 0 for p in probs] == [True, False, True, True]
+    b = hm_bem.Batch(probs)
+    b32 = hm_bem.Batch(probs)
+    try:
+        b.prepare(hm_bem.HM_FP64, autotune=False, use_graph=True)
+        b32.prepare(hm_bem.HM_FP32_IR, autotune=False, use_graph=True, nrefine=3)
+        print("waterplane batch", b.info(), [(p.n, p.ni, p.h) for p in probs], flush=True)
+        for w in (0.6, 1.7):
+            got = b.solve(w)
+            got32 = b32.solve(w)
+            for k, prob in enumerate(probs):
+                ref = _single(prob, w, 1)
+                for key in KEYS:
+                    e = relerr(got[k][key], ref[key])
+                    e32 = relerr(got32[k][key], got[k][key])
+                    assert e < 1e-13, (k, w, key, e)
+                    assert e32 < 1e-10, (k, w, key, e32, b32.info())
+    finally:
+        b.close()
+        b32.close()
