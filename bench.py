@@ -261,7 +261,8 @@ def bench_batch(args):
     b.prepare(prec, autotune=True, use_graph=True, nrefine=3)
     sync()
     info = b.info()
-    print(f"# batch prepared in {time.perf_counter() - t0:.1f} s: {info}", file=sys.stderr, flush=True)
+    tuning = b.tuning()
+    print(f"# batch prepared in {time.perf_counter() - t0:.1f} s: {info}; tuning {tuning}", file=sys.stderr, flush=True)
     for s in range(args.warmup):
         b.solve(float(omegas[s % len(omegas)]))
     sync()
@@ -314,10 +315,11 @@ def bench_batch(args):
                        "panels_min": min(p.n for p in probs), "panels_max": max(p.n for p in probs),
                        "headings": probs[0].nbeta, "frequencies": len(omegas),
                        "path": ("batch: one assembly launch per regime over all members' pairs, grouped LU and "
-                                "solves, one HIP graph replay per frequency, autotuned series variant; "
+                                "solves, one HIP graph replay per frequency, autotuned series variant and GEMM "
+                                "tile K chunk; "
                                 + ("FP64 LU" if prec == hm_bem.HM_FP64 else "FP32 LU + FP64 refinement")),
                        "parallelism": "1 GPU"},
-            "batch": info, "sequential_fp64": plain}
+            "batch": info, "tuning": tuning, "sequential_fp64": plain}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -19,8 +19,9 @@
 //     rewritten before each replay).
 // An autotuner times the assembly launches of one frequency for the series
 // variants (the lane-group shapes G x K of the epsilon-accelerated regimes:
-// one pair per lane, 16 x 7 / 8 x 7, 8 x 13 / 4 x 13) and keeps the fastest
-// before the graph is captured.
+// one pair per lane, 16 x 7 / 8 x 7, 8 x 13 / 4 x 13) and the grouped LU for
+// the K chunk of its MFMA GEMM tiles (8, 16, 32), and keeps the fastest of
+// each before the graph is captured.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -62,6 +63,8 @@ struct hm_batch {
   // tuning
   int variant = 1, grid = 0, cus = 256;   // grid: assembly workgroups per frequency
   double tuned_ms = 0.0;
+  int gemm_kb = 16;                        // K chunk of the grouped GEMM tiles (8, 16, 32)
+  double tuned_lu_ms[3] = {0.0, 0.0, 0.0}; // the grouped LU at K chunks 8, 16, 32
   // device
   void* pool = nullptr;
   size_t pool_bytes = 0;
@@ -236,25 +239,28 @@ int issue_chain(hm_batch* b) {
                                       b->variant), "batch assembly");
   // right-hand sides R = S W
   BCHK(b, hm::batch_build_w(s, b->d_prob, b->d_freq, cnt, b->nmax), "batch rhs");
-  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->ntmax, b->ncolmax), "batch rhs gemm");
+  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_rhs, cnt, b->ntmax, b->ncolmax, b->gemm_kb), "batch rhs gemm");
   // waterplane members: least squares through the normal equations with plain
   // transpose, C = A^T A and D = A^T R (AssbMatx_irr.f90:53-180)
   if (b->nirr) {
-    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmC, b->nirr, b->nmax_irr, b->nmax_irr), "batch A^T A");
-    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmD, b->nirr, b->nmax_irr, b->ncolmax_irr), "batch A^T R");
+    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmC, b->nirr, b->nmax_irr, b->nmax_irr, b->gemm_kb),
+         "batch A^T A");
+    BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmD, b->nirr, b->nmax_irr, b->ncolmax_irr, b->gemm_kb),
+         "batch A^T R");
   }
   // factor and solve
   BCHK(b, hm::fill_i32(s, b->d_info, 2LL * cnt, 0), "batch factor");
   if (b->precision == HM_FP64) {
-    BCHK(b, hm::lu_factor_grouped<double>(s, b->d_lu64, cnt, b->nmax), "batch lu");
+    BCHK(b, hm::lu_factor_grouped<double>(s, b->d_lu64, cnt, b->nmax, b->gemm_kb), "batch lu");
     BCHK(b, hm::lu_solve_grouped<double>(s, b->d_lu64, cnt, b->nmax, b->ncolmax), "batch solve");
   } else {
     BCHK(b, hm::convert_grouped(s, b->d_convA, cnt, b->max_nn, true, false), "batch demote");
-    BCHK(b, hm::lu_factor_grouped<float>(s, b->d_lu32, cnt, b->nmax), "batch lu fp32");
+    BCHK(b, hm::lu_factor_grouped<float>(s, b->d_lu32, cnt, b->nmax, b->gemm_kb), "batch lu fp32");
     for (int it = 0; it < b->nrefine; ++it) {
       // residual Rw = R - A X (the first sweep has X = 0: Rw = R)
       BCHK(b, hm::copy_grouped(s, b->d_copyR, cnt, b->max_ncol_elems), "batch residual");
-      if (it > 0) BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax), "batch residual");
+      if (it > 0)
+        BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax, b->gemm_kb), "batch residual");
       BCHK(b, hm::convert_grouped(s, b->d_convR, cnt, b->max_ncol_elems, true, false), "batch demote");
       BCHK(b, hm::lu_solve_grouped<float>(s, b->d_lu32, cnt, b->nmax, b->ncolmax), "batch solve fp32");
       BCHK(b, hm::convert_grouped(s, b->d_convX, cnt, b->max_ncol_elems, false, it == 0), "batch update");
@@ -269,7 +275,7 @@ int extra_sweep(hm_batch* b) {
   hipStream_t s = b->stream;
   const int cnt = b->count();
   BCHK(b, hm::copy_grouped(s, b->d_copyR, cnt, b->max_ncol_elems), "batch residual");
-  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax), "batch residual");
+  BCHK(b, hm::cgemm_grouped<double>(s, false, b->d_res, cnt, b->nmax, b->ncolmax, b->gemm_kb), "batch residual");
   BCHK(b, hm::convert_grouped(s, b->d_convR, cnt, b->max_ncol_elems, true, false), "batch demote");
   BCHK(b, hm::lu_solve_grouped<float>(s, b->d_lu32, cnt, b->nmax, b->ncolmax), "batch solve fp32");
   BCHK(b, hm::convert_grouped(s, b->d_convX, cnt, b->max_ncol_elems, false, false), "batch update");
@@ -324,7 +330,64 @@ int time_assembly(hm_batch* b, double* ms) {
   return rc;
 }
 
+// time the grouped LU of the members' systems of one frequency for each K
+// chunk of the GEMM tiles and keep the fastest. The systems are assembled
+// first (with the normal equations of the waterplane members); the FP64 LU
+// factors them in place, so each FP64 trial assembles again, while the FP32
+// trials factor fresh FP32 copies.
+int tune_lu(hm_batch* b) {
+  hipStream_t s = b->stream;
+  const int cnt = b->count();
+  const int kbs[3] = {8, 16, 32};
+  hipEvent_t e0, e1;
+  BCHK(b, hipEventCreate(&e0), "tune");
+  BCHK(b, hipEventCreate(&e1), "tune");
+  int rc = HM_OK;
+  double best = 1e30;
+  int bk = 16;
+  auto assemble = [&]() -> int {
+    BCHK(b, hm::batch_assemble_all(s, true, b->d_prob, b->d_freq, b->d_deep, b->d_deep_prefix, b->ndeep,
+                                   b->deep_total), "tune");
+    for (int r = 0; r < 4; ++r)
+      BCHK(b, hm::batch_assemble_regime(s, r, b->d_prob, b->d_freq, b->d_codes + b->seg_off[r], b->seg_cnt[r],
+                                        b->variant), "tune");
+    if (b->nirr) BCHK(b, hm::cgemm_grouped<double>(s, true, b->d_nrmC, b->nirr, b->nmax_irr, b->nmax_irr), "tune");
+    return HM_OK;
+  };
+  if (b->precision == HM_FP32_IR) rc = assemble();
+  for (int t = 0; t < 3 && rc == HM_OK; ++t) {
+    if (b->precision == HM_FP64) rc = assemble();
+    else rc = b->hip(hm::convert_grouped(s, b->d_convA, cnt, b->max_nn, true, false), "tune");
+    if (rc == HM_OK) rc = b->hip(hm::fill_i32(s, b->d_info, 2LL * cnt, 0), "tune");
+    if (rc != HM_OK) break;
+    (void)hipEventRecord(e0, s);
+    rc = b->hip(b->precision == HM_FP64 ? hm::lu_factor_grouped<double>(s, b->d_lu64, cnt, b->nmax, kbs[t])
+                                        : hm::lu_factor_grouped<float>(s, b->d_lu32, cnt, b->nmax, kbs[t]), "tune");
+    (void)hipEventRecord(e1, s);
+    if (rc == HM_OK) rc = b->hip(hipEventSynchronize(e1), "tune");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    b->tuned_lu_ms[t] = ms;
+    if (ms < best) { best = ms; bk = kbs[t]; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == HM_OK) b->gemm_kb = bk;
+  return rc;
+}
+
 }  // namespace
+
+extern "C" int hm_batch_tuning(hm_batch* b, int* series_variant, int* gemm_kchunk, double* assembly_ms,
+                               double* lu_ms) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (series_variant) *series_variant = b->variant;
+  if (gemm_kchunk) *gemm_kchunk = b->gemm_kb;
+  if (assembly_ms) *assembly_ms = b->tuned_ms;
+  if (lu_ms)
+    for (int t = 0; t < 3; ++t) lu_ms[t] = b->tuned_lu_ms[t];
+  return HM_OK;
+}
 
 extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int use_graph, int nrefine) {
   if (!b) return HM_ERR_ARG - 1;
@@ -572,9 +635,12 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   }
   b->grid = (int)((b->deep_total + 255) / 256);
   for (int r = 0; r < 4; ++r) b->grid += (int)((b->seg_cnt[r] + 255) / 256);
-  // ---- autotuning on the middle of the frequency range (1 rad/s)
+  // ---- autotuning on the middle of the frequency range (1 rad/s): the series
+  // variant of the assembly, then the K chunk of the grouped LU's GEMM tiles
   b->variant = 1;
   b->tuned_ms = 0.0;
+  b->gemm_kb = 16;
+  for (double& t : b->tuned_lu_ms) t = 0.0;
   if (autotune) {
     std::vector<double> w(cnt, 1.0);
     int rc = write_freq(b, w.data());
@@ -589,6 +655,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     }
     b->variant = bv;
     b->tuned_ms = best;
+    if ((rc = tune_lu(b))) return rc;
   }
   // ---- graph
   if (b->use_graph) {

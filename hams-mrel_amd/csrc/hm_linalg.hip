@@ -50,12 +50,16 @@ __device__ __forceinline__ vec4<float>::type mfma16(float a, float b, vec4<float
 constexpr int kBM = 64, kBN = 64, kBK = 16;
 
 // One 64 x 64 tile (rows row0.., columns col0..) of
-// C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C.
-template <typename T, bool TA>
+// C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C, K staged through LDS
+// in chunks of KB (8, 16 or 32; 16 by default, the grouped factorisation's
+// autotuner picks among them).
+template <typename T, bool TA, int KB = kBK>
 __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, const T* __restrict__ A, int lda,
                                            const T* __restrict__ B, int ldb, T br, T bi, T* __restrict__ C,
                                            int ldc, int row0, int col0) {
-  __shared__ T as_re[kBK][kBM + 1], as_im[kBK][kBM + 1], bs_re[kBK][kBN + 1], bs_im[kBK][kBN + 1];
+  static_assert(KB == 8 || KB == 16 || KB == 32, "K chunk");
+  constexpr int Q = KB / 4;            // elements of each operand chunk per thread
+  __shared__ T as_re[KB][kBM + 1], as_im[KB][kBM + 1], bs_re[KB][kBN + 1], bs_im[KB][kBN + 1];
   typedef typename vec4<T>::type v4;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -64,16 +68,16 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
     for (int j = 0; j < 2; ++j)
       for (int v = 0; v < 4; ++v) { cre[i][j][v] = 0; cim[i][j][v] = 0; }
 
-  // op(A)[row0:+64, k0:+16] and B[k0:+16, col0:+64] go through registers into
+  // op(A)[row0:+64, k0:+KB] and B[k0:+KB, col0:+64] go through registers into
   // LDS; the next chunk's global loads are issued before the current chunk's
   // MFMAs, so their latency hides behind the arithmetic
-  T pa_r[4], pa_i[4], pb_r[4], pb_i[4];
+  T pa_r[Q], pa_i[Q], pb_r[Q], pb_i[Q];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       int r, kk;
       if (!TA) { r = tid & 63; kk = (tid >> 6) + 4 * q; }
-      else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      else { kk = tid % KB; r = tid / KB + (256 / KB) * q; }
       const int gr = row0 + r, gk = k0 + kk;
       pa_r[q] = 0;
       pa_i[q] = 0;
@@ -82,7 +86,7 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
         pa_r[q] = A[2 * off];
         pa_i[q] = A[2 * off + 1];
       }
-      const int bk = tid & 15, bc = (tid >> 4) + 16 * q;
+      const int bk = tid % KB, bc = tid / KB + (256 / KB) * q;
       const int gbk = k0 + bk, gbc = col0 + bc;
       pb_r[q] = 0;
       pb_i[q] = 0;
@@ -95,13 +99,13 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       int r, kk;
       if (!TA) { r = tid & 63; kk = (tid >> 6) + 4 * q; }
-      else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      else { kk = tid % KB; r = tid / KB + (256 / KB) * q; }
       as_re[kk][r] = pa_r[q];
       as_im[kk][r] = pa_i[q];
-      const int bk = tid & 15, bc = (tid >> 4) + 16 * q;
+      const int bk = tid % KB, bc = tid / KB + (256 / KB) * q;
       bs_re[bk][bc] = pb_r[q];
       bs_im[bk][bc] = pb_i[q];
     }
@@ -111,10 +115,10 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
     stash();
   }
   __syncthreads();
-  for (int k0 = 0; k0 < k; k0 += kBK) {
-    const bool more = k0 + kBK < k;
-    if (more) fetch(k0 + kBK);
-    for (int k4 = 0; k4 < kBK; k4 += 4) {
+  for (int k0 = 0; k0 < k; k0 += KB) {
+    const bool more = k0 + KB < k;
+    if (more) fetch(k0 + KB);
+    for (int k4 = 0; k4 < KB; k4 += 4) {
       const int kr = k4 + (lane >> 4), li = lane & 15;
       T a_re[2], a_im[2], b_re[2], b_im[2];
       for (int s = 0; s < 2; ++s) {
@@ -168,13 +172,13 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, 
 }
 
 // blockIdx.z = group member; tiles outside the member's m x n exit
-template <typename T, bool TA>
+template <typename T, bool TA, int KB>
 __global__ __launch_bounds__(256) void k_cgemm_grouped(const GemmDesc<T>* __restrict__ d) {
   const GemmDesc<T> g = d[blockIdx.z];
   const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
   if (row0 >= g.m || col0 >= g.n || g.k <= 0) return;
-  cgemm_tile<T, TA>(g.m, g.n, g.k, g.ar, g.ai, g.A, (int)g.lda, g.B, (int)g.ldb, g.br, g.bi, g.C, (int)g.ldc, row0,
-                    col0);
+  cgemm_tile<T, TA, KB>(g.m, g.n, g.k, g.ar, g.ai, g.A, (int)g.lda, g.B, (int)g.ldb, g.br, g.bi, g.C, (int)g.ldc,
+                        row0, col0);
 }
 
 template <typename T>
@@ -191,11 +195,16 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
 }
 
 template <typename T>
-hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax) {
+hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax, int kb) {
   if (mmax <= 0 || nmax <= 0 || count <= 0) return hipSuccess;
   dim3 grid((mmax + kBM - 1) / kBM, (nmax + kBN - 1) / kBN, count);
-  if (transA) hipLaunchKernelGGL((k_cgemm_grouped<T, true>), grid, dim3(256), 0, s, d);
-  else hipLaunchKernelGGL((k_cgemm_grouped<T, false>), grid, dim3(256), 0, s, d);
+#define HM_GG(TA, KB) hipLaunchKernelGGL((k_cgemm_grouped<T, TA, KB>), grid, dim3(256), 0, s, d)
+  if (transA) {
+    if (kb == 8) HM_GG(true, 8); else if (kb == 32) HM_GG(true, 32); else HM_GG(true, 16);
+  } else {
+    if (kb == 8) HM_GG(false, 8); else if (kb == 32) HM_GG(false, 32); else HM_GG(false, 16);
+  }
+#undef HM_GG
   return hipGetLastError();
 }
 
@@ -1011,20 +1020,20 @@ __global__ __launch_bounds__(256) void k_lu_trsm_g(const LuDesc<T>* __restrict__
 }
 
 // trailing update A22 -= A21 A12
-template <typename T>
+template <typename T, int KB>
 __global__ __launch_bounds__(256) void k_lu_update_g(const LuDesc<T>* __restrict__ d, int k0) {
   const LuDesc<T> g = d[blockIdx.z];
   const int k1 = imin(k0 + kLuNb, g.n), nr = g.n - k1;
   const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
   if (nr <= 0 || row0 >= nr || col0 >= nr) return;
   const long long lda = g.lda;
-  cgemm_tile<T, false>(nr, nr, k1 - k0, T(-1), T(0), g.A + 2 * ((long long)k1 + (long long)k0 * lda), (int)lda,
+  cgemm_tile<T, false, KB>(nr, nr, k1 - k0, T(-1), T(0), g.A + 2 * ((long long)k1 + (long long)k0 * lda), (int)lda,
                        g.A + 2 * ((long long)k0 + (long long)k1 * lda), (int)lda, T(1), T(0),
                        g.A + 2 * ((long long)k1 + (long long)k1 * lda), (int)lda, row0, col0);
 }
 
 template <typename T>
-hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax) {
+hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax, int kb) {
   if (count <= 0 || nmax <= 0) return hipSuccess;
   for (int k0 = 0; k0 < nmax; k0 += kLuNb) {
     const int k1 = k0 + kLuNb < nmax ? k0 + kLuNb : nmax;
@@ -1038,8 +1047,10 @@ hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int n
     if (k1 < nmax) {
       const int nc = nmax - k1;
       hipLaunchKernelGGL((k_lu_trsm_g<T>), dim3((nc + 3) / 4, 1, count), dim3(256), 0, s, d, k0);
-      hipLaunchKernelGGL((k_lu_update_g<T>), dim3((nc + kBM - 1) / kBM, (nc + kBN - 1) / kBN, count), dim3(256), 0,
-                         s, d, k0);
+      const dim3 ug((nc + kBM - 1) / kBM, (nc + kBN - 1) / kBN, count);
+      if (kb == 8) hipLaunchKernelGGL((k_lu_update_g<T, 8>), ug, dim3(256), 0, s, d, k0);
+      else if (kb == 32) hipLaunchKernelGGL((k_lu_update_g<T, 32>), ug, dim3(256), 0, s, d, k0);
+      else hipLaunchKernelGGL((k_lu_update_g<T, 16>), ug, dim3(256), 0, s, d, k0);
     }
   }
   return hipGetLastError();
@@ -1221,10 +1232,10 @@ template hipError_t cgemm<double>(hipStream_t, bool, int, int, int, double, doub
                                   const double*, int, long long, double, double, double*, int, long long, int);
 template hipError_t cgemm<float>(hipStream_t, bool, int, int, int, float, float, const float*, int, long long,
                                  const float*, int, long long, float, float, float*, int, long long, int);
-template hipError_t cgemm_grouped<double>(hipStream_t, bool, const GemmDesc<double>*, int, int, int);
-template hipError_t cgemm_grouped<float>(hipStream_t, bool, const GemmDesc<float>*, int, int, int);
-template hipError_t lu_factor_grouped<double>(hipStream_t, const LuDesc<double>*, int, int);
-template hipError_t lu_factor_grouped<float>(hipStream_t, const LuDesc<float>*, int, int);
+template hipError_t cgemm_grouped<double>(hipStream_t, bool, const GemmDesc<double>*, int, int, int, int);
+template hipError_t cgemm_grouped<float>(hipStream_t, bool, const GemmDesc<float>*, int, int, int, int);
+template hipError_t lu_factor_grouped<double>(hipStream_t, const LuDesc<double>*, int, int, int);
+template hipError_t lu_factor_grouped<float>(hipStream_t, const LuDesc<float>*, int, int, int);
 template hipError_t lu_solve_grouped<double>(hipStream_t, const LuDesc<double>*, int, int, int);
 template hipError_t lu_solve_grouped<float>(hipStream_t, const LuDesc<float>*, int, int, int);
 template hipError_t lu_factor<double>(hipStream_t, double*, int, long long, int, int*, int*, unsigned*, int, void*);

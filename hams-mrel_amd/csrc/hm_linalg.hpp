@@ -76,10 +76,11 @@ struct CopyDesc {            // FP64 m x n block copy
 hipError_t copy_grouped(hipStream_t s, const CopyDesc* d, int count, long long max_elems);
 // out[z] = max |correction (f32)| / max |solution (f64)| of member z
 hipError_t correction_ratio_grouped(hipStream_t s, const ConvDesc* d, int count, double* out);
+// kb: the GEMM tiles' K chunk (8, 16 or 32), the shape the batch autotuner picks
 template <typename T>
-hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax);
+hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax, int kb = 16);
 template <typename T>
-hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax);
+hipError_t lu_factor_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax, int kb = 16);
 template <typename T>
 hipError_t lu_solve_grouped(hipStream_t s, const LuDesc<T>* d, int count, int nmax, int nrhs_max);
 hipError_t convert_grouped(hipStream_t s, const ConvDesc* d, int count, long long max_elems, bool to_single,

@@ -38,7 +38,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
-           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane")
+           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning")
 
 _lib = None
 
@@ -554,6 +554,15 @@ class Batch:
             oa += m * m
             oe += m * p.nbeta
         return out
+
+    def tuning(self):
+        """the autotuner's choices: series variant, K chunk of the grouped GEMM
+        tiles, assembly time, grouped LU time at K chunks 8, 16, 32"""
+        var, kb, ams = _i(), _i(), _d()
+        lu = np.zeros(3)
+        self._chk(self.L.hm_batch_tuning(self.ptr, ctypes.byref(var), ctypes.byref(kb), ctypes.byref(ams), _p(lu)),
+                  "hm_batch_tuning")
+        return dict(series_variant=var.value, gemm_kchunk=kb.value, assembly_ms=ams.value, lu_ms=lu.tolist())
 
     def info(self):
         mem, nmax, var, grid, es = _i(), _i(), _i(), _i(), _i()

@@ -318,6 +318,12 @@ int hm_batch_prepare(hm_batch *batch, int precision, int autotune, int use_graph
 int hm_batch_solve(hm_batch *batch, const double *w1, double *amas, double *bdmp, double *exfc);
 int hm_batch_info(hm_batch *batch, int *members, int *nmax, int *series_variant, int *grid_blocks,
                   double *tuned_assembly_ms, double *last_correction, int *extra_sweeps, long long *device_bytes);
+/* What hm_batch_prepare's autotuner chose: the series variant, the K chunk of
+ * the grouped GEMM tiles (8, 16 or 32 complex columns staged through LDS per
+ * step; default 16 without autotuning), the assembly time of the chosen
+ * variant and the grouped LU's time at K chunks 8, 16, 32 (lu_ms[3]; zeros
+ * without autotuning). No counterpart in the reference. */
+int hm_batch_tuning(hm_batch *batch, int *series_variant, int *gemm_kchunk, double *assembly_ms, double *lu_ms);
 
 /* Stage timing: when enabled, a pair of HIP events is recorded on the context's
  * stream around each stage (no extra synchronisation); hm_get_timing waits for

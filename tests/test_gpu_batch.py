@@ -54,6 +54,12 @@ def test_batch_fp64_matches_single_problems():
         print("batch info", info, "sizes", [p.n for p in probs], flush=True)
         assert info["members"] == len(probs) and info["nmax"] == max(p.n for p in probs)
         assert info["series_variant"] in (0, 1, 2) and info["grid_blocks"] > 0
+        tu = b.tuning()
+        print("batch tuning", tu, flush=True)
+        # the K chunk changes how the GEMM tiles stage K through LDS, not the
+        # order in which the MFMAs accumulate it: results stay as the single
+        # contexts' (K chunk 16)
+        assert tu["gemm_kchunk"] in (8, 16, 32) and min(tu["lu_ms"]) > 0
         ws = np.array([0.6, 0.9, 1.3, 1.7, 2.2])          # one frequency per member
         for w in (1.1, ws):
             got = b.solve(w)
@@ -86,23 +92,26 @@ def test_batch_graph_replays_match_eager():
 
 
 def test_batch_fp32_refinement():
+    """FP32 LU + refinement, autotuned (series variant and the K chunk of the
+    grouped GEMM tiles), against single FP64 contexts with the same series
+    variant"""
     probs = _members()
-    b64 = hm_bem.Batch(probs)
     b32 = hm_bem.Batch(probs)
     try:
-        b64.prepare(hm_bem.HM_FP64, autotune=False, use_graph=True)
-        b32.prepare(hm_bem.HM_FP32_IR, autotune=False, use_graph=True, nrefine=3)
+        b32.prepare(hm_bem.HM_FP32_IR, autotune=True, use_graph=True, nrefine=3)
+        tu = b32.tuning()
+        print("fp32 batch tuning", tu, flush=True)
+        assert tu["gemm_kchunk"] in (8, 16, 32) and min(tu["lu_ms"]) > 0
         for w in (0.7, 1.9):
-            ref = b64.solve(w)
             got = b32.solve(w)
             info = b32.info()
             assert info["last_correction"] < 1e-13, info
-            for k in range(len(probs)):
+            for k, prob in enumerate(probs):
+                ref = _single(prob, w, tu["series_variant"])
                 for key in KEYS:
-                    err = relerr(got[k][key], ref[k][key])
+                    err = relerr(got[k][key], ref[key])
                     assert err < 1e-10, (w, k, key, err, info)
     finally:
-        b64.close()
         b32.close()
 
 
