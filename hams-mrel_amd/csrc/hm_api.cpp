@@ -119,6 +119,18 @@ struct hm_ctx {
   hm::PidCache pcache() const {
     return pcache_on ? hm::PidCache{d_pcache.as<double>(), d_ptab.as<long long>()} : hm::PidCache{nullptr, nullptr};
   }
+  // the infinite-depth branch of the surrogate (its own network and error map)
+  DevBuf d_surd, d_surdcnt;
+  bool surd_ready = false, surd_on = false;
+  int surd_quant = 1;
+  double surd_tol = 1e-4;
+  static void* img_of(const DevBuf& b) { return static_cast<char*>(b.p) + hm::kSurImgOffset; }
+  static hm::SurDev dev_of(const DevBuf& b, int quant) {
+    const float* p = b.as<float>();
+    return hm::SurDev{p, p + hm::kSurParams, p + 2 * hm::kSurParams, p + 2 * hm::kSurParams + hm::kSurOut, quant,
+                      img_of(b)};
+  }
+  hm::SurDev surd_dev() const { return dev_of(d_surd, surd_quant); }
   void* sur_img() const { return static_cast<char*>(d_sur.p) + hm::kSurImgOffset; }
   hm::SurDev sur_dev() const {
     const float* b = d_sur.as<float>();
@@ -348,7 +360,8 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
                     &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
                     &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
-                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw};
+                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw,
+                    &c->d_surd, &c->d_surdcnt};
   for (DevBuf* b : bufs) b->release();
   c->htree.release();
   (void)c->resolve_timing();
@@ -592,7 +605,12 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
   const hm::GeomDev gd = c->geom();
   const hm::FreqDev f = c->freq();
   const bool limit = fabs(c->v) < F32(1.e-8) || fabs(c->v + 1.0) < F32(1.e-8);
-  if (c->h < 0.0 || limit) {
+  if (c->h < 0.0 && !limit && blk == 0 && c->surd_on && c->surd_ready) {
+    c->stage_begin(kAsmSurrogate);
+    HIPCHK(c, hm::sur_assemble_deep(c->stream, gd, f, c->surd_dev(), (float)c->surd_tol, A, S, ld,
+                                    c->d_surdcnt.as<unsigned long long>()), "assemble surrogate (deep)");
+    c->stage_end(kAsmSurrogate);
+  } else if (c->h < 0.0 || limit) {
     HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, ld, blk), "assemble");
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
@@ -1648,8 +1666,11 @@ struct TrainBufs {
 
 extern "C" int hm_surrogate_blob_size(void) { return hm::kSurBlob; }
 
-extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
-                                  unsigned long long seed, int quant, double* stats) {
+// training of the finite-depth network (deep = false) or of the
+// infinite-depth branch (deep = true): the same network, optimiser and error
+// map, on samples of the evanescent sums or of the Havelock function
+static int train_surrogate(hm_ctx* c, bool deep, long long nsamples, int epochs, int batch, double lr,
+                           unsigned long long seed, int quant, double* stats) {
   if (!c) return HM_ERR_ARG - 1;
   if (nsamples < 1024) return c->fail(HM_ERR_ARG - 2, "need at least 1024 samples");
   if (epochs < 0) return c->fail(HM_ERR_ARG - 3, "epochs must be >= 0");
@@ -1677,8 +1698,12 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   HIPCHK(c, tb.Y.ensure(3 * sizeof(float) * (size_t)nmax), "surrogate alloc");
   HIPCHK(c, hipMemcpyAsync(tb.kh.p, kh.data(), sizeof(double) * nkh, hipMemcpyHostToDevice, s), "surrogate");
   HIPCHK(c, hipMemcpyAsync(tb.wvn.p, wvn.data(), sizeof(double) * wvn.size(), hipMemcpyHostToDevice, s), "surrogate");
-  HIPCHK(c, hm::sur_generate(s, n, seed, nkh, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
-                             tb.Y.as<float>(), nullptr, nullptr), "surrogate samples");
+  if (deep)
+    HIPCHK(c, hm::sur_generate_deep(s, n, seed, tb.X.as<float>(), tb.Y.as<float>(), nullptr, nullptr),
+           "surrogate samples");
+  else
+    HIPCHK(c, hm::sur_generate(s, n, seed, nkh, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
+                               tb.Y.as<float>(), nullptr, nullptr), "surrogate samples");
   // output scales: RMS of each scaled sum
   std::vector<float> yh((size_t)n * 3);
   HIPCHK(c, hipMemcpyAsync(yh.data(), tb.Y.p, sizeof(float) * yh.size(), hipMemcpyDeviceToHost, s), "surrogate");
@@ -1700,8 +1725,9 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   fill(hm::kSurOffW0, hm::kSurH, hm::kSurIn);
   for (int l = 0; l < 3; ++l) fill(hm::kSurOffW1 + l * hm::kSurHiddenStride, hm::kSurH, hm::kSurH);
   fill(hm::kSurOffW4, hm::kSurOut, hm::kSurH);
-  HIPCHK(c, c->d_sur.ensure(hm::kSurBufBytes), "surrogate alloc");
-  float* P = c->d_sur.as<float>();
+  DevBuf& buf = deep ? c->d_surd : c->d_sur;
+  HIPCHK(c, buf.ensure(hm::kSurBufBytes), "surrogate alloc");
+  float* P = buf.as<float>();
   float* Q = P + hm::kSurParams;
   float* YS = Q + hm::kSurParams;
   float* MAP = YS + hm::kSurOut;
@@ -1746,7 +1772,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
     last_loss = l / (3.0 * (double)(steps_per_epoch * B));
   }
   // the inference image of the trained network
-  HIPCHK(c, hm::sur_round(s, P, Q, c->sur_img()), "surrogate");
+  HIPCHK(c, hm::sur_round(s, P, Q, hm_ctx::img_of(buf)), "surrogate");
   // validation set on other Kh values: error map and RMS error
   std::vector<double> khv, wvnv;
   kh_table(512, seed ^ 0xa11da7eull, c->g_disp, khv, wvnv);
@@ -1756,10 +1782,16 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   HIPCHK(c, tb.cell.ensure(sizeof(int) * (size_t)nval), "surrogate alloc");
   HIPCHK(c, tb.U.ensure(3 * sizeof(float) * (size_t)nval), "surrogate alloc");
   HIPCHK(c, tb.bits.ensure(sizeof(unsigned) * hm::kSurMap), "surrogate alloc");
-  HIPCHK(c, hm::sur_generate(s, nval, seed ^ 0x7a11dull, 512, tb.kh.as<double>(), tb.wvn.as<double>(), tb.X.as<float>(),
-                             tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>()), "surrogate validation");
-  c->sur_quant = quant ? 1 : 0;
-  HIPCHK(c, hm::sur_predict(s, nval, tb.X.as<float>(), c->sur_dev(), tb.U.as<float>()), "surrogate predict");
+  if (deep)
+    HIPCHK(c, hm::sur_generate_deep(s, nval, seed ^ 0x7a11dull, tb.X.as<float>(), tb.Y.as<float>(), tb.M.as<float>(),
+                                    tb.cell.as<int>()), "surrogate validation");
+  else
+    HIPCHK(c, hm::sur_generate(s, nval, seed ^ 0x7a11dull, 512, tb.kh.as<double>(), tb.wvn.as<double>(),
+                               tb.X.as<float>(), tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>()),
+           "surrogate validation");
+  (deep ? c->surd_quant : c->sur_quant) = quant ? 1 : 0;
+  HIPCHK(c, hm::sur_predict(s, nval, tb.X.as<float>(), hm_ctx::dev_of(buf, quant ? 1 : 0), tb.U.as<float>()),
+         "surrogate predict");
   HIPCHK(c, hipMemsetAsync(tb.bits.p, 0, sizeof(unsigned) * hm::kSurMap, s), "surrogate");
   HIPCHK(c, hipMemsetAsync(tb.loss.p, 0, sizeof(double), s), "surrogate");
   HIPCHK(c, hm::sur_error_map(s, nval, tb.Y.as<float>(), tb.M.as<float>(), tb.cell.as<int>(), tb.U.as<float>(),
@@ -1783,7 +1815,7 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   }
   HIPCHK(c, hipMemcpyAsync(MAP, map.data(), sizeof(float) * hm::kSurMap, hipMemcpyHostToDevice, s), "surrogate");
   HIPCHK(c, hipStreamSynchronize(s), "surrogate");
-  c->sur_ready = true;
+  (deep ? c->surd_ready : c->sur_ready) = true;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) {
     stats[0] = last_loss;
@@ -1797,6 +1829,53 @@ extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int
   }
   (void)rc;
   return HM_OK;
+}
+
+extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
+                                  unsigned long long seed, int quant, double* stats) {
+  return train_surrogate(c, false, nsamples, epochs, batch, lr, seed, quant, stats);
+}
+
+extern "C" int hm_surrogate_train_deep(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
+                                       unsigned long long seed, int quant, double* stats) {
+  return train_surrogate(c, true, nsamples, epochs, batch, lr, seed, quant, stats);
+}
+
+extern "C" int hm_set_surrogate_deep(hm_ctx* c, int enable, double tol) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (enable && !c->surd_ready) return c->fail(HM_ERR_STATE, "no infinite-depth surrogate trained");
+  if (!(tol >= 0.0)) return c->fail(HM_ERR_ARG - 3, "tol must be >= 0");
+  HIPCHK(c, c->d_surdcnt.ensure(sizeof(unsigned long long)), "surrogate alloc");
+  HIPCHK(c, hipMemsetAsync(c->d_surdcnt.p, 0, sizeof(unsigned long long), c->stream), "surrogate");
+  c->surd_on = enable != 0;
+  c->surd_tol = tol;
+  c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_deep_count(hm_ctx* c, long long* taken) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!taken) return c->fail(HM_ERR_ARG - 2, "taken is NULL");
+  unsigned long long net = 0;
+  if (c->d_surdcnt.p) {
+    HIPCHK(c, hipMemcpyAsync(&net, c->d_surdcnt.p, sizeof net, hipMemcpyDeviceToHost, c->stream), "surrogate counts");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate counts");
+  }
+  *taken = (long long)net;
+  return HM_OK;
+}
+
+extern "C" int hm_surrogate_eval_deep(hm_ctx* c, int n, const double* kr, const double* kv, double* net,
+                                      double* plain, double* bound) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (n < 1) return c->fail(HM_ERR_ARG - 2, "n must be >= 1");
+  if (!kr || !kv || !net || !plain || !bound) return c->fail(HM_ERR_ARG - 3, "arrays missing");
+  if (!c->surd_ready) return c->fail(HM_ERR_STATE, "no infinite-depth surrogate trained");
+  for (int i = 0; i < n; ++i)
+    if (!(kr[i] >= hm::kSurKrMin && kr[i] <= hm::kSurKrMax && kv[i] >= hm::kSurKvMin && kv[i] <= 0.0))
+      return c->fail(HM_ERR_ARG - 3, "point %d outside the infinite-depth surrogate's domain", i);
+  return hm::sur_eval_points_deep(c->stream, n, kr, kv, c->surd_dev(), net, plain, bound) == hipSuccess
+             ? HM_OK : c->fail(HM_ERR_HIP, "surrogate eval failed");
 }
 
 extern "C" int hm_surrogate_get(hm_ctx* c, float* blob, int n) {

@@ -1149,6 +1149,48 @@ HM_DEV void havelock(double hh, double vv, double& gf_re, double& gf_im, double&
   gfh_im = 2.0 * kPi * j1 * ev;
 }
 
+// the closed-form imaginary parts of GF and GFh (the wave terms above)
+HM_DEV void havelock_imag(double hh, double vv, double& gf_im, double& gfh_im) {
+  const double ev = exp(vv);
+  gf_im = -2.0 * kPi * inf_j0(hh) * ev;
+  gfh_im = 2.0 * kPi * inf_j1(hh) * ev;
+}
+
+// the singular terms of Re GF and Re GFh near d = 0: the log(0.5 (d - vv)) of
+// L0 and the (beta + hh) / (d - vv) of Ls, with their 2 / (1 + d^3) factors
+HM_DEV void havelock_singular(double hh, double vv, double& s0, double& s1) {
+  const double dd = sqrt(hh * hh + vv * vv), beta = hh / dd, d3 = dd * dd * dd;
+  s0 = 2.0 * exp(vv) * log(0.5 * (dd - vv)) / (1.0 + d3);
+  s1 = 2.0 * (beta + hh) / ((dd - vv) * (1.0 + d3));
+}
+
+// INFGREEN3D for a regular V (not the zero- or infinite-frequency limit) with
+// the Havelock function GF, GFh given (the learned surrogate's path)
+HM_DEV void infgreen3d_given(double xf, double xp, double yf, double yp, double zf, double zp, double v, int flag,
+                             double gr, double gi, double ghr, double ghi, Grn4& g) {
+  const double xpq = xf - xp, ypq = yf - yp, zpq = zp + zf;
+  const double rr = horiz_dist(xf, xp, yf, yp);
+  const double r = sqrt(rr * rr + (zp - zf) * (zp - zf)), r1 = sqrt(rr * rr + zpq * zpq);
+  const double r13 = r1 * r1 * r1, r3 = r * r * r;
+  for (int k = 0; k < 4; ++k) g.re[k] = g.im[k] = 0.0;
+  if (rr > F32(1.e-6)) {
+    g.re[1] = -(v * v) * ghr * xpq / rr; g.im[1] = -(v * v) * ghi * xpq / rr;
+    g.re[2] = -(v * v) * ghr * ypq / rr; g.im[2] = -(v * v) * ghi * ypq / rr;
+  }
+  g.re[0] = -v * gr; g.im[0] = -v * gi;
+  g.re[3] = -(v * v) * gr + 2.0 * v / r1; g.im[3] = -(v * v) * gi;
+  g.re[0] += 1.0 / r1;
+  g.re[1] += -xpq / r13;
+  g.re[2] += -ypq / r13;
+  g.re[3] += -zpq / r13;
+  if (flag != 1) {
+    g.re[0] += 1.0 / r;
+    g.re[1] -= xpq / r3;
+    g.re[2] -= ypq / r3;
+    g.re[3] -= (zf - zp) / r3;
+  }
+}
+
 HM_DEV void infgreen3d(double xf, double xp, double yf, double yp, double zf, double zp, double v,
                        int flag, Grn4& g) {
   const double xpq = xf - xp, ypq = yf - yp, zpq = zp + zf;

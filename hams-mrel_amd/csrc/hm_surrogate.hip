@@ -689,7 +689,162 @@ __global__ __launch_bounds__(256) void k_eval_points(int n, const double* rh, co
   bound[i] = m.map[sur_cell(x)];
 }
 
+// ---- infinite-depth branch --------------------------------------------------------
+// inputs: log KR, the phase of the wave terms (sin KR, cos KR: the Struve and
+// Bessel parts oscillate in KR with period 2 pi, a dozen times over the upper
+// decade of the domain, which a smooth function of log KR alone does not
+// follow) and K(z+zeta)
+__device__ __forceinline__ void sur_inputs_deep(double kr, double kv, float* x) {
+  double sk, ck;
+  sincos(kr, &sk, &ck);
+  x[0] = (float)(2.0 * (log(kr) - log(kSurKrMin)) / (log(kSurKrMax) - log(kSurKrMin)) - 1.0);
+  x[1] = (float)sk;
+  x[2] = (float)ck;
+  x[3] = (float)(2.0 * kv / -kSurKvMin + 1.0);
+}
+// targets: Re GF and Re GFh less their singular terms (third output unused,
+// trained to 0); mag: the dimensionless sizes of the full components
+__device__ __forceinline__ void sur_target_deep(double kr, double kv, double* u, double* mag) {
+  double gr, gi, ghr, ghi, s0, s1;
+  havelock(kr, kv, gr, gi, ghr, ghi);
+  havelock_singular(kr, kv, s0, s1);
+  u[0] = gr - s0;
+  u[1] = ghr - s1;
+  u[2] = 0.0;
+  // sizes of the full dimensionless components: the value |GF| + 1/d, and the
+  // gradient vector (the horizontal part GFh, the vertical part GF - 2/d of
+  // INFGREEN3D, and the Rankine image's 1/d^2)
+  const double d = sqrt(kr * kr + kv * kv);
+  mag[0] = hypot(gr, gi) + 1.0 / d;
+  mag[1] = hypot(ghr, ghi) + hypot(gr, gi) + 2.0 / d + 1.0 / (d * d);
+  mag[2] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_generate_deep(long long n, uint64_t seed, float* X, float* Y, float* M,
+                                                       int* cell) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double kr = exp(log(kSurKrMin) + unit(seed, i, 0) * (log(kSurKrMax) - log(kSurKrMin)));
+    // K (z + zeta) in [-20, 0], denser near the free surface
+    const double u1 = unit(seed, i, 1);
+    const double kv = kSurKvMin * u1 * u1;
+    double u[3], mag[3];
+    sur_target_deep(kr, kv, u, mag);
+    for (int k = 0; k < 3; ++k) Y[3 * i + k] = (float)u[k];
+    float x[4];
+    sur_inputs_deep(kr, kv, x);
+    for (int k = 0; k < 4; ++k) X[4 * i + k] = x[k];
+    if (M) {
+      for (int k = 0; k < 3; ++k) M[3 * i + k] = (float)mag[k];
+      cell[i] = sur_cell(x);
+    }
+  }
+}
+
+// the 2-D grid of all (field point, panel) pairs in deep water, as
+// k_assemble_grid<-1>, each pair in a cell whose bound is <= tol taking the
+// Havelock function from the network (its imaginary parts and the singular
+// terms of its real parts in closed form). The MFMA network needs the whole
+// workgroup, so a workgroup runs it when any of its pairs takes it.
+template <bool QUANT>
+__global__ __launch_bounds__(256) void k_sur_assemble_deep(GeomDev g, FreqDev f, SurDev m, float tol, double* A,
+                                                           double* S, long long ld, unsigned long long* count) {
+  __shared__ SurLds L;
+  if (QUANT) load_sur_lds(L.m, m.img);
+  __syncthreads();
+  const int i0 = blockIdx.y * 16 + (threadIdx.x >> 4), j0 = blockIdx.x * 16 + (threadIdx.x & 15);
+  const bool valid = i0 < g.nt && j0 < g.n;
+  const int i = valid ? i0 : 0, j = valid ? j0 : 0;
+  const double* xp = g.fp + 3 * (long long)i;
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const double kr = f.v * rr, kv = f.v * (xp[2] + xq[2]);
+  const bool limit = fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8);
+  const bool inside = !limit && kr >= kSurKrMin && kr <= kSurKrMax && kv >= kSurKvMin && kv <= 0.0;
+  float x[4], y[kSurOut];
+  sur_inputs_deep(fmin(fmax(kr, kSurKrMin), kSurKrMax), fmin(fmax(kv, kSurKvMin), 0.0), x);
+  const bool take = valid && inside && m.map[sur_cell(x)] <= tol;
+  const int ntake = __syncthreads_count(take);
+  if (ntake > 0) {
+    if (QUANT) mlp_block_mfma(L, x, y);
+    else mlp_eval(m.p, x, y, nullptr, nullptr, 0, false);
+  }
+  if (threadIdx.x == 0 && ntake > 0) atomicAdd(count, (unsigned long long)ntake);
+  if (!valid) return;
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  Grn4 gr;
+  if (take) {
+    double s0, s1, gi, ghi;
+    havelock_singular(kr, kv, s0, s1);
+    havelock_imag(kr, kv, gi, ghi);
+    const double gfr = (double)(y[0] * m.yscale[0]) + s0, ghr = (double)(y[1] * m.yscale[1]) + s1;
+    infgreen3d_given(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gfr, gi, ghr, ghi, gr);
+  } else {
+    infgreen3d(xq[0], xp[0], xq[1], xp[1], xq[2], xp[2], f.v, flag, gr);
+  }
+  write_pair_g(g, i, j, xp, flag, gr, A, S, ld);
+}
+
+__global__ __launch_bounds__(256) void k_eval_points_deep(int n, const double* kr, const double* kv, SurDev m,
+                                                          double* net, double* plain, double* bound) {
+  __shared__ SurLds L;
+  if (m.quant) load_sur_lds(L.m, m.img);
+  __syncthreads();
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = i0 < n ? i0 : n - 1;
+  float x[4], y[kSurOut];
+  sur_inputs_deep(kr[i], kv[i], x);
+  if (m.quant) mlp_block_mfma(L, x, y);
+  else mlp_eval(m.p, x, y, nullptr, nullptr, 0, false);
+  if (i0 >= n) return;
+  double gr, gi, ghr, ghi, s0, s1;
+  havelock(kr[i], kv[i], gr, gi, ghr, ghi);
+  havelock_singular(kr[i], kv[i], s0, s1);
+  plain[2 * i] = gr;
+  plain[2 * i + 1] = ghr;
+  net[2 * i] = (double)(y[0] * m.yscale[0]) + s0;
+  net[2 * i + 1] = (double)(y[1] * m.yscale[1]) + s1;
+  bound[i] = m.map[sur_cell(x)];
+}
+
 }  // namespace
+
+hipError_t sur_generate_deep(hipStream_t s, long long n, uint64_t seed, float* X, float* Y, float* M, int* cell) {
+  const long long blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_generate_deep, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, n, seed, X,
+                     Y, M, cell);
+  return hipGetLastError();
+}
+
+hipError_t sur_assemble_deep(hipStream_t s, const GeomDev& g, const FreqDev& f, SurDev m, float tol, double* A,
+                             double* S, long long ld, unsigned long long* count) {
+  if (m.quant && !m.img) return hipErrorInvalidValue;
+  const dim3 grid((g.n + 15) / 16, (g.nt + 15) / 16);
+  if (m.quant) hipLaunchKernelGGL((k_sur_assemble_deep<true>), grid, dim3(256), 0, s, g, f, m, tol, A, S, ld, count);
+  else hipLaunchKernelGGL((k_sur_assemble_deep<false>), grid, dim3(256), 0, s, g, f, m, tol, A, S, ld, count);
+  return hipGetLastError();
+}
+
+hipError_t sur_eval_points_deep(hipStream_t s, int n, const double* kr, const double* kv, SurDev m, double* net,
+                                double* plain, double* bound) {
+  if (n <= 0) return hipSuccess;
+  if (m.quant && !m.img) return hipErrorInvalidValue;
+  double* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(double) * 7 * (size_t)n);
+  if (e != hipSuccess) return e;
+  double *dkr = d, *dkv = d + n, *dn = d + 2 * (size_t)n, *dp = d + 4 * (size_t)n, *db = d + 6 * (size_t)n;
+  const size_t nb = sizeof(double) * n;
+  if ((e = hipMemcpyAsync(dkr, kr, nb, hipMemcpyHostToDevice, s)) == hipSuccess &&
+      (e = hipMemcpyAsync(dkv, kv, nb, hipMemcpyHostToDevice, s)) == hipSuccess) {
+    hipLaunchKernelGGL(k_eval_points_deep, dim3((n + 255) / 256), dim3(256), 0, s, n, dkr, dkv, m, dn, dp, db);
+    if ((e = hipGetLastError()) == hipSuccess && (e = hipMemcpyAsync(net, dn, 2 * nb, hipMemcpyDeviceToHost, s)) == hipSuccess &&
+        (e = hipMemcpyAsync(plain, dp, 2 * nb, hipMemcpyDeviceToHost, s)) == hipSuccess &&
+        (e = hipMemcpyAsync(bound, db, nb, hipMemcpyDeviceToHost, s)) == hipSuccess)
+      e = hipStreamSynchronize(s);
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(d);
+  return e;
+}
 
 hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, const double* kh, const double* wvn,
                         float* X, float* Y, float* M, int* cell) {

@@ -33,6 +33,19 @@
 // of surface-piercing hulls) and Kh in [0.01, 50]; R/h < 0.05 (PIDCOCK,
 // LINTON), R/h >= 0.5 (EIGEN, at most 9 terms, cheaper than a network
 // evaluation) and deeper points always take the plain series.
+//
+// Infinite-depth branch: the wave part of INFGREEN3D (InfGreen_Appr.f90:104-183)
+// is the Havelock function GF(KR, K(z+zeta)) and its KR-derivative GFh
+// (HavelockGF :185). Their imaginary parts are closed forms (-2 pi e^v J0,
+// 2 pi e^v J1); a second network of the same architecture predicts the real
+// parts less their singular terms (havelock_singular: the log(d - v) of L0 and
+// the (beta + h)/(d - v) of Ls), from (log KR, sin KR, cos KR, K(z+zeta)) over
+// KR in [1e-3, 50], K(z+zeta) in [-20, 0] (the phase inputs let the network
+// follow the oscillation of the Struve and Bessel terms in KR; the error map's
+// cells split on all four inputs as the finite-depth map does). Its error map is
+// against |GF| + 1/d and |GFh| + |GF| + 2/d + 1/d^2 (d = K R1), the
+// dimensionless sizes of the full value and gradient vector. The deep-water assembly (sur_assemble_deep) takes the
+// network for the pairs in cells whose bound is <= the tolerance.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -54,6 +67,7 @@ constexpr int kSurMap = kMapR * kMapS * kMapD * kMapK;       // 36864 cells
 constexpr int kSurBlob = kSurParams + kSurOut + kSurMap;     // params, output scales, error map
 constexpr double kSurRhMin = 0.05, kSurRhMax = 0.5, kSurKhMin = 0.01, kSurKhMax = 50.0;
 constexpr double kSurDepth = 0.25;   // z/h and zeta/h in [-0.25, 0]: the draft range of surface-piercing hulls
+constexpr double kSurKrMin = 1e-3, kSurKrMax = 50.0, kSurKvMin = -20.0;   // infinite-depth branch
 
 // LDS image of the BF16 network for the MFMA inference kernels (hidden weights
 // as BF16 bits, the FP32 first and output layers and the biases), built on the
@@ -83,6 +97,16 @@ struct SurTrainStats {
 // training and validation data from the plain kernels (h = 1)
 hipError_t sur_generate(hipStream_t s, long long n, uint64_t seed, int nkh, const double* kh, const double* wvn,
                         float* X, float* Y, float* M, int* cell);
+// the infinite-depth branch's data from the plain Havelock function
+hipError_t sur_generate_deep(hipStream_t s, long long n, uint64_t seed, float* X, float* Y, float* M, int* cell);
+// deep-water assembly of all pairs (as assemble_grid with deep = true), the
+// network taking the Havelock function of pairs in cells whose bound is <= tol
+// (count += pairs taken)
+hipError_t sur_assemble_deep(hipStream_t s, const GeomDev& g, const FreqDev& f, SurDev m, float tol, double* A,
+                             double* S, long long ld, unsigned long long* count);
+// network and plain Re GF, Re GFh at host points (kr, kv), with each point's cell bound
+hipError_t sur_eval_points_deep(hipStream_t s, int n, const double* kr, const double* kv, SurDev m, double* net,
+                                double* plain, double* bound);
 // q = BF16-rounded p; img (may be null) = the LDS image of q
 hipError_t sur_round(hipStream_t s, const float* p, float* q, void* img = nullptr);
 hipError_t sur_forward(hipStream_t s, int B, const float* X, const float* q, float* Z, float* A, float* Yh,

@@ -38,7 +38,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
-           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning")
+           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
+           "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep")
 
 _lib = None
 
@@ -272,6 +273,33 @@ class Context:
         self._chk(self.L.hm_surrogate_eval(self.ptr, _i(n), *[_p(v) for v in a], _p(tn), _p(tp), _p(bd)),
                   "hm_surrogate_eval")
         return tn, tp, bd
+
+    def surrogate_train_deep(self, nsamples=4_000_000, epochs=40, batch=4096, lr=2e-3, seed=20261205, quant=True):
+        """the infinite-depth branch: Re GF and Re GFh of the Havelock function of (KR, K(z+zeta))"""
+        st = np.zeros(8)
+        self._chk(self.L.hm_surrogate_train_deep(self.ptr, ctypes.c_longlong(nsamples), _i(epochs), _i(batch),
+                                                 _d(lr), ctypes.c_ulonglong(seed), _i(1 if quant else 0), _p(st)),
+                  "hm_surrogate_train_deep")
+        keys = ("train_loss", "val_rmse", "val_max_err", "seconds", "cells_below_1e-4", "cells_below_1e-3",
+                "steps", "samples")
+        return dict(zip(keys, st.tolist()))
+
+    def set_surrogate_deep(self, enable=True, tol=1e-4):
+        self._chk(self.L.hm_set_surrogate_deep(self.ptr, _i(1 if enable else 0), _d(tol)), "hm_set_surrogate_deep")
+
+    def surrogate_deep_count(self):
+        k = ctypes.c_longlong(0)
+        self._chk(self.L.hm_surrogate_deep_count(self.ptr, ctypes.byref(k)), "hm_surrogate_deep_count")
+        return k.value
+
+    def surrogate_eval_deep(self, kr, kv):
+        kr = np.ascontiguousarray(kr, dtype=np.float64)
+        kv = np.ascontiguousarray(kv, dtype=np.float64)
+        n = len(kr)
+        net, plain, bd = np.zeros((n, 2)), np.zeros((n, 2)), np.zeros(n)
+        self._chk(self.L.hm_surrogate_eval_deep(self.ptr, _i(n), _p(kr), _p(kv), _p(net), _p(plain), _p(bd)),
+                  "hm_surrogate_eval_deep")
+        return net, plain, bd
 
     def surrogate_grad(self, x, u, quant=True):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -275,12 +275,31 @@ int hm_surrogate_grad(hm_ctx *ctx, int batch, const float *x, const float *u, in
 /* network inference (the BF16 network on MFMA, as the assembly evaluates it)
  * for n normalised inputs x (4n) -> scaled outputs u (3n) */
 int hm_surrogate_predict(hm_ctx *ctx, int n, const float *x, float *u);
+/* Infinite-depth branch (no counterpart in the reference). The deep-water wave
+ * part of INFGREEN3D (InfGreen_Appr.f90:104-183) is the Havelock function GF and
+ * its KR-derivative GFh of (KR, K(z+zeta)) (HavelockGF :185): a second network
+ * of the same architecture learns Re GF and Re GFh less their singular terms
+ * (the log(d - v) of L0 and the (beta + KR)/(d - v) of Ls, d = K R1) from
+ * (log KR, sin KR, cos KR, K(z+zeta)) over KR in [1e-3, 50], K(z+zeta) in
+ * [-20, 0]; the imaginary parts are closed forms.
+ * hm_surrogate_train_deep trains it (arguments and stats as hm_surrogate_train;
+ * its error map: cells over the four inputs, against |GF| + 1/d
+ * and the gradient size |GFh| + |GF| + 2/d + 1/d^2). hm_set_surrogate_deep: use it in the deep-water
+ * assembly for the pairs in cells whose bound is <= tol; hm_surrogate_deep_count:
+ * pairs it took since then. hm_surrogate_eval_deep: network and plain Re GF,
+ * Re GFh (2 per point) at n points, with each point's cell bound. */
+int hm_surrogate_train_deep(hm_ctx *ctx, long long nsamples, int epochs, int batch, double lr,
+                            unsigned long long seed, int quant, double *stats);
+int hm_set_surrogate_deep(hm_ctx *ctx, int enable, double tol);
+int hm_surrogate_deep_count(hm_ctx *ctx, long long *taken);
+int hm_surrogate_eval_deep(hm_ctx *ctx, int n, const double *kr, const double *kv, double *net, double *plain,
+                           double *bound);
 
 /* ---- batches of independent problems ------------------------------------------
  * The reference solves one problem at a time, calling the per-frequency chain
  * (assembly, ZGETRF/ZGETRS, forces; HAMS_Prog.f90:615-724) serially. A batch
- * keeps many problems (members; any panel counts, finite or infinite depth, no
- * interior waterplane points) resident in HBM and runs that chain for all of
+ * keeps many problems (members; any panel counts, finite or infinite depth,
+ * with or without interior waterplane points) resident in HBM and runs that chain for all of
  * them at once per call: one assembly launch per regime over the pairs of
  * every member (each wave's pairs from one member), grouped launches (each
  * member at its own order) for
