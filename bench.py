@@ -104,7 +104,8 @@ PATH_DESC = {
               "MFMA, FP32 block-diagonal LU preconditioner (one block per body), FP64 residuals"),
     "fp32ir": "FP64 assembly + FP32 LU (MFMA) + FP64 iterative refinement",
     "hmatrix": ("H-matrix: dense FP64 diagonal block per body (each pair with its mirror, bit for bit the plain "
-                "path's blocks), off-diagonal blocks by adaptive cross approximation (tolerance 1e-6 for A, 1e-7 "
+                "path's blocks; bodies whose local hull panels are equal share one block, assembled, factored and "
+                "inverted once per frequency), off-diagonal blocks by adaptive cross approximation (tolerance 1e-6 for A, 1e-7 "
                 "for S) whose entries come straight from the FP64 Green function with directly summed evanescent "
                 "modes, right-hand sides S W from the H-matrix of S; restarted GMRES(50), tol 1e-6, with FP32 "
                 "MFMA H-matrix products, FP32 block-diagonal LU preconditioner and FP64 H-matrix residuals"),
@@ -281,7 +282,7 @@ def bench_batch(args):
         ctxs = []
         for p in probs:
             c = hm_bem.Context()
-            c.setup(p)
+            c.setup(p, frames=False)         # the batch takes global coordinates
             ctxs.append(c)
         sync()
         for c in ctxs:
@@ -353,6 +354,9 @@ def main():
     ap.add_argument("--lu-panel", choices=("coop", "column"), default="coop",
                     help="dense LU panels: one cooperative launch each (default) or per-column launches (for "
                          "profilers that do not handle cooperative launches)")
+    ap.add_argument("--no-block-sharing", action="store_true",
+                    help="H-matrix path: assemble, factor and invert every body's diagonal block, also where bodies "
+                         "are equal (the blocks are the same bit for bit either way)")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
     ap.add_argument("--field-points", type=int, default=0,
                     help="also time the potential, pressure and elevation at this many field points after the "
@@ -421,6 +425,7 @@ def main():
         # their PIDCOCK terms
         sync()
         t0 = time.perf_counter()
+        ctx.set_block_sharing(not args.no_block_sharing)
         ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
         sync()
         hm_setup_s = time.perf_counter() - t0
@@ -461,9 +466,21 @@ def main():
         aca_alone = {"systems": min(2, args.steps), "stages_ms": {k: v for k, v in st_alone.items() if v[1]},
                      "ranks": ranks_alone}
         print(f"# cross approximation alone: {aca_alone}", file=sys.stderr, flush=True)
+    no_share = None
+    if args.path == "hmatrix" and not args.no_block_sharing and args.steps > 0:
+        # the same path with every body's diagonal block assembled, factored and
+        # inverted (outside the timed region; the outputs are the same bit for bit)
+        ctx.set_block_sharing(False)
+        k = min(3, args.steps)
+        dt_ns, res_ns, _ = time_path(ctx, prob, prec, k, args.warmup, stage_times=False)
+        same = all(np.array_equal(a[q], b[q]) for a, b in zip(res_ns, res_mx[:k]) for q in ("amas", "bdmp", "exfc"))
+        no_share = {"systems": k, "value": k / dt_ns, "ms_per_step": 1e3 * dt_ns / k, "outputs_bitwise_equal": same}
+        ctx.set_block_sharing(True)
+        print(f"# without block sharing: {no_share}", file=sys.stderr, flush=True)
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
-                    setup_s=hm_setup_s)
+                    setup_s=hm_setup_s, distinct_bodies=int(len(set(ctx.body_classes().tolist()))),
+                    block_sharing=not args.no_block_sharing, without_block_sharing=no_share)
         ctx.set_hmatrix(False)
         print(f"# hmatrix {hmat}", file=sys.stderr, flush=True)
     if sur is not None:

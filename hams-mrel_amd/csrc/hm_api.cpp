@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <functional>
 #include <random>
 #include <string>
@@ -72,6 +73,27 @@ struct hm_ctx {
   std::vector<int> body_start;
   DevBuf d_body, d_fp, d_verts, d_ncn, d_ds, d_pnsz, d_dxyz;
   bool have_geom = false;
+  // body frames (hm_set_body_origins): the coordinates are local to each body
+  // and d_org holds the horizontal placement of each field point's body
+  DevBuf d_org;
+  bool frames = false;
+  // host copies of the per-panel inputs of the influence matrices (the
+  // comparison of bodies for shared diagonal blocks)
+  std::vector<double> fp_host, verts_host, ds_host, pnsz_host, dxyz_host;
+  std::vector<int> ncn_host;
+  // body_rep[b]: the first body whose hull panels are bitwise those of body b
+  // (frames on); H-matrix diagonal blocks are assembled and factored once per
+  // distinct body and copied to the others (block_share)
+  std::vector<int> body_rep;
+  bool block_share = true;
+  DevBuf d_ublk;                // the distinct bodies (classification of the diagonal blocks)
+  int n_ublk = 0;
+  bool sharing() const {
+    if (!block_share || !hm_on) return false;
+    for (int b = 0; b < nbody; ++b)
+      if (body_rep[b] != b) return true;
+    return false;
+  }
   // depth + classification
   double h = 0.0;
   bool have_depth = false;
@@ -214,6 +236,7 @@ struct hm_ctx {
     gd.ds = d_ds.as<double>();
     gd.pnsz = d_pnsz.as<double>();
     gd.dxyz = d_dxyz.as<double>();
+    gd.org = frames ? d_org.as<double>() : nullptr;
     return gd;
   }
   hm::FreqDev freq() const {
@@ -354,7 +377,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
   if (!c) return HM_ERR_ARG - 1;
   if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->d_body, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
+  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
@@ -395,6 +418,28 @@ static int upload(hm_ctx* c, DevBuf& b, const T* src, size_t count, const char* 
   return HM_OK;
 }
 
+// smallest horizontal gap between the bodies' bounding circles, from the
+// global panel centroids in xyz_host
+static void set_body_gap(hm_ctx* c) {
+  const int nbody = c->nbody;
+  const double* xyz_p = c->xyz_host.data();
+  std::vector<double> cx(nbody, 0.0), cy(nbody, 0.0), rad(nbody, 0.0);
+  for (int b = 0; b < nbody; ++b) {
+    const int j0 = c->body_start[b], j1 = c->body_start[b + 1];
+    for (int j = j0; j < j1; ++j) { cx[b] += xyz_p[3 * (size_t)j]; cy[b] += xyz_p[3 * (size_t)j + 1]; }
+    cx[b] /= (j1 - j0);
+    cy[b] /= (j1 - j0);
+    for (int j = j0; j < j1; ++j)
+      rad[b] = std::max(rad[b], std::hypot(xyz_p[3 * (size_t)j] - cx[b], xyz_p[3 * (size_t)j + 1] - cy[b]));
+  }
+  c->min_body_gap = nbody > 1 ? 1e300 : 0.0;
+  for (int b1 = 0; b1 < nbody; ++b1)
+    for (int b2 = b1 + 1; b2 < nbody; ++b2)
+      c->min_body_gap = std::min(c->min_body_gap, std::hypot(cx[b1] - cx[b2], cy[b1] - cy[b2]) - rad[b1] - rad[b2]);
+}
+
+static int classify(hm_ctx* c);
+
 extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int ntnd, const double* xyz,
                                const int* ncn, const int* ncon, const double* xyz_p, const double* ds,
                                const double* pnsz, const double* dxyz_p, int ni, const double* ixyz_p) {
@@ -423,21 +468,8 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   if (ni) memcpy(fp.data() + 3 * (size_t)n, ixyz_p, sizeof(double) * 3 * (size_t)ni);
   c->n = n; c->ni = ni; c->nbody = nbody;
   c->body_start.assign(body_start, body_start + nbody + 1);
-  {
-    std::vector<double> cx(nbody, 0.0), cy(nbody, 0.0), rad(nbody, 0.0);
-    for (int b = 0; b < nbody; ++b) {
-      const int j0 = body_start[b], j1 = body_start[b + 1];
-      for (int j = j0; j < j1; ++j) { cx[b] += xyz_p[3 * (size_t)j]; cy[b] += xyz_p[3 * (size_t)j + 1]; }
-      cx[b] /= (j1 - j0);
-      cy[b] /= (j1 - j0);
-      for (int j = j0; j < j1; ++j)
-        rad[b] = std::max(rad[b], std::hypot(xyz_p[3 * (size_t)j] - cx[b], xyz_p[3 * (size_t)j + 1] - cy[b]));
-    }
-    c->min_body_gap = nbody > 1 ? 1e300 : 0.0;
-    for (int b1 = 0; b1 < nbody; ++b1)
-      for (int b2 = b1 + 1; b2 < nbody; ++b2)
-        c->min_body_gap = std::min(c->min_body_gap, std::hypot(cx[b1] - cx[b2], cy[b1] - cy[b2]) - rad[b1] - rad[b2]);
-  }
+  c->xyz_host.assign(xyz_p, xyz_p + 3 * (size_t)n);
+  set_body_gap(c);
   int rc;
   if ((rc = upload(c, c->d_body, body_start, nbody + 1, "geometry")) ||
       (rc = upload(c, c->d_fp, fp.data(), fp.size(), "geometry")) ||
@@ -446,13 +478,107 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
       (rc = upload(c, c->d_pnsz, pnsz, n, "geometry")) || (rc = upload(c, c->d_dxyz, dxyz_p, 6 * (size_t)n, "geometry")))
     return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream), "geometry upload");
-  c->xyz_host.assign(xyz_p, xyz_p + 3 * (size_t)n);
+  c->fp_host = std::move(fp);
+  c->verts_host = std::move(verts);
+  c->ncn_host.assign(ncn, ncn + n);
+  c->ds_host.assign(ds, ds + n);
+  c->pnsz_host.assign(pnsz, pnsz + n);
+  c->dxyz_host.assign(dxyz_p, dxyz_p + 6 * (size_t)n);
+  c->frames = false;
+  c->body_rep.resize(nbody);
+  for (int b = 0; b < nbody; ++b) c->body_rep[b] = b;
   c->have_geom = true;
   c->have_depth = false;
   c->hm_on = false;
   c->hm_tree = false;
   c->ht_ready = false;
   c->assembled = c->factored = c->solved = false;
+  return HM_OK;
+}
+
+// body b's hull panels are bitwise those of body a: every input of the
+// influence matrices' (a, a) block equals that of (b, b)
+static bool same_body(const hm_ctx* c, int a, int b) {
+  const int ja = c->body_start[a], jb = c->body_start[b], m = c->body_start[a + 1] - ja;
+  if (c->body_start[b + 1] - jb != m) return false;
+  auto eq = [&](const std::vector<double>& v, int w) {
+    return memcmp(v.data() + (size_t)w * ja, v.data() + (size_t)w * jb, sizeof(double) * (size_t)w * m) == 0;
+  };
+  if (memcmp(c->ncn_host.data() + ja, c->ncn_host.data() + jb, sizeof(int) * (size_t)m) != 0) return false;
+  if (!eq(c->fp_host, 3) || !eq(c->verts_host, 12) || !eq(c->ds_host, 1) || !eq(c->pnsz_host, 1)) return false;
+  for (int j = 0; j < m; ++j)
+    for (int k = 0; k < 3; ++k)
+      if (memcmp(&c->dxyz_host[6 * (size_t)(ja + j) + k], &c->dxyz_host[6 * (size_t)(jb + j) + k], sizeof(double)))
+        return false;
+  return true;
+}
+
+extern "C" int hm_set_body_origins(hm_ctx* c, int nbody, const double* origin, const int* wp_start) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_body_origins before hm_set_geometry");
+  if (nbody != c->nbody) return c->fail(HM_ERR_ARG - 2, "nbody = %d, the geometry has %d bodies", nbody, c->nbody);
+  if (!origin) return c->fail(HM_ERR_ARG - 3, "origin is NULL");
+  for (int k = 0; k < 2 * nbody; ++k)
+    if (!std::isfinite(origin[k])) return c->fail(HM_ERR_ARG - 3, "origin[%d] is not finite", k);
+  if (c->ni > 0) {
+    if (!wp_start) return c->fail(HM_ERR_ARG - 4, "wp_start is NULL with %d waterplane points", c->ni);
+    if (wp_start[0] != 0 || wp_start[nbody] != c->ni)
+      return c->fail(HM_ERR_ARG - 4, "wp_start must run from 0 to ni = %d", c->ni);
+    for (int b = 0; b < nbody; ++b)
+      if (wp_start[b + 1] < wp_start[b]) return c->fail(HM_ERR_ARG - 4, "wp_start decreases at body %d", b);
+  }
+  const int n = c->n, nt = c->nt();
+  std::vector<double> org(2 * (size_t)nt);
+  for (int b = 0; b < nbody; ++b) {
+    for (int j = c->body_start[b]; j < c->body_start[b + 1]; ++j) {
+      org[2 * (size_t)j] = origin[2 * b];
+      org[2 * (size_t)j + 1] = origin[2 * b + 1];
+    }
+    if (c->ni > 0)
+      for (int p = wp_start[b]; p < wp_start[b + 1]; ++p) {
+        org[2 * (size_t)(n + p)] = origin[2 * b];
+        org[2 * (size_t)(n + p) + 1] = origin[2 * b + 1];
+      }
+  }
+  int rc = upload(c, c->d_org, org.data(), org.size(), "body origins");
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "body origins");
+  c->frames = true;
+  // global centroids for the body gaps and the cluster tree
+  for (int j = 0; j < n; ++j) {
+    c->xyz_host[3 * (size_t)j] = c->fp_host[3 * (size_t)j] + org[2 * (size_t)j];
+    c->xyz_host[3 * (size_t)j + 1] = c->fp_host[3 * (size_t)j + 1] + org[2 * (size_t)j + 1];
+  }
+  set_body_gap(c);
+  std::vector<int> u;
+  for (int b = 0; b < nbody; ++b) {
+    c->body_rep[b] = b;
+    for (int a = 0; a < b; ++a)
+      if (c->body_rep[a] == a && same_body(c, a, b)) { c->body_rep[b] = a; break; }
+    if (c->body_rep[b] == b) u.push_back(b);
+  }
+  if ((rc = upload(c, c->d_ublk, u.data(), u.size(), "body origins"))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "body origins");
+  c->n_ublk = (int)u.size();
+  c->assembled = c->factored = c->solved = false;
+  if (c->have_depth && c->h > 0.0) return classify(c);
+  return HM_OK;
+}
+
+extern "C" int hm_get_body_classes(hm_ctx* c, int nbody, int* rep) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
+  if (nbody != c->nbody || !rep) return c->fail(HM_ERR_ARG - 2, "bad output array");
+  for (int b = 0; b < nbody; ++b) rep[b] = c->body_rep[b];
+  return HM_OK;
+}
+
+extern "C" int hm_set_block_sharing(hm_ctx* c, int enable) {
+  if (!c) return HM_ERR_ARG - 1;
+  const bool was = c->block_share;
+  c->block_share = enable != 0;
+  c->assembled = c->factored = c->solved = false;
+  if (was != c->block_share && c->hm_on && c->have_depth && c->h > 0.0) return classify(c);
   return HM_OK;
 }
 
@@ -464,14 +590,23 @@ static int classify(hm_ctx* c) {
   // PIDCOCK pairs with their mirrors (the field points are the panel centroids)
   const int blk = c->hm_on ? c->bm() : 0;
   const bool sym = blk > 0 && c->pair_sym;
-  const long long np = blk ? (long long)c->nbody * blk * blk : (long long)c->nt() * c->n;
+  // bodies whose hull panels equal an earlier body's share its diagonal block:
+  // only the distinct bodies' blocks are classified (and assembled)
+  const int* ublk = nullptr;
+  int nblk = blk ? c->nbody : 0;
+  if (blk && c->sharing()) {
+    ublk = c->d_ublk.as<int>();
+    nblk = c->n_ublk;
+  }
+  const long long np = blk ? (long long)nblk * blk * blk : (long long)c->nt() * c->n;
   for (int r = 0; r < 4; ++r) c->seg_off[r] = c->seg_cnt[r] = 0;
   c->stage_begin(kClassify);
   HIPCHK(c, c->d_hist.ensure(sizeof(unsigned) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_cursor.ensure(sizeof(unsigned long long) * hm::kNumKeys), "classify");
   HIPCHK(c, c->d_pairs.ensure(sizeof(uint32_t) * (size_t)np), "classify");
   HIPCHK(c, hipMemsetAsync(c->d_hist.p, 0, sizeof(unsigned) * hm::kNumKeys, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true, blk, sym), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, c->d_hist.as<unsigned>(), nullptr, nullptr, true, blk, sym, ublk, nblk),
+         "classify");
   std::vector<unsigned> hist(hm::kNumKeys);
   HIPCHK(c, hipMemcpyAsync(hist.data(), c->d_hist.p, sizeof(unsigned) * hm::kNumKeys, hipMemcpyDeviceToHost, c->stream), "classify");
   HIPCHK(c, hipStreamSynchronize(c->stream), "classify");
@@ -494,7 +629,8 @@ static int classify(hm_ctx* c) {
     c->seg_cnt[r] = (long long)((r + 1 < 4 ? cur[(r + 1) * hm::kKeyStride] : acc) - cur[r * hm::kKeyStride]);
   }
   HIPCHK(c, hipMemcpyAsync(c->d_cursor.p, cur.data(), sizeof(unsigned long long) * hm::kNumKeys, hipMemcpyHostToDevice, c->stream), "classify");
-  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(), false, blk, sym), "classify");
+  HIPCHK(c, hm::classify_pairs(c->stream, gd, h, nullptr, c->d_cursor.as<unsigned long long>(), c->d_pairs.as<uint32_t>(),
+                               false, blk, sym, ublk, nblk), "classify");
   // mixed path (mirrored pairs): cache the i pi / h halves of the PIDCOCK terms
   c->pcache_on = false;
   if (sym && c->seg_cnt[hm::kPidcock] > 0) {
@@ -611,7 +747,9 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
                                     c->d_surdcnt.as<unsigned long long>()), "assemble surrogate (deep)");
     c->stage_end(kAsmSurrogate);
   } else if (c->h < 0.0 || limit) {
-    HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, ld, blk), "assemble");
+    const bool share = blk > 0 && c->sharing();
+    HIPCHK(c, hm::assemble_grid(c->stream, gd, f, c->h < 0.0, A, S, ld, blk, share ? c->d_ublk.as<int>() : nullptr,
+                                share ? c->n_ublk : 0), "assemble");
   } else {
     const uint32_t* pairs = c->d_pairs.as<uint32_t>();
     const bool sur = c->sur_on && c->sur_ready;
@@ -666,6 +804,45 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
 
 static int precond_early(hm_ctx* c);
 
+// H-matrix diagonal blocks shared by bodies with equal hull panels: body b's
+// block (elements [b stride, b stride + count) of `base`, `esize` bytes each)
+// becomes a copy of its representative's
+static int copy_shared_blocks(hm_ctx* c, hipStream_t s, void* base, long long stride, long long count, size_t esize) {
+  if (!c->sharing()) return HM_OK;
+  char* p = static_cast<char*>(base);
+  for (int b = 0; b < c->nbody; ++b) {
+    const int r = c->body_rep[b];
+    if (r == b) continue;
+    HIPCHK(c, hipMemcpyAsync(p + esize * (size_t)(b * stride), p + esize * (size_t)(r * stride), esize * (size_t)count,
+                             hipMemcpyDeviceToDevice, s), "shared diagonal blocks");
+  }
+  return HM_OK;
+}
+
+// FP32 LU of the H-matrix form's preconditioner blocks (d_P32, one per body,
+// pivots in d_pipiv): with shared diagonal blocks each distinct body's block is
+// factored once and its factors, pivots and info word copied to the bodies
+// that share it
+static int factor_body_blocks(hm_ctx* c, hipStream_t s, int* info, unsigned* counter) {
+  const int nb = c->nbody, m = c->bm();
+  const long long sc = (long long)m * m;
+  float* P = c->d_P32.as<float>();
+  int* piv = c->d_pipiv.as<int>();
+  if (!c->sharing()) {
+    HIPCHK(c, hm::lu_factor<float>(s, P, m, sc, m, piv, info, counter, nb), "block preconditioner");
+    return HM_OK;
+  }
+  for (int b = 0; b < nb; ++b)
+    if (c->body_rep[b] == b)
+      HIPCHK(c, hm::lu_factor<float>(s, P + 2 * b * sc, m, sc, m, piv + (size_t)b * m, info + b, counter + b, 1),
+             "block preconditioner");
+  int rc;
+  if ((rc = copy_shared_blocks(c, s, P, sc, sc, 2 * sizeof(float))) ||
+      (rc = copy_shared_blocks(c, s, piv, m, m, sizeof(int))) || (rc = copy_shared_blocks(c, s, info, 1, 1, sizeof(int))))
+    return rc;
+  return HM_OK;
+}
+
 // H-matrix form: diagonal blocks of A and S densely (the classified pairs of
 // those blocks), the off-diagonal blocks by cross approximation, and the
 // right-hand sides S W through the H-matrix product in FP64
@@ -701,6 +878,9 @@ static int assemble_hmatrix(hm_ctx* c) {
   }
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
+  if ((rc = copy_shared_blocks(c, c->stream, D, m * (m + 1), m * m, 2 * sizeof(double))) ||
+      (rc = copy_shared_blocks(c, c->stream, DS, m * (m + 1), m * m, 2 * sizeof(double))))
+    return rc;
   if ((rc = precond_early(c))) return rc;
   // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
   c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
@@ -771,8 +951,21 @@ static int precond_prepare(hm_ctx* c, hipStream_t s) {
   HIPCHK(c, c->d_Pinv.ensure(2 * sizeof(float) * tot), "preconditioner alloc");
   HIPCHK(c, hm::gmres_block_perm(s, nblk, c->gm_blocks.data(), c->d_pipiv.as<int>(), c->d_pperm.as<int>()),
          "preconditioner permutation");
-  HIPCHK(c, hm::gmres_block_inverse(s, nblk, c->gm_blocks.data(), c->d_P32.as<float>(), c->d_Pinv.as<float>()),
-         "preconditioner inverse");
+  if (c->sharing() && c->gm_blocks == c->body_start) {
+    // the H-matrix form's body blocks: one inverse per distinct body, copied
+    const int m = c->bm();
+    const long long sc = (long long)m * m;
+    const int one[2] = {0, m};
+    for (int b = 0; b < nblk; ++b)
+      if (c->body_rep[b] == b)
+        HIPCHK(c, hm::gmres_block_inverse(s, 1, one, c->d_P32.as<float>() + 2 * b * sc, c->d_Pinv.as<float>() + 2 * b * sc),
+               "preconditioner inverse");
+    int rc = copy_shared_blocks(c, s, c->d_Pinv.p, sc, sc, 2 * sizeof(float));
+    if (rc) return rc;
+  } else {
+    HIPCHK(c, hm::gmres_block_inverse(s, nblk, c->gm_blocks.data(), c->d_P32.as<float>(), c->d_Pinv.as<float>()),
+           "preconditioner inverse");
+  }
   c->pinv_ready = true;
   return HM_OK;
 }
@@ -801,11 +994,10 @@ static int precond_early(hm_ctx* c) {
   HIPCHK(c, hipMemsetAsync(c->d_counter2.p, 0, sizeof(unsigned) * nb, s), "preconditioner");
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
-  HIPCHK(c, hm::lu_factor<float>(s, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info2.as<int>(),
-                                 c->d_counter2.as<unsigned>(), nb), "block preconditioner");
-  c->gm_blocks = c->body_start;
-  int rc = precond_prepare(c, s);
+  int rc = factor_body_blocks(c, s, c->d_info2.as<int>(), c->d_counter2.as<unsigned>());
   if (rc) return rc;
+  c->gm_blocks = c->body_start;
+  if ((rc = precond_prepare(c, s))) return rc;
   c->prec_early = true;
   return HM_OK;
 }
@@ -843,8 +1035,8 @@ static int factor_hmatrix(hm_ctx* c) {
          "demote");
   HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
          "demote");
-  HIPCHK(c, hm::lu_factor<float>(c->stream, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info.as<int>(),
-                                 c->d_counter.as<unsigned>(), nb), "block preconditioner");
+  int rc = factor_body_blocks(c, c->stream, c->d_info.as<int>(), c->d_counter.as<unsigned>());
+  if (rc) return rc;
   c->gm_blocks = c->body_start;
   if (c->gm_precond == 0) c->gm_blocks.clear();
   return precond_prepare(c, c->stream);

@@ -37,7 +37,8 @@ __device__ __forceinline__ void load_wvn(double* s, const double* g, int nk) {
 template <int REG>
 __device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
                                               double* A, double* S, long long ld) {
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const int flag = near_flag(xp, xq, g.pnsz[j]);
   Grn4 gr;
@@ -62,11 +63,12 @@ __device__ __forceinline__ void assemble_pair(const GeomDev& g, const FreqDev& f
 // 2-D grid over all pairs (infinite depth, or finite depth with dispatch per pair)
 template <int REG>
 __global__ __launch_bounds__(256) void k_assemble_grid(GeomDev g, FreqDev f, double* A, double* S, long long ld,
-                                                       int blk) {
+                                                       int blk, const int* __restrict__ blist) {
   __shared__ double wvn[kNk];
   if (REG != -1) load_wvn(wvn, f.wvn, kNk);
-  // blk > 0: only the diagonal blocks of blk x blk pairs (blockIdx.z = block)
-  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  // blk > 0: only the diagonal blocks of blk x blk pairs (blockIdx.z = block,
+  // or blist[blockIdx.z])
+  const int b0 = blk > 0 ? (blist ? blist[blockIdx.z] : (int)blockIdx.z) * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
   if (i >= (blk > 0 ? b0 + blk : g.nt) || j >= (blk > 0 ? b0 + blk : g.n)) return;
@@ -80,7 +82,8 @@ template <int REG>
 __device__ __forceinline__ void coop_finish(const GeomDev& g, const double* wvn, double h, int i, int j, double rr,
                                             double L1, double L2, double L3, double* A, double* S, long long ld,
                                             bool regular = false) {
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double zf = xq[2], zp = xp[2];
   FinOut fo{0, 0, 0, 0, 0, 0};
@@ -116,7 +119,8 @@ __global__ __launch_bounds__(256) void k_assemble_list(GeomDev g, FreqDev f, con
   const uint32_t p = pairs[t];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
   if (REG == kEigene && done && done[t]) {
-    const double* xp = g.fp + 3 * (long long)i;
+    double xp[3];
+    field_point(g, i, j, xp);
     const double* xq = g.xyz_p + 3 * (long long)j;
     coop_finish<kEigene>(g, wvn, f.h, i, j, horiz_dist(xq[0], xp[0], xq[1], xp[1]), vreg[3 * t], vreg[3 * t + 1],
                          vreg[3 * t + 2], A, S, ld, true);
@@ -150,7 +154,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
   const bool given = REG == kEigene && valid && done && done[t];
   const uint32_t p = pairs[valid ? t : np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
@@ -194,7 +199,8 @@ k_assemble_sym(GeomDev g, FreqDev f, const uint32_t* __restrict__ pairs, long lo
   const bool given = REG == kEigene && valid && done && done[t];
   const uint32_t p = pairs[valid ? t : np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
@@ -240,7 +246,8 @@ __global__ __launch_bounds__(256) void k_pidcock_cache(GeomDev g, double h, cons
   if (t >= np) return;
   const uint32_t p = pairs[t];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const int nt = fin_pidcock_terms(rr, h);
@@ -278,7 +285,8 @@ hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqD
 
 // ---- classification --------------------------------------------------------
 __device__ __forceinline__ int pair_key(const GeomDev& g, double h, int i, int j) {
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const int reg = fin_regime(rr, h);
@@ -295,11 +303,12 @@ __device__ __forceinline__ bool mirrored_out(bool sym, int key, int i, int j) {
   return sym && (reg == kPidcock || reg == kEigene) && i > j;
 }
 
-__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk, int sym) {
+__global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, unsigned* hist, int blk, int sym,
+                                                        const int* __restrict__ blist) {
   __shared__ unsigned sh[kNumKeys];
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) sh[k] = 0;
   __syncthreads();
-  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  const int b0 = blk > 0 ? (blist ? blist[blockIdx.z] : (int)blockIdx.z) * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
   if (i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n)) {
@@ -313,12 +322,13 @@ __global__ __launch_bounds__(256) void k_classify_count(GeomDev g, double h, uns
 
 // one global atomic per (workgroup, key): ranks within the workgroup come from LDS
 __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, unsigned long long* cursor,
-                                                          uint32_t* pairs, int blk, int sym) {
+                                                          uint32_t* pairs, int blk, int sym,
+                                                          const int* __restrict__ blist) {
   __shared__ unsigned cnt[kNumKeys];
   __shared__ unsigned long long base[kNumKeys];
   for (int k = threadIdx.x; k < kNumKeys; k += blockDim.x) cnt[k] = 0;
   __syncthreads();
-  const int b0 = blk > 0 ? (int)blockIdx.z * blk : 0;
+  const int b0 = blk > 0 ? (blist ? blist[blockIdx.z] : (int)blockIdx.z) * blk : 0;
   const int i = b0 + blockIdx.y * 16 + (threadIdx.x >> 4);
   const int j = b0 + blockIdx.x * 16 + (threadIdx.x & 15);
   const bool inb = i < (blk > 0 ? b0 + blk : g.nt) && j < (blk > 0 ? b0 + blk : g.n);
@@ -332,24 +342,27 @@ __global__ __launch_bounds__(256) void k_classify_scatter(GeomDev g, double h, u
   if (in) pairs[base[key] + rank] = ((uint32_t)i << 16) | (uint32_t)j;
 }
 
-static dim3 pair_grid(const GeomDev& g, int blk) {
-  if (blk > 0) return dim3((blk + 15) / 16, (blk + 15) / 16, g.n / blk);
+static dim3 pair_grid(const GeomDev& g, int blk, int nblk = 0) {
+  if (blk > 0) return dim3((blk + 15) / 16, (blk + 15) / 16, nblk > 0 ? nblk : g.n / blk);
   return dim3((g.n + 15) / 16, (g.nt + 15) / 16);
 }
 
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist,
-                          unsigned long long* cursor, uint32_t* pairs, bool count_only, int blk, bool sym) {
-  const dim3 grid = pair_grid(g, blk);
-  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist, blk, sym ? 1 : 0);
-  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs, blk, sym ? 1 : 0);
+                          unsigned long long* cursor, uint32_t* pairs, bool count_only, int blk, bool sym,
+                          const int* blist, int nblist) {
+  if (blist && (blk <= 0 || nblist <= 0)) return hipErrorInvalidValue;
+  const dim3 grid = pair_grid(g, blk, blist ? nblist : 0);
+  if (count_only) hipLaunchKernelGGL(k_classify_count, grid, dim3(256), 0, s, g, h, hist, blk, sym ? 1 : 0, blist);
+  else hipLaunchKernelGGL(k_classify_scatter, grid, dim3(256), 0, s, g, h, cursor, pairs, blk, sym ? 1 : 0, blist);
   return hipGetLastError();
 }
 
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
-                         long long ld, int blk) {
-  const dim3 grid = pair_grid(g, blk);
-  if (deep) hipLaunchKernelGGL((k_assemble_grid<-1>), grid, dim3(256), 0, s, g, f, A, S, ld, blk);
-  else hipLaunchKernelGGL((k_assemble_grid<-2>), grid, dim3(256), 0, s, g, f, A, S, ld, blk);
+                         long long ld, int blk, const int* blist, int nblist) {
+  if (blist && (blk <= 0 || nblist <= 0)) return hipErrorInvalidValue;
+  const dim3 grid = pair_grid(g, blk, blist ? nblist : 0);
+  if (deep) hipLaunchKernelGGL((k_assemble_grid<-1>), grid, dim3(256), 0, s, g, f, A, S, ld, blk, blist);
+  else hipLaunchKernelGGL((k_assemble_grid<-2>), grid, dim3(256), 0, s, g, f, A, S, ld, blk, blist);
   return hipGetLastError();
 }
 
@@ -444,8 +457,9 @@ __device__ __forceinline__ void build_w_body(const GeomDev& g, const FreqDev& f,
     W[2 * off + 1] = 0.0;
   }
   for (int h = 0; h < nbeta; ++h) {
-    double qr, qi;
-    dinp_dot_n(f, g.xyz_p + 3 * (long long)j, g.dxyz + 6 * (long long)j, beta[h], xw, yw, qr, qi);
+    double qr, qi, xg[3];
+    global_centroid(g, j, xg);
+    dinp_dot_n(f, xg, g.dxyz + 6 * (long long)j, beta[h], xw, yw, qr, qi);
     const long long off = (long long)j + (long long)(m + h) * ldw;
     W[2 * off] = -qr;
     W[2 * off + 1] = -qi;
@@ -477,8 +491,9 @@ __device__ __forceinline__ void forces_body(const GeomDev& g, const FreqDev& f, 
     const long long off = (long long)i + (long long)c * ldx;
     double pr = X[2 * off], pi = X[2 * off + 1];
     if (c >= m) {
-      double vr, vi;
-      vinp(f, g.xyz_p + 3 * (long long)i, beta[c - m], xw, yw, vr, vi);
+      double vr, vi, xg[3];
+      global_centroid(g, i, xg);
+      vinp(f, xg, beta[c - m], xw, yw, vr, vi);
       pr += vr;
       pi += vi;
     }
@@ -625,7 +640,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CoopWaves<K
     int pp, i, j;
     decode_pair(c, pp, i, j);
     const GeomDev& g = P[p].g;
-    const double* xp = g.fp + 3 * (long long)i;
+    double xp[3];
+    field_point(g, i, j, xp);
     const double* xq = g.xyz_p + 3 * (long long)j;
     const double h = F[p].h, v = F[p].v;
     const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
@@ -811,7 +827,8 @@ __global__ __launch_bounds__(256) void k_field_rows(GeomDev g, FreqDev f, const 
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nfp * g.n) return;
   const int p = (int)(t % nfp), j = (int)(t / nfp);
-  const double* xp = xfp + 3 * (long long)p;
+  double xp[3];
+  global_to_src(g, j, xfp + 3 * (long long)p, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const int flag = near_flag(xp, xq, g.pnsz[j]);
   Grn4 gr;
@@ -828,7 +845,9 @@ __global__ __launch_bounds__(256) void k_field_rows(GeomDev g, FreqDev f, const 
 // the field point lies on hull panel j: within max(1e-6, 1e-3 PNSZ) of its plane
 // and inside its polygon (the cross products of the edges with the point keep
 // one sign, to 1e-3 PNSZ^2) -- CalPotentialMulti's free-term test
-__device__ bool on_panel(const GeomDev& g, int j, const double* x) {
+__device__ bool on_panel(const GeomDev& g, int j, const double* xg) {
+  double x[3];
+  global_to_src(g, j, xg, x);
   const double* v = g.verts + 12 * (long long)j;
   const double* en = g.dxyz + 6 * (long long)j;
   const int nv = g.ncn[j];

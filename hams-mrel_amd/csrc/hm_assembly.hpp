@@ -20,7 +20,48 @@ struct GeomDev {
   const double* ds;        // n areas
   const double* pnsz;      // n longest edges
   const double* dxyz;      // n*6 generalised normals
+  // body frames (nullptr: fp, xyz_p and verts are global coordinates). Otherwise
+  // they are local to each point's body and org[2 p], org[2 p + 1] is the
+  // horizontal placement of field point p's body (p < n: panel p's body)
+  const double* org = nullptr;
 };
+
+// Field point i as seen from source panel j: with body frames the field point
+// moves by the difference of the two bodies' placements, which is exactly zero
+// within one body, so that a body's own pairs see only its local coordinates.
+__device__ __forceinline__ void field_point(const GeomDev& g, int i, int j, double* xp) {
+  const double* x = g.fp + 3 * (long long)i;
+  xp[0] = x[0];
+  xp[1] = x[1];
+  xp[2] = x[2];
+  if (g.org) {
+    xp[0] = x[0] + (g.org[2 * (long long)i] - g.org[2 * (long long)j]);
+    xp[1] = x[1] + (g.org[2 * (long long)i + 1] - g.org[2 * (long long)j + 1]);
+  }
+}
+
+// a point x given in global coordinates, in the frame of source panel j
+__device__ __forceinline__ void global_to_src(const GeomDev& g, int j, const double* x, double* xp) {
+  xp[0] = x[0];
+  xp[1] = x[1];
+  xp[2] = x[2];
+  if (g.org) {
+    xp[0] = x[0] - g.org[2 * (long long)j];
+    xp[1] = x[1] - g.org[2 * (long long)j + 1];
+  }
+}
+
+// the global position of panel j's centroid (incident-wave phases)
+__device__ __forceinline__ void global_centroid(const GeomDev& g, int j, double* x) {
+  const double* c = g.xyz_p + 3 * (long long)j;
+  x[0] = c[0];
+  x[1] = c[1];
+  x[2] = c[2];
+  if (g.org) {
+    x[0] = c[0] + g.org[2 * (long long)j];
+    x[1] = c[1] + g.org[2 * (long long)j + 1];
+  }
+}
 
 // Per-frequency wave state.
 struct FreqDev {
@@ -30,8 +71,10 @@ struct FreqDev {
 
 // blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0);
 // sym keeps PIDCOCK and EIGENE pairs only as (i, j) with i < j (for assemble_sym)
+// blist (device, nblist entries): with blk > 0, only those diagonal blocks
 hipError_t classify_pairs(hipStream_t s, const GeomDev& g, double h, unsigned* hist, unsigned long long* cursor,
-                          uint32_t* pairs, bool count_only, int blk = 0, bool sym = false);
+                          uint32_t* pairs, bool count_only, int blk = 0, bool sym = false, const int* blist = nullptr,
+                          int nblist = 0);
 // PIDCOCK or EIGENE pairs (i, j), i < j, and their mirrors (j, i) together
 // (field points = panel centroids, ni = 0): one series evaluation serves both.
 // done, vreg (EIGENE, optional): pairs with done[t] != 0 take the regular parts
@@ -51,7 +94,7 @@ hipError_t assemble_sym(hipStream_t s, int regime, const GeomDev& g, const FreqD
                         long long np, double* A, double* S, long long ld, const uint8_t* done = nullptr,
                         const float* vreg = nullptr, PidCache pc = PidCache{nullptr, nullptr});
 hipError_t assemble_grid(hipStream_t s, const GeomDev& g, const FreqDev& f, bool deep, double* A, double* S,
-                         long long ld, int blk = 0);
+                         long long ld, int blk = 0, const int* blist = nullptr, int nblist = 0);
 // done, vreg (optional, EIGENE only): pairs with done[t] != 0 take the regular
 // part V = T - (1/r + 1/r1) of their evanescent sum T from vreg[3 t .. 3 t + 2]
 // (value, R- and z-derivative) instead of evaluating the series

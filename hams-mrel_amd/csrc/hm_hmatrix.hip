@@ -133,7 +133,8 @@ __device__ __forceinline__ int sorted_entry(const GeomDev& g, const FreqDev& f, 
     key = 0;
     if (f.h > 0.0 && !h.dense) {
       const int i = q.b1 * m + (row ? fixed : c0 + t), j = q.b2 * m + (row ? c0 + t : fixed);
-      const double* xp = g.fp + 3 * (long long)i;
+      double xp[3];
+      field_point(g, i, j, xp);
       const double* xq = g.xyz_p + 3 * (long long)j;
       const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
       const int reg = fin_regime(rr, f.h);

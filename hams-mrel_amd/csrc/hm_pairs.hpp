@@ -47,7 +47,8 @@ __device__ __forceinline__ void write_pair_g(const GeomDev& g, int i, int j, con
 // (cross approximation); wvn may be global or LDS memory
 __device__ __forceinline__ void pair_entry(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
                                            double& ar, double& ai, double& sr, double& si) {
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const int flag = near_flag(xp, xq, g.pnsz[j]);
   Grn4 gr;
@@ -69,7 +70,8 @@ __device__ __forceinline__ void pair_entry(const GeomDev& g, const FreqDev& f, c
 // take K0/K1 from sf_k01_fast.
 __device__ __forceinline__ bool deferred_entry(const GeomDev& g, const FreqDev& f, int i, int j, bool pid) {
   if (!(f.h > 0.0) || fabs(f.v) < F32(1.e-8) || fabs(f.v + 1.0) < F32(1.e-8)) return false;
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const int reg = fin_regime(horiz_dist(xq[0], xp[0], xq[1], xp[1]), f.h);
   return reg == kLinton || (!pid && reg == kPidcock);
@@ -81,7 +83,8 @@ template <int G, bool PID = true, bool DIRECT = false>
 __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev& f, const double* wvn, bool active,
                                                 int i, int j, double& ar, double& ai, double& sr, double& si) {
   constexpr int KP = (kPidcockMaxTerms + G - 1) / G, KE = (kEigeneMaxTerms + G - 1) / G;
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);

@@ -555,7 +555,8 @@ __global__ __launch_bounds__(256) void k_sur_regular(GeomDev g, FreqDev f, doubl
   const bool valid = t < np;
   const uint32_t p = valid ? pairs[t] : pairs[np - 1];
   const int i = (int)(p >> 16), j = (int)(p & 0xffffu);
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double h = f.h;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
@@ -754,7 +755,8 @@ __global__ __launch_bounds__(256) void k_sur_assemble_deep(GeomDev g, FreqDev f,
   const int i0 = blockIdx.y * 16 + (threadIdx.x >> 4), j0 = blockIdx.x * 16 + (threadIdx.x & 15);
   const bool valid = i0 < g.nt && j0 < g.n;
   const int i = valid ? i0 : 0, j = valid ? j0 : 0;
-  const double* xp = g.fp + 3 * (long long)i;
+  double xp[3];
+  field_point(g, i, j, xp);
   const double* xq = g.xyz_p + 3 * (long long)j;
   const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
   const double kr = f.v * rr, kv = f.v * (xp[2] + xq[2]);

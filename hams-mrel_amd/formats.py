@@ -243,7 +243,7 @@ def problem_from_control(inp, hull="HullMesh_{k}.pnl", single_hull="HullMesh.pnl
             wpath = os.path.join(inp, wname)
             if os.path.exists(wpath) or cf.nbody == 1:
                 wp = _waterplane(wpath, lcs, log)
-                body.ixyz, body.incn, body.incon = wp.xyz, wp.ncn, wp.ncon
+                body.ixyz, body.incn, body.incon, body.ilocal = wp.xyz, wp.ncn, wp.ncon, wp.local
             else:
                 log(" Warning: The waterplane mesh file does not exist for Body %d" % (b + 1))
         bodies.append(body)
@@ -361,6 +361,8 @@ def apply_generalized_modes(p, inp, log=print):
                     log("  Warning: Could not read line %d from %s" % (k + 1, path))
                     break
                 p.dxyz_p[n0 + pid - 1, dof - 1] = val
+                if getattr(p, "frames", False):
+                    p.local["dxyz_p"][n0 + pid - 1, dof - 1] = val
             done.append((b + 1, dof))
     return done
 

@@ -39,7 +39,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
-           "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep")
+           "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
+           "hm_get_body_classes", "hm_set_block_sharing")
 
 _lib = None
 
@@ -127,18 +128,23 @@ class Context:
             raise HMError(rc, f"{what}: {self.L.hm_error_string(self.ptr).decode()}")
 
     # ---- problem set-up -------------------------------------------------------
-    def setup(self, prob):
-        """prob: meshes.Problem"""
+    def setup(self, prob, frames=True):
+        """prob: meshes.Problem. frames: where the problem's bodies carry body
+        frames (local meshes and placements), upload the local geometry and
+        the placements (hm_set_body_origins) instead of the global geometry"""
         self.problem = prob
+        fr = frames and getattr(prob, "frames", False)
+        src = prob.local if fr else dict(xyz=prob.xyz, xyz_p=prob.xyz_p, ds=prob.ds, pnsz=prob.pnsz,
+                                         dxyz_p=prob.dxyz_p, ixyz_p=prob.ixyz_p)
         keep = dict(bs=np.ascontiguousarray(prob.body_start, dtype=np.int32),
-                    xyz=np.ascontiguousarray(prob.xyz, dtype=np.float64),
+                    xyz=np.ascontiguousarray(src["xyz"], dtype=np.float64),
                     ncn=np.ascontiguousarray(prob.ncn, dtype=np.int32),
                     ncon=np.ascontiguousarray(prob.ncon, dtype=np.int32),
-                    xp=np.ascontiguousarray(prob.xyz_p, dtype=np.float64),
-                    ds=np.ascontiguousarray(prob.ds, dtype=np.float64),
-                    pn=np.ascontiguousarray(prob.pnsz, dtype=np.float64),
-                    dx=np.ascontiguousarray(prob.dxyz_p, dtype=np.float64),
-                    ix=np.ascontiguousarray(prob.ixyz_p if prob.ni else np.zeros((1, 3)), dtype=np.float64),
+                    xp=np.ascontiguousarray(src["xyz_p"], dtype=np.float64),
+                    ds=np.ascontiguousarray(src["ds"], dtype=np.float64),
+                    pn=np.ascontiguousarray(src["pnsz"], dtype=np.float64),
+                    dx=np.ascontiguousarray(src["dxyz_p"], dtype=np.float64),
+                    ix=np.ascontiguousarray(src["ixyz_p"] if prob.ni else np.zeros((1, 3)), dtype=np.float64),
                     beta=np.ascontiguousarray(prob.beta, dtype=np.float64))
         self._chk(self.L.hm_set_constants(self.ptr, _d(prob.rho), _d(prob.g), _d(prob.g_disp), _d(prob.amp),
                                           _d(prob.xw[0]), _d(prob.xw[1])), "hm_set_constants")
@@ -146,8 +152,23 @@ class Context:
                                          _p(keep["xyz"]), _p(keep["ncn"]), _p(keep["ncon"]), _p(keep["xp"]),
                                          _p(keep["ds"]), _p(keep["pn"]), _p(keep["dx"]), _i(prob.ni),
                                          _p(keep["ix"])), "hm_set_geometry")
+        self.frames = fr
+        if fr:
+            org = np.ascontiguousarray(prob.local["origin"], dtype=np.float64)
+            wps = np.ascontiguousarray(prob.local["wp_start"], dtype=np.int32)
+            self._chk(self.L.hm_set_body_origins(self.ptr, _i(prob.nbody), _p(org), _p(wps) if prob.ni else None),
+                      "hm_set_body_origins")
         self._chk(self.L.hm_set_depth(self.ptr, _d(prob.h)), "hm_set_depth")
         self._chk(self.L.hm_set_headings(self.ptr, _i(prob.nbeta), _p(keep["beta"])), "hm_set_headings")
+
+    def body_classes(self):
+        """rep[b]: the first body whose local hull panels equal body b's"""
+        rep = np.zeros(self.problem.nbody, dtype=np.int32)
+        self._chk(self.L.hm_get_body_classes(self.ptr, _i(self.problem.nbody), _p(rep)), "hm_get_body_classes")
+        return rep
+
+    def set_block_sharing(self, enable=True):
+        self._chk(self.L.hm_set_block_sharing(self.ptr, _i(1 if enable else 0)), "hm_set_block_sharing")
 
     def set_frequency(self, w1):
         self._chk(self.L.hm_set_frequency(self.ptr, _d(w1)), "hm_set_frequency")

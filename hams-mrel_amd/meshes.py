@@ -20,13 +20,20 @@ import numpy as np
 
 @dataclass
 class Body:
-    xyz: np.ndarray            # (nnode, 3)
+    xyz: np.ndarray            # (nnode, 3) global coordinates
     ncn: np.ndarray            # (npanel,)  3 or 4
     ncon: np.ndarray           # (npanel, 4) 1-based node numbers (0 pads triangles)
     xr: tuple = (0.0, 0.0, 0.0)   # rotation centre of the rigid-body modes
     ixyz: np.ndarray = None    # waterplane nodes (IRR), or None
     incn: np.ndarray = None
     incon: np.ndarray = None
+    # body frame: the nodes in the body's own frame (local, ilocal) and its
+    # horizontal placement, xyz = local + (origin, 0) (the reference's local
+    # meshes placed by LCS_MULTI, ReadPanelMeshMulti.f90:52-60); None: the body
+    # frame is the global frame
+    local: np.ndarray = None
+    ilocal: np.ndarray = None
+    origin: tuple = (0.0, 0.0)
 
     @property
     def n(self):
@@ -123,30 +130,40 @@ def cylinder(radius, draft, n_theta, n_z, n_r, center=(0.0, 0.0), waterplane_nr=
     zs = -draft * np.arange(n_z + 1) / n_z
     for z in zs:
         for t in th:
-            nodes.append((cx + radius * np.cos(t), cy + radius * np.sin(t), z))
+            nodes.append((radius * np.cos(t), radius * np.sin(t), z))
     side = lambda k, t: k * n_theta + (t % n_theta) + 1
     pan = []
     for k in range(n_z):
         for t in range(n_theta):
             pan.append((side(k, t), side(k, t + 1), side(k + 1, t + 1), side(k + 1, t)))
-    pan += _disk(nodes, cx, cy, -draft, radius, n_theta, n_r)
-    xyz = np.array(nodes)
+    pan += _disk(nodes, 0.0, 0.0, -draft, radius, n_theta, n_r)
+    local = np.array(nodes)
     ncon = np.array(pan, dtype=np.int32)
     ncn = np.full(len(pan), 4, dtype=np.int32)
 
     def inside(xp):
         c = np.empty_like(xp)
-        c[:, 0], c[:, 1], c[:, 2] = cx, cy, -0.5 * draft
+        c[:, 0], c[:, 1], c[:, 2] = 0.0, 0.0, -0.5 * draft
         return c
-    ncon = _orient_inward(xyz, ncon, ncn, inside)
-    body = Body(xyz=xyz, ncn=ncn, ncon=ncon, xr=(cx, cy, 0.0))
+    ncon = _orient_inward(local, ncon, ncn, inside)
+    body = Body(xyz=_placed(local, center), ncn=ncn, ncon=ncon, xr=(cx, cy, 0.0), local=local,
+                origin=(float(cx), float(cy)))
     if waterplane_nr:
         wn = []
-        wp = _disk(wn, cx, cy, 0.0, radius, n_theta, waterplane_nr)
-        body.ixyz = np.array(wn)
+        wp = _disk(wn, 0.0, 0.0, 0.0, radius, n_theta, waterplane_nr)
+        body.ilocal = np.array(wn)
+        body.ixyz = _placed(body.ilocal, center)
         body.incon = np.array(wp, dtype=np.int32)
         body.incn = np.full(len(wp), 4, dtype=np.int32)
     return body
+
+
+def _placed(local, center):
+    """global nodes of a body frame placed at center"""
+    g = np.array(local, dtype=np.float64)
+    g[:, 0] = center[0] + g[:, 0]
+    g[:, 1] = center[1] + g[:, 1]
+    return g
 
 
 def hemisphere(radius, n_theta, n_phi, center=(0.0, 0.0)):
@@ -160,9 +177,9 @@ def hemisphere(radius, n_theta, n_phi, center=(0.0, 0.0)):
     for k in range(n_phi):
         a = 0.5 * np.pi * k / n_phi
         for t in th:
-            nodes.append((cx + radius * np.cos(a) * np.cos(t), cy + radius * np.cos(a) * np.sin(t),
+            nodes.append((radius * np.cos(a) * np.cos(t), radius * np.cos(a) * np.sin(t),
                           -radius * np.sin(a)))
-    nodes.append((cx, cy, -radius))
+    nodes.append((0.0, 0.0, -radius))
     pole = len(nodes)                                   # 1-based index of the pole
     ring = lambda k, t: k * n_theta + (t % n_theta) + 1
     pan = []
@@ -171,16 +188,16 @@ def hemisphere(radius, n_theta, n_phi, center=(0.0, 0.0)):
             pan.append((ring(k, t), ring(k, t + 1), ring(k + 1, t + 1), ring(k + 1, t)))
     for t in range(n_theta):
         pan.append((pole, ring(n_phi - 1, t), ring(n_phi - 1, t + 1), pole))
-    xyz = np.array(nodes)
+    local = np.array(nodes)
     ncon = np.array(pan, dtype=np.int32)
     ncn = np.full(len(pan), 4, dtype=np.int32)
 
     def inside(xp):
-        c = np.empty_like(xp)
-        c[:, 0], c[:, 1], c[:, 2] = cx, cy, 0.0
+        c = np.zeros_like(xp)
         return c
-    ncon = _orient_inward(xyz, ncon, ncn, inside)
-    return Body(xyz=xyz, ncn=ncn, ncon=ncon, xr=(cx, cy, 0.0))
+    ncon = _orient_inward(local, ncon, ncn, inside)
+    return Body(xyz=_placed(local, center), ncn=ncn, ncon=ncon, xr=(cx, cy, 0.0), local=local,
+                origin=(float(cx), float(cy)))
 
 
 def box(length, beam, draft, nx, ny, nz, center=(0.0, 0.0)):
@@ -252,7 +269,12 @@ def read_pnl(path, lcs=(0.0, 0.0, 0.0, 0.0), xr=(0.0, 0.0, 0.0)):
     xyz[:, 0] = lcs[0] + c * loc[:, 0] - s * loc[:, 1]
     xyz[:, 1] = lcs[1] + s * loc[:, 0] + c * loc[:, 1]
     xyz[:, 2] = loc[:, 2]
-    return Body(xyz=xyz, ncn=ncn, ncon=ncon, xr=tuple(xr))
+    # the body frame: the local nodes turned by the placement's rotation
+    local = np.empty_like(loc)
+    local[:, 0] = c * loc[:, 0] - s * loc[:, 1]
+    local[:, 1] = s * loc[:, 0] + c * loc[:, 1]
+    local[:, 2] = loc[:, 2]
+    return Body(xyz=xyz, ncn=ncn, ncon=ncon, xr=tuple(xr), local=local, origin=(float(lcs[0]), float(lcs[1])))
 
 
 # ---------------------------------------------------------------------------
@@ -305,6 +327,31 @@ class Problem:
         self.ixyz_p = np.concatenate(ixp) if ixp else np.zeros((0, 3))
         self.n = len(self.ncn)
         self.ni = len(self.ixyz_p)
+        # body frames: the same inputs from each body's local nodes (the panel
+        # geometry the device uses when frames are on) and the placements
+        self.frames = any(b.local is not None for b in self.bodies)
+        if self.frames:
+            lx, lxp, lds, lpn, ldx, lix, org, wps = [], [], [], [], [], [], [], [0]
+            for b in self.bodies:
+                o = np.array([b.origin[0], b.origin[1], 0.0]) if b.local is not None else np.zeros(3)
+                loc = b.local if b.local is not None else b.xyz
+                if b.local is not None:
+                    scale = max(1.0, float(np.max(np.abs(b.xyz))))
+                    if b.local.shape != b.xyz.shape or np.max(np.abs(b.local + o - b.xyz)) > 1e-9 * scale:
+                        raise ValueError("a body's local nodes placed at its origin are not its global nodes "
+                                         "(set local=None, or move local with xyz)")
+                g = panel_geometry(loc, b.ncn, b.ncon, np.asarray(b.xr, dtype=np.float64) - o)
+                lx.append(loc)
+                lxp.append(g[0]); lds.append(g[1]); lpn.append(g[2]); ldx.append(g[3])
+                if b.ni:
+                    iloc = b.ilocal if b.local is not None and b.ilocal is not None else b.ixyz - o
+                    lix.append(panel_geometry(iloc, b.incn, b.incon)[0])
+                org.append(o[:2])
+                wps.append(wps[-1] + b.ni)
+            self.local = dict(xyz=np.concatenate(lx), xyz_p=np.concatenate(lxp), ds=np.concatenate(lds),
+                              pnsz=np.concatenate(lpn), dxyz_p=np.concatenate(ldx),
+                              ixyz_p=np.concatenate(lix) if lix else np.zeros((0, 3)),
+                              origin=np.array(org, dtype=np.float64), wp_start=np.array(wps, dtype=np.int32))
 
     @property
     def m(self):

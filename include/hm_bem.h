@@ -52,6 +52,27 @@ int hm_set_constants(hm_ctx *ctx, double rho, double g, double g_disp, double am
 int hm_set_geometry(hm_ctx *ctx, int nbody, const int *body_start, int ntnd, const double *xyz, const int *ncn,
                     const int *ncon, const double *xyz_p, const double *ds, const double *pnsz, const double *dxyz_p,
                     int ni, const double *ixyz_p);
+/* Body frames: declares that the coordinates given to hm_set_geometry (nodes,
+ * centroids, waterplane points) are each body's local coordinates, and places
+ * body b at the horizontal offset (origin[2b], origin[2b+1]). The reference
+ * reads body meshes the same way, local nodes placed by LCS_MULTI
+ * (ReadPanelMeshMulti.f90:52-60; pass the rotation already applied to the
+ * local coordinates). Pair geometry is then formed as the local difference
+ * plus the difference of the placements, exactly the local difference within
+ * one body: bodies with bitwise equal local hull panels have bitwise equal
+ * diagonal blocks, on every path. wp_start[nbody+1] assigns the ni waterplane
+ * points to bodies (NULL when ni = 0). Call after hm_set_geometry (which
+ * clears the frames). No counterpart in the reference. */
+int hm_set_body_origins(hm_ctx *ctx, int nbody, const double *origin, const int *wp_start);
+/* rep[b] = the first body whose local hull panels equal body b's (b itself
+ * when none does; every b without body frames). */
+int hm_get_body_classes(hm_ctx *ctx, int nbody, int *rep);
+/* H-matrix form (hm_set_hmatrix): bodies that share a representative (see
+ * hm_get_body_classes) share its diagonal block: it is assembled, factored
+ * and inverted once per frequency and copied (enable = 1, the default), or
+ * every body's block is computed (enable = 0). Either way the blocks are the
+ * same bit for bit. No counterpart in the reference. */
+int hm_set_block_sharing(hm_ctx *ctx, int enable);
 /* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
 int hm_set_depth(hm_ctx *ctx, double h);
 /* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */

@@ -38,7 +38,7 @@ def _members():
 def _single(prob, w, variant):
     c = hm_bem.Context()
     try:
-        c.setup(prob)
+        c.setup(prob, frames=False)          # the batch takes global coordinates
         c.set_series_variant(variant)
         return c.solve_frequency(w, hm_bem.HM_FP64)
     finally:

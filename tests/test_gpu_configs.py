@@ -31,9 +31,9 @@ pytestmark = pytest.mark.gpu
 KEYS = ("amas", "bdmp", "exfc")
 
 
-def _ctx(p):
+def _ctx(p, frames=True):
     c = hm_bem.Context()
-    c.setup(p)
+    c.setup(p, frames=frames)
     return c
 
 
@@ -91,7 +91,7 @@ def test_c3_batch_members_against_single_contexts():
         for w in ws:
             got = b.solve(w)
             for k, prob in enumerate(members):
-                c = _ctx(prob)
+                c = _ctx(prob, frames=False)          # the batch takes global coordinates
                 try:
                     c.set_series_variant(info["series_variant"])
                     ref = c.solve_frequency(w, hm_bem.HM_FP64)

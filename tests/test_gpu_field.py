@@ -47,7 +47,11 @@ def test_field_points_match_oracle(oracle, h):
     assert ref["info"] == 0
     errs = {}
     for k in ("pot", "prs", "elv"):
-        scale = np.max(np.abs(ref[k]), axis=0)
+        # per column, at least 1e-3 of the largest value: the yaw columns of a
+        # vertical cylinder are rounding noise (~1e-15 of the largest value, its
+        # yaw normals vanish), whose pattern depends on how the normals were
+        # rounded (body frames or global coordinates)
+        scale = np.maximum(np.max(np.abs(ref[k]), axis=0), 1e-3 * np.max(np.abs(ref[k])))
         errs[k] = float(np.max(np.abs(got[k] - ref[k]) / scale))
     print("h", h, "field-point errors against the oracle", errs, flush=True)
     bar = 2e-5 if h > 0 else 1e-12

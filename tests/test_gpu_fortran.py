@@ -93,7 +93,7 @@ def _case(tmp_path, p, w1):
     grn, sij, dij, out, x, info, (hy, hz, hinfo) = _read(fout, p, len(pts), len(sgl))
     c = hm_bem.Context()
     try:
-        c.setup(p)
+        c.setup(p, frames=False)             # the Fortran layer passes global coordinates
         v = w1 * w1 / p.g
         if p.h > 0:
             wvn = hm_bem.dispersion(w1, p.h, p.g_disp)

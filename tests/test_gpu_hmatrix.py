@@ -173,7 +173,8 @@ def test_linton_entries_between_stacked_bodies():
     assembly at eps = 1e-7, full rank allowed."""
     import dataclasses
     top = meshes.cylinder(3.0, 4.0, 16, 4, 2)
-    low = dataclasses.replace(top, xyz=top.xyz + np.array([0.0, 0.0, -10.0]), xr=(0.0, 0.0, -10.0))
+    low = dataclasses.replace(top, xyz=top.xyz + np.array([0.0, 0.0, -10.0]), local=top.local + np.array([0.0, 0.0, -10.0]),
+                              xr=(0.0, 0.0, -10.0))
     p = meshes.Problem([top, low], h=60.0, omegas=[0.5, 1.6], beta=np.array([0.0]))
     m = p.n // 2
     xy = p.xyz_p[:, :2]
