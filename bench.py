@@ -480,6 +480,7 @@ def main():
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
                     setup_s=hm_setup_s, distinct_bodies=int(len(set(ctx.body_classes().tolist()))),
+                    diag_blocks_assembled=ctx.block_sharing()[0], aca_processes_run=ctx.block_sharing()[1],
                     block_sharing=not args.no_block_sharing, without_block_sharing=no_share)
         ctx.set_hmatrix(False)
         print(f"# hmatrix {hmat}", file=sys.stderr, flush=True)

@@ -88,6 +88,12 @@ struct hm_ctx {
   bool block_share = true;
   DevBuf d_ublk;                // the distinct bodies (classification of the diagonal blocks)
   int n_ublk = 0;
+  std::vector<double> origin_host;   // the placements (2 per body)
+  // cross approximation: process p's block equals process pshare[p]'s (equal
+  // bodies at an equal offset, with block sharing on)
+  DevBuf d_pshare;
+  bool pshare_ok = false;            // d_pshare holds the current geometry's map
+  int n_pshared = 0;                 // processes that copy another's block
   bool sharing() const {
     if (!block_share || !hm_on) return false;
     for (int b = 0; b < nbody; ++b)
@@ -122,6 +128,7 @@ struct hm_ctx {
   bool lu_coop = true;        // dense LU panels by one cooperative launch each where the grid fits
   void* luws() const { return lu_coop ? d_luws.p : nullptr; }
   DevBuf d_Pinv;              // explicit FP32 inverses of the preconditioner blocks
+  DevBuf d_gjws;              // workspace of their Gauss-Jordan inversion (H-matrix form)
   DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
   bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
   std::vector<int> gm_blocks;
@@ -377,11 +384,11 @@ extern "C" int hm_destroy(hm_ctx* c) {
   if (!c) return HM_ERR_ARG - 1;
   if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
+  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_gjws, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
                     &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
                     &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw,
                     &c->d_surd, &c->d_surdcnt};
@@ -485,6 +492,8 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   c->pnsz_host.assign(pnsz, pnsz + n);
   c->dxyz_host.assign(dxyz_p, dxyz_p + 6 * (size_t)n);
   c->frames = false;
+  c->pshare_ok = false;
+  c->origin_host.assign(2 * (size_t)nbody, 0.0);
   c->body_rep.resize(nbody);
   for (int b = 0; b < nbody; ++b) c->body_rep[b] = b;
   c->have_geom = true;
@@ -495,6 +504,8 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   c->assembled = c->factored = c->solved = false;
   return HM_OK;
 }
+
+static int pshare_build(hm_ctx* c);
 
 // body b's hull panels are bitwise those of body a: every input of the
 // influence matrices' (a, a) block equals that of (b, b)
@@ -544,6 +555,8 @@ extern "C" int hm_set_body_origins(hm_ctx* c, int nbody, const double* origin, c
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream), "body origins");
   c->frames = true;
+  c->origin_host.assign(origin, origin + 2 * nbody);
+  c->pshare_ok = false;
   // global centroids for the body gaps and the cluster tree
   for (int j = 0; j < n; ++j) {
     c->xyz_host[3 * (size_t)j] = c->fp_host[3 * (size_t)j] + org[2 * (size_t)j];
@@ -570,6 +583,19 @@ extern "C" int hm_get_body_classes(hm_ctx* c, int nbody, int* rep) {
   if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
   if (nbody != c->nbody || !rep) return c->fail(HM_ERR_ARG - 2, "bad output array");
   for (int b = 0; b < nbody; ++b) rep[b] = c->body_rep[b];
+  return HM_OK;
+}
+
+extern "C" int hm_get_block_sharing(hm_ctx* c, int* diag_blocks, int* aca_processes) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
+  const bool sh = c->sharing();
+  if (sh) {
+    int rc = pshare_build(c);
+    if (rc) return rc;
+  }
+  if (diag_blocks) *diag_blocks = sh ? c->n_ublk : c->nbody;
+  if (aca_processes) *aca_processes = hm::hmat_processes(c->nbody) - (sh ? c->n_pshared : 0);
   return HM_OK;
 }
 
@@ -804,6 +830,41 @@ static int assemble_pairs(hm_ctx* c, double* A, double* S, long long ld, int blk
 
 static int precond_early(hm_ctx* c);
 
+// the cross approximation's process map (hm::HmatDev::share): process (mat,
+// b1, b2) takes the block of the first process of the same matrix whose bodies
+// have the same representatives and whose placements differ by the same
+// vector, computed as the entry kernels compute it (o_b1 - o_b2): its entries
+// are the same bit for bit
+static int pshare_build(hm_ctx* c) {
+  if (c->pshare_ok) return HM_OK;
+  const int nb = c->nbody, per = nb * (nb - 1), P = hm::hmat_processes(nb);
+  std::vector<int> sh(P);
+  c->n_pshared = 0;
+  for (int p = 0; p < P; ++p) {
+    const int mat = p / per, r = p % per, b1 = r / (nb - 1), b2p = r % (nb - 1), b2 = b2p + (b2p >= b1 ? 1 : 0);
+    const double dx = c->origin_host[2 * b1] - c->origin_host[2 * b2];
+    const double dy = c->origin_host[2 * b1 + 1] - c->origin_host[2 * b2 + 1];
+    sh[p] = p;
+    for (int q = mat * per; q < p; ++q) {
+      if (sh[q] != q) continue;
+      const int rq = q % per, c1 = rq / (nb - 1), c2p = rq % (nb - 1), c2 = c2p + (c2p >= c1 ? 1 : 0);
+      if (c->body_rep[c1] != c->body_rep[b1] || c->body_rep[c2] != c->body_rep[b2]) continue;
+      const double ex = c->origin_host[2 * c1] - c->origin_host[2 * c2];
+      const double ey = c->origin_host[2 * c1 + 1] - c->origin_host[2 * c2 + 1];
+      if (memcmp(&ex, &dx, sizeof(double)) == 0 && memcmp(&ey, &dy, sizeof(double)) == 0) {
+        sh[p] = q;
+        ++c->n_pshared;
+        break;
+      }
+    }
+  }
+  int rc = upload(c, c->d_pshare, sh.data(), sh.size(), "cross approximation map");
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "cross approximation map");
+  c->pshare_ok = true;
+  return HM_OK;
+}
+
 // H-matrix diagonal blocks shared by bodies with equal hull panels: body b's
 // block (elements [b stride, b stride + count) of `base`, `esize` bytes each)
 // becomes a copy of its representative's
@@ -858,6 +919,11 @@ static int assemble_hmatrix(hm_ctx* c) {
   HIPCHK(c, c->d_W.ensure(cplx * (size_t)n * ncol), "assemble alloc W");
   HIPCHK(c, c->d_R.ensure(cplx * (size_t)n * ncol), "assemble alloc RHS");
   c->hmat = hm::hmat_carve(c->d_hmat.p, nb, m, kmax, c->hm_eps, c->hm_eps_s);
+  if (c->sharing()) {
+    int rc0 = pshare_build(c);
+    if (rc0) return rc0;
+    if (c->n_pshared > 0) c->hmat.share = c->d_pshare.as<int>();
+  }
   const hm::GeomDev gd = c->geom();
   const hm::FreqDev f = c->freq();
   double* D = c->d_hD.as<double>();
@@ -970,6 +1036,38 @@ static int precond_prepare(hm_ctx* c, hipStream_t s) {
   return HM_OK;
 }
 
+// the H-matrix form's preconditioner as explicit inverses (precond 2 and -1):
+// the FP32 copy of each distinct body's diagonal block inverted in place in
+// d_Pinv by Gauss-Jordan elimination without row interchanges (the blocks are
+// 2 pi I plus a compact part: no pivoting is needed, and D_b M_b^-1 is the
+// identity to FP32 rounding as the Krylov steps assume), copied to the bodies
+// that share it; the permutations are the identity
+static int precond_gj(hm_ctx* c, hipStream_t s, int* info) {
+  const int nb = c->nbody, m = c->bm();
+  const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+  const double* D = c->d_hD.as<double>();
+  float* Pinv = c->d_Pinv.as<float>();
+  const bool share = c->sharing();
+  const int nw = share ? 1 : nb;
+  HIPCHK(c, c->d_gjws.ensure(hm::gj_inverse_ws_bytes(m) * (size_t)nw), "preconditioner alloc");
+  if (share) {
+    for (int b = 0; b < nb; ++b) {
+      if (c->body_rep[b] != b) continue;
+      HIPCHK(c, hm::demote(s, D + 2 * b * sD, m, sD, Pinv + 2 * b * sc, m, sc, m, m, 1), "demote");
+      HIPCHK(c, hm::gj_inverse_nopiv(s, Pinv + 2 * b * sc, m, sc, m, c->d_gjws.p, info + b, 1), "preconditioner inverse");
+    }
+    int rc = copy_shared_blocks(c, s, Pinv, sc, sc, 2 * sizeof(float));
+    if (rc) return rc;
+  } else {
+    HIPCHK(c, hm::demote(s, D, m, sD, Pinv, m, sc, m, m, nb), "demote");
+    HIPCHK(c, hm::gj_inverse_nopiv(s, Pinv, m, sc, m, c->d_gjws.p, info, nb), "preconditioner inverse");
+  }
+  HIPCHK(c, hm::fill_iota(s, c->d_pperm.as<int>(), c->n), "preconditioner permutation");
+  c->gm_blocks = c->body_start;
+  c->pinv_ready = true;
+  return HM_OK;
+}
+
 // H-matrix form, right after the diagonal blocks are assembled: their FP32
 // copies, the block preconditioner's LU and (precond 2, -1) its inverses on the
 // main stream, overlapping the cross approximation on the side stream (own info
@@ -993,9 +1091,14 @@ static int precond_early(hm_ctx* c) {
   HIPCHK(c, hipMemsetAsync(c->d_info2.p, 0, sizeof(int) * nb, s), "preconditioner");
   HIPCHK(c, hipMemsetAsync(c->d_counter2.p, 0, sizeof(unsigned) * nb, s), "preconditioner");
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
+  int rc;
+  if (block_inverse(c)) {
+    if ((rc = precond_gj(c, s, c->d_info2.as<int>()))) return rc;
+    c->prec_early = true;
+    return HM_OK;
+  }
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
-  int rc = factor_body_blocks(c, s, c->d_info2.as<int>(), c->d_counter2.as<unsigned>());
-  if (rc) return rc;
+  if ((rc = factor_body_blocks(c, s, c->d_info2.as<int>(), c->d_counter2.as<unsigned>()))) return rc;
   c->gm_blocks = c->body_start;
   if ((rc = precond_prepare(c, s))) return rc;
   c->prec_early = true;
@@ -1035,6 +1138,12 @@ static int factor_hmatrix(hm_ctx* c) {
          "demote");
   HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
          "demote");
+  if (c->gm_precond != 0 && block_inverse(c)) {
+    HIPCHK(c, c->d_pperm.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
+    HIPCHK(c, c->d_Pinv.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
+    HIPCHK(c, hipMemsetAsync(c->d_info.p, 0, sizeof(int) * nb, c->stream), "factor");
+    return precond_gj(c, c->stream, c->d_info.as<int>());
+  }
   int rc = factor_body_blocks(c, c->stream, c->d_info.as<int>(), c->d_counter.as<unsigned>());
   if (rc) return rc;
   c->gm_blocks = c->body_start;

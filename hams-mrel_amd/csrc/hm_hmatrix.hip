@@ -358,6 +358,32 @@ __global__ __launch_bounds__(256) void k_aca_finish_col(HmatDev h) {
   }
 }
 
+// processes whose block is another process's: marked done (conv = 3) before
+// the first step
+__global__ void k_aca_share_mark(HmatDev h, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < P && h.share[p] != p) h.conv[p] = 3;
+}
+
+// after the last step: a shared process takes its representative's factors
+// (the first rank columns of U and V; the rest stay zero), rank, norm and state
+__global__ __launch_bounds__(256) void k_aca_share_copy(HmatDev h) {
+  const int p = blockIdx.y;
+  const int r = h.share[p];
+  if (r == p) return;
+  const Proc q = proc(h, p), qr = proc(h, r);
+  const long long len = 2LL * h.m * h.rank[r];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long long)gridDim.x * blockDim.x) {
+    q.U[e] = qr.U[e];
+    q.V[e] = qr.V[e];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    h.rank[p] = h.rank[r];
+    h.norm2[p] = h.norm2[r];
+    h.conv[p] = h.conv[r];
+  }
+}
+
 __global__ void k_count_open(const int* conv, int n, int* out) {
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
@@ -441,6 +467,7 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.col_used = (int*)take(4 * P * m);
   h.counter = (int*)take(4);
   h.narrow = 0;
+  h.share = nullptr;
   return h;
 }
 
@@ -461,6 +488,10 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   if ((e = hipMemsetAsync(h.col_used, 0, 4 * (size_t)P * h.m, s)) != hipSuccess) return e;
   int* cnt = h.counter;
   int open = P, st = 0;
+  if (h.share) {
+    hipLaunchKernelGGL(k_aca_share_mark, dim3((P + 255) / 256), dim3(256), 0, s, h, P);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   // each row step either adds a rank (then a column step) or retires a zero row;
   // at most kmax + m row steps per process
   const int max_steps = h.kmax + h.m;
@@ -488,6 +519,10 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
       if ((e = hipMemcpyAsync(&open, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
       if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     }
+  }
+  if (h.share) {
+    hipLaunchKernelGGL(k_aca_share_copy, dim3(64, P), dim3(256), 0, s, h);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_count_open, dim3(1), dim3(256), 0, s, h.conv, P, cnt);
   int bad = 0;

@@ -45,6 +45,10 @@ struct HmatDev {
   int *rank, *conv, *next_row, *pivot, *skip;   // [process]
   int *row_used, *col_used;                     // [process][m]
   int* counter;                                 // one int for host polls
+  // share[p] (nullptr: none): the process whose block equals process p's bit
+  // for bit (equal bodies at an equal offset); p itself for the processes
+  // that run. The others are skipped (conv = 3) and copied at the end.
+  const int* share;
   // 1: the bodies' horizontal bounding circles are >= 0.05 h apart, so no
   // off-diagonal entry is PIDCOCK or LINTON; the entry kernels then take the
   // EIGENE-only lane groups of 8 (any such entry would still be redone per lane)

@@ -1247,4 +1247,167 @@ template hipError_t lu_solve<float>(hipStream_t, const float*, int, long long, i
 template hipError_t lu_inverse_nopiv<float>(hipStream_t, const float*, int, long long, int, float*, int, long long,
                                             int);
 
+// ---------------------------------------------------------------------------
+// Explicit inverse by blocked Gauss-Jordan elimination, no row interchanges
+// ---------------------------------------------------------------------------
+// For matrices whose elimination needs no pivoting (the diagonal blocks of the
+// influence matrix: 2 pi on the diagonal plus a compact part; growth factor
+// ~1): step k, with the 64 rows and columns R_k = [k0, k0 + w),
+//   Pinv = (A[R_k, R_k])^-1                        (every workgroup, in LDS)
+//   T    = Pinv A[R_k, :], T[:, R_k] = Pinv        (k_gj_row, one 64-column tile each)
+//   C    = A[:, R_k] with rows R_k zeroed; A[:, R_k] = 0 off R_k; A[R_k, :] = T   (k_gj_col)
+//   A   -= C T                                     (one n x n x w complex GEMM on MFMA)
+// after the last step A holds A^-1. n^3 complex multiply-adds in all, nearly all
+// in the GEMMs, against LU + triangular inverses whose panels are latency bound.
+constexpr int kGj = 64;
+
+// the pivot tile (planes re/im, rows padded to 65) inverted in place; thread t
+// owns row t >> 2 and columns 16 (t & 3) .. +15. *bad = 1 on a zero pivot.
+__device__ void gj_tile_invert(float (*tr)[kGj + 1], float (*ti)[kGj + 1], int* bad) {
+  const int i = threadIdx.x >> 2, c0 = 16 * (threadIdx.x & 3);
+  for (int j = 0; j < kGj; ++j) {
+    const float dr = tr[j][j], di = ti[j][j];
+    const float den = dr * dr + di * di;
+    float rr = 0.0f, ri = 0.0f;
+    if (den > 0.0f) { rr = dr / den; ri = -di / den; }
+    else if (threadIdx.x == 0) *bad = 1;
+    const float fr = tr[i][j], fi = ti[i][j];
+    float ur[16], ui[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const float xr = tr[j][c0 + q], xi = ti[j][c0 + q];
+      ur[q] = xr * rr - xi * ri;          // row j scaled by 1 / pivot
+      ui[q] = xr * ri + xi * rr;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = c0 + q;
+      if (i == j) {
+        tr[i][c] = c == j ? rr : ur[q];
+        ti[i][c] = c == j ? ri : ui[q];
+      } else if (c == j) {
+        tr[i][c] = -(fr * rr - fi * ri);
+        ti[i][c] = -(fr * ri + fi * rr);
+      } else {
+        tr[i][c] -= fr * ur[q] - fi * ui[q];
+        ti[i][c] -= fr * ui[q] + fi * ur[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// T[:, c0 .. c0 + 63] of step k0 (blockIdx.x = column tile, blockIdx.z = matrix)
+__global__ __launch_bounds__(256) void k_gj_row(const float* __restrict__ A, int lda, long long sA, int n, int k0,
+                                                float* __restrict__ Tb, long long sT, int* info) {
+  __shared__ float pr[kGj][kGj + 1], pi[kGj][kGj + 1], ar[kGj][kGj + 1], ai[kGj][kGj + 1];
+  __shared__ int bad;
+  const float* a = A + 2 * sA * blockIdx.z;
+  float* tb = Tb + 2 * sT * blockIdx.z;
+  const int w = n - k0 < kGj ? n - k0 : kGj, c0 = kGj * blockIdx.x;
+  const int wc = n - c0 < kGj ? n - c0 : kGj;
+  if (threadIdx.x == 0) bad = 0;
+  for (int e = threadIdx.x; e < kGj * kGj; e += blockDim.x) {
+    const int r = e % kGj, c = e / kGj;
+    const bool in = r < w && c < w;
+    const long long o = (long long)(k0 + r) + (long long)(k0 + c) * lda;
+    pr[r][c] = in ? a[2 * o] : (r == c ? 1.0f : 0.0f);
+    pi[r][c] = in ? a[2 * o + 1] : 0.0f;
+    const bool ina = r < w && c < wc;
+    const long long oa = (long long)(k0 + r) + (long long)(c0 + c) * lda;
+    ar[r][c] = ina ? a[2 * oa] : 0.0f;
+    ai[r][c] = ina ? a[2 * oa + 1] : 0.0f;
+  }
+  __syncthreads();
+  gj_tile_invert(pr, pi, &bad);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && bad && info[blockIdx.z] == 0) info[blockIdx.z] = k0 + 1;
+  const int i = threadIdx.x >> 2, q0 = 16 * (threadIdx.x & 3);
+  if (i >= w) return;
+  const bool diag = c0 == k0;
+  for (int q = 0; q < 16; ++q) {
+    const int c = q0 + q;
+    if (c >= wc) break;
+    float sr = 0.0f, si = 0.0f;
+    if (diag) {
+      sr = pr[i][c];
+      si = pi[i][c];
+    } else {
+      for (int k = 0; k < w; ++k) {
+        sr += pr[i][k] * ar[k][c] - pi[i][k] * ai[k][c];
+        si += pr[i][k] * ai[k][c] + pi[i][k] * ar[k][c];
+      }
+    }
+    const long long o = (long long)i + (long long)(c0 + c) * kGj;
+    tb[2 * o] = sr;
+    tb[2 * o + 1] = si;
+  }
+}
+
+// C = A[:, R_k] with rows R_k zero, A[:, R_k] = 0 off R_k, A[R_k, :] = T
+__global__ __launch_bounds__(256) void k_gj_col(float* __restrict__ A, int lda, long long sA, int n, int k0,
+                                                const float* __restrict__ Tb, long long sT, float* __restrict__ Cb,
+                                                long long sC) {
+  float* a = A + 2 * sA * blockIdx.z;
+  const float* tb = Tb + 2 * sT * blockIdx.z;
+  float* cb = Cb + 2 * sC * blockIdx.z;
+  const int w = n - k0 < kGj ? n - k0 : kGj;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x, half = (long long)n * kGj;
+  if (e < half) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    if (c >= w) return;
+    const long long o = (long long)r + (long long)(k0 + c) * lda, oc = (long long)r + (long long)c * n;
+    if (r >= k0 && r < k0 + w) {
+      cb[2 * oc] = 0.0f;
+      cb[2 * oc + 1] = 0.0f;
+    } else {
+      cb[2 * oc] = a[2 * o];
+      cb[2 * oc + 1] = a[2 * o + 1];
+      a[2 * o] = 0.0f;
+      a[2 * o + 1] = 0.0f;
+    }
+  } else if (e < 2 * half) {
+    const long long f = e - half;
+    const int r = (int)(f % kGj), c = (int)(f / kGj);
+    if (r >= w) return;
+    const long long o = (long long)(k0 + r) + (long long)c * lda, ot = (long long)r + (long long)c * kGj;
+    a[2 * o] = tb[2 * ot];
+    a[2 * o + 1] = tb[2 * ot + 1];
+  }
+}
+
+__global__ void k_iota(int* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int)i;
+}
+
+hipError_t fill_iota(hipStream_t s, int* p, long long n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+size_t gj_inverse_ws_bytes(int n) { return 2 * 2 * sizeof(float) * (size_t)n * kGj; }
+
+hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int n, void* ws, int* info, int batch) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  float* T = static_cast<float*>(ws);
+  float* C = T + 2 * (size_t)batch * n * kGj;
+  const long long sT = (long long)n * kGj, sC = (long long)n * kGj;
+  const int ntile = (n + kGj - 1) / kGj;
+  const long long colw = 2LL * n * kGj;
+  hipError_t e;
+  for (int k0 = 0; k0 < n; k0 += kGj) {
+    const int w = n - k0 < kGj ? n - k0 : kGj;
+    hipLaunchKernelGGL(k_gj_row, dim3(ntile, 1, batch), dim3(256), 0, s, A, lda, sA, n, k0, T, sT, info);
+    hipLaunchKernelGGL(k_gj_col, dim3((unsigned)((colw + 255) / 256), 1, batch), dim3(256), 0, s, A, lda, sA, n, k0,
+                       T, sT, C, sC);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = cgemm<float>(s, false, n, n, w, -1.0f, 0.0f, C, n, sC, T, kGj, sT, 1.0f, 0.0f, A, lda, sA, batch)) !=
+        hipSuccess)
+      return e;
+  }
+  return hipSuccess;
+}
+
 }  // namespace hm

@@ -35,6 +35,15 @@ template <typename T>
 hipError_t lu_inverse_nopiv(hipStream_t s, const T* A, int lda, long long sA, int n, T* B, int ldb, long long sB,
                             int batch);
 
+// In-place explicit inverse of `batch` n x n FP32 matrices (stride sA) by
+// blocked Gauss-Jordan elimination without row interchanges (for matrices that
+// need none: no pivot search, nearly all the work in n x n x 64 GEMMs on MFMA).
+// ws: gj_inverse_ws_bytes(n) per matrix; info[b] = first column + 1 of a tile
+// with a zero pivot, or left as it is (zeroed by the caller).
+size_t gj_inverse_ws_bytes(int n);
+hipError_t fill_iota(hipStream_t s, int* p, long long n);   // p[i] = i
+hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int n, void* ws, int* info, int batch);
+
 // ---- grouped operations: members of different sizes, one launch per step ----
 template <typename T>
 struct GemmDesc {            // C = alpha op(A) B + beta C for one member

@@ -40,7 +40,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
-           "hm_get_body_classes", "hm_set_block_sharing")
+           "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing")
 
 _lib = None
 
@@ -166,6 +166,13 @@ class Context:
         rep = np.zeros(self.problem.nbody, dtype=np.int32)
         self._chk(self.L.hm_get_body_classes(self.ptr, _i(self.problem.nbody), _p(rep)), "hm_get_body_classes")
         return rep
+
+    def block_sharing(self):
+        """(diagonal blocks assembled, cross-approximation processes run) per
+        frequency of the H-matrix form"""
+        d, a = _i(0), _i(0)
+        self._chk(self.L.hm_get_block_sharing(self.ptr, ctypes.byref(d), ctypes.byref(a)), "hm_get_block_sharing")
+        return d.value, a.value
 
     def set_block_sharing(self, enable=True):
         self._chk(self.L.hm_set_block_sharing(self.ptr, _i(1 if enable else 0)), "hm_set_block_sharing")

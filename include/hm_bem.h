@@ -69,10 +69,16 @@ int hm_set_body_origins(hm_ctx *ctx, int nbody, const double *origin, const int 
 int hm_get_body_classes(hm_ctx *ctx, int nbody, int *rep);
 /* H-matrix form (hm_set_hmatrix): bodies that share a representative (see
  * hm_get_body_classes) share its diagonal block: it is assembled, factored
- * and inverted once per frequency and copied (enable = 1, the default), or
- * every body's block is computed (enable = 0). Either way the blocks are the
- * same bit for bit. No counterpart in the reference. */
+ * and inverted once per frequency and copied; off-diagonal blocks between
+ * such bodies at an equal offset share one cross approximation (enable = 1,
+ * the default). enable = 0: every block is computed. Either way the blocks
+ * are the same bit for bit. No counterpart in the reference. */
 int hm_set_block_sharing(hm_ctx *ctx, int enable);
+/* With the H-matrix form on: the diagonal blocks assembled per frequency and
+ * the cross-approximation processes run (one per off-diagonal block and
+ * matrix; blocks of equal bodies at an equal offset, o_b1 - o_b2 bit for bit,
+ * take one process's factors). No counterpart in the reference. */
+int hm_get_block_sharing(hm_ctx *ctx, int *diag_blocks, int *aca_processes);
 /* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
 int hm_set_depth(hm_ctx *ctx, double h);
 /* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */

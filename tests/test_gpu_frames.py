@@ -6,7 +6,8 @@ bodies' placements, which is exactly zero within one body. Bodies whose local
 hull panels are bitwise equal (translated copies, as in an array of identical
 devices) then have bitwise equal diagonal blocks, on the dense path too. The
 H-matrix form assembles, factors and inverts such a block once per frequency
-and copies it to the bodies that share it.
+and copies it to the bodies that share it; the off-diagonal blocks of pairs of
+such bodies at an equal offset share one cross approximation.
 
 Bars:
 - the dense FP64 path's diagonal blocks of equal bodies: equal bit for bit;
@@ -92,6 +93,10 @@ def test_hmatrix_sharing_is_bitwise(h):
             c.set_hmatrix(True, eps=1e-6, eps_s=1e-7, kmax=50)
             c.set_gmres(restart=50, tol=1e-6)
             c.set_block_sharing(share)
+        # 2 x 2 equal bodies: one diagonal block; 12 ordered pairs at 8 distinct
+        # offsets, per matrix (A and S)
+        assert on.block_sharing() == (1, 16)
+        assert off.block_sharing() == (4, 24)
         for w in p.omegas:
             a = on.solve_frequency(w, hm_bem.HM_GMRES)
             b = off.solve_frequency(w, hm_bem.HM_GMRES)
