@@ -108,7 +108,8 @@ PATH_DESC = {
                 "inverted once per frequency), off-diagonal blocks by adaptive cross approximation (tolerance 1e-6 for A, 1e-7 "
                 "for S) whose entries come straight from the FP64 Green function with directly summed evanescent "
                 "modes, right-hand sides S W from the H-matrix of S; restarted GMRES(50), tol 1e-6, with FP32 "
-                "MFMA H-matrix products, FP32 block-diagonal LU preconditioner and FP64 H-matrix residuals"),
+                "MFMA H-matrix products, FP32 block-diagonal preconditioner (explicit inverses) and FP64 H-matrix "
+                "residuals; several frequencies in flight (clones of the context, one host thread each)"),
 }
 
 
@@ -132,6 +133,27 @@ def time_path(ctx, prob, precision, steps, warmup, stage_times=True, ranks=None)
     dt = time.perf_counter() - t0
     stages = ctx.timing()
     ctx.set_timing(False)
+    return dt, res, stages
+
+
+def time_path_inflight(ctx, prob, precision, steps, warmup, workers, stage_times=True):
+    """the same frequencies through hm_run_frequencies with `workers`
+    frequencies in flight (clones of the context made by the warm-up call);
+    stage times are those of the frequencies the context itself took"""
+    om = prob.omegas
+    ctx.set_workers(workers)
+    ctx.run_frequencies([float(om[k % len(om)]) for k in range(warmup)], precision)
+    sync()
+    ctx.set_timing(stage_times)
+    w = [float(om[(warmup + k) % len(om)]) for k in range(steps)]
+    t0 = time.perf_counter()
+    a, b, f = ctx.run_frequencies(w, precision)
+    sync()
+    dt = time.perf_counter() - t0
+    stages = ctx.timing()
+    ctx.set_timing(False)
+    ctx.set_workers(1)
+    res = [dict(amas=a[k], bdmp=b[k], exfc=f[k]) for k in range(steps)]
     return dt, res, stages
 
 
@@ -354,6 +376,9 @@ def main():
     ap.add_argument("--lu-panel", choices=("coop", "column"), default="coop",
                     help="dense LU panels: one cooperative launch each (default) or per-column launches (for "
                          "profilers that do not handle cooperative launches)")
+    ap.add_argument("--workers", type=int, default=None,
+                    help="frequencies in flight on the measured path (hm_set_workers: clones of the context, one "
+                         "host thread each); default 2 on the H-matrix path, 1 otherwise")
     ap.add_argument("--no-block-sharing", action="store_true",
                     help="H-matrix path: assemble, factor and invert every body's diagonal block, also where bodies "
                          "are equal (the blocks are the same bit for bit either way)")
@@ -432,8 +457,15 @@ def main():
         print(f"# hmatrix set-up (diagonal-block classification, PIDCOCK term cache) {hm_setup_s:.2f} s",
               file=sys.stderr, flush=True)
     ranks_mx = []
-    dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup,
-                                     ranks=ranks_mx if args.path == "hmatrix" else None)
+    if args.workers is None:
+        args.workers = 2 if args.path == "hmatrix" else 1
+    if args.workers > 1:
+        dt_mx, res_mx, st_mx = time_path_inflight(ctx, prob, prec, args.steps, args.warmup, args.workers)
+    else:
+        dt_mx, res_mx, st_mx = time_path(ctx, prob, prec, args.steps, args.warmup,
+                                         ranks=ranks_mx if args.path == "hmatrix" else None)
+    # systems the context itself solved in the timed region (its stage times cover those)
+    nsys_main = st_mx["forces"][1] if st_mx.get("forces", (0, 0))[1] > 0 else args.steps
     gm = ctx.gmres_stats() if prec == hm_bem.HM_GMRES else None
     field = None
     if args.field_points > 0:
@@ -487,7 +519,8 @@ def main():
     if sur is not None:
         sur["pairs"] = ctx.surrogate_counts()
         ctx.set_surrogate(False)
-    print(f"# {args.path}: {args.steps} systems in {dt_mx:.3f} s; stages {st_mx}; gmres {gm}; surrogate {sur}",
+    print(f"# {args.path}: {args.steps} systems in {dt_mx:.3f} s ({args.workers} in flight); stages {st_mx}; gmres {gm}; "
+          f"surrogate {sur}",
           file=sys.stderr, flush=True)
     others = {}
     if args.path != "fp32ir" and args.with_fp32ir:
@@ -535,7 +568,7 @@ def main():
     st_rf = st_mx
     # per system: the regime launches' own times (one launch each) and the
     # cross approximation's entry launches measured alone
-    cand = {k: (v[0] / args.steps, v[1] / args.steps) for k, v in st_rf.items()
+    cand = {k: (v[0] / nsys_main, v[1] / nsys_main) for k, v in st_rf.items()
             if k in REGIME_STAGE and v[1] > 0}
     if aca_alone is not None and "aca_entries" in aca_alone["stages_ms"]:
         ms, cnt = aca_alone["stages_ms"]["aca_entries"]
@@ -601,8 +634,13 @@ def main():
             "data": "synthetic (deterministic meshes of the named workload)",
             "config": {"workload": prob.name, "panels": prob.n, "waterplane_panels": prob.ni,
                        "modes": prob.m, "headings": prob.nbeta, "frequencies": len(prob.omegas),
-                       "path": PATH_DESC[args.path], "parallelism": "1 GPU"},
-            "stages_ms": {k: v[0] for k, v in st_mx.items()},
+                       "path": PATH_DESC[args.path], "frequencies_in_flight": args.workers,
+                       "parallelism": "1 GPU"},
+            "workers": args.workers,
+            "stages_ms_per_system": {k: v[0] / nsys_main for k, v in st_mx.items()},
+            "stages_note": (f"per system, over the {nsys_main} of the {args.steps} timed systems that the context itself "
+                            f"solved (the other {args.steps - nsys_main} ran on its clones, concurrently)"
+                            if nsys_main != args.steps else "per system over the timed systems"),
             "plain_fp64": plain, "roofline": roofline}
     if value is None:
         line["value_without_accuracy"] = args.steps / dt_mx

@@ -12,11 +12,13 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <functional>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hm_noalloc.hpp"
@@ -89,6 +91,21 @@ struct hm_ctx {
   DevBuf d_ublk;                // the distinct bodies (classification of the diagonal blocks)
   int n_ublk = 0;
   std::vector<double> origin_host;   // the placements (2 per body)
+  std::vector<int> wp_start_host;    // the waterplane points' bodies (hm_set_body_origins)
+  // the arguments of hm_set_geometry that the host copies above do not keep
+  std::vector<double> geom_xyz;
+  std::vector<int> geom_ncon;
+  int geom_ntnd = 0;
+  int device = 0;
+  // hm_run_frequencies keeps `nworkers` frequencies in flight: this context and
+  // nworkers - 1 clones of its configuration (own streams and buffers), each
+  // solving the frequencies it takes in a host thread of its own. cfg_gen
+  // counts the changes of geometry, depth, headings, constants and surrogate
+  // parameters; the clones are rebuilt when it or a setting differs.
+  int nworkers = 1;
+  std::vector<hm_ctx*> workers;
+  unsigned long long cfg_gen = 0, workers_gen = ~0ULL;
+  std::string workers_sig;
   // cross approximation: process p's block equals process pshare[p]'s (equal
   // bodies at an equal offset, with block sharing on)
   DevBuf d_pshare;
@@ -363,7 +380,8 @@ extern "C" int hm_create(hm_ctx** out) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return HM_ERR_HIP - (int)hipErrorNoDevice;
   hm_ctx* c = new hm_ctx();
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
@@ -382,6 +400,8 @@ extern "C" int hm_create(hm_ctx** out) {
 
 extern "C" int hm_destroy(hm_ctx* c) {
   if (!c) return HM_ERR_ARG - 1;
+  for (hm_ctx* w : c->workers) hm_destroy(w);
+  c->workers.clear();
   if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
@@ -415,6 +435,7 @@ extern "C" int hm_set_constants(hm_ctx* c, double rho, double g, double g_disp, 
   if (!(g_disp > 0)) return c->fail(HM_ERR_ARG - 4, "g_disp must be > 0");
   c->rho = rho; c->g = g; c->g_disp = g_disp; c->amp = amp; c->xw = xw; c->yw = yw;
   c->have_freq = false;
+  ++c->cfg_gen;
   return HM_OK;
 }
 
@@ -487,6 +508,11 @@ extern "C" int hm_set_geometry(hm_ctx* c, int nbody, const int* body_start, int 
   HIPCHK(c, hipStreamSynchronize(c->stream), "geometry upload");
   c->fp_host = std::move(fp);
   c->verts_host = std::move(verts);
+  c->geom_xyz.assign(xyz, xyz + 3 * (size_t)ntnd);
+  c->geom_ncon.assign(ncon, ncon + 4 * (size_t)n);
+  c->geom_ntnd = ntnd;
+  c->wp_start_host.clear();
+  ++c->cfg_gen;
   c->ncn_host.assign(ncn, ncn + n);
   c->ds_host.assign(ds, ds + n);
   c->pnsz_host.assign(pnsz, pnsz + n);
@@ -556,6 +582,9 @@ extern "C" int hm_set_body_origins(hm_ctx* c, int nbody, const double* origin, c
   HIPCHK(c, hipStreamSynchronize(c->stream), "body origins");
   c->frames = true;
   c->origin_host.assign(origin, origin + 2 * nbody);
+  c->wp_start_host.assign(nbody + 1, 0);
+  if (c->ni > 0) c->wp_start_host.assign(wp_start, wp_start + nbody + 1);
+  ++c->cfg_gen;
   c->pshare_ok = false;
   // global centroids for the body gaps and the cluster tree
   for (int j = 0; j < n; ++j) {
@@ -698,6 +727,7 @@ extern "C" int hm_set_depth(hm_ctx* c, double h) {
   if (!c->have_geom) return c->fail(HM_ERR_STATE, "hm_set_depth before hm_set_geometry");
   if (h == 0.0) return c->fail(HM_ERR_ARG - 2, "depth must be > 0 (finite) or < 0 (infinite)");
   c->h = h;
+  ++c->cfg_gen;
   c->have_depth = true;
   c->have_freq = false;
   c->assembled = c->factored = c->solved = false;
@@ -711,6 +741,7 @@ extern "C" int hm_set_headings(hm_ctx* c, int nbeta, const double* beta) {
   if (!c) return HM_ERR_ARG - 1;
   if (nbeta < 0 || (nbeta > 0 && !beta)) return c->fail(HM_ERR_ARG - 2, "bad headings");
   c->beta.assign(beta, beta + nbeta);
+  ++c->cfg_gen;
   if (nbeta) {
     int rc = upload(c, c->d_beta, beta, nbeta, "headings");
     if (rc) return rc;
@@ -944,10 +975,6 @@ static int assemble_hmatrix(hm_ctx* c) {
   }
   int rc = assemble_pairs(c, D, DS, m, m);
   if (rc) return rc;
-  if ((rc = copy_shared_blocks(c, c->stream, D, m * (m + 1), m * m, 2 * sizeof(double))) ||
-      (rc = copy_shared_blocks(c, c->stream, DS, m * (m + 1), m * m, 2 * sizeof(double))))
-    return rc;
-  if ((rc = precond_early(c))) return rc;
   // no PIDCOCK / LINTON entry between bodies whose bounding circles are >= 0.05 h apart
   c->hmat.narrow = c->h > 0.0 && c->min_body_gap > 0.05 * c->h * (1.0 + 1e-9) ? 1 : 0;
   c->stage_begin(kAsmCross, acs);
@@ -958,8 +985,20 @@ static int assemble_hmatrix(hm_ctx* c) {
                     if (b) cc->stage_begin(kAcaEntries, cc->aca_stream);
                     else cc->stage_end(kAcaEntries, cc->aca_stream);
                   }};
-  HIPCHK(c, hm::aca_build(acs, gd, f, c->hmat, &c->hm_unconverged, &c->hm_steps, c->timing ? &tm : nullptr),
+  // with the overlap, the steps a rank-kmax block needs are enqueued on the side
+  // stream before the main stream's copies and preconditioner, so that the cross
+  // approximation (the longer of the two) starts with the diagonal blocks; the
+  // host then polls for processes still open
+  int steps = 0;
+  HIPCHK(c, hm::aca_begin(acs, gd, f, c->hmat, c->hm_overlap ? kmax + 4 : 0, &steps, c->timing ? &tm : nullptr),
          "cross approximation");
+  if ((rc = copy_shared_blocks(c, c->stream, D, m * (m + 1), m * m, 2 * sizeof(double))) ||
+      (rc = copy_shared_blocks(c, c->stream, DS, m * (m + 1), m * m, 2 * sizeof(double))))
+    return rc;
+  if ((rc = precond_early(c))) return rc;
+  HIPCHK(c, hm::aca_end(acs, gd, f, c->hmat, &steps, &c->hm_unconverged, c->timing ? &tm : nullptr),
+         "cross approximation");
+  c->hm_steps = steps;
   c->stage_end(kAsmCross, acs);
   c->join_side();
   if ((rc = c->stage_end(kAssembly))) return rc;
@@ -2134,11 +2173,13 @@ static int train_surrogate(hm_ctx* c, bool deep, long long nsamples, int epochs,
 
 extern "C" int hm_surrogate_train(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
                                   unsigned long long seed, int quant, double* stats) {
+  if (c) ++c->cfg_gen;
   return train_surrogate(c, false, nsamples, epochs, batch, lr, seed, quant, stats);
 }
 
 extern "C" int hm_surrogate_train_deep(hm_ctx* c, long long nsamples, int epochs, int batch, double lr,
                                        unsigned long long seed, int quant, double* stats) {
+  if (c) ++c->cfg_gen;
   return train_surrogate(c, true, nsamples, epochs, batch, lr, seed, quant, stats);
 }
 
@@ -2202,6 +2243,7 @@ extern "C" int hm_surrogate_set(hm_ctx* c, const float* blob, int n) {
   HIPCHK(c, hm::sur_round(c->stream, b, b + hm::kSurParams, c->sur_img()), "surrogate set");
   HIPCHK(c, hipStreamSynchronize(c->stream), "surrogate set");
   c->sur_ready = true;
+  ++c->cfg_gen;
   c->assembled = c->factored = c->solved = false;
   return HM_OK;
 }
@@ -2340,16 +2382,113 @@ extern "C" int hm_solve_frequency(hm_ctx* c, double w1, int precision, double* a
   return hm_forces(c, amas, bdmp, exfc);
 }
 
+// the settings a clone must share with its context (with cfg_gen)
+static std::string worker_signature(const hm_ctx* c) {
+  char b[512];
+  snprintf(b, sizeof b, "%d %d %d %d %d %d %d %.17g %.17g %d %d %d %.17g %d %d %d %d %.17g %d %d %d %d %.17g %d",
+           c->series_variant, c->lu_coop, c->pair_sym, c->block_share, c->hm_on, c->hm_tree, c->ht_mode, c->hm_eps,
+           c->hm_eps_s, c->hm_kmax, c->hm_overlap, c->gm_restart, c->gm_tol, c->gm_maxiter, c->gm_precond,
+           c->sur_on, c->sur_quant, c->sur_tol, c->surd_on, c->frames, c->ht_leaf, c->nbody, c->ht_eta,
+           c->sur_ready);
+  return b;
+}
+
+// a clone of c's configuration: geometry (and body frames), depth, headings,
+// constants, kernel and solver settings, the learned surrogate's parameters
+static int clone_config(hm_ctx* w, hm_ctx* c) {
+  int rc;
+  if ((rc = hm_set_constants(w, c->rho, c->g, c->g_disp, c->amp, c->xw, c->yw))) return rc;
+  const int n = c->n;
+  if ((rc = hm_set_geometry(w, c->nbody, c->body_start.data(), c->geom_ntnd, c->geom_xyz.data(), c->ncn_host.data(),
+                            c->geom_ncon.data(), c->fp_host.data(), c->ds_host.data(), c->pnsz_host.data(),
+                            c->dxyz_host.data(), c->ni, c->ni ? c->fp_host.data() + 3 * (size_t)n : nullptr)))
+    return rc;
+  if (c->frames &&
+      (rc = hm_set_body_origins(w, c->nbody, c->origin_host.data(), c->ni ? c->wp_start_host.data() : nullptr)))
+    return rc;
+  w->series_variant = c->series_variant;
+  w->lu_coop = c->lu_coop;
+  w->pair_sym = c->pair_sym;
+  w->block_share = c->block_share;
+  w->hm_overlap = c->hm_overlap;
+  w->ht_mode = c->ht_mode;
+  w->ht_leaf = c->ht_leaf;
+  w->ht_eta = c->ht_eta;
+  if ((rc = hm_set_depth(w, c->h))) return rc;
+  if ((rc = hm_set_headings(w, (int)c->beta.size(), c->beta.data()))) return rc;
+  if ((rc = hm_set_gmres(w, c->gm_restart, c->gm_tol, c->gm_maxiter, c->gm_precond))) return rc;
+  if (c->sur_ready) {
+    std::vector<float> blob(hm::kSurBlob);
+    if ((rc = hm_surrogate_get(c, blob.data(), hm::kSurBlob)) || (rc = hm_surrogate_set(w, blob.data(), hm::kSurBlob)))
+      return rc;
+    w->sur_quant = c->sur_quant;
+    if ((rc = hm_set_surrogate(w, c->sur_on ? 1 : 0, c->sur_tol))) return rc;
+  }
+  if ((c->hm_on || c->hm_tree) && (rc = hm_set_hmatrix(w, 1, c->hm_eps, c->hm_eps_s, c->hm_kmax))) return rc;
+  return HM_OK;
+}
+
+extern "C" int hm_set_workers(hm_ctx* c, int nworkers) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (nworkers < 1 || nworkers > 8) return c->fail(HM_ERR_ARG - 2, "nworkers must be in 1..8");
+  c->nworkers = nworkers;
+  return HM_OK;
+}
+
 extern "C" int hm_run_frequencies(hm_ctx* c, int nfreq, const double* w1, int precision, double* amas, double* bdmp,
                                   double* exfc) {
   if (!c) return HM_ERR_ARG - 1;
   if (nfreq < 0 || (nfreq && !w1)) return c->fail(HM_ERR_ARG - 2, "bad frequency list");
   const size_t mm = (size_t)c->m() * c->m(), mb = 2 * (size_t)c->m() * c->beta.size();
-  for (int k = 0; k < nfreq; ++k) {
-    int rc = hm_solve_frequency(c, w1[k], precision, amas ? amas + k * mm : nullptr, bdmp ? bdmp + k * mm : nullptr,
-                                exfc ? exfc + k * mb : nullptr);
-    if (rc) return rc;
+  auto one = [&](hm_ctx* x, int k) {
+    return hm_solve_frequency(x, w1[k], precision, amas ? amas + k * mm : nullptr, bdmp ? bdmp + k * mm : nullptr,
+                              exfc ? exfc + k * mb : nullptr);
+  };
+  // several frequencies in flight need clones of a configuration they can copy
+  // (not the infinite-depth surrogate or a cluster tree built from a matrix)
+  const int nw = std::min(c->nworkers, nfreq);
+  const bool can = nw > 1 && c->have_depth && !c->surd_on && !c->ht_from_build && !c->hd_ready;
+  if (!can) {
+    for (int k = 0; k < nfreq; ++k) {
+      int rc = one(c, k);
+      if (rc) return rc;
+    }
+    return HM_OK;
   }
+  const std::string sig = worker_signature(c);
+  if (c->workers_gen != c->cfg_gen || c->workers_sig != sig || (int)c->workers.size() < nw - 1) {
+    for (hm_ctx* w : c->workers) hm_destroy(w);
+    c->workers.clear();
+    for (int k = 0; k < c->nworkers - 1; ++k) {
+      hm_ctx* w = nullptr;
+      int rc = hm_create(&w);
+      if (rc) return c->fail(rc, "worker context: hm_create failed");
+      c->workers.push_back(w);
+      if ((rc = clone_config(w, c))) return c->fail(rc, "worker context: %s", w->err.c_str());
+    }
+    c->workers_gen = c->cfg_gen;
+    c->workers_sig = sig;
+  }
+  // each context takes the next frequency not yet taken
+  std::atomic<int> next(0);
+  std::vector<int> rcs(nw, HM_OK);
+  auto run = [&](int t) {
+    hm_ctx* x = t == 0 ? c : c->workers[t - 1];
+    (void)hipSetDevice(c->device);
+    for (int k = next++; k < nfreq; k = next++) {
+      int rc = one(x, k);
+      if (rc) { rcs[t] = rc; next = nfreq; break; }
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nw; ++t) th.emplace_back(run, t);
+  run(0);
+  for (std::thread& t : th) t.join();
+  for (int t = 0; t < nw; ++t)
+    if (rcs[t]) {
+      if (t > 0) return c->fail(rcs[t], "frequency worker %d: %s", t, c->workers[t - 1]->err.c_str());
+      return rcs[t];
+    }
   return HM_OK;
 }
 

@@ -471,9 +471,33 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   return h;
 }
 
-hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
-                     int* steps, const AcaTimer* tm) {
+// one step of every open process: residual pivot row, pivot column, residual
+// pivot column, norm estimate and next pivot row
+static hipError_t aca_step(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int P,
+                           const AcaTimer* tm) {
   auto mark = [tm](int b) { if (tm) tm->mark(tm->user, b); };
+  const dim3 eval((h.m + 255) / 256, P);
+  mark(1);
+  if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);
+  else hipLaunchKernelGGL((k_aca_eval_row<16, true, false>), eval, dim3(256), 0, s, g, f, h);
+  mark(0);
+  // narrow: every pair of bodies is more than 0.05 h apart (bounding circles of
+  // the panel centroids), so no entry is LINTON or PIDCOCK and the fix-up
+  // kernels would only test and return; skipping them keeps the stream from
+  // waiting for CUs behind the other stream's launches for nothing
+  if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
+  hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
+  mark(1);
+  if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false, true>), eval, dim3(256), 0, s, g, f, h);
+  else hipLaunchKernelGGL((k_aca_eval_col<16, true, false>), eval, dim3(256), 0, s, g, f, h);
+  mark(0);
+  if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
+  hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
+  return hipGetLastError();
+}
+
+hipError_t aca_begin(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int blind, int* steps,
+                     const AcaTimer* tm) {
   const int P = hmat_processes(h.nb, h.nmat);
   const size_t uv = 16 * (size_t)P * h.m * h.kmax;
   hipError_t e;
@@ -486,39 +510,36 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   if ((e = hipMemsetAsync(h.skip, 0, 4 * (size_t)P, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(h.row_used, 0, 4 * (size_t)P * h.m, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(h.col_used, 0, 4 * (size_t)P * h.m, s)) != hipSuccess) return e;
-  int* cnt = h.counter;
-  int open = P, st = 0;
   if (h.share) {
     hipLaunchKernelGGL(k_aca_share_mark, dim3((P + 255) / 256), dim3(256), 0, s, h, P);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // each row step either adds a rank (then a column step) or retires a zero row;
+  // each row step either adds a rank (then a column step) or retires a zero row:
   // at most kmax + m row steps per process
+  const int n = blind < h.kmax + h.m ? blind : h.kmax + h.m;
+  for (int st = 0; st < n; ++st)
+    if ((e = aca_step(s, g, f, h, P, tm)) != hipSuccess) return e;
+  *steps = n;
+  return hipSuccess;
+}
+
+hipError_t aca_end(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* steps, int* unconverged,
+                   const AcaTimer* tm) {
+  const int P = hmat_processes(h.nb, h.nmat);
+  int* cnt = h.counter;
+  hipError_t e;
+  int st = *steps;
   const int max_steps = h.kmax + h.m;
-  for (; st < max_steps && open > 0; ++st) {
-    const dim3 eval((h.m + 255) / 256, P);
-    mark(1);
-    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);
-    else hipLaunchKernelGGL((k_aca_eval_row<16, true, false>), eval, dim3(256), 0, s, g, f, h);
-    mark(0);
-    // narrow: every pair of bodies is more than 0.05 h apart (bounding circles of
-    // the panel centroids), so no entry is LINTON or PIDCOCK and the fix-up
-    // kernels would only test and return; skipping them keeps the stream from
-    // waiting for CUs behind the other stream's launches for nothing
-    if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_row, eval, dim3(256), 0, s, g, f, h);
-    hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
-    mark(1);
-    if (h.narrow) hipLaunchKernelGGL((k_aca_eval_col<8, false, true>), eval, dim3(256), 0, s, g, f, h);
-    else hipLaunchKernelGGL((k_aca_eval_col<16, true, false>), eval, dim3(256), 0, s, g, f, h);
-    mark(0);
-    if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
-    hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (st % 4 == 3) {
-      hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, h.conv, P, cnt);
-      if ((e = hipMemcpyAsync(&open, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-      if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    }
+  // further steps while a process is open (a process stops at its tolerance,
+  // at rank kmax, or when every row is used): polled every 4 steps
+  for (;;) {
+    int open = 0;
+    hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, h.conv, P, cnt);
+    if ((e = hipMemcpyAsync(&open, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (open == 0 || st >= max_steps) break;
+    for (int q = 0; q < 4 && st < max_steps; ++q, ++st)
+      if ((e = aca_step(s, g, f, h, P, tm)) != hipSuccess) return e;
   }
   if (h.share) {
     hipLaunchKernelGGL(k_aca_share_copy, dim3(64, P), dim3(256), 0, s, h);
@@ -529,8 +550,18 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   if ((e = hipMemcpyAsync(&bad, cnt, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   if (unconverged) *unconverged = bad;
-  if (steps) *steps = st;
+  *steps = st;
   return hipSuccess;
+}
+
+hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
+                     int* steps, const AcaTimer* tm) {
+  int st = 0;
+  hipError_t e = aca_begin(s, g, f, h, 0, &st, tm);
+  if (e != hipSuccess) return e;
+  e = aca_end(s, g, f, h, &st, unconverged, tm);
+  if (steps) *steps = st;
+  return e;
 }
 
 hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, double* Uc, double* Vc) {

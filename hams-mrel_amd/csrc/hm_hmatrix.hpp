@@ -72,6 +72,15 @@ struct AcaTimer {
 // unconverged processes in *unconverged
 hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* unconverged,
                      int* steps, const AcaTimer* tm = nullptr);
+// the same in two parts: aca_begin clears the state and enqueues `blind` steps
+// without waiting for the device (*steps = the steps enqueued); aca_end polls,
+// runs further steps while a process is open, then counts the unconverged
+// processes (*steps = all steps). The host returns from aca_begin at once, so
+// other streams' work can be enqueued while the steps run.
+hipError_t aca_begin(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int blind, int* steps,
+                     const AcaTimer* tm = nullptr);
+hipError_t aca_end(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* steps, int* unconverged,
+                   const AcaTimer* tm = nullptr);
 
 // the first kuse columns of the U and V factors of matrix `mat` (0: A, 1: S),
 // packed with stride m kuse per block (same block order) into Uc, Vc

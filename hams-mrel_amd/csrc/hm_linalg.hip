@@ -1261,44 +1261,47 @@ template hipError_t lu_inverse_nopiv<float>(hipStream_t, const float*, int, long
 // in the GEMMs, against LU + triangular inverses whose panels are latency bound.
 constexpr int kGj = 64;
 
-// the pivot tile (planes re/im, rows padded to 65) inverted in place; thread t
-// owns row t >> 2 and columns 16 (t & 3) .. +15. *bad = 1 on a zero pivot.
-__device__ void gj_tile_invert(float (*tr)[kGj + 1], float (*ti)[kGj + 1], int* bad) {
-  const int i = threadIdx.x >> 2, c0 = 16 * (threadIdx.x & 3);
+// the pivot tile inverted by one wave, lane r holding row r in registers: per
+// column j the pivot row is broadcast from lane j with readlane (j is a
+// constant of the unrolled loop) and every lane applies one rank-1 update,
+//   g = f / p (lane != j, f = its entry in column j) or 1 - 1/p (lane j),
+//   x[c] -= g row_j[c] (c != j),   x[j] = -g (lane != j) or 1/p (lane j),
+// which leaves lane j's row scaled by 1/p and column j eliminated elsewhere.
+// *bad = 1 on a zero pivot.
+__device__ __forceinline__ float lane_bcast(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+__device__ __forceinline__ void gj_wave_invert(float (&xr)[kGj], float (&xi)[kGj], int lane, int* bad) {
+  // the rows stay in registers: column j is picked by selects over the unrolled
+  // columns, never by a dynamic index
   for (int j = 0; j < kGj; ++j) {
-    const float dr = tr[j][j], di = ti[j][j];
+    float fr = 0.0f, fi = 0.0f;
+#pragma unroll
+    for (int c = 0; c < kGj; ++c) {
+      fr = c == j ? xr[c] : fr;
+      fi = c == j ? xi[c] : fi;
+    }
+    const float dr = lane_bcast(fr, j), di = lane_bcast(fi, j);
     const float den = dr * dr + di * di;
     float rr = 0.0f, ri = 0.0f;
     if (den > 0.0f) { rr = dr / den; ri = -di / den; }
-    else if (threadIdx.x == 0) *bad = 1;
-    const float fr = tr[i][j], fi = ti[i][j];
-    float ur[16], ui[16];
+    else if (lane == 0) *bad = 1;
+    float gr = fr * rr - fi * ri, gi = fr * ri + fi * rr;
+    if (lane == j) { gr = 1.0f - rr; gi = -ri; }
+    const float jr = lane == j ? rr : -gr, ji = lane == j ? ri : -gi;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const float xr = tr[j][c0 + q], xi = ti[j][c0 + q];
-      ur[q] = xr * rr - xi * ri;          // row j scaled by 1 / pivot
-      ui[q] = xr * ri + xi * rr;
+    for (int c = 0; c < kGj; ++c) {
+      const float ur = lane_bcast(xr[c], j), ui = lane_bcast(xi[c], j);
+      const float nr = xr[c] - (gr * ur - gi * ui), ni = xi[c] - (gr * ui + gi * ur);
+      xr[c] = c == j ? jr : nr;
+      xi[c] = c == j ? ji : ni;
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = c0 + q;
-      if (i == j) {
-        tr[i][c] = c == j ? rr : ur[q];
-        ti[i][c] = c == j ? ri : ui[q];
-      } else if (c == j) {
-        tr[i][c] = -(fr * rr - fi * ri);
-        ti[i][c] = -(fr * ri + fi * rr);
-      } else {
-        tr[i][c] -= fr * ur[q] - fi * ui[q];
-        ti[i][c] -= fr * ui[q] + fi * ur[q];
-      }
-    }
-    __syncthreads();
   }
 }
 
-// T[:, c0 .. c0 + 63] of step k0 (blockIdx.x = column tile, blockIdx.z = matrix)
+// T[:, c0 .. c0 + 63] of step k0 (blockIdx.x = column tile, blockIdx.z = matrix):
+// wave 0 inverts the pivot tile, then the four waves form Pinv A[R_k, tile]
 __global__ __launch_bounds__(256) void k_gj_row(const float* __restrict__ A, int lda, long long sA, int n, int k0,
                                                 float* __restrict__ Tb, long long sT, int* info) {
   __shared__ float pr[kGj][kGj + 1], pi[kGj][kGj + 1], ar[kGj][kGj + 1], ai[kGj][kGj + 1];
@@ -1307,24 +1310,38 @@ __global__ __launch_bounds__(256) void k_gj_row(const float* __restrict__ A, int
   float* tb = Tb + 2 * sT * blockIdx.z;
   const int w = n - k0 < kGj ? n - k0 : kGj, c0 = kGj * blockIdx.x;
   const int wc = n - c0 < kGj ? n - c0 : kGj;
+  const bool diag = c0 == k0;
   if (threadIdx.x == 0) bad = 0;
-  for (int e = threadIdx.x; e < kGj * kGj; e += blockDim.x) {
-    const int r = e % kGj, c = e / kGj;
-    const bool in = r < w && c < w;
-    const long long o = (long long)(k0 + r) + (long long)(k0 + c) * lda;
-    pr[r][c] = in ? a[2 * o] : (r == c ? 1.0f : 0.0f);
-    pi[r][c] = in ? a[2 * o + 1] : 0.0f;
-    const bool ina = r < w && c < wc;
-    const long long oa = (long long)(k0 + r) + (long long)(c0 + c) * lda;
-    ar[r][c] = ina ? a[2 * oa] : 0.0f;
-    ai[r][c] = ina ? a[2 * oa + 1] : 0.0f;
+  if (!diag)
+    for (int e = threadIdx.x; e < kGj * kGj; e += blockDim.x) {
+      const int r = e % kGj, c = e / kGj;
+      const bool ina = r < w && c < wc;
+      const long long oa = (long long)(k0 + r) + (long long)(c0 + c) * lda;
+      ar[r][c] = ina ? a[2 * oa] : 0.0f;
+      ai[r][c] = ina ? a[2 * oa + 1] : 0.0f;
+    }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int r = threadIdx.x;
+    float xr[kGj], xi[kGj];
+#pragma unroll
+    for (int c = 0; c < kGj; ++c) {
+      const bool in = r < w && c < w;
+      const long long o = (long long)(k0 + r) + (long long)(k0 + c) * lda;
+      xr[c] = in ? a[2 * o] : (r == c ? 1.0f : 0.0f);
+      xi[c] = in ? a[2 * o + 1] : 0.0f;
+    }
+    gj_wave_invert(xr, xi, r, &bad);
+#pragma unroll
+    for (int c = 0; c < kGj; ++c) {
+      pr[r][c] = xr[c];
+      pi[r][c] = xi[c];
+    }
   }
   __syncthreads();
-  gj_tile_invert(pr, pi, &bad);
   if (blockIdx.x == 0 && threadIdx.x == 0 && bad && info[blockIdx.z] == 0) info[blockIdx.z] = k0 + 1;
   const int i = threadIdx.x >> 2, q0 = 16 * (threadIdx.x & 3);
   if (i >= w) return;
-  const bool diag = c0 == k0;
   for (int q = 0; q < 16; ++q) {
     const int c = q0 + q;
     if (c >= wc) break;

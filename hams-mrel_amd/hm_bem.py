@@ -40,7 +40,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
-           "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing")
+           "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing",
+           "hm_set_workers")
 
 _lib = None
 
@@ -218,6 +219,10 @@ class Context:
         iters = self.solve()
         a, b, f = self.forces()
         return dict(amas=a, bdmp=b, exfc=f, iters=iters)
+
+    def set_workers(self, n=1):
+        """frequencies in flight in run_frequencies (clones of this context)"""
+        self._chk(self.L.hm_set_workers(self.ptr, _i(n)), "hm_set_workers")
 
     def run_frequencies(self, omegas, precision=HM_FP64):
         p = self.problem

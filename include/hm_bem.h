@@ -223,9 +223,17 @@ int hm_ziter_solve(hm_ctx *ctx, int isolv, int n, const double *a, int lda, cons
                    int ldx, int nrhs, int nblocks, const int *block_sizes, int *info);
 /* set_frequency + assemble + factor + solve + forces */
 int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, double *bdmp, double *exfc);
-/* nfreq frequencies back to back; outputs stacked per frequency */
+/* nfreq frequencies (the driver's frequency loop, HAMS_Prog.f90:613-724);
+ * outputs stacked per frequency. With hm_set_workers(n > 1), n frequencies are
+ * in flight at once: this context and n - 1 clones of its configuration (own
+ * streams and device buffers, made at the first call and kept while the
+ * configuration holds) each take the next frequency in a host thread of its
+ * own; every frequency's outputs are those of hm_solve_frequency on it. */
 int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, double *amas, double *bdmp,
                        double *exfc);
+/* frequencies in flight in hm_run_frequencies (1..8, default 1). No
+ * counterpart in the reference. */
+int hm_set_workers(hm_ctx *ctx, int nworkers);
 
 /* Kernel variant of the two epsilon-accelerated finite-depth regimes (EIGENE,
  * PIDCOCK; FinGreen3D.f90:413,:558): 0 one pair per lane (series arrays in
