@@ -119,7 +119,9 @@ def run_steps(ctx, omegas, k0, steps, precision, ranks=None):
         w = float(omegas[(k0 + s) % len(omegas)])
         out.append(ctx.solve_frequency(w, precision))
         if ranks is not None:
-            ranks.append(ctx.hmatrix_stats()["mean_rank"])
+            # mean rank over all cross-approximated blocks, and over the processes
+            # that ran (the others copy an equal block's factors)
+            ranks.append((ctx.hmatrix_stats()["mean_rank"], ctx.block_sharing(with_rank=True)[2]))
     return out
 
 
@@ -496,7 +498,8 @@ def main():
         _, _, st_alone = time_path(ctx, prob, prec, min(2, args.steps), args.warmup, ranks=ranks_alone)
         ctx.set_hmatrix_overlap(True)
         aca_alone = {"systems": min(2, args.steps), "stages_ms": {k: v for k, v in st_alone.items() if v[1]},
-                     "ranks": ranks_alone}
+                     "ranks": [r[0] for r in ranks_alone], "ranks_of_processes_run": [r[1] for r in ranks_alone],
+                     "processes_run": ctx.block_sharing()[1]}
         print(f"# cross approximation alone: {aca_alone}", file=sys.stderr, flush=True)
     no_share = None
     if args.path == "hmatrix" and not args.no_block_sharing and args.steps > 0:
@@ -580,14 +583,15 @@ def main():
             # entries evaluated: one row and one column of m entries per rank step
             # of each of the P processes (A and S of every ordered pair of bodies)
             m = prob.n // prob.nbody
-            procs = 2 * prob.nbody * (prob.nbody - 1)
-            entries = sum(2.0 * m * procs * r for r in aca_alone["ranks"]) / aca_alone["systems"]
+            procs = aca_alone["processes_run"]
+            entries = sum(2.0 * m * procs * r for r in aca_alone["ranks_of_processes_run"]) / aca_alone["systems"]
             units = entries / cand[dom][1]
             per_entry = offdiag_flops_per_entry(prob, fpp)
             flops = units * per_entry
             abytes = 16.0 * units
             what = (f"cross-approximation entry kernels (residual rows and columns of all open A and S "
-                    f"processes), measured alone after the timed region: {cand[dom][1]:.0f} launches and "
+                    f"processes; {procs} processes run, the others copy an equal block's factors), measured alone "
+                    f"after the timed region: {cand[dom][1]:.0f} launches and "
                     f"{entries:.4g} entries per system (2 m P x the mean rank), {per_entry:.0f} counted flop per entry (the "
                     f"reference's FINGREEN3D + SGLINTBD work for the off-diagonal pairs of this geometry); the "
                     f"kernels sum the evanescent modes directly to a bounded tail instead of the epsilon tables")

@@ -109,7 +109,10 @@ struct hm_ctx {
   // cross approximation: process p's block equals process pshare[p]'s (equal
   // bodies at an equal offset, with block sharing on)
   DevBuf d_pshare;
+  DevBuf d_evt, d_pmt;               // the cross approximation's per-frequency depth-factor tables
+  std::vector<int> pshare_host;
   bool pshare_ok = false;            // d_pshare holds the current geometry's map
+  double hm_rank_mean_run = 0.0;     // mean rank of the processes that ran
   int n_pshared = 0;                 // processes that copy another's block
   bool sharing() const {
     if (!block_share || !hm_on) return false;
@@ -404,7 +407,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
   c->workers.clear();
   if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
+  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_evt, &c->d_pmt, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
@@ -615,7 +618,7 @@ extern "C" int hm_get_body_classes(hm_ctx* c, int nbody, int* rep) {
   return HM_OK;
 }
 
-extern "C" int hm_get_block_sharing(hm_ctx* c, int* diag_blocks, int* aca_processes) {
+extern "C" int hm_get_block_sharing(hm_ctx* c, int* diag_blocks, int* aca_processes, double* mean_rank_run) {
   if (!c) return HM_ERR_ARG - 1;
   if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
   const bool sh = c->sharing();
@@ -625,6 +628,7 @@ extern "C" int hm_get_block_sharing(hm_ctx* c, int* diag_blocks, int* aca_proces
   }
   if (diag_blocks) *diag_blocks = sh ? c->n_ublk : c->nbody;
   if (aca_processes) *aca_processes = hm::hmat_processes(c->nbody) - (sh ? c->n_pshared : 0);
+  if (mean_rank_run) *mean_rank_run = c->hm_rank_mean_run;
   return HM_OK;
 }
 
@@ -889,6 +893,7 @@ static int pshare_build(hm_ctx* c) {
       }
     }
   }
+  c->pshare_host = sh;
   int rc = upload(c, c->d_pshare, sh.data(), sh.size(), "cross approximation map");
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream), "cross approximation map");
@@ -990,13 +995,25 @@ static int assemble_hmatrix(hm_ctx* c) {
   // approximation (the longer of the two) starts with the diagonal blocks; the
   // host then polls for processes still open
   int steps = 0;
-  HIPCHK(c, hm::aca_begin(acs, gd, f, c->hmat, c->hm_overlap ? kmax + 4 : 0, &steps, c->timing ? &tm : nullptr),
+  // finite depth: the evanescent modes' depth factors of every point and the
+  // modes' normalisations, once per frequency, for the entries' direct sums
+  hm::FreqDev fa = f;
+  const bool limit = fabs(c->v) < F32(1.e-8) || fabs(c->v + 1.0) < F32(1.e-8);
+  if (c->h > 0.0 && !limit) {
+    HIPCHK(c, c->d_evt.ensure(2 * sizeof(double) * (size_t)kNk * c->nt()), "hmatrix alloc");
+    HIPCHK(c, c->d_pmt.ensure(sizeof(double) * kNk), "hmatrix alloc");
+    HIPCHK(c, hm::evan_table(acs, gd, f, c->d_evt.as<double>(), c->d_pmt.as<double>()), "cross approximation");
+    fa.evt = c->d_evt.as<double>();
+    fa.pmt = c->d_pmt.as<double>();
+    fa.evt_ld = c->nt();
+  }
+  HIPCHK(c, hm::aca_begin(acs, gd, fa, c->hmat, c->hm_overlap ? kmax + 4 : 0, &steps, c->timing ? &tm : nullptr),
          "cross approximation");
   if ((rc = copy_shared_blocks(c, c->stream, D, m * (m + 1), m * m, 2 * sizeof(double))) ||
       (rc = copy_shared_blocks(c, c->stream, DS, m * (m + 1), m * m, 2 * sizeof(double))))
     return rc;
   if ((rc = precond_early(c))) return rc;
-  HIPCHK(c, hm::aca_end(acs, gd, f, c->hmat, &steps, &c->hm_unconverged, c->timing ? &tm : nullptr),
+  HIPCHK(c, hm::aca_end(acs, gd, fa, c->hmat, &steps, &c->hm_unconverged, c->timing ? &tm : nullptr),
          "cross approximation");
   c->hm_steps = steps;
   c->stage_end(kAsmCross, acs);
@@ -1010,6 +1027,13 @@ static int assemble_hmatrix(hm_ctx* c) {
   c->hm_rank_max = 0;
   for (int r : rank) { rs += r; c->hm_rank_max = r > c->hm_rank_max ? r : c->hm_rank_max; }
   c->hm_rank_mean = P ? (double)rs / P : 0.0;
+  {
+    const bool sh = c->hmat.share != nullptr && (int)c->pshare_host.size() == P;
+    long long rr = 0, cnt = 0;
+    for (int p = 0; p < P; ++p)
+      if (!sh || c->pshare_host[p] == p) { rr += rank[p]; ++cnt; }
+    c->hm_rank_mean_run = cnt ? (double)rr / cnt : 0.0;
+  }
   // pack each matrix's factors at its largest rank, so that the products run
   // over the ranks the blocks need instead of kmax
   for (int mat = 0; mat < 2; ++mat) {
@@ -1684,6 +1708,13 @@ extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int 
   c->hm_kuse[0] = ku;
   c->hm_rank_max = ku;
   c->hm_rank_mean = P ? (double)rs / P : 0.0;
+  {
+    const bool sh = c->hmat.share != nullptr && (int)c->pshare_host.size() == P;
+    long long rr = 0, cnt = 0;
+    for (int p = 0; p < P; ++p)
+      if (!sh || c->pshare_host[p] == p) { rr += rank[p]; ++cnt; }
+    c->hm_rank_mean_run = cnt ? (double)rr / cnt : 0.0;
+  }
   const size_t lr = (size_t)nb * (nb - 1) * m * (ku ? ku : 1);
   HIPCHK(c, c->d_hUc[0].ensure(cplx * lr), "hmatrix alloc");
   HIPCHK(c, c->d_hVc[0].ensure(cplx * lr), "hmatrix alloc");

@@ -473,6 +473,31 @@ __global__ void k_build_w(GeomDev g, FreqDev f, const double* beta, int nbeta, d
   build_w_body(g, f, beta, nbeta, xw, yw, W, ldw, j);
 }
 
+// the evanescent modes' depth factors per field point and their normalisations
+// (FreqDev::evt, pmt), formed as fin_evanescent forms them
+__global__ __launch_bounds__(256) void k_evan_table(GeomDev g, FreqDev f, double* __restrict__ evt,
+                                                    double* __restrict__ pmt) {
+#pragma clang fp contract(off)
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y + 1;
+  const double km = f.wvn[m];
+  if (e == 0) {
+    const double twokmh = 2.0 * km * f.h;
+    pmt[m] = 4.0 / (1.0 + sin(twokmh) / twokmh) / f.h;
+  }
+  if (e >= g.nt) return;
+  const double zh = g.fp[3 * e + 2] + f.h;
+  double sk, ck;
+  sincos(km * zh, &sk, &ck);
+  evt[2 * ((long long)m * g.nt + e)] = sk;
+  evt[2 * ((long long)m * g.nt + e) + 1] = ck;
+}
+
+hipError_t evan_table(hipStream_t s, const GeomDev& g, const FreqDev& f, double* evt, double* pmt) {
+  hipLaunchKernelGGL(k_evan_table, dim3((g.nt + 255) / 256, kNk - 1), dim3(256), 0, s, g, f, evt, pmt);
+  return hipGetLastError();
+}
+
 hipError_t build_w(hipStream_t s, const GeomDev& g, const FreqDev& f, const double* beta, int nbeta, double xw,
                    double yw, double* W, int ldw) {
   hipLaunchKernelGGL(k_build_w, dim3((g.n + 255) / 256), dim3(256), 0, s, g, f, beta, nbeta, xw, yw, W, ldw);

@@ -67,7 +67,15 @@ __device__ __forceinline__ void global_centroid(const GeomDev& g, int j, double*
 struct FreqDev {
   double v, w1, wk, h, rho, g, amp;
   const double* wvn;       // kNk roots on the device
+  // optional (nullptr: absent): the evanescent modes' depth factors of every
+  // field point, sin and cos of k_m (z_p + h) at evt[2 (m ld + p)], and each
+  // mode's normalisation pm_m = 4 / (1 + sin(2 k_m h) / (2 k_m h)) / h at
+  // pmt[m], m = 1 .. kNk - 1 (built by evan_table)
+  const double* evt = nullptr;
+  const double* pmt = nullptr;
+  long long evt_ld = 0;
 };
+hipError_t evan_table(hipStream_t s, const GeomDev& g, const FreqDev& f, double* evt, double* pmt);
 
 // blk > 0 restricts to the diagonal blocks of blk x blk pairs (n a multiple of blk, ni = 0);
 // sym keeps PIDCOCK and EIGENE pairs only as (i, j) with i < j (for assemble_sym)

@@ -461,6 +461,39 @@ HM_DEV int fin_direct_terms(double rr, double h) {
   return n < 2 ? 2 : (n > 200 ? 200 : n);       // 200 roots per frequency: modes 1..199
 }
 
+// fin_evanescent with its depth factors and normalisation given: the same
+// operations on the same values (pm as fin_evanescent forms it; skzf, ckzf the
+// sine and cosine of km zfh, ckzp the cosine of km zph)
+template <bool FAST = false>
+HM_DEV void fin_evanescent_given(double km, double pm, double r, double skzf, double ckzf, double ckzp, double& t1,
+                                 double& t2, double& t3) {
+#pragma clang fp contract(off)
+  double k0, k1;
+  sf_k01_t<FAST>(km * r, k0, k1);
+  const double cc = ckzf * ckzp;
+  t1 = (pm * k0) * cc;
+  t2 = (-pm * km * k1) * cc;
+  t3 = (-pm * km * k0) * (skzf * ckzp);
+}
+
+// the evanescent sums of modes 1 .. n-1 (n = fin_direct_terms), FAST K0/K1,
+// with the depth factors of source panel j and field point i and the modes'
+// normalisations from per-frequency tables (evt: sin, cos of k_m (z + h) per
+// mode and point, leading dimension ld; pmt: pm per mode)
+HM_DEV void fin_evanescent_direct_tab(double r, const double* wvn, const double* evt, long long ld,
+                                      const double* pmt, int i, int j, int n, double& s1, double& s2, double& s3) {
+  s1 = s2 = s3 = 0.0;
+  for (int m = 1; m < n; ++m) {
+    const double* ej = evt + 2 * ((long long)m * ld + j);
+    const double ckzp = evt[2 * ((long long)m * ld + i) + 1];
+    double t1, t2, t3;
+    fin_evanescent_given<true>(wvn[m], pmt[m], r, ej[0], ej[1], ckzp, t1, t2, t3);
+    s1 += t1;
+    s2 += t2;
+    s3 += t3;
+  }
+}
+
 // the evanescent sums of modes 1 .. n-1 (n = fin_direct_terms), FAST K0/K1
 HM_DEV void fin_evanescent_direct(double r, double zfh, double zph, const double* wvn, double h, int n,
                                   double& s1, double& s2, double& s3) {

@@ -127,7 +127,10 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
         fin_pidcock_tail(L1, L2, L3, rr, zf, zp, h, fo);
       } else if (reg == kEigene) {
         fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
-        if (DIRECT) fin_evanescent_direct(rr, zf + h, zp + h, wvn, h, fin_direct_terms(rr, h), L1, L2, L3);
+        if (DIRECT && f.evt)
+          fin_evanescent_direct_tab(rr, wvn, f.evt, f.evt_ld, f.pmt, i, j, fin_direct_terms(rr, h), L1, L2, L3);
+        else if (DIRECT)
+          fin_evanescent_direct(rr, zf + h, zp + h, wvn, h, fin_direct_terms(rr, h), L1, L2, L3);
         fo.g_re += L1;
         fo.gr_re += L2;
         fo.gz_re += L3;

@@ -168,12 +168,14 @@ class Context:
         self._chk(self.L.hm_get_body_classes(self.ptr, _i(self.problem.nbody), _p(rep)), "hm_get_body_classes")
         return rep
 
-    def block_sharing(self):
+    def block_sharing(self, with_rank=False):
         """(diagonal blocks assembled, cross-approximation processes run) per
-        frequency of the H-matrix form"""
-        d, a = _i(0), _i(0)
-        self._chk(self.L.hm_get_block_sharing(self.ptr, ctypes.byref(d), ctypes.byref(a)), "hm_get_block_sharing")
-        return d.value, a.value
+        frequency of the H-matrix form; with_rank: and the mean rank of the
+        processes that ran at the last assembly"""
+        d, a, r = _i(0), _i(0), _d(0.0)
+        self._chk(self.L.hm_get_block_sharing(self.ptr, ctypes.byref(d), ctypes.byref(a), ctypes.byref(r)),
+                  "hm_get_block_sharing")
+        return (d.value, a.value, r.value) if with_rank else (d.value, a.value)
 
     def set_block_sharing(self, enable=True):
         self._chk(self.L.hm_set_block_sharing(self.ptr, _i(1 if enable else 0)), "hm_set_block_sharing")

@@ -74,11 +74,12 @@ int hm_get_body_classes(hm_ctx *ctx, int nbody, int *rep);
  * the default). enable = 0: every block is computed. Either way the blocks
  * are the same bit for bit. No counterpart in the reference. */
 int hm_set_block_sharing(hm_ctx *ctx, int enable);
-/* With the H-matrix form on: the diagonal blocks assembled per frequency and
- * the cross-approximation processes run (one per off-diagonal block and
- * matrix; blocks of equal bodies at an equal offset, o_b1 - o_b2 bit for bit,
- * take one process's factors). No counterpart in the reference. */
-int hm_get_block_sharing(hm_ctx *ctx, int *diag_blocks, int *aca_processes);
+/* With the H-matrix form on: the diagonal blocks assembled per frequency, the
+ * cross-approximation processes run (one per off-diagonal block and matrix;
+ * blocks of equal bodies at an equal offset, o_b1 - o_b2 bit for bit, take one
+ * process's factors) and their mean rank at the last assembly. No
+ * counterpart in the reference. */
+int hm_get_block_sharing(hm_ctx *ctx, int *diag_blocks, int *aca_processes, double *mean_rank_run);
 /* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
 int hm_set_depth(hm_ctx *ctx, double h);
 /* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */
