@@ -108,7 +108,7 @@ struct hm_ctx {
   std::string workers_sig;
   // cross approximation: process p's block equals process pshare[p]'s (equal
   // bodies at an equal offset, with block sharing on)
-  DevBuf d_pshare;
+  DevBuf d_pshare, d_prun;           // the map and the list of the processes that run
   DevBuf d_evt, d_pmt;               // the cross approximation's per-frequency depth-factor tables
   std::vector<int> pshare_host;
   bool pshare_ok = false;            // d_pshare holds the current geometry's map
@@ -407,7 +407,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
   c->workers.clear();
   if (c->side) (void)hipStreamSynchronize(c->side);
   (void)hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_evt, &c->d_pmt, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
+  DevBuf* bufs[] = {&c->d_body, &c->d_org, &c->d_ublk, &c->d_pshare, &c->d_prun, &c->d_evt, &c->d_pmt, &c->d_fp, &c->d_verts, &c->d_ncn, &c->d_ds, &c->d_pnsz, &c->d_dxyz,
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
@@ -894,8 +894,12 @@ static int pshare_build(hm_ctx* c) {
     }
   }
   c->pshare_host = sh;
+  std::vector<int> run;
+  for (int p = 0; p < P; ++p)
+    if (sh[p] == p) run.push_back(p);
   int rc = upload(c, c->d_pshare, sh.data(), sh.size(), "cross approximation map");
   if (rc) return rc;
+  if ((rc = upload(c, c->d_prun, run.data(), run.size(), "cross approximation map"))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream), "cross approximation map");
   c->pshare_ok = true;
   return HM_OK;
@@ -958,7 +962,11 @@ static int assemble_hmatrix(hm_ctx* c) {
   if (c->sharing()) {
     int rc0 = pshare_build(c);
     if (rc0) return rc0;
-    if (c->n_pshared > 0) c->hmat.share = c->d_pshare.as<int>();
+    if (c->n_pshared > 0) {
+      c->hmat.share = c->d_pshare.as<int>();
+      c->hmat.run = c->d_prun.as<int>();
+      c->hmat.nrun = hm::hmat_processes(nb) - c->n_pshared;
+    }
   }
   const hm::GeomDev gd = c->geom();
   const hm::FreqDev f = c->freq();

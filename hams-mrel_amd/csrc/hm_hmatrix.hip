@@ -13,11 +13,16 @@ namespace hm {
 
 namespace {
 
+constexpr int kFinishThreads = 1024;
+
 struct Proc {
   int mat, b1, b2;
   double* U;
   double* V;
 };
+// the process of grid row y: the y-th process that runs (h.run), or y itself
+__device__ __forceinline__ int proc_of(const HmatDev& h, int y) { return h.run ? h.run[y] : y; }
+
 __device__ __forceinline__ Proc proc(const HmatDev& h, int p) {
   const int nb = h.nb, per = nb * (nb - 1);
   Proc q;
@@ -69,7 +74,8 @@ __device__ void block_argmax(double v, int idx, double* sv, int* si, double& bes
 }
 
 // block-wide sum of 4 doubles
-__device__ void block_sum4(double* v, double (*sh)[256]) {
+template <int NT>
+__device__ void block_sum4(double* v, double (*sh)[NT]) {
   for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = v[c];
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -166,7 +172,7 @@ __device__ __forceinline__ int sorted_entry(const GeomDev& g, const FreqDev& f, 
 template <int G, bool PID, bool DIRECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_row(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
-  const int p = blockIdx.y;
+  const int p = proc_of(h, blockIdx.y);
   if (h.conv[p]) return;
   __shared__ double wvn[kNk];
   if (f.h > 0.0 && !h.dense)
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // pivot column j* = argmax |r| over unused columns, v_k = r / r(j*); a zero
 // residual row is retired and the pivot row moves to the next unused row
 __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
-  const int p = blockIdx.x;
+  const int p = proc_of(h, blockIdx.x);
   if (h.conv[p]) return;
   __shared__ double sv[256];
   __shared__ int si[256];
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
 template <int G, bool PID, bool DIRECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_col(GeomDev g, FreqDev f,
                                                                                           HmatDev h) {
-  const int p = blockIdx.y;
+  const int p = proc_of(h, blockIdx.y);
   if (h.conv[p] || h.skip[p]) return;
   __shared__ double wvn[kNk];
   if (f.h > 0.0 && !h.dense)
@@ -261,7 +267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // aligned panels of two bodies; rare), and PIDCOCK entries when the
 // EIGENE-only entry kernels ran, one lane each
 __global__ __launch_bounds__(256) void k_aca_fix_row(GeomDev g, FreqDev f, HmatDev h) {
-  const int p = blockIdx.y;
+  const int p = proc_of(h, blockIdx.y);
   if (h.dense || h.conv[p]) return;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= h.m) return;
@@ -275,7 +281,7 @@ __global__ __launch_bounds__(256) void k_aca_fix_row(GeomDev g, FreqDev f, HmatD
   rb[2 * j + 1] = ei;
 }
 __global__ __launch_bounds__(256) void k_aca_fix_col(GeomDev g, FreqDev f, HmatDev h) {
-  const int p = blockIdx.y;
+  const int p = proc_of(h, blockIdx.y);
   if (h.dense || h.conv[p] || h.skip[p]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= h.m) return;
@@ -289,12 +295,13 @@ __global__ __launch_bounds__(256) void k_aca_fix_col(GeomDev g, FreqDev f, HmatD
 }
 
 // norm estimate, stopping test and next pivot row = argmax |u_k| over unused rows
-__global__ __launch_bounds__(256) void k_aca_finish_col(HmatDev h) {
-  const int p = blockIdx.x;
+// (1024 threads: 16 waves share the cross terms' rank loop)
+__global__ __launch_bounds__(kFinishThreads) void k_aca_finish_col(HmatDev h) {
+  const int p = proc_of(h, blockIdx.x);
   if (h.conv[p] || h.skip[p]) return;
-  __shared__ double sh[4][256];
-  __shared__ double sv[256];
-  __shared__ int si[256];
+  __shared__ double sh[4][kFinishThreads];
+  __shared__ double sv[kFinishThreads];
+  __shared__ int si[kFinishThreads];
   const Proc q = proc(h, p);
   const int m = h.m, k = h.rank[p], ir0 = h.next_row[p];
   const double* uk = q.U + 2LL * k * m;
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(256) void k_aca_finish_col(HmatDev h) {
     acc[0] += uk[2 * t] * uk[2 * t] + uk[2 * t + 1] * uk[2 * t + 1];
     acc[1] += vk[2 * t] * vk[2 * t] + vk[2 * t + 1] * vk[2 * t + 1];
   }
-  block_sum4(acc, sh);
+  block_sum4<kFinishThreads>(acc, sh);
   const double nu2 = acc[0], nv2 = acc[1];
   // |sum_l u_l v_l^T|_F^2 = sum_{l,l'} (u_l^H u_l')(v_l^H v_l'): add the new
   // diagonal term and twice the real part of the cross terms with l < k, one
@@ -331,7 +338,7 @@ __global__ __launch_bounds__(256) void k_aca_finish_col(HmatDev h) {
   }
   {
     double c[4] = {lane == 0 ? cross : 0.0, 0.0, 0.0, 0.0};
-    block_sum4(c, sh);
+    block_sum4<kFinishThreads>(c, sh);
     cross = c[0];
   }
   const double n2 = h.norm2[p] + nu2 * nv2 + 2.0 * cross;
@@ -468,6 +475,8 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.counter = (int*)take(4);
   h.narrow = 0;
   h.share = nullptr;
+  h.run = nullptr;
+  h.nrun = 0;
   return h;
 }
 
@@ -476,6 +485,8 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
 static hipError_t aca_step(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int P,
                            const AcaTimer* tm) {
   auto mark = [tm](int b) { if (tm) tm->mark(tm->user, b); };
+  // the processes that run: all, or the list h.run of h.nrun
+  if (h.run) P = h.nrun;
   const dim3 eval((h.m + 255) / 256, P);
   mark(1);
   if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);
@@ -492,7 +503,7 @@ static hipError_t aca_step(hipStream_t s, const GeomDev& g, const FreqDev& f, co
   else hipLaunchKernelGGL((k_aca_eval_col<16, true, false>), eval, dim3(256), 0, s, g, f, h);
   mark(0);
   if (!h.narrow) hipLaunchKernelGGL(k_aca_fix_col, eval, dim3(256), 0, s, g, f, h);
-  hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(256), 0, s, h);
+  hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(kFinishThreads), 0, s, h);
   return hipGetLastError();
 }
 

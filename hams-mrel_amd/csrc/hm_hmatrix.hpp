@@ -49,6 +49,10 @@ struct HmatDev {
   // for bit (equal bodies at an equal offset); p itself for the processes
   // that run. The others are skipped (conv = 3) and copied at the end.
   const int* share;
+  // run (nullptr: all processes): the nrun processes that run (share[p] == p),
+  // so that the step launches' grids cover those only
+  const int* run;
+  int nrun;
   // 1: the bodies' horizontal bounding circles are >= 0.05 h apart, so no
   // off-diagonal entry is PIDCOCK or LINTON; the entry kernels then take the
   // EIGENE-only lane groups of 8 (any such entry would still be redone per lane)
