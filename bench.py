@@ -49,17 +49,20 @@ KERNEL_OF_STAGE = {"asm_linton": "k_assemble_list<0>", "asm_pidcock": "k_assembl
                    "aca_entries": "k_aca_eval_row/col<8,false,true>"}
 ACCURACY_BAR = 1e-4
 
-# HBM bytes per launch of the dominant kernel, measured on one MI355X by two
+# HBM bytes per launch of the candidate kernels, measured on one MI355X by two
 # counter passes of `bench.py --steps 1 --assemble-only` (rocprofv3 --pmc
-# FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter k_assemble|k_aca_eval), scaled
-# by the readings of a 1 GiB device-to-device copy (fetch x2, write x1). The
-# H-matrix path's PIDCOCK kernel assembles pairs with their mirrors from the
-# cached frequency-independent term halves (31 GB read per launch); it runs at 5
-# waves per SIMD, and register-spill traffic (scratch that misses the L2) is
-# most of the rest.
+# FETCH_SIZE, then --pmc WRITE_SIZE, kernel filter
+# k_assemble_sym|k_aca_eval|k_assemble_list), scaled by the readings of a 1 GiB
+# device-to-device copy (fetch x2, write x1). The H-matrix path's PIDCOCK
+# kernel assembles the one distinct diagonal block's pairs with their mirrors
+# from the cached frequency-independent term halves (1.9 GB read per launch);
+# it runs at 5 waves per SIMD, and register-spill traffic (scratch that misses
+# the L2) is most of the rest.
 MEASURED_TRAFFIC = {
-    ("c5", "hmatrix", "asm_pidcock"): 43902678.5 * 1024 * 2 + 44113979.84375 * 1024,
-    ("c5", "hmatrix", "asm_eigene"): 18188861.6875 * 1024 * 2 + 42729947.96875 * 1024,
+    ("c5", "hmatrix", "asm_pidcock"): 2643203.875 * 1024 * 2 + 2692394.9375 * 1024,
+    ("c5", "hmatrix", "asm_eigene"): 1120176.6875 * 1024 * 2 + 2771524.1875 * 1024,
+    # per launch, mean of the row and column launches
+    ("c5", "hmatrix", "aca_entries"): (712232.4375 + 677136.9375) / 108 * 1024 * 2 + (85371.4375 + 85220.0625) / 108 * 1024,
 }
 
 def flop_model(name):
@@ -384,6 +387,9 @@ def main():
     ap.add_argument("--no-block-sharing", action="store_true",
                     help="H-matrix path: assemble, factor and invert every body's diagonal block, also where bodies "
                          "are equal (the blocks are the same bit for bit either way)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="H-matrix path: skip the passes after the timed region (the cross approximation alone, "
+                         "the path without block sharing), e.g. for profiler runs")
     ap.add_argument("--with-fp32ir", action="store_true", help="also time the FP32 LU + refinement path")
     ap.add_argument("--field-points", type=int, default=0,
                     help="also time the potential, pressure and elevation at this many field points after the "
@@ -488,7 +494,7 @@ def main():
         print(f"# field points: {field}", file=sys.stderr, flush=True)
     hist_mx = ctx.pair_histogram().ravel()     # pairs assembled densely on the measured path
     aca_alone = None
-    if args.path == "hmatrix" and args.steps > 0:
+    if args.path == "hmatrix" and args.steps > 0 and not args.no_extras:
         # the cross approximation's entry kernels without the diagonal blocks
         # beside them (on the context's stream after them), for the timed
         # region's first two frequencies once more, outside it: their launch
@@ -502,7 +508,7 @@ def main():
                      "processes_run": ctx.block_sharing()[1]}
         print(f"# cross approximation alone: {aca_alone}", file=sys.stderr, flush=True)
     no_share = None
-    if args.path == "hmatrix" and not args.no_block_sharing and args.steps > 0:
+    if args.path == "hmatrix" and not args.no_block_sharing and args.steps > 0 and not args.no_extras:
         # the same path with every body's diagonal block assembled, factored and
         # inverted (outside the timed region; the outputs are the same bit for bit)
         ctx.set_block_sharing(False)

@@ -101,9 +101,16 @@ template <> struct LaneGroup<8> {
 // a full DPP row: lanes 16q .. 16q+15
 template <> struct LaneGroup<16> {
   static __device__ __forceinline__ double next(double x, double xn, int l) {
-    // row_shl:1 reads lane l+1; the last lane reads lane 0 of the next slot (row_shr:15)
-    const double a = dpp_d<0x101>(x), b = dpp_d<0x11F>(xn);
-    return l == 15 ? b : a;
+    // row_shr:15 brings lane 0 of the next slot to the last lane; row_shl:1 then
+    // brings lane l+1 to lanes 0 .. 14, and the last lane, whose source lies
+    // outside the row, keeps that value (bound_ctrl off): two moves a half, no
+    // select
+    (void)l;
+    const int blo = __builtin_amdgcn_update_dpp(0, __double2loint(xn), 0x11F, 0xF, 0xF, true);
+    const int bhi = __builtin_amdgcn_update_dpp(0, __double2hiint(xn), 0x11F, 0xF, 0xF, true);
+    const int lo = __builtin_amdgcn_update_dpp(blo, __double2loint(x), 0x101, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(bhi, __double2hiint(x), 0x101, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
   }
   template <int D> static __device__ __forceinline__ double prev(double x) {
     return dpp_d<0x110 + D>(x);   // row_shr:D
