@@ -13,7 +13,7 @@ namespace hm {
 
 namespace {
 
-constexpr int kFinishThreads = 1024;
+constexpr int kFinishThreads = 256;
 
 struct Proc {
   int mat, b1, b2;
@@ -126,6 +126,39 @@ __device__ __forceinline__ void residual_entry_direct(const GeomDev& g, const Fr
   }
 }
 
+// the norm estimate's inner products from the entry kernels: with every entry of
+// a row or column final after its eval launch (no fix-up launches: bodies at
+// least 0.05 h apart, or an assembled matrix) and ranks below kFuseMaxRank
+constexpr int kFuseMaxRank = 64;
+__device__ __forceinline__ bool aca_fused(const HmatDev& h) {
+  return (h.dense != nullptr || h.narrow) && h.kmax <= kFuseMaxRank && h.prow != nullptr;
+}
+
+// out[2 l], out[2 l + 1] = sum over this workgroup's lanes of conj(W(x, l)) * (er + i ei),
+// l < k (W = V, x = column index of a row launch; W = U, x = row index of a
+// column launch); every lane of the workgroup calls it (inactive lanes with 0)
+__device__ void aca_partials(const double* W, int x, int m, int k, double er, double ei, double* out) {
+  __shared__ double red[4][kFuseMaxRank][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int l = 0; l < k; ++l) {
+    const double wr = W[2 * (x + (long long)l * m)], wi = W[2 * (x + (long long)l * m) + 1];
+    double pr = wr * er + wi * ei, pi = wr * ei - wi * er;
+    for (int o = 32; o > 0; o >>= 1) {
+      pr += __shfl_xor(pr, o);
+      pi += __shfl_xor(pi, o);
+    }
+    if (lane == 0) {
+      red[wv][l][0] = pr;
+      red[wv][l][1] = pi;
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < k; l += blockDim.x) {
+    out[2 * l] = ((red[0][l][0] + red[1][l][0]) + red[2][l][0]) + red[3][l][0];
+    out[2 * l + 1] = ((red[0][l][1] + red[1][l][1]) + red[2][l][1]) + red[3][l][1];
+  }
+}
+
 // The entries of one workgroup in the order this lane evaluates: the 256
 // entries (row i, columns c0 + t; or rows c0 + t, column j) sorted by near flag
 // (the exact Rankine panel integral), regime and series length (bitonic sort in
@@ -186,6 +219,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const int j = active ? j0 : h.m - 1;
   double er = 0.0, ei = 0.0;
   residual_entry<G, PID, DIRECT>(g, f, wvn, h, q, h.m, h.rank[p], h.next_row[p], j, active, er, ei);
+  if (aca_fused(h))
+    aca_partials(q.V, j, h.m, h.rank[p], active ? er : 0.0, active ? ei : 0.0,
+                 h.prow + 2LL * ((long long)p * gridDim.x + blockIdx.x) * h.kmax);
   if (!active) return;
   double* rb = h.rowbuf + 2LL * p * h.m;
   rb[2 * j] = er;
@@ -258,6 +294,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const int k = h.rank[p];
   double er = 0.0, ei = 0.0;
   residual_entry<G, PID, DIRECT>(g, f, wvn, h, q, h.m, k, i, h.pivot[p], active, er, ei);
+  if (aca_fused(h))
+    aca_partials(q.U, i, h.m, k, active ? er : 0.0, active ? ei : 0.0,
+                 h.pcol + 2LL * ((long long)p * gridDim.x + blockIdx.x) * h.kmax);
   if (!active) return;
   q.U[2 * (i + (long long)k * h.m)] = er;
   q.U[2 * (i + (long long)k * h.m) + 1] = ei;
@@ -295,7 +334,6 @@ __global__ __launch_bounds__(256) void k_aca_fix_col(GeomDev g, FreqDev f, HmatD
 }
 
 // norm estimate, stopping test and next pivot row = argmax |u_k| over unused rows
-// (1024 threads: 16 waves share the cross terms' rank loop)
 __global__ __launch_bounds__(kFinishThreads) void k_aca_finish_col(HmatDev h) {
   const int p = proc_of(h, blockIdx.x);
   if (h.conv[p] || h.skip[p]) return;
@@ -314,10 +352,35 @@ __global__ __launch_bounds__(kFinishThreads) void k_aca_finish_col(HmatDev h) {
   block_sum4<kFinishThreads>(acc, sh);
   const double nu2 = acc[0], nv2 = acc[1];
   // |sum_l u_l v_l^T|_F^2 = sum_{l,l'} (u_l^H u_l')(v_l^H v_l'): add the new
-  // diagonal term and twice the real part of the cross terms with l < k, one
-  // wave per l
+  // diagonal term and twice the real part of the cross terms with l < k
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double cross = 0.0;
+  if (aca_fused(h)) {
+    // from the partial sums of the row and column launches, in chunk order:
+    // u_l^H u_k, and v_l^H v_k = (v_l^H r) / r(j*)
+    const int nch = (m + 255) / 256;
+    const double* rb = h.rowbuf + 2LL * p * m;
+    const int js = h.pivot[p];
+    const double pr = rb[2 * js], pi = rb[2 * js + 1], den = pr * pr + pi * pi;
+    const double ir = pr / den, ii = -pi / den;
+    const double* prow = h.prow + 2LL * (long long)p * nch * h.kmax;
+    const double* pcol = h.pcol + 2LL * (long long)p * nch * h.kmax;
+    double c = 0.0;
+    for (int l = threadIdx.x; l < k; l += blockDim.x) {
+      double ur = 0.0, ui = 0.0, rr = 0.0, ri = 0.0;
+      for (int ch = 0; ch < nch; ++ch) {
+        ur += pcol[2 * ((long long)ch * h.kmax + l)];
+        ui += pcol[2 * ((long long)ch * h.kmax + l) + 1];
+        rr += prow[2 * ((long long)ch * h.kmax + l)];
+        ri += prow[2 * ((long long)ch * h.kmax + l) + 1];
+      }
+      const double vr = rr * ir - ri * ii, vi = rr * ii + ri * ir;
+      c += ur * vr - ui * vi;
+    }
+    double cc[4] = {c, 0.0, 0.0, 0.0};
+    block_sum4<kFinishThreads>(cc, sh);
+    cross = cc[0];
+  } else {
   for (int l = wv; l < k; l += nw) {
     const double* ul = q.U + 2LL * l * m;
     const double* vl = q.V + 2LL * l * m;
@@ -340,6 +403,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_aca_finish_col(HmatDev h) {
     double c[4] = {lane == 0 ? cross : 0.0, 0.0, 0.0, 0.0};
     block_sum4<kFinishThreads>(c, sh);
     cross = c[0];
+  }
   }
   const double n2 = h.norm2[p] + nu2 * nv2 + 2.0 * cross;
   // next pivot row
@@ -444,7 +508,8 @@ size_t hmat_bytes(int nb, int m, int kmax, int nmat) {
   b += 4 * 5 * P;                  // rank, conv, next_row, pivot, skip
   b += 4 * 2 * P * m;              // row_used, col_used
   b += 4;                          // counter
-  return b + 13 * 256;             // alignment of the 13 arrays
+  b += 2 * 16 * P * (size_t)((m + 255) / 256) * kmax;   // prow, pcol
+  return b + 15 * 256;             // alignment of the 15 arrays
 }
 
 HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s, int nmat, const double* dense,
@@ -473,6 +538,9 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.row_used = (int*)take(4 * P * m);
   h.col_used = (int*)take(4 * P * m);
   h.counter = (int*)take(4);
+  const size_t part = 16 * P * (size_t)((m + 255) / 256) * kmax;
+  h.prow = (double*)take(part);
+  h.pcol = (double*)take(part);
   h.narrow = 0;
   h.share = nullptr;
   h.run = nullptr;
