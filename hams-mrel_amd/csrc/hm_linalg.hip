@@ -23,6 +23,8 @@
 //   diagonal-block solve (64 lanes per right-hand side) and a GEMM update.
 #include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
+
+#include <utility>
 #include <stdint.h>
 
 #include "hm_linalg.hpp"
@@ -1272,32 +1274,37 @@ __device__ __forceinline__ float lane_bcast(float v, int j) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
-__device__ __forceinline__ void gj_wave_invert(float (&xr)[kGj], float (&xi)[kGj], int lane, int* bad) {
-  // the rows stay in registers: column j is picked by selects over the unrolled
-  // columns, never by a dynamic index
-  for (int j = 0; j < kGj; ++j) {
-    float fr = 0.0f, fi = 0.0f;
+// one pivot step; J is a compile-time constant, so column J of the row and the
+// broadcast lane are static (no dynamic register index, no selects)
+template <int J>
+__device__ __forceinline__ void gj_wave_step(float (&xr)[kGj], float (&xi)[kGj], int lane, int* bad) {
+  const float fr = xr[J], fi = xi[J];
+  const float dr = lane_bcast(fr, J), di = lane_bcast(fi, J);
+  const float den = dr * dr + di * di;
+  float rr = 0.0f, ri = 0.0f;
+  if (den > 0.0f) { rr = dr / den; ri = -di / den; }
+  else if (lane == 0) *bad = 1;
+  float gr = fr * rr - fi * ri, gi = fr * ri + fi * rr;
+  if (lane == J) { gr = 1.0f - rr; gi = -ri; }
 #pragma unroll
-    for (int c = 0; c < kGj; ++c) {
-      fr = c == j ? xr[c] : fr;
-      fi = c == j ? xi[c] : fi;
-    }
-    const float dr = lane_bcast(fr, j), di = lane_bcast(fi, j);
-    const float den = dr * dr + di * di;
-    float rr = 0.0f, ri = 0.0f;
-    if (den > 0.0f) { rr = dr / den; ri = -di / den; }
-    else if (lane == 0) *bad = 1;
-    float gr = fr * rr - fi * ri, gi = fr * ri + fi * rr;
-    if (lane == j) { gr = 1.0f - rr; gi = -ri; }
-    const float jr = lane == j ? rr : -gr, ji = lane == j ? ri : -gi;
-#pragma unroll
-    for (int c = 0; c < kGj; ++c) {
-      const float ur = lane_bcast(xr[c], j), ui = lane_bcast(xi[c], j);
-      const float nr = xr[c] - (gr * ur - gi * ui), ni = xi[c] - (gr * ui + gi * ur);
-      xr[c] = c == j ? jr : nr;
-      xi[c] = c == j ? ji : ni;
-    }
+  for (int c = 0; c < kGj; ++c) {
+    if (c == J) continue;
+    const float ur = lane_bcast(xr[c], J), ui = lane_bcast(xi[c], J);
+    xr[c] -= gr * ur - gi * ui;
+    xi[c] -= gr * ui + gi * ur;
   }
+  xr[J] = lane == J ? rr : -gr;
+  xi[J] = lane == J ? ri : -gi;
+}
+
+template <int... J>
+__device__ __forceinline__ void gj_wave_steps(float (&xr)[kGj], float (&xi)[kGj], int lane, int* bad,
+                                              std::integer_sequence<int, J...>) {
+  (gj_wave_step<J>(xr, xi, lane, bad), ...);
+}
+
+__device__ __forceinline__ void gj_wave_invert(float (&xr)[kGj], float (&xi)[kGj], int lane, int* bad) {
+  gj_wave_steps(xr, xi, lane, bad, std::make_integer_sequence<int, kGj>{});
 }
 
 // T[:, c0 .. c0 + 63] of step k0 (blockIdx.x = column tile, blockIdx.z = matrix):
