@@ -149,6 +149,10 @@ struct hm_ctx {
   void* luws() const { return lu_coop ? d_luws.p : nullptr; }
   DevBuf d_Pinv;              // explicit FP32 inverses of the preconditioner blocks
   DevBuf d_gjws;              // workspace of their Gauss-Jordan inversion (H-matrix form)
+  DevBuf d_Pinv16;            // FP16 copy of d_Pinv (H-matrix form, pinv16_on)
+  bool pinv16_on = true;      // the H-matrix form's preconditioner applied by FP16 MFMA GEMMs
+  bool pinv16_ready = false;
+  bool pinv16_shared = false; // d_Pinv16 holds one block, shared by every body
   DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
   bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
   std::vector<int> gm_blocks;
@@ -411,7 +415,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_pairs, &c->d_hist, &c->d_cursor, &c->d_beta, &c->d_wvn, &c->d_A, &c->d_S,
                     &c->d_W, &c->d_R, &c->d_C, &c->d_D, &c->d_X, &c->d_A32, &c->d_X32, &c->d_Rw,
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
-                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_gjws, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
+                    &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_gjws, &c->d_Pinv16, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
                     &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
                     &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw,
                     &c->d_surd, &c->d_surdcnt};
@@ -615,6 +619,36 @@ extern "C" int hm_get_body_classes(hm_ctx* c, int nbody, int* rep) {
   if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
   if (nbody != c->nbody || !rep) return c->fail(HM_ERR_ARG - 2, "bad output array");
   for (int b = 0; b < nbody; ++b) rep[b] = c->body_rep[b];
+  return HM_OK;
+}
+
+extern "C" int hm_set_precond_fp16(hm_ctx* c, int enable) {
+  if (!c) return HM_ERR_ARG - 1;
+  c->pinv16_on = enable != 0;
+  c->pinv16_ready = false;
+  c->prec_early = false;
+  c->factored = c->solved = false;
+  return HM_OK;
+}
+
+extern "C" int hm_cgemm_f16_check(hm_ctx* c, int m, int n, int k, const float* a, const float* b, float* out) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (m < 1 || n < 1 || k < 1) return c->fail(HM_ERR_ARG - 2, "sizes must be >= 1");
+  if (!a || !b || !out) return c->fail(HM_ERR_ARG - 5, "arrays missing");
+  const size_t na = 2 * (size_t)m * k, nb = 2 * (size_t)k * n, nc = 2 * (size_t)m * n;
+  const int kp = hm::h16_kp(k);
+  DevBuf da, db, dc, ha, hb;
+  int rc = 0;
+  if ((rc = upload(c, da, a, na, "cgemm f16")) || (rc = upload(c, db, b, nb, "cgemm f16"))) return rc;
+  HIPCHK(c, ha.ensure(2 * sizeof(_Float16) * (size_t)m * kp), "cgemm f16");
+  HIPCHK(c, hb.ensure(2 * sizeof(_Float16) * (size_t)kp * n), "cgemm f16");
+  HIPCHK(c, dc.ensure(sizeof(float) * nc), "cgemm f16");
+  HIPCHK(c, hm::pack_a16(c->stream, da.as<float>(), m, 0, m, k, ha.as<_Float16>(), 1), "cgemm f16");
+  HIPCHK(c, hm::pack_b16(c->stream, db.as<float>(), k, nullptr, 0, k, 1, n, hb.as<_Float16>(), 2LL * kp), "cgemm f16");
+  HIPCHK(c, hm::cgemm_h16(c->stream, m, n, kp, ha.as<_Float16>(), 2LL * m * kp, hb.as<_Float16>(), 2LL * kp, 2LL * kp,
+                          dc.as<float>(), m, (long long)m * n, 1), "cgemm f16");
+  HIPCHK(c, hipMemcpyAsync(out, dc.p, sizeof(float) * nc, hipMemcpyDeviceToHost, c->stream), "cgemm f16");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "cgemm f16");
   return HM_OK;
 }
 
@@ -1077,6 +1111,7 @@ static bool block_inverse(const hm_ctx* c) { return c->gm_precond == 2 || c->gm_
 // (U^-1 L^-1) and their pivot permutations
 static int precond_prepare(hm_ctx* c, hipStream_t s) {
   c->pinv_ready = false;
+  c->pinv16_ready = false;
   if (c->gm_blocks.size() < 2 || !block_inverse(c)) return HM_OK;
   const int nblk = (int)c->gm_blocks.size() - 1;
   size_t tot = 0;
@@ -1134,6 +1169,15 @@ static int precond_gj(hm_ctx* c, hipStream_t s, int* info) {
     HIPCHK(c, hm::gj_inverse_nopiv(s, Pinv, m, sc, m, c->d_gjws.p, info, nb), "preconditioner inverse");
   }
   HIPCHK(c, hm::fill_iota(s, c->d_pperm.as<int>(), c->n), "preconditioner permutation");
+  c->pinv16_ready = false;
+  if (c->pinv16_on) {
+    // packed FP16 copies for the Arnoldi steps: one if every body shares one block
+    const int np = share && c->n_ublk == 1 ? 1 : nb;
+    HIPCHK(c, c->d_Pinv16.ensure(2 * sizeof(_Float16) * (size_t)np * m * hm::h16_kp(m)), "preconditioner alloc");
+    HIPCHK(c, hm::pack_a16(s, Pinv, m, sc, m, m, c->d_Pinv16.as<_Float16>(), np), "preconditioner FP16 copy");
+    c->pinv16_shared = np == 1 && nb > 1;
+    c->pinv16_ready = true;
+  }
   c->gm_blocks = c->body_start;
   c->pinv_ready = true;
   return HM_OK;
@@ -1469,9 +1513,14 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
       if (c->pinv_ready && block_inverse(c)) {
         ga.pperm = c->d_pperm.as<int>();
         ga.Pinv = c->d_Pinv.as<float>();
+        if (c->hm_on && c->pinv16_on && c->pinv16_ready) {
+          ga.Pinv16 = c->d_Pinv16.as<_Float16>();
+          ga.pinv16_shared = c->pinv16_shared;
+        }
       }
     }
-    HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, ncol, c->gm_restart)), "gmres alloc");
+    HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, ncol, c->gm_restart, ga.Pinv16 ? ga.nblk : 0)),
+           "gmres alloc");
     HIPCHK(c, hm::gmres_solve(c->stream, ga, c->d_gwork.p, &c->gm_stats), "gmres");
     it = c->gm_stats.iterations;
   } else {
@@ -1524,6 +1573,7 @@ extern "C" int hm_set_gmres(hm_ctx* c, int restart, double tol, int maxiter, int
   c->gm_precond = precond;
   c->factored = c->solved = false;
   c->pinv_ready = false;
+  c->pinv16_ready = false;
   c->prec_early = false;
   return HM_OK;
 }
@@ -2424,11 +2474,11 @@ extern "C" int hm_solve_frequency(hm_ctx* c, double w1, int precision, double* a
 // the settings a clone must share with its context (with cfg_gen)
 static std::string worker_signature(const hm_ctx* c) {
   char b[512];
-  snprintf(b, sizeof b, "%d %d %d %d %d %d %d %.17g %.17g %d %d %d %.17g %d %d %d %d %.17g %d %d %d %d %.17g %d",
+  snprintf(b, sizeof b, "%d %d %d %d %d %d %d %.17g %.17g %d %d %d %.17g %d %d %d %d %.17g %d %d %d %d %.17g %d %d",
            c->series_variant, c->lu_coop, c->pair_sym, c->block_share, c->hm_on, c->hm_tree, c->ht_mode, c->hm_eps,
            c->hm_eps_s, c->hm_kmax, c->hm_overlap, c->gm_restart, c->gm_tol, c->gm_maxiter, c->gm_precond,
            c->sur_on, c->sur_quant, c->sur_tol, c->surd_on, c->frames, c->ht_leaf, c->nbody, c->ht_eta,
-           c->sur_ready);
+           c->sur_ready, c->pinv16_on);
   return b;
 }
 
@@ -2453,6 +2503,7 @@ static int clone_config(hm_ctx* w, hm_ctx* c) {
   w->ht_mode = c->ht_mode;
   w->ht_leaf = c->ht_leaf;
   w->ht_eta = c->ht_eta;
+  w->pinv16_on = c->pinv16_on;
   if ((rc = hm_set_depth(w, c->h))) return rc;
   if ((rc = hm_set_headings(w, (int)c->beta.size(), c->beta.data()))) return rc;
   if ((rc = hm_set_gmres(w, c->gm_restart, c->gm_tol, c->gm_maxiter, c->gm_precond))) return rc;

@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ V, l
 
 }  // namespace
 
-size_t gmres_workspace_bytes(int n, int ncol, int restart) {
+size_t gmres_workspace_bytes(int n, int ncol, int restart, int nblk16) {
   const size_t m = (size_t)restart, m1 = m + 1, cx = 2 * sizeof(double);
   size_t b = 0;
   b += cx * m1 * ncol * n;          // V
@@ -283,6 +283,7 @@ size_t gmres_workspace_bytes(int n, int ncol, int restart) {
   b += 3 * sizeof(int) * ncol + 64; // active, steps, nactive
   b += cx * (size_t)n * ncol * 2;   // R, W64
   b += 2 * 2 * sizeof(float) * (size_t)n * ncol;  // Z32, W32
+  b += 2 * sizeof(_Float16) * (size_t)(n + 31 * nblk16) * ncol + 256;   // W16 (rows padded to 32 per block)
   return b + 64 * 16;               // alignment slack
 }
 
@@ -389,8 +390,20 @@ static hipError_t apply_precond(hipStream_t s, const GmresArgs& a, float* Z, int
 }
 
 // Z = M^{-1} v = (U^{-1} L^{-1}) (P v): the pivot permutation applied while v is
-// demoted to FP32 (into W), then one batched GEMM per run of equal blocks
-static hipError_t apply_inverse(hipStream_t s, const GmresArgs& a, const double* v, float* W, float* Z, int ncol) {
+// demoted to FP32 (into W), or to FP16 (into W16) with the FP16 inverses (half:
+// Arnoldi vectors, of unit norm, so that no entry that matters falls below
+// FP16's normal range), then one batched GEMM per run of equal blocks
+static hipError_t apply_inverse(hipStream_t s, const GmresArgs& a, const double* v, float* W, _Float16* W16, float* Z,
+                                int ncol, bool half) {
+  if (half && a.Pinv16) {
+    // blocks of one size (gmres_solve checks): packed FP16 rows, kp per block
+    const int r0 = a.blk_start[0], nb = a.blk_start[1] - r0, kp = h16_kp(nb);
+    const long long ldw = 2LL * kp * a.nblk;
+    hipError_t e = pack_b16(s, v, (long long)a.n, a.pperm, r0, nb, a.nblk, ncol, W16, ldw);
+    if (e != hipSuccess) return e;
+    return cgemm_h16(s, nb, ncol, kp, a.Pinv16, a.pinv16_shared ? 0 : 2LL * nb * kp, W16, ldw, 2LL * kp,
+                     Z + 2 * (long long)r0, a.n, nb, a.nblk);
+  }
   hipLaunchKernelGGL(k_demote_perm, dim3((a.n + 255) / 256, ncol), dim3(256), 0, s, v, (long long)a.n, a.pperm, a.n, W);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -424,6 +437,11 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
   double* W = (double*)take(cx * (size_t)n * ncol);
   float* Z32 = (float*)take(2 * sizeof(float) * (size_t)n * ncol);
   float* W32 = (float*)take(2 * sizeof(float) * (size_t)n * ncol);
+  _Float16* W16 = (_Float16*)take(2 * sizeof(_Float16) * (size_t)(n + 31 * (a.Pinv16 ? a.nblk : 0)) * ncol);
+  if (a.Pinv16) {
+    for (int b = 1; b < a.nblk; ++b)
+      if (a.blk_start[b + 1] - a.blk_start[b] != a.blk_start[1] - a.blk_start[0]) return hipErrorInvalidValue;
+  }
   const long long vstride = (long long)ncol * n;
   const dim3 gcol((n + 255) / 256, ncol);
   hipError_t e;
@@ -458,7 +476,7 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       const double* vj = V + 2 * vstride * j;
       // W = A * M^{-1} v_j on MFMA in FP32
       if (a.nblk > 0 && a.Pinv) {
-        GM_CHECK(apply_inverse(s, a, vj, W32, Z32, ncol));
+        GM_CHECK(apply_inverse(s, a, vj, W32, W16, Z32, ncol, true));
       } else {
         GM_CHECK(demote(s, vj, n, 0, Z32, n, 0, n, ncol, 1));
         if (a.nblk > 0) GM_CHECK(apply_precond(s, a, Z32, ncol));
@@ -494,7 +512,8 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       hipLaunchKernelGGL(k_combine, gcol, dim3(256), 0, s, V, vstride, n, m, y, steps, W, (long long)n, 0);
       GM_CHECK(hipGetLastError());
       if (a.Pinv) {
-        GM_CHECK(apply_inverse(s, a, W, W32, Z32, ncol));
+        // the update itself in FP32: its scale follows the residual's
+        GM_CHECK(apply_inverse(s, a, W, W32, W16, Z32, ncol, false));
       } else {
         GM_CHECK(demote(s, W, n, 0, Z32, n, 0, n, ncol, 1));
         GM_CHECK(apply_precond(s, a, Z32, ncol));

@@ -27,6 +27,16 @@ struct GmresArgs {
   const int* pipiv = nullptr;   // block-local pivots at the block's row offset
   const float* Pinv = nullptr;  // (U^-1 L^-1) of the blocks, packed like P (gmres_block_inverse), or null
   const int* pperm = nullptr;   // the blocks' pivot permutations (gmres_block_perm), needed with Pinv
+  // FP16 copy of Pinv, packed for cgemm_h16 (pack_a16: one block if
+  // pinv16_shared, else nblk of them; the blocks must all have one size): the
+  // Arnoldi steps then apply the preconditioner by FP16 MFMA GEMMs with FP32
+  // accumulation and result instead of FP32 ones (the update of X keeps the
+  // FP32 inverses). With op32_off, D M^-1 then differs from the identity by the
+  // FP16 rounding of M^-1 (~1e-3), which the FP64 residual of each restart
+  // corrects like any other error of the FP32 inner solve. The workspace must
+  // be sized with nblk16 = nblk.
+  const _Float16* Pinv16 = nullptr;
+  bool pinv16_shared = false;
   // operator form (used instead of A32 / A64 when set): op32 writes W = A Z in
   // FP32, op64 writes W = A X in FP64, both n x ncol with leading dimension n
   std::function<hipError_t(hipStream_t, const float*, float*)> op32;
@@ -42,7 +52,7 @@ struct GmresStats {
   double rel_residual = 0.0;
 };
 
-size_t gmres_workspace_bytes(int n, int ncol, int restart);
+size_t gmres_workspace_bytes(int n, int ncol, int restart, int nblk16 = 0);
 // FP32 copies of the diagonal blocks of A, LU-factored in place
 hipError_t gmres_block_precond_setup(hipStream_t s, const double* A, int lda, int nblk, const int* blk_start,
                                      float* P, int* pipiv, int* info, unsigned* counter);

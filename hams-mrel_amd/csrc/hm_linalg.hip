@@ -1434,4 +1434,198 @@ hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int 
   return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Complex GEMM with FP16 operands and FP32 accumulation (MFMA f32 16x16x32 f16)
+// ---------------------------------------------------------------------------
+// One wave per 64 x 32 output tile (4 x 2 MFMA tiles of 16 x 16), no LDS: with
+// the packed layouts each lane's MFMA operand (8 k of one row of A or one
+// column of B) is a pair of 16-byte loads straight from memory; tiles that
+// share rows of A or columns of B reuse them through the caches. Both operands
+// give lane group q (lanes 16q..16q+15) the k of group q of the 32-k chunk, so
+// the products pair equal k whatever order the matrix core sums them in. A
+// complex product is four real MFMAs into Re / Im accumulators (Re gets
+// Ar Br + Ai (-Bi)). The next chunk's operands are loaded before the current
+// chunk's MFMAs.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_cgemm_h16(int m, int ncol, int kp, const _Float16* __restrict__ A,
+                                                  long long sA, const _Float16* __restrict__ B, long long ldb,
+                                                  long long sB, float* __restrict__ C, long long ldc, long long sC,
+                                                  int cnt, int joint) {
+  const int lane = threadIdx.x, li = lane & 15, q = lane >> 4;
+  const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 32;
+  // joint: the members' columns tile together (one A); else member blockIdx.z
+  const int ntot = joint ? ncol * cnt : ncol;
+  const int mb = joint ? 0 : (int)blockIdx.z;
+  const _Float16* a = A + sA * mb;
+  const _Float16* pa[4];
+  const _Float16* pb[2];
+  const long long kg = kp / 8;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int r = min(row0 + 16 * t + li, m - 1);
+    pa[t] = a + ((long long)r * kg + q) * 16;
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int vc = min(col0 + 16 * u + li, ntot - 1);
+    const int b = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
+    pb[u] = B + sB * b + ldb * c + q * 16;
+  }
+  float4_t cre[4][2], cim[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) { cre[t][u][v] = 0.0f; cim[t][u][v] = 0.0f; }
+  half8_t ar[4], ai[4], br[2], bi[2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    ar[t] = *(const half8_t*)pa[t];
+    ai[t] = *(const half8_t*)(pa[t] + 8);
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    br[u] = *(const half8_t*)pb[u];
+    bi[u] = *(const half8_t*)(pb[u] + 8);
+  }
+  const int nk = kp / 32;
+  for (int kc = 0; kc < nk; ++kc) {
+    half8_t nar[4], nai[4], nbr[2], nbi[2];
+    const long long o = (long long)min(kc + 1, nk - 1) * 64;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      nar[t] = *(const half8_t*)(pa[t] + o);
+      nai[t] = *(const half8_t*)(pa[t] + o + 8);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      nbr[u] = *(const half8_t*)(pb[u] + o);
+      nbi[u] = *(const half8_t*)(pb[u] + o + 8);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const half8_t bn = -bi[u];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        cre[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[t], br[u], cre[t][u], 0, 0, 0);
+        cre[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[t], bn, cre[t][u], 0, 0, 0);
+        cim[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ar[t], bi[u], cim[t][u], 0, 0, 0);
+        cim[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[t], br[u], cim[t][u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) { ar[t] = nar[t]; ai[t] = nai[t]; }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) { br[u] = nbr[u]; bi[u] = nbi[u]; }
+  }
+  // lane holds rows 4q..4q+3 of each 16 x 16 tile at its column li
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int vc = col0 + 16 * u + li;
+    if (vc >= ntot) continue;
+    const int b = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
+    float* cc = C + 2 * (sC * b + ldc * c);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 16 * t + 4 * q + v;
+        if (r < m) {
+          cc[2 * (long long)r] = cre[t][u][v];
+          cc[2 * (long long)r + 1] = cim[t][u][v];
+        }
+      }
+  }
+}
+
+hipError_t cgemm_h16(hipStream_t s, int m, int ncol, int kp, const _Float16* A, long long sA, const _Float16* B,
+                     long long ldb, long long sB, float* C, long long ldc, long long sC, int cnt) {
+  if (m <= 0 || ncol <= 0 || cnt <= 0) return hipSuccess;
+  if (kp <= 0 || kp % 32) return hipErrorInvalidValue;
+  const int joint = sA == 0 ? 1 : 0;
+  const int cols = joint ? ncol * cnt : ncol;
+  hipLaunchKernelGGL(k_cgemm_h16, dim3((m + 63) / 64, (cols + 31) / 32, joint ? 1 : cnt), dim3(64), 0, s, m, ncol, kp,
+                     A, sA, B, ldb, sB, C, ldc, sC, cnt, joint);
+  return hipGetLastError();
+}
+
+// thread: row r of member blockIdx.z, k group blockIdx.y (8 k)
+__global__ void k_pack_a16(const float* __restrict__ src, int lda, long long ssrc, int m, int k, int kp,
+                           _Float16* __restrict__ dst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (r >= m) return;
+  const float* a = src + 2 * ssrc * blockIdx.z;
+  half8_t re, im;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int kk = 8 * g + e;
+    float xr = 0.0f, xi = 0.0f;
+    if (kk < k) {
+      const long long o = 2 * ((long long)r + (long long)kk * lda);
+      xr = a[o];
+      xi = a[o + 1];
+    }
+    re[e] = (_Float16)xr;
+    im[e] = (_Float16)xi;
+  }
+  _Float16* d = dst + (long long)blockIdx.z * m * kp * 2 + ((long long)r * (kp / 8) + g) * 16;
+  *(half8_t*)d = re;
+  *(half8_t*)(d + 8) = im;
+}
+
+hipError_t pack_a16(hipStream_t s, const float* src, int lda, long long ssrc, int m, int k, _Float16* dst, int cnt) {
+  if (m <= 0 || k <= 0 || cnt <= 0) return hipSuccess;
+  const int kp = h16_kp(k);
+  hipLaunchKernelGGL(k_pack_a16, dim3((m + 127) / 128, kp / 8, cnt), dim3(128), 0, s, src, lda, ssrc, m, k, kp, dst);
+  return hipGetLastError();
+}
+
+// thread: group gq (8 rows) of the members' padded rows, column blockIdx.y
+template <typename T>
+__global__ void k_pack_b16(const T* __restrict__ V, long long ldv, const int* __restrict__ perm, int r0, int nb,
+                           int kp, int cnt, _Float16* __restrict__ W, long long ldw) {
+  const int gq = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y, kg = kp / 8;
+  if (gq >= kg * cnt) return;
+  const int b = gq / kg, g = gq % kg;
+  half8_t re, im;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int i = 8 * g + e;
+    T xr = 0, xi = 0;
+    if (i < nb) {
+      const int row = r0 + b * nb + i;
+      const long long o = 2 * ((long long)(perm ? perm[row] : row) + (long long)c * ldv);
+      xr = V[o];
+      xi = V[o + 1];
+    }
+    re[e] = (_Float16)xr;
+    im[e] = (_Float16)xi;
+  }
+  _Float16* d = W + ldw * c + (long long)gq * 16;
+  *(half8_t*)d = re;
+  *(half8_t*)(d + 8) = im;
+}
+
+template <typename T>
+static hipError_t pack_b16_t(hipStream_t s, const T* V, long long ldv, const int* perm, int r0, int nb, int cnt,
+                             int ncol, _Float16* W, long long ldw) {
+  if (nb <= 0 || cnt <= 0 || ncol <= 0) return hipSuccess;
+  const int kp = h16_kp(nb), groups = kp / 8 * cnt;
+  hipLaunchKernelGGL(k_pack_b16<T>, dim3((groups + 255) / 256, ncol), dim3(256), 0, s, V, ldv, perm, r0, nb, kp, cnt,
+                     W, ldw);
+  return hipGetLastError();
+}
+
+hipError_t pack_b16(hipStream_t s, const double* V, long long ldv, const int* perm, int r0, int nb, int cnt, int ncol,
+                    _Float16* W, long long ldw) {
+  return pack_b16_t(s, V, ldv, perm, r0, nb, cnt, ncol, W, ldw);
+}
+hipError_t pack_b16(hipStream_t s, const float* V, long long ldv, const int* perm, int r0, int nb, int cnt, int ncol,
+                    _Float16* W, long long ldw) {
+  return pack_b16_t(s, V, ldv, perm, r0, nb, cnt, ncol, W, ldw);
+}
+
 }  // namespace hm

@@ -42,6 +42,29 @@ hipError_t lu_inverse_nopiv(hipStream_t s, const T* A, int lda, long long sA, in
 // with a zero pivot, or left as it is (zeroed by the caller).
 size_t gj_inverse_ws_bytes(int n);
 hipError_t fill_iota(hipStream_t s, int* p, long long n);   // p[i] = i
+
+// Complex GEMM with FP16 operands and FP32 accumulation on MFMA, over packed
+// operands: the contraction index k (padded with zeros to kp, a multiple of
+// 32) in groups of 8, each group 16 halves [re of k..k+7 | im of k..k+7].
+// - A (m x kp): row r, group g at (r * kp / 8 + g) * 16 halves (pack_a16);
+// - B (kp x ncol): column c, group g at c * ldb + g * 16 halves (pack_b16).
+// Member b of cnt: C_b (m x ncol, FP32 interleaved complex, column-major, ldc
+// in complex entries) at C + 2 b sC, A_b at A + b sA, B_b at B + b sB (halves).
+// sA = 0: one A for every member (the members' columns then tile together).
+hipError_t cgemm_h16(hipStream_t s, int m, int ncol, int kp, const _Float16* A, long long sA, const _Float16* B,
+                     long long ldb, long long sB, float* C, long long ldc, long long sC, int cnt);
+inline int h16_kp(int k) { return (k + 31) / 32 * 32; }
+// A packed from FP32 interleaved complex column-major matrices (m x k, lda;
+// cnt of them at stride ssrc complex entries) into dst (stride m * kp * 2 halves)
+hipError_t pack_a16(hipStream_t s, const float* src, int lda, long long ssrc, int m, int k, _Float16* dst, int cnt);
+// B packed from column-major complex vectors: member b's rows
+// [r0 + b nb, r0 + (b + 1) nb) of each of ncol columns (row i of the member is
+// row perm[r0 + b nb + i] of V when perm is given), into W at column stride ldw
+// halves, member b at row offset b kp (W's rows kp * cnt, zero padded)
+hipError_t pack_b16(hipStream_t s, const double* V, long long ldv, const int* perm, int r0, int nb, int cnt, int ncol,
+                    _Float16* W, long long ldw);
+hipError_t pack_b16(hipStream_t s, const float* V, long long ldv, const int* perm, int r0, int nb, int cnt, int ncol,
+                    _Float16* W, long long ldw);
 hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int n, void* ws, int* info, int batch);
 
 // ---- grouped operations: members of different sizes, one launch per step ----

@@ -41,7 +41,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
            "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing",
-           "hm_set_workers")
+           "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check")
 
 _lib = None
 
@@ -221,6 +221,19 @@ class Context:
         iters = self.solve()
         a, b, f = self.forces()
         return dict(amas=a, bdmp=b, exfc=f, iters=iters)
+
+    def set_precond_fp16(self, enable=True):
+        self._chk(self.L.hm_set_precond_fp16(self.ptr, _i(1 if enable else 0)), "hm_set_precond_fp16")
+
+    def cgemm_f16(self, a, b):
+        """a @ b with FP16 operands and FP32 accumulation on the device (a, b complex)"""
+        a = np.asfortranarray(a, dtype=np.complex64)
+        b = np.asfortranarray(b, dtype=np.complex64)
+        m, k = a.shape
+        n = b.shape[1]
+        out = np.zeros((m, n), dtype=np.complex64, order="F")
+        self._chk(self.L.hm_cgemm_f16_check(self.ptr, _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)), "hm_cgemm_f16_check")
+        return out
 
     def set_workers(self, n=1):
         """frequencies in flight in run_frequencies (clones of this context)"""

@@ -80,6 +80,17 @@ int hm_set_block_sharing(hm_ctx *ctx, int enable);
  * process's factors) and their mean rank at the last assembly. No
  * counterpart in the reference. */
 int hm_get_block_sharing(hm_ctx *ctx, int *diag_blocks, int *aca_processes, double *mean_rank_run);
+/* H-matrix form with explicit preconditioner inverses: the Arnoldi steps apply
+ * them by FP16 MFMA GEMMs with FP32 accumulation (enable = 1, the default) or
+ * by FP32 ones (0); the update of the solution uses the FP32 inverses either
+ * way. GMRES still stops on the FP64 residual; the FP16 rounding may cost one
+ * more restart cycle. No counterpart in the reference. */
+int hm_set_precond_fp16(hm_ctx *ctx, int enable);
+/* Parity entry point of the FP16 complex GEMM that applies those inverses:
+ * out (m x n) = a (m x k) * b (k x n), column-major, interleaved complex FP32
+ * host arrays, rounded to FP16 and packed on the device, FP32 accumulation.
+ * No counterpart in the reference. */
+int hm_cgemm_f16_check(hm_ctx *ctx, int m, int n, int k, const float *a, const float *b, float *out);
 /* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
 int hm_set_depth(hm_ctx *ctx, double h);
 /* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */

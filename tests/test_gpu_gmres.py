@@ -76,8 +76,11 @@ def test_explicit_block_inverse_matches_triangular_solves(quad):
     batched GEMM a Krylov step, precond 2 and the default) against the blocks'
     LU triangular solves (precond 1), dense and H-matrix: both are FP32
     approximations of the same M^-1, so GMRES takes the same number of steps
-    (within one) and converges to the same solution."""
+    (within one) and converges to the same solution. (The H-matrix form's FP16
+    application of the inverses, its default, is held to its FP32 one in
+    tests/test_gpu_fp16.py.)"""
     p, c = quad
+    c.set_precond_fp16(False)
     try:
         for hmat in (False, True):
             c.set_hmatrix(hmat, eps=1e-7, eps_s=1e-8, kmax=50)
@@ -92,6 +95,7 @@ def test_explicit_block_inverse_matches_triangular_solves(quad):
             for k in KEYS:
                 assert relerr(got[2][k], got[1][k]) < 1e-7, (hmat, k, relerr(got[2][k], got[1][k]))
     finally:
+        c.set_precond_fp16(True)
         c.set_hmatrix(False, eps=1e-5, eps_s=1e-6, kmax=50)
         c.set_gmres()
 
