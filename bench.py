@@ -387,6 +387,9 @@ def main():
     ap.add_argument("--no-block-sharing", action="store_true",
                     help="H-matrix path: assemble, factor and invert every body's diagonal block, also where bodies "
                          "are equal (the blocks are the same bit for bit either way)")
+    ap.add_argument("--no-symmetries", action="store_true",
+                    help="H-matrix path: share cross approximations between equal bodies at equal offsets only, not "
+                         "across offsets that a symmetry of the body relates")
     ap.add_argument("--no-extras", action="store_true",
                     help="H-matrix path: skip the passes after the timed region (the cross approximation alone, "
                          "the path without block sharing), e.g. for profiler runs")
@@ -458,7 +461,7 @@ def main():
         # their PIDCOCK terms
         sync()
         t0 = time.perf_counter()
-        ctx.set_block_sharing(not args.no_block_sharing)
+        ctx.set_block_sharing(not args.no_block_sharing, symmetries=not args.no_symmetries)
         ctx.set_hmatrix(True, eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax)
         sync()
         hm_setup_s = time.perf_counter() - t0
@@ -510,19 +513,27 @@ def main():
     no_share = None
     if args.path == "hmatrix" and not args.no_block_sharing and args.steps > 0 and not args.no_extras:
         # the same path with every body's diagonal block assembled, factored and
-        # inverted (outside the timed region; the outputs are the same bit for bit)
+        # inverted and every cross approximation run (outside the timed region;
+        # the outputs are the same bit for bit with sharing at equal offsets only,
+        # within the cross approximation's tolerance with the body symmetries)
         ctx.set_block_sharing(False)
         k = min(3, args.steps)
         dt_ns, res_ns, _ = time_path(ctx, prob, prec, k, args.warmup, stage_times=False)
         same = all(np.array_equal(a[q], b[q]) for a, b in zip(res_ns, res_mx[:k]) for q in ("amas", "bdmp", "exfc"))
-        no_share = {"systems": k, "value": k / dt_ns, "ms_per_step": 1e3 * dt_ns / k, "outputs_bitwise_equal": same}
-        ctx.set_block_sharing(True)
+        dev = max(float(np.max(np.abs(a[q] - b[q])) / max(float(np.max(np.abs(a[q]))), 1e-300))
+                  for a, b in zip(res_ns, res_mx[:k]) for q in ("amas", "bdmp", "exfc"))
+        no_share = {"systems": k, "value": k / dt_ns, "ms_per_step": 1e3 * dt_ns / k, "outputs_bitwise_equal": same,
+                    "largest_normwise_deviation": dev}
+        ctx.set_block_sharing(True, symmetries=not args.no_symmetries)
         print(f"# without block sharing: {no_share}", file=sys.stderr, flush=True)
     if args.path == "hmatrix":
         hmat = dict(ctx.hmatrix_stats(), eps=args.aca_eps, eps_s=args.aca_eps_s, kmax=args.aca_kmax,
                     setup_s=hm_setup_s, distinct_bodies=int(len(set(ctx.body_classes().tolist()))),
                     diag_blocks_assembled=ctx.block_sharing()[0], aca_processes_run=ctx.block_sharing()[1],
-                    block_sharing=not args.no_block_sharing, without_block_sharing=no_share)
+                    block_sharing=not args.no_block_sharing,
+                    body_symmetries_used=(not args.no_block_sharing and not args.no_symmetries
+                                          and int(ctx.body_symmetries().sum())),
+                    without_block_sharing=no_share)
         ctx.set_hmatrix(False)
         print(f"# hmatrix {hmat}", file=sys.stderr, flush=True)
     if sur is not None:

@@ -19,6 +19,7 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <algorithm>
 #include <vector>
 
 #include "hm_noalloc.hpp"
@@ -111,6 +112,14 @@ struct hm_ctx {
   DevBuf d_pshare, d_prun;           // the map and the list of the processes that run
   DevBuf d_evt, d_pmt;               // the cross approximation's per-frequency depth-factor tables
   std::vector<int> pshare_host;
+  // share_sym: a process may also take the block of a process whose offset a
+  // symmetry of the (one) body class maps onto its own, its rows and columns
+  // permuted by that symmetry's panel map (psym_host[p]: 0 = plain copy, s =
+  // map s of kSymMap); sym_ok[s]: map s takes the body's hull panels onto
+  // themselves, sym_iperm[s m + k]: the panel that map s takes onto panel k
+  bool share_sym = true;
+  std::vector<int> psym_host, sym_ok, sym_iperm;
+  DevBuf d_psym, d_sperm;
   bool pshare_ok = false;            // d_pshare holds the current geometry's map
   double hm_rank_mean_run = 0.0;     // mean rank of the processes that ran
   int n_pshared = 0;                 // processes that copy another's block
@@ -668,8 +677,12 @@ extern "C" int hm_get_block_sharing(hm_ctx* c, int* diag_blocks, int* aca_proces
 
 extern "C" int hm_set_block_sharing(hm_ctx* c, int enable) {
   if (!c) return HM_ERR_ARG - 1;
+  if (enable < 0 || enable > 2) return c->fail(HM_ERR_ARG - 2, "enable must be 0, 1 or 2, got %d", enable);
   const bool was = c->block_share;
   c->block_share = enable != 0;
+  if (c->share_sym != (enable == 2)) c->pshare_ok = false;
+  c->share_sym = enable == 2;
+  ++c->cfg_gen;
   c->assembled = c->factored = c->solved = false;
   if (was != c->block_share && c->hm_on && c->have_depth && c->h > 0.0) return classify(c);
   return HM_OK;
@@ -904,17 +917,96 @@ static int precond_early(hm_ctx* c);
 // have the same representatives and whose placements differ by the same
 // vector, computed as the entry kernels compute it (o_b1 - o_b2): its entries
 // are the same bit for bit
+// the horizontal maps of the square's symmetry group, (x, y) -> (a x + b y,
+// c x + d y): identity, rotations by 90, 180, 270 degrees, reflections in the
+// x axis, the y axis and the two diagonals
+static const int kSymMap[8][4] = {{1, 0, 0, 1},  {0, -1, 1, 0}, {-1, 0, 0, -1}, {0, 1, -1, 0},
+                                  {1, 0, 0, -1}, {-1, 0, 0, 1}, {0, 1, 1, 0},   {0, -1, -1, 0}};
+
+// which maps of kSymMap take the hull panels of body 0 (local frame, the
+// vertical axis through the body's origin) onto themselves: every panel onto
+// one with the mapped centroid and normal and the same area and vertex count,
+// to 1e-9 of the body's size. Only for one body class (every body equal).
+static void find_symmetries(const hm_ctx* c, std::vector<int>& sym_ok, std::vector<int>& sym_iperm) {
+  sym_ok.assign(8, 0);
+  sym_iperm.clear();
+  if (!c->frames || c->n_ublk != 1) return;
+  const int j0 = c->body_start[0], m = c->body_start[1] - j0;
+  if (m < 1) return;
+  auto cen = [&](int j, int k) { return c->fp_host[3 * (size_t)(j0 + j) + k]; };
+  auto nrm = [&](int j, int k) { return c->dxyz_host[6 * (size_t)(j0 + j) + k]; };
+  double L = 0.0;
+  for (int j = 0; j < m; ++j)
+    for (int k = 0; k < 3; ++k) L = std::max(L, fabs(cen(j, k)));
+  const double tol = 1e-9 * (L > 0.0 ? L : 1.0);
+  // panels sorted by centroid x: the candidates of a mapped centroid are a range
+  std::vector<int> ord(m);
+  for (int j = 0; j < m; ++j) ord[j] = j;
+  std::sort(ord.begin(), ord.end(), [&](int a, int b) { return cen(a, 0) < cen(b, 0); });
+  std::vector<double> xs(m);
+  for (int j = 0; j < m; ++j) xs[j] = cen(ord[j], 0);
+  sym_iperm.assign(8 * (size_t)m, -1);
+  for (int j = 0; j < m; ++j) sym_iperm[j] = j;
+  sym_ok[0] = 1;
+  for (int sidx = 1; sidx < 8; ++sidx) {
+    const int* q = kSymMap[sidx];
+    int* ip = sym_iperm.data() + (size_t)sidx * m;
+    bool ok = true;
+    for (int i = 0; i < m && ok; ++i) {
+      const double tx = q[0] * cen(i, 0) + q[1] * cen(i, 1), ty = q[2] * cen(i, 0) + q[3] * cen(i, 1);
+      const double nx = q[0] * nrm(i, 0) + q[1] * nrm(i, 1), ny = q[2] * nrm(i, 0) + q[3] * nrm(i, 1);
+      int hit = -1;
+      for (auto it = std::lower_bound(xs.begin(), xs.end(), tx - tol); it != xs.end() && *it <= tx + tol; ++it) {
+        const int k = ord[it - xs.begin()];
+        if (fabs(cen(k, 1) - ty) <= tol && fabs(cen(k, 2) - cen(i, 2)) <= tol && fabs(nrm(k, 0) - nx) <= 1e-9 &&
+            fabs(nrm(k, 1) - ny) <= 1e-9 && fabs(nrm(k, 2) - nrm(i, 2)) <= 1e-9 &&
+            fabs(c->ds_host[j0 + k] - c->ds_host[j0 + i]) <= 1e-9 * c->ds_host[j0 + i] &&
+            c->ncn_host[j0 + k] == c->ncn_host[j0 + i]) {
+          hit = k;
+          break;
+        }
+      }
+      if (hit < 0 || ip[hit] >= 0) ok = false;
+      else ip[hit] = i;
+    }
+    sym_ok[sidx] = ok ? 1 : 0;
+  }
+}
+
+static void body_symmetries(hm_ctx* c) {
+  if (c->share_sym) find_symmetries(c, c->sym_ok, c->sym_iperm);
+  else {
+    c->sym_ok.assign(8, 0);
+    c->sym_iperm.clear();
+  }
+}
+
+extern "C" int hm_get_body_symmetries(hm_ctx* c, int* ok) {
+  if (!c) return HM_ERR_ARG - 1;
+  if (!c->have_geom) return c->fail(HM_ERR_STATE, "no geometry");
+  if (!ok) return c->fail(HM_ERR_ARG - 2, "ok is NULL");
+  std::vector<int> so, ip;
+  find_symmetries(c, so, ip);
+  for (int k = 0; k < 8; ++k) ok[k] = so[k];
+  return HM_OK;
+}
+
 static int pshare_build(hm_ctx* c) {
   if (c->pshare_ok) return HM_OK;
   const int nb = c->nbody, per = nb * (nb - 1), P = hm::hmat_processes(nb);
-  std::vector<int> sh(P);
+  body_symmetries(c);
+  std::vector<int> sh(P), sy(P, 0);
   c->n_pshared = 0;
+  int nsym = 0;
+  double span = 0.0;
+  for (int k = 0; k < 2 * nb; ++k) span = std::max(span, fabs(c->origin_host[k]));
+  const double otol = 1e-12 * (span > 1.0 ? span : 1.0);
   for (int p = 0; p < P; ++p) {
     const int mat = p / per, r = p % per, b1 = r / (nb - 1), b2p = r % (nb - 1), b2 = b2p + (b2p >= b1 ? 1 : 0);
     const double dx = c->origin_host[2 * b1] - c->origin_host[2 * b2];
     const double dy = c->origin_host[2 * b1 + 1] - c->origin_host[2 * b2 + 1];
     sh[p] = p;
-    for (int q = mat * per; q < p; ++q) {
+    for (int q = mat * per; q < p && sh[p] == p; ++q) {
       if (sh[q] != q) continue;
       const int rq = q % per, c1 = rq / (nb - 1), c2p = rq % (nb - 1), c2 = c2p + (c2p >= c1 ? 1 : 0);
       if (c->body_rep[c1] != c->body_rep[b1] || c->body_rep[c2] != c->body_rep[b2]) continue;
@@ -922,18 +1014,35 @@ static int pshare_build(hm_ctx* c) {
       const double ey = c->origin_host[2 * c1 + 1] - c->origin_host[2 * c2 + 1];
       if (memcmp(&ex, &dx, sizeof(double)) == 0 && memcmp(&ey, &dy, sizeof(double)) == 0) {
         sh[p] = q;
-        ++c->n_pshared;
         break;
       }
+      // a symmetry of the body that maps q's offset onto p's: p's block is
+      // q's with rows and columns permuted by the symmetry's panel map
+      for (int sidx = 1; sidx < (int)c->sym_ok.size(); ++sidx) {
+        if (!c->sym_ok[sidx]) continue;
+        const int* m4 = kSymMap[sidx];
+        if (fabs(m4[0] * ex + m4[1] * ey - dx) <= otol && fabs(m4[2] * ex + m4[3] * ey - dy) <= otol) {
+          sh[p] = q;
+          sy[p] = sidx;
+          ++nsym;
+          break;
+        }
+      }
     }
+    if (sh[p] != p) ++c->n_pshared;
   }
   c->pshare_host = sh;
+  c->psym_host = nsym ? sy : std::vector<int>();
   std::vector<int> run;
   for (int p = 0; p < P; ++p)
     if (sh[p] == p) run.push_back(p);
   int rc = upload(c, c->d_pshare, sh.data(), sh.size(), "cross approximation map");
   if (rc) return rc;
   if ((rc = upload(c, c->d_prun, run.data(), run.size(), "cross approximation map"))) return rc;
+  if (nsym) {
+    if ((rc = upload(c, c->d_psym, sy.data(), sy.size(), "cross approximation map"))) return rc;
+    if ((rc = upload(c, c->d_sperm, c->sym_iperm.data(), c->sym_iperm.size(), "cross approximation map"))) return rc;
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream), "cross approximation map");
   c->pshare_ok = true;
   return HM_OK;
@@ -1000,6 +1109,10 @@ static int assemble_hmatrix(hm_ctx* c) {
       c->hmat.share = c->d_pshare.as<int>();
       c->hmat.run = c->d_prun.as<int>();
       c->hmat.nrun = hm::hmat_processes(nb) - c->n_pshared;
+      if (!c->psym_host.empty()) {
+        c->hmat.psym = c->d_psym.as<int>();
+        c->hmat.sperm = c->d_sperm.as<int>();
+      }
     }
   }
   const hm::GeomDev gd = c->geom();
@@ -2474,11 +2587,11 @@ extern "C" int hm_solve_frequency(hm_ctx* c, double w1, int precision, double* a
 // the settings a clone must share with its context (with cfg_gen)
 static std::string worker_signature(const hm_ctx* c) {
   char b[512];
-  snprintf(b, sizeof b, "%d %d %d %d %d %d %d %.17g %.17g %d %d %d %.17g %d %d %d %d %.17g %d %d %d %d %.17g %d %d",
+  snprintf(b, sizeof b, "%d %d %d %d %d %d %d %.17g %.17g %d %d %d %.17g %d %d %d %d %.17g %d %d %d %d %.17g %d %d %d",
            c->series_variant, c->lu_coop, c->pair_sym, c->block_share, c->hm_on, c->hm_tree, c->ht_mode, c->hm_eps,
            c->hm_eps_s, c->hm_kmax, c->hm_overlap, c->gm_restart, c->gm_tol, c->gm_maxiter, c->gm_precond,
            c->sur_on, c->sur_quant, c->sur_tol, c->surd_on, c->frames, c->ht_leaf, c->nbody, c->ht_eta,
-           c->sur_ready, c->pinv16_on);
+           c->sur_ready, c->pinv16_on, c->share_sym);
   return b;
 }
 
@@ -2499,6 +2612,7 @@ static int clone_config(hm_ctx* w, hm_ctx* c) {
   w->lu_coop = c->lu_coop;
   w->pair_sym = c->pair_sym;
   w->block_share = c->block_share;
+  w->share_sym = c->share_sym;
   w->hm_overlap = c->hm_overlap;
   w->ht_mode = c->ht_mode;
   w->ht_leaf = c->ht_leaf;

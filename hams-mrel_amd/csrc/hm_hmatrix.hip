@@ -437,16 +437,25 @@ __global__ void k_aca_share_mark(HmatDev h, int P) {
 }
 
 // after the last step: a shared process takes its representative's factors
-// (the first rank columns of U and V; the rest stay zero), rank, norm and state
+// (the first rank columns of U and V; the rest stay zero), rank, norm and state.
+// Related by a body symmetry s, block(p)[pi(i), pi(j)] = block(r)[i, j] for
+// the symmetry's panel map pi, so row k of p's factors is row pi^-1(k) of r's.
 __global__ __launch_bounds__(256) void k_aca_share_copy(HmatDev h) {
   const int p = blockIdx.y;
   const int r = h.share[p];
   if (r == p) return;
   const Proc q = proc(h, p), qr = proc(h, r);
   const long long len = 2LL * h.m * h.rank[r];
+  const int sy = h.psym ? h.psym[p] : 0;
+  const int* ip = sy ? h.sperm + (long long)sy * h.m : nullptr;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long long)gridDim.x * blockDim.x) {
-    q.U[e] = qr.U[e];
-    q.V[e] = qr.V[e];
+    long long src = e;
+    if (ip) {
+      const long long col = e / (2LL * h.m), w = e - col * 2LL * h.m;
+      src = col * 2LL * h.m + 2LL * ip[w >> 1] + (w & 1);
+    }
+    q.U[e] = qr.U[src];
+    q.V[e] = qr.V[src];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     h.rank[p] = h.rank[r];
@@ -545,6 +554,8 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.share = nullptr;
   h.run = nullptr;
   h.nrun = 0;
+  h.psym = nullptr;
+  h.sperm = nullptr;
   return h;
 }
 

@@ -56,6 +56,12 @@ struct HmatDev {
   // so that the step launches' grids cover those only
   const int* run;
   int nrun;
+  // psym (nullptr: none): for a process that copies another's block, 0 for a
+  // plain copy or the index s of the body symmetry whose panel map relates the
+  // two blocks; row k of the copy's U and V factors is then row sperm[s m + k]
+  // of the source's (sperm: the panel each map takes onto panel k, m per map)
+  const int* psym;
+  const int* sperm;
   // 1: the bodies' horizontal bounding circles are >= 0.05 h apart, so no
   // off-diagonal entry is PIDCOCK or LINTON; the entry kernels then take the
   // EIGENE-only lane groups of 8 (any such entry would still be redone per lane)

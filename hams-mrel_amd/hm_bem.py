@@ -41,7 +41,8 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
            "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing",
-           "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check")
+           "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check",
+           "hm_get_body_symmetries")
 
 _lib = None
 
@@ -177,8 +178,21 @@ class Context:
                   "hm_get_block_sharing")
         return (d.value, a.value, r.value) if with_rank else (d.value, a.value)
 
-    def set_block_sharing(self, enable=True):
-        self._chk(self.L.hm_set_block_sharing(self.ptr, _i(1 if enable else 0)), "hm_set_block_sharing")
+    def body_symmetries(self):
+        """flags of the maps identity, rotations by 90/180/270 degrees,
+        reflections in x axis, y axis, y = x, y = -x that take the (equal)
+        bodies' hull panels onto themselves"""
+        ok = np.zeros(8, dtype=np.int32)
+        self._chk(self.L.hm_get_body_symmetries(self.ptr, _p(ok)), "hm_get_body_symmetries")
+        return ok.astype(bool)
+
+    def set_block_sharing(self, enable=True, symmetries=True):
+        """enable: equal bodies share diagonal blocks and, at equal offsets,
+        cross approximations; symmetries: also at offsets that a symmetry of the
+        body (rotations by multiples of 90 degrees, reflections in the axes and
+        diagonals) maps onto each other"""
+        mode = (2 if symmetries else 1) if enable else 0
+        self._chk(self.L.hm_set_block_sharing(self.ptr, _i(mode)), "hm_set_block_sharing")
 
     def set_frequency(self, w1):
         self._chk(self.L.hm_set_frequency(self.ptr, _d(w1)), "hm_set_frequency")

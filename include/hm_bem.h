@@ -70,10 +70,24 @@ int hm_get_body_classes(hm_ctx *ctx, int nbody, int *rep);
 /* H-matrix form (hm_set_hmatrix): bodies that share a representative (see
  * hm_get_body_classes) share its diagonal block: it is assembled, factored
  * and inverted once per frequency and copied; off-diagonal blocks between
- * such bodies at an equal offset share one cross approximation (enable = 1,
- * the default). enable = 0: every block is computed. Either way the blocks
- * are the same bit for bit. No counterpart in the reference. */
+ * such bodies at an equal offset share one cross approximation (enable = 1:
+ * the blocks are the same bit for bit as with enable = 0, where every block is
+ * computed). enable = 2 (the default): when every body is equal, also blocks at
+ * offsets that a symmetry of the body maps onto each other (a rotation by a
+ * multiple of 90 degrees or a reflection in an axis or a diagonal of its
+ * frame that takes its hull panels onto themselves, to 1e-9 of its size):
+ * such a block is the other's with rows and columns permuted by the
+ * symmetry's panel map, and takes its cross approximation so permuted (the
+ * same approximation error, other rounding and pivots than its own).
+ * No counterpart in the reference. */
 int hm_set_block_sharing(hm_ctx *ctx, int enable);
+/* ok[8]: which horizontal maps take the hull panels of every body (all equal,
+ * body frames on) onto themselves, in the order identity, rotations by 90,
+ * 180 and 270 degrees, reflections in the x axis, the y axis, the diagonal
+ * y = x and the diagonal y = -x (about the vertical axis through the frame's
+ * origin); all 0 without body frames or with unequal bodies. No counterpart in
+ * the reference. */
+int hm_get_body_symmetries(hm_ctx *ctx, int *ok);
 /* With the H-matrix form on: the diagonal blocks assembled per frequency, the
  * cross-approximation processes run (one per off-diagonal block and matrix;
  * blocks of equal bodies at an equal offset, o_b1 - o_b2 bit for bit, take one

@@ -11,9 +11,17 @@ such bodies at an equal offset share one cross approximation.
 
 Bars:
 - the dense FP64 path's diagonal blocks of equal bodies: equal bit for bit;
-- the H-matrix solution with and without sharing: equal bit for bit (the
-  shared blocks are the blocks each body would get, and the rest of the
-  computation is the same launches on the same data);
+- the H-matrix solution with and without sharing at equal offsets: equal bit
+  for bit (the shared blocks are the blocks each body would get, and the rest
+  of the computation is the same launches on the same data);
+- body symmetries: the maps found are exactly those of the body's shape (a
+  cylinder with 16 panels a ring: all eight; a 2:1 box: identity, 180 degrees
+  and the two axis reflections; the cylinder moved off its frame's origin
+  along x: identity and the x-axis reflection); sharing across offsets that
+  they relate (a permuted copy of the other block's cross approximation, the
+  same approximation error) keeps A/B/F within the H-matrix form's 1e-4 of the
+  dense FP64 solve, and its block products are held to the dense matrices in
+  tests/test_gpu_hmatrix.py (20 eps, the default sharing);
 - frames against the global coordinates (the oracle's): infinite depth within
   1e-12 (rounding of the pair differences only); finite depth within 20x the
   oracle's own band for +-1 ulp moves of the nodes (the reference's epsilon
@@ -92,7 +100,7 @@ def test_hmatrix_sharing_is_bitwise(h):
         for c, share in ((on, True), (off, False)):
             c.set_hmatrix(True, eps=1e-6, eps_s=1e-7, kmax=50)
             c.set_gmres(restart=50, tol=1e-6)
-            c.set_block_sharing(share)
+            c.set_block_sharing(share, symmetries=False)
         # 2 x 2 equal bodies: one diagonal block; 12 ordered pairs at 8 distinct
         # offsets, per matrix (A and S)
         assert on.block_sharing() == (1, 16)
@@ -141,3 +149,54 @@ def test_frames_against_global_coordinates(oracle):
         assert max(errs.values()) <= tol, (errs, tol)
     finally:
         c.close()
+
+
+def _equal_bodies(body, centers):
+    """copies of one body (its nodes taken as its local frame) placed at centers"""
+    local = body.local if body.local is not None else body.xyz
+    return [meshes.Body(xyz=meshes._placed(local, c), ncn=body.ncn, ncon=body.ncon, xr=(c[0], c[1], 0.0),
+                        local=local.copy(), origin=(float(c[0]), float(c[1]))) for c in centers]
+
+
+def test_body_symmetries():
+    centers = [(0.0, 0.0), (15.0, 0.0)]
+    cyl = meshes.cylinder(3.0, 4.0, 16, 4, 2)
+    moved = meshes.Body(xyz=cyl.xyz, ncn=cyl.ncn, ncon=cyl.ncon, local=cyl.local + np.array([0.5, 0.0, 0.0]))
+    cases = [(cyl, [1, 1, 1, 1, 1, 1, 1, 1]),
+             (meshes.box(8.0, 4.0, 3.0, 8, 4, 3), [1, 0, 1, 0, 1, 1, 0, 0]),
+             (moved, [1, 0, 0, 0, 1, 0, 0, 0])]
+    for body, want in cases:
+        p = meshes.Problem(_equal_bodies(body, centers), h=40.0, omegas=[0.5], beta=np.array([0.0]))
+        c = _ctx(p)
+        try:
+            assert c.body_symmetries().astype(int).tolist() == want
+        finally:
+            c.close()
+    # unequal bodies or no frames: none
+    p = _array(2, 1, 40.0)
+    c = _ctx(p, frames=False)
+    try:
+        assert not c.body_symmetries().any()
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("h", [60.0, -1.0], ids=["finite", "deep"])
+def test_symmetric_sharing_against_dense(h):
+    p = _array(2, 2, h)
+    sym, dense = _ctx(p), _ctx(p)
+    try:
+        sym.set_hmatrix(True, eps=1e-6, eps_s=1e-7, kmax=50)
+        sym.set_gmres(restart=50, tol=1e-6)
+        # 2 x 2 equal cylinders: the axis offsets are one class and the diagonal
+        # ones another, per matrix (A and S)
+        assert sym.block_sharing() == (1, 4)
+        for w in p.omegas:
+            got = sym.solve_frequency(w, hm_bem.HM_GMRES)
+            ref = dense.solve_frequency(w, hm_bem.HM_FP64)
+            errs = {k: relerr(got[k], ref[k]) for k in KEYS}
+            print(f"h={h} w={w}: symmetric sharing against dense FP64 {errs}", flush=True)
+            assert max(errs.values()) < 1e-4, errs
+    finally:
+        sym.close()
+        dense.close()
