@@ -207,8 +207,11 @@ struct hm_ctx {
   double hm_eps = 1e-4, hm_eps_s = 1e-4;
   int hm_kmax = 50;
   DevBuf d_hmat, d_hD, d_hDS, d_hD32, d_hU32, d_hV32, d_htb, d_hts;
-  DevBuf d_hUc[2], d_hVc[2];   // factors of A and S packed at their largest rank
-  int hm_kuse[2] = {0, 0};
+  DevBuf d_hUc[2], d_hVc[2];   // factors of A and S packed at their ranks (hmat_pack)
+  DevBuf d_lrmeta[2];           // the packing's device metadata
+  hm::LowRankPlan lr_plan[2];
+  hm::LowRank hm_lr[2];
+  int hm_kuse[2] = {0, 0};      // the largest rank of each matrix
   hm::HmatDev hmat{};
   // H-matrix built by hm_ziter_solve (ISOLV = 2) from an assembled matrix:
   // hd_nb clusters of hd_m rows
@@ -426,7 +429,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
                     &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_gjws, &c->d_Pinv16, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
                     &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
-                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_htw,
+                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_lrmeta[0], &c->d_lrmeta[1], &c->d_htw,
                     &c->d_surd, &c->d_surdcnt};
   for (DevBuf* b : bufs) b->release();
   c->htree.release();
@@ -1048,6 +1051,24 @@ static int pshare_build(hm_ctx* c) {
   return HM_OK;
 }
 
+// matrix mat's factors packed at their ranks (rank: every process's rank)
+static int pack_lowrank(hm_ctx* c, hipStream_t s, int mat, const std::vector<int>& rank) {
+  const int nb = c->hmat.nb, m = c->hmat.m, per = nb * (nb - 1);
+  hm::LowRankPlan& pl = c->lr_plan[mat];
+  hm::lowrank_plan(nb, rank.data() + (size_t)mat * per, pl);
+  const size_t cplx = 2 * sizeof(double);
+  HIPCHK(c, c->d_hUc[mat].ensure(cplx * (size_t)nb * m * std::max(pl.ru, 1)), "hmatrix alloc");
+  HIPCHK(c, c->d_hVc[mat].ensure(cplx * (size_t)nb * m * std::max(pl.rv, 1)), "hmatrix alloc");
+  HIPCHK(c, c->d_lrmeta[mat].ensure(sizeof(int) * hm::lowrank_meta_ints(nb, pl.ru)), "hmatrix alloc");
+  HIPCHK(c, hm::hmat_pack(s, c->hmat, mat, pl, c->d_lrmeta[mat].as<int>(), c->d_hUc[mat].as<double>(),
+                          c->d_hVc[mat].as<double>(), &c->hm_lr[mat]), "hmatrix pack");
+  return HM_OK;
+}
+
+// complex entries of the packed U and V factors of matrix mat
+static size_t lowrank_u_elems(const hm_ctx* c, int mat) { return (size_t)c->hmat.nb * c->hmat.m * std::max(c->hm_lr[mat].ru, 1); }
+static size_t lowrank_v_elems(const hm_ctx* c, int mat) { return (size_t)c->hmat.nb * c->hmat.m * std::max(c->hm_lr[mat].rv, 1); }
+
 // H-matrix diagonal blocks shared by bodies with equal hull panels: body b's
 // block (elements [b stride, b stride + count) of `base`, `esize` bytes each)
 // becomes a copy of its representative's
@@ -1189,22 +1210,19 @@ static int assemble_hmatrix(hm_ctx* c) {
       if (!sh || c->pshare_host[p] == p) { rr += rank[p]; ++cnt; }
     c->hm_rank_mean_run = cnt ? (double)rr / cnt : 0.0;
   }
-  // pack each matrix's factors at its largest rank, so that the products run
-  // over the ranks the blocks need instead of kmax
+  // pack each matrix's factors at their ranks, so that the products run over
+  // the ranks the blocks have instead of kmax
   for (int mat = 0; mat < 2; ++mat) {
     int ku = 0;
     for (int p = mat * P / 2; p < (mat + 1) * P / 2; ++p) ku = rank[p] > ku ? rank[p] : ku;
     c->hm_kuse[mat] = ku;
-    const size_t bytes = cplx * (size_t)nb * (nb - 1) * m * (ku ? ku : 1);
-    HIPCHK(c, c->d_hUc[mat].ensure(bytes), "hmatrix alloc");
-    HIPCHK(c, c->d_hVc[mat].ensure(bytes), "hmatrix alloc");
-    if (ku) HIPCHK(c, hm::hmat_compact(c->stream, c->hmat, mat, ku, c->d_hUc[mat].as<double>(), c->d_hVc[mat].as<double>()),
-                   "hmatrix pack");
+    int rc1 = pack_lowrank(c, c->stream, mat, rank);
+    if (rc1) return rc1;
   }
   // right-hand sides RHS = S * W
   c->stage_begin(kRhs);
   HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
-  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_kuse[1], DS, m, (long long)m * (m + 1),
+  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[1], DS, m, (long long)m * (m + 1),
                                    c->d_hUc[1].as<double>(), c->d_hVc[1].as<double>(), c->d_W.as<double>(), n,
                                    c->d_R.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
          "rhs hmatrix product");
@@ -1337,34 +1355,34 @@ static int precond_early(hm_ctx* c) {
 // low-rank factors for the Krylov products, and the block-diagonal
 // preconditioner (one block per body) factored in FP32
 static int factor_hmatrix(hm_ctx* c) {
-  const int nb = c->nbody, m = c->bm(), k = c->hm_kuse[0] ? c->hm_kuse[0] : 1;
+  const int nb = c->nbody, m = c->bm();
   const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
-  const size_t lr = (size_t)nb * (nb - 1) * m * k;   // complex entries of A's packed U (or V)
+  const size_t lu = lowrank_u_elems(c, 0), lv = lowrank_v_elems(c, 0);   // complex entries of A's packed U, V
   c->join_side();
   if (c->prec_early) {
     // the diagonal blocks were demoted, factored and inverted during the
     // assembly, beside the cross approximation: their info words, then A's factors
     c->prec_early = false;
     HIPCHK(c, hipMemcpyAsync(c->d_info.p, c->d_info2.p, sizeof(int) * nb, hipMemcpyDeviceToDevice, c->stream), "factor");
-    HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "factor alloc");
-    HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "factor alloc");
-    HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1),
+    HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lu), "factor alloc");
+    HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lv), "factor alloc");
+    HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lu / m), 1),
            "demote");
-    HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
+    HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lv / m), 1),
            "demote");
     return HM_OK;
   }
   HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
   HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "factor alloc");
   HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
-  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "factor alloc");
-  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "factor alloc");
+  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lu), "factor alloc");
+  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lv), "factor alloc");
   const double* D = c->d_hD.as<double>();
   HIPCHK(c, hm::demote(c->stream, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
   HIPCHK(c, hm::demote(c->stream, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
-  HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1),
+  HIPCHK(c, hm::demote(c->stream, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lu / m), 1),
          "demote");
-  HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1),
+  HIPCHK(c, hm::demote(c->stream, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lv / m), 1),
          "demote");
   if (c->gm_precond != 0 && block_inverse(c)) {
     HIPCHK(c, c->d_pperm.ensure(sizeof(int) * (size_t)c->n), "factor alloc");
@@ -1586,7 +1604,8 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
     ga.tol = c->gm_tol;
     ga.inner_tol = 0.5 * c->gm_tol;
     if (c->hm_on) {
-      const int nb = c->nbody, m = c->bm(), k = c->hm_kuse[0];
+      const int nb = c->nbody, m = c->bm();
+      const hm::LowRank k = c->hm_lr[0];
       const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
       const float* D32 = c->d_hD32.as<float>();
       const float* U32 = c->d_hU32.as<float>();
@@ -1806,7 +1825,7 @@ extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, d
   if (ncol < 1 || (from_assembly && ncol > c->ncol())) return c->fail(HM_ERR_ARG - 3, "bad ncol %d", ncol);
   if (!x || !y) return c->fail(HM_ERR_ARG - 4, "x or y is NULL");
   const int nb = from_assembly ? c->nbody : c->hd_nb, m = from_assembly ? c->bm() : c->hd_m, n = nb * m;
-  const size_t tneed = 2 * sizeof(double) * (size_t)nb * (nb - 1) * (c->hm_kuse[mat] ? c->hm_kuse[mat] : 1) * ncol;
+  const size_t tneed = 2 * sizeof(double) * (size_t)nb * std::max(1, std::max(c->hm_lr[mat].ru, c->hm_lr[mat].rv)) * ncol;
   if (c->d_htb.bytes < tneed || c->d_hts.bytes < tneed) {
     int e = c->hip(c->d_htb.ensure(tneed), "hmatrix apply");
     if (e == HM_OK) e = c->hip(c->d_hts.ensure(tneed), "hmatrix apply");
@@ -1817,7 +1836,7 @@ extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, d
   int rc = HM_OK;
   if ((rc = c->hip(dx.ensure(bytes), "hmatrix apply")) == HM_OK && (rc = c->hip(dy.ensure(bytes), "hmatrix apply")) == HM_OK &&
       (rc = c->hip(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, c->stream), "hmatrix apply")) == HM_OK &&
-      (rc = c->hip(hm::hmat_apply<double>(c->stream, nb, m, c->hm_kuse[mat],
+      (rc = c->hip(hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[mat],
                                           mat ? c->d_hDS.as<double>() : c->d_hD.as<double>(), m, (long long)m * (m + 1),
                                           c->d_hUc[mat].as<double>(), c->d_hVc[mat].as<double>(), dx.as<double>(), n,
                                           dy.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
@@ -1886,20 +1905,21 @@ extern "C" int hm_hmatrix_build(hm_ctx* c, int n, const double* a, int lda, int 
       if (!sh || c->pshare_host[p] == p) { rr += rank[p]; ++cnt; }
     c->hm_rank_mean_run = cnt ? (double)rr / cnt : 0.0;
   }
-  const size_t lr = (size_t)nb * (nb - 1) * m * (ku ? ku : 1);
-  HIPCHK(c, c->d_hUc[0].ensure(cplx * lr), "hmatrix alloc");
-  HIPCHK(c, c->d_hVc[0].ensure(cplx * lr), "hmatrix alloc");
-  if (ku) HIPCHK(c, hm::hmat_compact(s, c->hmat, 0, ku, c->d_hUc[0].as<double>(), c->d_hVc[0].as<double>()), "hmatrix pack");
+  {
+    int rc1 = pack_lowrank(c, s, 0, rank);
+    if (rc1) return rc1;
+  }
+  const size_t lu = lowrank_u_elems(c, 0), lv = lowrank_v_elems(c, 0);
   HIPCHK(c, c->d_hD32.ensure(2 * sizeof(float) * (size_t)nb * sc), "hmatrix alloc");
   HIPCHK(c, c->d_P32.ensure(2 * sizeof(float) * (size_t)nb * sc), "hmatrix alloc");
   HIPCHK(c, c->d_pipiv.ensure(sizeof(int) * (size_t)n), "hmatrix alloc");
-  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lr), "hmatrix alloc");
-  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lr), "hmatrix alloc");
+  HIPCHK(c, c->d_hU32.ensure(2 * sizeof(float) * lu), "hmatrix alloc");
+  HIPCHK(c, c->d_hV32.ensure(2 * sizeof(float) * lv), "hmatrix alloc");
   const double* D = c->d_hD.as<double>();
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_hD32.as<float>(), m, sc, m, m, nb), "demote");
   HIPCHK(c, hm::demote(s, D, m, sD, c->d_P32.as<float>(), m, sc, m, m, nb), "demote");
-  HIPCHK(c, hm::demote(s, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lr / m), 1), "demote");
-  HIPCHK(c, hm::demote(s, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lr / m), 1), "demote");
+  HIPCHK(c, hm::demote(s, c->d_hUc[0].as<double>(), m, 0, c->d_hU32.as<float>(), m, 0, m, (int)(lu / m), 1), "demote");
+  HIPCHK(c, hm::demote(s, c->d_hVc[0].as<double>(), m, 0, c->d_hV32.as<float>(), m, 0, m, (int)(lv / m), 1), "demote");
   HIPCHK(c, hm::lu_factor<float>(s, c->d_P32.as<float>(), m, sc, m, c->d_pipiv.as<int>(), c->d_info.as<int>(),
                                  c->d_counter.as<unsigned>(), nb), "block preconditioner");
   std::vector<int> infos(nb, 0);
@@ -2106,9 +2126,10 @@ extern "C" int hm_ziter_solve(hm_ctx* c, int isolv, int n, const double* a, int 
       if (rc != HM_OK) return rc;
     }
   } else {
-    const int nb = c->hd_nb, m = c->hd_m, ku = c->hm_kuse[0];
+    const int nb = c->hd_nb, m = c->hd_m;
+    const hm::LowRank ku = c->hm_lr[0];
     const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
-    const size_t tsz = cplx * (size_t)nb * (nb - 1) * (ku ? ku : 1) * nrhs;
+    const size_t tsz = cplx * (size_t)nb * std::max(1, std::max(ku.ru, ku.rv)) * nrhs;
     HIPCHK(c, c->d_htb.ensure(tsz), "ziter alloc");
     HIPCHK(c, c->d_hts.ensure(tsz), "ziter alloc");
     const float* D32 = c->d_hD32.as<float>();

@@ -2,6 +2,7 @@
 // and the H-matrix products (see hm_hmatrix.hpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "hm_device.hpp"
@@ -486,24 +487,57 @@ __global__ void k_count_active(const int* conv, int n, int* out) {
   if (threadIdx.x == 0) *out = cnt;
 }
 
-// Tstack[b1][b2' kmax + r, c] = Tbuf[b2][b1' kmax + r, c]
+// Tstack[b1][t, c] = Tbuf[b2][row, c] for tmap[b1 ru + t] = b2 rv + row (0 for -1)
 template <typename T>
-__global__ void k_permute(int nb, int kmax, int ncol, const T* __restrict__ tb, T* __restrict__ ts) {
-  const long long rows = (long long)(nb - 1) * kmax;
+__global__ void k_permute_map(int nb, int ru, int rv, int ncol, const int* __restrict__ tmap,
+                              const T* __restrict__ tb, T* __restrict__ ts) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = (long long)nb * rows * ncol;
-  if (e >= total) return;
-  const int b1 = (int)(e / (rows * ncol));
-  const long long rem = e % (rows * ncol);
-  const int c = (int)(rem / rows);
-  const int rr = (int)(rem % rows);
-  const int b2p = rr / kmax, r = rr % kmax;
-  const int b2 = b2p + (b2p >= b1 ? 1 : 0);
-  const int b1p = b1 - (b1 > b2 ? 1 : 0);
-  const long long src = (long long)b2 * rows * ncol + (long long)c * rows + (long long)b1p * kmax + r;
-  const long long dst = (long long)b1 * rows * ncol + (long long)c * rows + rr;
-  ts[2 * dst] = tb[2 * src];
-  ts[2 * dst + 1] = tb[2 * src + 1];
+  const long long per = (long long)ru * ncol;
+  if (e >= (long long)nb * per) return;
+  const int b1 = (int)(e / per);
+  const long long rem = e - b1 * per;
+  const int c = (int)(rem / ru), t = (int)(rem - (long long)c * ru);
+  const int v = tmap[(long long)b1 * ru + t];
+  T re = T(0), im = T(0);
+  if (v >= 0) {
+    const int b2 = v / rv, row = v - b2 * rv;
+    const long long src = ((long long)b2 * ncol + c) * rv + row;
+    re = tb[2 * src];
+    im = tb[2 * src + 1];
+  }
+  ts[2 * e] = re;
+  ts[2 * e + 1] = im;
+}
+
+// block q's first rank columns (r m contiguous complex entries) to their place
+// in the packed factors: U blocks by block row (blockIdx.y = b1 (nb-1) + b2'),
+// V blocks by block column (blockIdx.y = b2 (nb-1) + b1')
+__global__ __launch_bounds__(256) void k_pack_lowrank(HmatDev h, int mat, const int* __restrict__ offu,
+                                                      const int* __restrict__ offv, int ru, int rv, double* Uc,
+                                                      double* Vc) {
+  const int nb = h.nb, per = nb * (nb - 1), q = blockIdx.y;
+  const long long blk = 2LL * h.m * h.kmax;
+  // U: block row b1, partner b2'
+  {
+    const int b1 = q / (nb - 1), b2p = q % (nb - 1);
+    const int p = mat * per + q;
+    const long long len = 2LL * h.rank[p] * h.m;
+    const double* src = h.U + ((long long)(mat * nb + b1) * (nb - 1) + b2p) * blk;
+    double* dst = Uc + 2LL * ((long long)b1 * h.m * ru + (long long)offu[q] * h.m);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long long)gridDim.x * blockDim.x)
+      dst[e] = src[e];
+  }
+  // V: block column b2, partner b1'
+  {
+    const int b2 = q / (nb - 1), b1p = q % (nb - 1);
+    const int b1 = b1p + (b1p >= b2 ? 1 : 0), b2p = b2 - (b2 > b1 ? 1 : 0);
+    const int p = mat * per + b1 * (nb - 1) + b2p;
+    const long long len = 2LL * h.rank[p] * h.m;
+    const double* src = h.V + ((long long)(mat * nb + b2) * (nb - 1) + b1p) * blk;
+    double* dst = Vc + 2LL * ((long long)b2 * h.m * rv + (long long)offv[q] * h.m);
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < len; e += (long long)gridDim.x * blockDim.x)
+      dst[e] = src[e];
+  }
 }
 
 }  // namespace
@@ -654,43 +688,91 @@ hipError_t aca_build(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
   return e;
 }
 
-hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, double* Uc, double* Vc) {
-  const size_t nblk = (size_t)h.nb * (h.nb - 1);
-  const size_t src_pitch = 16 * (size_t)h.m * h.kmax, dst_pitch = 16 * (size_t)h.m * kuse;
-  const double* U = h.U + 2 * (size_t)mat * nblk * h.m * h.kmax;
-  const double* V = h.V + 2 * (size_t)mat * nblk * h.m * h.kmax;
-  hipError_t e = hipMemcpy2DAsync(Uc, dst_pitch, U, src_pitch, dst_pitch, nblk, hipMemcpyDeviceToDevice, s);
-  if (e != hipSuccess) return e;
-  return hipMemcpy2DAsync(Vc, dst_pitch, V, src_pitch, dst_pitch, nblk, hipMemcpyDeviceToDevice, s);
+void lowrank_plan(int nb, const int* rank, LowRankPlan& plan) {
+  const int nq = nb * (nb - 1);
+  plan.offu.assign(nq, 0);
+  plan.offv.assign(nq, 0);
+  plan.ru = plan.rv = 0;
+  std::vector<int> rowsum(nb, 0), colsum(nb, 0);
+  for (int b1 = 0; b1 < nb; ++b1)
+    for (int b2p = 0; b2p < nb - 1; ++b2p) {
+      const int q = b1 * (nb - 1) + b2p;
+      plan.offu[q] = rowsum[b1];
+      rowsum[b1] += rank[q];
+    }
+  for (int b2 = 0; b2 < nb; ++b2)
+    for (int b1p = 0; b1p < nb - 1; ++b1p) {
+      const int b1 = b1p + (b1p >= b2 ? 1 : 0), b2p = b2 - (b2 > b1 ? 1 : 0);
+      const int q = b2 * (nb - 1) + b1p;
+      plan.offv[q] = colsum[b2];
+      colsum[b2] += rank[b1 * (nb - 1) + b2p];
+    }
+  for (int b = 0; b < nb; ++b) {
+    plan.ru = std::max(plan.ru, rowsum[b]);
+    plan.rv = std::max(plan.rv, colsum[b]);
+  }
+  plan.tmap.assign((size_t)nb * std::max(plan.ru, 1), -1);
+  for (int b1 = 0; b1 < nb; ++b1)
+    for (int b2p = 0; b2p < nb - 1; ++b2p) {
+      const int b2 = b2p + (b2p >= b1 ? 1 : 0), b1p = b1 - (b1 > b2 ? 1 : 0);
+      const int q = b1 * (nb - 1) + b2p, r = rank[q];
+      const int vrow = plan.offv[b2 * (nb - 1) + b1p];
+      for (int k = 0; k < r; ++k) plan.tmap[(size_t)b1 * plan.ru + plan.offu[q] + k] = b2 * plan.rv + vrow + k;
+    }
+}
+
+hipError_t hmat_pack(hipStream_t s, const HmatDev& h, int mat, const LowRankPlan& plan, int* meta, double* Uc,
+                     double* Vc, LowRank* out) {
+  const int nb = h.nb, nq = nb * (nb - 1);
+  out->ru = plan.ru;
+  out->rv = plan.rv;
+  out->tmap = meta + 2 * nq;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(meta, plan.offu.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(meta + nq, plan.offv.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s)) != hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(meta + 2 * nq, plan.tmap.data(), sizeof(int) * plan.tmap.size(), hipMemcpyHostToDevice, s)) !=
+      hipSuccess)
+    return e;
+  if (plan.ru > 0 && plan.rv > 0) {
+    if ((e = hipMemsetAsync(Uc, 0, 2 * sizeof(double) * (size_t)nb * h.m * plan.ru, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(Vc, 0, 2 * sizeof(double) * (size_t)nb * h.m * plan.rv, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pack_lowrank, dim3(16, nq), dim3(256), 0, s, h, mat, meta, meta + nq, plan.ru, plan.rv, Uc,
+                       Vc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // the plan's host vectors may change after the return
+  return hipStreamSynchronize(s);
 }
 
 template <typename T>
-hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long ldd, long long sD, const T* U,
-                      const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack) {
-  const long long rows = (long long)(nb - 1) * k;
+hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* D, long long ldd, long long sD,
+                      const T* U, const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf,
+                      T* tstack) {
   hipError_t e;
   // diagonal blocks (D null: Y holds the caller's values and only the
   // off-diagonal part is added)
   e = D ? cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
         : hipSuccess;
-  if (e != hipSuccess || k == 0) return e;
-  // T_b2 = [V_{b1,b2}]_b1^T x_b2 for every column block b2
-  e = cgemm<T>(s, true, (int)rows, ncol, m, T(1), T(0), V, m, rows * m, X, (int)ldx, m, T(0), T(0), tbuf,
-               (int)rows, rows * ncol, nb);
+  if (e != hipSuccess || lr.ru == 0 || lr.rv == 0) return e;
+  // T_b2 = V_b2^T x_b2 for every block column b2
+  e = cgemm<T>(s, true, lr.rv, ncol, m, T(1), T(0), V, m, (long long)lr.rv * m, X, (int)ldx, m, T(0), T(0), tbuf,
+               lr.rv, (long long)lr.rv * ncol, nb);
   if (e != hipSuccess) return e;
-  const long long tot = (long long)nb * rows * ncol;
-  hipLaunchKernelGGL((k_permute<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nb, k, ncol, tbuf,
-                     tstack);
+  const long long tot = (long long)nb * lr.ru * ncol;
+  hipLaunchKernelGGL((k_permute_map<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nb, lr.ru, lr.rv, ncol,
+                     lr.tmap, tbuf, tstack);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // y_b1 += [U_{b1,b2}]_b2 [T_{b1,b2}]_b2
-  return cgemm<T>(s, false, m, ncol, (int)rows, T(1), T(0), U, m, rows * m, tstack, (int)rows, rows * ncol, T(1),
-                  T(0), Y, (int)ldy, m, nb);
+  // y_b1 += U_b1 [T_{b1,b2}]_b2
+  return cgemm<T>(s, false, m, ncol, lr.ru, T(1), T(0), U, m, (long long)lr.ru * m, tstack, lr.ru,
+                  (long long)lr.ru * ncol, T(1), T(0), Y, (int)ldy, m, nb);
 }
 
-template hipError_t hmat_apply<double>(hipStream_t, int, int, int, const double*, long long, long long, const double*,
-                                       const double*, const double*, long long, double*, long long, int, double*,
-                                       double*);
-template hipError_t hmat_apply<float>(hipStream_t, int, int, int, const float*, long long, long long, const float*,
-                                      const float*, const float*, long long, float*, long long, int, float*, float*);
+template hipError_t hmat_apply<double>(hipStream_t, int, int, const LowRank&, const double*, long long, long long,
+                                       const double*, const double*, const double*, long long, double*, long long, int,
+                                       double*, double*);
+template hipError_t hmat_apply<float>(hipStream_t, int, int, const LowRank&, const float*, long long, long long,
+                                      const float*, const float*, const float*, long long, float*, long long, int,
+                                      float*, float*);
 
 }  // namespace hm

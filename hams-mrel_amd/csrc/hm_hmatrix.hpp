@@ -23,9 +23,10 @@
 //   y_b  = D_b x_b                                   (batched GEMM, diagonal blocks)
 //   T_b2 = [V_{b1,b2}]_b1^T x_b2                      (batched GEMM per column block)
 //   y_b1 += [U_{b1,b2}]_b2 [T_{b1,b2}]_b2               (batched GEMM per row block)
-// in FP32 on MFMA for the GMRES Krylov products and in FP64 for the residuals
+// with each block's factors at its own rank (hmat_pack), in FP32 on MFMA for the GMRES Krylov products and in FP64 for the residuals
 // and the right-hand sides.
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 
 #include "hm_assembly.hpp"
@@ -95,16 +96,39 @@ hipError_t aca_begin(hipStream_t s, const GeomDev& g, const FreqDev& f, const Hm
 hipError_t aca_end(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int* steps, int* unconverged,
                    const AcaTimer* tm = nullptr);
 
-// the first kuse columns of the U and V factors of matrix `mat` (0: A, 1: S),
-// packed with stride m kuse per block (same block order) into Uc, Vc
-hipError_t hmat_compact(hipStream_t s, const HmatDev& h, int mat, int kuse, double* Uc, double* Vc);
+// One matrix's low-rank factors packed at their ranks: block row b1's U
+// factors side by side, [U_{b1,b2}(:, 1:r_{b1,b2})]_{b2} (m x ru, ru the
+// largest such column count, zero padded), and block column b2's V factors
+// likewise (m x rv). The products then run over the ranks the blocks have
+// instead of the largest rank of all.
+struct LowRank {
+  int ru = 0, rv = 0;
+  // [b1][t < ru]: b2 rv + row of the V products (T_b2 = V_b2^T x_b2) that row
+  // t of block row b1's stack takes, or -1 (padding)
+  const int* tmap = nullptr;
+};
+// host part: column offsets of each block in its block row (offu, by b1 (nb-1)
+// + b2') and block column (offv, by b2 (nb-1) + b1'), ru, rv and the map
+struct LowRankPlan {
+  int ru = 0, rv = 0;
+  std::vector<int> offu, offv, tmap;
+};
+// rank[q]: the rank of block q = b1 (nb-1) + b2' of the matrix
+void lowrank_plan(int nb, const int* rank, LowRankPlan& plan);
+// ints of device metadata hmat_pack needs
+inline size_t lowrank_meta_ints(int nb, int ru) { return 2 * (size_t)nb * (nb - 1) + (size_t)nb * (ru > 0 ? ru : 1); }
+// packs matrix `mat`'s factors by `plan` into Uc (nb m ru complex) and Vc (nb
+// m rv complex), with the plan's metadata uploaded to meta (synchronises s)
+hipError_t hmat_pack(hipStream_t s, const HmatDev& h, int mat, const LowRankPlan& plan, int* meta, double* Uc,
+                     double* Vc, LowRank* out);
 
 // Y = H X: diagonal blocks D (block b at D + b sD complex elements,
-// column-major, ld ldd) plus the low-rank blocks of one matrix, packed as by
-// hmat_compact with rank stride k. X, Y: n x ncol (n = nb m); tbuf, tstack:
-// nb (nb-1) k ncol complex each. D null: Y += the low-rank blocks' product only.
+// column-major, ld ldd) plus the low-rank blocks of one matrix, packed by
+// hmat_pack. X, Y: n x ncol (n = nb m); tbuf, tstack: nb max(ru, rv) ncol
+// complex each. D null: Y += the low-rank blocks' product only.
 template <typename T>
-hipError_t hmat_apply(hipStream_t s, int nb, int m, int k, const T* D, long long ldd, long long sD, const T* U,
-                      const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf, T* tstack);
+hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* D, long long ldd, long long sD,
+                      const T* U, const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf,
+                      T* tstack);
 
 }  // namespace hm
