@@ -1065,6 +1065,10 @@ static int pack_lowrank(hm_ctx* c, hipStream_t s, int mat, const std::vector<int
   return HM_OK;
 }
 
+// stride of the H-matrix form's diagonal blocks for hmat_apply: 0 when every
+// body shares one block (the copies are bitwise the representative's)
+static long long diag_stride(const hm_ctx* c, long long s) { return c->sharing() && c->n_ublk == 1 ? 0 : s; }
+
 // complex entries of the packed U and V factors of matrix mat
 static size_t lowrank_u_elems(const hm_ctx* c, int mat) { return (size_t)c->hmat.nb * c->hmat.m * std::max(c->hm_lr[mat].ru, 1); }
 static size_t lowrank_v_elems(const hm_ctx* c, int mat) { return (size_t)c->hmat.nb * c->hmat.m * std::max(c->hm_lr[mat].rv, 1); }
@@ -1222,7 +1226,7 @@ static int assemble_hmatrix(hm_ctx* c) {
   // right-hand sides RHS = S * W
   c->stage_begin(kRhs);
   HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
-  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[1], DS, m, (long long)m * (m + 1),
+  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[1], DS, m, diag_stride(c, (long long)m * (m + 1)),
                                    c->d_hUc[1].as<double>(), c->d_hVc[1].as<double>(), c->d_W.as<double>(), n,
                                    c->d_R.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
          "rhs hmatrix product");
@@ -1606,7 +1610,7 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
     if (c->hm_on) {
       const int nb = c->nbody, m = c->bm();
       const hm::LowRank k = c->hm_lr[0];
-      const long long sD = (long long)m * (m + 1), sc = (long long)m * m;
+      const long long sD = diag_stride(c, (long long)m * (m + 1)), sc = diag_stride(c, (long long)m * m);
       const float* D32 = c->d_hD32.as<float>();
       const float* U32 = c->d_hU32.as<float>();
       const float* V32 = c->d_hV32.as<float>();
@@ -1837,7 +1841,8 @@ extern "C" int hm_hmatrix_apply(hm_ctx* c, int mat, int ncol, const double* x, d
   if ((rc = c->hip(dx.ensure(bytes), "hmatrix apply")) == HM_OK && (rc = c->hip(dy.ensure(bytes), "hmatrix apply")) == HM_OK &&
       (rc = c->hip(hipMemcpyAsync(dx.p, x, bytes, hipMemcpyHostToDevice, c->stream), "hmatrix apply")) == HM_OK &&
       (rc = c->hip(hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[mat],
-                                          mat ? c->d_hDS.as<double>() : c->d_hD.as<double>(), m, (long long)m * (m + 1),
+                                          mat ? c->d_hDS.as<double>() : c->d_hD.as<double>(), m,
+                                          from_assembly ? diag_stride(c, (long long)m * (m + 1)) : (long long)m * (m + 1),
                                           c->d_hUc[mat].as<double>(), c->d_hVc[mat].as<double>(), dx.as<double>(), n,
                                           dy.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
                    "hmatrix apply")) == HM_OK &&

@@ -751,9 +751,15 @@ hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* 
                       T* tstack) {
   hipError_t e;
   // diagonal blocks (D null: Y holds the caller's values and only the
-  // off-diagonal part is added)
-  e = D ? cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
-        : hipSuccess;
+  // off-diagonal part is added). sD = 0, one block for all bodies: with X and Y
+  // of leading dimension nb m, block b's rows of column c are column b + c nb
+  // of X and Y read as m x (nb ncol) matrices of leading dimension m, so the
+  // product is one GEMM over all bodies' columns (the same sums per entry)
+  if (D && sD == 0 && ldx == (long long)nb * m && ldy == ldx)
+    e = cgemm<T>(s, false, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1);
+  else
+    e = D ? cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
+          : hipSuccess;
   if (e != hipSuccess || lr.ru == 0 || lr.rv == 0) return e;
   // T_b2 = V_b2^T x_b2 for every block column b2
   e = cgemm<T>(s, true, lr.rv, ncol, m, T(1), T(0), V, m, (long long)lr.rv * m, X, (int)ldx, m, T(0), T(0), tbuf,

@@ -123,7 +123,8 @@ hipError_t hmat_pack(hipStream_t s, const HmatDev& h, int mat, const LowRankPlan
                      double* Vc, LowRank* out);
 
 // Y = H X: diagonal blocks D (block b at D + b sD complex elements,
-// column-major, ld ldd) plus the low-rank blocks of one matrix, packed by
+// column-major, ld ldd; sD = 0: one block for every body) plus the low-rank
+// blocks of one matrix, packed by
 // hmat_pack. X, Y: n x ncol (n = nb m); tbuf, tstack: nb max(ru, rv) ncol
 // complex each. D null: Y += the low-rank blocks' product only.
 template <typename T>
