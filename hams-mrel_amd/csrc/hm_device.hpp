@@ -494,6 +494,22 @@ HM_DEV void fin_evanescent_direct_tab(double r, const double* wvn, const double*
   }
 }
 
+// the part of fin_evanescent_direct_tab over modes m0, m0 + step, ... < n
+HM_DEV void fin_evanescent_direct_tab_part(double r, const double* wvn, const double* evt, long long ld,
+                                           const double* pmt, int i, int j, int n, int m0, int step, double& s1,
+                                           double& s2, double& s3) {
+  s1 = s2 = s3 = 0.0;
+  for (int m = m0; m < n; m += step) {
+    const double* ej = evt + 2 * ((long long)m * ld + j);
+    const double ckzp = evt[2 * ((long long)m * ld + i) + 1];
+    double t1, t2, t3;
+    fin_evanescent_given<true>(wvn[m], pmt[m], r, ej[0], ej[1], ckzp, t1, t2, t3);
+    s1 += t1;
+    s2 += t2;
+    s3 += t3;
+  }
+}
+
 // the evanescent sums of modes 1 .. n-1 (n = fin_direct_terms), FAST K0/K1
 HM_DEV void fin_evanescent_direct(double r, double zfh, double zph, const double* wvn, double h, int n,
                                   double& s1, double& s2, double& s3) {

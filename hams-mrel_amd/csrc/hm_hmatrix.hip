@@ -229,6 +229,108 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   rb[2 * j + 1] = ei;
 }
 
+// Cross approximation with kSplit lanes per entry (bodies at least 0.05 h
+// apart in finite depth, depth-factor tables set): a workgroup takes
+// kSplitEntries entries, the evanescent sums of each split over its group of
+// lanes (pair_entry_split), so that a launch has kSplit times the lanes of the
+// one-lane-per-entry kernels and each lane a kSplit-th of the modes.
+constexpr int kSplit = 4, kSplitEntries = 256 / kSplit;
+
+// sorted_entry over the workgroup's kSplitEntries entries (key: near flag,
+// regime, number of directly summed modes); returns the entry index of slot e
+__device__ __forceinline__ int sorted_entry_split(const GeomDev& g, const FreqDev& f, const HmatDev& h, const Proc& q,
+                                                  bool row, int fixed, int c0, unsigned* sk, int e) {
+  const int t = threadIdx.x, m = h.m;
+  if (t < kSplitEntries) {
+    unsigned key = 0xfffu;                   // past the end: last
+    if (c0 + t < m) {
+      const int i = q.b1 * m + (row ? fixed : c0 + t), j = q.b2 * m + (row ? c0 + t : fixed);
+      double xp[3];
+      field_point(g, i, j, xp);
+      const double* xq = g.xyz_p + 3 * (long long)j;
+      const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+      const int reg = fin_regime(rr, f.h);
+      const int nt = reg == kEigene ? min(fin_direct_terms(rr, f.h), 255) : 0;
+      key = (unsigned)(near_flag(xp, xq, g.pnsz[j]) * 1024 + reg * 256 + nt);
+    }
+    sk[t] = (key << 8) | (unsigned)t;
+  }
+  __syncthreads();
+  for (int k = 2; k <= kSplitEntries; k <<= 1)
+    for (int d = k >> 1; d > 0; d >>= 1) {
+      const int o = t ^ d;
+      if (t < kSplitEntries && o > t) {
+        const unsigned a = sk[t], b = sk[o];
+        if ((a > b) == ((t & k) == 0)) {
+          sk[t] = b;
+          sk[o] = a;
+        }
+      }
+      __syncthreads();
+    }
+  return (int)(sk[e] & 255u);
+}
+
+// residual_entry with pair_entry_split; the rank sum split over the group alike
+__device__ __forceinline__ void residual_entry_split(const GeomDev& g, const FreqDev& f, const double* wvn,
+                                                     const Proc& q, int m, int k, int i, int j, int sub, double& er,
+                                                     double& ei) {
+  double ar, ai, sr, si;
+  pair_entry_split<kSplit>(g, f, wvn, q.b1 * m + i, q.b2 * m + j, sub, ar, ai, sr, si);
+  er = q.mat ? sr : ar;
+  ei = q.mat ? si : ai;
+  double pr = 0.0, pi = 0.0;
+  for (int l = sub; l < k; l += kSplit) {
+    const double ur = q.U[2 * (i + (long long)l * m)], ui = q.U[2 * (i + (long long)l * m) + 1];
+    const double vr = q.V[2 * (j + (long long)l * m)], vi = q.V[2 * (j + (long long)l * m) + 1];
+    pr += ur * vr - ui * vi;
+    pi += ur * vi + ui * vr;
+  }
+#pragma unroll
+  for (int o = 1; o < kSplit; o <<= 1) {
+    pr += __shfl_xor(pr, o);
+    pi += __shfl_xor(pi, o);
+  }
+  er -= pr;
+  ei -= pi;
+}
+
+// k_aca_eval_row / k_aca_eval_col with kSplit lanes per entry
+template <bool ROW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_aca_eval_split(GeomDev g,
+                                                                                            FreqDev f, HmatDev h) {
+  const int p = proc_of(h, blockIdx.y);
+  if (h.conv[p] || (!ROW && h.skip[p])) return;
+  __shared__ double wvn[kNk];
+  for (int k = threadIdx.x; k < kNk; k += blockDim.x) wvn[k] = f.wvn[k];
+  __syncthreads();
+  __shared__ unsigned sk[kSplitEntries];
+  const Proc q = proc(h, p);
+  const int e = threadIdx.x / kSplit, sub = threadIdx.x % kSplit;
+  const int c0 = blockIdx.x * kSplitEntries;
+  const int fixed = ROW ? h.next_row[p] : h.pivot[p];
+  const int x0 = c0 + sorted_entry_split(g, f, h, q, ROW, fixed, c0, sk, e);
+  const bool active = x0 < h.m;
+  const int x = active ? x0 : h.m - 1;
+  const int k = h.rank[p];
+  double er, ei;
+  residual_entry_split(g, f, wvn, q, h.m, k, ROW ? fixed : x, ROW ? x : fixed, sub, er, ei);
+  const bool lead = active && sub == 0;
+  if (aca_fused(h))
+    aca_partials(ROW ? q.V : q.U, x, h.m, k, lead ? er : 0.0, lead ? ei : 0.0,
+                 (ROW ? h.prow : h.pcol) + 2LL * ((long long)p * gridDim.x + blockIdx.x) * h.kmax);
+  if (!lead) return;
+  if (ROW) {
+    double* rb = h.rowbuf + 2LL * p * h.m;
+    rb[2 * x] = er;
+    rb[2 * x + 1] = ei;
+  } else {
+    double* u = q.U + 2LL * k * h.m;
+    u[2 * x] = er;
+    u[2 * x + 1] = ei;
+  }
+}
+
 // pivot column j* = argmax |r| over unused columns, v_k = r / r(j*); a zero
 // residual row is retired and the pivot row moves to the next unused row
 __global__ __launch_bounds__(256) void k_aca_pivot_row(HmatDev h) {
@@ -359,7 +461,7 @@ __global__ __launch_bounds__(kFinishThreads) void k_aca_finish_col(HmatDev h) {
   if (aca_fused(h)) {
     // from the partial sums of the row and column launches, in chunk order:
     // u_l^H u_k, and v_l^H v_k = (v_l^H r) / r(j*)
-    const int nch = (m + 255) / 256;
+    const int nch = (m + h.ech - 1) / h.ech;
     const double* rb = h.rowbuf + 2LL * p * m;
     const int js = h.pivot[p];
     const double pr = rb[2 * js], pi = rb[2 * js + 1], den = pr * pr + pi * pi;
@@ -551,7 +653,7 @@ size_t hmat_bytes(int nb, int m, int kmax, int nmat) {
   b += 4 * 5 * P;                  // rank, conv, next_row, pivot, skip
   b += 4 * 2 * P * m;              // row_used, col_used
   b += 4;                          // counter
-  b += 2 * 16 * P * (size_t)((m + 255) / 256) * kmax;   // prow, pcol
+  b += 2 * 16 * P * (size_t)((m + 63) / 64) * kmax;   // prow, pcol
   return b + 15 * 256;             // alignment of the 15 arrays
 }
 
@@ -581,9 +683,10 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
   h.row_used = (int*)take(4 * P * m);
   h.col_used = (int*)take(4 * P * m);
   h.counter = (int*)take(4);
-  const size_t part = 16 * P * (size_t)((m + 255) / 256) * kmax;
+  const size_t part = 16 * P * (size_t)((m + 63) / 64) * kmax;
   h.prow = (double*)take(part);
   h.pcol = (double*)take(part);
+  h.ech = 256;
   h.narrow = 0;
   h.share = nullptr;
   h.run = nullptr;
@@ -595,11 +698,28 @@ HmatDev hmat_carve(void* base, int nb, int m, int kmax, double eps, double eps_s
 
 // one step of every open process: residual pivot row, pivot column, residual
 // pivot column, norm estimate and next pivot row
-static hipError_t aca_step(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h, int P,
+static hipError_t aca_step(hipStream_t s, const GeomDev& g, const FreqDev& f, const HmatDev& h0, int P,
                            const AcaTimer* tm) {
   auto mark = [tm](int b) { if (tm) tm->mark(tm->user, b); };
   // the processes that run: all, or the list h.run of h.nrun
-  if (h.run) P = h.nrun;
+  if (h0.run) P = h0.nrun;
+  // bodies at least 0.05 h apart in finite depth with the depth-factor
+  // tables: kSplit lanes per entry
+  const bool split = h0.narrow && !h0.dense && f.h > 0.0 && f.evt != nullptr;
+  HmatDev h = h0;
+  h.ech = split ? kSplitEntries : 256;
+  if (split) {
+    const dim3 ev((h.m + kSplitEntries - 1) / kSplitEntries, P);
+    mark(1);
+    hipLaunchKernelGGL(k_aca_eval_split<true>, ev, dim3(256), 0, s, g, f, h);
+    mark(0);
+    hipLaunchKernelGGL(k_aca_pivot_row, dim3(P), dim3(256), 0, s, h);
+    mark(1);
+    hipLaunchKernelGGL(k_aca_eval_split<false>, ev, dim3(256), 0, s, g, f, h);
+    mark(0);
+    hipLaunchKernelGGL(k_aca_finish_col, dim3(P), dim3(kFinishThreads), 0, s, h);
+    return hipGetLastError();
+  }
   const dim3 eval((h.m + 255) / 256, P);
   mark(1);
   if (h.narrow) hipLaunchKernelGGL((k_aca_eval_row<8, false, true>), eval, dim3(256), 0, s, g, f, h);

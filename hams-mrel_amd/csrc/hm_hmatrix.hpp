@@ -46,9 +46,11 @@ struct HmatDev {
   int *rank, *conv, *next_row, *pivot, *skip;   // [process]
   int *row_used, *col_used;                     // [process][m]
   int* counter;                                 // one int for host polls
-  // [process][chunk of 256][kmax] complex: the row and column launches' partial
-  // inner products with the earlier factor columns (the norm estimate's cross terms)
+  // [process][chunk of ech entries][kmax] complex: the row and column launches'
+  // partial inner products with the earlier factor columns (the norm estimate's
+  // cross terms); room for chunks of 64 entries
   double *prow, *pcol;
+  int ech;           // entries per chunk (workgroup) of the row and column launches
   // share[p] (nullptr: none): the process whose block equals process p's bit
   // for bit (equal bodies at an equal offset); p itself for the processes
   // that run. The others are skipped (conv = 3) and copied at the end.

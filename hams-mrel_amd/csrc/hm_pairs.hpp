@@ -147,4 +147,55 @@ __device__ __forceinline__ void pair_entry_coop(const GeomDev& g, const FreqDev&
   pair_values(g, i, j, xp, flag, gr, ar, ai, sr, si);
 }
 
+// pair_entry for the cross approximation between bodies at least 0.05 h apart
+// in finite depth (no PIDCOCK or LINTON entries; the depth-factor tables
+// f.evt, f.pmt set), with S consecutive lanes per entry: lane `sub` of the
+// entry's group sums the evanescent modes 1 + sub, 1 + sub + S, ... and the
+// group combines the partial sums by a butterfly (every lane gets the same
+// value); the rest of the entry is formed by every lane of the group alike. All
+// S lanes of a group call it with the same (i, j).
+template <int S>
+__device__ __forceinline__ void pair_entry_split(const GeomDev& g, const FreqDev& f, const double* wvn, int i, int j,
+                                                 int sub, double& ar, double& ai, double& sr, double& si) {
+  double xp[3];
+  field_point(g, i, j, xp);
+  const double* xq = g.xyz_p + 3 * (long long)j;
+  const double h = f.h;
+  const double rr = horiz_dist(xq[0], xp[0], xq[1], xp[1]);
+  const double zf = xq[2], zp = xp[2];
+  const bool zero = fabs(f.v) < F32(1.e-8), inf = !zero && fabs(f.v + 1.0) < F32(1.e-8);
+  const int flag = near_flag(xp, xq, g.pnsz[j]);
+  const double xpq = xq[0] - xp[0], ypq = xq[1] - xp[1];
+  Grn4 gr;
+  if (inf) {
+    fin_limit_inf(xpq, ypq, zf, zp, rr, flag, gr);
+  } else {
+    FinOut fo{0, 0, 0, 0, 0, 0};
+    if (!zero) {
+      const int reg = fin_regime(rr, h);
+      if (reg == kEigene) {
+        fin_propagating(rr, zf + h, zp + h, wvn[0], h, fo);
+        double L1, L2, L3;
+        fin_evanescent_direct_tab_part(rr, wvn, f.evt, f.evt_ld, f.pmt, i, j, fin_direct_terms(rr, h), 1 + sub, S,
+                                       L1, L2, L3);
+#pragma unroll
+        for (int o = 1; o < S; o <<= 1) {
+          L1 += __shfl_xor(L1, o);
+          L2 += __shfl_xor(L2, o);
+          L3 += __shfl_xor(L3, o);
+        }
+        fo.g_re += L1;
+        fo.gr_re += L2;
+        fo.gz_re += L3;
+        fin_strip(rr, zf, zp, fo);
+      } else {
+        fin_eigen(rr, zf, zp, wvn, h, fo);
+      }
+      if (rr == 0.0) { fo.gr_re = 0.0; fo.gr_im = 0.0; }
+    }
+    fin_to_grn(fo, xpq, ypq, zf, zp, rr, !zero, flag, gr);
+  }
+  pair_values(g, i, j, xp, flag, gr, ar, ai, sr, si);
+}
+
 }  // namespace hm
