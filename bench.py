@@ -59,10 +59,10 @@ ACCURACY_BAR = 1e-4
 # it runs at 5 waves per SIMD, and register-spill traffic (scratch that misses
 # the L2) is most of the rest.
 MEASURED_TRAFFIC = {
-    ("c5", "hmatrix", "asm_pidcock"): 2643203.875 * 1024 * 2 + 2692394.9375 * 1024,
-    ("c5", "hmatrix", "asm_eigene"): 1120176.6875 * 1024 * 2 + 2771524.1875 * 1024,
-    # per launch, mean of the row and column launches
-    ("c5", "hmatrix", "aca_entries"): (712232.4375 + 677136.9375) / 108 * 1024 * 2 + (85371.4375 + 85220.0625) / 108 * 1024,
+    ("c5", "hmatrix", "asm_pidcock"): 2685335.9375 * 1024 * 2 + 2737365.875 * 1024,
+    ("c5", "hmatrix", "asm_eigene"): 1121673.8125 * 1024 * 2 + 2776009.15625 * 1024,
+    # per launch, mean of the row and column launches (54 of each)
+    ("c5", "hmatrix", "aca_entries"): (252819.0 + 179202.3125) / 108 * 1024 * 2 + (23670.625 + 31830.25) / 108 * 1024,
 }
 
 def flop_model(name):
@@ -116,11 +116,13 @@ PATH_DESC = {
 }
 
 
-def run_steps(ctx, omegas, k0, steps, precision, ranks=None):
+def run_steps(ctx, omegas, k0, steps, precision, ranks=None, progress=None):
     out = []
     for s in range(steps):
         w = float(omegas[(k0 + s) % len(omegas)])
         out.append(ctx.solve_frequency(w, precision))
+        if progress:
+            print(f"# {progress}: system {s + 1}/{steps} (omega {w:.4f})", file=sys.stderr, flush=True)
         if ranks is not None:
             # mean rank over all cross-approximated blocks, and over the processes
             # that ran (the others copy an equal block's factors)
@@ -128,12 +130,12 @@ def run_steps(ctx, omegas, k0, steps, precision, ranks=None):
     return out
 
 
-def time_path(ctx, prob, precision, steps, warmup, stage_times=True, ranks=None):
-    run_steps(ctx, prob.omegas, 0, warmup, precision)
+def time_path(ctx, prob, precision, steps, warmup, stage_times=True, ranks=None, progress=None):
+    run_steps(ctx, prob.omegas, 0, warmup, precision, progress=progress and progress + " warm-up")
     sync()
     ctx.set_timing(stage_times)
     t0 = time.perf_counter()
-    res = run_steps(ctx, prob.omegas, warmup, steps, precision, ranks)
+    res = run_steps(ctx, prob.omegas, warmup, steps, precision, ranks, progress=progress)
     sync()
     dt = time.perf_counter() - t0
     stages = ctx.timing()
@@ -549,7 +551,8 @@ def main():
         others["fp32_lu_ir"] = {"value": args.steps / dt_ir, "ms_per_step": 1e3 * dt_ir / args.steps}
     plain = None
     if not args.no_plain:
-        dt_pl, res_pl, st_pl = time_path(ctx, prob, hm_bem.HM_FP64, args.steps, args.warmup)
+        # one progress line per system (5 s each at C5)
+        dt_pl, res_pl, st_pl = time_path(ctx, prob, hm_bem.HM_FP64, args.steps, args.warmup, progress="fp64")
         print(f"# fp64: {args.steps} systems in {dt_pl:.3f} s; stages {st_pl}", file=sys.stderr, flush=True)
         nw, ew, ew3, inbar, cnt = {}, {}, {}, {}, {}
         per_freq = []
@@ -651,7 +654,8 @@ def main():
                               "timed frequency; the line's value is null when it exceeds the bar"),
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt_mx / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f64+f32+bf16" if sur is not None else "f64+f32",
+            "dtype": ("f64+f32+f16+bf16" if sur is not None else "f64+f32+f16") if args.path == "hmatrix" else
+                     ("f64+f32+bf16" if sur is not None else "f64+f32"),
             "data": "synthetic (deterministic meshes of the named workload)",
             "config": {"workload": prob.name, "panels": prob.n, "waterplane_panels": prob.ni,
                        "modes": prob.m, "headings": prob.nbeta, "frequencies": len(prob.omegas),
