@@ -1653,6 +1653,9 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
           ga.Pinv16 = c->d_Pinv16.as<_Float16>();
           ga.pinv16_shared = c->pinv16_shared;
         }
+        // the H-matrix form's one shared block: its inverse copied to every body
+        ga.pinv_shared = c->hm_on && c->gm_blocks == c->body_start && c->sharing() && c->n_ublk == 1 &&
+                         c->nbody > 1;
       }
     }
     HIPCHK(c, c->d_gwork.ensure(hm::gmres_workspace_bytes(n, ncol, c->gm_restart, ga.Pinv16 ? ga.nblk : 0)),

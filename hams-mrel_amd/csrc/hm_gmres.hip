@@ -407,6 +407,13 @@ static hipError_t apply_inverse(hipStream_t s, const GmresArgs& a, const double*
   hipLaunchKernelGGL(k_demote_perm, dim3((a.n + 255) / 256, ncol), dim3(256), 0, s, v, (long long)a.n, a.pperm, a.n, W);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (a.pinv_shared) {
+    // one inverse for all blocks (gmres_solve checks the layout): W and Z
+    // (n x ncol, ld n = nblk nb) read as nb x (nblk ncol) matrices of leading
+    // dimension nb put block b's rows of column c in column b + c nblk
+    const int nb = a.blk_start[1];
+    return cgemm<float>(s, false, nb, a.nblk * ncol, nb, 1.f, 0.f, a.Pinv, nb, 0, W, nb, 0, 0.f, 0.f, Z, nb, 0, 1);
+  }
   return for_block_runs(a.nblk, a.blk_start, [&](int r0, int nb, int cnt, size_t off) {
     return cgemm<float>(s, false, nb, ncol, nb, 1.f, 0.f, a.Pinv + 2 * off, nb, (long long)nb * nb,
                         W + 2 * (long long)r0, a.n, nb, 0.f, 0.f, Z + 2 * (long long)r0, a.n, nb, cnt);
@@ -438,10 +445,11 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
   float* Z32 = (float*)take(2 * sizeof(float) * (size_t)n * ncol);
   float* W32 = (float*)take(2 * sizeof(float) * (size_t)n * ncol);
   _Float16* W16 = (_Float16*)take(2 * sizeof(_Float16) * (size_t)(n + 31 * (a.Pinv16 ? a.nblk : 0)) * ncol);
-  if (a.Pinv16) {
+  if (a.Pinv16 || a.pinv_shared) {
     for (int b = 1; b < a.nblk; ++b)
       if (a.blk_start[b + 1] - a.blk_start[b] != a.blk_start[1] - a.blk_start[0]) return hipErrorInvalidValue;
   }
+  if (a.pinv_shared && (a.blk_start[0] != 0 || a.blk_start[a.nblk] != n)) return hipErrorInvalidValue;
   const long long vstride = (long long)ncol * n;
   const dim3 gcol((n + 255) / 256, ncol);
   hipError_t e;

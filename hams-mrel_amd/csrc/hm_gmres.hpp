@@ -37,6 +37,9 @@ struct GmresArgs {
   // be sized with nblk16 = nblk.
   const _Float16* Pinv16 = nullptr;
   bool pinv16_shared = false;
+  // every block's inverse in Pinv is the same (blocks of one size covering
+  // 0..n): the FP32 application is one GEMM over all blocks' columns
+  bool pinv_shared = false;
   // operator form (used instead of A32 / A64 when set): op32 writes W = A Z in
   // FP32, op64 writes W = A X in FP64, both n x ncol with leading dimension n
   std::function<hipError_t(hipStream_t, const float*, float*)> op32;
