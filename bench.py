@@ -385,7 +385,7 @@ def main():
                          "profilers that do not handle cooperative launches)")
     ap.add_argument("--workers", type=int, default=None,
                     help="frequencies in flight on the measured path (hm_set_workers: clones of the context, one "
-                         "host thread each); default 3 on the H-matrix path, 1 otherwise")
+                         "host thread each); default 4 on the H-matrix path, 1 otherwise")
     ap.add_argument("--no-block-sharing", action="store_true",
                     help="H-matrix path: assemble, factor and invert every body's diagonal block, also where bodies "
                          "are equal (the blocks are the same bit for bit either way)")
@@ -471,7 +471,7 @@ def main():
               file=sys.stderr, flush=True)
     ranks_mx = []
     if args.workers is None:
-        args.workers = 3 if args.path == "hmatrix" else 1
+        args.workers = 4 if args.path == "hmatrix" else 1
     if args.workers > 1:
         dt_mx, res_mx, st_mx = time_path_inflight(ctx, prob, prec, args.steps, args.warmup, args.workers)
     else:
