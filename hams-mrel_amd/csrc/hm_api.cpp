@@ -161,6 +161,7 @@ struct hm_ctx {
   DevBuf d_Pinv16;            // FP16 copy of d_Pinv (H-matrix form, pinv16_on)
   bool pinv16_on = true;      // the H-matrix form's preconditioner applied by FP16 MFMA GEMMs
   bool pinv16_ready = false;
+  double gm_cycle_rtol = 1e-4;  // with the FP16 preconditioner: the Arnoldi target of a cycle (GmresArgs)
   bool pinv16_shared = false; // d_Pinv16 holds one block, shared by every body
   DevBuf d_pperm;             // the factored blocks' pivot permutations (with d_Pinv)
   bool pinv_ready = false;    // d_Pinv and d_pperm belong to the current d_P32 factors
@@ -1652,6 +1653,7 @@ extern "C" int hm_solve(hm_ctx* c, int* iters) {
         if (c->hm_on && c->pinv16_on && c->pinv16_ready) {
           ga.Pinv16 = c->d_Pinv16.as<_Float16>();
           ga.pinv16_shared = c->pinv16_shared;
+          ga.cycle_rtol = c->gm_cycle_rtol;
         }
         // the H-matrix form's one shared block: its inverse copied to every body
         ga.pinv_shared = c->hm_on && c->gm_blocks == c->body_start && c->sharing() && c->n_ublk == 1 &&

@@ -476,6 +476,9 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
     if (worst <= a.tol) { done = true; break; }
     if (total >= a.maxiter || cycles >= a.max_cycles) break;
     ++cycles;
+    // this cycle's Arnoldi target: inner_tol, or a reduction by cycle_rtol of the
+    // largest relative residual when the inner operator cannot get further
+    const double target = fmax(a.inner_tol, a.cycle_rtol * worst);
     hipLaunchKernelGGL(k_start, gcol, dim3(256), 0, s, R, (long long)n, n, ncol, rnorm, bnorm, a.inner_tol, m1, V, g,
                        active, steps);
     GM_CHECK(hipGetLastError());
@@ -507,7 +510,7 @@ hipError_t gmres_solve(hipStream_t s, const GmresArgs& a, void* work, GmresStats
       hipLaunchKernelGGL(k_colnorm, dim3(ncol), dim3(256), 0, s, W, (long long)n, n, wnorm);
       hipLaunchKernelGGL(k_normalize, gcol, dim3(256), 0, s, W, wnorm, n, active, V + 2 * vstride * (j + 1));
       hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, s, j, ncol, m1, H, cs, sn, g, hc1, hc2, wnorm, bnorm,
-                         a.inner_tol, active, steps, est, nactive);
+                         target, active, steps, est, nactive);
       GM_CHECK(hipGetLastError());
       int na = 0;
       GM_CHECK(hipMemcpyAsync(&na, nactive, sizeof(int), hipMemcpyDeviceToHost, s));

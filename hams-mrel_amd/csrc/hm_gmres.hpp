@@ -14,6 +14,10 @@ struct GmresArgs {
   int restart = 50, maxiter = 500, max_cycles = 50;
   double tol = 1e-6;        // on the FP64 residual: ||B - A X|| <= tol ||B|| per column
   double inner_tol = 5e-7;  // on the Arnoldi residual estimate of a cycle
+  // > 0: a cycle's Arnoldi steps stop once the estimate is below cycle_rtol
+  // times the largest relative residual at the cycle's start (if that is above
+  // inner_tol): for an inner operator whose rounding limits what a cycle reaches
+  double cycle_rtol = 0.0;
   const double* A64 = nullptr;  // FP64 system matrix (n x n, leading dimension lda64)
   int lda64 = 0;
   const float* A32 = nullptr;   // FP32 copy, leading dimension n
