@@ -29,11 +29,13 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "hm_assembly.hpp"
 #include "hm_bem.h"
+#include "hm_gmres.hpp"
 #include "hm_linalg.hpp"
 
 namespace {
@@ -92,6 +94,7 @@ struct hm_batch {
   double* d_ratio = nullptr;
   std::vector<size_t> off_amas, off_bdmp, off_exfc;
   std::vector<hm::BatchProb> h_prob;       // host copy of the member descriptors
+  std::vector<const float*> h_a32;         // the members' FP32 copies of A (HM_GMRES)
   double *d_amas = nullptr, *d_bdmp = nullptr, *d_exfc = nullptr;
   size_t out_amas = 0, out_exfc = 0;
   int nmax = 0, ncolmax = 0, mmax = 0;
@@ -100,6 +103,12 @@ struct hm_batch {
   hipGraphExec_t exec = nullptr;
   double last_ratio = 0.0;
   int extra_sweeps = 0;
+  // HM_GMRES members: restarted GMRES per member after the captured assembly,
+  // FP32 copies of A for the Krylov products, FP64 residuals, no preconditioner
+  int gm_restart = 50, gm_maxiter = 500;
+  double gm_tol = 1e-6;
+  void* d_gws = nullptr;              // one workspace, sized for the largest member
+  std::vector<hm::GmresStats> gm_stats;
 
   int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -123,6 +132,8 @@ struct hm_batch {
     graph = nullptr;
     if (pool) (void)hipFree(pool);
     pool = nullptr;
+    if (d_gws) (void)hipFree(d_gws);
+    d_gws = nullptr;
     if (h_freq) (void)hipHostFree(h_freq);
     h_freq = nullptr;
     prepared = false;
@@ -250,6 +261,12 @@ int issue_chain(hm_batch* b) {
   }
   // factor and solve
   BCHK(b, hm::fill_i32(s, b->d_info, 2LL * cnt, 0), "batch factor");
+  if (b->precision == HM_GMRES) {
+    // the FP32 copies for the Krylov products; the solves and the forces follow
+    // outside the chain (GMRES reads its residual norms on the host)
+    BCHK(b, hm::convert_grouped(s, b->d_convA, cnt, b->max_nn, true, false), "batch demote");
+    return HM_OK;
+  }
   if (b->precision == HM_FP64) {
     BCHK(b, hm::lu_factor_grouped<double>(s, b->d_lu64, cnt, b->nmax, b->gemm_kb), "batch lu");
     BCHK(b, hm::lu_solve_grouped<double>(s, b->d_lu64, cnt, b->nmax, b->ncolmax), "batch solve");
@@ -392,7 +409,13 @@ extern "C" int hm_batch_tuning(hm_batch* b, int* series_variant, int* gemm_kchun
 extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int use_graph, int nrefine) {
   if (!b) return HM_ERR_ARG - 1;
   if (b->count() == 0) return b->fail(HM_ERR_STATE, "hm_batch_prepare with no problems");
-  if (precision != HM_FP64 && precision != HM_FP32_IR) return b->fail(HM_ERR_ARG - 2, "precision must be HM_FP64 or HM_FP32_IR");
+  if (precision != HM_FP64 && precision != HM_FP32_IR && precision != HM_GMRES)
+    return b->fail(HM_ERR_ARG - 2, "precision must be HM_FP64, HM_FP32_IR or HM_GMRES");
+  if (precision == HM_GMRES)
+    for (int p = 0; p < b->count(); ++p)
+      if (b->mem[p].ni)
+        return b->fail(HM_ERR_ARG - 2, "member %d has waterplane points: HM_GMRES members solve the collocation "
+                                       "system only (use HM_FP64 or HM_FP32_IR)", p);
   if (nrefine < 1 || nrefine > 20) return b->fail(HM_ERR_ARG - 5, "nrefine must be in 1..20");
   b->release();
   b->precision = precision;
@@ -400,6 +423,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   b->nrefine = nrefine;
   const int cnt = b->count();
   const bool ir = precision == HM_FP32_IR;
+  const bool gm = precision == HM_GMRES;
   // ---- layout of the pool
   struct Off { size_t body, fp, verts, ncn, ds, pnsz, dxyz, beta, A, S, W, R, Rw, A32, X32, C, D, ipiv, amas, bdmp, exfc; };
   std::vector<Off> off(cnt);
@@ -419,7 +443,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     // A, S: nt x n; W, R, Rw: nt x ncol (leading dimension nt for all of them)
     o.A = take(16 * nt * n); o.S = take(16 * nt * n); o.W = take(16 * nt * nc); o.R = take(16 * nt * nc);
     o.Rw = ir ? take(16 * nt * nc) : 0;
-    o.A32 = ir ? take(8 * n * n) : 0;
+    o.A32 = ir || gm ? take(8 * n * n) : 0;
     o.X32 = ir ? take(8 * n * nc) : 0;
     // waterplane members: C = A^T A (n x n), D = A^T R (n x ncol, leading dimension nt)
     o.C = m.ni ? take(16 * n * n) : 0;
@@ -486,6 +510,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   std::vector<int> deep;
   std::vector<long long> prefix(1, 0);
   b->off_amas.assign(cnt, 0);
+  b->h_a32.assign(cnt, nullptr);
   b->off_bdmp.assign(cnt, 0);
   b->off_exfc.assign(cnt, 0);
   b->out_amas = b->out_exfc = 0;
@@ -536,6 +561,13 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     bp.X = sysB;   // FP64: the solve overwrites the right-hand sides (IR: set below)
     int* info = b->d_info + 2 * p;
     lu64[p] = {sysA, lds, m.n, (int*)at(o.ipiv), info, (unsigned*)(info + 1), sysB, (long long)nt, m.ncol};
+    if (gm) {
+      // GMRES: the solution in W's slot (dead once R = S W is formed), the
+      // right-hand sides kept in R; A demoted for the Krylov products
+      bp.X = bp.W;
+      b->h_a32[p] = (const float*)at(o.A32);
+      convA[p] = {sysA, lds, (float*)at(o.A32), (long long)n, m.n, m.n};
+    }
     if (ir) {
       // IR: sysB keeps the right-hand sides, X accumulates in W's slot (W is
       // dead once R = S W is formed), Rw holds residuals, X32 the FP32 corrections
@@ -571,6 +603,12 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
   if (b->nirr) {
     BCHK(b, hipMemcpyAsync(b->d_nrmC, nrmC.data(), sizeof(nrmC[0]) * nrmC.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
     BCHK(b, hipMemcpyAsync(b->d_nrmD, nrmD.data(), sizeof(nrmD[0]) * nrmD.size(), hipMemcpyHostToDevice, b->stream), "batch upload");
+  }
+  if (gm) {
+    BCHK(b, hipMemcpyAsync(b->d_convA, convA.data(), sizeof(convA[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
+    size_t ws = 0;
+    for (int p = 0; p < cnt; ++p) ws = std::max(ws, hm::gmres_workspace_bytes(b->mem[p].n, b->mem[p].ncol, b->gm_restart));
+    BCHK(b, hipMalloc(&b->d_gws, ws), "batch alloc");
   }
   if (ir) {
     BCHK(b, hipMemcpyAsync(b->d_res, res.data(), sizeof(res[0]) * cnt, hipMemcpyHostToDevice, b->stream), "batch upload");
@@ -655,7 +693,7 @@ extern "C" int hm_batch_prepare(hm_batch* b, int precision, int autotune, int us
     }
     b->variant = bv;
     b->tuned_ms = best;
-    if ((rc = tune_lu(b))) return rc;
+    if (!gm && (rc = tune_lu(b))) return rc;
   }
   // ---- graph
   if (b->use_graph) {
@@ -680,8 +718,30 @@ extern "C" int hm_batch_solve(hm_batch* b, const double* w1, double* amas, doubl
   if (rc) return rc;
   if (b->use_graph) BCHK(b, hipGraphLaunch(b->exec, b->stream), "batch graph launch");
   else if ((rc = issue_chain(b))) return rc;
-  // zero pivots
   const int cnt = b->count();
+  if (b->precision == HM_GMRES) {
+    b->gm_stats.assign(cnt, hm::GmresStats{});
+    const std::vector<hm::BatchProb>& pr = b->h_prob;
+    for (int p = 0; p < cnt; ++p) {
+      const Member& m = b->mem[p];
+      hm::GmresArgs ga;
+      ga.n = m.n;
+      ga.ncol = m.ncol;
+      ga.restart = b->gm_restart;
+      ga.maxiter = b->gm_maxiter;
+      ga.tol = b->gm_tol;
+      ga.inner_tol = 0.5 * b->gm_tol;
+      ga.A64 = pr[p].A;
+      ga.lda64 = (int)pr[p].ld;
+      ga.A32 = b->h_a32[p];
+      ga.B = pr[p].R;
+      ga.ldb = (int)pr[p].ld;
+      ga.X = pr[p].X;
+      BCHK(b, hm::gmres_solve(b->stream, ga, b->d_gws, &b->gm_stats[p]), "batch gmres");
+    }
+    BCHK(b, hm::batch_forces(b->stream, b->d_prob, b->d_freq, cnt, b->mmax, b->ncolmax), "batch forces");
+  }
+  // zero pivots
   std::vector<int> info(2 * (size_t)cnt);
   BCHK(b, hipMemcpyAsync(info.data(), b->d_info, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost, b->stream), "batch info");
   b->last_ratio = 0.0;
@@ -718,6 +778,39 @@ extern "C" int hm_batch_solve(hm_batch* b, const double* w1, double* amas, doubl
       BCHK(b, hipMemcpyAsync(exfc + b->off_exfc[p], prob[p].exfc, 16 * (size_t)m.m * m.nbeta, hipMemcpyDeviceToHost, b->stream), "batch output");
   }
   BCHK(b, hipStreamSynchronize(b->stream), "batch output");
+  return HM_OK;
+}
+
+extern "C" int hm_batch_set_gmres(hm_batch* b, int restart, double tol, int maxiter) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (restart < 1 || restart > hm::kGmresMaxRestart)
+    return b->fail(HM_ERR_ARG - 2, "restart must be in 1..%d", hm::kGmresMaxRestart);
+  if (!(tol > 0.0 && tol < 1.0)) return b->fail(HM_ERR_ARG - 3, "tol must be in (0, 1)");
+  if (maxiter < 1) return b->fail(HM_ERR_ARG - 4, "maxiter must be >= 1");
+  b->gm_restart = restart;
+  b->gm_tol = tol;
+  b->gm_maxiter = maxiter;
+  b->prepared = false;   // the workspace is sized for the restart length
+  return HM_OK;
+}
+
+extern "C" int hm_batch_gmres_stats(hm_batch* b, int* max_iterations, int* max_cycles, int* all_converged,
+                                    double* worst_rel_residual) {
+  if (!b) return HM_ERR_ARG - 1;
+  if (b->precision != HM_GMRES || b->gm_stats.empty())
+    return b->fail(HM_ERR_STATE, "no HM_GMRES batch solve yet");
+  int it = 0, cy = 0, ok = 1;
+  double worst = 0.0;
+  for (const hm::GmresStats& st : b->gm_stats) {
+    it = std::max(it, st.iterations);
+    cy = std::max(cy, st.cycles);
+    ok = ok && st.converged;
+    worst = std::max(worst, st.rel_residual);
+  }
+  if (max_iterations) *max_iterations = it;
+  if (max_cycles) *max_cycles = cy;
+  if (all_converged) *all_converged = ok;
+  if (worst_rel_residual) *worst_rel_residual = worst;
   return HM_OK;
 }
 

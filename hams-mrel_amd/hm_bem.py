@@ -38,7 +38,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_batch_error_string", "hm_batch_set_constants", "hm_batch_add_problem", "hm_batch_prepare",
            "hm_batch_solve", "hm_batch_info", "hm_hmatrix_build", "hm_hmatrix_release", "hm_ziter_solve",
            "hm_surrogate_predict", "hm_set_cluster_tree", "hm_get_cluster_tree_stats", "hm_hmatrix_build_tree",
-           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_surrogate_train_deep",
+           "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_batch_set_gmres", "hm_batch_gmres_stats", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
            "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing",
            "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check",
@@ -644,6 +644,18 @@ class Batch:
             oa += m * m
             oe += m * p.nbeta
         return out
+
+    def set_gmres(self, restart=50, tol=1e-6, maxiter=500):
+        """settings of the HM_GMRES members (before prepare)"""
+        self._chk(self.L.hm_batch_set_gmres(self.ptr, _i(restart), _d(tol), _i(maxiter)), "hm_batch_set_gmres")
+
+    def gmres_stats(self):
+        it, cy, ok = _i(), _i(), _i()
+        rr = _d()
+        self._chk(self.L.hm_batch_gmres_stats(self.ptr, ctypes.byref(it), ctypes.byref(cy), ctypes.byref(ok),
+                                              ctypes.byref(rr)), "hm_batch_gmres_stats")
+        return dict(max_iterations=it.value, max_cycles=cy.value, all_converged=bool(ok.value),
+                    worst_rel_residual=rr.value)
 
     def tuning(self):
         """the autotuner's choices: series variant, K chunk of the grouped GEMM

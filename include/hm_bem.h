@@ -373,12 +373,17 @@ int hm_surrogate_eval_deep(hm_ctx *ctx, int n, const double *kr, const double *k
  * irregular-frequency removal, the least squares of ASSB_LEFT_IRR /
  * RADIATION_SOLVER_IRR (AssbMatx_irr.f90:53-180, :472) through the normal
  * equations C = A^T A, D = A^T B (plain transpose), as hm_assemble does.
- * hm_batch_prepare(precision HM_FP64 or HM_FP32_IR, autotune, use_graph,
- * nrefine): allocates, classifies, times the series variants (lane-group
- * shapes) of the assembly when autotune != 0 and keeps the fastest, and
- * captures the chain when use_graph != 0. HM_FP32_IR factors in FP32 and runs
- * nrefine FP64-residual refinement sweeps inside the chain, then more (at most
- * 10) until every member's last correction is below 1e-13 of its solution.
+ * hm_batch_prepare(precision HM_FP64, HM_FP32_IR or HM_GMRES, autotune,
+ * use_graph, nrefine): allocates, classifies, times the series variants
+ * (lane-group shapes) of the assembly when autotune != 0 and keeps the
+ * fastest, and captures the chain when use_graph != 0. HM_FP32_IR factors in
+ * FP32 and runs nrefine FP64-residual refinement sweeps inside the chain, then
+ * more (at most 10) until every member's last correction is below 1e-13 of
+ * its solution. HM_GMRES (members without waterplane points) captures the
+ * assembly and the FP32 copies of A, then solves each member by the restarted
+ * GMRES of hm_ziter_solve's isolv = 1 form (ITERATIVE_SOLVER_MULTI,
+ * IterativeSolvers.f90:232-437: FP32 Krylov products, FP64 residuals, no
+ * preconditioner) one member after another, and forms the forces.
  * hm_batch_solve(w1[member] > 0): outputs stacked by member in member order
  * (amas, bdmp: (6 nbody)^2 each; exfc: 6 nbody x nbeta complex). A positive
  * return is a zero pivot of some member (hm_batch_error_string names it).
@@ -404,6 +409,14 @@ int hm_batch_info(hm_batch *batch, int *members, int *nmax, int *series_variant,
  * variant and the grouped LU's time at K chunks 8, 16, 32 (lu_ms[3]; zeros
  * without autotuning). No counterpart in the reference. */
 int hm_batch_tuning(hm_batch *batch, int *series_variant, int *gemm_kchunk, double *assembly_ms, double *lu_ms);
+/* HM_GMRES members: restart length, tolerance on ||B - A X|| / ||B|| per
+ * column and iteration cap (before hm_batch_prepare; defaults 50, 1e-6, 500),
+ * and after a solve the largest iteration and restart counts over the members,
+ * whether every member converged and the largest final relative residual.
+ * No counterpart in the reference (its GMRES is per problem). */
+int hm_batch_set_gmres(hm_batch *batch, int restart, double tol, int maxiter);
+int hm_batch_gmres_stats(hm_batch *batch, int *max_iterations, int *max_cycles, int *all_converged,
+                         double *worst_rel_residual);
 
 /* Stage timing: when enabled, a pair of HIP events is recorded on the context's
  * stream around each stage (no extra synchronisation); hm_get_timing waits for

@@ -9,7 +9,9 @@ in the two kernels). HIP graph replays are compared bit for bit with the same
 chain issued eagerly, three frequencies in a row with eager work in between.
 FP32 LU + FP64 refinement is held to 1e-10 of FP64 (it refines until the last
 correction is below 1e-13 of the solution). Members with interior waterplane
-points (irregular-frequency removal) are held to the same bars.
+points (irregular-frequency removal) are held to the same bars. GMRES members
+stop on ||B - A X|| <= tol ||B|| per column (tol 1e-8 here), so A/B/F agree with
+FP64 to ~cond * tol: bar 1e-6, graph replays and eager chains bit for bit.
 """
 import numpy as np
 import pytest
@@ -122,7 +124,7 @@ def test_batch_argument_errors():
         with pytest.raises(hm_bem.HMError, match="before hm_batch_prepare"):
             b.solve(1.0)
         with pytest.raises(hm_bem.HMError, match="precision"):
-            b.prepare(hm_bem.HM_GMRES)
+            b.prepare(7)
         b.prepare(hm_bem.HM_FP64, autotune=False, use_graph=False)
         with pytest.raises(hm_bem.HMError, match="must be > 0"):
             b.solve([1.0, 0.0])
@@ -179,3 +181,44 @@ def test_batch_waterplane_members():
     finally:
         b.close()
         b32.close()
+
+
+def test_batch_gmres_members():
+    probs = [p for p in _members()]
+    ws = [0.7, 1.3]
+    ref = hm_bem.Batch(probs)
+    gm = hm_bem.Batch(probs)
+    eager = hm_bem.Batch(probs)
+    try:
+        ref.prepare(hm_bem.HM_FP64, autotune=False, use_graph=False)
+        for b, graph in ((gm, True), (eager, False)):
+            b.set_gmres(restart=50, tol=1e-8, maxiter=400)
+            b.prepare(hm_bem.HM_GMRES, autotune=False, use_graph=graph)
+        for w in ws:
+            want = ref.solve(w)
+            got = gm.solve(w)
+            st = gm.gmres_stats()
+            again = eager.solve(w)
+            print(f"w={w}: gmres {st}", flush=True)
+            assert st["all_converged"] and st["worst_rel_residual"] <= 1e-8, st
+            for k, (a, b2, c) in enumerate(zip(got, want, again)):
+                for key in KEYS:
+                    assert relerr(a[key], b2[key]) < 1e-6, (w, k, key, relerr(a[key], b2[key]))
+                    assert np.array_equal(a[key], c[key]), (w, k, key)
+    finally:
+        ref.close()
+        gm.close()
+        eager.close()
+
+
+def test_batch_gmres_refuses_waterplane_members():
+    p = meshes.Problem([meshes.cylinder(3.0, 4.0, 12, 3, 2, waterplane_nr=2)], h=-1.0, omegas=[1.0],
+                       beta=np.array([0.0]))
+    b = hm_bem.Batch([p])
+    try:
+        with pytest.raises(hm_bem.HMError, match="waterplane"):
+            b.prepare(hm_bem.HM_GMRES, autotune=False, use_graph=False)
+        with pytest.raises(hm_bem.HMError, match="restart"):
+            b.set_gmres(restart=0)
+    finally:
+        b.close()
