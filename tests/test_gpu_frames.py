@@ -158,6 +158,18 @@ def _equal_bodies(body, centers):
                         local=local.copy(), origin=(float(c[0]), float(c[1]))) for c in centers]
 
 
+def test_sharing_and_fp16_arguments():
+    p = _array(2, 1, 40.0)
+    c = _ctx(p)
+    try:
+        with pytest.raises(hm_bem.HMError, match="0, 1 or 2"):
+            c._chk(c.L.hm_set_block_sharing(c.ptr, hm_bem._i(3)), "hm_set_block_sharing")
+        with pytest.raises(hm_bem.HMError, match="sizes"):
+            c.cgemm_f16(np.zeros((0, 2), np.complex64), np.zeros((0, 2), np.complex64))
+    finally:
+        c.close()
+
+
 def test_body_symmetries():
     centers = [(0.0, 0.0), (15.0, 0.0)]
     cyl = meshes.cylinder(3.0, 4.0, 16, 4, 2)
