@@ -2667,6 +2667,11 @@ extern "C" int hm_set_workers(hm_ctx* c, int nworkers) {
   if (!c) return HM_ERR_ARG - 1;
   if (nworkers < 1 || nworkers > 8) return c->fail(HM_ERR_ARG - 2, "nworkers must be in 1..8");
   c->nworkers = nworkers;
+  // clones beyond the new count release their device memory now
+  while ((int)c->workers.size() > nworkers - 1) {
+    hm_destroy(c->workers.back());
+    c->workers.pop_back();
+  }
   return HM_OK;
 }
 

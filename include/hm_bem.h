@@ -257,8 +257,8 @@ int hm_solve_frequency(hm_ctx *ctx, double w1, int precision, double *amas, doub
  * own; every frequency's outputs are those of hm_solve_frequency on it. */
 int hm_run_frequencies(hm_ctx *ctx, int nfreq, const double *w1, int precision, double *amas, double *bdmp,
                        double *exfc);
-/* frequencies in flight in hm_run_frequencies (1..8, default 1). No
- * counterpart in the reference. */
+/* frequencies in flight in hm_run_frequencies (1..8, default 1); clones
+ * beyond n - 1 are released at once. No counterpart in the reference. */
 int hm_set_workers(hm_ctx *ctx, int nworkers);
 
 /* Kernel variant of the two epsilon-accelerated finite-depth regimes (EIGENE,
