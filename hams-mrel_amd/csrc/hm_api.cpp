@@ -210,6 +210,7 @@ struct hm_ctx {
   DevBuf d_hmat, d_hD, d_hDS, d_hD32, d_hU32, d_hV32, d_htb, d_hts;
   DevBuf d_hUc[2], d_hVc[2];   // factors of A and S packed at their ranks (hmat_pack)
   DevBuf d_lrmeta[2];           // the packing's device metadata
+  DevBuf d_rhsx, d_rhsy;        // the right-hand sides' nonzero columns per body and their products
   hm::LowRankPlan lr_plan[2];
   hm::LowRank hm_lr[2];
   int hm_kuse[2] = {0, 0};      // the largest rank of each matrix
@@ -430,7 +431,7 @@ extern "C" int hm_destroy(hm_ctx* c) {
                     &c->d_ipiv, &c->d_info, &c->d_counter, &c->d_norm, &c->d_amas, &c->d_bdmp, &c->d_exfc,
                     &c->d_P32, &c->d_pipiv, &c->d_gwork, &c->d_luws, &c->d_Pinv, &c->d_gjws, &c->d_Pinv16, &c->d_pperm, &c->d_sur, &c->d_fb, &c->d_fbcnt, &c->d_pcache, &c->d_ptab, &c->d_info2, &c->d_counter2,
                     &c->d_hmat, &c->d_hD, &c->d_hDS, &c->d_hD32, &c->d_hU32, &c->d_hV32, &c->d_htb, &c->d_hts,
-                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_lrmeta[0], &c->d_lrmeta[1], &c->d_htw,
+                    &c->d_hUc[0], &c->d_hUc[1], &c->d_hVc[0], &c->d_hVc[1], &c->d_lrmeta[0], &c->d_lrmeta[1], &c->d_rhsx, &c->d_rhsy, &c->d_htw,
                     &c->d_surd, &c->d_surdcnt};
   for (DevBuf* b : bufs) b->release();
   c->htree.release();
@@ -1227,7 +1228,18 @@ static int assemble_hmatrix(hm_ctx* c) {
   // right-hand sides RHS = S * W
   c->stage_begin(kRhs);
   HIPCHK(c, hm::build_w(c->stream, gd, f, c->d_beta.as<double>(), (int)c->beta.size(), c->xw, c->yw, c->d_W.as<double>(), n), "rhs");
-  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[1], DS, m, diag_stride(c, (long long)m * (m + 1)),
+  const bool rhs_sparse = diag_stride(c, 1) == 0 && ncol == 6 * nb + (int)c->beta.size();
+  if (rhs_sparse) {
+    // the diagonal blocks' product over the columns that are nonzero on each body
+    const size_t ce = (size_t)m * nb * (6 + c->beta.size());
+    HIPCHK(c, c->d_rhsx.ensure(cplx * ce), "rhs alloc");
+    HIPCHK(c, c->d_rhsy.ensure(cplx * ce), "rhs alloc");
+    HIPCHK(c, hm::hmat_diag_rhs(c->stream, nb, m, (int)c->beta.size(), DS, m, c->d_W.as<double>(), n,
+                                c->d_R.as<double>(), n, c->d_rhsx.as<double>(), c->d_rhsy.as<double>()),
+           "rhs diagonal blocks");
+  }
+  HIPCHK(c, hm::hmat_apply<double>(c->stream, nb, m, c->hm_lr[1], rhs_sparse ? nullptr : DS, m,
+                                   diag_stride(c, (long long)m * (m + 1)),
                                    c->d_hUc[1].as<double>(), c->d_hVc[1].as<double>(), c->d_W.as<double>(), n,
                                    c->d_R.as<double>(), n, ncol, c->d_htb.as<double>(), c->d_hts.as<double>()),
          "rhs hmatrix product");

@@ -865,6 +865,52 @@ hipError_t hmat_pack(hipStream_t s, const HmatDev& h, int mat, const LowRankPlan
   return hipStreamSynchronize(s);
 }
 
+// Wc[i, b q + t] = X[b m + i, column t of body b]: its own 6 radiation
+// columns (t < 6), then the nbeta diffraction ones (q = 6 + nbeta)
+__global__ void k_rhs_gather(int nb, int m, int nbeta, const double* __restrict__ X, long long ldx,
+                             double* __restrict__ xc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, bt = blockIdx.y;
+  if (i >= m) return;
+  const int q = 6 + nbeta, b = bt / q, t = bt % q;
+  const int c = t < 6 ? 6 * b + t : 6 * nb + (t - 6);
+  const long long src = (long long)b * m + i + (long long)c * ldx, dst = (long long)i + (long long)bt * m;
+  xc[2 * dst] = X[2 * src];
+  xc[2 * dst + 1] = X[2 * src + 1];
+}
+
+// Y = the products back in place: radiation column c of body b's rows from
+// body b's own columns, zero for the other bodies' radiation columns
+__global__ void k_rhs_scatter(int nb, int m, int nbeta, const double* __restrict__ yc, double* __restrict__ Y,
+                              long long ldy) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r >= nb * m) return;
+  const int q = 6 + nbeta, b = r / m, i = r % m;
+  double yr = 0.0, yi = 0.0;
+  const int t = c < 6 * nb ? (c / 6 == b ? c % 6 : -1) : 6 + (c - 6 * nb);
+  if (t >= 0) {
+    const long long src = (long long)i + (long long)(b * q + t) * m;
+    yr = yc[2 * src];
+    yi = yc[2 * src + 1];
+  }
+  const long long dst = (long long)r + (long long)c * ldy;
+  Y[2 * dst] = yr;
+  Y[2 * dst + 1] = yi;
+}
+
+hipError_t hmat_diag_rhs(hipStream_t s, int nb, int m, int nbeta, const double* D, long long ldd, const double* X,
+                         long long ldx, double* Y, long long ldy, double* xc, double* yc) {
+  const int q = 6 + nbeta;
+  hipLaunchKernelGGL(k_rhs_gather, dim3((m + 255) / 256, nb * q), dim3(256), 0, s, nb, m, nbeta, X, ldx, xc);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = cgemm<double>(s, false, m, nb * q, m, 1.0, 0.0, D, (int)ldd, 0, xc, m, 0, 0.0, 0.0, yc, m, 0, 1)) !=
+      hipSuccess)
+    return e;
+  hipLaunchKernelGGL(k_rhs_scatter, dim3((nb * m + 255) / 256, 6 * nb + nbeta), dim3(256), 0, s, nb, m, nbeta, yc, Y,
+                     ldy);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* D, long long ldd, long long sD,
                       const T* U, const T* V, const T* X, long long ldx, T* Y, long long ldy, int ncol, T* tbuf,

@@ -124,6 +124,15 @@ inline size_t lowrank_meta_ints(int nb, int ru) { return 2 * (size_t)nb * (nb - 
 hipError_t hmat_pack(hipStream_t s, const HmatDev& h, int mat, const LowRankPlan& plan, int* meta, double* Uc,
                      double* Vc, LowRank* out);
 
+// Y = D X for the right-hand sides W of the H-matrix form (column c < 6 nb is
+// zero outside body c / 6, the nbeta columns after them are full), with one
+// diagonal block D shared by all nb bodies: body b's rows need its own 6
+// columns and the diffraction ones only, so the products of all bodies are one
+// GEMM over nb (6 + nbeta) columns (xc, yc: m nb (6 + nbeta) complex each).
+// Every entry of Y is written; add the low-rank part with hmat_apply (D null).
+hipError_t hmat_diag_rhs(hipStream_t s, int nb, int m, int nbeta, const double* D, long long ldd, const double* X,
+                         long long ldx, double* Y, long long ldy, double* xc, double* yc);
+
 // Y = H X: diagonal blocks D (block b at D + b sD complex elements,
 // column-major, ld ldd; sD = 0: one block for every body) plus the low-rank
 // blocks of one matrix, packed by
