@@ -1437,42 +1437,63 @@ hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int 
 // ---------------------------------------------------------------------------
 // Complex GEMM with FP16 operands and FP32 accumulation (MFMA f32 16x16x32 f16)
 // ---------------------------------------------------------------------------
-// One wave per 64 x 32 output tile (4 x 2 MFMA tiles of 16 x 16), no LDS: with
-// the packed layouts each lane's MFMA operand (8 k of one row of A or one
-// column of B) is a pair of 16-byte loads straight from memory; tiles that
-// share rows of A or columns of B reuse them through the caches. Both operands
-// give lane group q (lanes 16q..16q+15) the k of group q of the 32-k chunk, so
-// the products pair equal k whatever order the matrix core sums them in. A
-// complex product is four real MFMAs into Re / Im accumulators (Re gets
-// Ar Br + Ai (-Bi)). The next chunk's operands are loaded before the current
-// chunk's MFMAs.
+// A workgroup of 4 waves per 128 x 64 output tile, each wave 64 x 32 (4 x 2
+// MFMA tiles of 16 x 16). Per 32-k chunk the workgroup stages the tile's rows
+// of A and columns of B in LDS (the packed layout, rows padded to 72 halves),
+// so that each is read from memory once for the two waves that share it; the
+// next chunk is loaded into registers while the current one is multiplied.
+// Both operands give lane group q (lanes 16q..16q+15) the k of group q of the
+// chunk, so the products pair equal k whatever order the matrix core sums them
+// in. A complex product is four real MFMAs into Re / Im accumulators (Re gets
+// Ar Br + Ai (-Bi)).
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+constexpr int kH16Ld = 72;   // halves per LDS row: 4 groups of 16 and a pad of 8
 
-__global__ __launch_bounds__(64) void k_cgemm_h16(int m, int ncol, int kp, const _Float16* __restrict__ A,
-                                                  long long sA, const _Float16* __restrict__ B, long long ldb,
-                                                  long long sB, float* __restrict__ C, long long ldc, long long sC,
-                                                  int cnt, int joint) {
-  const int lane = threadIdx.x, li = lane & 15, q = lane >> 4;
-  const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 32;
+__global__ __launch_bounds__(256) void k_cgemm_h16(int m, int ncol, int kp, const _Float16* __restrict__ A,
+                                                   long long sA, const _Float16* __restrict__ B, long long ldb,
+                                                   long long sB, float* __restrict__ C, long long ldc, long long sC,
+                                                   int cnt, int joint) {
+  __shared__ __attribute__((aligned(16))) _Float16 la[128 * kH16Ld];
+  __shared__ __attribute__((aligned(16))) _Float16 lb[64 * kH16Ld];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, q = lane >> 4;
+  const int row0 = blockIdx.x * 128, col0 = blockIdx.y * 64;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 32;
   // joint: the members' columns tile together (one A); else member blockIdx.z
   const int ntot = joint ? ncol * cnt : ncol;
   const int mb = joint ? 0 : (int)blockIdx.z;
-  const _Float16* a = A + sA * mb;
-  const _Float16* pa[4];
-  const _Float16* pb[2];
   const long long kg = kp / 8;
+  // this thread's 16-byte pieces of the A tile (4: rows (tid + 256 i) / 8) and
+  // of the B tile (2: columns (tid + 256 i) / 8), part (tid % 8) of the row's
+  // 64 halves of the chunk
+  const int part = tid & 7;
+  const _Float16* ga[4];
+  const _Float16* gb[2];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int r = min(row0 + 16 * t + li, m - 1);
-    pa[t] = a + ((long long)r * kg + q) * 16;
+  for (int i = 0; i < 4; ++i) {
+    const int r = min(row0 + (tid + 256 * i) / 8, m - 1);
+    ga[i] = A + sA * mb + (long long)r * kg * 16 + part * 8;
   }
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int vc = min(col0 + 16 * u + li, ntot - 1);
-    const int b = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
-    pb[u] = B + sB * b + ldb * c + q * 16;
+  for (int i = 0; i < 2; ++i) {
+    const int vc = min(col0 + (tid + 256 * i) / 8, ntot - 1);
+    const int bb = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
+    gb[i] = B + sB * bb + ldb * c + part * 8;
   }
+  half8_t pa[4], pb[2];
+  auto load = [&](int kc) {
+    const long long o = (long long)kc * 64;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pa[i] = *(const half8_t*)(ga[i] + o);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) pb[i] = *(const half8_t*)(gb[i] + o);
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *(half8_t*)(la + ((tid + 256 * i) / 8) * kH16Ld + part * 8) = pa[i];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) *(half8_t*)(lb + ((tid + 256 * i) / 8) * kH16Ld + part * 8) = pb[i];
+  };
   float4_t cre[4][2], cim[4][2];
 #pragma unroll
   for (int t = 0; t < 4; ++t)
@@ -1480,30 +1501,24 @@ __global__ __launch_bounds__(64) void k_cgemm_h16(int m, int ncol, int kp, const
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int v = 0; v < 4; ++v) { cre[t][u][v] = 0.0f; cim[t][u][v] = 0.0f; }
-  half8_t ar[4], ai[4], br[2], bi[2];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    ar[t] = *(const half8_t*)pa[t];
-    ai[t] = *(const half8_t*)(pa[t] + 8);
-  }
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    br[u] = *(const half8_t*)pb[u];
-    bi[u] = *(const half8_t*)(pb[u] + 8);
-  }
   const int nk = kp / 32;
+  load(0);
+  stash();
+  __syncthreads();
   for (int kc = 0; kc < nk; ++kc) {
-    half8_t nar[4], nai[4], nbr[2], nbi[2];
-    const long long o = (long long)min(kc + 1, nk - 1) * 64;
+    if (kc + 1 < nk) load(kc + 1);
+    half8_t ar[4], ai[4], br[2], bi[2];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      nar[t] = *(const half8_t*)(pa[t] + o);
-      nai[t] = *(const half8_t*)(pa[t] + o + 8);
+      const _Float16* x = la + (wr + 16 * t + li) * kH16Ld + q * 16;
+      ar[t] = *(const half8_t*)x;
+      ai[t] = *(const half8_t*)(x + 8);
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      nbr[u] = *(const half8_t*)(pb[u] + o);
-      nbi[u] = *(const half8_t*)(pb[u] + o + 8);
+      const _Float16* y = lb + (wc + 16 * u + li) * kH16Ld + q * 16;
+      br[u] = *(const half8_t*)y;
+      bi[u] = *(const half8_t*)(y + 8);
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
@@ -1516,23 +1531,24 @@ __global__ __launch_bounds__(64) void k_cgemm_h16(int m, int ncol, int kp, const
         cim[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ai[t], br[u], cim[t][u], 0, 0, 0);
       }
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) { ar[t] = nar[t]; ai[t] = nai[t]; }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) { br[u] = nbr[u]; bi[u] = nbi[u]; }
+    __syncthreads();
+    if (kc + 1 < nk) {
+      stash();
+      __syncthreads();
+    }
   }
   // lane holds rows 4q..4q+3 of each 16 x 16 tile at its column li
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int vc = col0 + 16 * u + li;
+    const int vc = col0 + wc + 16 * u + li;
     if (vc >= ntot) continue;
-    const int b = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
-    float* cc = C + 2 * (sC * b + ldc * c);
+    const int bb = joint ? vc / ncol : mb, c = joint ? vc % ncol : vc;
+    float* cc = C + 2 * (sC * bb + ldc * c);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = row0 + 16 * t + 4 * q + v;
+        const int r = row0 + wr + 16 * t + 4 * q + v;
         if (r < m) {
           cc[2 * (long long)r] = cre[t][u][v];
           cc[2 * (long long)r + 1] = cim[t][u][v];
@@ -1547,8 +1563,8 @@ hipError_t cgemm_h16(hipStream_t s, int m, int ncol, int kp, const _Float16* A, 
   if (kp <= 0 || kp % 32) return hipErrorInvalidValue;
   const int joint = sA == 0 ? 1 : 0;
   const int cols = joint ? ncol * cnt : ncol;
-  hipLaunchKernelGGL(k_cgemm_h16, dim3((m + 63) / 64, (cols + 31) / 32, joint ? 1 : cnt), dim3(64), 0, s, m, ncol, kp,
-                     A, sA, B, ldb, sB, C, ldc, sC, cnt, joint);
+  hipLaunchKernelGGL(k_cgemm_h16, dim3((m + 127) / 128, (cols + 63) / 64, joint ? 1 : cnt), dim3(256), 0, s, m, ncol,
+                     kp, A, sA, B, ldb, sB, C, ldc, sC, cnt, joint);
   return hipGetLastError();
 }
 
