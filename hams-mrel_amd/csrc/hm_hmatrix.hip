@@ -903,7 +903,7 @@ hipError_t hmat_diag_rhs(hipStream_t s, int nb, int m, int nbeta, const double* 
   hipLaunchKernelGGL(k_rhs_gather, dim3((m + 255) / 256, nb * q), dim3(256), 0, s, nb, m, nbeta, X, ldx, xc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if ((e = cgemm<double>(s, false, m, nb * q, m, 1.0, 0.0, D, (int)ldd, 0, xc, m, 0, 0.0, 0.0, yc, m, 0, 1)) !=
+  if ((e = cgemm_3m<double>(s, m, nb * q, m, 1.0, 0.0, D, (int)ldd, 0, xc, m, 0, 0.0, 0.0, yc, m, 0, 1)) !=
       hipSuccess)
     return e;
   hipLaunchKernelGGL(k_rhs_scatter, dim3((nb * m + 255) / 256, 6 * nb + nbeta), dim3(256), 0, s, nb, m, nbeta, yc, Y,
@@ -920,12 +920,19 @@ hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* 
   // off-diagonal part is added). sD = 0, one block for all bodies: with X and Y
   // of leading dimension nb m, block b's rows of column c are column b + c nb
   // of X and Y read as m x (nb ncol) matrices of leading dimension m, so the
-  // product is one GEMM over all bodies' columns (the same sums per entry)
+  // product is one GEMM over all bodies' columns (the same sums per entry).
+  // In FP64 (the residuals) the diagonal blocks take three real products per
+  // complex one: their error stays at FP64's, far below the GMRES tolerance
+  const bool m3 = sizeof(T) == 8;
   if (D && sD == 0 && ldx == (long long)nb * m && ldy == ldx)
-    e = cgemm<T>(s, false, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1);
+    e = m3 ? cgemm_3m<T>(s, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1)
+           : cgemm<T>(s, false, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1);
+  else if (D)
+    e = m3 ? cgemm_3m<T>(s, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
+           : cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m,
+                      nb);
   else
-    e = D ? cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
-          : hipSuccess;
+    e = hipSuccess;
   if (e != hipSuccess || lr.ru == 0 || lr.rv == 0) return e;
   // T_b2 = V_b2^T x_b2 for every block column b2
   e = cgemm<T>(s, true, lr.rv, ncol, m, T(1), T(0), V, m, (long long)lr.rv * m, X, (int)ldx, m, T(0), T(0), tbuf,

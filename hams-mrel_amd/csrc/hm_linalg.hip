@@ -54,8 +54,12 @@ constexpr int kBM = 64, kBN = 64, kBK = 16;
 // One 64 x 64 tile (rows row0.., columns col0..) of
 // C[m x n] = alpha * op(A)[m x k] * B[k x n] + beta * C, K staged through LDS
 // in chunks of KB (8, 16 or 32; 16 by default, the grouped factorisation's
-// autotuner picks among them).
-template <typename T, bool TA, int KB = kBK>
+// autotuner picks among them). M3: the complex product from three real ones
+// (P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2,
+// Im = P3 - P1 - P2), three MFMAs per complex step instead of four; the
+// imaginary part's error is then bounded by (|Ar| + |Ai|)(|Br| + |Bi|) rather
+// than by its own terms.
+template <typename T, bool TA, int KB = kBK, bool M3 = false>
 __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, const T* __restrict__ A, int lda,
                                            const T* __restrict__ B, int ldb, T br, T bi, T* __restrict__ C,
                                            int ldc, int row0, int col0) {
@@ -65,10 +69,10 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
   typedef typename vec4<T>::type v4;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  v4 cre[2][2], cim[2][2];
+  v4 cre[2][2], cim[2][2], cx[2][2];   // M3: P1, P3, P2
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
-      for (int v = 0; v < 4; ++v) { cre[i][j][v] = 0; cim[i][j][v] = 0; }
+      for (int v = 0; v < 4; ++v) { cre[i][j][v] = 0; cim[i][j][v] = 0; cx[i][j][v] = 0; }
 
   // op(A)[row0:+64, k0:+KB] and B[k0:+KB, col0:+64] go through registers into
   // LDS; the next chunk's global loads are issued before the current chunk's
@@ -129,13 +133,27 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
         b_re[s] = bs_re[kr][wc + 16 * s + li];
         b_im[s] = bs_im[kr][wc + 16 * s + li];
       }
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
-          cre[i][j] = mfma16(a_re[i], b_re[j], cre[i][j]);
-          cre[i][j] = mfma16(-a_im[i], b_im[j], cre[i][j]);
-          cim[i][j] = mfma16(a_re[i], b_im[j], cim[i][j]);
-          cim[i][j] = mfma16(a_im[i], b_re[j], cim[i][j]);
+      if (M3) {
+        T a_s[2], b_s[2];
+        for (int s = 0; s < 2; ++s) {
+          a_s[s] = a_re[s] + a_im[s];
+          b_s[s] = b_re[s] + b_im[s];
         }
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            cre[i][j] = mfma16(a_re[i], b_re[j], cre[i][j]);
+            cx[i][j] = mfma16(a_im[i], b_im[j], cx[i][j]);
+            cim[i][j] = mfma16(a_s[i], b_s[j], cim[i][j]);
+          }
+      } else {
+        for (int i = 0; i < 2; ++i)
+          for (int j = 0; j < 2; ++j) {
+            cre[i][j] = mfma16(a_re[i], b_re[j], cre[i][j]);
+            cre[i][j] = mfma16(-a_im[i], b_im[j], cre[i][j]);
+            cim[i][j] = mfma16(a_re[i], b_im[j], cim[i][j]);
+            cim[i][j] = mfma16(a_im[i], b_re[j], cim[i][j]);
+          }
+      }
     }
     __syncthreads();
     if (more) {
@@ -150,7 +168,8 @@ __device__ __forceinline__ void cgemm_tile(int m, int n, int k, T ar, T ai, cons
         const int gc = col0 + wc + 16 * j + (lane & 15);
         if (gr < m && gc < n) {
           const long long off = (long long)gr + (long long)gc * ldc;
-          const T pr = cre[i][j][v], pi = cim[i][j][v];
+          const T pr = M3 ? cre[i][j][v] - cx[i][j][v] : cre[i][j][v];
+          const T pi = M3 ? cim[i][j][v] - cre[i][j][v] - cx[i][j][v] : cim[i][j][v];
           T outr = ar * pr - ai * pi, outi = ar * pi + ai * pr;
           if (br != T(0) || bi != T(0)) {
             const T c_r = C[2 * off], c_i = C[2 * off + 1];
@@ -171,6 +190,15 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, 
                                                long long sC) {
   cgemm_tile<T, TA>(m, n, k, ar, ai, A + 2 * sA * blockIdx.z, lda, B + 2 * sB * blockIdx.z, ldb, br, bi,
                     C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
+}
+
+// blockIdx.z = batch index; A not transposed, three real products a step
+template <typename T>
+__global__ __launch_bounds__(256, 2) void k_cgemm_3m(int m, int n, int k, T ar, T ai, const T* __restrict__ A, int lda,
+                                                  long long sA, const T* __restrict__ B, int ldb, long long sB, T br,
+                                                  T bi, T* __restrict__ C, int ldc, long long sC) {
+  cgemm_tile<T, false, kBK, true>(m, n, k, ar, ai, A + 2 * sA * blockIdx.z, lda, B + 2 * sB * blockIdx.z, ldb, br, bi,
+                                  C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
 }
 
 // blockIdx.z = group member; tiles outside the member's m x n exit
@@ -195,6 +223,20 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
     hipLaunchKernelGGL((k_cgemm<T, false>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc, sC);
   return hipGetLastError();
 }
+
+template <typename T>
+hipError_t cgemm_3m(hipStream_t s, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA, const T* B,
+                    int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return hipSuccess;
+  dim3 grid((m + kBM - 1) / kBM, (n + kBN - 1) / kBN, batch);
+  hipLaunchKernelGGL((k_cgemm_3m<T>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc,
+                     sC);
+  return hipGetLastError();
+}
+template hipError_t cgemm_3m<double>(hipStream_t, int, int, int, double, double, const double*, int, long long,
+                                     const double*, int, long long, double, double, double*, int, long long, int);
+template hipError_t cgemm_3m<float>(hipStream_t, int, int, int, float, float, const float*, int, long long,
+                                    const float*, int, long long, float, float, float*, int, long long, int);
 
 template <typename T>
 hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax, int kb) {

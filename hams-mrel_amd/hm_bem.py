@@ -41,7 +41,7 @@ EXPORTS = ("hm_create", "hm_destroy", "hm_error_string", "hm_set_constants", "hm
            "hm_cluster_tree_partition", "hm_set_lu_panel", "hm_batch_set_waterplane", "hm_batch_tuning", "hm_batch_set_gmres", "hm_batch_gmres_stats", "hm_surrogate_train_deep",
            "hm_set_surrogate_deep", "hm_surrogate_deep_count", "hm_surrogate_eval_deep", "hm_set_body_origins",
            "hm_get_body_classes", "hm_set_block_sharing", "hm_get_block_sharing",
-           "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check",
+           "hm_set_workers", "hm_set_precond_fp16", "hm_cgemm_f16_check", "hm_cgemm_3m_check",
            "hm_get_body_symmetries")
 
 _lib = None
@@ -247,6 +247,16 @@ class Context:
         n = b.shape[1]
         out = np.zeros((m, n), dtype=np.complex64, order="F")
         self._chk(self.L.hm_cgemm_f16_check(self.ptr, _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)), "hm_cgemm_f16_check")
+        return out
+
+    def cgemm_3m(self, a, b):
+        """a @ b in FP64 on the device, each complex product from three real ones"""
+        a = np.asfortranarray(a, dtype=np.complex128)
+        b = np.asfortranarray(b, dtype=np.complex128)
+        m, k = a.shape
+        n = b.shape[1]
+        out = np.zeros((m, n), dtype=np.complex128, order="F")
+        self._chk(self.L.hm_cgemm_3m_check(self.ptr, _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)), "hm_cgemm_3m_check")
         return out
 
     def set_workers(self, n=1):
