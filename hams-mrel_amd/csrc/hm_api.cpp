@@ -666,16 +666,19 @@ extern "C" int hm_cgemm_f16_check(hm_ctx* c, int m, int n, int k, const float* a
   return HM_OK;
 }
 
-extern "C" int hm_cgemm_3m_check(hm_ctx* c, int m, int n, int k, const double* a, const double* b, double* out) {
+extern "C" int hm_cgemm_3m_check(hm_ctx* c, int trans_a, int m, int n, int k, const double* a, const double* b,
+                                 double* out) {
   if (!c) return HM_ERR_ARG - 1;
-  if (m < 1 || n < 1 || k < 1) return c->fail(HM_ERR_ARG - 2, "sizes must be >= 1");
-  if (!a || !b || !out) return c->fail(HM_ERR_ARG - 5, "arrays missing");
+  if (trans_a != 0 && trans_a != 1) return c->fail(HM_ERR_ARG - 2, "trans_a must be 0 or 1");
+  if (m < 1 || n < 1 || k < 1) return c->fail(HM_ERR_ARG - 3, "sizes must be >= 1");
+  if (!a || !b || !out) return c->fail(HM_ERR_ARG - 6, "arrays missing");
   const size_t na = 2 * (size_t)m * k, nb = 2 * (size_t)k * n, nc = 2 * (size_t)m * n;
   DevBuf da, db, dc;
   int rc = 0;
   if ((rc = upload(c, da, a, na, "cgemm 3m")) || (rc = upload(c, db, b, nb, "cgemm 3m"))) return rc;
   HIPCHK(c, dc.ensure(sizeof(double) * nc), "cgemm 3m");
-  HIPCHK(c, hm::cgemm_3m<double>(c->stream, m, n, k, 1.0, 0.0, da.as<double>(), m, 0, db.as<double>(), k, 0, 0.0, 0.0,
+  HIPCHK(c, hm::cgemm_3m<double>(c->stream, trans_a == 1, m, n, k, 1.0, 0.0, da.as<double>(), trans_a ? k : m, 0,
+                                 db.as<double>(), k, 0, 0.0, 0.0,
                                  dc.as<double>(), m, 0, 1), "cgemm 3m");
   HIPCHK(c, hipMemcpyAsync(out, dc.p, sizeof(double) * nc, hipMemcpyDeviceToHost, c->stream), "cgemm 3m");
   HIPCHK(c, hipStreamSynchronize(c->stream), "cgemm 3m");

@@ -903,7 +903,7 @@ hipError_t hmat_diag_rhs(hipStream_t s, int nb, int m, int nbeta, const double* 
   hipLaunchKernelGGL(k_rhs_gather, dim3((m + 255) / 256, nb * q), dim3(256), 0, s, nb, m, nbeta, X, ldx, xc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if ((e = cgemm_3m<double>(s, m, nb * q, m, 1.0, 0.0, D, (int)ldd, 0, xc, m, 0, 0.0, 0.0, yc, m, 0, 1)) !=
+  if ((e = cgemm_3m<double>(s, false, m, nb * q, m, 1.0, 0.0, D, (int)ldd, 0, xc, m, 0, 0.0, 0.0, yc, m, 0, 1)) !=
       hipSuccess)
     return e;
   hipLaunchKernelGGL(k_rhs_scatter, dim3((nb * m + 255) / 256, 6 * nb + nbeta), dim3(256), 0, s, nb, m, nbeta, yc, Y,
@@ -921,30 +921,31 @@ hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* 
   // of leading dimension nb m, block b's rows of column c are column b + c nb
   // of X and Y read as m x (nb ncol) matrices of leading dimension m, so the
   // product is one GEMM over all bodies' columns (the same sums per entry).
-  // In FP64 (the residuals) the diagonal blocks take three real products per
+  // In FP64 (the residuals) every product takes three real products per
   // complex one: their error stays at FP64's, far below the GMRES tolerance
-  const bool m3 = sizeof(T) == 8;
+  auto gemm = [&](bool ta, int gm, int gn, int gk, const T* A, int lda, long long sA, const T* B, int ldb,
+                  long long sB, T beta, T* C, int ldc, long long sC, int cnt) {
+    if constexpr (sizeof(T) == 8)
+      return cgemm_3m<T>(s, ta, gm, gn, gk, T(1), T(0), A, lda, sA, B, ldb, sB, beta, T(0), C, ldc, sC, cnt);
+    else
+      return cgemm<T>(s, ta, gm, gn, gk, T(1), T(0), A, lda, sA, B, ldb, sB, beta, T(0), C, ldc, sC, cnt);
+  };
   if (D && sD == 0 && ldx == (long long)nb * m && ldy == ldx)
-    e = m3 ? cgemm_3m<T>(s, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1)
-           : cgemm<T>(s, false, m, nb * ncol, m, T(1), T(0), D, (int)ldd, 0, X, m, 0, T(0), T(0), Y, m, 0, 1);
-  else if (D)
-    e = m3 ? cgemm_3m<T>(s, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m, nb)
-           : cgemm<T>(s, false, m, ncol, m, T(1), T(0), D, (int)ldd, sD, X, (int)ldx, m, T(0), T(0), Y, (int)ldy, m,
-                      nb);
+    e = gemm(false, m, nb * ncol, m, D, (int)ldd, 0, X, m, 0, T(0), Y, m, 0, 1);
   else
-    e = hipSuccess;
+    e = D ? gemm(false, m, ncol, m, D, (int)ldd, sD, X, (int)ldx, m, T(0), Y, (int)ldy, m, nb) : hipSuccess;
   if (e != hipSuccess || lr.ru == 0 || lr.rv == 0) return e;
   // T_b2 = V_b2^T x_b2 for every block column b2
-  e = cgemm<T>(s, true, lr.rv, ncol, m, T(1), T(0), V, m, (long long)lr.rv * m, X, (int)ldx, m, T(0), T(0), tbuf,
-               lr.rv, (long long)lr.rv * ncol, nb);
+  e = gemm(true, lr.rv, ncol, m, V, m, (long long)lr.rv * m, X, (int)ldx, m, T(0), tbuf, lr.rv,
+           (long long)lr.rv * ncol, nb);
   if (e != hipSuccess) return e;
   const long long tot = (long long)nb * lr.ru * ncol;
   hipLaunchKernelGGL((k_permute_map<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nb, lr.ru, lr.rv, ncol,
                      lr.tmap, tbuf, tstack);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // y_b1 += U_b1 [T_{b1,b2}]_b2
-  return cgemm<T>(s, false, m, ncol, lr.ru, T(1), T(0), U, m, (long long)lr.ru * m, tstack, lr.ru,
-                  (long long)lr.ru * ncol, T(1), T(0), Y, (int)ldy, m, nb);
+  return gemm(false, m, ncol, lr.ru, U, m, (long long)lr.ru * m, tstack, lr.ru, (long long)lr.ru * ncol, T(1), Y,
+              (int)ldy, m, nb);
 }
 
 template hipError_t hmat_apply<double>(hipStream_t, int, int, const LowRank&, const double*, long long, long long,

@@ -192,13 +192,13 @@ __global__ __launch_bounds__(256) void k_cgemm(int m, int n, int k, T ar, T ai, 
                     C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
 }
 
-// blockIdx.z = batch index; A not transposed, three real products a step
-template <typename T>
+// blockIdx.z = batch index; three real products a complex step
+template <typename T, bool TA>
 __global__ __launch_bounds__(256, 2) void k_cgemm_3m(int m, int n, int k, T ar, T ai, const T* __restrict__ A, int lda,
                                                   long long sA, const T* __restrict__ B, int ldb, long long sB, T br,
                                                   T bi, T* __restrict__ C, int ldc, long long sC) {
-  cgemm_tile<T, false, kBK, true>(m, n, k, ar, ai, A + 2 * sA * blockIdx.z, lda, B + 2 * sB * blockIdx.z, ldb, br, bi,
-                                  C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
+  cgemm_tile<T, TA, kBK, true>(m, n, k, ar, ai, A + 2 * sA * blockIdx.z, lda, B + 2 * sB * blockIdx.z, ldb, br, bi,
+                               C + 2 * sC * blockIdx.z, ldc, blockIdx.x * kBM, blockIdx.y * kBN);
 }
 
 // blockIdx.z = group member; tiles outside the member's m x n exit
@@ -225,18 +225,20 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
 }
 
 template <typename T>
-hipError_t cgemm_3m(hipStream_t s, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA, const T* B,
-                    int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch) {
+hipError_t cgemm_3m(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA,
+                    const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch) {
   if (m <= 0 || n <= 0 || batch <= 0) return hipSuccess;
   dim3 grid((m + kBM - 1) / kBM, (n + kBN - 1) / kBN, batch);
-  hipLaunchKernelGGL((k_cgemm_3m<T>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi, C, ldc,
-                     sC);
+  if (transA)
+    hipLaunchKernelGGL((k_cgemm_3m<T, true>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi,
+                       C, ldc, sC);
+  else
+    hipLaunchKernelGGL((k_cgemm_3m<T, false>), grid, dim3(256), 0, s, m, n, k, ar, ai, A, lda, sA, B, ldb, sB, br, bi,
+                       C, ldc, sC);
   return hipGetLastError();
 }
-template hipError_t cgemm_3m<double>(hipStream_t, int, int, int, double, double, const double*, int, long long,
+template hipError_t cgemm_3m<double>(hipStream_t, bool, int, int, int, double, double, const double*, int, long long,
                                      const double*, int, long long, double, double, double*, int, long long, int);
-template hipError_t cgemm_3m<float>(hipStream_t, int, int, int, float, float, const float*, int, long long,
-                                    const float*, int, long long, float, float, float*, int, long long, int);
 
 template <typename T>
 hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax, int kb) {

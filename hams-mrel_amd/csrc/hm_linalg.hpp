@@ -10,13 +10,13 @@ constexpr int kLuNb = 64;   // LU panel width (also the triangular-solve block)
 template <typename T>
 hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA,
                  const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch);
-// The same with A not transposed, each complex product formed from three real
-// ones (Re = Ar Br - Ai Bi, Im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi): a quarter
+// The same with each complex product formed from three real ones
+// (Re = Ar Br - Ai Bi, Im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi): a quarter
 // fewer MFMAs; the imaginary part's error is bounded by
-// (|Ar| + |Ai|)(|Br| + |Bi|) instead of by its own terms.
+// (|Ar| + |Ai|)(|Br| + |Bi|) instead of by its own terms. FP64 only.
 template <typename T>
-hipError_t cgemm_3m(hipStream_t s, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA, const T* B,
-                    int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch);
+hipError_t cgemm_3m(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA,
+                    const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch);
 
 // In-place partial-pivot LU of `batch` n x n matrices (0-based ipiv, n per matrix;
 // info[b] = first zero-pivot column + 1 or 0). counter: `batch` zeroed uints.

@@ -249,14 +249,16 @@ class Context:
         self._chk(self.L.hm_cgemm_f16_check(self.ptr, _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)), "hm_cgemm_f16_check")
         return out
 
-    def cgemm_3m(self, a, b):
-        """a @ b in FP64 on the device, each complex product from three real ones"""
+    def cgemm_3m(self, a, b, trans_a=False):
+        """a @ b (a.T @ b with trans_a, not conjugated) in FP64 on the device,
+        each complex product from three real ones"""
         a = np.asfortranarray(a, dtype=np.complex128)
         b = np.asfortranarray(b, dtype=np.complex128)
-        m, k = a.shape
+        k, m = a.shape if trans_a else a.shape[::-1]
         n = b.shape[1]
         out = np.zeros((m, n), dtype=np.complex128, order="F")
-        self._chk(self.L.hm_cgemm_3m_check(self.ptr, _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)), "hm_cgemm_3m_check")
+        self._chk(self.L.hm_cgemm_3m_check(self.ptr, _i(int(trans_a)), _i(m), _i(n), _i(k), _p(a), _p(b), _p(out)),
+                  "hm_cgemm_3m_check")
         return out
 
     def set_workers(self, n=1):

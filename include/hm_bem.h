@@ -105,12 +105,13 @@ int hm_set_precond_fp16(hm_ctx *ctx, int enable);
  * host arrays, rounded to FP16 and packed on the device, FP32 accumulation.
  * No counterpart in the reference. */
 int hm_cgemm_f16_check(hm_ctx *ctx, int m, int n, int k, const float *a, const float *b, float *out);
-/* Parity entry point of the FP64 complex GEMM that applies the H-matrix form's
- * diagonal blocks in its residuals: out (m x n) = a (m x k) * b (k x n),
- * column-major, interleaved complex FP64 host arrays, each complex product
- * formed from three real ones (Re = ArBr - AiBi, Im = (Ar + Ai)(Br + Bi) -
- * ArBr - AiBi). No counterpart in the reference. */
-int hm_cgemm_3m_check(hm_ctx *ctx, int m, int n, int k, const double *a, const double *b, double *out);
+/* Parity entry point of the FP64 complex GEMM of the H-matrix form's
+ * residuals: out (m x n) = op(a) * b (k x n), op(a) = a (m x k) or, with
+ * trans_a = 1, a^T (a k x m, not conjugated); column-major, interleaved
+ * complex FP64 host arrays, each complex product formed from three real ones
+ * (Re = ArBr - AiBi, Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi). No counterpart in
+ * the reference. */
+int hm_cgemm_3m_check(hm_ctx *ctx, int trans_a, int m, int n, int k, const double *a, const double *b, double *out);
 /* Water depth h (h < 0: infinite). Classifies the panel pairs by R/h regime. */
 int hm_set_depth(hm_ctx *ctx, double h);
 /* Wave headings [rad] of the diffraction problems (WVHD*PI/180, HAMS_Prog.f90:696). */

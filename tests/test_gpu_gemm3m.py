@@ -19,15 +19,17 @@ import hm_bem
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("trans", [False, True], ids=["a", "a^T"])
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (70, 33, 45), (128, 98, 300), (200, 7, 17), (64, 64, 2048),
                                    (2048, 130, 2048)])
-def test_cgemm_3m_within_rounding_bound(m, n, k):
+def test_cgemm_3m_within_rounding_bound(m, n, k, trans):
     rng = np.random.default_rng(m * 7919 + n * 31 + k)
     a = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
     b = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
     c = hm_bem.Context()
     try:
-        got = c.cgemm_3m(a, b)
+        # a^T is passed as its k x m transpose, stored column-major
+        got = c.cgemm_3m(np.ascontiguousarray(a.T), b, trans_a=True) if trans else c.cgemm_3m(a, b)
     finally:
         c.close()
     ref = a @ b
@@ -35,5 +37,5 @@ def test_cgemm_3m_within_rounding_bound(m, n, k):
     bound = 2 * (k + 4) * 2.0 ** -53 * mag
     err = np.maximum(np.abs(got.real - ref.real), np.abs(got.imag - ref.imag))
     worst = float(np.max(err / bound))
-    print(f"m={m} n={n} k={k}: worst error / rounding bound {worst:.4f}", flush=True)
+    print(f"m={m} n={n} k={k} trans={trans}: worst error / rounding bound {worst:.4f}", flush=True)
     assert worst <= 1.0
