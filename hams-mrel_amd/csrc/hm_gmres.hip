@@ -412,10 +412,10 @@ static hipError_t apply_inverse(hipStream_t s, const GmresArgs& a, const double*
     // (n x ncol, ld n = nblk nb) read as nb x (nblk ncol) matrices of leading
     // dimension nb put block b's rows of column c in column b + c nblk
     const int nb = a.blk_start[1];
-    return cgemm<float>(s, false, nb, a.nblk * ncol, nb, 1.f, 0.f, a.Pinv, nb, 0, W, nb, 0, 0.f, 0.f, Z, nb, 0, 1);
+    return cgemm_3m<float>(s, false, nb, a.nblk * ncol, nb, 1.f, 0.f, a.Pinv, nb, 0, W, nb, 0, 0.f, 0.f, Z, nb, 0, 1);
   }
   return for_block_runs(a.nblk, a.blk_start, [&](int r0, int nb, int cnt, size_t off) {
-    return cgemm<float>(s, false, nb, ncol, nb, 1.f, 0.f, a.Pinv + 2 * off, nb, (long long)nb * nb,
+    return cgemm_3m<float>(s, false, nb, ncol, nb, 1.f, 0.f, a.Pinv + 2 * off, nb, (long long)nb * nb,
                         W + 2 * (long long)r0, a.n, nb, 0.f, 0.f, Z + 2 * (long long)r0, a.n, nb, cnt);
   });
 }

@@ -921,14 +921,12 @@ hipError_t hmat_apply(hipStream_t s, int nb, int m, const LowRank& lr, const T* 
   // of leading dimension nb m, block b's rows of column c are column b + c nb
   // of X and Y read as m x (nb ncol) matrices of leading dimension m, so the
   // product is one GEMM over all bodies' columns (the same sums per entry).
-  // In FP64 (the residuals) every product takes three real products per
-  // complex one: their error stays at FP64's, far below the GMRES tolerance
+  // Every product takes three real products per complex one: in FP64 (the
+  // residuals) the error stays at FP64's, far below the GMRES tolerance; in
+  // FP32 (the Arnoldi operator) far below the FP16 preconditioner's rounding
   auto gemm = [&](bool ta, int gm, int gn, int gk, const T* A, int lda, long long sA, const T* B, int ldb,
                   long long sB, T beta, T* C, int ldc, long long sC, int cnt) {
-    if constexpr (sizeof(T) == 8)
-      return cgemm_3m<T>(s, ta, gm, gn, gk, T(1), T(0), A, lda, sA, B, ldb, sB, beta, T(0), C, ldc, sC, cnt);
-    else
-      return cgemm<T>(s, ta, gm, gn, gk, T(1), T(0), A, lda, sA, B, ldb, sB, beta, T(0), C, ldc, sC, cnt);
+    return cgemm_3m<T>(s, ta, gm, gn, gk, T(1), T(0), A, lda, sA, B, ldb, sB, beta, T(0), C, ldc, sC, cnt);
   };
   if (D && sD == 0 && ldx == (long long)nb * m && ldy == ldx)
     e = gemm(false, m, nb * ncol, m, D, (int)ldd, 0, X, m, 0, T(0), Y, m, 0, 1);

@@ -239,6 +239,8 @@ hipError_t cgemm_3m(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai,
 }
 template hipError_t cgemm_3m<double>(hipStream_t, bool, int, int, int, double, double, const double*, int, long long,
                                      const double*, int, long long, double, double, double*, int, long long, int);
+template hipError_t cgemm_3m<float>(hipStream_t, bool, int, int, int, float, float, const float*, int, long long,
+                                    const float*, int, long long, float, float, float*, int, long long, int);
 
 template <typename T>
 hipError_t cgemm_grouped(hipStream_t s, bool transA, const GemmDesc<T>* d, int count, int mmax, int nmax, int kb) {
@@ -1471,7 +1473,7 @@ hipError_t gj_inverse_nopiv(hipStream_t s, float* A, int lda, long long sA, int 
     hipLaunchKernelGGL(k_gj_col, dim3((unsigned)((colw + 255) / 256), 1, batch), dim3(256), 0, s, A, lda, sA, n, k0,
                        T, sT, C, sC);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = cgemm<float>(s, false, n, n, w, -1.0f, 0.0f, C, n, sC, T, kGj, sT, 1.0f, 0.0f, A, lda, sA, batch)) !=
+    if ((e = cgemm_3m<float>(s, false, n, n, w, -1.0f, 0.0f, C, n, sC, T, kGj, sT, 1.0f, 0.0f, A, lda, sA, batch)) !=
         hipSuccess)
       return e;
   }

@@ -13,7 +13,7 @@ hipError_t cgemm(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, co
 // The same with each complex product formed from three real ones
 // (Re = Ar Br - Ai Bi, Im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi): a quarter
 // fewer MFMAs; the imaginary part's error is bounded by
-// (|Ar| + |Ai|)(|Br| + |Bi|) instead of by its own terms. FP64 only.
+// (|Ar| + |Ai|)(|Br| + |Bi|) instead of by its own terms.
 template <typename T>
 hipError_t cgemm_3m(hipStream_t s, bool transA, int m, int n, int k, T ar, T ai, const T* A, int lda, long long sA,
                     const T* B, int ldb, long long sB, T br, T bi, T* C, int ldc, long long sC, int batch);
