@@ -925,7 +925,7 @@ hipError_t lu_factor(hipStream_t s, T* A, int lda, long long sA, int n, int* ipi
       const long long d = (long long)k0 + (long long)k0 * lda;
       hipLaunchKernelGGL((k_trsm_block<T, false>), dim3((nc + 3) / 4, 1, batch), dim3(256), 0, s,
                          A + 2 * d, lda, sA, k1 - k0, A + 2 * ((long long)k0 + (long long)k1 * lda), lda, sA, nc);
-      hipError_t e = cgemm<T>(s, false, n - k1, nc, k1 - k0, T(-1), T(0),
+      hipError_t e = cgemm_3m<T>(s, false, n - k1, nc, k1 - k0, T(-1), T(0),
                               A + 2 * ((long long)k1 + (long long)k0 * lda), lda, sA,
                               A + 2 * ((long long)k0 + (long long)k1 * lda), lda, sA, T(1), T(0),
                               A + 2 * ((long long)k1 + (long long)k1 * lda), lda, sA, batch);
@@ -1069,15 +1069,15 @@ __global__ __launch_bounds__(256) void k_lu_trsm_g(const LuDesc<T>* __restrict__
 
 // trailing update A22 -= A21 A12
 template <typename T, int KB>
-__global__ __launch_bounds__(256) void k_lu_update_g(const LuDesc<T>* __restrict__ d, int k0) {
+__global__ __launch_bounds__(256, 2) void k_lu_update_g(const LuDesc<T>* __restrict__ d, int k0) {
   const LuDesc<T> g = d[blockIdx.z];
   const int k1 = imin(k0 + kLuNb, g.n), nr = g.n - k1;
   const int row0 = blockIdx.x * kBM, col0 = blockIdx.y * kBN;
   if (nr <= 0 || row0 >= nr || col0 >= nr) return;
   const long long lda = g.lda;
-  cgemm_tile<T, false, KB>(nr, nr, k1 - k0, T(-1), T(0), g.A + 2 * ((long long)k1 + (long long)k0 * lda), (int)lda,
-                       g.A + 2 * ((long long)k0 + (long long)k1 * lda), (int)lda, T(1), T(0),
-                       g.A + 2 * ((long long)k1 + (long long)k1 * lda), (int)lda, row0, col0);
+  cgemm_tile<T, false, KB, true>(nr, nr, k1 - k0, T(-1), T(0), g.A + 2 * ((long long)k1 + (long long)k0 * lda),
+                                 (int)lda, g.A + 2 * ((long long)k0 + (long long)k1 * lda), (int)lda, T(1), T(0),
+                                 g.A + 2 * ((long long)k1 + (long long)k1 * lda), (int)lda, row0, col0);
 }
 
 template <typename T>
